@@ -1,0 +1,126 @@
+"""Run configuration for the IMEX-RK stage of the gas-particle shallow-water
+system (gas carrier, Voellmy friction, KT fluxes, minmod MUSCL, IMEX RK2).
+
+The fields are the namelist values the reference reads for this path
+(src/inpout_2d.f90:326-391 namelists; defaults at :408-970) plus the derived
+constants read_param computes (src/inpout_2d.f90:1356 inv_pres, :1379
+rho_a_amb, :2045-2049 n_vars/idx_u/idx_v, :2070-2080 inv_rho_s,
+:2271-2272 limiter of u and v).
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+from typing import List
+
+import numpy as np
+
+
+@dataclasses.dataclass
+class RunConfig:
+    # grid (RUN_PARAMETERS / DEM header)
+    nx: int = 256
+    ny: int = 256
+    dx: float = 10.0                  # cell_size; dx = dy (src/geometry_2d.f90:272-284)
+    x0: float = 0.0
+    y0: float = 0.0
+    # numerics (NUMERIC_PARAMETERS)
+    cfl: float = 0.24
+    dt0: float = 1.0e-4
+    max_dt: float = 1.0
+    limiter: int = 1                  # 1 = minmod (src/geometry_2d.f90:1803-1806)
+    bc_flag: int = 1                  # 1 = Neumann zero slope on all sides
+    t_start: float = 0.0
+    t_end: float = 1.0e6              # no clipping of dt by the end time
+    # physics (EXPL_TERMS / GAS / SOLID / RHEOLOGY parameters)
+    n_solid: int = 1
+    energy_flag: bool = True
+    settling_flag: bool = False
+    entrainment_flag: bool = False
+    grav: float = 9.81
+    mu: float = 0.1
+    xi: float = 1000.0
+    pres: float = 101325.0
+    T_ambient: float = 300.0
+    sp_heat_a: float = 1004.0
+    sp_gas_const_a: float = 287.0
+    kin_visc_a: float = 1.48e-5
+    rho_s: List[float] = dataclasses.field(default_factory=lambda: [2500.0])
+    diam_s: List[float] = dataclasses.field(default_factory=lambda: [1.0e-4])
+    sp_heat_s: List[float] = dataclasses.field(default_factory=lambda: [1200.0])
+    # initial collapsing volume (COLLAPSING_VOLUME_PARAMETERS)
+    collapsing_volume_flag: bool = True
+    x_collapse: float = 640.0
+    y_collapse: float = 1280.0
+    r_collapse: float = 300.0
+    h_collapse: float = 20.0
+    T_collapse: float = 600.0
+    alphas_collapse: List[float] = dataclasses.field(default_factory=lambda: [0.05])
+    # radial lateral source (RADIAL_SOURCE_PARAMETERS)
+    radial_source_flag: bool = False
+    x_source: float = 0.0
+    y_source: float = 0.0
+    r_source: float = 0.0
+    vel_source: float = 0.0
+    T_source: float = 0.0
+    mfr_source: float = -1.0
+    h_source: float = 0.0             # derived from mfr_source when > 0
+    time_param: List[float] = dataclasses.field(default_factory=lambda: [0.0, 0.0, 0.0, 0.0])
+    alphas_source: List[float] = dataclasses.field(default_factory=lambda: [0.0])
+    # vulnerability table (VULNERABILITY_TABLE_PARAMETERS)
+    thickness_levels: List[float] = dataclasses.field(default_factory=list)
+    dyn_pres_levels: List[float] = dataclasses.field(default_factory=list)
+    name: str = "run"
+
+    # ---- derived quantities ------------------------------------------------
+    @property
+    def n_vars(self) -> int:
+        return 4 + self.n_solid
+
+    @property
+    def n_eqns(self) -> int:
+        return self.n_vars
+
+    @property
+    def inv_pres(self) -> float:
+        return 1.0 / self.pres
+
+    @property
+    def rho_a_amb(self) -> float:
+        return self.pres / (self.sp_gas_const_a * self.T_ambient)
+
+    @property
+    def eps_sing(self) -> float:
+        # src/geometry_2d.f90:300-310: MIN(MIN(dx**4, dy**4), 1e-10)
+        d4 = math.pow(self.dx, 4.0)
+        return min(min(d4, d4), 1.0e-10)
+
+    @property
+    def n_cells(self) -> int:
+        return self.nx * self.ny
+
+    def validate(self) -> None:
+        if self.nx < 5 or self.ny < 5:
+            raise ValueError("the 5-point bed-slope stencil needs nx, ny >= 5")
+        if not (1 <= self.n_solid <= 3):
+            raise ValueError("n_solid must be in 1..3")
+        for name in ("rho_s", "diam_s", "sp_heat_s", "alphas_collapse", "alphas_source"):
+            if len(getattr(self, name)) < self.n_solid:
+                raise ValueError(f"{name} needs n_solid entries")
+        if self.cfl > 0.25:
+            # src/inpout_2d.f90:2261-2267 pauses for cfl > 0.25 in 2-D
+            raise ValueError("cfl must be <= 0.25 for a 2-D run")
+        if len(self.thickness_levels) > 8 or len(self.dyn_pres_levels) > 8:
+            raise ValueError("at most 8 thickness and 8 dynamic-pressure levels")
+        if self.limiter != 1:
+            raise ValueError("only the minmod limiter (1) is on this path")
+        if self.bc_flag != 1:
+            raise ValueError("only Neumann (flag 1) boundaries are on this path")
+
+
+def solid_arrays(cfg: RunConfig):
+    """rho_s, inv_rho_s, diam_s, sp_heat_s as float64 arrays of length n_solid."""
+    ns = cfg.n_solid
+    rho = np.asarray(cfg.rho_s[:ns], dtype=np.float64)
+    return rho, 1.0 / rho, np.asarray(cfg.diam_s[:ns], np.float64), \
+        np.asarray(cfg.sp_heat_s[:ns], np.float64)

@@ -1,0 +1,71 @@
+/*
+ * CPU restatement of the IMEX_SfloW2D IMEX-RK stage (test infrastructure).
+ *
+ * This is the oracle: tests/ and bench.py's cpu_baseline leg use it as the
+ * checker for the HIP path.  It is never part of the product path.
+ */
+#ifndef IMEX_ORACLE_H
+#define IMEX_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Integer and real parameter vectors share the layout of oracle/refpy.py
+ * pack_params(): ip[12], rp[63]. */
+typedef struct oracle_s oracle_t;
+
+oracle_t *oracle_create(const int *ip, const double *rp);
+void oracle_destroy(oracle_t *o);
+
+/* Static per-cell fields (x index fastest, nx*ny each).  sides bit 0..3 =
+ * sourceE, sourceW, sourceN, sourceS. */
+void oracle_set_static(oracle_t *o, const double *bprime_x, const double *bprime_y,
+                       const int *source_cell, const int *sides,
+                       const double *arc_perim, const double *arc_nx,
+                       const double *arc_ny, double h_source);
+/* q in Fortran order q(var, j, k); also runs check_solve(.TRUE.) and the
+ * initial driver epilogue. */
+void oracle_init_state(oracle_t *o, const double *q);
+
+/* n full driver steps; dts[n], ncells[n] receive dt and the active count.
+ * Returns the OR of the physics error bits raised. */
+int oracle_run(oracle_t *o, int n, double *dts, int *ncells);
+
+void oracle_get_q(const oracle_t *o, double *q);
+void oracle_set_q(oracle_t *o, const double *q);
+void oracle_get_fields(const oracle_t *o, double *hmax, double *pdynmax,
+                       double *modvelmax, double *deposit, double *t, double *dt);
+void oracle_get_vuln(const oracle_t *o, int *v);
+void oracle_get_mask(const oracle_t *o, int *m);
+/* number of cells that went from wet (q1 > 0) to exactly dry in a step, and
+ * reads of a face or stage state left from an earlier step that differ from
+ * the state a fresh evaluation gives (the reference's stale-state quirk) */
+void oracle_get_counters(const oracle_t *o, long *dried, long *stale_reads);
+
+/* per-function entry points: n states of n_vars / n_vars+2 values each */
+void oracle_qc_to_qp(const oracle_t *o, int n, const double *qc, double *qp, double *pdyn);
+void oracle_qp_to_qc(const oracle_t *o, int n, const double *qp, double *qc);
+void oracle_mixt_var(const oracle_t *o, int n, const double *qp, double *out6);
+void oracle_fluxes(const oracle_t *o, int n, int dir, const double *qc, const double *qp,
+                   double *flux);
+void oracle_speeds(const oracle_t *o, int n, int dir, const double *qp, double *out2);
+void oracle_implicit(const oracle_t *o, int n, const double *qc, double *nh);
+void oracle_implicit_cplx(const oracle_t *o, int n, const double *qre, const double *qim,
+                          double *ore, double *oim);
+void oracle_semi_impl(const oracle_t *o, int n, const double *qc, const double *qp,
+                      double *si);
+void oracle_expl(const oracle_t *o, int n, const double *bx, const double *by, double t,
+                 const double *qp, double *expl);
+void oracle_settling(const oracle_t *o, int n, const double *diam, const double *rhos,
+                     const double *rhoc, const double *ikv, double *out);
+void oracle_mass_exchange(const oracle_t *o, int n, const double *qp, double dt,
+                          double *dep, double *eq);
+/* one Newton solve: q_si, q0, R (n_vars each) -> q (n_vars); returns iterations */
+int oracle_newton(const oracle_t *o, double dt, const double *q_si, const double *q0,
+                  const double *R, double *q);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,250 @@
+"""ctypes access to oracle/_ref/liboracle.so, the C restatement of the
+IMEX-RK stage (oracle/imex_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker.  Product code never imports it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+from . import refpy
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int)
+
+
+def lib_path() -> str:
+    return os.path.join(_HERE, "_ref", "liboracle.so")
+
+
+def build() -> None:
+    subprocess.run(["make", "-C", _HERE, "oracle"], check=True,
+                   stdout=subprocess.DEVNULL)
+
+
+def _lib():
+    global _LIB
+    if _LIB is None:
+        p = lib_path()
+        if not os.path.exists(p):
+            build()
+        lib = ctypes.CDLL(p)
+        lib.oracle_create.restype = ctypes.c_void_p
+        lib.oracle_create.argtypes = [_I, _D]
+        for name in ("oracle_destroy", "oracle_set_static", "oracle_init_state",
+                     "oracle_run", "oracle_get_q", "oracle_set_q", "oracle_get_fields",
+                     "oracle_get_vuln", "oracle_get_mask", "oracle_get_counters",
+                     "oracle_qc_to_qp", "oracle_qp_to_qc", "oracle_mixt_var",
+                     "oracle_fluxes", "oracle_speeds", "oracle_implicit",
+                     "oracle_implicit_cplx", "oracle_semi_impl", "oracle_expl",
+                     "oracle_settling", "oracle_mass_exchange", "oracle_newton"):
+            getattr(lib, name).restype = ctypes.c_int if name in (
+                "oracle_run", "oracle_newton") else None
+        _LIB = lib
+    return _LIB
+
+
+def _dp(a):
+    return a.ctypes.data_as(_D)
+
+
+def _ip(a):
+    return a.ctypes.data_as(_I)
+
+
+class Oracle:
+    """One oracle run (the C restatement).  Arrays use [k, j, var] order
+    (identical memory to the reference's q(var, j, k))."""
+
+    def __init__(self, cfg, static=None, q0=None):
+        self.lib = _lib()
+        self.cfg = cfg
+        ip, rp = refpy.pack_params(cfg)
+        self.ip, self.rp = ip, rp
+        self.h = ctypes.c_void_p(self.lib.oracle_create(_ip(ip), _dp(rp)))
+        self.nv = cfg.n_vars
+        if static is not None:
+            self.set_static(static)
+        if q0 is not None:
+            self.init_state(q0)
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.lib.oracle_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def set_static(self, st):
+        c = self.cfg
+        N = c.nx * c.ny
+
+        def a(name, dt, default=0):
+            v = st.get(name)
+            if v is None:
+                return np.full(N, default, dtype=dt)
+            return np.ascontiguousarray(v, dtype=dt).reshape(-1)
+
+        self._st = [a("B_prime_x", np.float64), a("B_prime_y", np.float64),
+                    a("source_cell", np.int32), a("source_sides", np.int32),
+                    a("arc_perim", np.float64), a("arc_n_x", np.float64),
+                    a("arc_n_y", np.float64)]
+        bx, by, sc, sd, ap, ax, ay = self._st
+        self.lib.oracle_set_static(self.h, _dp(bx), _dp(by), _ip(sc), _ip(sd), _dp(ap),
+                                   _dp(ax), _dp(ay), ctypes.c_double(st.get("h_source", 0.0)))
+
+    def init_state(self, q):
+        a = np.ascontiguousarray(q, dtype=np.float64)
+        assert a.shape == (self.cfg.ny, self.cfg.nx, self.nv)
+        self.lib.oracle_init_state(self.h, _dp(a))
+
+    def run(self, n):
+        dts = np.zeros(max(n, 1))
+        nc = np.zeros(max(n, 1), np.int32)
+        err = self.lib.oracle_run(self.h, ctypes.c_int(n), _dp(dts), _ip(nc))
+        return dts[:n], nc[:n], err
+
+    def q(self):
+        out = np.zeros((self.cfg.ny, self.cfg.nx, self.nv))
+        self.lib.oracle_get_q(self.h, _dp(out))
+        return out
+
+    def set_q(self, q):
+        a = np.ascontiguousarray(q, dtype=np.float64)
+        self.lib.oracle_set_q(self.h, _dp(a))
+
+    def fields(self):
+        c = self.cfg
+        hm = np.zeros((c.ny, c.nx))
+        pd = np.zeros((c.ny, c.nx))
+        mv = np.zeros((c.ny, c.nx))
+        dep = np.zeros((c.n_solid, c.ny, c.nx))
+        t = ctypes.c_double()
+        dt = ctypes.c_double()
+        self.lib.oracle_get_fields(self.h, _dp(hm), _dp(pd), _dp(mv), _dp(dep),
+                                   ctypes.byref(t), ctypes.byref(dt))
+        return dict(hmax=hm, pdynmax=pd, mod_vel_max=mv, deposit=dep, t=t.value, dt=dt.value)
+
+    def vuln(self):
+        c = self.cfg
+        nl = max(1, len(c.thickness_levels) * len(c.dyn_pres_levels))
+        v = np.zeros((c.ny, c.nx, nl), np.int32)
+        self.lib.oracle_get_vuln(self.h, _ip(v))
+        return v
+
+    def mask(self):
+        m = np.zeros((self.cfg.ny, self.cfg.nx), np.int32)
+        self.lib.oracle_get_mask(self.h, _ip(m))
+        return m
+
+    def counters(self):
+        d = ctypes.c_long()
+        s = ctypes.c_long()
+        self.lib.oracle_get_counters(self.h, ctypes.byref(d), ctypes.byref(s))
+        return d.value, s.value
+
+    # ---- per-function ------------------------------------------------------
+    def qc_to_qp(self, qc):
+        qc = np.ascontiguousarray(qc, np.float64)
+        n = qc.shape[0]
+        qp = np.zeros((n, self.nv + 2))
+        pd = np.zeros(n)
+        self.lib.oracle_qc_to_qp(self.h, ctypes.c_int(n), _dp(qc), _dp(qp), _dp(pd))
+        return qp, pd
+
+    def qp_to_qc(self, qp):
+        qp = np.ascontiguousarray(qp, np.float64)
+        n = qp.shape[0]
+        qc = np.zeros((n, self.nv))
+        self.lib.oracle_qp_to_qc(self.h, ctypes.c_int(n), _dp(qp), _dp(qc))
+        return qc
+
+    def mixt_var(self, qp):
+        qp = np.ascontiguousarray(qp, np.float64)
+        n = qp.shape[0]
+        out = np.zeros((n, 6))
+        self.lib.oracle_mixt_var(self.h, ctypes.c_int(n), _dp(qp), _dp(out))
+        return out
+
+    def fluxes(self, qc, qp, direction):
+        qc = np.ascontiguousarray(qc, np.float64)
+        qp = np.ascontiguousarray(qp, np.float64)
+        n = qc.shape[0]
+        out = np.zeros((n, self.nv))
+        self.lib.oracle_fluxes(self.h, ctypes.c_int(n), ctypes.c_int(direction), _dp(qc),
+                               _dp(qp), _dp(out))
+        return out
+
+    def speeds(self, qp, direction):
+        qp = np.ascontiguousarray(qp, np.float64)
+        n = qp.shape[0]
+        out = np.zeros((n, 2))
+        self.lib.oracle_speeds(self.h, ctypes.c_int(n), ctypes.c_int(direction), _dp(qp),
+                               _dp(out))
+        return out
+
+    def implicit(self, qc):
+        qc = np.ascontiguousarray(qc, np.float64)
+        n = qc.shape[0]
+        out = np.zeros((n, self.nv))
+        self.lib.oracle_implicit(self.h, ctypes.c_int(n), _dp(qc), _dp(out))
+        return out
+
+    def implicit_cplx(self, qre, qim):
+        qre = np.ascontiguousarray(qre, np.float64)
+        qim = np.ascontiguousarray(qim, np.float64)
+        n = qre.shape[0]
+        ore = np.zeros((n, self.nv))
+        oim = np.zeros((n, self.nv))
+        self.lib.oracle_implicit_cplx(self.h, ctypes.c_int(n), _dp(qre), _dp(qim), _dp(ore),
+                                      _dp(oim))
+        return ore, oim
+
+    def semi_impl(self, qc, qp):
+        qc = np.ascontiguousarray(qc, np.float64)
+        qp = np.ascontiguousarray(qp, np.float64)
+        n = qc.shape[0]
+        out = np.zeros((n, self.nv))
+        self.lib.oracle_semi_impl(self.h, ctypes.c_int(n), _dp(qc), _dp(qp), _dp(out))
+        return out
+
+    def expl(self, bx, by, t, qp):
+        bx = np.ascontiguousarray(bx, np.float64)
+        by = np.ascontiguousarray(by, np.float64)
+        qp = np.ascontiguousarray(qp, np.float64)
+        n = qp.shape[0]
+        out = np.zeros((n, self.nv))
+        self.lib.oracle_expl(self.h, ctypes.c_int(n), _dp(bx), _dp(by), ctypes.c_double(t),
+                             _dp(qp), _dp(out))
+        return out
+
+    def settling(self, diam, rhos, rhoc, ikv):
+        a = [np.ascontiguousarray(x, np.float64) for x in (diam, rhos, rhoc, ikv)]
+        n = a[0].shape[0]
+        out = np.zeros(n)
+        self.lib.oracle_settling(self.h, ctypes.c_int(n), *[_dp(x) for x in a], _dp(out))
+        return out
+
+    def mass_exchange(self, qp, dt):
+        qp = np.ascontiguousarray(qp, np.float64)
+        n = qp.shape[0]
+        dep = np.zeros((n, self.cfg.n_solid))
+        eq = np.zeros((n, self.nv))
+        self.lib.oracle_mass_exchange(self.h, ctypes.c_int(n), _dp(qp), ctypes.c_double(dt),
+                                      _dp(dep), _dp(eq))
+        return dep, eq
+
+    def newton(self, dt, q_si, q0, R):
+        a = [np.ascontiguousarray(x, np.float64) for x in (q_si, q0, R)]
+        out = np.zeros(self.nv)
+        it = self.lib.oracle_newton(self.h, ctypes.c_double(dt), _dp(a[0]), _dp(a[1]),
+                                    _dp(a[2]), _dp(out))
+        return out, it
