@@ -1,0 +1,1915 @@
+// MI355X (gfx950) IMEX-RK stage of IMEX_SfloW2D: kernels and the C ABI.
+//
+// Build: hipcc --offload-arch=gfx950 -O3 -ffp-contract=off -fPIC -shared
+//        (see imex_sflow2d_dev_amd/build.py).  -ffp-contract=off keeps every
+//        multiply and add separately rounded, as in the reference build.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "imex_hip.h"
+#include "imex_kernels.h"
+
+using namespace imex;
+
+#define DINF __longlong_as_double(0x7ff0000000000000ULL)
+
+// ===========================================================================
+// active set: check_solve, src/solver_2d.f90:667-858
+// ===========================================================================
+__global__ __launch_bounds__(NTHR) void k_mask(Params P, Grid G, int solve_all) {
+  __shared__ unsigned char seed[RY][RX];
+  __shared__ int s_any;
+  __shared__ int s_cnt;
+  const int nb = gridDim.x;
+  const int tile = xcd_tile(blockIdx.x, nb);
+  const int j0 = (tile % G.ntx) * TX, k0 = (tile / G.ntx) * TY;
+  const int tid = threadIdx.x;
+  if (tid == 0) {
+    s_any = 0;
+    s_cnt = 0;
+  }
+  for (int e = tid; e < RX * RY; e += NTHR) {
+    const int lj = e % RX, lk = e / RX;
+    const int j = j0 + lj - HALO, k = k0 + lk - HALO;
+    unsigned char s = 0;
+    if (j >= 0 && j < G.nx && k >= 0 && k < G.ny) {
+      const int c = j + G.nx * k;
+      const bool ring = (j == 0 || j == G.nx - 1 || k == 0 || k == G.ny - 1);
+      if (ring || solve_all || G.q[c] > 0.0 || (P.radial && G.scell[c] == 2)) s = 1;
+    }
+    seed[lk][lj] = s;
+  }
+  __syncthreads();
+  const int lj = tid % TX, lk = tid / TX;
+  const int j = j0 + lj, k = k0 + lk;
+  int act = 0;
+  if (j < G.nx && k < G.ny) {
+    const int c = j + G.nx * k;
+    const bool ring = (j == 0 || j == G.nx - 1 || k == 0 || k == G.ny - 1);
+    unsigned char m = 0;
+    if (ring) {
+      m = 1;
+    } else {
+#pragma unroll
+      for (int dk = -2; dk <= 2; dk++) {
+        const int r = 2 - (dk < 0 ? -dk : dk);
+        for (int dj = -r; dj <= r; dj++) m |= seed[lk + HALO + dk][lj + HALO + dj];
+      }
+    }
+    if (P.radial && G.scell[c] == 1) m = 0;
+    G.mask[c] = m;
+    act = m;
+  }
+  if (act) {
+    s_any = 1;
+    atomicAdd(&s_cnt, 1);
+  }
+  __syncthreads();
+  if (tid == 0) {
+    G.tile_act[tile] = (unsigned char)s_any;
+    if (s_cnt) atomicAdd(&G.st->active, (unsigned long long)s_cnt);
+  }
+}
+
+// ===========================================================================
+// reconstruction of one cell in one direction: src/solver_2d.f90:3095-3924
+// ===========================================================================
+// LDS primitive-state tile: sq[(i*RY + lk)*RX + lj]
+struct TileView {
+  const double* sq;
+  const signed char* ssc;
+  const unsigned char* ssd;
+  bool tsvar;  // timestep variant: cells of the source disc carry T = T_ambient
+  double T_amb;
+  __device__ __forceinline__ double at(int i, int idx) const {
+    double v = sq[i * RX * RY + idx];
+    if (tsvar && i == 3 && ssc[idx] == 1) v = T_amb;
+    return v;
+  }
+};
+
+__device__ __forceinline__ void qp_to_qp2(const double* qp, double& u, double& v) {
+  // src/constitutive_2d.f90:2164-2188 (qp2(1) = h + B is not needed here)
+  if (qp[0] <= 0.0) {
+    u = 0.0;
+    v = 0.0;
+  } else {
+    u = qp[1] / qp[0];
+    v = qp[2] / qp[0];
+  }
+}
+
+// x direction: W and E face states (primitive) of cell (j,k) at local index idx.
+template <int NS>
+__device__ void recon_x(const Params& P, const Grid& G, const TileView& T, int idx, int j, int k,
+                        double* W, double* E) {
+  constexpr int NV = 4 + NS;
+  const int nx = G.nx;
+  const signed char sc = T.ssc[idx];
+  const unsigned char sd = T.ssd[idx];
+  const bool src2 = (sc == 2);
+  const bool sE = sd & 1, sW = sd & 2, sN = sd & 4, sS = sd & 8;
+  const double xs1 = G.xc[j];
+  double xs0, xs2;
+#pragma unroll
+  for (int i = 0; i < NV; i++) {
+    const double v1 = T.at(i, idx);
+    double pr;
+    if (j == 0) {
+      xs0 = G.xs[0];
+      xs2 = G.xc[1];
+      pr = src2 ? limit_minmod(v1, v1, T.at(i, idx + 1), xs0, xs1, xs2) : 0.0;
+    } else if (j == nx - 1) {
+      xs2 = G.xs[nx];
+      xs0 = G.xc[nx - 2];
+      pr = src2 ? limit_minmod(T.at(i, idx - 1), v1, v1, xs0, xs1, xs2) : 0.0;
+    } else {
+      xs0 = G.xc[j - 1];
+      xs2 = G.xc[j + 1];
+      double v0 = T.at(i, idx - 1), v2 = T.at(i, idx + 1);
+      if (src2) {
+        if (sE) {
+          xs2 = G.xs[j + 1];
+          v2 = v1;
+        } else if (sW) {
+          xs0 = G.xs[j];
+          v0 = v1;
+        }
+      }
+      pr = limit_minmod(v0, v1, v2, xs0, xs1, xs2);
+    }
+    const double dq = 1.0 * P.dx2 * pr;
+    W[i] = v1 - dq;
+    E[i] = v1 + dq;
+    if (j == 0) {
+      if (src2)
+        W[i] = v1;
+      else if (i == 1)
+        W[i] = fmin_f(W[i], 0.0);
+    }
+    if (j == nx - 1) {
+      if (src2)
+        E[i] = v1;
+      else if (i == 1)
+        E[i] = fmax_f(E[i], 0.0);
+    }
+  }
+  // u and v with the limiters of hu and hv; the y part of u may have stored
+  // the north ring-face point in the x stencil (src/solver_2d.f90:3607)
+  const bool typo = src2 && sN && !sS && (k > 0) && (k < G.ny - 1);
+#pragma unroll
+  for (int ii = 0; ii < 2; ii++) {
+    const int i = NV + ii;
+    const double v1 = T.at(i, idx);
+    if (ii == 1 && typo) xs2 = G.ys[k + 1];
+    double v0, v2, q2u, q2v;
+    if (j == 0) {
+      qp_to_qp2(W, q2u, q2v);
+      v0 = ii ? q2v : q2u;
+      v2 = T.at(i, idx + 1);
+    } else if (j == nx - 1) {
+      qp_to_qp2(E, q2u, q2v);
+      v0 = T.at(i, idx - 1);
+      v2 = ii ? q2v : q2u;
+    } else {
+      v0 = T.at(i, idx - 1);
+      v2 = T.at(i, idx + 1);
+      if (src2) {
+        if (sE) {
+          xs2 = G.xs[j + 1];
+          v2 = v1;
+        } else if (sW) {
+          xs0 = G.xs[j];
+          v0 = v1;
+        }
+      }
+    }
+    const double pr = limit_minmod(v0, v1, v2, xs0, xs1, xs2);
+    const double dq = 1.0 * P.dx2 * pr;
+    W[i] = v1 - dq;
+    E[i] = v1 + dq;
+    if (j == 0) {
+      qp_to_qp2(W, q2u, q2v);
+      W[i] = ii ? q2v : q2u;
+    } else if (j == nx - 1) {
+      qp_to_qp2(E, q2u, q2v);
+      E[i] = ii ? q2v : q2u;
+    } else if (src2) {
+      if (sE) {
+#pragma unroll
+        for (int m = 0; m < NV + 2; m++) E[m] = T.at(m, idx);
+      } else if (sW) {
+#pragma unroll
+        for (int m = 0; m < NV + 2; m++) W[m] = T.at(m, idx);
+      }
+    }
+  }
+}
+
+// y direction: S and N face states of cell (j,k).
+template <int NS>
+__device__ void recon_y(const Params& P, const Grid& G, const TileView& T, int idx, int j, int k,
+                        double* S, double* Nn) {
+  constexpr int NV = 4 + NS;
+  const int ny = G.ny;
+  const signed char sc = T.ssc[idx];
+  const unsigned char sd = T.ssd[idx];
+  const bool src2 = (sc == 2);
+  const bool sN = sd & 4, sS = sd & 8;
+  const double ys1 = G.yc[k];
+  double ys0, ys2;
+#pragma unroll
+  for (int i = 0; i < NV; i++) {
+    const double v1 = T.at(i, idx);
+    double pr;
+    if (k == 0) {
+      ys0 = G.ys[0];
+      ys2 = G.yc[1];
+      pr = 0.0;
+    } else if (k == ny - 1) {
+      ys0 = G.yc[ny - 2];
+      ys2 = G.ys[ny];
+      pr = 0.0;
+    } else {
+      ys0 = G.yc[k - 1];
+      ys2 = G.yc[k + 1];
+      double v0 = T.at(i, idx - RX), v2 = T.at(i, idx + RX);
+      if (src2) {
+        if (sS) {
+          ys0 = G.ys[k];
+          v0 = v1;
+        } else if (sN) {
+          ys2 = G.ys[k + 1];
+          v2 = v1;
+        }
+      }
+      pr = limit_minmod(v0, v1, v2, ys0, ys1, ys2);
+    }
+    const double dq = 1.0 * P.dy2 * pr;
+    S[i] = v1 - dq;
+    Nn[i] = v1 + dq;
+    if (k == 0 && i == 2) S[i] = fmin_f(S[i], 0.0);
+    if (k == ny - 1 && i == 2) Nn[i] = fmax_f(Nn[i], 0.0);
+  }
+#pragma unroll
+  for (int ii = 0; ii < 2; ii++) {
+    const int i = NV + ii;
+    const double v1 = T.at(i, idx);
+    double v0, v2, q2u, q2v;
+    if (k == 0) {
+      qp_to_qp2(S, q2u, q2v);
+      v0 = ii ? q2v : q2u;
+      v2 = T.at(i, idx + RX);
+    } else if (k == ny - 1) {
+      qp_to_qp2(Nn, q2u, q2v);
+      v0 = T.at(i, idx - RX);
+      v2 = ii ? q2v : q2u;
+    } else {
+      v0 = T.at(i, idx - RX);
+      v2 = T.at(i, idx + RX);
+      if (src2) {
+        if (sS) {
+          ys0 = G.ys[k];
+          v0 = v1;
+        } else if (sN) {
+          v2 = v1;
+        }
+      }
+    }
+    const double pr = limit_minmod(v0, v1, v2, ys0, ys1, ys2);
+    const double dq = 1.0 * P.dy2 * pr;
+    S[i] = v1 - dq;
+    Nn[i] = v1 + dq;
+    if (k == 0) {
+      qp_to_qp2(S, q2u, q2v);
+      S[i] = ii ? q2v : q2u;
+    } else if (k == ny - 1) {
+      qp_to_qp2(Nn, q2u, q2v);
+      Nn[i] = ii ? q2v : q2u;
+    } else if (src2) {
+      if (sS) {
+#pragma unroll
+        for (int m = 0; m < NV + 2; m++) S[m] = T.at(m, idx);
+      } else if (sN) {
+#pragma unroll
+        for (int m = 0; m < NV + 2; m++) Nn[m] = T.at(m, idx);
+      }
+    }
+  }
+}
+
+// The two states of one face.  dir 1: face between local cells idxA (west) and
+// idxB (east), global cell columns ja = jf-1, jb = jf; dir 2: south/north.
+// A side owned by a source-disc cell is the ring mirror when the ring cell on
+// the other side faces the disc across this face, else the zero state
+// (a face state the reference never writes).
+template <int NS>
+__device__ void face_states(const Params& P, const Grid& G, const TileView& T, int dir, int idxA,
+                            int idxB, int ja, int ka, int jb, int kb, bool hasA, bool hasB,
+                            double* qpA, double* qpB, double* qA, double* qB) {
+  constexpr int NV = 4 + NS, NP = NV + 2;
+  double tmp[NP];
+  if (hasA) {
+    if (T.ssc[idxA] == 1) {
+#pragma unroll
+      for (int i = 0; i < NP; i++) qpA[i] = 0.0;
+    } else if (dir == 1) {
+      recon_x<NS>(P, G, T, idxA, ja, ka, tmp, qpA);
+    } else {
+      recon_y<NS>(P, G, T, idxA, ja, ka, tmp, qpA);
+    }
+    qp_to_qc<NS>(P, qpA, qA);
+  }
+  if (hasB) {
+    if (T.ssc[idxB] == 1) {
+#pragma unroll
+      for (int i = 0; i < NP; i++) qpB[i] = 0.0;
+    } else if (dir == 1) {
+      recon_x<NS>(P, G, T, idxB, jb, kb, qpB, tmp);
+    } else {
+      recon_y<NS>(P, G, T, idxB, jb, kb, qpB, tmp);
+    }
+    qp_to_qc<NS>(P, qpB, qB);
+  }
+  const int mom = dir;               // hu (1) or hv (2)
+  const int vel = (dir == 1) ? NV : NV + 1;
+  const unsigned char bitA = (dir == 1) ? 1 : 4;  // sourceE / sourceN of the A cell
+  const unsigned char bitB = (dir == 1) ? 2 : 8;  // sourceW / sourceS of the B cell
+  if (!hasA) {
+#pragma unroll
+    for (int i = 0; i < NP; i++) qpA[i] = qpB[i];
+#pragma unroll
+    for (int i = 0; i < NV; i++) qA[i] = qB[i];
+  } else if (!hasB) {
+#pragma unroll
+    for (int i = 0; i < NP; i++) qpB[i] = qpA[i];
+#pragma unroll
+    for (int i = 0; i < NV; i++) qB[i] = qA[i];
+  } else if (T.ssc[idxA] == 2 && (T.ssd[idxA] & bitA)) {
+#pragma unroll
+    for (int i = 0; i < NP; i++) qpB[i] = qpA[i];
+#pragma unroll
+    for (int i = 0; i < NV; i++) qB[i] = qA[i];
+    qB[mom] = -qA[mom];
+    qpB[vel] = -qpA[vel];
+  } else if (T.ssc[idxB] == 2 && (T.ssd[idxB] & bitB)) {
+#pragma unroll
+    for (int i = 0; i < NP; i++) qpA[i] = qpB[i];
+#pragma unroll
+    for (int i = 0; i < NV; i++) qA[i] = qB[i];
+    qA[mom] = -qB[mom];
+    qpA[vel] = -qpB[vel];
+  }
+}
+
+// eval_speeds + eval_flux_KT + average_KT for one face:
+// src/solver_2d.f90:3937-4014, :2834-3002, :3019-3049
+template <int NS>
+__device__ void kt_flux(const Params& P, int dir, const double* qpA, const double* qpB,
+                        const double* qA, const double* qB, double* H, double& an, double& ap) {
+  constexpr int NV = 4 + NS;
+  double amin, amax, bmin, bmax;
+  local_speeds<NS>(P, qpA, dir, amin, amax);
+  local_speeds<NS>(P, qpB, dir, bmin, bmax);
+  an = fmin_f(fmin_f(amin, bmin), 0.0);
+  ap = fmax_f(fmax_f(amax, bmax), 0.0);
+  double fA[NV], fB[NV];
+  phys_flux<NS>(P, qA, qpA, dir, fA);
+  phys_flux<NS>(P, qB, qpB, dir, fB);
+#pragma unroll
+  for (int i = 0; i < NV; i++) {
+    if (an == ap) {
+      H[i] = 0.0;
+    } else {
+      const double avg = (ap * fA[i] - an * fB[i]) / (ap - an);
+      H[i] = avg + (ap * an) / (ap - an) * (qB[i] - qA[i]);
+    }
+  }
+  double S = 0.0;
+#pragma unroll
+  for (int i = 0; i < NS; i++) S += H[4 + i];
+  if (dir == 1) {
+    if ((S / H[0]) >= 1.0) {
+      double S2 = 0.0;
+#pragma unroll
+      for (int i = 0; i < NS; i++) S2 += H[4 + i] / H[0];
+#pragma unroll
+      for (int i = 0; i < NS; i++) H[4 + i] = H[4 + i] / S2;
+    }
+    if ((qpA[1] == 0.0) && (qpB[1] == 0.0)) {
+      H[0] = 0.0;
+#pragma unroll
+      for (int i = 3; i < NV; i++) H[i] = 0.0;
+    }
+  } else {
+    if ((S / H[0]) > 1.0) {
+      const double S2 = S / H[0];
+#pragma unroll
+      for (int i = 0; i < NS; i++) H[4 + i] = H[4 + i] / S2;
+    }
+    if ((qA[2] == 0.0) && (qB[2] == 0.0)) {
+      H[0] = 0.0;
+#pragma unroll
+      for (int i = 3; i < NV; i++) H[i] = 0.0;
+    }
+  }
+}
+
+// ===========================================================================
+// k_flux: timestep (src/solver_2d.f90:873-1006) fused with the stage-1
+// explicit terms and eval_hyperbolic_terms (src/solver_2d.f90:1299-1338,
+// :2616-2693).  R = divFlux - expl_terms per active cell.
+// mode 0: finisher stores dt_cfl only; mode 1: also the driver's clip and
+// growth limit (src/IMEX_SfloW2D.f90:417-439) into st->dt.
+// ===========================================================================
+template <int NS>
+constexpr size_t flux_lds_bytes() {
+  constexpr int NV = 4 + NS, NP = NV + 2;
+  return sizeof(double) * ((size_t)NP * RY * RX + (size_t)NV * (TX + 1) * TY +
+                           (size_t)NV * TX * (TY + 1) + (size_t)(TX + 1) * TY +
+                           (size_t)TX * (TY + 1)) +
+         2 * RY * RX + TX * TY;
+}
+
+template <int NS>
+__global__ __launch_bounds__(NTHR) void k_flux(Params P, Grid G, int mode, double t_arg,
+                                               int step_idx) {
+  constexpr int NV = 4 + NS, NP = NV + 2;
+  extern __shared__ double smem[];
+  double* sq = smem;
+  double* sHx = sq + NP * RY * RX;
+  double* sHy = sHx + NV * (TX + 1) * TY;
+  double* sAx = sHy + NV * TX * (TY + 1);
+  double* sAy = sAx + (TX + 1) * TY;
+  signed char* ssc = (signed char*)(sAy + TX * (TY + 1));
+  unsigned char* ssd = (unsigned char*)(ssc + RY * RX);
+  unsigned char* sact = ssd + RY * RX;
+  __shared__ double red[NTHR / 64];
+  __shared__ int s_last;
+
+  const int tid = threadIdx.x;
+  const int nb = gridDim.x;
+  const int tile = xcd_tile(blockIdx.x, nb);
+  const int j0 = (tile % G.ntx) * TX, k0 = (tile / G.ntx) * TY;
+  const int nx = G.nx, ny = G.ny, N = G.N;
+  double my_dt = DINF;
+
+  if (G.tile_act[tile]) {
+    const double t = (mode == 1) ? G.st->t : t_arg;
+    // ---- primitive states of the tile and its two-cell halo ----------------
+    for (int e = tid; e < RX * RY; e += NTHR) {
+      const int lj = e % RX, lk = e / RX;
+      const int j = j0 + lj - HALO, k = k0 + lk - HALO;
+      double qp[NP];
+      signed char sc = 0;
+      unsigned char sd = 0;
+#pragma unroll
+      for (int i = 0; i < NP; i++) qp[i] = 0.0;
+      if (j >= 0 && j < nx && k >= 0 && k < ny) {
+        const int c = j + nx * k;
+        sc = G.scell[c];
+        sd = G.sides[c];
+        if (sc != 1) {
+          double q[NV];
+#pragma unroll
+          for (int i = 0; i < NV; i++) q[i] = G.q[(size_t)i * N + c];
+          if (q[0] > 0.0) {
+            double pd;
+            qc_to_qp<NS>(P, q, qp, pd);
+          } else {
+            qp[3] = P.T_amb;
+          }
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < NP; i++) sq[i * RX * RY + e] = qp[i];
+      ssc[e] = sc;
+      ssd[e] = sd;
+    }
+    for (int e = tid; e < TX * TY; e += NTHR) {
+      const int j = j0 + e % TX, k = k0 + e / TX;
+      sact[e] = (j < nx && k < ny) ? G.mask[j + nx * k] : 0;
+    }
+    __syncthreads();
+    TileView T1{sq, ssc, ssd, false, P.T_amb};
+    TileView T0{sq, ssc, ssd, true, P.T_amb};
+    // ---- x faces: (TX+1) x TY ---------------------------------------------
+    for (int e = tid; e < (TX + 1) * TY; e += NTHR) {
+      const int fl = e % (TX + 1), fk = e / (TX + 1);
+      const int jf = j0 + fl, k = k0 + fk;
+      const bool need = (k < ny) && (jf <= nx) &&
+                        ((fl > 0 && sact[fk * TX + fl - 1]) || (fl < TX && jf < nx && sact[fk * TX + fl]));
+      double H[NV];
+      double amax = 0.0;
+#pragma unroll
+      for (int i = 0; i < NV; i++) H[i] = 0.0;
+      if (need) {
+        const int lk = fk + HALO, ljb = fl + HALO;
+        const int idxB = lk * RX + ljb, idxA = idxB - 1;
+        const bool hasA = jf - 1 >= 0, hasB = jf < nx;
+        double qpA[NP], qpB[NP], qA[NV], qB[NV], an, ap;
+        face_states<NS>(P, G, T1, 1, idxA, idxB, jf - 1, k, jf, k, hasA, hasB, qpA, qpB, qA, qB);
+        kt_flux<NS>(P, 1, qpA, qpB, qA, qB, H, an, ap);
+        bool disc = false;
+        if (P.radial) {
+          for (int d = -2; d <= 1; d++) {
+            const int jj = jf + d;
+            if (jj >= 0 && jj < nx && ssc[lk * RX + ljb + d] == 1) disc = true;
+          }
+        }
+        if (disc) {
+          double H0[NV];
+          face_states<NS>(P, G, T0, 1, idxA, idxB, jf - 1, k, jf, k, hasA, hasB, qpA, qpB, qA, qB);
+          kt_flux<NS>(P, 1, qpA, qpB, qA, qB, H0, an, ap);
+        }
+        amax = fmax_f(ap, -an);
+      }
+#pragma unroll
+      for (int i = 0; i < NV; i++) sHx[i * (TX + 1) * TY + e] = H[i];
+      sAx[e] = amax;
+    }
+    // ---- y faces: TX x (TY+1) ---------------------------------------------
+    for (int e = tid; e < TX * (TY + 1); e += NTHR) {
+      const int fj = e % TX, fl = e / TX;
+      const int j = j0 + fj, kf = k0 + fl;
+      const bool need = (j < nx) && (kf <= ny) &&
+                        ((fl > 0 && sact[(fl - 1) * TX + fj]) || (fl < TY && kf < ny && sact[fl * TX + fj]));
+      double H[NV];
+      double amax = 0.0;
+#pragma unroll
+      for (int i = 0; i < NV; i++) H[i] = 0.0;
+      if (need) {
+        const int lj = fj + HALO, lkb = fl + HALO;
+        const int idxB = lkb * RX + lj, idxA = idxB - RX;
+        const bool hasA = kf - 1 >= 0, hasB = kf < ny;
+        double qpA[NP], qpB[NP], qA[NV], qB[NV], an, ap;
+        face_states<NS>(P, G, T1, 2, idxA, idxB, j, kf - 1, j, kf, hasA, hasB, qpA, qpB, qA, qB);
+        kt_flux<NS>(P, 2, qpA, qpB, qA, qB, H, an, ap);
+        bool disc = false;
+        if (P.radial) {
+          for (int d = -2; d <= 1; d++) {
+            const int kk = kf + d;
+            if (kk >= 0 && kk < ny && ssc[(lkb + d) * RX + lj] == 1) disc = true;
+          }
+        }
+        if (disc) {
+          double H0[NV];
+          face_states<NS>(P, G, T0, 2, idxA, idxB, j, kf - 1, j, kf, hasA, hasB, qpA, qpB, qA, qB);
+          kt_flux<NS>(P, 2, qpA, qpB, qA, qB, H0, an, ap);
+        }
+        amax = fmax_f(ap, -an);
+      }
+#pragma unroll
+      for (int i = 0; i < NV; i++) sHy[i * TX * (TY + 1) + e] = H[i];
+      sAy[e] = amax;
+    }
+    __syncthreads();
+    // ---- cells: divergence, explicit terms, CFL candidate -----------------
+    {
+      const int lj = tid % TX, lk = tid / TX;
+      const int j = j0 + lj, k = k0 + lk;
+      if (j < nx && k < ny && sact[tid]) {
+        const int c = j + nx * k;
+        const int ew = lk * (TX + 1) + lj;
+        const int es = lk * TX + lj;
+        double qp[NP], ex[NV];
+        const int idx = (lk + HALO) * RX + lj + HALO;
+#pragma unroll
+        for (int i = 0; i < NP; i++) qp[i] = sq[i * RX * RY + idx];
+        const signed char sc = ssc[idx];
+        double ap = 0.0, ax = 0.0, ay = 0.0;
+        if (sc == 2) {
+          ap = G.arcp[c];
+          ax = G.arcx[c];
+          ay = G.arcy[c];
+        }
+        expl_terms<NS>(P, G.bx[c], G.by[c], qp, ex, t, ap, ax, ay);
+#pragma unroll
+        for (int i = 0; i < NV; i++) {
+          double d = 0.0;
+          d = d + (sHx[i * (TX + 1) * TY + ew + 1] - sHx[i * (TX + 1) * TY + ew]) * P.one_by_dx;
+          d = d + (sHy[i * TX * (TY + 1) + es + TX] - sHy[i * TX * (TY + 1) + es]) * P.one_by_dy;
+          G.R[(size_t)i * N + c] = d - ex[i];
+        }
+        double ma = fmax_f(sAx[ew], sAx[ew + 1]);
+        const double dtx = (ma > 0.0) ? P.cfl * P.dx / ma : DINF;
+        ma = fmax_f(sAy[es], sAy[es + TX]);
+        const double dty = (ma > 0.0) ? P.cfl * P.dy / ma : DINF;
+        my_dt = fmin_f(dtx, dty);
+      }
+    }
+  }
+  // ---- CFL min: wavefront shuffle, LDS, one atomic per block ---------------
+  double v = my_dt;
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    const double o = __shfl_xor(v, off, 64);
+    v = (o < v) ? o : v;
+  }
+  if ((tid & 63) == 0) red[tid >> 6] = v;
+  __syncthreads();
+  if (tid == 0) {
+    double b = red[0];
+    for (int w = 1; w < NTHR / 64; w++) b = (red[w] < b) ? red[w] : b;
+    if (b < DINF) atomicMin(&G.st->dtmin_bits, (unsigned long long)__double_as_longlong(b));
+    __threadfence();
+    const unsigned int prev = atomicAdd(&G.st->blocks_done, 1u);
+    s_last = (prev == (unsigned int)nb - 1u);
+  }
+  __syncthreads();
+  if (s_last && tid == 0) {
+    __threadfence();
+    DevState* st = G.st;
+    const double dtmin =
+        __longlong_as_double((long long)atomicAdd(&st->dtmin_bits, 0ULL));
+    double dt = P.max_dt;
+    dt = fmin_f(dt, dtmin);
+    st->dt_cfl = dt;
+    if (mode == 1) {
+      const double t = st->t;
+      if (t + dt > st->t_end) dt = st->t_end - t;
+      dt = fmin_f(dt, 1.1 * 0.5 * (st->dt_old + st->dt_old_old));
+      st->dt_old_old = st->dt_old;
+      st->dt_old = dt;
+      st->dt = dt;
+      if (G.dts) G.dts[step_idx] = dt;
+    }
+    st->dtmin_bits = 0x7ff0000000000000ULL;
+    st->blocks_done = 0u;
+  }
+}
+
+// ===========================================================================
+// stage 2 of imex_RK_solver for one cell: src/solver_2d.f90:1103-1338 with
+// solve_rk_step / lnsrch / eval_f / eval_jacobian (:1662-2367) and the
+// assembly checks (:1342-1630)
+// ===========================================================================
+template <int NS>
+__device__ void eval_f(const Params& P, double dt, const double* qj, const double* q0,
+                       const double* Rni, double* f, double& scal) {
+  constexpr int NV = 4 + NS;
+  cx qc[NV];
+#pragma unroll
+  for (int i = 0; i < NV; i++) qc[i] = C(qj[i], 0.0);
+  cx n1, n2;
+  implicit_c<NS>(P, qc, n1, n2);
+  double s = 0.0;
+#pragma unroll
+  for (int i = 0; i < NV; i++) {
+    const double nh = (i == 1) ? n1.re : ((i == 2) ? n2.re : 0.0);
+    const double Rj = Rni[i] - 1.0 * nh;
+    f[i] = qj[i] - q0[i] + dt * Rj;
+    f[i] = 1.0 * f[i];
+  }
+#pragma unroll
+  for (int i = 0; i < NV; i++) s += f[i] * f[i];
+  scal = 0.5 * s;
+}
+
+// Jacobian: diagonal qorg for the explicit rows, the 2x2 momentum block from
+// the complex step (all other entries are zero).
+template <int NS>
+__device__ void eval_jacobian(const Params& P, double dt, const double* qrel, const double* qorg,
+                              double& L11, double& L21, double& L12, double& L22) {
+  constexpr int NV = 4 + NS;
+#pragma unroll
+  for (int col = 1; col <= 2; col++) {
+    cx qc[NV];
+#pragma unroll
+    for (int m = 0; m < NV; m++)
+      qc[m] = cmul(C(qrel[m], (m == col) ? P.hcs : 0.0), C(qorg[m], 0.0));
+    cx n1, n2;
+    implicit_c<NS>(P, qc, n1, n2);
+    const double J1 = 1.0 * n1.im * P.one_by_hcs;
+    const double J2 = 1.0 * n2.im * P.one_by_hcs;
+    if (col == 1) {
+      L11 = (1.0 * qorg[1]) - dt * 1.0 * J1;
+      L21 = 0.0 - dt * 1.0 * J2;
+    } else {
+      L12 = 0.0 - dt * 1.0 * J1;
+      L22 = (1.0 * qorg[2]) - dt * 1.0 * J2;
+    }
+  }
+}
+
+// column entry L(r, c) of the sparse Jacobian
+__device__ __forceinline__ double Lrc(int r, int c, const double* qorg, double L11, double L21,
+                                      double L12, double L22) {
+  if (c == 1) return (r == 1) ? L11 : ((r == 2) ? L21 : 0.0);
+  if (c == 2) return (r == 1) ? L12 : ((r == 2) ? L22 : 0.0);
+  return (r == c) ? 1.0 * qorg[c] : 0.0;
+}
+
+template <int NS>
+__device__ void lnsrch(const Params& P, double dt, const double* xold, const double* qorg,
+                       const double* q0, double fold, const double* g, double* p, double* x,
+                       double& f_out, double* rt, double stpmax, const double* Rni,
+                       unsigned int& err) {
+  constexpr int NV = 4 + NS;
+  const double TOLX = kEps;
+  double sum = 0.0;
+#pragma unroll
+  for (int i = 0; i < NV; i++) sum += p[i] * p[i];
+  const double pabs = sqrt(sum);
+  if (pabs > stpmax) {
+#pragma unroll
+    for (int i = 0; i < NV; i++) p[i] = p[i] * stpmax / pabs;
+  }
+  double slope = 0.0;
+#pragma unroll
+  for (int i = 0; i < NV; i++) slope += g[i] * p[i];
+  double test = 0.0;
+#pragma unroll
+  for (int i = 0; i < NV; i++) {
+    const double tmp = fabs(p[i]) / fmax_f(fabs(xold[i]), 1.0);
+    if (i == 0 || tmp > test) test = tmp;
+  }
+  const double alamin = TOLX / test;
+  if (alamin == 0.0) {
+#pragma unroll
+    for (int i = 0; i < NV; i++) x[i] = xold[i];
+    return;
+  }
+  double alam = 1.0, alam2 = 0.0, f2 = 0.0, tmplam = 0.0, f = 0.0;
+  double qj[NV];
+  // the reference loop ends by construction (alam at least halves after the
+  // first trial); 400 trials bound it against NaN states
+  int trial;
+  for (trial = 0; trial < 400; trial++) {
+#pragma unroll
+    for (int i = 0; i < NV; i++) x[i] = xold[i] + alam * p[i];
+#pragma unroll
+    for (int i = 0; i < NV; i++) qj[i] = x[i] * qorg[i];
+    eval_f<NS>(P, dt, qj, q0, Rni, rt, f);
+    if (f <= 0.9 * fold) {
+      break;
+    } else if (alam < alamin) {
+#pragma unroll
+      for (int i = 0; i < NV; i++) x[i] = xold[i];
+      f = fold;
+      break;
+    } else {
+      if (alam == 1.0) {
+        tmplam = -(slope / (2.0 * (f - fold - slope)));
+      } else {
+        const double rhs1 = f - fold - alam * slope;
+        const double rhs2 = f2 - fold - alam2 * slope;
+        const double a = (rhs1 / (alam * alam) - rhs2 / (alam2 * alam2)) / (alam - alam2);
+        const double b =
+            (-(alam2 * rhs1 / (alam * alam)) + alam * rhs2 / (alam2 * alam2)) / (alam - alam2);
+        if (a == 0.0) {
+          tmplam = -(slope / (2.0 * b));
+        } else {
+          const double disc = b * b - 3.0 * a * slope;
+          if (disc < 0.0)
+            tmplam = 0.5 * alam;
+          else if (b <= 0.0)
+            tmplam = (-b + sqrt(disc)) / (3.0 * a);
+          else
+            tmplam = -(slope / (b + sqrt(disc)));
+        }
+        if (tmplam > 0.5 * alam) tmplam = 0.5 * alam;
+      }
+    }
+    alam2 = alam;
+    f2 = f;
+    alam = fmax_f(tmplam, 0.5 * alam);
+  }
+  if (trial == 400) err |= IMEX_ERR_NAN;
+  f_out = f;
+}
+
+template <int NS>
+__device__ int solve_rk_step(const Params& P, double dt, double* qj, const double* q0,
+                             const double* Rni, unsigned int& err) {
+  constexpr int NV = 4 + NS;
+  const double TOLF = 1.0e-10, STPMX = 100.0;
+  double qorg[NV], qrel[NV], rt[NV], xold[NV], p[NV], grad[NV];
+  double scal_f, scal_f_old;
+  double L11 = 0.0, L21 = 0.0, L12 = 0.0, L22 = 0.0;
+#pragma unroll
+  for (int i = 0; i < NV; i++) grad[i] = 0.0;
+#pragma unroll
+  for (int i = 0; i < NV; i++) qorg[i] = fmax_f(fabs(qj[i]), 1.0e-3);
+#pragma unroll
+  for (int i = 0; i < NV; i++) qrel[i] = qj[i] / qorg[i];
+  int it;
+  for (it = 1; it <= 100; it++) {
+    eval_f<NS>(P, dt, qj, q0, Rni, rt, scal_f);
+    double mx = 0.0;
+#pragma unroll
+    for (int i = 0; i < NV; i++) mx = fmax_f(mx, fabs(rt[i]));
+    if (mx < TOLF) break;
+    eval_jacobian<NS>(P, dt, qrel, qorg, L11, L21, L12, L22);
+    double d1[NV - 2];
+    int ne = 0;
+#pragma unroll
+    for (int r = 0; r < NV; r++) {
+      if (r == 1 || r == 2) continue;
+      d1[ne] = rt[r] / (1.0 * qorg[r]);
+      ne++;
+    }
+    double d2a = rt[1], d2b = rt[2];
+    {
+      double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+      for (int a = 0; a < NV - 2; a++) s0 += d1[a] * 0.0;
+#pragma unroll
+      for (int a = 0; a < NV - 2; a++) s1 += d1[a] * 0.0;
+      d2a = d2a - s0;
+      d2b = d2b - s1;
+    }
+    const double inv_det = 1.0 / (L11 * L22 - L21 * L12);
+    const double s1 = (d2a * L22 - d2b * L12) * inv_det;
+    const double s2 = (L11 * d2b - L21 * d2a) * inv_det;
+    ne = 0;
+#pragma unroll
+    for (int r = 0; r < NV; r++) {
+      if (r == 1)
+        p[r] = -s1 + 0.0;
+      else if (r == 2)
+        p[r] = -s2 + 0.0;
+      else {
+        p[r] = 0.0 + -d1[ne];
+        ne++;
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < NV; i++) xold[i] = qrel[i];
+    scal_f_old = scal_f;
+    if (it > 1) {
+      double ss = 0.0;
+#pragma unroll
+      for (int i = 0; i < NV; i++) ss += qrel[i] * qrel[i];
+      const double stpmax = STPMX * fmax_f(sqrt(ss), (double)NV);
+#pragma unroll
+      for (int i = 0; i < NV; i++) {
+        double s = 0.0;
+#pragma unroll
+        for (int r = 0; r < NV; r++) s += rt[r] * Lrc(r, i, qorg, L11, L21, L12, L22);
+        grad[i] = s;
+      }
+      lnsrch<NS>(P, dt, xold, qorg, q0, scal_f_old, grad, p, qrel, scal_f, rt, stpmax, Rni, err);
+    } else {
+#pragma unroll
+      for (int i = 0; i < NV; i++) qrel[i] = xold[i] + p[i];
+#pragma unroll
+      for (int i = 0; i < NV; i++) qj[i] = qrel[i] * qorg[i];
+      eval_f<NS>(P, dt, qj, q0, Rni, rt, scal_f);
+    }
+#pragma unroll
+    for (int i = 0; i < NV; i++) qj[i] = qrel[i] * qorg[i];
+    mx = 0.0;
+#pragma unroll
+    for (int i = 0; i < NV; i++) mx = fmax_f(mx, fabs(rt[i]));
+    if (mx < TOLF) break;
+    double test = 0.0;
+#pragma unroll
+    for (int i = 0; i < NV; i++) {
+      const double tmp = fabs(qrel[i] - xold[i]) / fmax_f(fabs(qrel[i]), 1.0);
+      if (i == 0 || tmp > test) test = tmp;
+    }
+    if (test < kEps) break;
+  }
+  return it;
+}
+
+// stage 2 + assembly for one cell; q holds q0 on entry, the new state on exit
+template <int NS>
+__device__ void stage2(const Params& P, double dt, double* q, const double* X,
+                       unsigned int& err) {
+  constexpr int NV = 4 + NS, NP = NV + 2;
+  double q0[NV], qfv[NV], qp[NP], q_si[NV], qg[NV], si[NV], Rj[NV], pd;
+#pragma unroll
+  for (int i = 0; i < NV; i++) q0[i] = q[i];
+#pragma unroll
+  for (int i = 0; i < NV; i++) {
+    double A = 0.0;
+    A = A + X[i] * 1.0;
+    A = A + (0.0 - 0.0) * 0.0;
+    double B = 0.0;
+    B = B + (0.0 + 0.0) * 0.0;
+    B = B + (0.0 + 0.0) * 0.0;
+    qfv[i] = q0[i] - dt * (A - B);
+  }
+  qc_to_qp<NS>(P, qfv, qp, pd);
+  if (qfv[0] > 0.0) {
+    semi_impl<NS>(P, qp, si);
+#pragma unroll
+    for (int i = 0; i < NV; i++) q_si[i] = qfv[i] + dt * 1.0 * si[i];
+    if ((qfv[1] * qfv[1] + qfv[2] * qfv[2]) == 0.0) {
+      q_si[1] = 0.0;
+      q_si[2] = 0.0;
+    } else if ((q_si[1] * qfv[1] < 0.0) || (q_si[2] * qfv[2] < 0.0)) {
+      q_si[1] = 0.0;
+      q_si[2] = 0.0;
+    } else {
+      const double m = sqrt(q_si[1] * q_si[1] + q_si[2] * q_si[2]);
+      const double d = sqrt(qfv[1] * qfv[1] + qfv[2] * qfv[2]);
+      const double a1 = m * qfv[1] / d, a2 = m * qfv[2] / d;
+      q_si[1] = a1;
+      q_si[2] = a2;
+    }
+#pragma unroll
+    for (int i = 0; i < NV; i++) si[i] = (q_si[i] - qfv[i]) / (dt * 1.0);
+#pragma unroll
+    for (int i = 0; i < NV; i++) qg[i] = q_si[i];
+#pragma unroll
+    for (int i = 0; i < NV; i++) {
+      const double A = 0.0 + X[i] * 1.0;
+      const double B = 0.0 + (0.0 + 0.0) * 0.0;
+      Rj[i] = (A - B) - 1.0 * si[i];
+    }
+    solve_rk_step<NS>(P, dt, qg, q0, Rj, err);
+    if (q_si[1] * q_si[1] + q_si[2] * q_si[2] == 0.0) {
+      qg[1] = 0.0;
+      qg[2] = 0.0;
+    } else if ((qg[1] * q_si[1] <= 0.0) && (qg[2] * q_si[2] <= 0.0)) {
+      qg[1] = 0.0;
+      qg[2] = 0.0;
+    } else {
+      const double m = sqrt(qg[1] * qg[1] + qg[2] * qg[2]);
+      const double d = sqrt(q_si[1] * q_si[1] + q_si[2] * q_si[2]);
+      const double a1 = m * q_si[1] / d, a2 = m * q_si[2] / d;
+      qg[1] = a1;
+      qg[2] = a2;
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < NV; i++) qg[i] = qfv[i];
+  }
+  // assemble_sol
+#pragma unroll
+  for (int i = 0; i < NV; i++) q[i] = qg[i];
+#pragma unroll
+  for (int i = 0; i < NV; i++)
+    if (isnan(q[i])) err |= IMEX_ERR_NAN;
+  if (q[0] < kEps) {
+    if (q[0] > -1.0e-7) {
+#pragma unroll
+      for (int i = 0; i < NV; i++) q[i] = 0.0;
+    } else {
+      err |= IMEX_ERR_NEG_H;
+    }
+  }
+  bool anyneg = false, anybad = false;
+#pragma unroll
+  for (int i = 0; i < NS; i++) {
+    if (q[4 + i] < 0.0) anyneg = true;
+    if (q[4 + i] <= -1.0e-7) anybad = true;
+  }
+  if (anyneg) {
+    if (anybad) {
+      err |= IMEX_ERR_NEG_SOLID;
+    } else {
+#pragma unroll
+      for (int i = 0; i < NS; i++)
+        if (q[4 + i] > -1.0e-7) q[4 + i] = 0.0;
+    }
+  }
+  qc_to_qp<NS>(P, q, qp, pd);
+  if (qp[3] < 273.0) err |= IMEX_ERR_COLD;
+  double S = 0.0;
+#pragma unroll
+  for (int i = 0; i < NS; i++) S += q[4 + i];
+  if (S > q[0]) {
+    double S2 = 0.0;
+#pragma unroll
+    for (int i = 0; i < NS; i++) S2 += q[4 + i];
+    const bool ok = ((S2 - q[0]) / q[0] < 1.0e-10) || (q[0] < kEps);
+    if (!ok) err |= IMEX_WARN_RESCALE;
+    double S3 = 0.0;
+#pragma unroll
+    for (int i = 0; i < NS; i++) S3 += q[4 + i];
+#pragma unroll
+    for (int i = 0; i < NS; i++) q[4 + i] = q[4 + i] / S3 * q[0];
+  }
+}
+
+// update_erosion_deposition_cell for one cell: src/solver_2d.f90:2429-2593
+template <int NS>
+__device__ void erosion_deposition(const Params& P, double dt, double* q, double* depo,
+                                   unsigned int& err) {
+  constexpr int NV = 4 + NS, NP = NV + 2;
+  double qp[NP], pd, dep[NS], eq[NV];
+  if (q[0] > 0.0) {
+    qc_to_qp<NS>(P, q, qp, pd);
+  } else {
+#pragma unroll
+    for (int i = 0; i < NP; i++) qp[i] = 0.0;
+    qp[3] = P.T_amb;
+  }
+  if (mass_exchange<NS>(P, qp, dt, dep, eq)) err |= IMEX_ERR_NEG_DEP;
+#pragma unroll
+  for (int i = 0; i < NV; i++) q[i] = q[i] + dt * eq[i];
+#pragma unroll
+  for (int i = 0; i < NS; i++) q[4 + i] = fmax_f(0.0, q[4 + i]);
+#pragma unroll
+  for (int i = 0; i < NS; i++) depo[i] = depo[i] + dt * dep[i];
+#pragma unroll
+  for (int i = 0; i < NS; i++)
+    if (q[4 + i] < 0.0) err |= IMEX_ERR_NEG_SOLID;
+  if (q[0] <= 0.0) {
+    if (q[0] > -1.0e-10) {
+#pragma unroll
+      for (int i = 0; i < NV; i++) q[i] = 0.0;
+    } else {
+      err |= IMEX_ERR_NEG_H;
+    }
+  }
+  double S = 0.0;
+#pragma unroll
+  for (int i = 0; i < NS; i++) S += q[4 + i];
+  if (S > q[0]) {
+    if (q[0] < 1.0e-10) {
+      double S2 = 0.0;
+#pragma unroll
+      for (int i = 0; i < NS; i++) S2 += q[4 + i];
+#pragma unroll
+      for (int i = 0; i < NS; i++) q[4 + i] = q[4 + i] / S2 * q[0];
+    } else {
+      err |= IMEX_ERR_SOLID_GT_TOT;
+    }
+  }
+  double rg;
+  if (q[0] > 0.0) {
+    qc_to_qp<NS>(P, q, qp, pd);
+    rg = mixt_var<NS>(P, qp, false).red_grav;
+  } else {
+    rg = 0.0;
+  }
+  if (rg <= 0.0) {
+#pragma unroll
+    for (int i = 0; i < NV; i++) q[i] = 0.0;
+  }
+}
+
+// driver epilogue for one cell: src/IMEX_SfloW2D.f90:449-505 (first: :306-353)
+template <int NS>
+__device__ void epilogue(const Params& P, const double* q, double& hmax, double& pdynmax,
+                         double& mvel, unsigned long long& vuln, bool first) {
+  constexpr int NV = 4 + NS, NP = NV + 2;
+  if (q[0] > 0.0) {
+    double qp[NP], pd;
+    qc_to_qp<NS>(P, q, qp, pd);
+    hmax = first ? qp[0] : fmax_f(hmax, qp[0]);
+    const double ru = qp[NV], rv = qp[NV + 1];
+    const double mv = sqrt(ru * ru + rv * rv);
+    if (qp[0] > 0.001) {
+      pdynmax = first ? pd : fmax_f(pdynmax, pd);
+      mvel = first ? mv : fmax_f(mvel, mv);
+    }
+    int it = 0;
+    unsigned long long bits = 0ULL;
+    for (int a = 0; a < P.n_thk; a++) {
+      const bool thk = (qp[0] >= P.thk[a]);
+      for (int b = 0; b < P.n_pdyn; b++) {
+        const bool pdn = (pd >= P.pdyn[b]);
+        if (thk && pdn) bits |= (1ULL << it);
+        it++;
+      }
+    }
+    vuln = first ? bits : (vuln | bits);
+  } else if (first) {
+    hmax = 0.0;
+    pdynmax = 0.0;
+  }
+}
+
+// parts: 1 stage 2 + assembly, 2 erosion/deposition, 4 epilogue,
+//        8 initial epilogue (over every cell).  mode 1: dt from st->dt and
+//        t += dt (device-side driver).
+template <int NS>
+__global__ __launch_bounds__(256) void k_update(Params P, Grid G, int parts, double dt_arg,
+                                                int mode) {
+  constexpr int NV = 4 + NS;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int N = G.N;
+  const double dt = (mode == 1) ? G.st->dt : dt_arg;
+  unsigned int err = 0;
+  int dried = 0;
+  if (c < N && (G.mask[c] || (parts & 8))) {
+    double q[NV];
+#pragma unroll
+    for (int i = 0; i < NV; i++) q[i] = G.q[(size_t)i * N + c];
+    const bool wet0 = q[0] > 0.0;
+    if (parts & 1) {
+      double X[NV];
+#pragma unroll
+      for (int i = 0; i < NV; i++) X[i] = G.R[(size_t)i * N + c];
+      stage2<NS>(P, dt, q, X, err);
+    }
+    if ((parts & 2) && (P.settling || P.entrainment)) {
+      double depo[NS];
+#pragma unroll
+      for (int i = 0; i < NS; i++) depo[i] = G.dep[(size_t)i * N + c];
+      erosion_deposition<NS>(P, dt, q, depo, err);
+#pragma unroll
+      for (int i = 0; i < NS; i++) G.dep[(size_t)i * N + c] = depo[i];
+    }
+    if (parts & 3) {
+#pragma unroll
+      for (int i = 0; i < NV; i++) G.q[(size_t)i * N + c] = q[i];
+    }
+    if (parts & 12) {
+      double hm = G.hmax[c], pm = G.pdyn[c], mv = G.mvel[c];
+      unsigned long long vb = G.vuln ? G.vuln[c] : 0ULL;
+      epilogue<NS>(P, q, hm, pm, mv, vb, (parts & 8) != 0);
+      G.hmax[c] = hm;
+      G.pdyn[c] = pm;
+      G.mvel[c] = mv;
+      if (G.vuln) G.vuln[c] = vb;
+    }
+    if ((parts & 3) && wet0 && q[0] == 0.0) dried = 1;
+  }
+  if (err) atomicOr(&G.st->err, err);
+  if (dried) atomicAdd(&G.st->dried, 1ULL);
+  if (mode == 1 && blockIdx.x == 0 && threadIdx.x == 0) G.st->t = G.st->t + dt;
+}
+
+// ===========================================================================
+// layout conversion and diagnostics
+// ===========================================================================
+__global__ void k_aos_to_soa(const double* aos, double* soa, int nv, int N) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < N)
+    for (int i = 0; i < nv; i++) soa[(size_t)i * N + c] = aos[(size_t)c * nv + i];
+}
+__global__ void k_soa_to_aos(const double* soa, double* aos, int nv, int N) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < N)
+    for (int i = 0; i < nv; i++) aos[(size_t)c * nv + i] = soa[(size_t)i * N + c];
+}
+
+// Neumaier-compensated partial sums of the stdout diagnostics, one
+// (sum, compensation) pair per block and quantity; finished on the host.
+template <int NS>
+__global__ __launch_bounds__(256) void k_diag(Params P, Grid G, double* part) {
+  constexpr int NV = 4 + NS, NP = NV + 2;
+  __shared__ double sh[4][256];
+  const int tid = threadIdx.x;
+  double acc[4] = {0, 0, 0, 0};
+  for (int c = blockIdx.x * blockDim.x + tid; c < G.N; c += gridDim.x * blockDim.x) {
+    double q[NV];
+#pragma unroll
+    for (int i = 0; i < NV; i++) q[i] = G.q[(size_t)i * G.N + c];
+    double h = 0.0;
+    if (q[0] > 0.0) {
+      double qp[NP], pd;
+      qc_to_qp<NS>(P, q, qp, pd);
+      h = qp[0];
+    }
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < NS; i++) s += q[4 + i];
+    acc[0] += q[0];
+    acc[1] += h;
+    acc[2] += (q[0] > 1.0e-7) ? 1.0 : 0.0;
+    acc[3] += s;
+  }
+  for (int m = 0; m < 4; m++) sh[m][tid] = acc[m];
+  __syncthreads();
+  if (tid < 4) {
+    double sum = 0.0, comp = 0.0;
+    for (int i = 0; i < 256; i++) {
+      const double x = sh[tid][i];
+      const double t = sum + x;
+      if (fabs(sum) >= fabs(x))
+        comp += (sum - t) + x;
+      else
+        comp += (x - t) + sum;
+      sum = t;
+    }
+    part[(blockIdx.x * 4 + tid) * 2] = sum;
+    part[(blockIdx.x * 4 + tid) * 2 + 1] = comp;
+  }
+}
+
+// ===========================================================================
+// per-function batch kernels (known-answer tests)
+// ===========================================================================
+template <int NS>
+__global__ void kt_qc_to_qp(Params P, int n, const double* qc, double* qp, double* pd) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) qc_to_qp<NS>(P, qc + (size_t)(4 + NS) * i, qp + (size_t)(6 + NS) * i, pd[i]);
+}
+template <int NS>
+__global__ void kt_qp_to_qc(Params P, int n, const double* qp, double* qc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) qp_to_qc<NS>(P, qp + (size_t)(6 + NS) * i, qc + (size_t)(4 + NS) * i);
+}
+template <int NS>
+__global__ void kt_fluxes(Params P, int n, int dir, const double* qc, const double* qp, double* f) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) phys_flux<NS>(P, qc + (size_t)(4 + NS) * i, qp + (size_t)(6 + NS) * i, dir, f + (size_t)(4 + NS) * i);
+}
+template <int NS>
+__global__ void kt_speeds(Params P, int n, int dir, const double* qp, double* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) local_speeds<NS>(P, qp + (size_t)(6 + NS) * i, dir, out[2 * i], out[2 * i + 1]);
+}
+template <int NS>
+__global__ void kt_implicit_cplx(Params P, int n, const double* qre, const double* qim, double* ore,
+                                 double* oim) {
+  constexpr int NV = 4 + NS;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    cx q[NV];
+    for (int m = 0; m < NV; m++) q[m] = C(qre[NV * i + m], qim[NV * i + m]);
+    cx n1, n2;
+    implicit_c<NS>(P, q, n1, n2);
+    for (int m = 0; m < NV; m++) {
+      ore[NV * i + m] = (m == 1) ? n1.re : ((m == 2) ? n2.re : 0.0);
+      oim[NV * i + m] = (m == 1) ? n1.im : ((m == 2) ? n2.im : 0.0);
+    }
+  }
+}
+template <int NS>
+__global__ void kt_semi_impl(Params P, int n, const double* qp, double* si) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) semi_impl<NS>(P, qp + (size_t)(6 + NS) * i, si + (size_t)(4 + NS) * i);
+}
+template <int NS>
+__global__ void kt_expl(Params P, int n, const double* bx, const double* by, double t,
+                        const double* qp, double* ex) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) expl_terms<NS>(P, bx[i], by[i], qp + (size_t)(6 + NS) * i, ex + (size_t)(4 + NS) * i, t, 0.0, 0.0, 0.0);
+}
+__global__ void kt_settling(Params P, int n, const double* d, const double* rs, const double* rc,
+                            const double* ikv, double* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = settling_velocity(P, d[i], rs[i], rc[i], ikv[i]);
+}
+template <int NS>
+__global__ void kt_mass_exchange(Params P, int n, const double* qp, double dt, double* dep,
+                                 double* eq) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) mass_exchange<NS>(P, qp + (size_t)(6 + NS) * i, dt, dep + (size_t)NS * i, eq + (size_t)(4 + NS) * i);
+}
+template <int NS>
+__global__ void kt_newton(Params P, int n, double dt, const double* qsi, const double* q0,
+                          const double* R, double* q, int* its) {
+  constexpr int NV = 4 + NS;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    double qq[NV];
+    for (int m = 0; m < NV; m++) qq[m] = qsi[NV * i + m];
+    unsigned int err = 0;
+    its[i] = solve_rk_step<NS>(P, dt, qq, q0 + (size_t)NV * i, R + (size_t)NV * i, err);
+    for (int m = 0; m < NV; m++) q[NV * i + m] = qq[m];
+  }
+}
+__global__ void kt_arith(int n, const double* a, const double* b, double* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    out[4 * i] = a[i] / b[i];
+    out[4 * i + 1] = sqrt(fabs(a[i]));
+    cx z = csqrt_c(C(a[i], b[i]));
+    out[4 * i + 2] = z.re;
+    out[4 * i + 3] = z.im;
+  }
+}
+
+// ===========================================================================
+// host side
+// ===========================================================================
+struct imex_handle {
+  imex_config cfg;
+  Params P;
+  Grid G;
+  hipStream_t stream = nullptr;
+  int device = 0;
+  int ntiles = 0;
+  size_t lds = 0;
+  std::vector<void*> allocs;
+  double* d_tmp = nullptr;  // N*nv scratch for layout conversion
+  double* d_diag = nullptr;
+  int diag_blocks = 0;
+  int dts_cap = 0;
+  std::string err;
+};
+
+static int set_err(imex_handle* h, const char* what, hipError_t e) {
+  if (h) {
+    h->err = std::string(what) + ": " + hipGetErrorString(e);
+  }
+  return -1;
+}
+#define HCHK(h, call)                                   \
+  do {                                                  \
+    hipError_t e_ = (call);                             \
+    if (e_ != hipSuccess) return set_err((h), #call, e_); \
+  } while (0)
+
+template <typename T>
+static int dalloc(imex_handle* h, T** p, size_t n) {
+  void* v = nullptr;
+  hipError_t e = hipMalloc(&v, n ? n * sizeof(T) : 16);
+  if (e != hipSuccess) return set_err(h, "hipMalloc", e);
+  HCHK(h, hipMemsetAsync(v, 0, n ? n * sizeof(T) : 16, h->stream));
+  h->allocs.push_back(v);
+  *p = (T*)v;
+  return 0;
+}
+
+template <int NS>
+static size_t flux_lds() { return flux_lds_bytes<NS>(); }
+
+static size_t lds_for(int ns) {
+  switch (ns) {
+    case 1: return flux_lds<1>();
+    case 2: return flux_lds<2>();
+    default: return flux_lds<3>();
+  }
+}
+
+extern "C" int imex_create(const imex_config* cfg, imex_handle** out) {
+  if (!cfg || !out) return -2;
+  *out = nullptr;
+  if (cfg->nx < 5 || cfg->ny < 5 || cfg->n_solid < 1 || cfg->n_solid > 3 ||
+      cfg->n_thickness_levels < 0 || cfg->n_thickness_levels > 8 || cfg->n_dyn_pres_levels < 0 ||
+      cfg->n_dyn_pres_levels > 8 || cfg->precision != 0)
+    return -3;
+  imex_handle* h = new imex_handle();
+  h->cfg = *cfg;
+  const char* dev = getenv("IMEX_DEVICE");
+  h->device = dev ? atoi(dev) : 0;
+  {
+    hipError_t e = hipSetDevice(h->device);
+    if (e != hipSuccess) {
+      set_err(h, "hipSetDevice", e);
+      *out = h;
+      return -1;
+    }
+    e = hipStreamCreate(&h->stream);
+    if (e != hipSuccess) {
+      set_err(h, "hipStreamCreate", e);
+      *out = h;
+      return -1;
+    }
+  }
+  Params& P = h->P;
+  memset(&P, 0, sizeof(P));
+  P.nx = cfg->nx;
+  P.ny = cfg->ny;
+  P.ns = cfg->n_solid;
+  P.nv = 4 + cfg->n_solid;
+  P.energy = cfg->energy_flag;
+  P.settling = cfg->settling_flag;
+  P.entrainment = cfg->entrainment_flag;
+  P.radial = cfg->radial_source_flag;
+  P.n_thk = cfg->n_thickness_levels;
+  P.n_pdyn = cfg->n_dyn_pres_levels;
+  if (P.n_thk * P.n_pdyn > 64) P.n_pdyn = 64 / (P.n_thk ? P.n_thk : 1);
+  P.dx = cfg->dx;
+  P.dy = cfg->dx;
+  P.dx2 = P.dx / 2.0;
+  P.dy2 = P.dy / 2.0;
+  P.one_by_dx = 1.0 / P.dx;
+  P.one_by_dy = 1.0 / P.dy;
+  P.cfl = cfg->cfl;
+  P.max_dt = cfg->max_dt;
+  P.grav = cfg->grav;
+  P.mu = cfg->mu;
+  P.xi = cfg->xi;
+  P.pres = cfg->pres;
+  P.inv_pres = 1.0 / cfg->pres;
+  P.T_amb = cfg->T_ambient;
+  P.cp_a = cfg->sp_heat_a;
+  P.R_a = cfg->sp_gas_const_a;
+  P.kin_visc_c = cfg->kin_visc_a;
+  P.rho_a_amb = cfg->pres / (cfg->sp_gas_const_a * cfg->T_ambient);
+  {
+    const double d4 = pow(cfg->dx, 4.0);
+    P.eps_sing = std::min(std::min(d4, d4), 1.0e-10);
+    const double e = P.eps_sing;
+    P.eps_sing4 = e * (e * (e * e));
+  }
+  for (int i = 0; i < 3; i++) {
+    P.rho_s[i] = cfg->rho_s[i];
+    P.inv_rho_s[i] = (i < cfg->n_solid) ? 1.0 / cfg->rho_s[i] : 0.0;
+    P.diam_s[i] = cfg->diam_s[i];
+    P.cp_s[i] = cfg->sp_heat_s[i];
+    P.alphas_source[i] = cfg->alphas_source[i];
+  }
+  P.T_erodible = 300.0;
+  P.rho_c_sub = cfg->pres / (cfg->sp_gas_const_a * P.T_erodible);
+  P.vel_source = cfg->vel_source;
+  P.T_source = cfg->T_source;
+  P.h_source = cfg->h_source;
+  for (int i = 0; i < 4; i++) P.time_param[i] = cfg->time_param[i];
+  P.pi_g = 4.0 * atan(1.0);
+  for (int i = 0; i < 8; i++) {
+    P.thk[i] = cfg->thickness_levels[i];
+    P.pdyn[i] = cfg->dyn_pres_levels[i];
+  }
+  P.hcs = (4 + cfg->n_solid) * 2.220446049250313e-16;
+  P.one_by_hcs = 1.0 / P.hcs;
+
+  Grid& G = h->G;
+  memset(&G, 0, sizeof(G));
+  G.nx = cfg->nx;
+  G.ny = cfg->ny;
+  G.N = cfg->nx * cfg->ny;
+  G.ntx = (cfg->nx + TX - 1) / TX;
+  G.nty = (cfg->ny + TY - 1) / TY;
+  h->ntiles = G.ntx * G.nty;
+  const size_t N = (size_t)G.N;
+  const int nv = P.nv, ns = P.ns;
+  int rc = 0;
+  *out = h;
+  if ((rc = dalloc(h, &G.q, N * nv))) return rc;
+  if ((rc = dalloc(h, &G.R, N * nv))) return rc;
+  if ((rc = dalloc(h, &G.hmax, N))) return rc;
+  if ((rc = dalloc(h, &G.pdyn, N))) return rc;
+  if ((rc = dalloc(h, &G.mvel, N))) return rc;
+  if ((rc = dalloc(h, &G.dep, N * ns))) return rc;
+  if (P.n_thk * P.n_pdyn > 0)
+    if ((rc = dalloc(h, &G.vuln, N))) return rc;
+  if ((rc = dalloc(h, &G.mask, N))) return rc;
+  if ((rc = dalloc(h, &G.tile_act, (size_t)h->ntiles))) return rc;
+  double *bx, *by, *ap, *ax, *ay, *xc, *xs, *yc, *ys;
+  signed char* sc;
+  unsigned char* sd;
+  if ((rc = dalloc(h, &bx, N))) return rc;
+  if ((rc = dalloc(h, &by, N))) return rc;
+  if ((rc = dalloc(h, &ap, N))) return rc;
+  if ((rc = dalloc(h, &ax, N))) return rc;
+  if ((rc = dalloc(h, &ay, N))) return rc;
+  if ((rc = dalloc(h, &sc, N))) return rc;
+  if ((rc = dalloc(h, &sd, N))) return rc;
+  if ((rc = dalloc(h, &xc, (size_t)G.nx))) return rc;
+  if ((rc = dalloc(h, &xs, (size_t)G.nx + 1))) return rc;
+  if ((rc = dalloc(h, &yc, (size_t)G.ny))) return rc;
+  if ((rc = dalloc(h, &ys, (size_t)G.ny + 1))) return rc;
+  G.bx = bx; G.by = by; G.arcp = ap; G.arcx = ax; G.arcy = ay;
+  G.scell = sc; G.sides = sd; G.xc = xc; G.xs = xs; G.yc = yc; G.ys = ys;
+  DevState* st;
+  if ((rc = dalloc(h, &st, 1))) return rc;
+  G.st = st;
+  if ((rc = dalloc(h, &h->d_tmp, N * nv))) return rc;
+  h->diag_blocks = 512;
+  if ((rc = dalloc(h, &h->d_diag, (size_t)h->diag_blocks * 8))) return rc;
+  // grid coordinates as init_grid builds them (src/geometry_2d.f90:312-330)
+  {
+    std::vector<double> hxs(G.nx + 1), hxc(G.nx), hys(G.ny + 1), hyc(G.ny);
+    for (int j = 0; j <= G.nx; j++) hxs[j] = cfg->x0 + j * cfg->dx;
+    for (int k = 0; k <= G.ny; k++) hys[k] = cfg->y0 + k * cfg->dx;
+    for (int j = 0; j < G.nx; j++) hxc[j] = 0.5 * (hxs[j] + hxs[j + 1]);
+    for (int k = 0; k < G.ny; k++) hyc[k] = 0.5 * (hys[k] + hys[k + 1]);
+    HCHK(h, hipMemcpyAsync(xs, hxs.data(), 8 * hxs.size(), hipMemcpyHostToDevice, h->stream));
+    HCHK(h, hipMemcpyAsync(xc, hxc.data(), 8 * hxc.size(), hipMemcpyHostToDevice, h->stream));
+    HCHK(h, hipMemcpyAsync(ys, hys.data(), 8 * hys.size(), hipMemcpyHostToDevice, h->stream));
+    HCHK(h, hipMemcpyAsync(yc, hyc.data(), 8 * hyc.size(), hipMemcpyHostToDevice, h->stream));
+    DevState s0;
+    memset(&s0, 0, sizeof(s0));
+    s0.dtmin_bits = 0x7ff0000000000000ULL;
+    HCHK(h, hipMemcpyAsync(st, &s0, sizeof(s0), hipMemcpyHostToDevice, h->stream));
+    HCHK(h, hipStreamSynchronize(h->stream));
+  }
+  h->lds = lds_for(ns);
+  switch (ns) {
+    case 1: HCHK(h, hipFuncSetAttribute((const void*)k_flux<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds)); break;
+    case 2: HCHK(h, hipFuncSetAttribute((const void*)k_flux<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds)); break;
+    default: HCHK(h, hipFuncSetAttribute((const void*)k_flux<3>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds)); break;
+  }
+  return 0;
+}
+
+extern "C" int imex_set_static(imex_handle* h, const imex_static* s) {
+  if (!h || !s || !s->B_prime_x || !s->B_prime_y) return -2;
+  HCHK(h, hipSetDevice(h->device));
+  const size_t N = (size_t)h->G.N;
+  std::vector<signed char> sc(N, 0);
+  std::vector<unsigned char> sd(N, 0);
+  for (size_t c = 0; c < N; c++) {
+    if (s->source_cell) sc[c] = (signed char)s->source_cell[c];
+    if (s->source_sides) sd[c] = (unsigned char)s->source_sides[c];
+  }
+  HCHK(h, hipMemcpyAsync((void*)h->G.bx, s->B_prime_x, 8 * N, hipMemcpyHostToDevice, h->stream));
+  HCHK(h, hipMemcpyAsync((void*)h->G.by, s->B_prime_y, 8 * N, hipMemcpyHostToDevice, h->stream));
+  if (s->arc_perim)
+    HCHK(h, hipMemcpyAsync((void*)h->G.arcp, s->arc_perim, 8 * N, hipMemcpyHostToDevice, h->stream));
+  if (s->arc_n_x)
+    HCHK(h, hipMemcpyAsync((void*)h->G.arcx, s->arc_n_x, 8 * N, hipMemcpyHostToDevice, h->stream));
+  if (s->arc_n_y)
+    HCHK(h, hipMemcpyAsync((void*)h->G.arcy, s->arc_n_y, 8 * N, hipMemcpyHostToDevice, h->stream));
+  HCHK(h, hipMemcpyAsync((void*)h->G.scell, sc.data(), N, hipMemcpyHostToDevice, h->stream));
+  HCHK(h, hipMemcpyAsync((void*)h->G.sides, sd.data(), N, hipMemcpyHostToDevice, h->stream));
+  HCHK(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+template <int NS>
+static void launch_update(imex_handle* h, int parts, double dt, int mode) {
+  const int nb = (h->G.N + 255) / 256;
+  hipLaunchKernelGGL(k_update<NS>, dim3(nb), dim3(256), 0, h->stream, h->P, h->G, parts, dt, mode);
+}
+static int update(imex_handle* h, int parts, double dt, int mode) {
+  switch (h->P.ns) {
+    case 1: launch_update<1>(h, parts, dt, mode); break;
+    case 2: launch_update<2>(h, parts, dt, mode); break;
+    default: launch_update<3>(h, parts, dt, mode); break;
+  }
+  HCHK(h, hipGetLastError());
+  return 0;
+}
+static int mask(imex_handle* h, int solve_all) {
+  hipLaunchKernelGGL(k_mask, dim3(h->ntiles), dim3(NTHR), 0, h->stream, h->P, h->G, solve_all);
+  HCHK(h, hipGetLastError());
+  return 0;
+}
+static int flux(imex_handle* h, int mode, double t, int step_idx) {
+  switch (h->P.ns) {
+    case 1: hipLaunchKernelGGL(k_flux<1>, dim3(h->ntiles), dim3(NTHR), h->lds, h->stream, h->P, h->G, mode, t, step_idx); break;
+    case 2: hipLaunchKernelGGL(k_flux<2>, dim3(h->ntiles), dim3(NTHR), h->lds, h->stream, h->P, h->G, mode, t, step_idx); break;
+    default: hipLaunchKernelGGL(k_flux<3>, dim3(h->ntiles), dim3(NTHR), h->lds, h->stream, h->P, h->G, mode, t, step_idx); break;
+  }
+  HCHK(h, hipGetLastError());
+  return 0;
+}
+
+static int read_state(imex_handle* h, DevState* s) {
+  HCHK(h, hipMemcpyAsync(s, h->G.st, sizeof(DevState), hipMemcpyDeviceToHost, h->stream));
+  HCHK(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+extern "C" int imex_upload_state(imex_handle* h, const double* q_aos, double t, double dt0) {
+  if (!h || !q_aos) return -2;
+  HCHK(h, hipSetDevice(h->device));
+  const int N = h->G.N, nv = h->P.nv;
+  HCHK(h, hipMemcpyAsync(h->d_tmp, q_aos, 8 * (size_t)N * nv, hipMemcpyHostToDevice, h->stream));
+  hipLaunchKernelGGL(k_aos_to_soa, dim3((N + 255) / 256), dim3(256), 0, h->stream, h->d_tmp, h->G.q, nv, N);
+  HCHK(h, hipGetLastError());
+  DevState s;
+  if (read_state(h, &s)) return -1;
+  s.t = t;
+  s.dt = dt0;
+  s.dt_old = dt0;
+  s.dt_old_old = dt0;
+  s.dtmin_bits = 0x7ff0000000000000ULL;
+  s.blocks_done = 0;
+  s.err = 0;
+  HCHK(h, hipMemcpyAsync(h->G.st, &s, sizeof(s), hipMemcpyHostToDevice, h->stream));
+  int rc;
+  if ((rc = mask(h, 1))) return rc;
+  if ((rc = update(h, 8, 0.0, 0))) return rc;
+  HCHK(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+extern "C" int imex_check_solve(imex_handle* h, int solve_all) {
+  if (!h) return -2;
+  HCHK(h, hipSetDevice(h->device));
+  return mask(h, solve_all);
+}
+
+extern "C" int imex_timestep(imex_handle* h, double t, double* dt_out) {
+  if (!h) return -2;
+  HCHK(h, hipSetDevice(h->device));
+  int rc;
+  if ((rc = flux(h, 0, t, 0))) return rc;
+  DevState s;
+  if (read_state(h, &s)) return -1;
+  if (dt_out) *dt_out = s.dt_cfl;
+  return 0;
+}
+
+static int take_err(imex_handle* h, unsigned* out) {
+  DevState s;
+  if (read_state(h, &s)) return -1;
+  *out = s.err;
+  return 0;
+}
+
+extern "C" int imex_rk_step(imex_handle* h, double dt) {
+  if (!h) return -2;
+  HCHK(h, hipSetDevice(h->device));
+  return update(h, 1, dt, 0);
+}
+
+extern "C" int imex_erosion_deposition(imex_handle* h, double dt) {
+  if (!h) return -2;
+  HCHK(h, hipSetDevice(h->device));
+  return update(h, 2, dt, 0);
+}
+
+extern "C" int imex_post_step(imex_handle* h, double t_new, imex_step_stats* out) {
+  if (!h) return -2;
+  HCHK(h, hipSetDevice(h->device));
+  int rc;
+  if ((rc = update(h, 4, 0.0, 0))) return rc;
+  DevState s;
+  if (read_state(h, &s)) return -1;
+  s.t = t_new;
+  unsigned err = s.err;
+  s.err = 0;
+  HCHK(h, hipMemcpyAsync(h->G.st, &s, sizeof(s), hipMemcpyHostToDevice, h->stream));
+  if (out) {
+    switch (h->P.ns) {
+      case 1: hipLaunchKernelGGL(k_diag<1>, dim3(h->diag_blocks), dim3(256), 0, h->stream, h->P, h->G, h->d_diag); break;
+      case 2: hipLaunchKernelGGL(k_diag<2>, dim3(h->diag_blocks), dim3(256), 0, h->stream, h->P, h->G, h->d_diag); break;
+      default: hipLaunchKernelGGL(k_diag<3>, dim3(h->diag_blocks), dim3(256), 0, h->stream, h->P, h->G, h->d_diag); break;
+    }
+    HCHK(h, hipGetLastError());
+    std::vector<double> part((size_t)h->diag_blocks * 8);
+    HCHK(h, hipMemcpyAsync(part.data(), h->d_diag, 8 * part.size(), hipMemcpyDeviceToHost, h->stream));
+    HCHK(h, hipStreamSynchronize(h->stream));
+    double tot[4];
+    for (int m = 0; m < 4; m++) {
+      double sum = 0.0, comp = 0.0;
+      for (int b = 0; b < h->diag_blocks; b++) {
+        const double x = part[(b * 4 + m) * 2];
+        comp += part[(b * 4 + m) * 2 + 1];
+        const double t = sum + x;
+        if (fabs(sum) >= fabs(x))
+          comp += (sum - t) + x;
+        else
+          comp += (x - t) + sum;
+        sum = t;
+      }
+      tot[m] = sum + comp;
+    }
+    const double a = h->P.dx * h->P.dy;
+    out->t = t_new;
+    out->dt = s.dt;
+    out->mass = a * tot[0];
+    out->volume = a * tot[1];
+    out->area = a * tot[2];
+    out->solid_mass = a * tot[3];
+    out->active_cells = (int64_t)s.active;
+    out->dried_cells = (int64_t)s.dried;
+    out->err = err;
+  } else {
+    HCHK(h, hipStreamSynchronize(h->stream));
+  }
+  return (int)(err & ~IMEX_WARN_RESCALE);
+}
+
+extern "C" int imex_run_steps(imex_handle* h, int nsteps, double t_end, double* dts) {
+  if (!h || nsteps < 0) return -2;
+  HCHK(h, hipSetDevice(h->device));
+  if (dts && nsteps > h->dts_cap) {
+    double* p = nullptr;
+    int rc = dalloc(h, &p, (size_t)nsteps);
+    if (rc) return rc;
+    h->G.dts = p;
+    h->dts_cap = nsteps;
+  }
+  if (!dts) h->G.dts = nullptr;
+  {
+    DevState s;
+    if (read_state(h, &s)) return -1;
+    s.t_end = t_end;
+    HCHK(h, hipMemcpyAsync(h->G.st, &s, sizeof(s), hipMemcpyHostToDevice, h->stream));
+  }
+  int rc;
+  for (int n = 0; n < nsteps; n++) {
+    if ((rc = mask(h, 0))) return rc;
+    if ((rc = flux(h, 1, 0.0, n))) return rc;
+    if ((rc = update(h, 1 | 2 | 4, 0.0, 1))) return rc;
+  }
+  if (dts && nsteps > 0)
+    HCHK(h, hipMemcpyAsync(dts, h->G.dts, 8 * (size_t)nsteps, hipMemcpyDeviceToHost, h->stream));
+  unsigned err = 0;
+  if (take_err(h, &err)) return -1;
+  return (int)(err & ~IMEX_WARN_RESCALE);
+}
+
+extern "C" int imex_get_time(imex_handle* h, double* t, double* dt) {
+  if (!h) return -2;
+  DevState s;
+  if (read_state(h, &s)) return -1;
+  if (t) *t = s.t;
+  if (dt) *dt = s.dt;
+  return 0;
+}
+
+extern "C" int imex_set_time(imex_handle* h, double t, double dt_old, double dt_old_old) {
+  if (!h) return -2;
+  HCHK(h, hipSetDevice(h->device));
+  DevState s;
+  if (read_state(h, &s)) return -1;
+  s.t = t;
+  s.dt_old = dt_old;
+  s.dt_old_old = dt_old_old;
+  HCHK(h, hipMemcpyAsync(h->G.st, &s, sizeof(s), hipMemcpyHostToDevice, h->stream));
+  HCHK(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+extern "C" int imex_download(imex_handle* h, unsigned what, double* q_aos, double* hmax,
+                             double* pdynmax, double* modvelmax, double* deposit, int32_t* vuln,
+                             int32_t* maskout) {
+  if (!h) return -2;
+  HCHK(h, hipSetDevice(h->device));
+  const int N = h->G.N, nv = h->P.nv, ns = h->P.ns;
+  if ((what & IMEX_DL_Q) && q_aos) {
+    hipLaunchKernelGGL(k_soa_to_aos, dim3((N + 255) / 256), dim3(256), 0, h->stream, h->G.q, h->d_tmp, nv, N);
+    HCHK(h, hipGetLastError());
+    HCHK(h, hipMemcpyAsync(q_aos, h->d_tmp, 8 * (size_t)N * nv, hipMemcpyDeviceToHost, h->stream));
+  }
+  if (what & IMEX_DL_STATS) {
+    if (hmax) HCHK(h, hipMemcpyAsync(hmax, h->G.hmax, 8 * (size_t)N, hipMemcpyDeviceToHost, h->stream));
+    if (pdynmax) HCHK(h, hipMemcpyAsync(pdynmax, h->G.pdyn, 8 * (size_t)N, hipMemcpyDeviceToHost, h->stream));
+    if (modvelmax) HCHK(h, hipMemcpyAsync(modvelmax, h->G.mvel, 8 * (size_t)N, hipMemcpyDeviceToHost, h->stream));
+  }
+  if ((what & IMEX_DL_DEPOSIT) && deposit)
+    HCHK(h, hipMemcpyAsync(deposit, h->G.dep, 8 * (size_t)N * ns, hipMemcpyDeviceToHost, h->stream));
+  std::vector<unsigned long long> vb;
+  std::vector<unsigned char> mb;
+  if ((what & IMEX_DL_VULN) && vuln && h->G.vuln) {
+    vb.resize(N);
+    HCHK(h, hipMemcpyAsync(vb.data(), h->G.vuln, 8 * (size_t)N, hipMemcpyDeviceToHost, h->stream));
+  }
+  if ((what & IMEX_DL_MASK) && maskout) {
+    mb.resize(N);
+    HCHK(h, hipMemcpyAsync(mb.data(), h->G.mask, (size_t)N, hipMemcpyDeviceToHost, h->stream));
+  }
+  HCHK(h, hipStreamSynchronize(h->stream));
+  if (!vb.empty()) {
+    const int nl = h->P.n_thk * h->P.n_pdyn;
+    for (int c = 0; c < N; c++)
+      for (int l = 0; l < nl; l++) vuln[(size_t)c * nl + l] = (int32_t)((vb[c] >> l) & 1ULL);
+  }
+  if (!mb.empty())
+    for (int c = 0; c < N; c++) maskout[c] = mb[c];
+  return 0;
+}
+
+extern "C" int imex_last_error(imex_handle* h, char* buf, size_t len) {
+  if (!h || !buf || len == 0) return -2;
+  snprintf(buf, len, "%s", h->err.c_str());
+  return 0;
+}
+
+extern "C" void imex_destroy(imex_handle* h) {
+  if (!h) return;
+  (void)hipSetDevice(h->device);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  for (void* p : h->allocs) (void)hipFree(p);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+}
+
+// ---- per-function batch entry points -------------------------------------
+struct DevBufs {
+  imex_handle* h;
+  std::vector<void*> p;
+  ~DevBufs() {
+    for (void* x : p) (void)hipFree(x);
+  }
+  double* up(const double* src, size_t n) {
+    void* d = nullptr;
+    if (hipMalloc(&d, n ? 8 * n : 16) != hipSuccess) return nullptr;
+    p.push_back(d);
+    if (src && hipMemcpy(d, src, 8 * n, hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+    return (double*)d;
+  }
+  int* upi(size_t n) {
+    void* d = nullptr;
+    if (hipMalloc(&d, n ? 4 * n : 16) != hipSuccess) return nullptr;
+    p.push_back(d);
+    return (int*)d;
+  }
+};
+
+#define NSDISPATCH(h, KER, ...)                                                             \
+  do {                                                                                      \
+    const int nb_ = (n + 127) / 128;                                                        \
+    switch ((h)->P.ns) {                                                                    \
+      case 1: hipLaunchKernelGGL(KER<1>, dim3(nb_), dim3(128), 0, 0, (h)->P, __VA_ARGS__); break; \
+      case 2: hipLaunchKernelGGL(KER<2>, dim3(nb_), dim3(128), 0, 0, (h)->P, __VA_ARGS__); break; \
+      default: hipLaunchKernelGGL(KER<3>, dim3(nb_), dim3(128), 0, 0, (h)->P, __VA_ARGS__); break; \
+    }                                                                                       \
+  } while (0)
+
+static int fin(imex_handle* h) {
+  HCHK(h, hipGetLastError());
+  HCHK(h, hipDeviceSynchronize());
+  return 0;
+}
+static int down(imex_handle* h, double* dst, const double* src, size_t n) {
+  HCHK(h, hipMemcpy(dst, src, 8 * n, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+extern "C" int imex_test_qc_to_qp(imex_handle* h, int n, const double* qc, double* qp, double* pdyn) {
+  if (!h || n <= 0) return -2;
+  const int nv = h->P.nv;
+  DevBufs b{h};
+  double *dq = b.up(qc, (size_t)n * nv), *dqp = b.up(nullptr, (size_t)n * (nv + 2)), *dp = b.up(nullptr, n);
+  NSDISPATCH(h, kt_qc_to_qp, n, dq, dqp, dp);
+  if (fin(h)) return -1;
+  if (down(h, qp, dqp, (size_t)n * (nv + 2)) || down(h, pdyn, dp, n)) return -1;
+  return 0;
+}
+extern "C" int imex_test_qp_to_qc(imex_handle* h, int n, const double* qp, double* qc) {
+  if (!h || n <= 0) return -2;
+  const int nv = h->P.nv;
+  DevBufs b{h};
+  double *dqp = b.up(qp, (size_t)n * (nv + 2)), *dq = b.up(nullptr, (size_t)n * nv);
+  NSDISPATCH(h, kt_qp_to_qc, n, dqp, dq);
+  if (fin(h)) return -1;
+  return down(h, qc, dq, (size_t)n * nv);
+}
+extern "C" int imex_test_fluxes(imex_handle* h, int n, int dir, const double* qc, const double* qp,
+                                double* flux) {
+  if (!h || n <= 0) return -2;
+  const int nv = h->P.nv;
+  DevBufs b{h};
+  double *dq = b.up(qc, (size_t)n * nv), *dqp = b.up(qp, (size_t)n * (nv + 2)), *df = b.up(nullptr, (size_t)n * nv);
+  NSDISPATCH(h, kt_fluxes, n, dir, dq, dqp, df);
+  if (fin(h)) return -1;
+  return down(h, flux, df, (size_t)n * nv);
+}
+extern "C" int imex_test_speeds(imex_handle* h, int n, int dir, const double* qp, double* out2) {
+  if (!h || n <= 0) return -2;
+  const int nv = h->P.nv;
+  DevBufs b{h};
+  double *dqp = b.up(qp, (size_t)n * (nv + 2)), *d = b.up(nullptr, (size_t)n * 2);
+  NSDISPATCH(h, kt_speeds, n, dir, dqp, d);
+  if (fin(h)) return -1;
+  return down(h, out2, d, (size_t)n * 2);
+}
+extern "C" int imex_test_implicit_cplx(imex_handle* h, int n, const double* qre, const double* qim,
+                                       double* ore, double* oim) {
+  if (!h || n <= 0) return -2;
+  const int nv = h->P.nv;
+  DevBufs b{h};
+  double *a = b.up(qre, (size_t)n * nv), *c = b.up(qim, (size_t)n * nv);
+  double *o1 = b.up(nullptr, (size_t)n * nv), *o2 = b.up(nullptr, (size_t)n * nv);
+  NSDISPATCH(h, kt_implicit_cplx, n, a, c, o1, o2);
+  if (fin(h)) return -1;
+  if (down(h, ore, o1, (size_t)n * nv) || down(h, oim, o2, (size_t)n * nv)) return -1;
+  return 0;
+}
+extern "C" int imex_test_semi_impl(imex_handle* h, int n, const double* qp, double* si) {
+  if (!h || n <= 0) return -2;
+  const int nv = h->P.nv;
+  DevBufs b{h};
+  double *dqp = b.up(qp, (size_t)n * (nv + 2)), *d = b.up(nullptr, (size_t)n * nv);
+  NSDISPATCH(h, kt_semi_impl, n, dqp, d);
+  if (fin(h)) return -1;
+  return down(h, si, d, (size_t)n * nv);
+}
+extern "C" int imex_test_expl(imex_handle* h, int n, const double* bx, const double* by, double t,
+                              const double* qp, double* expl) {
+  if (!h || n <= 0) return -2;
+  const int nv = h->P.nv;
+  DevBufs b{h};
+  double *dbx = b.up(bx, n), *dby = b.up(by, n), *dqp = b.up(qp, (size_t)n * (nv + 2));
+  double* d = b.up(nullptr, (size_t)n * nv);
+  NSDISPATCH(h, kt_expl, n, dbx, dby, t, dqp, d);
+  if (fin(h)) return -1;
+  return down(h, expl, d, (size_t)n * nv);
+}
+extern "C" int imex_test_settling(imex_handle* h, int n, const double* diam, const double* rhos,
+                                  const double* rhoc, const double* ikv, double* out) {
+  if (!h || n <= 0) return -2;
+  DevBufs b{h};
+  double *a = b.up(diam, n), *c = b.up(rhos, n), *d = b.up(rhoc, n), *e = b.up(ikv, n), *o = b.up(nullptr, n);
+  hipLaunchKernelGGL(kt_settling, dim3((n + 127) / 128), dim3(128), 0, 0, h->P, n, a, c, d, e, o);
+  if (fin(h)) return -1;
+  return down(h, out, o, n);
+}
+extern "C" int imex_test_mass_exchange(imex_handle* h, int n, const double* qp, double dt,
+                                       double* dep, double* eq) {
+  if (!h || n <= 0) return -2;
+  const int nv = h->P.nv, ns = h->P.ns;
+  DevBufs b{h};
+  double *dqp = b.up(qp, (size_t)n * (nv + 2)), *dd = b.up(nullptr, (size_t)n * ns), *de = b.up(nullptr, (size_t)n * nv);
+  NSDISPATCH(h, kt_mass_exchange, n, dqp, dt, dd, de);
+  if (fin(h)) return -1;
+  if (down(h, dep, dd, (size_t)n * ns) || down(h, eq, de, (size_t)n * nv)) return -1;
+  return 0;
+}
+extern "C" int imex_test_newton(imex_handle* h, int n, double dt, const double* q_si,
+                                const double* q0, const double* R, double* q, int32_t* iters) {
+  if (!h || n <= 0) return -2;
+  const int nv = h->P.nv;
+  DevBufs b{h};
+  double *a = b.up(q_si, (size_t)n * nv), *c = b.up(q0, (size_t)n * nv), *r = b.up(R, (size_t)n * nv);
+  double* o = b.up(nullptr, (size_t)n * nv);
+  int* it = b.upi(n);
+  NSDISPATCH(h, kt_newton, n, dt, a, c, r, o, it);
+  if (fin(h)) return -1;
+  if (down(h, q, o, (size_t)n * nv)) return -1;
+  HCHK(h, hipMemcpy(iters, it, 4 * (size_t)n, hipMemcpyDeviceToHost));
+  return 0;
+}
+extern "C" int imex_test_arith(imex_handle* h, int n, const double* a, const double* bb, double* out) {
+  if (!h || n <= 0) return -2;
+  DevBufs b{h};
+  double *da = b.up(a, n), *db = b.up(bb, n), *o = b.up(nullptr, (size_t)4 * n);
+  hipLaunchKernelGGL(kt_arith, dim3((n + 127) / 128), dim3(128), 0, 0, n, da, db, o);
+  if (fin(h)) return -1;
+  return down(h, out, o, (size_t)4 * n);
+}

@@ -1,0 +1,77 @@
+// Kernels of the FP64 IMEX-RK stage on gfx950.
+//
+// Per time step (one HIP stream, no host synchronisation in the device-side
+// driver loop):
+//   k_mask    active set (check_solve): seeds = wet cells, ring cells and the
+//             boundary frame, dilated by the diamond of radius n_RK = 2; one
+//             byte per cell plus a per-tile "has active cells" flag.
+//   k_flux    2-D tile of TX x TY cells with a two-cell halo in LDS:
+//             primitive states -> MUSCL/minmod face states -> local speeds ->
+//             Kurganov-Tadmor face fluxes -> flux divergence minus explicit
+//             terms (the stage-1 right-hand side, stored per active cell), and
+//             the CFL time-step candidates reduced per block and finished by
+//             the last block (device-side dt).
+//   k_update  one thread per active cell, state in registers: stage-2
+//             semi-implicit Coulomb friction, Newton solve of the turbulent
+//             friction (complex-step Jacobian, 2x2 Cramer, backtracking line
+//             search), assembly checks, sedimentation / entrainment, and the
+//             driver epilogue (running maxima, vulnerability table).
+#pragma once
+
+#include "imex_physics.h"
+
+namespace imex {
+
+constexpr int TX = 32;  // tile width (cells, x fastest)
+constexpr int TY = 8;   // tile height
+constexpr int HALO = 2;
+constexpr int RX = TX + 2 * HALO;
+constexpr int RY = TY + 2 * HALO;
+constexpr int NTHR = 256;
+
+// device-side driver state (one per handle)
+struct DevState {
+  double t, dt, dt_old, dt_old_old, t_end, dt_cfl;
+  unsigned long long dtmin_bits;
+  unsigned int blocks_done;
+  unsigned int err;
+  unsigned long long dried;
+  unsigned long long active;
+};
+
+struct Grid {
+  double* q;     // [nv][N] conservative state
+  double* R;     // [nv][N] stage-1 right-hand side divF - expl
+  double* hmax;  // [N]
+  double* pdyn;  // [N]
+  double* mvel;  // [N]
+  double* dep;   // [ns][N]
+  unsigned long long* vuln;  // [N] bit (i_thk*n_pdyn + i_pdyn)
+  unsigned char* mask;       // [N]
+  unsigned char* tile_act;   // [ntiles]
+  const double* bx;          // [N] B_prime_x_geom
+  const double* by;          // [N] B_prime_y_geom
+  const signed char* scell;  // [N] source_cell
+  const unsigned char* sides;  // [N] sourceE/W/N/S bits
+  const double* arcp;        // [N]
+  const double* arcx;        // [N]
+  const double* arcy;        // [N]
+  const double* xc;          // [nx]   x_comp
+  const double* xs;          // [nx+1] x_stag
+  const double* yc;          // [ny]
+  const double* ys;          // [ny+1]
+  DevState* st;
+  double* dts;               // per-step dt record of the device-side driver
+  int nx, ny, N, ntx, nty;
+};
+
+// Block index -> tile index so that each XCD (blocks are dispatched round-robin
+// over the 8 XCDs) works on a contiguous band of tiles and finds its halo
+// neighbours in its own L2.
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+  const int x = b & 7, slot = b >> 3;
+  const int q = nb >> 3, r = nb & 7;
+  return x * q + (x < r ? x : r) + slot;  // a bijection of 0..nb-1
+}
+
+}  // namespace imex

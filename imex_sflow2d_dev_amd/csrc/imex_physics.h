@@ -1,0 +1,661 @@
+// Per-cell and per-face physics of the IMEX-RK stage, as device functions.
+//
+// Every function evaluates the same floating-point operations in the same
+// order as the reference routine it is named after (cited per function), so
+// the FP64 instantiation reproduces the reference bit for bit wherever only
+// +, -, *, / and sqrt are involved.  This file is compiled with
+// -ffp-contract=off: no multiply-add may be fused.
+//
+// NS = number of solid classes, NV = 4 + NS conservative variables, primitive
+// vectors have NV + 2 entries (h, hu, hv, T, h*alpha_i, u, v).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+namespace imex {
+
+constexpr double kEps = 2.220446049250313e-16;  // EPSILON(1.0_wp)
+constexpr double kSqrt2 = 1.4142135623730951;     // SQRT(2.0_wp)
+
+// Run constants, copied to the device once per handle.
+struct Params {
+  int nx, ny, ns, nv;
+  int energy, settling, entrainment, radial;
+  int n_thk, n_pdyn;
+  double dx, dy, dx2, dy2, one_by_dx, one_by_dy;
+  double cfl, max_dt;
+  double grav, mu, xi, pres, inv_pres, T_amb, cp_a, R_a, kin_visc_c, rho_a_amb;
+  double eps_sing, eps_sing4;
+  double rho_s[3], inv_rho_s[3], diam_s[3], cp_s[3], alphas_source[3];
+  double rho_c_sub, T_erodible;
+  double vel_source, T_source, h_source, time_param[4], pi_g;
+  double thk[8], pdyn[8];
+  double hcs, one_by_hcs;  // complex step n_vars*eps and its reciprocal
+};
+
+__device__ __forceinline__ double fmin_f(double a, double b) { return (b < a) ? b : a; }
+__device__ __forceinline__ double fmax_f(double a, double b) { return (b > a) ? b : a; }
+
+// ---------------------------------------------------------------------------
+// complex arithmetic as the reference build evaluates it
+// ---------------------------------------------------------------------------
+struct cx {
+  double re, im;
+};
+__device__ __forceinline__ cx C(double re, double im) { return cx{re, im}; }
+__device__ __forceinline__ cx cadd(cx a, cx b) { return C(a.re + b.re, a.im + b.im); }
+__device__ __forceinline__ cx csub(cx a, cx b) { return C(a.re - b.re, a.im - b.im); }
+__device__ __forceinline__ cx cmul(cx a, cx b) {
+  return C(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
+}
+__device__ __forceinline__ cx cconj(cx a) { return C(a.re, -a.im); }
+__device__ __forceinline__ cx csq(cx a) {
+  return C(a.re * a.re - a.im * a.im, a.re * a.im + a.re * a.im);
+}
+
+// C99 Annex G.5.1 scaled division (logb / scalbn are exact operations).
+__device__ inline cx cdiv(cx z, cx w) {
+  double a = z.re, b = z.im, c = w.re, d = w.im;
+  int ilogbw = 0;
+  double logbw = logb(fmax(fabs(c), fabs(d)));
+  if (isfinite(logbw)) {
+    ilogbw = (int)logbw;
+    c = scalbn(c, -ilogbw);
+    d = scalbn(d, -ilogbw);
+  }
+  double denom = c * c + d * d;
+  double x = scalbn((a * c + b * d) / denom, -ilogbw);
+  double y = scalbn((b * c - a * d) / denom, -ilogbw);
+  if (isnan(x) && isnan(y)) {
+    if ((denom == 0.0) && (!isnan(a) || !isnan(b))) {
+      x = copysign(INFINITY, c) * a;
+      y = copysign(INFINITY, c) * b;
+    } else if ((isinf(a) || isinf(b)) && isfinite(c) && isfinite(d)) {
+      a = copysign(isinf(a) ? 1.0 : 0.0, a);
+      b = copysign(isinf(b) ? 1.0 : 0.0, b);
+      x = INFINITY * (a * c + b * d);
+      y = INFINITY * (b * c - a * d);
+    } else if (isinf(logbw) && logbw > 0.0 && isfinite(a) && isfinite(b)) {
+      c = copysign(isinf(c) ? 1.0 : 0.0, c);
+      d = copysign(isinf(d) ? 1.0 : 0.0, d);
+      x = 0.0 * (a * c + b * d);
+      y = 0.0 * (b * c - a * d);
+    }
+  }
+  return C(x, y);
+}
+
+// sqrt(x^2 + y^2) for finite x, y: scaled by a power of two, the sum of
+// squares is formed exactly as a double-double (each square split with an
+// exact product, Dekker/Knuth two-sum), then rounded once through sqrt and a
+// single correction step.  Exact (equal to |x|) whenever |y| is below half an
+// ulp of |x| in relative terms.
+__device__ inline double hypot_dd(double x, double y) {
+  x = fabs(x);
+  y = fabs(y);
+  if (x < y) {
+    double t = x;
+    x = y;
+    y = t;
+  }
+  if (x == 0.0 || isinf(x)) return x;
+  if (y == 0.0) return x;
+  int e = ilogb(x);
+  double xs = scalbn(x, -e), ys = scalbn(y, -e);
+  // exact squares as (hi, lo) pairs
+  double xh = xs * xs, xl = fma(xs, xs, -xh);
+  double yh = ys * ys, yl = fma(ys, ys, -yh);
+  // two-sum of the high parts
+  double s = xh + yh;
+  double bb = s - xh;
+  double err = (xh - (s - bb)) + (yh - bb);
+  double lo = err + xl + yl;
+  double r = sqrt(s + lo);
+  // one Newton correction: r += (s + lo - r*r) / (2r), r*r formed exactly
+  double rr = r * r, rrl = fma(r, r, -rr);
+  double resid = ((s - rr) + lo) - rrl;
+  r = r + resid / (2.0 * r);
+  return scalbn(r, e);
+}
+
+// csqrt with the case split of the C library implementation used by the
+// reference build: imaginary zero, real zero, and the general formula
+// r = sqrt((|z| + |x|)/2), s = y / (2 r).
+__device__ inline cx csqrt_c(cx z) {
+  double x = z.re, y = z.im;
+  if (y == 0.0) {
+    if (x < 0.0) return C(0.0, copysign(sqrt(-x), y));
+    return C(fabs(sqrt(x)), copysign(0.0, y));
+  }
+  if (x == 0.0) {
+    double r;
+    if (fabs(y) >= 2.0 * 2.2250738585072014e-308)
+      r = sqrt(0.5 * fabs(y));
+    else
+      r = 0.5 * sqrt(2.0 * fabs(y));
+    return C(r, copysign(r, y));
+  }
+  double d = hypot_dd(x, y);
+  double r, s;
+  if (x > 0.0) {
+    r = sqrt(0.5 * (d + x));
+    s = 0.5 * (y / r);
+  } else {
+    s = sqrt(0.5 * (d - x));
+    r = fabs(0.5 * (y / s));
+  }
+  return C(r, copysign(s, y));
+}
+
+// ===========================================================================
+// constitutive_2d
+// ===========================================================================
+
+// r_phys_var (configured branch) + qc_to_qp:
+// src/constitutive_2d.f90:550-976 and :1637-1690.  Also returns p_dyn.
+template <int NS>
+__device__ inline void qc_to_qp(const Params& P, const double* qc, double* qp, double& p_dyn) {
+  constexpr int NV = 4 + NS;
+  double xs[NS];
+  double inv_qj1;
+  if (qc[0] > kEps) {
+    inv_qj1 = 1.0 / qc[0];
+#pragma unroll
+    for (int i = 0; i < NS; i++) xs[i] = qc[4 + i] * inv_qj1;
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < NS; i++) s += qc[4 + i];
+    if (s == qc[0]) {
+      double sx = 0.0;
+#pragma unroll
+      for (int i = 0; i < NS; i++) sx += xs[i];
+#pragma unroll
+      for (int i = 0; i < NS; i++) xs[i] = xs[i] / sx;
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < NV + 2; i++) qp[i] = 0.0;
+    qp[3] = P.T_amb;
+    p_dyn = 0.0;
+    return;
+  }
+  double xs_tot = 0.0;
+#pragma unroll
+  for (int i = 0; i < NS; i++) xs_tot += xs[i];
+  double xc = 1.0 - xs_tot;
+  double cpc_by_xc = (xc - 0.0) * P.cp_a + 0.0;
+  double dot = 0.0;
+#pragma unroll
+  for (int i = 0; i < NS; i++) dot += xs[i] * P.cp_s[i];
+  double cp_mix = dot + cpc_by_xc;
+  double T;
+  if (qc[0] > P.eps_sing) {
+    if (P.energy)
+      T = (qc[3] - 0.5 * (qc[1] * qc[1] + qc[2] * qc[2]) * inv_qj1) / (qc[0] * cp_mix);
+    else
+      T = qc[3] / (qc[0] * cp_mix);
+    if (T <= 0.0) T = P.T_amb;
+  } else {
+    T = P.T_amb;
+  }
+  double Rc;
+  if (xc > kEps)
+    Rc = ((xc - 0.0) * P.R_a + 0.0) / xc;
+  else
+    Rc = P.R_a;
+  double inv_rho_c = Rc * T * P.inv_pres;
+  double inv_rhom = 0.0;
+#pragma unroll
+  for (int i = 0; i < NS; i++) inv_rhom += xs[i] * P.inv_rho_s[i];
+  inv_rhom = inv_rhom + xc * inv_rho_c;
+  double rm = 1.0 / inv_rhom;
+  double h = qc[0] * inv_rhom;
+  double u, v;
+  if (qc[0] > P.eps_sing) {
+    u = qc[1] * inv_qj1;
+    v = qc[2] * inv_qj1;
+  } else {
+    double q1 = qc[0];
+    double den = sqrt(q1 * (q1 * (q1 * q1)) + P.eps_sing4);
+    u = kSqrt2 * q1 * qc[1] / den;
+    v = kSqrt2 * q1 * qc[2] / den;
+  }
+  p_dyn = 0.5 * rm * (u * u + v * v);
+  qp[0] = h;
+  qp[1] = h * u;
+  qp[2] = h * v;
+  qp[3] = T;
+#pragma unroll
+  for (int i = 0; i < NS; i++) qp[4 + i] = xs[i] * rm * P.inv_rho_s[i] * h;
+  qp[NV] = u;
+  qp[NV + 1] = v;
+}
+
+// qp_to_qc (configured branch): src/constitutive_2d.f90:1718-2149
+template <int NS>
+__device__ inline void qp_to_qc(const Params& P, const double* qp, double* qc) {
+  constexpr int NV = 4 + NS;
+  double h = qp[0];
+  if (!(h > kEps)) {
+#pragma unroll
+    for (int i = 0; i < NV; i++) qc[i] = 0.0;
+    return;
+  }
+  double hu = qp[1], hv = qp[2], u = qp[NV], v = qp[NV + 1], T = qp[3];
+  double alphas[NS], ar[NS];
+#pragma unroll
+  for (int i = 0; i < NS; i++) alphas[i] = qp[4 + i] / qp[0];
+  double at = 0.0;
+#pragma unroll
+  for (int i = 0; i < NS; i++) at += alphas[i];
+#pragma unroll
+  for (int i = 0; i < NS; i++) ar[i] = alphas[i] * P.rho_s[i];
+  double rho_a = P.pres / (P.R_a * T);
+  double rho_c = (((1.0 - 0.0) - at - 0.0) * rho_a + 0.0) / ((1.0 - 0.0) - at);
+  double alphac = 1.0 - at;
+  double sar = 0.0;
+#pragma unroll
+  for (int i = 0; i < NS; i++) sar += ar[i];
+  double rho_m = alphac * rho_c + sar;
+  double inv_rhom = 1.0 / rho_m;
+  double xs[NS];
+#pragma unroll
+  for (int i = 0; i < NS; i++) xs[i] = ar[i] * inv_rhom;
+  double xc = alphac * rho_c * inv_rhom;
+  double cp_c;
+  if (xc > kEps)
+    cp_c = ((xc - 0.0) * P.cp_a + 0.0) / xc;
+  else
+    cp_c = P.cp_a;
+  double dot = 0.0;
+#pragma unroll
+  for (int i = 0; i < NS; i++) dot += xs[i] * P.cp_s[i];
+  double cp_mix = dot + xc * cp_c;
+  qc[0] = rho_m * h;
+  qc[1] = rho_m * hu;
+  qc[2] = rho_m * hv;
+  if (P.energy) {
+    if (h > 0.0)
+      qc[3] = h * rho_m * (cp_mix * T + 0.5 * (u * u + v * v));
+    else
+      qc[3] = 0.0;
+  } else {
+    qc[3] = h * rho_m * cp_mix * T;
+  }
+#pragma unroll
+  for (int i = 0; i < NS; i++) qc[4 + i] = xs[i] * qc[0];
+}
+
+struct Mixt {
+  double Ri, rho_m, rho_c, red_grav, cp_c, cp_mix;
+};
+
+// mixt_var (+ eval_sp_heat when sp_heat): src/constitutive_2d.f90:1342-1607
+template <int NS>
+__device__ inline Mixt mixt_var(const Params& P, const double* qp, bool sp_heat) {
+  constexpr int NV = 4 + NS;
+  Mixt m;
+  double h = qp[0];
+  if (qp[0] <= kEps) {
+    m.red_grav = 0.0;
+    m.rho_m = P.rho_a_amb;
+    m.rho_c = P.rho_a_amb;
+    m.cp_c = P.cp_a;
+    m.cp_mix = P.cp_a;
+    m.Ri = 0.0;
+    return m;
+  }
+  double u = qp[NV], v = qp[NV + 1], T = qp[3];
+  double alphas[NS];
+#pragma unroll
+  for (int i = 0; i < NS; i++) alphas[i] = qp[4 + i] / qp[0];
+  double at = 0.0;
+#pragma unroll
+  for (int i = 0; i < NS; i++) at += alphas[i];
+  double rho_a = P.pres / (P.R_a * T);
+  double rc = (((1.0 - 0.0) - at - 0.0) * rho_a + 0.0) / ((1.0 - 0.0) - at);
+  double dot = 0.0;
+#pragma unroll
+  for (int i = 0; i < NS; i++) dot += alphas[i] * P.rho_s[i];
+  double rm = (1.0 - at) * rc + dot;
+  double inv_rhom = 1.0 / rm;
+  double rg = (rm - P.rho_a_amb) / rm * P.grav;
+  double u2 = u * u + v * v;
+  m.Ri = (u2 > 0.0) ? fmin_f(1.0e15, rg * h / u2) : 0.0;
+  m.rho_m = rm;
+  m.rho_c = rc;
+  m.red_grav = rg;
+  m.cp_c = 0.0;
+  m.cp_mix = 0.0;
+  if (sp_heat) {
+    double xs[NS];
+#pragma unroll
+    for (int i = 0; i < NS; i++) xs[i] = alphas[i] * P.rho_s[i] * inv_rhom;
+    double sxs = 0.0;
+#pragma unroll
+    for (int i = 0; i < NS; i++) sxs += xs[i];
+    double xc = 1.0 - sxs;
+    double c = (xc > kEps) ? ((xc - 0.0) * P.cp_a + 0.0) / xc : P.cp_a;
+    double d2 = 0.0;
+#pragma unroll
+    for (int i = 0; i < NS; i++) d2 += xs[i] * P.cp_s[i];
+    m.cp_c = c;
+    m.cp_mix = d2 + xc * c;
+  }
+  return m;
+}
+
+// eval_local_speeds_x / _y with grav_coeff = 1: src/constitutive_2d.f90:2203-2307
+template <int NS>
+__device__ inline void local_speeds(const Params& P, const double* qp, int dir, double& vmin,
+                                    double& vmax) {
+  constexpr int NV = 4 + NS;
+  Mixt m = mixt_var<NS>(P, qp, false);
+  double h = qp[0];
+  double w = (dir == 1) ? qp[NV] : qp[NV + 1];
+  double rg = m.red_grav;
+  if (rg * h < 0.0) {
+    vmin = w;
+    vmax = w;
+  } else {
+    double cel = (dir == 1) ? sqrt(rg * h * 1.0) : sqrt(1.0 * rg * h);
+    vmin = w - cel;
+    vmax = w + cel;
+  }
+}
+
+// eval_fluxes with grav_coeff = 1: src/constitutive_2d.f90:2325-2518
+template <int NS>
+__device__ inline void phys_flux(const Params& P, const double* qc, const double* qp, int dir,
+                                 double* f) {
+  constexpr int NV = 4 + NS;
+  if (qp[0] > kEps) {
+    double h = qp[0];
+    double w = (dir == 1) ? qp[NV] : qp[NV + 1];
+    Mixt m = mixt_var<NS>(P, qp, false);
+    double Pr = 0.5 * m.rho_m * 1.0 * m.red_grav * (h * h);
+    f[0] = w * qc[0] * 1.0;
+    if (dir == 1) {
+      f[1] = w * qc[1] * 1.0 + Pr;
+      f[2] = w * qc[2] * 1.0;
+    } else {
+      f[1] = w * qc[1] * 1.0;
+      f[2] = w * qc[2] * 1.0 + Pr;
+    }
+    f[3] = P.energy ? w * (qc[3] * 1.0 + Pr) : w * qc[3] * 1.0;
+#pragma unroll
+    for (int i = 0; i < NS; i++) f[4 + i] = w * qc[4 + i] * 1.0;
+    double S = 0.0;
+#pragma unroll
+    for (int i = 0; i < NS; i++) S += f[4 + i];
+    if ((f[0] > 0.0) && (S / f[0] > 1.0)) {
+#pragma unroll
+      for (int i = 0; i < NS; i++) f[4 + i] = f[4 + i] / S * f[0];
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < NV; i++) f[i] = 0.0;
+  }
+}
+
+// eval_expl_terms (gravity, lateral radial volume source), grav_coeff = 1,
+// no curvature, d_grav_coeff = 0: src/constitutive_2d.f90:3024-3431
+template <int NS>
+__device__ inline void expl_terms(const Params& P, double Bx, double By, const double* qp,
+                                  double* ex, double time, double arc_p, double arc_x,
+                                  double arc_y) {
+  constexpr int NV = 4 + NS;
+#pragma unroll
+  for (int i = 0; i < NV; i++) ex[i] = 0.0;
+  if ((qp[0] <= kEps) && (arc_p == 0.0)) return;
+  if (qp[0] > kEps) {
+    double h = qp[0], u = qp[NV], v = qp[NV + 1];
+    Mixt m = mixt_var<NS>(P, qp, true);
+    double tg = m.red_grav + 0.0;
+    ex[1] = -(1.0 * m.rho_m * tg * h * Bx) + 0.5 * m.rho_m * m.red_grav * (h * h) * 0.0;
+    ex[2] = -(1.0 * m.rho_m * tg * h * By) + 0.5 * m.rho_m * m.red_grav * (h * h) * 0.0;
+    ex[3] = P.energy ? ex[1] * u + ex[2] * v : 0.0;
+  }
+  if (P.radial && (arc_p > 0.0) && (P.h_source > 0.0)) {
+    const double* tp = P.time_param;
+    double t_coeff;
+    if (time >= tp[3]) {
+      if ((tp[2] > 0.0) && (time < tp[3] + tp[2]))
+        t_coeff = 0.5 * (1.0 + cos(P.pi_g * (time - tp[3]) / tp[2]));
+      else
+        t_coeff = 0.0;
+    } else {
+      double t_rem = fmod(time, tp[0]);
+      t_coeff = 0.0;
+      if (tp[2] == 0.0) {
+        if (t_rem <= tp[1]) t_coeff = 1.0;
+      } else {
+        if (t_rem < tp[2])
+          t_coeff = 0.5 * (1.0 - cos(P.pi_g * t_rem / tp[2]));
+        else if (t_rem <= (tp[1] - tp[2]))
+          t_coeff = 1.0;
+        else if (t_rem <= tp[1])
+          t_coeff = 0.5 * (1.0 + cos(P.pi_g * ((t_rem - tp[1]) / tp[2] + 1.0)));
+      }
+    }
+    if (t_coeff > 0.0) {
+      double qs[NV + 2];
+#pragma unroll
+      for (int i = 0; i < NV + 2; i++) qs[i] = 0.0;
+      qs[0] = 1.0;
+      qs[3] = P.T_source;
+#pragma unroll
+      for (int i = 0; i < NS; i++) qs[4 + i] = P.alphas_source[i] * qs[0];
+      Mixt m = mixt_var<NS>(P, qs, true);
+      double h_dot = P.h_source * P.vel_source * arc_p / (P.dx * P.dy);
+      ex[0] = ex[0] + t_coeff * h_dot * m.rho_m;
+      ex[1] = ex[1] + t_coeff * h_dot * m.rho_m * P.vel_source * arc_x;
+      ex[2] = ex[2] + t_coeff * h_dot * m.rho_m * P.vel_source * arc_y;
+      ex[3] = ex[3] + t_coeff * h_dot * m.rho_m * m.cp_mix * P.T_source;
+#pragma unroll
+      for (int i = 0; i < NS; i++)
+        ex[4 + i] = ex[4 + i] + t_coeff * h_dot * P.alphas_source[i] * P.rho_s[i];
+    }
+  }
+}
+
+// eval_nh_semi_impl_terms, Voellmy Coulomb part, grav_coeff = 1:
+// src/constitutive_2d.f90:4027-4547 (:4164-4213)
+template <int NS>
+__device__ inline void semi_impl(const Params& P, const double* qp, double* si) {
+  constexpr int NV = 4 + NS;
+#pragma unroll
+  for (int i = 0; i < NV; i++) si[i] = 0.0;
+  double h = qp[0], u = qp[NV], v = qp[NV + 1];
+  Mixt m = mixt_var<NS>(P, qp, false);
+  double w = 0.0;
+  double mod_vel = sqrt(u * u + v * v + w * w);
+  if (mod_vel > 0.0) {
+    double tt = P.mu * m.rho_m * (m.red_grav + 0.0) * h * sqrt(1.0);
+    tt = fmax_f(0.0, tt);
+    tt = fmax_f(0.0, tt);
+    si[1] = si[1] - tt * u / mod_vel;
+    si[2] = si[2] - tt * v / mod_vel;
+  }
+}
+
+// c_phys_var + eval_implicit_terms (Voellmy turbulent part) in complex
+// arithmetic: src/constitutive_2d.f90:1154-1322, :3513-3683, :3994-4009.
+// Only the momentum rows are non-zero; they are returned in nh1, nh2.
+template <int NS>
+__device__ inline void implicit_c(const Params& P, const cx* qj, cx& nh1, cx& nh2) {
+  nh1 = C(0.0, 0.0);
+  nh2 = C(0.0, 0.0);
+  cx u, v, rho_m;
+  if (qj[0].re > kEps) {
+    cx inv_cqj1 = cdiv(C(1.0, 0.0), qj[0]);
+    cx xs[NS];
+#pragma unroll
+    for (int i = 0; i < NS; i++) xs[i] = cmul(qj[4 + i], inv_cqj1);
+    cx xs_tot = C(0.0, 0.0);
+#pragma unroll
+    for (int i = 0; i < NS; i++) xs_tot = cadd(xs_tot, xs[i]);
+    cx xc = csub(C(1.0, 0.0), xs_tot);
+    cx sp_heat_c = cdiv(cadd(cmul(csub(xc, C(0.0, 0.0)), C(P.cp_a, 0.0)), C(0.0, 0.0)), xc);
+    cx dot = C(0.0, 0.0);
+#pragma unroll
+    for (int i = 0; i < NS; i++) dot = cadd(dot, cmul(cconj(xs[i]), C(P.cp_s[i], 0.0)));
+    cx sp_heat_mix = cadd(dot, cmul(xc, sp_heat_c));
+    cx T;
+    if (qj[0].re > P.eps_sing) {
+      if (P.energy) {
+        cx ke = cmul(cmul(C(0.5, 0.0), cadd(csq(qj[1]), csq(qj[2]))), inv_cqj1);
+        T = cdiv(csub(qj[3], ke), cmul(qj[0], sp_heat_mix));
+      } else {
+        T = cdiv(qj[3], cmul(qj[0], sp_heat_mix));
+      }
+      if (T.re <= 0.0) T = C(P.T_amb, 0.0);
+    } else {
+      T = C(P.T_amb, 0.0);
+    }
+    cx Rc = cdiv(cadd(cmul(csub(xc, C(0.0, 0.0)), C(P.R_a, 0.0)), C(0.0, 0.0)), xc);
+    cx inv_rho_c = cmul(cmul(Rc, T), C(P.inv_pres, 0.0));
+    cx d2 = C(0.0, 0.0);
+#pragma unroll
+    for (int i = 0; i < NS; i++) d2 = cadd(d2, cmul(cconj(xs[i]), C(P.inv_rho_s[i], 0.0)));
+    cx inv_rhom = cadd(d2, cmul(xc, inv_rho_c));
+    rho_m = cdiv(C(1.0, 0.0), inv_rhom);
+    if (qj[0].re > P.eps_sing) {
+      u = cmul(qj[1], inv_cqj1);
+      v = cmul(qj[2], inv_cqj1);
+    } else {
+      cx c4 = cmul(qj[0], cmul(qj[0], cmul(qj[0], qj[0])));
+      cx den = csqrt_c(cadd(c4, C(P.eps_sing4, 0.0)));
+      cx s2 = C(kSqrt2, 0.0);
+      u = cdiv(cmul(cmul(s2, qj[0]), qj[1]), den);
+      v = cdiv(cmul(cmul(s2, qj[0]), qj[2]), den);
+    }
+  } else {
+    u = C(0.0, 0.0);
+    v = C(0.0, 0.0);
+    rho_m = C(P.rho_a_amb, 0.0);
+  }
+  cx red_grav = cmul(cdiv(csub(rho_m, C(P.rho_a_amb, 0.0)), rho_m), C(P.grav, 0.0));
+  cx w = C(0.0, 0.0);
+  cx mod_vel2 = cadd(cadd(csq(u), csq(v)), csq(w));
+  cx mod_vel = csqrt_c(mod_vel2);
+  if (mod_vel.re != 0.0) {
+    cx ts = cmul(cdiv(cmul(rho_m, red_grav), C(P.xi, 0.0)), mod_vel2);
+    nh1 = csub(nh1, cmul(ts, cdiv(u, mod_vel)));
+    nh2 = csub(nh2, cmul(ts, cdiv(v, mod_vel)));
+  }
+}
+
+// settling_velocity (Schiller-Naumann): src/constitutive_2d.f90:5326-5404
+__device__ inline double settling_velocity(const Params& P, double diam, double rhos,
+                                           double rhoc, double ikv) {
+  double inv_sqrt_C_D = 1.0;
+  if (rhos <= rhoc) return 0.0;
+  double const_part = sqrt(1.3333333333333333 * (rhos / rhoc - 1.0) * diam * P.grav);
+  double sv = const_part * inv_sqrt_C_D;
+  double Rey = diam * sv * ikv;
+  if (Rey <= 1000.0) {
+    for (int i = 0; i < 20; i++) {
+      double old = sv;
+      inv_sqrt_C_D = sqrt(Rey / (24.0 * (1.0 + 0.15 * pow(Rey, 0.687))));
+      sv = const_part * inv_sqrt_C_D;
+      if (fabs(old - sv) / old < 1.0e-6) return sv;
+      Rey = diam * sv * ikv;
+    }
+  }
+  return sv;
+}
+
+// eval_mass_exchange_terms without erosion / pore pressure / vaporisation:
+// src/constitutive_2d.f90:4574-5153.  Returns true when the reference would
+// pause on a negative deposition rate.
+template <int NS>
+__device__ inline bool mass_exchange(const Params& P, const double* qp, double dt, double* dep,
+                                     double* eq) {
+  constexpr int NV = 4 + NS;
+  double ers[NS];
+#pragma unroll
+  for (int i = 0; i < NS; i++) {
+    ers[i] = 0.0;
+    dep[i] = 0.0;
+  }
+#pragma unroll
+  for (int i = 0; i < NV; i++) eq[i] = 0.0;
+  if (qp[0] <= kEps) return false;
+  double h = qp[0], u = qp[NV], v = qp[NV + 1];
+  double alphas[NS];
+#pragma unroll
+  for (int i = 0; i < NS; i++) alphas[i] = qp[4 + i] / qp[0];
+  double at = 0.0;
+#pragma unroll
+  for (int i = 0; i < NS; i++) at += alphas[i];
+  double w = 0.0;
+  double mod_vel2 = u * u + v * v + w * w;
+#pragma unroll
+  for (int i = 0; i < NS; i++) ers[i] = fmax_f(0.0, fmin_f(ers[i], 0.0 / dt));
+  double tse = 0.0;
+#pragma unroll
+  for (int i = 0; i < NS; i++) tse += ers[i];
+  double cpe = (tse / (1.0 - 0.0)) * 0.0;
+  if (at <= 0.0) return false;
+  double T = qp[3];
+  Mixt m = mixt_var<NS>(P, qp, true);
+  double ikv = 1.0 / P.kin_visc_c;
+  bool err = false;
+#pragma unroll
+  for (int i = 0; i < NS; i++) {
+    if ((alphas[i] > 0.0) && P.settling) {
+      double sv = settling_velocity(P, P.diam_s[i], P.rho_s[i], m.rho_c, ikv);
+      dep[i] = alphas[i] * sv * 1.0;
+      dep[i] = dep[i] * pow(1.0 - fmin_f(1.0, at / 0.6), 4.65);
+      dep[i] = fmin_f(dep[i], h * alphas[i] / dt);
+      if (dep[i] < 0.0) err = true;
+    }
+  }
+  double sdep = 0.0;
+#pragma unroll
+  for (int i = 0; i < NS; i++) sdep += dep[i];
+  double cpl = 0.0 + 0.0 * sdep;
+  double air = 0.0;
+  if (P.entrainment && (h > 0.0) && (m.Ri > 0.0)) {
+    double ec = 0.075 / sqrt(1.0 + 718.0 * pow(m.Ri, 2.4));
+    air = ec * sqrt(mod_vel2);
+  }
+  double rdt = 0.0, rert = 0.0, A = 0.0, D = 0.0;
+#pragma unroll
+  for (int i = 0; i < NS; i++) rdt += P.rho_s[i] * dep[i];
+#pragma unroll
+  for (int i = 0; i < NS; i++) rert += P.rho_s[i] * ers[i];
+#pragma unroll
+  for (int i = 0; i < NS; i++) A += P.rho_s[i] * P.cp_s[i] * dep[i];
+#pragma unroll
+  for (int i = 0; i < NS; i++) D += P.rho_s[i] * P.cp_s[i] * ers[i];
+  double rc = m.rho_c, cpc = m.cp_c;
+  eq[0] = P.rho_a_amb * air + rert - rdt + P.rho_c_sub * cpe - rc * cpl;
+  eq[1] = -(u * (rdt + rc * cpl));
+  eq[2] = -(v * (rdt + rc * cpl));
+  if (P.energy)
+    eq[3] = -(T * (A + rc * cpc * cpl)) - 0.5 * mod_vel2 * (rdt + rc * cpl) +
+            P.T_erodible * (D + P.rho_c_sub * cpc * cpe) + P.T_amb * P.cp_a * P.rho_a_amb * air;
+  else
+    eq[3] = -(T * (A + rc * cpc * cpl)) + P.T_erodible * (D + P.rho_c_sub * cpc * cpe) +
+            P.T_amb * P.cp_a * P.rho_a_amb * air;
+#pragma unroll
+  for (int i = 0; i < NS; i++) eq[4 + i] = P.rho_s[i] * (ers[i] - dep[i]);
+  return err;
+}
+
+// limit with the minmod limiter: src/geometry_2d.f90:1776-1806, :1855-1876
+__device__ __forceinline__ double limit_minmod(double v0, double v1, double v2, double z0,
+                                               double z1, double z2) {
+  double a = (v2 - v1) / (z2 - z1);
+  double b = (v1 - v0) / (z1 - z0);
+  double m = fmin_f(fabs(a), fabs(b));
+  if (m <= 0.0) return 0.0;
+  double sa = a / fabs(a), sb = b / fabs(b);
+  return 0.5 * (sa + sb) * m;
+}
+
+}  // namespace imex

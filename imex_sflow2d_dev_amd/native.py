@@ -1,0 +1,177 @@
+"""ctypes binding of libimexhip.so (the C ABI declared in include/imex_hip.h).
+
+The library is built in-tree by imex_sflow2d_dev_amd/build.py.  There is no
+fallback: if the library is missing or no GPU is present every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from .config import RunConfig
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_lib", "libimexhip.so")
+
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int32)
+
+# error bits (include/imex_hip.h)
+ERR_NAN = 1
+ERR_NEG_H = 2
+ERR_NEG_SOLID = 4
+ERR_COLD = 8
+ERR_NEG_DEP = 16
+ERR_SOLID_GT_TOT = 32
+WARN_RESCALE = 64
+
+DL_Q, DL_STATS, DL_DEPOSIT, DL_VULN, DL_MASK = 1, 2, 4, 8, 16
+
+EXPORTS = [
+    "imex_create", "imex_set_static", "imex_upload_state", "imex_check_solve",
+    "imex_timestep", "imex_rk_step", "imex_erosion_deposition", "imex_post_step",
+    "imex_run_steps", "imex_get_time", "imex_set_time", "imex_download", "imex_last_error", "imex_destroy",
+    "imex_test_qc_to_qp", "imex_test_qp_to_qc", "imex_test_fluxes", "imex_test_speeds",
+    "imex_test_implicit_cplx", "imex_test_semi_impl", "imex_test_expl",
+    "imex_test_settling", "imex_test_mass_exchange", "imex_test_newton", "imex_test_arith",
+]
+
+
+class ImexConfig(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("n_solid", ctypes.c_int32),
+        ("energy_flag", ctypes.c_int32), ("settling_flag", ctypes.c_int32),
+        ("entrainment_flag", ctypes.c_int32), ("radial_source_flag", ctypes.c_int32),
+        ("n_thickness_levels", ctypes.c_int32), ("n_dyn_pres_levels", ctypes.c_int32),
+        ("precision", ctypes.c_int32),
+        ("dx", ctypes.c_double), ("x0", ctypes.c_double), ("y0", ctypes.c_double),
+        ("cfl", ctypes.c_double), ("max_dt", ctypes.c_double),
+        ("grav", ctypes.c_double), ("mu", ctypes.c_double), ("xi", ctypes.c_double),
+        ("pres", ctypes.c_double), ("T_ambient", ctypes.c_double),
+        ("sp_heat_a", ctypes.c_double), ("sp_gas_const_a", ctypes.c_double),
+        ("kin_visc_a", ctypes.c_double),
+        ("rho_s", ctypes.c_double * 3), ("diam_s", ctypes.c_double * 3),
+        ("sp_heat_s", ctypes.c_double * 3),
+        ("vel_source", ctypes.c_double), ("T_source", ctypes.c_double),
+        ("h_source", ctypes.c_double), ("time_param", ctypes.c_double * 4),
+        ("alphas_source", ctypes.c_double * 3),
+        ("thickness_levels", ctypes.c_double * 8), ("dyn_pres_levels", ctypes.c_double * 8),
+    ]
+
+
+class ImexStatic(ctypes.Structure):
+    _fields_ = [("B_prime_x", _D), ("B_prime_y", _D), ("source_cell", _I),
+                ("source_sides", _I), ("arc_perim", _D), ("arc_n_x", _D), ("arc_n_y", _D)]
+
+
+class ImexStepStats(ctypes.Structure):
+    _fields_ = [("t", ctypes.c_double), ("dt", ctypes.c_double), ("mass", ctypes.c_double),
+                ("volume", ctypes.c_double), ("area", ctypes.c_double),
+                ("solid_mass", ctypes.c_double), ("active_cells", ctypes.c_int64),
+                ("dried_cells", ctypes.c_int64), ("err", ctypes.c_uint32)]
+
+
+_LIB = None
+
+
+class NativeError(RuntimeError):
+    pass
+
+
+def lib():
+    """Loads libimexhip.so; raises if it has not been built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise NativeError(f"{LIB_PATH} is not built (python -m imex_sflow2d_dev_amd.build)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name in EXPORTS:
+            f = getattr(L, name)
+            f.restype = None if name == "imex_destroy" else ctypes.c_int
+        L.imex_create.argtypes = [ctypes.POINTER(ImexConfig), ctypes.POINTER(ctypes.c_void_p)]
+        _LIB = L
+    return _LIB
+
+
+def exported_symbols():
+    L = lib()
+    return {n: hasattr(L, n) for n in EXPORTS}
+
+
+def make_config(cfg: RunConfig) -> ImexConfig:
+    c = ImexConfig()
+    c.nx, c.ny, c.n_solid = cfg.nx, cfg.ny, cfg.n_solid
+    c.energy_flag = int(cfg.energy_flag)
+    c.settling_flag = int(cfg.settling_flag)
+    c.entrainment_flag = int(cfg.entrainment_flag)
+    c.radial_source_flag = int(cfg.radial_source_flag)
+    c.n_thickness_levels = len(cfg.thickness_levels)
+    c.n_dyn_pres_levels = len(cfg.dyn_pres_levels)
+    c.precision = 0
+    c.dx, c.x0, c.y0 = cfg.dx, cfg.x0, cfg.y0
+    c.cfl, c.max_dt = cfg.cfl, cfg.max_dt
+    c.grav, c.mu, c.xi, c.pres = cfg.grav, cfg.mu, cfg.xi, cfg.pres
+    c.T_ambient, c.sp_heat_a = cfg.T_ambient, cfg.sp_heat_a
+    c.sp_gas_const_a, c.kin_visc_a = cfg.sp_gas_const_a, cfg.kin_visc_a
+    for i in range(cfg.n_solid):
+        c.rho_s[i] = cfg.rho_s[i]
+        c.diam_s[i] = cfg.diam_s[i]
+        c.sp_heat_s[i] = cfg.sp_heat_s[i]
+        c.alphas_source[i] = cfg.alphas_source[i] if i < len(cfg.alphas_source) else 0.0
+    c.vel_source, c.T_source, c.h_source = cfg.vel_source, cfg.T_source, cfg.h_source
+    for i in range(4):
+        c.time_param[i] = cfg.time_param[i]
+    for i, v in enumerate(cfg.thickness_levels):
+        c.thickness_levels[i] = v
+    for i, v in enumerate(cfg.dyn_pres_levels):
+        c.dyn_pres_levels[i] = v
+    return c
+
+
+def _dp(a):
+    return a.ctypes.data_as(_D)
+
+
+def _ip(a):
+    return a.ctypes.data_as(_I)
+
+
+class Handle:
+    """Owns one imex_handle (device state of one run)."""
+
+    def __init__(self, cfg: RunConfig):
+        self.L = lib()
+        self.cfg = cfg
+        self.nv = cfg.n_vars
+        self._c = make_config(cfg)
+        h = ctypes.c_void_p()
+        rc = self.L.imex_create(ctypes.byref(self._c), ctypes.byref(h))
+        self.h = h
+        if rc != 0:
+            msg = self.last_error() if h.value else ""
+            self.close()
+            raise NativeError(f"imex_create failed ({rc}): {msg}")
+
+    def last_error(self) -> str:
+        buf = ctypes.create_string_buffer(512)
+        if self.h and self.h.value:
+            self.L.imex_last_error(self.h, buf, ctypes.c_size_t(512))
+        return buf.value.decode(errors="replace")
+
+    def check(self, rc: int, what: str) -> int:
+        if rc < 0:
+            raise NativeError(f"{what} failed ({rc}): {self.last_error()}")
+        return rc
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.L.imex_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

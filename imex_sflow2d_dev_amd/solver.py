@@ -1,0 +1,315 @@
+"""Host-side mirror of the reference's MODULE solver_2d (src/solver_2d.f90)
+over the device-resident HIP implementation.
+
+Method names and their meaning follow the reference's public procedures
+(check_solve, timestep, imex_RK_solver, update_erosion_deposition_cell) and
+the driver loop of PROGRAM IMEX_SfloW2D (src/IMEX_SfloW2D.f90:395-505).  The
+state never leaves the GPU between steps; the q / hmax / ... properties copy
+it back on demand (the reference's module arrays are read the same way by its
+output routines).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import native
+from .config import RunConfig
+
+
+class PhysicsError(RuntimeError):
+    """A condition on which the reference prints diagnostics and pauses."""
+
+    def __init__(self, bits: int):
+        names = [n for n, b in (("NaN", 1), ("negative thickness", 2),
+                                ("negative solid mass", 4), ("T < 273 K", 8),
+                                ("negative deposition", 16),
+                                ("solid mass > total mass", 32)) if bits & b]
+        super().__init__("physics error: " + ", ".join(names))
+        self.bits = bits
+
+
+class Solver:
+    def __init__(self, cfg: RunConfig, static: dict, q0: np.ndarray, t: float = None,
+                 raise_on_error: bool = True):
+        self.cfg = cfg
+        self.nv = cfg.n_vars
+        self.H = native.Handle(cfg)
+        self.L = self.H.L
+        self.h = self.H.h
+        self.raise_on_error = raise_on_error
+        self.t = cfg.t_start if t is None else t
+        self.dt = cfg.dt0
+        self.dt_old = cfg.dt0
+        self.dt_old_old = cfg.dt0
+        self.err = 0
+        self._set_static(static)
+        self._upload(q0)
+
+    # ---- set-up -------------------------------------------------------------
+    def _set_static(self, st: dict):
+        N = self.cfg.nx * self.cfg.ny
+
+        def d(name):
+            v = st.get(name)
+            if v is None:
+                return None
+            return np.ascontiguousarray(v, dtype=np.float64).reshape(N)
+
+        def i(name):
+            v = st.get(name)
+            if v is None:
+                return None
+            return np.ascontiguousarray(v, dtype=np.int32).reshape(N)
+
+        self._static = [d("B_prime_x"), d("B_prime_y"), i("source_cell"), i("source_sides"),
+                        d("arc_perim"), d("arc_n_x"), d("arc_n_y")]
+        s = native.ImexStatic()
+        names = ["B_prime_x", "B_prime_y", "source_cell", "source_sides", "arc_perim",
+                 "arc_n_x", "arc_n_y"]
+        for name, arr in zip(names, self._static):
+            if arr is not None:
+                ptr = native._ip(arr) if arr.dtype == np.int32 else native._dp(arr)
+                setattr(s, name, ptr)
+        self.H.check(self.L.imex_set_static(self.h, ctypes.byref(s)), "imex_set_static")
+
+    def _upload(self, q0: np.ndarray):
+        a = np.ascontiguousarray(q0, dtype=np.float64)
+        assert a.shape == (self.cfg.ny, self.cfg.nx, self.nv), a.shape
+        self.H.check(self.L.imex_upload_state(self.h, native._dp(a), ctypes.c_double(self.t),
+                                              ctypes.c_double(self.cfg.dt0)), "imex_upload_state")
+
+    # ---- reference procedures ------------------------------------------------
+    def check_solve(self, solve_all: bool = False):
+        self.H.check(self.L.imex_check_solve(self.h, ctypes.c_int(int(solve_all))), "check_solve")
+
+    def timestep(self) -> float:
+        dt = ctypes.c_double()
+        self.H.check(self.L.imex_timestep(self.h, ctypes.c_double(self.t), ctypes.byref(dt)),
+                     "timestep")
+        self.dt = dt.value
+        return self.dt
+
+    def imex_RK_solver(self):
+        self.H.check(self.L.imex_rk_step(self.h, ctypes.c_double(self.dt)), "imex_RK_solver")
+
+    def update_erosion_deposition_cell(self, dt: float):
+        self.H.check(self.L.imex_erosion_deposition(self.h, ctypes.c_double(dt)),
+                     "update_erosion_deposition_cell")
+
+    def post_step(self, want_stats: bool = False):
+        st = native.ImexStepStats()
+        rc = self.L.imex_post_step(self.h, ctypes.c_double(self.t),
+                                   ctypes.byref(st) if want_stats else None)
+        self.H.check(rc, "post_step")
+        self._raise(rc)
+        return st if want_stats else None
+
+    def _raise(self, rc: int):
+        if rc > 0:
+            self.err |= rc
+            if self.raise_on_error:
+                raise PhysicsError(rc)
+
+    # ---- driver --------------------------------------------------------------
+    def step(self, t_end: float = None):
+        """One pass of the driver loop body with the host-side dt clip."""
+        t_end = self.cfg.t_end if t_end is None else t_end
+        self.check_solve(False)
+        dt = self.timestep()
+        if self.t + dt > t_end:
+            dt = t_end - self.t
+        dt = min(dt, 1.1 * 0.5 * (self.dt_old + self.dt_old_old))
+        self.dt_old_old = self.dt_old
+        self.dt_old = dt
+        self.dt = dt
+        self.imex_RK_solver()
+        self.update_erosion_deposition_cell(dt)
+        self.t = self.t + dt
+        self.post_step()
+        return dt
+
+    def run_steps(self, nsteps: int, t_end: float = None, record_dt: bool = True):
+        """nsteps driver steps with the whole loop on the device."""
+        t_end = self.cfg.t_end if t_end is None else t_end
+        self._sync_time_to_device()
+        dts = np.zeros(max(nsteps, 1)) if record_dt else None
+        rc = self.L.imex_run_steps(self.h, ctypes.c_int(nsteps), ctypes.c_double(t_end),
+                                   native._dp(dts) if record_dt else None)
+        self.H.check(rc, "imex_run_steps")
+        t = ctypes.c_double()
+        dt = ctypes.c_double()
+        self.H.check(self.L.imex_get_time(self.h, ctypes.byref(t), ctypes.byref(dt)), "get_time")
+        self.t, self.dt = t.value, dt.value
+        if record_dt and nsteps > 0:
+            self.dt_old_old = dts[nsteps - 2] if nsteps > 1 else self.dt_old
+            self.dt_old = dts[nsteps - 1]
+        self._raise(rc)
+        return dts[:nsteps] if record_dt else None
+
+    def _sync_time_to_device(self):
+        # the device-side driver keeps t and the dt history in device memory;
+        # host-driven steps advance them on the host, so push them first
+        self.H.check(self.L.imex_set_time(self.h, ctypes.c_double(self.t),
+                                          ctypes.c_double(self.dt_old),
+                                          ctypes.c_double(self.dt_old_old)), "imex_set_time")
+
+    # ---- lazy host mirrors -----------------------------------------------------
+    def _download(self, what, **kw):
+        c = self.cfg
+        bufs = dict(q=None, hmax=None, pdynmax=None, mvel=None, dep=None, vuln=None, mask=None)
+        bufs.update(kw)
+        ptr = lambda a, f: f(a) if a is not None else None
+        self.H.check(self.L.imex_download(
+            self.h, ctypes.c_uint(what), ptr(bufs["q"], native._dp), ptr(bufs["hmax"], native._dp),
+            ptr(bufs["pdynmax"], native._dp), ptr(bufs["mvel"], native._dp),
+            ptr(bufs["dep"], native._dp), ptr(bufs["vuln"], native._ip),
+            ptr(bufs["mask"], native._ip)), "imex_download")
+
+    @property
+    def q(self) -> np.ndarray:
+        out = np.zeros((self.cfg.ny, self.cfg.nx, self.nv))
+        self._download(native.DL_Q, q=out)
+        return out
+
+    def fields(self) -> dict:
+        c = self.cfg
+        hm = np.zeros((c.ny, c.nx))
+        pd = np.zeros((c.ny, c.nx))
+        mv = np.zeros((c.ny, c.nx))
+        dep = np.zeros((c.n_solid, c.ny, c.nx))
+        self._download(native.DL_STATS | native.DL_DEPOSIT, hmax=hm, pdynmax=pd, mvel=mv, dep=dep)
+        return dict(hmax=hm, pdynmax=pd, mod_vel_max=mv, deposit=dep, t=self.t, dt=self.dt)
+
+    def vuln(self) -> np.ndarray:
+        c = self.cfg
+        nl = max(1, len(c.thickness_levels) * len(c.dyn_pres_levels))
+        v = np.zeros((c.ny, c.nx, nl), np.int32)
+        if len(c.thickness_levels) * len(c.dyn_pres_levels) > 0:
+            self._download(native.DL_VULN, vuln=v)
+        return v
+
+    def mask(self) -> np.ndarray:
+        m = np.zeros((self.cfg.ny, self.cfg.nx), np.int32)
+        self._download(native.DL_MASK, mask=m)
+        return m
+
+    def stats(self):
+        st = native.ImexStepStats()
+        self.H.check(self.L.imex_post_step(self.h, ctypes.c_double(self.t), ctypes.byref(st)),
+                     "stats")
+        return st
+
+    def close(self):
+        self.H.close()
+
+
+class KernelProbe:
+    """Per-function device evaluations (for known-answer tests)."""
+
+    def __init__(self, cfg: RunConfig):
+        self.cfg = cfg
+        self.nv = cfg.n_vars
+        self.H = native.Handle(cfg)
+        self.L = self.H.L
+
+    def _run(self, name, *args):
+        self.H.check(getattr(self.L, name)(self.H.h, *args), name)
+
+    def qc_to_qp(self, qc):
+        qc = np.ascontiguousarray(qc, np.float64)
+        n = qc.shape[0]
+        qp = np.zeros((n, self.nv + 2))
+        pd = np.zeros(n)
+        self._run("imex_test_qc_to_qp", ctypes.c_int(n), native._dp(qc), native._dp(qp),
+                  native._dp(pd))
+        return qp, pd
+
+    def qp_to_qc(self, qp):
+        qp = np.ascontiguousarray(qp, np.float64)
+        n = qp.shape[0]
+        qc = np.zeros((n, self.nv))
+        self._run("imex_test_qp_to_qc", ctypes.c_int(n), native._dp(qp), native._dp(qc))
+        return qc
+
+    def fluxes(self, qc, qp, direction):
+        qc = np.ascontiguousarray(qc, np.float64)
+        qp = np.ascontiguousarray(qp, np.float64)
+        n = qc.shape[0]
+        out = np.zeros((n, self.nv))
+        self._run("imex_test_fluxes", ctypes.c_int(n), ctypes.c_int(direction), native._dp(qc),
+                  native._dp(qp), native._dp(out))
+        return out
+
+    def speeds(self, qp, direction):
+        qp = np.ascontiguousarray(qp, np.float64)
+        n = qp.shape[0]
+        out = np.zeros((n, 2))
+        self._run("imex_test_speeds", ctypes.c_int(n), ctypes.c_int(direction), native._dp(qp),
+                  native._dp(out))
+        return out
+
+    def implicit_cplx(self, qre, qim):
+        qre = np.ascontiguousarray(qre, np.float64)
+        qim = np.ascontiguousarray(qim, np.float64)
+        n = qre.shape[0]
+        ore = np.zeros((n, self.nv))
+        oim = np.zeros((n, self.nv))
+        self._run("imex_test_implicit_cplx", ctypes.c_int(n), native._dp(qre), native._dp(qim),
+                  native._dp(ore), native._dp(oim))
+        return ore, oim
+
+    def semi_impl(self, qp):
+        qp = np.ascontiguousarray(qp, np.float64)
+        n = qp.shape[0]
+        out = np.zeros((n, self.nv))
+        self._run("imex_test_semi_impl", ctypes.c_int(n), native._dp(qp), native._dp(out))
+        return out
+
+    def expl(self, bx, by, t, qp):
+        bx = np.ascontiguousarray(bx, np.float64)
+        by = np.ascontiguousarray(by, np.float64)
+        qp = np.ascontiguousarray(qp, np.float64)
+        n = qp.shape[0]
+        out = np.zeros((n, self.nv))
+        self._run("imex_test_expl", ctypes.c_int(n), native._dp(bx), native._dp(by),
+                  ctypes.c_double(t), native._dp(qp), native._dp(out))
+        return out
+
+    def settling(self, diam, rhos, rhoc, ikv):
+        a = [np.ascontiguousarray(x, np.float64) for x in (diam, rhos, rhoc, ikv)]
+        n = a[0].shape[0]
+        out = np.zeros(n)
+        self._run("imex_test_settling", ctypes.c_int(n), *[native._dp(x) for x in a],
+                  native._dp(out))
+        return out
+
+    def mass_exchange(self, qp, dt):
+        qp = np.ascontiguousarray(qp, np.float64)
+        n = qp.shape[0]
+        dep = np.zeros((n, self.cfg.n_solid))
+        eq = np.zeros((n, self.nv))
+        self._run("imex_test_mass_exchange", ctypes.c_int(n), native._dp(qp), ctypes.c_double(dt),
+                  native._dp(dep), native._dp(eq))
+        return dep, eq
+
+    def newton(self, dt, q_si, q0, R):
+        a = [np.ascontiguousarray(x, np.float64) for x in (q_si, q0, R)]
+        n = a[0].shape[0]
+        out = np.zeros((n, self.nv))
+        its = np.zeros(n, np.int32)
+        self._run("imex_test_newton", ctypes.c_int(n), ctypes.c_double(dt),
+                  *[native._dp(x) for x in a], native._dp(out), native._ip(its))
+        return out, its
+
+    def arith(self, a, b):
+        a = np.ascontiguousarray(a, np.float64)
+        b = np.ascontiguousarray(b, np.float64)
+        n = a.shape[0]
+        out = np.zeros((n, 4))
+        self._run("imex_test_arith", ctypes.c_int(n), native._dp(a), native._dp(b), native._dp(out))
+        return out
+
+    def close(self):
+        self.H.close()

@@ -1,0 +1,157 @@
+/*
+ * imex_hip.h -- C ABI of the MI355X (gfx950) IMEX-RK stage of IMEX_SfloW2D.
+ *
+ * Drop-in boundary for MODULE solver_2d of the reference (src/solver_2d.f90).
+ * The Fortran driver keeps its time loop (src/IMEX_SfloW2D.f90:395-591) and
+ * calls these entry points through ISO_C_BINDING.  All
+ * state lives on the GPU in SoA layout; host arrays are borrowed only for the
+ * duration of a call.  Arrays crossing the boundary use the reference's
+ * Fortran order: q(n_vars, nx, ny) -> C index (k*nx + j)*n_vars + i.
+ *
+ * Return codes: 0 ok; > 0 physics error bit mask (the conditions on which the
+ * reference prints and pauses, see IMEX_ERR_*); < 0 HIP / API error, text in
+ * imex_last_error().  One host thread and one HIP stream per handle; calls
+ * execute in order on that stream.
+ */
+#ifndef IMEX_HIP_H
+#define IMEX_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* physics error bits (reference pause / warning conditions) */
+#define IMEX_ERR_NAN 1u          /* NaN after the RK stage, src/solver_2d.f90:1382-1397 */
+#define IMEX_ERR_NEG_H 2u        /* q1 < -1e-7 (RK) or < -1e-10 (sedimentation), :1401-1438, :2514-2538 */
+#define IMEX_ERR_NEG_SOLID 4u    /* solid mass <= -1e-7, :1440-1474, :2499-2511 */
+#define IMEX_ERR_COLD 8u         /* T < 273 K after the RK stage, :1541-1560 */
+#define IMEX_ERR_NEG_DEP 16u     /* negative deposition rate, src/constitutive_2d.f90:4860-4866 */
+#define IMEX_ERR_SOLID_GT_TOT 32u /* sum of solid mass > q1 after sedimentation, :2546-2566 */
+#define IMEX_WARN_RESCALE 64u    /* solid mass rescaled after the RK stage (reference warns, goes on) */
+
+/* Run configuration: the namelist values of the configured path (gas
+ * carrier, Voellmy friction, KT, minmod, IMEX RK2, Neumann boundaries).
+ * Replaces the module variables read_param sets (src/inpout_2d.f90:986-5134). */
+typedef struct imex_config {
+  int32_t nx, ny;              /* comp_cells_x, comp_cells_y */
+  int32_t n_solid;             /* 1..3 */
+  int32_t energy_flag;
+  int32_t settling_flag;
+  int32_t entrainment_flag;
+  int32_t radial_source_flag;
+  int32_t n_thickness_levels;  /* 0..8 */
+  int32_t n_dyn_pres_levels;   /* 0..8 */
+  int32_t precision;           /* 0 = FP64 drop-in (only value accepted) */
+  double dx;                   /* cell_size (dx = dy) */
+  double x0, y0;
+  double cfl, max_dt;
+  double grav, mu, xi, pres, T_ambient, sp_heat_a, sp_gas_const_a, kin_visc_a;
+  double rho_s[3], diam_s[3], sp_heat_s[3];
+  double vel_source, T_source, h_source, time_param[4], alphas_source[3];
+  double thickness_levels[8], dyn_pres_levels[8];
+} imex_config;
+
+/* Per-cell static fields computed once by the host (src/geometry_2d.f90:555-895
+ * topography_reconstruction, :907-1323 init_source).  x index fastest,
+ * nx*ny entries each.  source arrays may be NULL when there is no source. */
+typedef struct imex_static {
+  const double *B_prime_x;     /* B_prime_x_geom */
+  const double *B_prime_y;     /* B_prime_y_geom */
+  const int32_t *source_cell;  /* 0 / 1 (inside the source disc) / 2 (ring) */
+  const int32_t *source_sides; /* bit0 sourceE, bit1 sourceW, bit2 sourceN, bit3 sourceS */
+  const double *arc_perim;     /* cell_arc_perim */
+  const double *arc_n_x;       /* cell_arc_n_x */
+  const double *arc_n_y;       /* cell_arc_n_y */
+} imex_static;
+
+/* Display diagnostics of the driver's stdout line (src/IMEX_SfloW2D.f90:507-544),
+ * full-grid compensated (Neumaier) sums. */
+typedef struct imex_step_stats {
+  double t, dt;
+  double mass;        /* dx*dy*SUM(q(1,:,:)) */
+  double volume;      /* dx*dy*SUM(qp(1,:,:)) over wet cells */
+  double area;        /* dx*dy*COUNT(q(1,:,:) > 1e-7) */
+  double solid_mass;  /* dx*dy*SUM(q(5:4+n_solid,:,:)) */
+  int64_t active_cells; /* active cells summed over every check_solve since create */
+  int64_t dried_cells; /* cells that went from wet to exactly dry since create */
+  uint32_t err;        /* OR of physics error bits since the last call */
+} imex_step_stats;
+
+typedef struct imex_handle imex_handle;
+
+/* download selection bits */
+#define IMEX_DL_Q 1u
+#define IMEX_DL_STATS 2u   /* hmax, pdynmax, mod_vel_max */
+#define IMEX_DL_DEPOSIT 4u
+#define IMEX_DL_VULN 8u
+#define IMEX_DL_MASK 16u
+
+/* Replaces allocate_solver_variables (src/solver_2d.f90:264-541): allocates
+ * the device state for one run.  Device 0 unless IMEX_DEVICE is set. */
+int imex_create(const imex_config *cfg, imex_handle **h);
+/* Static fields (copied). */
+int imex_set_static(imex_handle *h, const imex_static *s);
+/* Initial or restart state, Fortran order q(n_vars,nx,ny); also sets t and dt
+ * history and runs the initial active set and driver epilogue
+ * (check_solve(.TRUE.), src/IMEX_SfloW2D.f90:265, :306-353). */
+int imex_upload_state(imex_handle *h, const double *q_aos, double t, double dt0);
+/* check_solve(solve_all), src/solver_2d.f90:667-858. */
+int imex_check_solve(imex_handle *h, int solve_all);
+/* timestep, src/solver_2d.f90:873-1006: the CFL dt (before the driver's
+ * clipping).  Also evaluates the stage-1 explicit terms at time t.  The only
+ * per-step blocking read-back (8 bytes). */
+int imex_timestep(imex_handle *h, double t, double *dt_out);
+/* imex_RK_solver, src/solver_2d.f90:1022-1634, with the driver's dt. */
+int imex_rk_step(imex_handle *h, double dt);
+/* update_erosion_deposition_cell(dt), src/solver_2d.f90:2382-2597. */
+int imex_erosion_deposition(imex_handle *h, double dt);
+/* driver epilogue (src/IMEX_SfloW2D.f90:449-505) and the stdout diagnostics. */
+int imex_post_step(imex_handle *h, double t_new, imex_step_stats *out);
+/* nsteps full driver steps on the device, no host synchronisation inside:
+ * check_solve, timestep, the driver's dt clip to t_end and growth limit
+ * (src/IMEX_SfloW2D.f90:417-439), imex_RK_solver, sedimentation, t += dt,
+ * epilogue.  dts (may be NULL) receives each step's dt at the end. */
+int imex_run_steps(imex_handle *h, int nsteps, double t_end, double *dts);
+/* current t and dt of the device-side driver */
+int imex_get_time(imex_handle *h, double *t, double *dt);
+/* set t and the dt history (dt_old, dt_old_old) of the device-side driver,
+ * e.g. after host-driven steps or a restart */
+int imex_set_time(imex_handle *h, double t, double dt_old, double dt_old_old);
+/* Lazy host mirrors: copies the selected device arrays (any pointer may be
+ * NULL when not selected).  q in Fortran order; deposit(nx,ny,n_solid);
+ * vuln as one int32 per (level, cell) in Fortran order; mask one int32 per cell. */
+int imex_download(imex_handle *h, unsigned what, double *q_aos, double *hmax,
+                  double *pdynmax, double *modvelmax, double *deposit, int32_t *vuln,
+                  int32_t *mask);
+int imex_last_error(imex_handle *h, char *buf, size_t len);
+void imex_destroy(imex_handle *h);
+
+/* Per-function batch entry points (device evaluation of single reference
+ * routines on n host-provided states; for known-answer tests).  States are
+ * n contiguous vectors of n_vars (qc) or n_vars+2 (qp) values. */
+int imex_test_qc_to_qp(imex_handle *h, int n, const double *qc, double *qp, double *pdyn);
+int imex_test_qp_to_qc(imex_handle *h, int n, const double *qp, double *qc);
+int imex_test_fluxes(imex_handle *h, int n, int dir, const double *qc, const double *qp,
+                     double *flux);
+int imex_test_speeds(imex_handle *h, int n, int dir, const double *qp, double *out2);
+int imex_test_implicit_cplx(imex_handle *h, int n, const double *qre, const double *qim,
+                            double *ore, double *oim);
+int imex_test_semi_impl(imex_handle *h, int n, const double *qp, double *si);
+int imex_test_expl(imex_handle *h, int n, const double *bx, const double *by, double t,
+                   const double *qp, double *expl);
+int imex_test_settling(imex_handle *h, int n, const double *diam, const double *rhos,
+                       const double *rhoc, const double *ikv, double *out);
+int imex_test_mass_exchange(imex_handle *h, int n, const double *qp, double dt,
+                            double *dep, double *eq);
+int imex_test_newton(imex_handle *h, int n, double dt, const double *q_si, const double *q0,
+                     const double *R, double *q, int32_t *iters);
+/* correctly rounded arithmetic probes: out[4i..4i+3] = a/b, sqrt(a), csqrt(a+ib) */
+int imex_test_arith(imex_handle *h, int n, const double *a, const double *b, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
