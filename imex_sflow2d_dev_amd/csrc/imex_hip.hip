@@ -1683,6 +1683,80 @@ extern "C" int imex_run_steps(imex_handle* h, int nsteps, double t_end, double* 
   return (int)(err & ~IMEX_WARN_RESCALE);
 }
 
+extern "C" int imex_run_steps_timed(imex_handle* h, int nsteps, double t_end, double* dts,
+                                    double* kernel_ms, int64_t* cells) {
+  if (!h || nsteps < 0) return -2;
+  HCHK(h, hipSetDevice(h->device));
+  if (dts && nsteps > h->dts_cap) {
+    double* p = nullptr;
+    int rc = dalloc(h, &p, (size_t)nsteps);
+    if (rc) return rc;
+    h->G.dts = p;
+    h->dts_cap = nsteps;
+  }
+  if (!dts) h->G.dts = nullptr;
+  DevState s;
+  if (read_state(h, &s)) return -1;
+  s.t_end = t_end;
+  const unsigned long long act0 = s.active;
+  HCHK(h, hipMemcpyAsync(h->G.st, &s, sizeof(s), hipMemcpyHostToDevice, h->stream));
+  std::vector<hipEvent_t> ev((size_t)nsteps * 4);
+  for (auto& e : ev) HCHK(h, hipEventCreate(&e));
+  int rc = 0;
+  for (int n = 0; n < nsteps && !rc; n++) {
+    HCHK(h, hipEventRecord(ev[4 * n + 0], h->stream));
+    if ((rc = mask(h, 0))) break;
+    HCHK(h, hipEventRecord(ev[4 * n + 1], h->stream));
+    if ((rc = flux(h, 1, 0.0, n))) break;
+    HCHK(h, hipEventRecord(ev[4 * n + 2], h->stream));
+    if ((rc = update(h, 1 | 2 | 4, 0.0, 1))) break;
+    HCHK(h, hipEventRecord(ev[4 * n + 3], h->stream));
+  }
+  HCHK(h, hipStreamSynchronize(h->stream));
+  double ms[3] = {0, 0, 0};
+  for (int n = 0; n < nsteps && !rc; n++)
+    for (int k = 0; k < 3; k++) {
+      float f = 0.f;
+      HCHK(h, hipEventElapsedTime(&f, ev[4 * n + k], ev[4 * n + k + 1]));
+      ms[k] += f;
+    }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  if (rc) return rc;
+  if (kernel_ms)
+    for (int k = 0; k < 3; k++) kernel_ms[k] = ms[k];
+  if (dts && nsteps > 0)
+    HCHK(h, hipMemcpyAsync(dts, h->G.dts, 8 * (size_t)nsteps, hipMemcpyDeviceToHost, h->stream));
+  if (read_state(h, &s)) return -1;
+  if (cells) cells[0] = (int64_t)(s.active - act0);
+  return (int)(s.err & ~IMEX_WARN_RESCALE);
+}
+
+__global__ void k_copy(const double* __restrict__ a, double* __restrict__ b, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) b[i] = a[i];
+}
+
+extern "C" int imex_calib_copy(imex_handle* h, int64_t n, int reps) {
+  if (!h || n <= 0 || reps <= 0) return -2;
+  HCHK(h, hipSetDevice(h->device));
+  double *a = nullptr, *b = nullptr;
+  HCHK(h, hipMalloc(&a, 8 * (size_t)n));
+  hipError_t e = hipMalloc(&b, 8 * (size_t)n);
+  if (e != hipSuccess) {
+    (void)hipFree(a);
+    return set_err(h, "hipMalloc", e);
+  }
+  HCHK(h, hipMemsetAsync(a, 0, 8 * (size_t)n, h->stream));
+  for (int r = 0; r < reps; r++)
+    hipLaunchKernelGGL(k_copy, dim3(2048), dim3(256), 0, h->stream, a, b, (long long)n);
+  e = hipStreamSynchronize(h->stream);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  if (e != hipSuccess) return set_err(h, "k_copy", e);
+  return 0;
+}
+
 extern "C" int imex_get_time(imex_handle* h, double* t, double* dt) {
   if (!h) return -2;
   DevState s;

@@ -115,6 +115,16 @@ int imex_post_step(imex_handle *h, double t_new, imex_step_stats *out);
  * (src/IMEX_SfloW2D.f90:417-439), imex_RK_solver, sedimentation, t += dt,
  * epilogue.  dts (may be NULL) receives each step's dt at the end. */
 int imex_run_steps(imex_handle *h, int nsteps, double t_end, double *dts);
+/* imex_run_steps with HIP events around every kernel launch on the handle's
+ * stream; kernel_ms[0..2] receive the summed durations of the active-set,
+ * flux and update kernels over the nsteps steps, cells[0] the active cells
+ * summed over the steps. */
+int imex_run_steps_timed(imex_handle *h, int nsteps, double t_end, double *dts,
+                         double *kernel_ms, int64_t *cells);
+/* Byte-counter calibration: copies n_doubles doubles between two device
+ * buffers reps times (reads and writes 8*n_doubles bytes per launch). */
+int imex_calib_copy(imex_handle *h, int64_t n_doubles, int reps);
+
 /* current t and dt of the device-side driver */
 int imex_get_time(imex_handle *h, double *t, double *dt);
 /* set t and the dt history (dt_old, dt_old_old) of the device-side driver,

@@ -509,6 +509,14 @@ CONTAINS
     q = qin
   END SUBROUTINE imexref_set_q
 
+  SUBROUTINE imexref_set_time(tt, dtt, d1, d2) BIND(C, name='imexref_set_time')
+    REAL(c_double), VALUE :: tt, dtt, d1, d2
+    t = tt
+    dt = dtt
+    dt_old = d1
+    dt_old_old = d2
+  END SUBROUTINE imexref_set_time
+
   SUBROUTINE imexref_get_fields(hm, pdm, mvm, dep, tt, dtt) BIND(C, name='imexref_get_fields')
     REAL(c_double), INTENT(OUT) :: hm(comp_cells_x, comp_cells_y)
     REAL(c_double), INTENT(OUT) :: pdm(comp_cells_x, comp_cells_y)

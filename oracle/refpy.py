@@ -35,7 +35,9 @@ def _lib(flavour: str = "ieee"):
         p = lib_path(flavour)
         if not os.path.exists(p):
             raise FileNotFoundError(f"{p} not built (make -C oracle ref)")
-        _LIBS[flavour] = ctypes.CDLL(p)
+        L = ctypes.CDLL(p)
+        L.imexref_eps_sing.restype = ctypes.c_double
+        _LIBS[flavour] = L
     return _LIBS[flavour]
 
 
@@ -120,6 +122,10 @@ class RefRun:
         if rc != 0:
             raise RuntimeError(f"imexref_run failed: {rc}")
         return dts[:nsteps], nc[:nsteps]
+
+    def set_time(self, t, dt, dt_old, dt_old_old):
+        self.lib.imexref_set_time(ctypes.c_double(t), ctypes.c_double(dt),
+                                  ctypes.c_double(dt_old), ctypes.c_double(dt_old_old))
 
     def q(self):
         """Conservative state as [k, j, var] (C order of the Fortran q(var,j,k))."""
@@ -267,6 +273,3 @@ class RefRun:
         return dep, eq
 
 
-for _name, _res in (("imexref_n_vars", ctypes.c_int), ("imexref_eps_sing", ctypes.c_double),
-                    ("imexref_setup", ctypes.c_int), ("imexref_run", ctypes.c_int)):
-    pass
