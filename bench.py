@@ -1,0 +1,290 @@
+"""Throughput of the FP64 IMEX-RK step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2]
+
+Workload (default "c2"): the production-size radial-source flow, 2048 x 2048
+cells of 5 m on a seeded fractal cone, two solid classes, sedimentation and
+air entrainment on, FP64.  One step is one driver step of the reference's time
+loop (src/IMEX_SfloW2D.f90:395-505): check_solve, timestep, imex_RK_solver
+(two RK stages), update_erosion_deposition_cell and the epilogue, all run on
+the device with no host synchronisation inside.  The run starts from the
+initial state, takes W untimed steps, then times exactly K steps (the default
+W=0, K=5000 times the whole 5000-step run).
+
+Metric: cell-stage updates/s = 2 * (active cells summed over the timed steps)
+/ wall time, where active cells are the check_solve set of each step.
+
+The JSON line also carries
+  roofline:     the dominant kernel's algorithmic HBM bytes per launch over
+                its mean launch time (HIP events on the launch stream), vs
+                8 TB/s; traffic = measured HBM bytes per launch when a
+                rocprofv3 FETCH_SIZE/WRITE_SIZE summary of that kernel is
+                committed under profiles/, else null;
+  cpu_baseline: the reference OpenMP Fortran build (compiled from its own
+                sources into oracle/_ref) timed on this host's cores over a
+                bounded sample of steps that starts from the GPU state at the
+                middle of the timed region.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def kernel_bytes_per_cell(cfg):
+    """Algorithmic HBM bytes per active cell per launch of each kernel.
+
+    k_mask:   read q(1), write the mask byte                      9
+    k_flux:   read q (n_v) and the bed slopes B'_x, B'_y          8 (n_v + 2)
+    k_update: read and write q (2 n_v), deposit read and write
+              with settling (16 n_s), hmax / pdynmax / mod_vel_max
+              read and write (48)                                 16 n_v + 16 n_s S + 48
+    Their sum is the whole-step figure 8 (3 n_v + 2) + 16 n_s S + 57.
+    Intermediates that a fused single pass would not need (the flux
+    divergence handed from k_flux to k_update, the tile flags) are not
+    counted: they show up in the measured traffic instead.
+    """
+    nv, ns = cfg.n_vars, cfg.n_solid
+    s = 1 if cfg.settling_flag else 0
+    per = {"k_mask": 9, "k_flux": 8 * (nv + 2), "k_update": 16 * nv + 16 * ns * s + 48}
+    per["step"] = sum(per.values())
+    return per
+
+
+def read_traffic(kernel: str, workload: str):
+    """Measured HBM bytes per active cell of one kernel, from the committed
+    rocprofv3 PMC summary profiles/<kernel>_<workload>_traffic.md (line
+    'bytes_per_active_cell: X'), or None."""
+    p = os.path.join(HERE, "profiles", f"{kernel}_{workload}_traffic.md")
+    if not os.path.exists(p):
+        return None
+    m = re.search(r"^bytes_per_active_cell:\s*([0-9.eE+-]+)", open(p).read(), re.M)
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(workload: str, state: dict, seconds: float):
+    """Runs oracle/ref_timing.py in a child process on a bounded sample."""
+    from oracle import refpy
+    if not refpy.available("perf"):
+        return None
+    fd, path = tempfile.mkstemp(suffix=".npz", prefix="imex_state_")
+    os.close(fd)
+    try:
+        np.savez(path, **state)
+        env = dict(os.environ)
+        env.setdefault("OMP_NUM_THREADS", str(os.cpu_count() or 1))
+        env.setdefault("OMP_STACKSIZE", "1G")
+        cmd = [sys.executable, "-m", "oracle.ref_timing", "--workload", workload,
+               "--state", path, "--seconds", str(seconds)]
+        r = subprocess.run(cmd, cwd=HERE, env=env, stdin=subprocess.DEVNULL,
+                           capture_output=True, text=True, timeout=600)
+        if r.returncode != 0:
+            print(f"cpu_baseline: reference run failed ({r.returncode}): {r.stderr[-2000:]}",
+                  file=sys.stderr, flush=True)
+            return None
+        line = [ln for ln in r.stdout.splitlines() if ln.startswith("{")][-1]
+        res = json.loads(line)
+    finally:
+        os.unlink(path)
+    rate = 2.0 * res["active_cells"] / res["elapsed_s"]
+    return {
+        "value": rate, "unit": "cell-stage updates/s", "cores": res["threads"],
+        "kind": "reference",
+        "sample": (f"{res['steps']} driver steps of the same run from its state at step "
+                   f"{state['step']}, {res['active_cells']} active cells in total, "
+                   f"{res['elapsed_s']:.1f} s; reference src/*.f90 built with "
+                   f"amdflang -O3 -ffast-math -fstack-arrays -funroll-loops -fopenmp, "
+                   f"OMP_NUM_THREADS={res['threads']}"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5000)
+    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the reference CPU sample")
+    ap.add_argument("--calib-copy", type=int, default=0, metavar="N",
+                    help="only run the plain copy kernel over N doubles 20 times (the "
+                         "reference point for rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
+    ap.add_argument("--info", default=None,
+                    help="also write per-kernel launch counts and active cells to this JSON file")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU decomposition is a design, not part of this build)",
+              file=sys.stderr)
+        sys.exit(2)
+    if args.steps < 1 or args.warmup < 0:
+        sys.exit("--steps must be >= 1 and --warmup >= 0")
+
+    from imex_sflow2d_dev_amd import init, native, workloads
+    from imex_sflow2d_dev_amd.solver import Solver
+    if not os.path.exists(native.LIB_PATH):
+        from imex_sflow2d_dev_amd import build
+        build.build()
+
+    if args.calib_copy:
+        h = native.Handle(workloads.by_name("c2-small")[0])
+        h.check(h.L.imex_calib_copy(h.h, ctypes.c_int64(args.calib_copy), 20), "imex_calib_copy")
+        h.close()
+        print(json.dumps({"calib_copy_doubles": args.calib_copy, "launches": 20,
+                          "bytes_read_per_launch": 8 * args.calib_copy,
+                          "bytes_written_per_launch": 8 * args.calib_copy}), flush=True)
+        return
+
+    t_host = time.perf_counter()
+    cfg, B = workloads.by_name(args.workload)
+    static, q0 = init.setup(cfg, B)
+    print(f"[bench] workload {args.workload}: {cfg.nx}x{cfg.ny} n_solid={cfg.n_solid} "
+          f"host set-up {time.perf_counter() - t_host:.1f} s", file=sys.stderr, flush=True)
+
+    # load code objects and touch every kernel once on a throw-away run
+    s = Solver(cfg, static, q0)
+    s.run_steps(3)
+    s.close()
+
+    s = Solver(cfg, static, q0)
+    total_cells_all = 0
+    if args.warmup:
+        k = np.zeros(3)
+        c = np.zeros(1, np.int64)
+        s.H.check(s.L.imex_run_steps_timed(s.h, args.warmup, ctypes.c_double(cfg.t_end), None,
+                                           native._dp(k), _i64p(c)), "warmup")
+        total_cells_all += int(c[0])
+    s.t, s.dt = s_time(s)
+    print(f"[bench] warmup {args.warmup} steps done, t={s.t:.4f}", file=sys.stderr, flush=True)
+
+    # timed region: two halves, with the mid-run state copied out between
+    # them (outside the clock) for the CPU sample
+    halves = [args.steps // 2, args.steps - args.steps // 2]
+    kernel_ms = np.zeros(3)
+    cells = 0
+    wall = 0.0
+    mid = None
+    dts_all = []
+    err_bits = 0
+    for i, n in enumerate(halves):
+        if n == 0:
+            continue
+        k = np.zeros(3)
+        c = np.zeros(1, np.int64)
+        dts = np.zeros(n)
+        s_time(s)                               # synchronises the stream
+        t0 = time.perf_counter()
+        rc = s.L.imex_run_steps_timed(s.h, n, ctypes.c_double(cfg.t_end), native._dp(dts),
+                                      native._dp(k), _i64p(c))
+        t1 = time.perf_counter()                # the call returns after a stream sync
+        s.H.check(rc, "imex_run_steps_timed")
+        err_bits |= rc
+        wall += t1 - t0
+        kernel_ms += k
+        cells += int(c[0])
+        dts_all.append(dts)
+        print(f"[bench] timed part {i + 1}: {n} steps, {t1 - t0:.3f} s, "
+              f"{int(c[0])} active cells", file=sys.stderr, flush=True)
+        if i == 0:
+            t, _ = s_time(s)
+            dd = np.concatenate(dts_all)
+            mid = dict(q=s.q, t=t, dt_old=dd[-1] if len(dd) else cfg.dt0,
+                       dt_old_old=dd[-2] if len(dd) > 1 else cfg.dt0,
+                       step=args.warmup + n)
+    total_cells_all += cells
+    if mid is None:
+        t, _ = s_time(s)
+        mid = dict(q=s.q, t=t, dt_old=cfg.dt0, dt_old_old=cfg.dt0, step=args.warmup + args.steps)
+    t_end_phys, _ = s_time(s)
+    s.close()
+
+    K = args.steps
+    value = 2.0 * cells / wall
+    per = kernel_bytes_per_cell(cfg)
+    names = ["k_mask", "k_flux", "k_update"]
+    dom = 1 + int(np.argmax(kernel_ms[1:]))
+    dname = names[dom]
+    mean_launch_s = kernel_ms[dom] * 1e-3 / K
+    cells_per_launch = cells / K
+    alg_bytes = per[dname] * cells_per_launch
+    achieved = alg_bytes / mean_launch_s / 1e9
+    tb = read_traffic(dname, args.workload)
+    roofline = {
+        "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS,
+        "traffic": (tb * cells_per_launch) if tb is not None else None,
+        "kernel": dname,
+        "alg_bytes_per_active_cell": per[dname],
+        "mean_launch_us": mean_launch_s * 1e6,
+        "mean_active_cells_per_launch": cells_per_launch,
+    }
+    step_gbs = per["step"] * cells / wall / 1e9
+    out = {
+        "metric": "cell-stage updates/s",
+        "value": value,
+        "unit": "cell-stage updates/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / K,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded PCG64 DEM and source, generated in-process)",
+        "config": {
+            "workload": (f"{args.workload}: {cfg.nx}x{cfg.ny} cells dx={cfg.dx} m, "
+                         f"{cfg.n_solid} solid classes, settling={cfg.settling_flag}, "
+                         f"radial source={cfg.radial_source_flag}, FP64, driver steps "
+                         f"{args.warmup + 1}..{args.warmup + K}"),
+            "grid": [cfg.nx, cfg.ny],
+            "active_fraction": cells / (K * cfg.nx * cfg.ny),
+            "grid_cell_stage_updates_per_s": 2.0 * cfg.nx * cfg.ny * K / wall,
+            "t_physical_end_s": t_end_phys,
+            "step_hbm_alg_GBs": step_gbs,
+            "step_alg_bytes_per_active_cell": per["step"],
+            "kernel_ms": dict(zip(names, [float(x) for x in kernel_ms])),
+            "physics_error_bits": err_bits,
+        },
+        "roofline": roofline,
+        "cpu_baseline": None,
+    }
+    if args.info:
+        with open(args.info, "w") as f:
+            json.dump({"steps_total": args.warmup + K + 3, "active_cells_total": total_cells_all,
+                       "active_cells_timed": cells, "kernel_ms": out["config"]["kernel_ms"],
+                       "alg_bytes": per}, f, indent=1)
+    if not args.no_cpu:
+        print("[bench] CPU reference sample ...", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(args.workload, mid, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+def _i64p(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+
+
+def s_time(s):
+    """(t, dt) of the device-side driver; synchronises the handle's stream."""
+    t = ctypes.c_double()
+    dt = ctypes.c_double()
+    s.H.check(s.L.imex_get_time(s.h, ctypes.byref(t), ctypes.byref(dt)), "get_time")
+    return t.value, dt.value
+
+
+if __name__ == "__main__":
+    main()

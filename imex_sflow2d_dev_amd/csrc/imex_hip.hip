@@ -22,7 +22,7 @@ using namespace imex;
 // ===========================================================================
 // active set: check_solve, src/solver_2d.f90:667-858
 // ===========================================================================
-__global__ __launch_bounds__(NTHR) void k_mask(Params P, Grid G, int solve_all) {
+__global__ __launch_bounds__(NTHR) void k_mask(Params P, Grid G, int solve_all, int par) {
   __shared__ unsigned char seed[RY][RX];
   __shared__ int s_any;
   __shared__ int s_cnt;
@@ -72,7 +72,11 @@ __global__ __launch_bounds__(NTHR) void k_mask(Params P, Grid G, int solve_all) 
   }
   __syncthreads();
   if (tid == 0) {
-    G.tile_act[tile] = (unsigned char)s_any;
+    if (blockIdx.x == 0) G.st->n_tiles[par ^ 1] = 0u;  // the next step's slot
+    if (s_any) {
+      const unsigned int slot = atomicAdd(&G.st->n_tiles[par], 1u);
+      G.tile_list[(size_t)par * G.ntx * G.nty + slot] = tile;
+    }
     if (s_cnt) atomicAdd(&G.st->active, (unsigned long long)s_cnt);
   }
 }
@@ -107,7 +111,7 @@ __device__ __forceinline__ void qp_to_qp2(const double* qp, double& u, double& v
 
 // x direction: W and E face states (primitive) of cell (j,k) at local index idx.
 template <int NS>
-__device__ void recon_x(const Params& P, const Grid& G, const TileView& T, int idx, int j, int k,
+__device__ __forceinline__ void recon_x(const Params& P, const Grid& G, const TileView& T, int idx, int j, int k,
                         double* W, double* E) {
   constexpr int NV = 4 + NS;
   const int nx = G.nx;
@@ -214,7 +218,7 @@ __device__ void recon_x(const Params& P, const Grid& G, const TileView& T, int i
 
 // y direction: S and N face states of cell (j,k).
 template <int NS>
-__device__ void recon_y(const Params& P, const Grid& G, const TileView& T, int idx, int j, int k,
+__device__ __forceinline__ void recon_y(const Params& P, const Grid& G, const TileView& T, int idx, int j, int k,
                         double* S, double* Nn) {
   constexpr int NV = 4 + NS;
   const int ny = G.ny;
@@ -310,7 +314,7 @@ __device__ void recon_y(const Params& P, const Grid& G, const TileView& T, int i
 // the other side faces the disc across this face, else the zero state
 // (a face state the reference never writes).
 template <int NS>
-__device__ void face_states(const Params& P, const Grid& G, const TileView& T, int dir, int idxA,
+__device__ __forceinline__ void face_states(const Params& P, const Grid& G, const TileView& T, int dir, int idxA,
                             int idxB, int ja, int ka, int jb, int kb, bool hasA, bool hasB,
                             double* qpA, double* qpB, double* qA, double* qB) {
   constexpr int NV = 4 + NS, NP = NV + 2;
@@ -371,7 +375,7 @@ __device__ void face_states(const Params& P, const Grid& G, const TileView& T, i
 // eval_speeds + eval_flux_KT + average_KT for one face:
 // src/solver_2d.f90:3937-4014, :2834-3002, :3019-3049
 template <int NS>
-__device__ void kt_flux(const Params& P, int dir, const double* qpA, const double* qpB,
+__device__ __forceinline__ void kt_flux(const Params& P, int dir, const double* qpA, const double* qpB,
                         const double* qA, const double* qB, double* H, double& an, double& ap) {
   constexpr int NV = 4 + NS;
   double amin, amax, bmin, bmax;
@@ -439,7 +443,7 @@ constexpr size_t flux_lds_bytes() {
 
 template <int NS>
 __global__ __launch_bounds__(NTHR) void k_flux(Params P, Grid G, int mode, double t_arg,
-                                               int step_idx) {
+                                               int par) {
   constexpr int NV = 4 + NS, NP = NV + 2;
   extern __shared__ double smem[];
   double* sq = smem;
@@ -451,17 +455,18 @@ __global__ __launch_bounds__(NTHR) void k_flux(Params P, Grid G, int mode, doubl
   unsigned char* ssd = (unsigned char*)(ssc + RY * RX);
   unsigned char* sact = ssd + RY * RX;
   __shared__ double red[NTHR / 64];
-  __shared__ int s_last;
 
   const int tid = threadIdx.x;
-  const int nb = gridDim.x;
-  const int tile = xcd_tile(blockIdx.x, nb);
-  const int j0 = (tile % G.ntx) * TX, k0 = (tile / G.ntx) * TY;
   const int nx = G.nx, ny = G.ny, N = G.N;
   double my_dt = DINF;
+  const int n_list = (int)G.st->n_tiles[par];
+  const int* list = G.tile_list + (size_t)par * G.ntx * G.nty;
+  const double t = (mode == 1) ? G.st->t : t_arg;
 
-  if (G.tile_act[tile]) {
-    const double t = (mode == 1) ? G.st->t : t_arg;
+  for (int it = blockIdx.x; it < n_list; it += gridDim.x) {
+    const int tile = list[it];
+    const int j0 = (tile % G.ntx) * TX, k0 = (tile / G.ntx) * TY;
+    __syncthreads();  // the previous tile's LDS reads are done
     // ---- primitive states of the tile and its two-cell halo ----------------
     for (int e = tid; e < RX * RY; e += NTHR) {
       const int lj = e % RX, lk = e / RX;
@@ -601,11 +606,13 @@ __global__ __launch_bounds__(NTHR) void k_flux(Params P, Grid G, int mode, doubl
         const double dtx = (ma > 0.0) ? P.cfl * P.dx / ma : DINF;
         ma = fmax_f(sAy[es], sAy[es + TX]);
         const double dty = (ma > 0.0) ? P.cfl * P.dy / ma : DINF;
-        my_dt = fmin_f(dtx, dty);
+        const double dtc = fmin_f(dtx, dty);
+        my_dt = (dtc < my_dt) ? dtc : my_dt;
       }
     }
   }
   // ---- CFL min: wavefront shuffle, LDS, one atomic per block ---------------
+  // (every candidate is positive, so the IEEE bit patterns order as the values)
   double v = my_dt;
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) {
@@ -618,31 +625,28 @@ __global__ __launch_bounds__(NTHR) void k_flux(Params P, Grid G, int mode, doubl
     double b = red[0];
     for (int w = 1; w < NTHR / 64; w++) b = (red[w] < b) ? red[w] : b;
     if (b < DINF) atomicMin(&G.st->dtmin_bits, (unsigned long long)__double_as_longlong(b));
-    __threadfence();
-    const unsigned int prev = atomicAdd(&G.st->blocks_done, 1u);
-    s_last = (prev == (unsigned int)nb - 1u);
   }
-  __syncthreads();
-  if (s_last && tid == 0) {
-    __threadfence();
-    DevState* st = G.st;
-    const double dtmin =
-        __longlong_as_double((long long)atomicAdd(&st->dtmin_bits, 0ULL));
-    double dt = P.max_dt;
-    dt = fmin_f(dt, dtmin);
-    st->dt_cfl = dt;
-    if (mode == 1) {
-      const double t = st->t;
-      if (t + dt > st->t_end) dt = st->t_end - t;
-      dt = fmin_f(dt, 1.1 * 0.5 * (st->dt_old + st->dt_old_old));
-      st->dt_old_old = st->dt_old;
-      st->dt_old = dt;
-      st->dt = dt;
-      if (G.dts) G.dts[step_idx] = dt;
-    }
-    st->dtmin_bits = 0x7ff0000000000000ULL;
-    st->blocks_done = 0u;
+}
+
+// The CFL time step from the blocks' minimum, then (mode 1) the driver's clip
+// to t_end and growth limit, src/IMEX_SfloW2D.f90:417-439.  One thread; runs
+// after k_flux, so the kernel boundary orders it after every block's atomic.
+__global__ void k_dt(Params P, Grid G, int mode, int step_idx) {
+  DevState* st = G.st;
+  const double dtmin = __longlong_as_double((long long)st->dtmin_bits);
+  double dt = P.max_dt;
+  dt = fmin_f(dt, dtmin);
+  st->dt_cfl = dt;
+  if (mode == 1) {
+    const double t = st->t;
+    if (t + dt > st->t_end) dt = st->t_end - t;
+    dt = fmin_f(dt, 1.1 * 0.5 * (st->dt_old + st->dt_old_old));
+    st->dt_old_old = st->dt_old;
+    st->dt_old = dt;
+    st->dt = dt;
+    if (G.dts) G.dts[step_idx] = dt;
   }
+  st->dtmin_bits = 0x7ff0000000000000ULL;
 }
 
 // ===========================================================================
@@ -738,10 +742,12 @@ __device__ void lnsrch(const Params& P, double dt, const double* xold, const dou
   }
   double alam = 1.0, alam2 = 0.0, f2 = 0.0, tmplam = 0.0, f = 0.0;
   double qj[NV];
-  // the reference loop ends by construction (alam at least halves after the
-  // first trial); 400 trials bound it against NaN states
+  // The reference loop ends by construction: alam at least halves after the
+  // first trial, so it drops below any positive alamin within
+  // log2(DBL_MAX / DBL_TRUE_MIN) < 2100 trials.  Only a NaN alamin (a NaN
+  // step) would loop for ever; the bound turns that into an error.
   int trial;
-  for (trial = 0; trial < 400; trial++) {
+  for (trial = 0; trial < 2200; trial++) {
 #pragma unroll
     for (int i = 0; i < NV; i++) x[i] = xold[i] + alam * p[i];
 #pragma unroll
@@ -781,7 +787,7 @@ __device__ void lnsrch(const Params& P, double dt, const double* xold, const dou
     f2 = f;
     alam = fmax_f(tmplam, 0.5 * alam);
   }
-  if (trial == 400) err |= IMEX_ERR_NAN;
+  if (trial == 2200) err |= IMEX_ERR_NAN;
   f_out = f;
 }
 
@@ -1086,15 +1092,24 @@ __device__ void epilogue(const Params& P, const double* q, double& hmax, double&
 //        8 initial epilogue (over every cell).  mode 1: dt from st->dt and
 //        t += dt (device-side driver).
 template <int NS>
-__global__ __launch_bounds__(256) void k_update(Params P, Grid G, int parts, double dt_arg,
-                                                int mode) {
+__global__ __launch_bounds__(NTHR) void k_update(Params P, Grid G, int parts, double dt_arg,
+                                                 int mode, int par) {
   constexpr int NV = 4 + NS;
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
   const int N = G.N;
   const double dt = (mode == 1) ? G.st->dt : dt_arg;
+  // parts & 8 (the initial epilogue) visits every tile; otherwise the tiles
+  // of the active list
+  const bool all = (parts & 8) != 0;
+  const int n_tiles = all ? G.ntx * G.nty : (int)G.st->n_tiles[par];
+  const int* list = G.tile_list + (size_t)par * G.ntx * G.nty;
   unsigned int err = 0;
   int dried = 0;
-  if (c < N && (G.mask[c] || (parts & 8))) {
+  for (int it = blockIdx.x; it < n_tiles; it += gridDim.x) {
+  const int tile = all ? it : list[it];
+  const int j = (tile % G.ntx) * TX + (int)threadIdx.x % TX;
+  const int k = (tile / G.ntx) * TY + (int)threadIdx.x / TX;
+  const int c = j + G.nx * k;
+  if (j < G.nx && k < G.ny && (G.mask[c] || all)) {
     double q[NV];
 #pragma unroll
     for (int i = 0; i < NV; i++) q[i] = G.q[(size_t)i * N + c];
@@ -1126,10 +1141,11 @@ __global__ __launch_bounds__(256) void k_update(Params P, Grid G, int parts, dou
       G.mvel[c] = mv;
       if (G.vuln) G.vuln[c] = vb;
     }
-    if ((parts & 3) && wet0 && q[0] == 0.0) dried = 1;
+    if ((parts & 3) && wet0 && q[0] == 0.0) dried++;
+  }
   }
   if (err) atomicOr(&G.st->err, err);
-  if (dried) atomicAdd(&G.st->dried, 1ULL);
+  if (dried) atomicAdd(&G.st->dried, (unsigned long long)dried);
   if (mode == 1 && blockIdx.x == 0 && threadIdx.x == 0) G.st->t = G.st->t + dt;
 }
 
@@ -1276,6 +1292,11 @@ __global__ void kt_arith(int n, const double* a, const double* b, double* out) {
   }
 }
 
+__global__ void kt_pow(int n, const double* x, const double* y, double* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = pow_cr(x[i], y[i]);
+}
+
 // ===========================================================================
 // host side
 // ===========================================================================
@@ -1286,6 +1307,7 @@ struct imex_handle {
   hipStream_t stream = nullptr;
   int device = 0;
   int ntiles = 0;
+  int par = 0;  // slot of the active-tile list filled by the latest k_mask
   size_t lds = 0;
   std::vector<void*> allocs;
   double* d_tmp = nullptr;  // N*nv scratch for layout conversion
@@ -1433,7 +1455,7 @@ extern "C" int imex_create(const imex_config* cfg, imex_handle** out) {
   if (P.n_thk * P.n_pdyn > 0)
     if ((rc = dalloc(h, &G.vuln, N))) return rc;
   if ((rc = dalloc(h, &G.mask, N))) return rc;
-  if ((rc = dalloc(h, &G.tile_act, (size_t)h->ntiles))) return rc;
+  if ((rc = dalloc(h, &G.tile_list, 2 * (size_t)h->ntiles))) return rc;
   double *bx, *by, *ap, *ax, *ay, *xc, *xs, *yc, *ys;
   signed char* sc;
   unsigned char* sd;
@@ -1506,10 +1528,17 @@ extern "C" int imex_set_static(imex_handle* h, const imex_static* s) {
   return 0;
 }
 
+// Persistent grids: enough blocks to fill the 256 CUs at the kernels'
+// occupancy; each block walks the active-tile list with a stride of the grid.
+static int grid_for(const imex_handle* h, int per_cu) {
+  return std::max(1, std::min(h->ntiles, 256 * per_cu));
+}
+
 template <int NS>
 static void launch_update(imex_handle* h, int parts, double dt, int mode) {
-  const int nb = (h->G.N + 255) / 256;
-  hipLaunchKernelGGL(k_update<NS>, dim3(nb), dim3(256), 0, h->stream, h->P, h->G, parts, dt, mode);
+  const int nb = (parts & 8) ? h->ntiles : grid_for(h, 4);
+  hipLaunchKernelGGL(k_update<NS>, dim3(nb), dim3(NTHR), 0, h->stream, h->P, h->G, parts, dt,
+                     mode, h->par);
 }
 static int update(imex_handle* h, int parts, double dt, int mode) {
   switch (h->P.ns) {
@@ -1521,16 +1550,21 @@ static int update(imex_handle* h, int parts, double dt, int mode) {
   return 0;
 }
 static int mask(imex_handle* h, int solve_all) {
-  hipLaunchKernelGGL(k_mask, dim3(h->ntiles), dim3(NTHR), 0, h->stream, h->P, h->G, solve_all);
+  h->par ^= 1;
+  hipLaunchKernelGGL(k_mask, dim3(h->ntiles), dim3(NTHR), 0, h->stream, h->P, h->G, solve_all,
+                     h->par);
   HCHK(h, hipGetLastError());
   return 0;
 }
 static int flux(imex_handle* h, int mode, double t, int step_idx) {
+  const int nb = grid_for(h, 4);
   switch (h->P.ns) {
-    case 1: hipLaunchKernelGGL(k_flux<1>, dim3(h->ntiles), dim3(NTHR), h->lds, h->stream, h->P, h->G, mode, t, step_idx); break;
-    case 2: hipLaunchKernelGGL(k_flux<2>, dim3(h->ntiles), dim3(NTHR), h->lds, h->stream, h->P, h->G, mode, t, step_idx); break;
-    default: hipLaunchKernelGGL(k_flux<3>, dim3(h->ntiles), dim3(NTHR), h->lds, h->stream, h->P, h->G, mode, t, step_idx); break;
+    case 1: hipLaunchKernelGGL(k_flux<1>, dim3(nb), dim3(NTHR), h->lds, h->stream, h->P, h->G, mode, t, h->par); break;
+    case 2: hipLaunchKernelGGL(k_flux<2>, dim3(nb), dim3(NTHR), h->lds, h->stream, h->P, h->G, mode, t, h->par); break;
+    default: hipLaunchKernelGGL(k_flux<3>, dim3(nb), dim3(NTHR), h->lds, h->stream, h->P, h->G, mode, t, h->par); break;
   }
+  HCHK(h, hipGetLastError());
+  hipLaunchKernelGGL(k_dt, dim3(1), dim3(1), 0, h->stream, h->P, h->G, mode, step_idx);
   HCHK(h, hipGetLastError());
   return 0;
 }
@@ -1555,7 +1589,6 @@ extern "C" int imex_upload_state(imex_handle* h, const double* q_aos, double t, 
   s.dt_old = dt0;
   s.dt_old_old = dt0;
   s.dtmin_bits = 0x7ff0000000000000ULL;
-  s.blocks_done = 0;
   s.err = 0;
   HCHK(h, hipMemcpyAsync(h->G.st, &s, sizeof(s), hipMemcpyHostToDevice, h->stream));
   int rc;
@@ -1977,6 +2010,19 @@ extern "C" int imex_test_newton(imex_handle* h, int n, double dt, const double* 
   if (fin(h)) return -1;
   if (down(h, q, o, (size_t)n * nv)) return -1;
   HCHK(h, hipMemcpy(iters, it, 4 * (size_t)n, hipMemcpyDeviceToHost));
+  return 0;
+}
+extern "C" int imex_test_pow(imex_handle* h, int n, const double* x, const double* y, double* out) {
+  if (!h || n <= 0) return -2;
+  DevBufs b{h};
+  double *dx = b.up(x, n), *dy = b.up(y, n), *o = b.up(nullptr, n);
+  hipLaunchKernelGGL(kt_pow, dim3((n + 127) / 128), dim3(128), 0, 0, n, dx, dy, o);
+  if (fin(h)) return -1;
+  return down(h, out, o, (size_t)n);
+}
+extern "C" int imex_host_pow(int n, const double* x, const double* y, double* out) {
+  if (n < 0 || (n > 0 && (!x || !y || !out))) return -2;
+  for (int i = 0; i < n; i++) out[i] = pow_cr(x[i], y[i]);
   return 0;
 }
 extern "C" int imex_test_arith(imex_handle* h, int n, const double* a, const double* bb, double* out) {

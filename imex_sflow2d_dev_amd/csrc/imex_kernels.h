@@ -4,18 +4,24 @@
 // driver loop):
 //   k_mask    active set (check_solve): seeds = wet cells, ring cells and the
 //             boundary frame, dilated by the diamond of radius n_RK = 2; one
-//             byte per cell plus a per-tile "has active cells" flag.
-//   k_flux    2-D tile of TX x TY cells with a two-cell halo in LDS:
-//             primitive states -> MUSCL/minmod face states -> local speeds ->
-//             Kurganov-Tadmor face fluxes -> flux divergence minus explicit
-//             terms (the stage-1 right-hand side, stored per active cell), and
-//             the CFL time-step candidates reduced per block and finished by
-//             the last block (device-side dt).
-//   k_update  one thread per active cell, state in registers: stage-2
-//             semi-implicit Coulomb friction, Newton solve of the turbulent
-//             friction (complex-step Jacobian, 2x2 Cramer, backtracking line
-//             search), assembly checks, sedimentation / entrainment, and the
-//             driver epilogue (running maxima, vulnerability table).
+//             byte per cell, and the list of tiles holding active cells.
+//   k_flux    persistent blocks over the active-tile list; per TX x TY tile
+//             with a two-cell halo in LDS: primitive states -> MUSCL/minmod
+//             face states -> local speeds -> Kurganov-Tadmor face fluxes ->
+//             flux divergence minus explicit terms (the stage-1 right-hand
+//             side, stored per active cell), and the CFL time-step candidates
+//             reduced per block into one atomic minimum.
+//   k_dt      one thread: the CFL time step and the driver's clip and growth
+//             limit (device-side dt).
+//   k_update  persistent blocks over the active-tile list, one thread per
+//             cell, state in registers: stage-2 semi-implicit Coulomb
+//             friction, Newton solve of the turbulent friction (complex-step
+//             Jacobian, 2x2 Cramer, backtracking line search), assembly
+//             checks, sedimentation / entrainment, and the driver epilogue
+//             (running maxima, vulnerability table).
+// The active-tile list is double-buffered: the k_mask of a step fills slot
+// `par` and clears the other slot's counter for the next step, so no kernel
+// waits on another through device-scope fences.
 #pragma once
 
 #include "imex_physics.h"
@@ -33,8 +39,9 @@ constexpr int NTHR = 256;
 struct DevState {
   double t, dt, dt_old, dt_old_old, t_end, dt_cfl;
   unsigned long long dtmin_bits;
-  unsigned int blocks_done;
+  unsigned int n_tiles[2];  // active tiles listed by k_mask, per list slot
   unsigned int err;
+  unsigned int pad;
   unsigned long long dried;
   unsigned long long active;
 };
@@ -48,7 +55,7 @@ struct Grid {
   double* dep;   // [ns][N]
   unsigned long long* vuln;  // [N] bit (i_thk*n_pdyn + i_pdyn)
   unsigned char* mask;       // [N]
-  unsigned char* tile_act;   // [ntiles]
+  int* tile_list;            // [2][ntiles] active tiles, two slots
   const double* bx;          // [N] B_prime_x_geom
   const double* by;          // [N] B_prime_y_geom
   const signed char* scell;  // [N] source_cell

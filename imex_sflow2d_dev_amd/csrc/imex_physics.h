@@ -14,6 +14,8 @@
 #include <math.h>
 #include <stdint.h>
 
+#include "imex_math.h"
+
 namespace imex {
 
 constexpr double kEps = 2.220446049250313e-16;  // EPSILON(1.0_wp)
@@ -35,8 +37,11 @@ struct Params {
   double hcs, one_by_hcs;  // complex step n_vars*eps and its reciprocal
 };
 
-__device__ __forceinline__ double fmin_f(double a, double b) { return (b < a) ? b : a; }
-__device__ __forceinline__ double fmax_f(double a, double b) { return (b > a) ? b : a; }
+// Fortran MIN(a, b) / MAX(a, b) as the reference build evaluates them:
+// (a < b) ? a : b and (a > b) ? a : b, so a NaN or a signed-zero tie
+// returns the second argument.
+__device__ __forceinline__ double fmin_f(double a, double b) { return (a < b) ? a : b; }
+__device__ __forceinline__ double fmax_f(double a, double b) { return (a > b) ? a : b; }
 
 // ---------------------------------------------------------------------------
 // complex arithmetic as the reference build evaluates it
@@ -559,7 +564,7 @@ __device__ inline double settling_velocity(const Params& P, double diam, double 
   if (Rey <= 1000.0) {
     for (int i = 0; i < 20; i++) {
       double old = sv;
-      inv_sqrt_C_D = sqrt(Rey / (24.0 * (1.0 + 0.15 * pow(Rey, 0.687))));
+      inv_sqrt_C_D = sqrt(Rey / (24.0 * (1.0 + 0.15 * pow_cr(Rey, 0.687))));
       sv = const_part * inv_sqrt_C_D;
       if (fabs(old - sv) / old < 1.0e-6) return sv;
       Rey = diam * sv * ikv;
@@ -609,7 +614,7 @@ __device__ inline bool mass_exchange(const Params& P, const double* qp, double d
     if ((alphas[i] > 0.0) && P.settling) {
       double sv = settling_velocity(P, P.diam_s[i], P.rho_s[i], m.rho_c, ikv);
       dep[i] = alphas[i] * sv * 1.0;
-      dep[i] = dep[i] * pow(1.0 - fmin_f(1.0, at / 0.6), 4.65);
+      dep[i] = dep[i] * pow_cr(1.0 - fmin_f(1.0, at / 0.6), 4.65);
       dep[i] = fmin_f(dep[i], h * alphas[i] / dt);
       if (dep[i] < 0.0) err = true;
     }
@@ -618,9 +623,13 @@ __device__ inline bool mass_exchange(const Params& P, const double* qp, double d
 #pragma unroll
   for (int i = 0; i < NS; i++) sdep += dep[i];
   double cpl = 0.0 + 0.0 * sdep;
+  // the continuous-phase loss is limited by the maximum solid packing (0.64)
+  // and by the continuous phase present (src/constitutive_2d.f90:5034-5041)
+  cpl = fmin_f(cpl, h * fmax_f(0.0, 0.64 - at) / dt);
+  cpl = fmin_f(cpl, (h * (1.0 - at) - 0.0 * (h * at - dt * sdep)) / dt);
   double air = 0.0;
   if (P.entrainment && (h > 0.0) && (m.Ri > 0.0)) {
-    double ec = 0.075 / sqrt(1.0 + 718.0 * pow(m.Ri, 2.4));
+    double ec = 0.075 / sqrt(1.0 + 718.0 * pow_cr(m.Ri, 2.4));
     air = ec * sqrt(mod_vel2);
   }
   double rdt = 0.0, rert = 0.0, A = 0.0, D = 0.0;

@@ -37,6 +37,7 @@ EXPORTS = [
     "imex_test_qc_to_qp", "imex_test_qp_to_qc", "imex_test_fluxes", "imex_test_speeds",
     "imex_test_implicit_cplx", "imex_test_semi_impl", "imex_test_expl",
     "imex_test_settling", "imex_test_mass_exchange", "imex_test_newton", "imex_test_arith",
+    "imex_test_pow", "imex_host_pow",
 ]
 
 

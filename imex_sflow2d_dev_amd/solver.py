@@ -303,6 +303,14 @@ class KernelProbe:
                   *[native._dp(x) for x in a], native._dp(out), native._ip(its))
         return out, its
 
+    def pow(self, x, y):
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.ascontiguousarray(y, np.float64)
+        out = np.zeros_like(x)
+        self._run("imex_test_pow", ctypes.c_int(len(x)), native._dp(x), native._dp(y),
+                  native._dp(out))
+        return out
+
     def arith(self, a, b):
         a = np.ascontiguousarray(a, np.float64)
         b = np.ascontiguousarray(b, np.float64)

@@ -141,3 +141,19 @@ def config3(nx: int = 4096, ny: int = 4096, dx: float = 2.5, seed_member: int = 
     cfg.r_collapse = 0.03 * nx * dx
     cfg.validate()
     return cfg, B
+
+
+WORKLOADS = {
+    "c1": config1,
+    "c2": config2,
+    "c2-small": small_config2,
+    "c3": config3,
+}
+
+
+def by_name(name: str):
+    """(RunConfig, bed) of a named synthetic workload."""
+    try:
+        return WORKLOADS[name]()
+    except KeyError:
+        raise ValueError(f"unknown workload {name!r}; known: {sorted(WORKLOADS)}") from None

@@ -160,6 +160,11 @@ int imex_test_newton(imex_handle *h, int n, double dt, const double *q_si, const
                      const double *R, double *q, int32_t *iters);
 /* correctly rounded arithmetic probes: out[4i..4i+3] = a/b, sqrt(a), csqrt(a+ib) */
 int imex_test_arith(imex_handle *h, int n, const double *a, const double *b, double *out);
+/* the path's x**y (correctly rounded, as the reference's libm pow is in all
+ * but rare cases) on the device, and the same code compiled for the host CPU
+ * (no GPU needed; for checking the algorithm on the CPU) */
+int imex_test_pow(imex_handle *h, int n, const double *x, const double *y, double *out);
+int imex_host_pow(int n, const double *x, const double *y, double *out);
 
 #ifdef __cplusplus
 }

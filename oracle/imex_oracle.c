@@ -31,6 +31,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <quadmath.h>
+
 #include "imex_oracle.h"
 
 #define EPS 2.220446049250313e-16 /* EPSILON(1.0_wp) */
@@ -75,7 +77,24 @@ struct oracle_s {
      * states, used only to count reads of states left from an earlier call */
     int call_id;
     int *stL, *stR, *stB, *stT, *stQP;
+    /* 0: x**y by the C library's pow, as the reference build calls it;
+     * 1: x**y correctly rounded (quad-precision powq rounded once) */
+    int pow_mode;
 };
+
+/* x**y correctly rounded: powq in binary128 (113-bit significand) rounded
+ * once to double; exact unless x**y lies within 2**-58 ulp of a rounding
+ * boundary of the double result */
+double oracle_pow_cr(double x, double y)
+{
+    return (double)powq((__float128)x, (__float128)y);
+}
+
+/* REAL**REAL of the path: src/constitutive_2d.f90:4851, :5049, :5378 */
+static inline double rpow(const oracle_t *o, double x, double y)
+{
+    return o->pow_mode ? oracle_pow_cr(x, y) : pow(x, y);
+}
 
 /* ---------------------------------------------------------------------- */
 /* indexing: cells c = j + nx*k (0-based), x faces j + (nx+1)*k,          */
@@ -85,8 +104,12 @@ struct oracle_s {
 #define FX(o, j, k) ((j) + ((o)->nx + 1) * (k))
 #define FY(o, j, k) ((j) + (o)->nx * (k))
 
-static inline double fmin_f(double a, double b) { return (b < a) ? b : a; }
-static inline double fmax_f(double a, double b) { return (b > a) ? b : a; }
+/* Fortran MIN(a, b) / MAX(a, b) as the reference build (amdflang, IEEE
+ * flags) evaluates them: (a < b) ? a : b and (a > b) ? a : b, so a NaN or a
+ * signed-zero tie returns the second argument (checked on MIN(1e15, NaN),
+ * MIN(NaN, 1e15), MIN(0, -0), MIN(-0, 0) and the MAX counterparts). */
+static inline double fmin_f(double a, double b) { return (a < b) ? a : b; }
+static inline double fmax_f(double a, double b) { return (a > b) ? a : b; }
 
 /* ---------------------------------------------------------------------- */
 /* complex arithmetic as evaluated by the reference build                  */
@@ -541,7 +564,7 @@ static void eval_implicit_terms_c(const oracle_t *o, const cx *qj, cx *nh)
 
 static void eval_implicit_terms_r(const oracle_t *o, const double *q, double *nh)
 {
-    cx qj[8], r[8];
+    cx qj[8] = {{0.0, 0.0}}, r[8];
     for (int i = 0; i < o->nv; i++) qj[i] = C(q[i], 0.0);
     eval_implicit_terms_c(o, qj, r);
     for (int i = 0; i < o->nv; i++) nh[i] = r[i].re;
@@ -559,7 +582,7 @@ static double settling_velocity(const oracle_t *o, double diam, double rhos, dou
     if (Rey <= 1000.0) {
         for (int i = 0; i < 20; i++) {
             double old = sv;
-            inv_sqrt_C_D = sqrt(Rey / (24.0 * (1.0 + 0.15 * pow(Rey, 0.687))));
+            inv_sqrt_C_D = sqrt(Rey / (24.0 * (1.0 + 0.15 * rpow(o, Rey, 0.687))));
             sv = const_part * inv_sqrt_C_D;
             if (fabs(old - sv) / old < 1.0e-6) return sv;
             Rey = diam * sv * ikv;
@@ -603,7 +626,7 @@ static int eval_mass_exchange_terms(const oracle_t *o, const double *qp, double 
         if ((alphas[i] > 0.0) && o->settling) {
             double sv = settling_velocity(o, o->diam_s[i], o->rho_s[i], rhoc, ikv);
             dep[i] = alphas[i] * sv * 1.0;
-            dep[i] = dep[i] * pow(1.0 - fmin_f(1.0, at / 0.6), 4.65);
+            dep[i] = dep[i] * rpow(o, 1.0 - fmin_f(1.0, at / 0.6), 4.65);
             dep[i] = fmin_f(dep[i], h * alphas[i] / dt);
             if (dep[i] < 0.0) err = 1;
         }
@@ -612,9 +635,13 @@ static int eval_mass_exchange_terms(const oracle_t *o, const double *qp, double 
     for (int i = 0; i < ns; i++) sdep += dep[i];
     double cpl = 0.0;
     cpl = cpl + 0.0 * sdep; /* coeff_porosity = 0 */
+    /* limits by the maximum solid packing (0.64, src/inpout_2d.f90:854) and
+     * by the continuous phase present: src/constitutive_2d.f90:5034-5041 */
+    cpl = fmin_f(cpl, h * fmax_f(0.0, 0.64 - at) / dt);
+    cpl = fmin_f(cpl, (h * (1.0 - at) - 0.0 * (h * at - dt * sdep)) / dt);
     double air = 0.0;
     if (o->entrainment && (h > 0.0) && (Ri > 0.0)) {
-        double ec = 0.075 / sqrt(1.0 + 718.0 * pow(Ri, 2.4));
+        double ec = 0.075 / sqrt(1.0 + 718.0 * rpow(o, Ri, 2.4));
         air = ec * sqrt(mod_vel2);
     }
     double rdt = 0.0, rert = 0.0;
@@ -1786,6 +1813,15 @@ void oracle_get_mask(const oracle_t *o, int *m)
     for (size_t c = 0; c < (size_t)o->nx * o->ny; c++) m[c] = o->mask[c];
 }
 
+void oracle_set_pow_mode(oracle_t *o, int mode) { o->pow_mode = mode; }
+
+void oracle_set_time(oracle_t *o, double t, double dt_old, double dt_old_old)
+{
+    o->t = t;
+    o->dt_old = dt_old;
+    o->dt_old_old = dt_old_old;
+}
+
 void oracle_get_counters(const oracle_t *o, long *dried, long *stale)
 {
     *dried = o->dried;
@@ -1855,6 +1891,8 @@ void oracle_mass_exchange(const oracle_t *o, int n, const double *qp, double dt,
     for (int i = 0; i < n; i++)
         eval_mass_exchange_terms(o, qp + (size_t)(o->nv + 2) * i, dt, dep + (size_t)o->ns * i, eq + (size_t)o->nv * i);
 }
+double oracle_limit_minmod(const double *v3, const double *z3) { return limit_minmod(v3, z3); }
+
 int oracle_newton(const oracle_t *o, double dt, const double *q_si, const double *q0,
                   const double *R, double *q)
 {

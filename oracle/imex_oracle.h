@@ -38,6 +38,13 @@ void oracle_get_fields(const oracle_t *o, double *hmax, double *pdynmax,
                        double *modvelmax, double *deposit, double *t, double *dt);
 void oracle_get_vuln(const oracle_t *o, int *v);
 void oracle_get_mask(const oracle_t *o, int *m);
+/* x**y of the path: mode 0 (default) the C library's pow, as the reference
+ * build calls it; mode 1 correctly rounded (quad-precision pow rounded once),
+ * the rounding the HIP path computes */
+void oracle_set_pow_mode(oracle_t *o, int mode);
+double oracle_pow_cr(double x, double y);
+/* driver time and dt history (restart from a saved state) */
+void oracle_set_time(oracle_t *o, double t, double dt_old, double dt_old_old);
 /* number of cells that went from wet (q1 > 0) to exactly dry in a step, and
  * reads of a face or stage state left from an earlier step that differ from
  * the state a fresh evaluation gives (the reference's stale-state quirk) */
@@ -61,6 +68,9 @@ void oracle_settling(const oracle_t *o, int n, const double *diam, const double 
                      const double *rhoc, const double *ikv, double *out);
 void oracle_mass_exchange(const oracle_t *o, int n, const double *qp, double dt,
                           double *dep, double *eq);
+/* the limited slope of limit() with the minmod limiter from three values at
+ * three abscissae (src/geometry_2d.f90:1776-1806, :1855-1876) */
+double oracle_limit_minmod(const double *v3, const double *z3);
 /* one Newton solve: q_si, q0, R (n_vars each) -> q (n_vars); returns iterations */
 int oracle_newton(const oracle_t *o, double dt, const double *q_si, const double *q0,
                   const double *R, double *q);

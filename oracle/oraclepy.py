@@ -41,14 +41,30 @@ def _lib():
         for name in ("oracle_destroy", "oracle_set_static", "oracle_init_state",
                      "oracle_run", "oracle_get_q", "oracle_set_q", "oracle_get_fields",
                      "oracle_get_vuln", "oracle_get_mask", "oracle_get_counters",
+                     "oracle_set_time",
                      "oracle_qc_to_qp", "oracle_qp_to_qc", "oracle_mixt_var",
                      "oracle_fluxes", "oracle_speeds", "oracle_implicit",
                      "oracle_implicit_cplx", "oracle_semi_impl", "oracle_expl",
                      "oracle_settling", "oracle_mass_exchange", "oracle_newton"):
             getattr(lib, name).restype = ctypes.c_int if name in (
                 "oracle_run", "oracle_newton") else None
+        lib.oracle_limit_minmod.restype = ctypes.c_double
+        lib.oracle_pow_cr.restype = ctypes.c_double
+        lib.oracle_pow_cr.argtypes = [ctypes.c_double, ctypes.c_double]
+        lib.oracle_set_pow_mode.restype = None
         _LIB = lib
     return _LIB
+
+
+def pow_cr(x: float, y: float) -> float:
+    """x**y correctly rounded (quad-precision pow rounded once)."""
+    return _lib().oracle_pow_cr(x, y)
+
+
+def limit_minmod(v3, z3) -> float:
+    v = np.ascontiguousarray(v3, np.float64)
+    z = np.ascontiguousarray(z3, np.float64)
+    return _lib().oracle_limit_minmod(_dp(v), _dp(z))
 
 
 def _dp(a):
@@ -144,6 +160,14 @@ class Oracle:
         m = np.zeros((self.cfg.ny, self.cfg.nx), np.int32)
         self.lib.oracle_get_mask(self.h, _ip(m))
         return m
+
+    def set_pow_mode(self, mode: int):
+        """0: the C library's pow (the reference's); 1: correctly rounded."""
+        self.lib.oracle_set_pow_mode(self.h, ctypes.c_int(mode))
+
+    def set_time(self, t, dt_old, dt_old_old):
+        self.lib.oracle_set_time(self.h, ctypes.c_double(t), ctypes.c_double(dt_old),
+                                 ctypes.c_double(dt_old_old))
 
     def counters(self):
         d = ctypes.c_long()

@@ -1,0 +1,79 @@
+"""Times the reference OpenMP Fortran build (oracle/_ref/libimexref_perf.so,
+the solver modules compiled from src/*.f90 with -Ofast-equivalent flags and
+-fopenmp) on a bounded sample of driver steps.
+
+TEST INFRASTRUCTURE ONLY: run as a child process by bench.py's cpu_baseline
+leg.  It is the baseline that is reported beside the GPU figure, never the
+thing measured for it.
+
+Usage: python -m oracle.ref_timing --workload c2 --state STATE.npz
+           [--seconds 20] [--max-steps 200]
+
+STATE.npz holds q[k, j, var], t, dt_old, dt_old_old of the run at the point
+where the sample starts.  The sample is whole driver steps (check_solve,
+timestep, imex_RK_solver, update_erosion_deposition_cell, the epilogue;
+src/IMEX_SfloW2D.f90:395-505), taken one at a time until the time budget is
+spent.  The result is printed as one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import threading
+import time
+
+import numpy as np
+
+sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+
+from imex_sflow2d_dev_amd import workloads  # noqa: E402
+from oracle import refpy  # noqa: E402
+
+
+def _work(args, out):
+    cfg, B = workloads.by_name(args.workload)
+    st = np.load(args.state, allow_pickle=False)
+    t_setup = time.perf_counter()
+    run = refpy.RefRun(cfg, B, "perf")
+    t_setup = time.perf_counter() - t_setup
+    run.set_q(st["q"])
+    run.set_time(float(st["t"]), float(st["dt_old"]), float(st["dt_old"]),
+                 float(st["dt_old_old"]))
+    steps = 0
+    cells = 0
+    elapsed = 0.0
+    while steps < args.max_steps and elapsed < args.seconds:
+        t0 = time.perf_counter()
+        _, nc = run.run(1)
+        elapsed += time.perf_counter() - t0
+        steps += 1
+        cells += int(nc[0])
+    run.close()
+    out.update(steps=steps, elapsed_s=elapsed, active_cells=cells, setup_s=t_setup,
+               threads=int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--state", required=True)
+    ap.add_argument("--seconds", type=float, default=20.0)
+    ap.add_argument("--max-steps", type=int, default=200)
+    args = ap.parse_args()
+    out = {}
+    # -Ofast implies stack-allocated array temporaries, which at 2048^2 need
+    # far more than the default 8 MiB main-thread stack: run on a thread
+    # with a large (virtual, touched-on-demand) stack instead.
+    threading.stack_size(4 << 30)
+    th = threading.Thread(target=_work, args=(args, out))
+    th.start()
+    th.join()
+    if not out:
+        sys.exit(1)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

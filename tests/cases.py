@@ -1,0 +1,36 @@
+"""Small parity cases shared by the tests and by tests/golden/make_golden.py.
+
+Each returns (cfg, B) of a CPU-sized run with the physics of one of the
+synthetic workloads.
+"""
+from __future__ import annotations
+
+from imex_sflow2d_dev_amd import workloads
+
+
+def c1_48():
+    """Collapse on the slope-with-bumps bed, 48 x 48 cells of 10 m."""
+    cfg, B = workloads.config1(nx=48, ny=48, dx=10.0)
+    cfg.x_collapse, cfg.y_collapse, cfg.r_collapse = 150.0, 240.0, 90.0
+    return cfg, B
+
+
+def c1_64():
+    """Collapse on the slope-with-bumps bed, 64 x 64 cells of 10 m."""
+    cfg, B = workloads.config1(nx=64, ny=64, dx=10.0)
+    cfg.x_collapse, cfg.y_collapse, cfg.r_collapse = 200.0, 320.0, 100.0
+    return cfg, B
+
+
+def c2_64():
+    """Radial source on the fractal cone, two classes, settling and
+    entrainment, 64 x 64 cells of 20 m."""
+    return workloads.small_config2(nx=64, ny=64, dx=20.0)
+
+
+def c2_96():
+    """As c2_64 on 96 x 96 cells."""
+    return workloads.small_config2(nx=96, ny=96, dx=20.0)
+
+
+CASES = {"c1_48": c1_48, "c1_64": c1_64, "c2_64": c2_64, "c2_96": c2_96}

@@ -1,0 +1,24 @@
+"""bench.py's command line and its algorithmic byte counts (no GPU)."""
+import subprocess
+import sys
+
+from conftest import ROOT
+import bench
+from imex_sflow2d_dev_amd import workloads
+
+
+def test_more_than_one_gpu_is_refused():
+    r = subprocess.run([sys.executable, "bench.py", "--gpus", "2"], cwd=ROOT,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
+
+
+def test_algorithmic_bytes_per_active_cell():
+    # 8 (3 n_v + 2) + 16 n_s S_settle + 48 + 9 per active cell and step
+    c1, _ = workloads.config1(nx=16, ny=16)
+    assert bench.kernel_bytes_per_cell(c1)["step"] == 193
+    c2, _ = workloads.config2(nx=16, ny=16)
+    per = bench.kernel_bytes_per_cell(c2)
+    assert per["step"] == 249
+    assert per["k_flux"] == 64 and per["k_update"] == 176 and per["k_mask"] == 9

@@ -1,0 +1,142 @@
+"""The HIP kernels of single reference routines against the oracle and the
+golden fixtures (one thread per state, the same device functions the step
+kernels inline).  Bit-for-bit, except where noted."""
+import numpy as np
+import pytest
+
+from conftest import golden_path
+import cases
+from oracle import oraclepy
+from imex_sflow2d_dev_amd import init
+from imex_sflow2d_dev_amd.solver import KernelProbe
+
+pytestmark = pytest.mark.gpu
+
+
+def _eq(a, b, what):
+    a, b = np.asarray(a), np.asarray(b)
+    assert a.shape == b.shape, what
+    same = (a == b) | (np.isnan(a) & np.isnan(b))
+    if not same.all():
+        bad = np.argwhere(~same)
+        i = tuple(bad[0])
+        raise AssertionError(f"{what}: {len(bad)} mismatches, first at {i}: {a[i]!r} vs {b[i]!r}")
+
+
+def test_division_and_sqrt_correctly_rounded():
+    cfg, _ = cases.c1_48()
+    kp = KernelProbe(cfg)
+    rng = np.random.default_rng(0)
+    n = 200000
+    a = rng.normal(size=n) * 10.0 ** rng.uniform(-8, 8, n)
+    b = rng.normal(size=n) * 10.0 ** rng.uniform(-8, 8, n)
+    out = kp.arith(a, b)
+    _eq(out[:, 0], a / b, "a/b")
+    _eq(out[:, 1], np.sqrt(np.abs(a)), "sqrt")
+    kp.close()
+
+
+@pytest.mark.parametrize("y, lo, hi", [(0.687, -8, 3), (4.65, -6, 0), (2.4, -12, 15),
+                                       (None, -300, 300)])
+def test_pow_is_correctly_rounded(y, lo, hi):
+    """The device x**y equals the quad-precision pow rounded once."""
+    cfg, _ = cases.c1_48()
+    kp = KernelProbe(cfg)
+    rng = np.random.default_rng(1)
+    n = 100000
+    x = 10.0 ** rng.uniform(lo, hi, n)
+    ys = np.full(n, y) if y is not None else rng.uniform(-2.0, 2.0, n)
+    got = kp.pow(x, ys)
+    want = np.array([oraclepy.pow_cr(a, b) for a, b in zip(x, ys)])
+    _eq(got, want, f"pow(x, {y})")
+    kp.close()
+
+
+def _states(cfg, B, steps, seed):
+    st, q0 = init.setup(cfg, B)
+    o = oraclepy.Oracle(cfg, st, q0)
+    o.run(steps)
+    q = o.q().reshape(-1, cfg.n_vars)
+    wet = q[q[:, 0] > 0.0]
+    rng = np.random.default_rng(seed)
+    return np.concatenate([wet, wet * (1.0 + 1e-3 * rng.standard_normal(wet.shape))])
+
+
+@pytest.fixture(params=["c1_64", "c2_96"])
+def case(request):
+    return cases.CASES[request.param]()
+
+
+def test_functions_match_oracle(case):
+    cfg, B = case
+    qc = _states(cfg, B, 150, 2)
+    o = oraclepy.Oracle(cfg)
+    o.set_pow_mode(1)
+    kp = KernelProbe(cfg)
+    qp, pd = o.qc_to_qp(qc)
+    gq, gpd = kp.qc_to_qp(qc)
+    _eq(gq, qp, "qc_to_qp")
+    _eq(gpd, pd, "p_dyn")
+    _eq(kp.qp_to_qc(qp), o.qp_to_qc(qp), "qp_to_qc")
+    for d in (1, 2):
+        _eq(kp.fluxes(qc, qp, d), o.fluxes(qc, qp, d), f"fluxes {d}")
+        _eq(kp.speeds(qp, d), o.speeds(qp, d), f"speeds {d}")
+    _eq(kp.semi_impl(qp), o.semi_impl(qc, qp), "semi_impl")
+    rng = np.random.default_rng(3)
+    bx = 0.2 * rng.standard_normal(len(qc))
+    by = 0.2 * rng.standard_normal(len(qc))
+    _eq(kp.expl(bx, by, 2.0, qp), o.expl(bx, by, 2.0, qp), "expl")
+    qim = np.zeros_like(qc)
+    hcs = cfg.n_vars * np.finfo(np.float64).eps
+    half = len(qc) // 2
+    qim[:half, 1] = hcs * np.maximum(np.abs(qc[:half, 1]), 1e-3)
+    qim[half:, 2] = hcs * np.maximum(np.abs(qc[half:, 2]), 1e-3)
+    for g, w, lab in zip(kp.implicit_cplx(qc, qim), o.implicit_cplx(qc, qim), ("re", "im")):
+        _eq(g, w, "implicit complex " + lab)
+    for g, w, lab in zip(kp.mass_exchange(qp, 0.04), o.mass_exchange(qp, 0.04), ("dep", "eq")):
+        _eq(g, w, "mass exchange " + lab)
+    m = 2000
+    diam = 10.0 ** rng.uniform(-6, -2, m)
+    rhos = rng.choice([1500.0, 2500.0], m)
+    rhoc = rng.uniform(0.3, 1.5, m)
+    ikv = 1.0 / rng.uniform(1e-5, 3e-5, m)
+    _eq(kp.settling(diam, rhos, rhoc, ikv), o.settling(diam, rhos, rhoc, ikv), "settling")
+    # Newton on stage states: q_si and q0 from the states, R a perturbation
+    dt = 0.05
+    q0 = qc * (1.0 + 1e-3 * rng.standard_normal(qc.shape))
+    R = 0.1 * rng.standard_normal(qc.shape) * np.abs(qc)
+    g, its = kp.newton(dt, qc, q0, R)
+    want = np.array([o.newton(dt, qc[i], q0[i], R[i])[0] for i in range(len(qc))])
+    _eq(g, want, "newton")
+    kp.close()
+
+
+@pytest.mark.parametrize("name", ["c1_48", "c2_64"])
+def test_functions_match_reference_fixtures(name):
+    """Against the reference's own outputs.  x**y enters only the settling
+    velocity and the deposition / entrainment terms: there the reference's
+    libm pow differs from the correctly rounded value in about 0.1 % of the
+    calls, and the settling iteration's 1e-6 relative stop can turn such a
+    last-bit difference into a different iteration count, so those two are
+    compared to 1e-5 relative; everything else bit for bit."""
+    cfg, _ = cases.CASES[name]()
+    g = dict(np.load(golden_path(f"functions_{name}.npz"), allow_pickle=False))
+    kp = KernelProbe(cfg)
+    qp, pd = kp.qc_to_qp(g["qc"])
+    _eq(qp, g["qp"], "qp")
+    _eq(pd, g["pdyn"], "p_dyn")
+    _eq(kp.qp_to_qc(g["qp"]), g["qp_to_qc"], "qp_to_qc")
+    for d, lab in ((1, "x"), (2, "y")):
+        _eq(kp.fluxes(g["qc"], g["qp"], d), g["flux_" + lab], "flux " + lab)
+        _eq(kp.speeds(g["qp"], d), g["speeds_" + lab], "speeds " + lab)
+    re, im = kp.implicit_cplx(g["qc"], g["qim"])
+    _eq(re, g["implicit_cplx_re"], "implicit complex re")
+    _eq(im, g["implicit_cplx_im"], "implicit complex im")
+    _eq(kp.semi_impl(g["qp"]), g["semi_impl"], "semi_impl")
+    _eq(kp.expl(g["bx"], g["by"], 3.0, g["qp"]), g["expl"], "expl")
+    s = kp.settling(g["s_diam"], g["s_rhos"], g["s_rhoc"], g["s_ikv"])
+    np.testing.assert_allclose(s, g["settling"], rtol=1e-5, atol=0)
+    dep, eq = kp.mass_exchange(g["qp"], float(g["mx_dt"]))
+    np.testing.assert_allclose(dep, g["mx_dep"], rtol=1e-5, atol=0)
+    np.testing.assert_allclose(eq, g["mx_eq"], rtol=1e-5, atol=1e-12 * np.abs(g["mx_eq"]).max())
+    kp.close()

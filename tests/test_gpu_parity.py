@@ -1,0 +1,136 @@
+"""Whole driver steps on the GPU against the oracle and the reference
+fixtures, through the C ABI (imex_check_solve / imex_timestep /
+imex_rk_step / imex_erosion_deposition / imex_post_step for host-driven
+steps, imex_run_steps for the device-side driver loop).
+
+Bar: bit-for-bit.  The oracle runs with its correctly rounded x**y (the one
+place where the reference's arithmetic comes from a library, libm pow, whose
+last bit is not correctly rounded in about 0.1 % of calls); runs whose
+physics never calls x**y (no settling, no entrainment) are also compared bit
+for bit with the reference's own fixtures and libm-mode oracle."""
+import numpy as np
+import pytest
+
+from conftest import golden_path
+import cases
+from oracle import oraclepy
+from imex_sflow2d_dev_amd import init, workloads
+from imex_sflow2d_dev_amd.solver import Solver
+
+pytestmark = pytest.mark.gpu
+
+
+def _eq(a, b, what):
+    a, b = np.asarray(a), np.asarray(b)
+    assert a.shape == b.shape, what
+    same = (a == b) | (np.isnan(a) & np.isnan(b))
+    if not same.all():
+        bad = np.argwhere(~same)
+        i = tuple(bad[0])
+        rel = np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+        raise AssertionError(f"{what}: {len(bad)} mismatches, first at {i}: {a[i]!r} vs "
+                             f"{b[i]!r}; max |diff| / max |ref| = {rel:.3e}")
+
+
+def _compare_fields(s, o, what):
+    fo, fs = o.fields(), s.fields()
+    for k in ("hmax", "pdynmax", "mod_vel_max", "deposit"):
+        _eq(fs[k], fo[k], f"{what}: {k}")
+    _eq(s.vuln(), o.vuln(), f"{what}: vulnerability")
+    _eq(s.mask(), o.mask(), f"{what}: active set")
+
+
+@pytest.mark.parametrize("name, steps, mode", [("c1_64", 300, "host"), ("c1_64", 300, "device"),
+                                               ("c2_96", 400, "device"), ("c2_96", 100, "host")])
+def test_run_matches_oracle(name, steps, mode):
+    cfg, B = cases.CASES[name]()
+    st, q0 = init.setup(cfg, B)
+    o = oraclepy.Oracle(cfg, st, q0)
+    o.set_pow_mode(1)
+    s = Solver(cfg, st, q0)
+    _eq(s.q, o.q(), "initial q")
+    _eq(s.mask(), o.mask(), "initial active set")
+    done = 0
+    while done < steps:
+        n = min(50, steps - done)
+        if mode == "host":
+            dg = np.array([s.step() for _ in range(n)])
+        else:
+            dg = s.run_steps(n)
+        do, _, err = o.run(n)
+        done += n
+        _eq(dg, do, f"dt, steps {done - n + 1}..{done}")
+        _eq(s.q, o.q(), f"q after step {done}")
+    _compare_fields(s, o, f"{name} {mode}")
+    assert s.err == 0 and err == 0
+    s.close()
+
+
+@pytest.mark.parametrize("name", ["c1_48", "c2_64"])
+def test_run_matches_reference_fixture(name):
+    """Against the reference's own multi-step output (the c1 run has no x**y
+    at all; the c2 run, with settling and entrainment, matched it bit for bit
+    when the fixture was written)."""
+    cfg, B = cases.CASES[name]()
+    g = dict(np.load(golden_path(f"run_{name}.npz"), allow_pickle=False))
+    st = {k[len("static_"):]: g[k] for k in g if k.startswith("static_")}
+    st["h_source"] = float(g["h_source"])
+    s = Solver(cfg, st, g["q0"])
+    dts = s.run_steps(len(g["dts"]))
+    _eq(dts, g["dts"], "dt sequence")
+    _eq(s.q, g["q"], "q")
+    f = s.fields()
+    for k in ("hmax", "pdynmax", "mod_vel_max", "deposit"):
+        _eq(f[k], g[k], k)
+    _eq(s.vuln().astype(np.int8), g["vuln"], "vulnerability")
+    _eq(s.mask().astype(np.int8), g["mask"], "active set")
+    s.close()
+
+
+def test_c1_full_size_1000_steps_bit_exact():
+    """The 256 x 256, 1000-step collapse on the slope with bumps, against the
+    reference-arithmetic oracle (this physics calls no x**y)."""
+    cfg, B = workloads.config1()
+    st, q0 = init.setup(cfg, B)
+    o = oraclepy.Oracle(cfg, st, q0)
+    s = Solver(cfg, st, q0)
+    for block in range(4):
+        dg = s.run_steps(250)
+        do, _, err = o.run(250)
+        _eq(dg, do, f"dt, block {block}")
+        _eq(s.q, o.q(), f"q after {250 * (block + 1)} steps")
+    _compare_fields(s, o, "c1 full size")
+    s.close()
+
+
+def test_c2_full_size_bit_exact_first_steps():
+    """The 2048 x 2048 radial-source flow (two classes, settling and
+    entrainment) for 150 steps against the oracle with correctly rounded x**y:
+    the flux and update kernels at the real grid size."""
+    cfg, B = workloads.config2()
+    st, q0 = init.setup(cfg, B)
+    o = oraclepy.Oracle(cfg, st, q0)
+    o.set_pow_mode(1)
+    s = Solver(cfg, st, q0)
+    dg = s.run_steps(150)
+    do, nc, err = o.run(150)
+    _eq(dg, do, "dt sequence")
+    q, qo = s.q, o.q()
+    _eq(q, qo, "q after 150 steps")
+    _eq(s.mask(), o.mask(), "active set")
+    dep_s, dep_o = s.fields()["deposit"], o.fields()["deposit"]
+    _eq(dep_s, dep_o, "deposit")
+    s.close()
+
+
+def test_host_and_device_drivers_agree_full_size():
+    cfg, B = workloads.config2()
+    st, q0 = init.setup(cfg, B)
+    a = Solver(cfg, st, q0)
+    b = Solver(cfg, st, q0)
+    da = np.array([a.step() for _ in range(60)])
+    db = b.run_steps(60)
+    _eq(da, db, "dt sequence")
+    _eq(a.q, b.q, "q")
+    a.close()
+    b.close()

@@ -1,0 +1,76 @@
+"""The oracle against the reference solver modules compiled from their own
+sources (oracle/_ref/libimexref.so), live, on fresh inputs: per function on
+random states and over longer multi-step runs than the committed fixtures
+hold.  Skipped where the reference build is absent (it is built from the
+reference sources by `make -C oracle ref`)."""
+import numpy as np
+import pytest
+
+import cases
+from oracle import oraclepy
+
+pytestmark = pytest.mark.slow
+
+
+def _states(cfg, run, rng, n_pert=2):
+    q = run.q().reshape(-1, cfg.n_vars)
+    wet = q[q[:, 0] > 0.0]
+    out = [wet]
+    for _ in range(n_pert):
+        out.append(wet * (1.0 + 1e-2 * rng.standard_normal(wet.shape)))
+    return np.concatenate(out)
+
+
+@pytest.mark.parametrize("name, warm", [("c1_64", 150), ("c2_96", 200)])
+def test_functions_on_fresh_states(ref_lib, name, warm):
+    cfg, B = cases.CASES[name]()
+    run = ref_lib.RefRun(cfg, B)
+    run.run(warm)
+    rng = np.random.default_rng(123)
+    qc = _states(cfg, run, rng)
+    o = oraclepy.Oracle(cfg)
+    qp, pd = run.qc_to_qp(qc)
+    qp2, pd2 = o.qc_to_qp(qc)
+    np.testing.assert_array_equal(qp2, qp)
+    np.testing.assert_array_equal(pd2, pd)
+    np.testing.assert_array_equal(o.qp_to_qc(qp), run.qp_to_qc(qp))
+    np.testing.assert_array_equal(o.mixt_var(qp), run.mixt_var(qp))
+    for d in (1, 2):
+        np.testing.assert_array_equal(o.fluxes(qc, qp, d), run.fluxes(qc, qp, d))
+        np.testing.assert_array_equal(o.speeds(qp, d), run.speeds(qp, d))
+    np.testing.assert_array_equal(o.implicit(qc), run.implicit(qc))
+    qim = np.zeros_like(qc)
+    qim[:, 1] = 1e-14 * np.maximum(np.abs(qc[:, 1]), 1e-3)
+    qim[:, 2] = 1e-14 * rng.standard_normal(len(qc))
+    for a, b in zip(o.implicit_cplx(qc, qim), run.implicit_cplx(qc, qim)):
+        np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(o.semi_impl(qc, qp), run.semi_impl(qc, qp))
+    bx = 0.3 * rng.standard_normal(len(qc))
+    by = 0.3 * rng.standard_normal(len(qc))
+    np.testing.assert_array_equal(o.expl(bx, by, 1.5, qp), run.expl(bx, by, 1.5, qp))
+    for a, b in zip(o.mass_exchange(qp, 0.03), run.mass_exchange(qp, 0.03)):
+        np.testing.assert_array_equal(a, b)
+    run.close()
+
+
+@pytest.mark.parametrize("name, steps", [("c1_64", 300), ("c2_96", 400)])
+def test_multistep_bit_exact(ref_lib, name, steps):
+    cfg, B = cases.CASES[name]()
+    run = ref_lib.RefRun(cfg, B)
+    st = run.static()
+    q0 = run.q()
+    o = oraclepy.Oracle(cfg, st, q0)
+    for block in range(steps // 100):
+        d1, n1 = run.run(100)
+        d2, n2, err = o.run(100)
+        np.testing.assert_array_equal(d2, d1)
+        np.testing.assert_array_equal(n2, n1)
+        np.testing.assert_array_equal(o.q(), run.q())
+        assert err == 0
+    fr, fo = run.fields(), o.fields()
+    for k in ("hmax", "pdynmax", "mod_vel_max", "deposit"):
+        np.testing.assert_array_equal(fo[k], fr[k])
+    np.testing.assert_array_equal(o.vuln(), run.vuln())
+    np.testing.assert_array_equal(o.mask(), run.mask())
+    assert o.counters()[1] == 0          # no stale face or stage state was read
+    run.close()
