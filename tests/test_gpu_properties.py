@@ -48,10 +48,17 @@ def test_deterministic(c2):
 
 
 def test_active_set_rule_and_finite_state(c2):
+    """Also: the only physics error this run may raise is the NaN of the
+    x-direction solid-flux cap (src/solver_2d.f90:2903-2912), which divides
+    by a sum of +Inf and -Inf when the KT mass flux of a face is exactly zero
+    (two buoyant face states moving apart); the reference pauses there, the
+    cell is emptied by the lift-off rule of the same step, and the state
+    stays finite."""
     cfg, st, q0 = c2
-    s = Solver(cfg, st, q0)
+    s = Solver(cfg, st, q0, raise_on_error=False)
     for _ in range(3):
         s.run_steps(300)
+        assert (s.err & ~1) == 0
         q = s.q
         assert np.isfinite(q).all()
         s.check_solve(False)              # the set for the current state
