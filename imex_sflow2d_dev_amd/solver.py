@@ -215,10 +215,6 @@ class KernelProbe:
     def __init__(self, cfg: RunConfig):
         self.cfg = cfg
         self.nv = cfg.n_vars
-        if static.get("h_source") is not None:
-            # the source thickness the set-up derived from the mass flow rate
-            # (src/inpout_2d.f90:4522-4531) is part of the configuration
-            cfg.h_source = float(static["h_source"])
         self.H = native.Handle(cfg)
         self.L = self.H.L
 
