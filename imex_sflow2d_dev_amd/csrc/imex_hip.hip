@@ -72,7 +72,10 @@ __global__ __launch_bounds__(NTHR) void k_mask(Params P, Grid G, int solve_all, 
   }
   __syncthreads();
   if (tid == 0) {
-    if (blockIdx.x == 0) G.st->n_tiles[par ^ 1] = 0u;  // the next step's slot
+    if (blockIdx.x == 0) {
+      G.st->n_tiles[par ^ 1] = 0u;  // the next step's slot
+      G.st->n_work = 0u;            // filled by the k_flux that follows
+    }
     if (s_any) {
       const unsigned int slot = atomicAdd(&G.st->n_tiles[par], 1u);
       G.tile_list[(size_t)par * G.ntx * G.nty + slot] = tile;
@@ -579,8 +582,9 @@ __global__ __launch_bounds__(NTHR) void k_flux(Params P, Grid G, int mode, doubl
     {
       const int lj = tid % TX, lk = tid / TX;
       const int j = j0 + lj, k = k0 + lk;
+      bool work = false;
+      const int c = j + nx * k;
       if (j < nx && k < ny && sact[tid]) {
-        const int c = j + nx * k;
         const int ew = lk * (TX + 1) + lj;
         const int es = lk * TX + lj;
         double qp[NP], ex[NV];
@@ -600,8 +604,17 @@ __global__ __launch_bounds__(NTHR) void k_flux(Params P, Grid G, int mode, doubl
           double d = 0.0;
           d = d + (sHx[i * (TX + 1) * TY + ew + 1] - sHx[i * (TX + 1) * TY + ew]) * P.one_by_dx;
           d = d + (sHy[i * TX * (TY + 1) + es + TX] - sHy[i * TX * (TY + 1) + es]) * P.one_by_dy;
-          G.R[(size_t)i * N + c] = d - ex[i];
+          const double r = d - ex[i];
+          G.R[(size_t)i * N + c] = r;
+          work = work || (r != 0.0);
         }
+        // A cell whose state and right-hand side are all zero stays exactly
+        // zero through stage 2, the assembly, sedimentation and the epilogue,
+        // and raises no error unless T_ambient < 273 K: only the other cells
+        // go to k_update.
+#pragma unroll
+        for (int i = 0; i < NV; i++) work = work || (G.q[(size_t)i * N + c] != 0.0);
+        work = work || (P.T_amb < 273.0);
         double ma = fmax_f(sAx[ew], sAx[ew + 1]);
         const double dtx = (ma > 0.0) ? P.cfl * P.dx / ma : DINF;
         ma = fmax_f(sAy[es], sAy[es + TX]);
@@ -609,6 +622,13 @@ __global__ __launch_bounds__(NTHR) void k_flux(Params P, Grid G, int mode, doubl
         const double dtc = fmin_f(dtx, dty);
         my_dt = (dtc < my_dt) ? dtc : my_dt;
       }
+      // compact the work cells: one atomic per wavefront
+      const unsigned long long bal = __ballot(work);
+      const int lane = tid & 63;
+      unsigned int base = 0u;
+      if (lane == 0 && bal) base = atomicAdd(&G.st->n_work, (unsigned int)__popcll(bal));
+      base = __shfl(base, 0, 64);
+      if (work) G.work[base + __popcll(bal & ((1ULL << lane) - 1ULL))] = c;
     }
   }
   // ---- CFL min: wavefront shuffle, LDS, one atomic per block ---------------
@@ -1088,65 +1108,66 @@ __device__ void epilogue(const Params& P, const double* q, double& hmax, double&
   }
 }
 
-// parts: 1 stage 2 + assembly, 2 erosion/deposition, 4 epilogue,
-//        8 initial epilogue (over every cell).  mode 1: dt from st->dt and
-//        t += dt (device-side driver).
-template <int NS>
-__global__ __launch_bounds__(NTHR) void k_update(Params P, Grid G, int parts, double dt_arg,
-                                                 int mode, int par) {
+// PARTS: 1 stage 2 + assembly, 2 erosion/deposition, 4 epilogue, 8 initial
+// epilogue (over every cell).  Without 8 the kernel walks the work list the
+// step's k_flux built.  mode 1: dt from st->dt, and the launch holding the
+// epilogue advances t (device-side driver).  Each PARTS is its own kernel so
+// that the register budget of the Newton solve does not bound the occupancy
+// of the sedimentation and epilogue.
+template <int NS, int PARTS>
+__global__ __launch_bounds__(NTHR) void k_update(Params P, Grid G, double dt_arg, int mode) {
   constexpr int NV = 4 + NS;
+  constexpr bool ALL = (PARTS & 8) != 0;
   const int N = G.N;
   const double dt = (mode == 1) ? G.st->dt : dt_arg;
-  // parts & 8 (the initial epilogue) visits every tile; otherwise the tiles
-  // of the active list
-  const bool all = (parts & 8) != 0;
-  const int n_tiles = all ? G.ntx * G.nty : (int)G.st->n_tiles[par];
-  const int* list = G.tile_list + (size_t)par * G.ntx * G.nty;
+  const int n = ALL ? N : (int)G.st->n_work;
   unsigned int err = 0;
   int dried = 0;
-  for (int it = blockIdx.x; it < n_tiles; it += gridDim.x) {
-  const int tile = all ? it : list[it];
-  const int j = (tile % G.ntx) * TX + (int)threadIdx.x % TX;
-  const int k = (tile / G.ntx) * TY + (int)threadIdx.x / TX;
-  const int c = j + G.nx * k;
-  if (j < G.nx && k < G.ny && (G.mask[c] || all)) {
+  for (int i = blockIdx.x * NTHR + (int)threadIdx.x; i < n; i += gridDim.x * NTHR) {
+    const int c = ALL ? i : G.work[i];
     double q[NV];
 #pragma unroll
-    for (int i = 0; i < NV; i++) q[i] = G.q[(size_t)i * N + c];
-    const bool wet0 = q[0] > 0.0;
-    if (parts & 1) {
+    for (int v = 0; v < NV; v++) q[v] = G.q[(size_t)v * N + c];
+    if constexpr ((PARTS & 1) != 0) {
+      // remember "wet at the start of the step" in bit 1 of the mask byte for
+      // the dried-cell count of the sedimentation launch
+      const bool wet0 = q[0] > 0.0;
+      G.mask[c] = (unsigned char)(1u | (wet0 ? 2u : 0u));
       double X[NV];
 #pragma unroll
-      for (int i = 0; i < NV; i++) X[i] = G.R[(size_t)i * N + c];
+      for (int v = 0; v < NV; v++) X[v] = G.R[(size_t)v * N + c];
       stage2<NS>(P, dt, q, X, err);
     }
-    if ((parts & 2) && (P.settling || P.entrainment)) {
-      double depo[NS];
+    if constexpr ((PARTS & 2) != 0) {
+      if (P.settling || P.entrainment) {
+        double depo[NS];
 #pragma unroll
-      for (int i = 0; i < NS; i++) depo[i] = G.dep[(size_t)i * N + c];
-      erosion_deposition<NS>(P, dt, q, depo, err);
+        for (int v = 0; v < NS; v++) depo[v] = G.dep[(size_t)v * N + c];
+        erosion_deposition<NS>(P, dt, q, depo, err);
 #pragma unroll
-      for (int i = 0; i < NS; i++) G.dep[(size_t)i * N + c] = depo[i];
+        for (int v = 0; v < NS; v++) G.dep[(size_t)v * N + c] = depo[v];
+      }
+      // wet at the step start (flag left by the stage-2 launch), exactly dry
+      // after the step's last change of q
+      if ((G.mask[c] & 2u) && q[0] == 0.0) dried++;
     }
-    if (parts & 3) {
+    if constexpr ((PARTS & 3) != 0) {
 #pragma unroll
-      for (int i = 0; i < NV; i++) G.q[(size_t)i * N + c] = q[i];
+      for (int v = 0; v < NV; v++) G.q[(size_t)v * N + c] = q[v];
     }
-    if (parts & 12) {
+    if constexpr ((PARTS & 12) != 0) {
       double hm = G.hmax[c], pm = G.pdyn[c], mv = G.mvel[c];
       unsigned long long vb = G.vuln ? G.vuln[c] : 0ULL;
-      epilogue<NS>(P, q, hm, pm, mv, vb, (parts & 8) != 0);
+      epilogue<NS>(P, q, hm, pm, mv, vb, ALL);
       G.hmax[c] = hm;
       G.pdyn[c] = pm;
       G.mvel[c] = mv;
       if (G.vuln) G.vuln[c] = vb;
     }
-    if ((parts & 3) && wet0 && q[0] == 0.0) dried++;
-  }
   }
   if (err) atomicOr(&G.st->err, err);
   if (dried) atomicAdd(&G.st->dried, (unsigned long long)dried);
-  if (mode == 1 && blockIdx.x == 0 && threadIdx.x == 0) G.st->t = G.st->t + dt;
+  if ((PARTS & 4) && mode == 1 && blockIdx.x == 0 && threadIdx.x == 0) G.st->t = G.st->t + dt;
 }
 
 // ===========================================================================
@@ -1456,6 +1477,7 @@ extern "C" int imex_create(const imex_config* cfg, imex_handle** out) {
     if ((rc = dalloc(h, &G.vuln, N))) return rc;
   if ((rc = dalloc(h, &G.mask, N))) return rc;
   if ((rc = dalloc(h, &G.tile_list, 2 * (size_t)h->ntiles))) return rc;
+  if ((rc = dalloc(h, &G.work, N))) return rc;
   double *bx, *by, *ap, *ax, *ay, *xc, *xs, *yc, *ys;
   signed char* sc;
   unsigned char* sd;
@@ -1536,17 +1558,28 @@ static int grid_for(const imex_handle* h, int per_cu) {
 
 template <int NS>
 static void launch_update(imex_handle* h, int parts, double dt, int mode) {
-  const int nb = (parts & 8) ? h->ntiles : grid_for(h, 4);
-  hipLaunchKernelGGL(k_update<NS>, dim3(nb), dim3(NTHR), 0, h->stream, h->P, h->G, parts, dt,
-                     mode, h->par);
-}
-static int update(imex_handle* h, int parts, double dt, int mode) {
-  switch (h->P.ns) {
-    case 1: launch_update<1>(h, parts, dt, mode); break;
-    case 2: launch_update<2>(h, parts, dt, mode); break;
-    default: launch_update<3>(h, parts, dt, mode); break;
+  const int nb = (parts & 8) ? (h->G.N + NTHR - 1) / NTHR : grid_for(h, 4);
+  const dim3 g(nb), b(NTHR);
+  switch (parts) {
+    case 1: hipLaunchKernelGGL((k_update<NS, 1>), g, b, 0, h->stream, h->P, h->G, dt, mode); break;
+    case 2: hipLaunchKernelGGL((k_update<NS, 2>), g, b, 0, h->stream, h->P, h->G, dt, mode); break;
+    case 4: hipLaunchKernelGGL((k_update<NS, 4>), g, b, 0, h->stream, h->P, h->G, dt, mode); break;
+    case 6: hipLaunchKernelGGL((k_update<NS, 6>), g, b, 0, h->stream, h->P, h->G, dt, mode); break;
+    default: hipLaunchKernelGGL((k_update<NS, 8>), g, b, 0, h->stream, h->P, h->G, dt, mode); break;
   }
-  HCHK(h, hipGetLastError());
+}
+// parts: 1, 2, 4, 8 as the host-driven entry points call them, or 1|2|4 for
+// a device-driven step, which runs as two launches (1, then 2|4).
+static int update(imex_handle* h, int parts, double dt, int mode) {
+  const int seq[2] = {(parts == 7) ? 1 : parts, (parts == 7) ? 6 : 0};
+  for (int k = 0; k < 2 && seq[k]; k++) {
+    switch (h->P.ns) {
+      case 1: launch_update<1>(h, seq[k], dt, mode); break;
+      case 2: launch_update<2>(h, seq[k], dt, mode); break;
+      default: launch_update<3>(h, seq[k], dt, mode); break;
+    }
+    HCHK(h, hipGetLastError());
+  }
   return 0;
 }
 static int mask(imex_handle* h, int solve_all) {
@@ -1847,7 +1880,7 @@ extern "C" int imex_download(imex_handle* h, unsigned what, double* q_aos, doubl
       for (int l = 0; l < nl; l++) vuln[(size_t)c * nl + l] = (int32_t)((vb[c] >> l) & 1ULL);
   }
   if (!mb.empty())
-    for (int c = 0; c < N; c++) maskout[c] = mb[c];
+    for (int c = 0; c < N; c++) maskout[c] = mb[c] & 1;
   return 0;
 }
 

@@ -41,7 +41,7 @@ struct DevState {
   unsigned long long dtmin_bits;
   unsigned int n_tiles[2];  // active tiles listed by k_mask, per list slot
   unsigned int err;
-  unsigned int pad;
+  unsigned int n_work;      // cells k_flux listed for k_update
   unsigned long long dried;
   unsigned long long active;
 };
@@ -56,6 +56,7 @@ struct Grid {
   unsigned long long* vuln;  // [N] bit (i_thk*n_pdyn + i_pdyn)
   unsigned char* mask;       // [N]
   int* tile_list;            // [2][ntiles] active tiles, two slots
+  int* work;                 // [N] active cells with a non-zero state or right-hand side
   const double* bx;          // [N] B_prime_x_geom
   const double* by;          // [N] B_prime_y_geom
   const signed char* scell;  // [N] source_cell

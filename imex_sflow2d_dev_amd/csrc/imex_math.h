@@ -7,14 +7,17 @@
 // fraction of the calls, and the settling iteration's 1e-6 stop and the
 // Newton tolerances amplify such bits along a run.  pow_cr returns the
 // correctly rounded result (round to nearest) of the exact x**y instead:
-//   log x = e ln2 + 2 atanh(s),  s = (m - 1)/(m + 1),  m in [sqrt(1/2), sqrt(2)),
-//           the atanh series summed to s**43 in double-double (|s| <= 0.1716);
-//   exp z = 2**k (1 + p),  p = expm1(r / 256) by its Taylor series to r**10,
-//           squared back 8 times as p <- 2p + p**2;  ln2 split in three parts.
-// The relative error of the double-double result is below 2**-95, so the
-// final rounding is correct unless x**y lies within 2**-42 ulp of a rounding
-// boundary.  Every function here compiles for the host as well as the device
-// (the host build is what the CPU tests check against a quad-precision pow).
+//   log x = e ln2 + log c + 2 atanh(s),  x = 2**e m,  m in [sqrt(1/2), sqrt(2)),
+//           c = 1 + k/128 the nearest table point to m,  s = (m - c)/(m + c),
+//           |s| < 0.0028, the atanh series summed to s**15;
+//   exp z = 2**e 2**(j/128) (1 + expm1(r)),  z = (128 e + j) ln2/128 + r,
+//           |r| < 0.0028, the expm1 Taylor series summed to r**11;
+// with the tables log(1 + k/128) and 2**(j/128), and ln2 split in three
+// parts, all exact to about 2**-106.  The relative error of the
+// double-double result is below 2**-100, so the final rounding is correct
+// unless x**y lies within 2**-47 ulp of a rounding boundary.  Every function
+// here compiles for the host as well as the device (the host build is what
+// the CPU tests check against a quad-precision pow).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -76,47 +79,268 @@ __host__ __device__ __forceinline__ dd div(dd a, dd b) {
   return add(fast_two_sum(q1, q2), dd{q3, 0.0});
 }
 
-// 1/(2k+1), k = 0..21, as exact-to-2**-106 pairs
+// 1/(2k+1), k = 0..7
 __host__ __device__ __forceinline__ dd inv_odd(int k) {
-  switch (k) {
-    case 0: return dd{0x1.0000000000000p+0, 0x0.0p+0};
-    case 1: return dd{0x1.5555555555555p-2, 0x1.5555555555555p-56};
-    case 2: return dd{0x1.999999999999ap-3, -0x1.999999999999ap-57};
-    case 3: return dd{0x1.2492492492492p-3, 0x1.2492492492492p-57};
-    case 4: return dd{0x1.c71c71c71c71cp-4, 0x1.c71c71c71c71cp-58};
-    case 5: return dd{0x1.745d1745d1746p-4, -0x1.745d1745d1746p-59};
-    case 6: return dd{0x1.3b13b13b13b14p-4, -0x1.3b13b13b13b14p-58};
-    case 7: return dd{0x1.1111111111111p-4, 0x1.1111111111111p-60};
-    case 8: return dd{0x1.e1e1e1e1e1e1ep-5, 0x1.e1e1e1e1e1e1ep-61};
-    case 9: return dd{0x1.af286bca1af28p-5, 0x1.af286bca1af28p-59};
-    case 10: return dd{0x1.8618618618618p-5, 0x1.8618618618618p-59};
-    case 11: return dd{0x1.642c8590b2164p-5, 0x1.642c8590b2164p-60};
-    case 12: return dd{0x1.47ae147ae147bp-5, -0x1.eb851eb851eb8p-61};
-    case 13: return dd{0x1.2f684bda12f68p-5, 0x1.2f684bda12f68p-59};
-    case 14: return dd{0x1.1a7b9611a7b96p-5, 0x1.1a7b9611a7b96p-61};
-    case 15: return dd{0x1.0842108421084p-5, 0x1.0842108421084p-60};
-    case 16: return dd{0x1.f07c1f07c1f08p-6, -0x1.f07c1f07c1f08p-61};
-    case 17: return dd{0x1.d41d41d41d41dp-6, 0x1.0750750750750p-60};
-    case 18: return dd{0x1.bacf914c1bad0p-6, -0x1.bacf914c1bad0p-60};
-    case 19: return dd{0x1.a41a41a41a41ap-6, 0x1.0690690690690p-60};
-    case 20: return dd{0x1.8f9c18f9c18fap-6, -0x1.f3831f3831f38p-61};
-    default: return dd{0x1.7d05f417d05f4p-6, 0x1.7d05f417d05f4p-62};
-  }
+  static constexpr double T[8][2] = {
+      {0x1.0000000000000p+0, 0x0.0p+0},
+      {0x1.5555555555555p-2, 0x1.5555555555555p-56},
+      {0x1.999999999999ap-3, -0x1.999999999999ap-57},
+      {0x1.2492492492492p-3, 0x1.2492492492492p-57},
+      {0x1.c71c71c71c71cp-4, 0x1.c71c71c71c71cp-58},
+      {0x1.745d1745d1746p-4, -0x1.745d1745d1746p-59},
+      {0x1.3b13b13b13b14p-4, -0x1.3b13b13b13b14p-58},
+      {0x1.1111111111111p-4, 0x1.1111111111111p-60}};
+  return dd{T[k][0], T[k][1]};
 }
 
-// 1/n!, n = 2..10
+// 1/n!, n = 2..11
 __host__ __device__ __forceinline__ dd inv_fact(int n) {
-  switch (n) {
-    case 2: return dd{0x1.0000000000000p-1, 0x0.0p+0};
-    case 3: return dd{0x1.5555555555555p-3, 0x1.5555555555555p-57};
-    case 4: return dd{0x1.5555555555555p-5, 0x1.5555555555555p-59};
-    case 5: return dd{0x1.1111111111111p-7, 0x1.1111111111111p-63};
-    case 6: return dd{0x1.6c16c16c16c17p-10, -0x1.f49f49f49f49fp-65};
-    case 7: return dd{0x1.a01a01a01a01ap-13, 0x1.a01a01a01a01ap-73};
-    case 8: return dd{0x1.a01a01a01a01ap-16, 0x1.a01a01a01a01ap-76};
-    case 9: return dd{0x1.71de3a556c734p-19, -0x1.c154f8ddc6c00p-73};
-    default: return dd{0x1.27e4fb7789f5cp-22, 0x1.cbbc05b4fa99ap-76};
-  }
+  static constexpr double T[10][2] = {
+      {0x1.0000000000000p-1, 0x0.0p+0},
+      {0x1.5555555555555p-3, 0x1.5555555555555p-57},
+      {0x1.5555555555555p-5, 0x1.5555555555555p-59},
+      {0x1.1111111111111p-7, 0x1.1111111111111p-63},
+      {0x1.6c16c16c16c17p-10, -0x1.f49f49f49f49fp-65},
+      {0x1.a01a01a01a01ap-13, 0x1.a01a01a01a01ap-73},
+      {0x1.a01a01a01a01ap-16, 0x1.a01a01a01a01ap-76},
+      {0x1.71de3a556c734p-19, -0x1.c154f8ddc6c00p-73},
+      {0x1.27e4fb7789f5cp-22, 0x1.cbbc05b4fa99ap-76},
+      {0x1.ae64567f544e4p-26, -0x1.c062e06d1f209p-80}};
+  return dd{T[n - 2][0], T[n - 2][1]};
+}
+
+// log(1 + k/128), k = -38..53
+__host__ __device__ __forceinline__ dd log_table(int k) {
+  static constexpr double T[92][2] = {
+      {-0x1.68ac83e9c6a14p-2, -0x1.a64eadd740178p-58},
+      {-0x1.5d5bddf595f30p-2, 0x1.6541148cbb8a2p-56},
+      {-0x1.522ae0738a3d8p-2, 0x1.8f7e9b38a6979p-57},
+      {-0x1.4718dc271c41bp-2, -0x1.8fb4c14c56eefp-60},
+      {-0x1.3c25277333184p-2, 0x1.2ad27e50a8ec6p-56},
+      {-0x1.314f1e1d35ce4p-2, 0x1.3d69909e5c3dcp-56},
+      {-0x1.269621134db92p-2, -0x1.e0efadd9db02bp-56},
+      {-0x1.1bf99635a6b95p-2, 0x1.12aeb84249223p-57},
+      {-0x1.1178e8227e47cp-2, 0x1.0e63a5f01c691p-57},
+      {-0x1.07138604d5862p-2, -0x1.cdb16ed4e9138p-56},
+      {-0x1.f991c6cb3b379p-3, -0x1.f665066f980a2p-57},
+      {-0x1.e530effe71012p-3, -0x1.2276041f43042p-59},
+      {-0x1.d1037f2655e7bp-3, -0x1.60629242471a2p-57},
+      {-0x1.bd087383bd8adp-3, -0x1.dd355f6a516d7p-60},
+      {-0x1.a93ed3c8ad9e3p-3, -0x1.bcafa9de97203p-57},
+      {-0x1.95a5adcf7017fp-3, -0x1.142c507fb7a3dp-58},
+      {-0x1.823c16551a3c2p-3, 0x1.1232ce70be781p-57},
+      {-0x1.6f0128b756abcp-3, 0x1.8de59c21e166cp-57},
+      {-0x1.5bf406b543db2p-3, 0x1.1f5b44c0df7e7p-61},
+      {-0x1.4913d8333b561p-3, 0x1.0d5604930f135p-58},
+      {-0x1.365fcb0159016p-3, -0x1.7d411a5b944adp-58},
+      {-0x1.23d712a49c202p-3, 0x1.6e38161051d69p-57},
+      {-0x1.1178e8227e47cp-3, 0x1.0e63a5f01c691p-58},
+      {-0x1.fe89139dbd566p-4, 0x1.ac9f4215f9393p-58},
+      {-0x1.da727638446a2p-4, -0x1.401fa71733019p-58},
+      {-0x1.b6ac88dad5b1cp-4, 0x1.0057eed1ca59fp-59},
+      {-0x1.9335e5d594989p-4, 0x1.478a85704ccb7p-58},
+      {-0x1.700d30aeac0e1p-4, 0x1.72566212cdd05p-61},
+      {-0x1.4d3115d207eacp-4, -0x1.769f42c7842ccp-58},
+      {-0x1.2aa04a44717a5p-4, 0x1.d15d38d2fa3f7p-58},
+      {-0x1.08598b59e3a07p-4, 0x1.dd7009902bf32p-58},
+      {-0x1.ccb73cdddb2ccp-5, 0x1.e48fb0500efd4p-59},
+      {-0x1.894aa149fb343p-5, -0x1.a8be97660a23dp-60},
+      {-0x1.466aed42de3eap-5, 0x1.cdd6f7f4a137ep-59},
+      {-0x1.0415d89e74444p-5, -0x1.c05cf1d753622p-59},
+      {-0x1.8492528c8cabfp-6, 0x1.d192d0619fa67p-60},
+      {-0x1.0205658935847p-6, -0x1.27c8e8416e71fp-60},
+      {-0x1.010157588de71p-7, -0x1.46662d417ced0p-62},
+      {0x0.0p+0, 0x0.0p+0},
+      {0x1.fe02a6b106789p-8, -0x1.e44b7e3711ebfp-67},
+      {0x1.fc0a8b0fc03e4p-7, -0x1.83092c59642a1p-62},
+      {0x1.7b91b07d5b11bp-6, -0x1.5b602ace3a510p-60},
+      {0x1.f829b0e783300p-6, 0x1.33e3f04f1ef23p-60},
+      {0x1.39e87b9febd60p-5, -0x1.5bfa937f551bbp-59},
+      {0x1.77458f632dcfcp-5, 0x1.18d3ca87b9296p-59},
+      {0x1.b42dd711971bfp-5, -0x1.eb9759c130499p-60},
+      {0x1.f0a30c01162a6p-5, 0x1.85f325c5bbacdp-59},
+      {0x1.16536eea37ae1p-4, -0x1.79da3e8c22cdap-60},
+      {0x1.341d7961bd1d1p-4, -0x1.b599f227becbbp-58},
+      {0x1.51b073f06183fp-4, 0x1.a49e39a1a8be4p-58},
+      {0x1.6f0d28ae56b4cp-4, -0x1.906d99184b992p-58},
+      {0x1.8c345d6319b21p-4, -0x1.4a697ab3424a9p-61},
+      {0x1.a926d3a4ad563p-4, 0x1.942f48aa70ea9p-58},
+      {0x1.c5e548f5bc743p-4, 0x1.5d617ef8161b1p-60},
+      {0x1.e27076e2af2e6p-4, -0x1.61578001e0162p-60},
+      {0x1.fec9131dbeabbp-4, -0x1.5746b9981b36cp-58},
+      {0x1.0d77e7cd08e59p-3, 0x1.9a5dc5e9030acp-57},
+      {0x1.1b72ad52f67a0p-3, 0x1.483023472cd74p-58},
+      {0x1.29552f81ff523p-3, 0x1.301771c407dbfp-57},
+      {0x1.371fc201e8f74p-3, 0x1.de6cb62af18a0p-58},
+      {0x1.44d2b6ccb7d1ep-3, 0x1.9f4f6543e1f88p-57},
+      {0x1.526e5e3a1b438p-3, -0x1.746ff8a470d3ap-57},
+      {0x1.5ff3070a793d4p-3, -0x1.bc60efafc6f6ep-58},
+      {0x1.6d60fe719d21dp-3, -0x1.caae268ecd179p-57},
+      {0x1.7ab890210d909p-3, 0x1.be36b2d6a0608p-59},
+      {0x1.87fa06520c911p-3, -0x1.bf7fdbfa08d9ap-57},
+      {0x1.9525a9cf456b4p-3, 0x1.d904c1d4e2e26p-57},
+      {0x1.a23bc1fe2b563p-3, 0x1.93711b07a998cp-59},
+      {0x1.af3c94e80bff3p-3, -0x1.398cff3641985p-58},
+      {0x1.bc286742d8cd6p-3, 0x1.4fce744870f55p-58},
+      {0x1.c8ff7c79a9a22p-3, -0x1.4f689f8434012p-57},
+      {0x1.d5c216b4fbb91p-3, 0x1.6e443597e4d40p-57},
+      {0x1.e27076e2af2e6p-3, -0x1.61578001e0162p-59},
+      {0x1.ef0adcbdc5936p-3, 0x1.48637950dc20dp-57},
+      {0x1.fb9186d5e3e2bp-3, -0x1.caaae64f21acbp-57},
+      {0x1.0402594b4d041p-2, -0x1.28ec217a5022dp-57},
+      {0x1.0a324e27390e3p-2, 0x1.7dcfde8061c03p-56},
+      {0x1.1058bf9ae4ad5p-2, 0x1.89fa0ab4cb31dp-58},
+      {0x1.1675cababa60ep-2, 0x1.ce63eab883717p-61},
+      {0x1.1c898c16999fbp-2, -0x1.0e5c62aff1c44p-60},
+      {0x1.22941fbcf7966p-2, -0x1.76f5eb09628afp-56},
+      {0x1.2895a13de86a3p-2, 0x1.7ad24c13f040ep-56},
+      {0x1.2e8e2bae11d31p-2, -0x1.8f4cdb95ebdf9p-56},
+      {0x1.347dd9a987d55p-2, -0x1.4dd4c580919f8p-57},
+      {0x1.3a64c556945eap-2, -0x1.c68651945f97cp-57},
+      {0x1.404308686a7e4p-2, -0x1.0bcfb6082ce6dp-56},
+      {0x1.4618bc21c5ec2p-2, 0x1.f42decdeccf1dp-56},
+      {0x1.4be5f957778a1p-2, -0x1.259b35b04813dp-57},
+      {0x1.51aad872df82dp-2, 0x1.3927ac19f55e3p-59},
+      {0x1.5767717455a6cp-2, 0x1.526adb283660cp-56},
+      {0x1.5d1bdbf5809cap-2, 0x1.4236383dc7fe1p-56},
+      {0x1.62c82f2b9c795p-2, 0x1.7b7af915300e5p-57},
+  };
+  return dd{T[k + 38][0], T[k + 38][1]};
+}
+
+// 2**(j/128), j = 0..127
+__host__ __device__ __forceinline__ dd exp2_table(int j) {
+  static constexpr double T[128][2] = {
+      {0x1.0000000000000p+0, 0x0.0p+0},
+      {0x1.0163da9fb3335p+0, 0x1.b61299ab8cdb7p-54},
+      {0x1.02c9a3e778061p+0, -0x1.19083535b085dp-56},
+      {0x1.04315e86e7f85p+0, -0x1.0a31c1977c96ep-54},
+      {0x1.059b0d3158574p+0, 0x1.d73e2a475b465p-55},
+      {0x1.0706b29ddf6dep+0, -0x1.c91dfe2b13c27p-55},
+      {0x1.0874518759bc8p+0, 0x1.186be4bb284ffp-57},
+      {0x1.09e3ecac6f383p+0, 0x1.1487818316136p-54},
+      {0x1.0b5586cf9890fp+0, 0x1.8a62e4adc610bp-54},
+      {0x1.0cc922b7247f7p+0, 0x1.01edc16e24f71p-54},
+      {0x1.0e3ec32d3d1a2p+0, 0x1.03a1727c57b53p-59},
+      {0x1.0fb66affed31bp+0, -0x1.b9bedc44ebd7bp-57},
+      {0x1.11301d0125b51p+0, -0x1.6c51039449b3ap-54},
+      {0x1.12abdc06c31ccp+0, -0x1.1b514b36ca5c7p-58},
+      {0x1.1429aaea92de0p+0, -0x1.32fbf9af1369ep-54},
+      {0x1.15a98c8a58e51p+0, 0x1.2406ab9eeab0ap-55},
+      {0x1.172b83c7d517bp+0, -0x1.19041b9d78a76p-55},
+      {0x1.18af9388c8deap+0, -0x1.11023d1970f6cp-54},
+      {0x1.1a35beb6fcb75p+0, 0x1.e5b4c7b4968e4p-55},
+      {0x1.1bbe084045cd4p+0, -0x1.95386352ef607p-54},
+      {0x1.1d4873168b9aap+0, 0x1.e016e00a2643cp-54},
+      {0x1.1ed5022fcd91dp+0, -0x1.1df98027bb78cp-54},
+      {0x1.2063b88628cd6p+0, 0x1.dc775814a8495p-55},
+      {0x1.21f49917ddc96p+0, 0x1.2a97e9494a5eep-55},
+      {0x1.2387a6e756238p+0, 0x1.9b07eb6c70573p-54},
+      {0x1.251ce4fb2a63fp+0, 0x1.ac155bef4f4a4p-55},
+      {0x1.26b4565e27cddp+0, 0x1.2bd339940e9d9p-55},
+      {0x1.284dfe1f56381p+0, -0x1.a4c3a8c3f0d7ep-54},
+      {0x1.29e9df51fdee1p+0, 0x1.612e8afad1255p-55},
+      {0x1.2b87fd0dad990p+0, -0x1.10adcd6381aa4p-59},
+      {0x1.2d285a6e4030bp+0, 0x1.0024754db41d5p-54},
+      {0x1.2ecafa93e2f56p+0, 0x1.1ca0f45d52383p-56},
+      {0x1.306fe0a31b715p+0, 0x1.6f46ad23182e4p-55},
+      {0x1.32170fc4cd831p+0, 0x1.a9ce78e18047cp-55},
+      {0x1.33c08b26416ffp+0, 0x1.32721843659a6p-54},
+      {0x1.356c55f929ff1p+0, -0x1.b5cee5c4e4628p-55},
+      {0x1.371a7373aa9cbp+0, -0x1.63aeabf42eae2p-54},
+      {0x1.38cae6d05d866p+0, -0x1.e958d3c9904bdp-54},
+      {0x1.3a7db34e59ff7p+0, -0x1.5e436d661f5e3p-56},
+      {0x1.3c32dc313a8e5p+0, -0x1.efff8375d29c3p-54},
+      {0x1.3dea64c123422p+0, 0x1.ada0911f09ebcp-55},
+      {0x1.3fa4504ac801cp+0, -0x1.7d023f956f9f3p-54},
+      {0x1.4160a21f72e2ap+0, -0x1.ef3691c309278p-58},
+      {0x1.431f5d950a897p+0, -0x1.1c7dde35f7999p-55},
+      {0x1.44e086061892dp+0, 0x1.89b7a04ef80d0p-59},
+      {0x1.46a41ed1d0057p+0, 0x1.c944bd1648a76p-54},
+      {0x1.486a2b5c13cd0p+0, 0x1.3c1a3b69062f0p-56},
+      {0x1.4a32af0d7d3dep+0, 0x1.9cb62f3d1be56p-54},
+      {0x1.4bfdad5362a27p+0, 0x1.d4397afec42e2p-56},
+      {0x1.4dcb299fddd0dp+0, 0x1.8ecdbbc6a7833p-54},
+      {0x1.4f9b2769d2ca7p+0, -0x1.4b309d25957e3p-54},
+      {0x1.516daa2cf6642p+0, -0x1.f768569bd93efp-55},
+      {0x1.5342b569d4f82p+0, -0x1.07abe1db13cadp-55},
+      {0x1.551a4ca5d920fp+0, -0x1.d689cefede59bp-55},
+      {0x1.56f4736b527dap+0, 0x1.9bb2c011d93adp-54},
+      {0x1.58d12d497c7fdp+0, 0x1.295e15b9a1de8p-55},
+      {0x1.5ab07dd485429p+0, 0x1.6324c054647adp-54},
+      {0x1.5c9268a5946b7p+0, 0x1.c4b1b816986a2p-60},
+      {0x1.5e76f15ad2148p+0, 0x1.ba6f93080e65ep-54},
+      {0x1.605e1b976dc09p+0, -0x1.3e2429b56de47p-54},
+      {0x1.6247eb03a5585p+0, -0x1.383c17e40b497p-54},
+      {0x1.6434634ccc320p+0, -0x1.c483c759d8933p-55},
+      {0x1.6623882552225p+0, -0x1.bb60987591c34p-54},
+      {0x1.68155d44ca973p+0, 0x1.038ae44f73e65p-57},
+      {0x1.6a09e667f3bcdp+0, -0x1.bdd3413b26456p-54},
+      {0x1.6c012750bdabfp+0, -0x1.2895667ff0b0dp-56},
+      {0x1.6dfb23c651a2fp+0, -0x1.bbe3a683c88abp-57},
+      {0x1.6ff7df9519484p+0, -0x1.83c0f25860ef6p-55},
+      {0x1.71f75e8ec5f74p+0, -0x1.16e4786887a99p-55},
+      {0x1.73f9a48a58174p+0, -0x1.0a8d96c65d53cp-54},
+      {0x1.75feb564267c9p+0, -0x1.0245957316dd3p-54},
+      {0x1.780694fde5d3fp+0, 0x1.866b80a02162dp-54},
+      {0x1.7a11473eb0187p+0, -0x1.41577ee04992fp-55},
+      {0x1.7c1ed0130c132p+0, 0x1.f124cd1164dd6p-54},
+      {0x1.7e2f336cf4e62p+0, 0x1.05d02ba15797ep-56},
+      {0x1.80427543e1a12p+0, -0x1.27c86626d972bp-54},
+      {0x1.82589994cce13p+0, -0x1.d4c1dd41532d8p-54},
+      {0x1.8471a4623c7adp+0, -0x1.8d684a341cdfbp-55},
+      {0x1.868d99b4492edp+0, -0x1.fc6f89bd4f6bap-54},
+      {0x1.88ac7d98a6699p+0, 0x1.994c2f37cb53ap-54},
+      {0x1.8ace5422aa0dbp+0, 0x1.6e9f156864b27p-54},
+      {0x1.8cf3216b5448cp+0, -0x1.0d55e32e9e3aap-56},
+      {0x1.8f1ae99157736p+0, 0x1.5cc13a2e3976cp-55},
+      {0x1.9145b0b91ffc6p+0, -0x1.dd6792e582524p-54},
+      {0x1.93737b0cdc5e5p+0, -0x1.75fc781b57ebcp-57},
+      {0x1.95a44cbc8520fp+0, -0x1.64b7c96a5f039p-56},
+      {0x1.97d829fde4e50p+0, -0x1.d185b7c1b85d1p-54},
+      {0x1.9a0f170ca07bap+0, -0x1.173bd91cee632p-54},
+      {0x1.9c49182a3f090p+0, 0x1.c7c46b071f2bep-56},
+      {0x1.9e86319e32323p+0, 0x1.824ca78e64c6ep-56},
+      {0x1.a0c667b5de565p+0, -0x1.359495d1cd533p-54},
+      {0x1.a309bec4a2d33p+0, 0x1.6305c7ddc36abp-54},
+      {0x1.a5503b23e255dp+0, -0x1.d2f6edb8d41e1p-54},
+      {0x1.a799e1330b358p+0, 0x1.bcb7ecac563c7p-54},
+      {0x1.a9e6b5579fdbfp+0, 0x1.0fac90ef7fd31p-54},
+      {0x1.ac36bbfd3f37ap+0, -0x1.f9234cae76cd0p-55},
+      {0x1.ae89f995ad3adp+0, 0x1.7a1cd345dcc81p-54},
+      {0x1.b0e07298db666p+0, -0x1.bdef54c80e425p-54},
+      {0x1.b33a2b84f15fbp+0, -0x1.2805e3084d708p-57},
+      {0x1.b59728de5593ap+0, -0x1.c71dfbbba6de3p-54},
+      {0x1.b7f76f2fb5e47p+0, -0x1.5584f7e54ac3bp-56},
+      {0x1.ba5b030a1064ap+0, -0x1.efcd30e54292ep-54},
+      {0x1.bcc1e904bc1d2p+0, 0x1.23dd07a2d9e84p-55},
+      {0x1.bf2c25bd71e09p+0, -0x1.efdca3f6b9c73p-54},
+      {0x1.c199bdd85529cp+0, 0x1.11065895048ddp-55},
+      {0x1.c40ab5fffd07ap+0, 0x1.b4537e083c60ap-54},
+      {0x1.c67f12e57d14bp+0, 0x1.2884dff483cadp-54},
+      {0x1.c8f6d9406e7b5p+0, 0x1.1acbc48805c44p-56},
+      {0x1.cb720dcef9069p+0, 0x1.503cbd1e949dbp-56},
+      {0x1.cdf0b555dc3fap+0, -0x1.dd83b53829d72p-55},
+      {0x1.d072d4a07897cp+0, -0x1.cbc3743797a9cp-54},
+      {0x1.d2f87080d89f2p+0, -0x1.d487b719d8578p-54},
+      {0x1.d5818dcfba487p+0, 0x1.2ed02d75b3707p-55},
+      {0x1.d80e316c98398p+0, -0x1.11ec18beddfe8p-54},
+      {0x1.da9e603db3285p+0, 0x1.c2300696db532p-54},
+      {0x1.dd321f301b460p+0, 0x1.2da5778f018c3p-54},
+      {0x1.dfc97337b9b5fp+0, -0x1.1a5cd4f184b5cp-54},
+      {0x1.e264614f5a129p+0, -0x1.7b627817a1496p-54},
+      {0x1.e502ee78b3ff6p+0, 0x1.39e8980a9cc8fp-55},
+      {0x1.e7a51fbc74c83p+0, 0x1.2d522ca0c8de2p-54},
+      {0x1.ea4afa2a490dap+0, -0x1.e9c23179c2893p-54},
+      {0x1.ecf482d8e67f1p+0, -0x1.c93f3b411ad8cp-54},
+      {0x1.efa1bee615a27p+0, 0x1.dc7f486a4b6b0p-54},
+      {0x1.f252b376bba97p+0, 0x1.3a1a5bf0d8e43p-54},
+      {0x1.f50765b6e4540p+0, 0x1.9d3e12dd8a18bp-54},
+      {0x1.f7bfdad9cbe14p+0, -0x1.dbb12d006350ap-54},
+      {0x1.fa7c1819e90d8p+0, 0x1.74853f3a5931ep-55},
+      {0x1.fd3c22b8f71f1p+0, 0x1.2eb74966579e7p-57},
+  };
+  return dd{T[j][0], T[j][1]};
 }
 
 // ln 2 = LN2_1 + LN2_2 + LN2_3 to about 2**-160
@@ -133,15 +357,18 @@ __host__ __device__ __forceinline__ dd log_dd(double x) {
     m = 2.0 * m;
     e -= 1;
   }
-  const double f = m - 1.0;  // exact (Sterbenz)
-  const dd s = div(dd{f, 0.0}, two_sum(m, 1.0));
+  const int k = (int)rint((m - 1.0) * 128.0);  // -38..53
+  const double c = 1.0 + k * 0x1.0p-7;
+  // m - c is exact (c/2 <= m <= 2c)
+  const dd s = div(dd{m - c, 0.0}, two_sum(m, c));
   const dd s2 = mul(s, s);
-  dd acc = inv_odd(21);
-#pragma unroll
-  for (int k = 20; k >= 0; k--) acc = add(inv_odd(k), mul(s2, acc));
+  dd acc = inv_odd(7);
+#pragma unroll 1
+  for (int i = 6; i >= 0; i--) acc = add(inv_odd(i), mul(s2, acc));
   dd lm = mul(s, acc);
   lm.hi = 2.0 * lm.hi;
   lm.lo = 2.0 * lm.lo;
+  lm = add(log_table(k), lm);
   const double ed = (double)e;
   dd el = add(two_prod(ed, LN2_1), two_prod(ed, LN2_2));
   el = add(el, dd{ed * LN2_3, 0.0});
@@ -152,25 +379,24 @@ __host__ __device__ __forceinline__ dd log_dd(double x) {
 __host__ __device__ __forceinline__ double exp_dd(dd z) {
   if (z.hi > 709.8) return HUGE_VAL;
   if (z.hi < -745.2) return 0.0;
-  const double kd = floor(z.hi * 0x1.71547652b82fep+0 + 0.5);  // z / ln2
-  dd r = add(z, two_prod(-kd, LN2_1));
-  r = add(r, two_prod(-kd, LN2_2));
-  r = add(r, dd{-kd * LN2_3, 0.0});
-  r.hi = r.hi * 0x1.0p-8;  // exact scaling
-  r.lo = r.lo * 0x1.0p-8;
-  dd acc = inv_fact(10);
-#pragma unroll
-  for (int n = 9; n >= 2; n--) acc = add(inv_fact(n), mul(r, acc));
-  dd p = add(r, mul(mul(r, r), acc));  // expm1(r)
-#pragma unroll
-  for (int i = 0; i < 8; i++) p = add(dd{2.0 * p.hi, 2.0 * p.lo}, mul(p, p));
-  const dd res = add(dd{1.0, 0.0}, p);
-  const int k = (int)kd;
-  if (k > -1022) return ldexp(res.hi, k);
-  // (sub)normal result below 2**-1021: round the pair once on the grid of
-  // 2**-1074 -- scale so that grid is the integers (exact), round hi to an
-  // integer, then let the remainder hi - f + lo decide the final step
-  const double hs = ldexp(res.hi, k + 1074), ls = ldexp(res.lo, k + 1074);
+  const double nd = rint(z.hi * 0x1.71547652b82fep+7);  // z 128 / ln2
+  const int n = (int)nd;
+  const int j = n & 127;
+  const int e = (n - j) / 128;
+  dd r = add(z, two_prod(-nd, LN2_1 * 0x1.0p-7));
+  r = add(r, two_prod(-nd, LN2_2 * 0x1.0p-7));
+  r = add(r, dd{-nd * (LN2_3 * 0x1.0p-7), 0.0});
+  dd acc = inv_fact(11);
+#pragma unroll 1
+  for (int i = 10; i >= 2; i--) acc = add(inv_fact(i), mul(r, acc));
+  const dd p = add(r, mul(mul(r, r), acc));  // expm1(r)
+  const dd t = exp2_table(j);
+  const dd res = add(t, mul(t, p));
+  if (e > -1022) return ldexp(res.hi, e);
+  // result below 2**-1021: round the pair once on the grid of 2**-1074 --
+  // scale so that grid is the integers (exact), round hi to an integer, then
+  // let the remainder hi - f + lo decide the final step
+  const double hs = ldexp(res.hi, e + 1074), ls = ldexp(res.lo, e + 1074);
   double f = rint(hs);
   const double rem = (hs - f) + ls;
   if (rem > 0.5 || (rem == 0.5 && fmod(f, 2.0) != 0.0)) f += 1.0;

@@ -562,6 +562,7 @@ __device__ inline double settling_velocity(const Params& P, double diam, double 
   double sv = const_part * inv_sqrt_C_D;
   double Rey = diam * sv * ikv;
   if (Rey <= 1000.0) {
+#pragma unroll 1
     for (int i = 0; i < 20; i++) {
       double old = sv;
       inv_sqrt_C_D = sqrt(Rey / (24.0 * (1.0 + 0.15 * pow_cr(Rey, 0.687))));
