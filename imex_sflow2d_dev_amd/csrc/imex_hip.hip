@@ -26,6 +26,7 @@ __global__ __launch_bounds__(NTHR) void k_mask(Params P, Grid G, int solve_all, 
   __shared__ unsigned char seed[RY][RX];
   __shared__ int s_any;
   __shared__ int s_cnt;
+  __shared__ int s_hot;
   const int nb = gridDim.x;
   const int tile = xcd_tile(blockIdx.x, nb);
   const int j0 = (tile % G.ntx) * TX, k0 = (tile / G.ntx) * TY;
@@ -33,7 +34,9 @@ __global__ __launch_bounds__(NTHR) void k_mask(Params P, Grid G, int solve_all, 
   if (tid == 0) {
     s_any = 0;
     s_cnt = 0;
+    s_hot = P.all_tiles;
   }
+  __syncthreads();
   for (int e = tid; e < RX * RY; e += NTHR) {
     const int lj = e % RX, lk = e / RX;
     const int j = j0 + lj - HALO, k = k0 + lk - HALO;
@@ -41,7 +44,9 @@ __global__ __launch_bounds__(NTHR) void k_mask(Params P, Grid G, int solve_all, 
     if (j >= 0 && j < G.nx && k >= 0 && k < G.ny) {
       const int c = j + G.nx * k;
       const bool ring = (j == 0 || j == G.nx - 1 || k == 0 || k == G.ny - 1);
-      if (ring || solve_all || G.q[c] > 0.0 || (P.radial && G.scell[c] == 2)) s = 1;
+      const bool wet = G.q[c] > 0.0, src = P.radial && G.scell[c] == 2;
+      if (ring || solve_all || wet || src) s = 1;
+      if (wet || src) s_hot = 1;
     }
     seed[lk][lj] = s;
   }
@@ -76,7 +81,11 @@ __global__ __launch_bounds__(NTHR) void k_mask(Params P, Grid G, int solve_all, 
       G.st->n_tiles[par ^ 1] = 0u;  // the next step's slot
       G.st->n_work = 0u;            // filled by the k_flux that follows
     }
-    if (s_any) {
+    // k_flux visits the tiles with active cells that have a wet or
+    // source-ring cell within two cells: elsewhere every face stencil is dry,
+    // so every flux and wave speed is zero, and the (dry, all-zero) cells get
+    // a zero right-hand side that k_update does not need
+    if (s_any && s_hot) {
       const unsigned int slot = atomicAdd(&G.st->n_tiles[par], 1u);
       G.tile_list[(size_t)par * G.ntx * G.nty + slot] = tile;
     }
@@ -511,8 +520,14 @@ __global__ __launch_bounds__(NTHR) void k_flux(Params P, Grid G, int mode, doubl
     for (int e = tid; e < (TX + 1) * TY; e += NTHR) {
       const int fl = e % (TX + 1), fk = e / (TX + 1);
       const int jf = j0 + fl, k = k0 + fk;
-      const bool need = (k < ny) && (jf <= nx) &&
-                        ((fl > 0 && sact[fk * TX + fl - 1]) || (fl < TX && jf < nx && sact[fk * TX + fl]));
+      bool need = (k < ny) && (jf <= nx) &&
+                  ((fl > 0 && sact[fk * TX + fl - 1]) || (fl < TX && jf < nx && sact[fk * TX + fl]));
+      // a face whose four stencil cells are dry has zero states on both
+      // sides: H = 0 and no wave speed (the minmod slope of a dry cell is 0)
+      if (need) {
+        const int b = (fk + HALO) * RX + fl + HALO;
+        need = (sq[b - 2] != 0.0) || (sq[b - 1] != 0.0) || (sq[b] != 0.0) || (sq[b + 1] != 0.0);
+      }
       double H[NV];
       double amax = 0.0;
 #pragma unroll
@@ -546,8 +561,13 @@ __global__ __launch_bounds__(NTHR) void k_flux(Params P, Grid G, int mode, doubl
     for (int e = tid; e < TX * (TY + 1); e += NTHR) {
       const int fj = e % TX, fl = e / TX;
       const int j = j0 + fj, kf = k0 + fl;
-      const bool need = (j < nx) && (kf <= ny) &&
-                        ((fl > 0 && sact[(fl - 1) * TX + fj]) || (fl < TY && kf < ny && sact[fl * TX + fj]));
+      bool need = (j < nx) && (kf <= ny) &&
+                  ((fl > 0 && sact[(fl - 1) * TX + fj]) || (fl < TY && kf < ny && sact[fl * TX + fj]));
+      if (need) {
+        const int b = (fl + HALO) * RX + fj + HALO;
+        need = (sq[b - 2 * RX] != 0.0) || (sq[b - RX] != 0.0) || (sq[b] != 0.0) ||
+               (sq[b + RX] != 0.0);
+      }
       double H[NV];
       double amax = 0.0;
 #pragma unroll
@@ -1612,6 +1632,16 @@ extern "C" int imex_upload_state(imex_handle* h, const double* q_aos, double t, 
   if (!h || !q_aos) return -2;
   HCHK(h, hipSetDevice(h->device));
   const int N = h->G.N, nv = h->P.nv;
+  // a dry cell (q1 <= 0) with other non-zero components is zeroed by the
+  // assembly the first time it is active; until then k_flux must not skip it
+  h->P.all_tiles = 0;
+  for (size_t c = 0; c < (size_t)N && !h->P.all_tiles; c++)
+    if (!(q_aos[c * nv] > 0.0))
+      for (int i = 1; i < nv; i++)
+        if (q_aos[c * nv + i] != 0.0) {
+          h->P.all_tiles = 1;
+          break;
+        }
   HCHK(h, hipMemcpyAsync(h->d_tmp, q_aos, 8 * (size_t)N * nv, hipMemcpyHostToDevice, h->stream));
   hipLaunchKernelGGL(k_aos_to_soa, dim3((N + 255) / 256), dim3(256), 0, h->stream, h->d_tmp, h->G.q, nv, N);
   HCHK(h, hipGetLastError());

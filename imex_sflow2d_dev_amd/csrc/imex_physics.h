@@ -26,6 +26,10 @@ struct Params {
   int nx, ny, ns, nv;
   int energy, settling, entrainment, radial;
   int n_thk, n_pdyn;
+  // 1 when a cell of the uploaded state is dry (q1 <= 0) but not all zero:
+  // then every tile with active cells goes through k_flux, not only the
+  // tiles near wet or source-ring cells
+  int all_tiles;
   double dx, dy, dx2, dy2, one_by_dx, one_by_dy;
   double cfl, max_dt;
   double grav, mu, xi, pres, inv_pres, T_amb, cp_a, R_a, kin_visc_c, rho_a_amb;
