@@ -717,13 +717,16 @@ __device__ void eval_f(const Params& P, double dt, const double* qj, const doubl
 }
 
 // Jacobian: diagonal qorg for the explicit rows, the 2x2 momentum block from
-// the complex step (all other entries are zero).
-template <int NS>
+// the complex step (all other entries are zero).  LANES = 2: lane `lane` of an
+// aligned pair evaluates column 1 + lane and the pair swaps the entries, so
+// the two complex evaluations run side by side; both lanes call together.
+template <int NS, int LANES>
 __device__ void eval_jacobian(const Params& P, double dt, const double* qrel, const double* qorg,
-                              double& L11, double& L21, double& L12, double& L22) {
+                              double& L11, double& L21, double& L12, double& L22, int lane) {
   constexpr int NV = 4 + NS;
 #pragma unroll
-  for (int col = 1; col <= 2; col++) {
+  for (int k = 0; k < 3 - LANES; k++) {
+    const int col = (LANES == 2) ? 1 + lane : 1 + k;
     cx qc[NV];
 #pragma unroll
     for (int m = 0; m < NV; m++)
@@ -740,6 +743,17 @@ __device__ void eval_jacobian(const Params& P, double dt, const double* qrel, co
       L22 = (1.0 * qorg[2]) - dt * 1.0 * J2;
     }
   }
+  if constexpr (LANES == 2) {
+    const double a = (lane == 0) ? L11 : L12, b = (lane == 0) ? L21 : L22;
+    const double oa = __shfl_xor(a, 1), ob = __shfl_xor(b, 1);
+    if (lane == 0) {
+      L12 = oa;
+      L22 = ob;
+    } else {
+      L11 = oa;
+      L21 = ob;
+    }
+  }
 }
 
 // column entry L(r, c) of the sparse Jacobian
@@ -750,8 +764,10 @@ __device__ __forceinline__ double Lrc(int r, int c, const double* qorg, double L
   return (r == c) ? 1.0 * qorg[c] : 0.0;
 }
 
+// Returns true when rt and f_out are the residual at the returned x (the
+// caller then need not evaluate it again: eval_f is a pure function).
 template <int NS>
-__device__ void lnsrch(const Params& P, double dt, const double* xold, const double* qorg,
+__device__ bool lnsrch(const Params& P, double dt, const double* xold, const double* qorg,
                        const double* q0, double fold, const double* g, double* p, double* x,
                        double& f_out, double* rt, double stpmax, const double* Rni,
                        unsigned int& err) {
@@ -778,8 +794,9 @@ __device__ void lnsrch(const Params& P, double dt, const double* xold, const dou
   if (alamin == 0.0) {
 #pragma unroll
     for (int i = 0; i < NV; i++) x[i] = xold[i];
-    return;
+    return true;  // rt and f_out still hold the residual at xold
   }
+  bool at_x = true;
   double alam = 1.0, alam2 = 0.0, f2 = 0.0, tmplam = 0.0, f = 0.0;
   double qj[NV];
   // The reference loop ends by construction: alam at least halves after the
@@ -799,6 +816,7 @@ __device__ void lnsrch(const Params& P, double dt, const double* xold, const dou
 #pragma unroll
       for (int i = 0; i < NV; i++) x[i] = xold[i];
       f = fold;
+      at_x = false;  // rt is the last trial's residual
       break;
     } else {
       if (alam == 1.0) {
@@ -827,13 +845,17 @@ __device__ void lnsrch(const Params& P, double dt, const double* xold, const dou
     f2 = f;
     alam = fmax_f(tmplam, 0.5 * alam);
   }
-  if (trial == 2200) err |= IMEX_ERR_NAN;
+  if (trial == 2200) {
+    err |= IMEX_ERR_NAN;
+    at_x = false;
+  }
   f_out = f;
+  return at_x;
 }
 
-template <int NS>
+template <int NS, int LANES>
 __device__ int solve_rk_step(const Params& P, double dt, double* qj, const double* q0,
-                             const double* Rni, unsigned int& err) {
+                             const double* Rni, unsigned int& err, int lane) {
   constexpr int NV = 4 + NS;
   const double TOLF = 1.0e-10, STPMX = 100.0;
   double qorg[NV], qrel[NV], rt[NV], xold[NV], p[NV], grad[NV];
@@ -846,13 +868,14 @@ __device__ int solve_rk_step(const Params& P, double dt, double* qj, const doubl
 #pragma unroll
   for (int i = 0; i < NV; i++) qrel[i] = qj[i] / qorg[i];
   int it;
+  bool fresh = false;  // rt and scal_f already hold the residual at qj
   for (it = 1; it <= 100; it++) {
-    eval_f<NS>(P, dt, qj, q0, Rni, rt, scal_f);
+    if (!fresh) eval_f<NS>(P, dt, qj, q0, Rni, rt, scal_f);
     double mx = 0.0;
 #pragma unroll
     for (int i = 0; i < NV; i++) mx = fmax_f(mx, fabs(rt[i]));
     if (mx < TOLF) break;
-    eval_jacobian<NS>(P, dt, qrel, qorg, L11, L21, L12, L22);
+    eval_jacobian<NS, LANES>(P, dt, qrel, qorg, L11, L21, L12, L22, lane);
     double d1[NV - 2];
     int ne = 0;
 #pragma unroll
@@ -901,13 +924,15 @@ __device__ int solve_rk_step(const Params& P, double dt, double* qj, const doubl
         for (int r = 0; r < NV; r++) s += rt[r] * Lrc(r, i, qorg, L11, L21, L12, L22);
         grad[i] = s;
       }
-      lnsrch<NS>(P, dt, xold, qorg, q0, scal_f_old, grad, p, qrel, scal_f, rt, stpmax, Rni, err);
+      fresh = lnsrch<NS>(P, dt, xold, qorg, q0, scal_f_old, grad, p, qrel, scal_f, rt, stpmax,
+                         Rni, err);
     } else {
 #pragma unroll
       for (int i = 0; i < NV; i++) qrel[i] = xold[i] + p[i];
 #pragma unroll
       for (int i = 0; i < NV; i++) qj[i] = qrel[i] * qorg[i];
       eval_f<NS>(P, dt, qj, q0, Rni, rt, scal_f);
+      fresh = true;
     }
 #pragma unroll
     for (int i = 0; i < NV; i++) qj[i] = qrel[i] * qorg[i];
@@ -927,9 +952,9 @@ __device__ int solve_rk_step(const Params& P, double dt, double* qj, const doubl
 }
 
 // stage 2 + assembly for one cell; q holds q0 on entry, the new state on exit
-template <int NS>
+template <int NS, int LANES>
 __device__ void stage2(const Params& P, double dt, double* q, const double* X,
-                       unsigned int& err) {
+                       unsigned int& err, int lane) {
   constexpr int NV = 4 + NS, NP = NV + 2;
   double q0[NV], qfv[NV], qp[NP], q_si[NV], qg[NV], si[NV], Rj[NV], pd;
 #pragma unroll
@@ -972,7 +997,7 @@ __device__ void stage2(const Params& P, double dt, double* q, const double* X,
       const double B = 0.0 + (0.0 + 0.0) * 0.0;
       Rj[i] = (A - B) - 1.0 * si[i];
     }
-    solve_rk_step<NS>(P, dt, qg, q0, Rj, err);
+    solve_rk_step<NS, LANES>(P, dt, qg, q0, Rj, err, lane);
     if (q_si[1] * q_si[1] + q_si[2] * q_si[2] == 0.0) {
       qg[1] = 0.0;
       qg[2] = 0.0;
@@ -1039,9 +1064,9 @@ __device__ void stage2(const Params& P, double dt, double* q, const double* X,
 }
 
 // update_erosion_deposition_cell for one cell: src/solver_2d.f90:2429-2593
-template <int NS>
+template <int NS, int LANES>
 __device__ void erosion_deposition(const Params& P, double dt, double* q, double* depo,
-                                   unsigned int& err) {
+                                   unsigned int& err, int lane) {
   constexpr int NV = 4 + NS, NP = NV + 2;
   double qp[NP], pd, dep[NS], eq[NV];
   if (q[0] > 0.0) {
@@ -1051,7 +1076,7 @@ __device__ void erosion_deposition(const Params& P, double dt, double* q, double
     for (int i = 0; i < NP; i++) qp[i] = 0.0;
     qp[3] = P.T_amb;
   }
-  if (mass_exchange<NS>(P, qp, dt, dep, eq)) err |= IMEX_ERR_NEG_DEP;
+  if (mass_exchange<NS, LANES>(P, qp, dt, dep, eq, lane)) err |= IMEX_ERR_NEG_DEP;
 #pragma unroll
   for (int i = 0; i < NV; i++) q[i] = q[i] + dt * eq[i];
 #pragma unroll
@@ -1138,12 +1163,20 @@ template <int NS, int PARTS>
 __global__ __launch_bounds__(NTHR) void k_update(Params P, Grid G, double dt_arg, int mode) {
   constexpr int NV = 4 + NS;
   constexpr bool ALL = (PARTS & 8) != 0;
+  // lanes per cell: the stage-2 launch evaluates the two Jacobian columns of
+  // the Newton solve on an aligned lane pair, the sedimentation launch splits
+  // the solid classes' settling velocities over a pair; the pair's lead lane
+  // writes the results
+  constexpr int LANES = (!ALL && ((PARTS & 1) || ((PARTS & 2) && NS > 1))) ? 2 : 1;
+  const int lane = (LANES == 2) ? (int)(threadIdx.x & 1u) : 0;
+  const bool lead = lane == 0;
   const int N = G.N;
   const double dt = (mode == 1) ? G.st->dt : dt_arg;
   const int n = ALL ? N : (int)G.st->n_work;
   unsigned int err = 0;
   int dried = 0;
-  for (int i = blockIdx.x * NTHR + (int)threadIdx.x; i < n; i += gridDim.x * NTHR) {
+  const int stride = (int)(gridDim.x * NTHR) / LANES;
+  for (int i = (int)(blockIdx.x * NTHR + threadIdx.x) / LANES; i < n; i += stride) {
     const int c = ALL ? i : G.work[i];
     double q[NV];
 #pragma unroll
@@ -1152,40 +1185,46 @@ __global__ __launch_bounds__(NTHR) void k_update(Params P, Grid G, double dt_arg
       // remember "wet at the start of the step" in bit 1 of the mask byte for
       // the dried-cell count of the sedimentation launch
       const bool wet0 = q[0] > 0.0;
-      G.mask[c] = (unsigned char)(1u | (wet0 ? 2u : 0u));
+      if (lead) G.mask[c] = (unsigned char)(1u | (wet0 ? 2u : 0u));
       double X[NV];
 #pragma unroll
       for (int v = 0; v < NV; v++) X[v] = G.R[(size_t)v * N + c];
-      stage2<NS>(P, dt, q, X, err);
+      stage2<NS, LANES>(P, dt, q, X, err, lane);
     }
     if constexpr ((PARTS & 2) != 0) {
       if (P.settling || P.entrainment) {
         double depo[NS];
 #pragma unroll
         for (int v = 0; v < NS; v++) depo[v] = G.dep[(size_t)v * N + c];
-        erosion_deposition<NS>(P, dt, q, depo, err);
+        erosion_deposition<NS, LANES>(P, dt, q, depo, err, lane);
+        if (lead) {
 #pragma unroll
-        for (int v = 0; v < NS; v++) G.dep[(size_t)v * N + c] = depo[v];
+          for (int v = 0; v < NS; v++) G.dep[(size_t)v * N + c] = depo[v];
+        }
       }
       // wet at the step start (flag left by the stage-2 launch), exactly dry
       // after the step's last change of q
-      if ((G.mask[c] & 2u) && q[0] == 0.0) dried++;
+      if (lead && (G.mask[c] & 2u) && q[0] == 0.0) dried++;
     }
     if constexpr ((PARTS & 3) != 0) {
+      if (lead) {
 #pragma unroll
-      for (int v = 0; v < NV; v++) G.q[(size_t)v * N + c] = q[v];
+        for (int v = 0; v < NV; v++) G.q[(size_t)v * N + c] = q[v];
+      }
     }
     if constexpr ((PARTS & 12) != 0) {
-      double hm = G.hmax[c], pm = G.pdyn[c], mv = G.mvel[c];
-      unsigned long long vb = G.vuln ? G.vuln[c] : 0ULL;
-      epilogue<NS>(P, q, hm, pm, mv, vb, ALL);
-      G.hmax[c] = hm;
-      G.pdyn[c] = pm;
-      G.mvel[c] = mv;
-      if (G.vuln) G.vuln[c] = vb;
+      if (lead) {
+        double hm = G.hmax[c], pm = G.pdyn[c], mv = G.mvel[c];
+        unsigned long long vb = G.vuln ? G.vuln[c] : 0ULL;
+        epilogue<NS>(P, q, hm, pm, mv, vb, ALL);
+        G.hmax[c] = hm;
+        G.pdyn[c] = pm;
+        G.mvel[c] = mv;
+        if (G.vuln) G.vuln[c] = vb;
+      }
     }
   }
-  if (err) atomicOr(&G.st->err, err);
+  if (err && lead) atomicOr(&G.st->err, err);
   if (dried) atomicAdd(&G.st->dried, (unsigned long long)dried);
   if ((PARTS & 4) && mode == 1 && blockIdx.x == 0 && threadIdx.x == 0) G.st->t = G.st->t + dt;
 }
@@ -1318,7 +1357,7 @@ __global__ void kt_newton(Params P, int n, double dt, const double* qsi, const d
     double qq[NV];
     for (int m = 0; m < NV; m++) qq[m] = qsi[NV * i + m];
     unsigned int err = 0;
-    its[i] = solve_rk_step<NS>(P, dt, qq, q0 + (size_t)NV * i, R + (size_t)NV * i, err);
+    its[i] = solve_rk_step<NS, 1>(P, dt, qq, q0 + (size_t)NV * i, R + (size_t)NV * i, err, 0);
     for (int m = 0; m < NV; m++) q[NV * i + m] = qq[m];
   }
 }

@@ -581,9 +581,13 @@ __device__ inline double settling_velocity(const Params& P, double diam, double 
 // eval_mass_exchange_terms without erosion / pore pressure / vaporisation:
 // src/constitutive_2d.f90:4574-5153.  Returns true when the reference would
 // pause on a negative deposition rate.
-template <int NS>
+// LANES = 2: the two lanes of an aligned pair evaluate the same cell and
+// must call this together; lane `lane` computes the settling velocities of
+// the classes i with i % 2 == lane and the pair swaps them, so the slowest
+// part (the drag fixed point) runs on both lanes at once.
+template <int NS, int LANES = 1>
 __device__ inline bool mass_exchange(const Params& P, const double* qp, double dt, double* dep,
-                                     double* eq) {
+                                     double* eq, int lane = 0) {
   constexpr int NV = 4 + NS;
   double ers[NS];
 #pragma unroll
@@ -614,12 +618,31 @@ __device__ inline bool mass_exchange(const Params& P, const double* qp, double d
   Mixt m = mixt_var<NS>(P, qp, true);
   double ikv = 1.0 / P.kin_visc_c;
   bool err = false;
+  // settling velocities (pure functions of the class and rho_c) and the
+  // hindered-settling factor, the same for every class of the cell
+  double sv[NS];
+  bool any = false;
+#pragma unroll
+  for (int i = 0; i < NS; i++) {
+    sv[i] = 0.0;
+    const bool settles = (alphas[i] > 0.0) && P.settling;
+    any = any || settles;
+    if (settles && (LANES == 1 || (i & 1) == lane))
+      sv[i] = settling_velocity(P, P.diam_s[i], P.rho_s[i], m.rho_c, ikv);
+  }
+  if constexpr (LANES == 2) {
+#pragma unroll
+    for (int i = 0; i < NS; i++) {
+      const double o = __shfl_xor(sv[i], 1);
+      if ((i & 1) != lane) sv[i] = o;
+    }
+  }
+  const double hindered = any ? pow_cr(1.0 - fmin_f(1.0, at / 0.6), 4.65) : 0.0;
 #pragma unroll
   for (int i = 0; i < NS; i++) {
     if ((alphas[i] > 0.0) && P.settling) {
-      double sv = settling_velocity(P, P.diam_s[i], P.rho_s[i], m.rho_c, ikv);
-      dep[i] = alphas[i] * sv * 1.0;
-      dep[i] = dep[i] * pow_cr(1.0 - fmin_f(1.0, at / 0.6), 4.65);
+      dep[i] = alphas[i] * sv[i] * 1.0;
+      dep[i] = dep[i] * hindered;
       dep[i] = fmin_f(dep[i], h * alphas[i] / dt);
       if (dep[i] < 0.0) err = true;
     }
