@@ -22,15 +22,18 @@ using namespace imex;
 // ===========================================================================
 // active set: check_solve, src/solver_2d.f90:667-858
 // ===========================================================================
-__global__ __launch_bounds__(NTHR) void k_mask(Params P, Grid G, int solve_all, int par) {
+// One 32x8 tile: seeds (frame, wet, source ring) of the tile and its two-cell
+// halo in LDS, the diamond dilation of radius 2, the mask bytes, and the
+// lists: the tile goes to k_flux's list when it is active and "hot" (a wet
+// or ring cell within two cells), and a hot tile makes itself and its eight
+// neighbours candidates of the next step's k_mask.
+__device__ __forceinline__ void mask_tile(const Params& P, const Grid& G, int tile, int solve_all,
+                                          int par, unsigned int seq) {
   __shared__ unsigned char seed[RY][RX];
-  __shared__ int s_any;
-  __shared__ int s_cnt;
-  __shared__ int s_hot;
-  const int nb = gridDim.x;
-  const int tile = xcd_tile(blockIdx.x, nb);
+  __shared__ int s_any, s_cnt, s_hot;
   const int j0 = (tile % G.ntx) * TX, k0 = (tile / G.ntx) * TY;
   const int tid = threadIdx.x;
+  __syncthreads();  // the previous tile's use of the shared flags is over
   if (tid == 0) {
     s_any = 0;
     s_cnt = 0;
@@ -77,19 +80,57 @@ __global__ __launch_bounds__(NTHR) void k_mask(Params P, Grid G, int solve_all, 
   }
   __syncthreads();
   if (tid == 0) {
-    if (blockIdx.x == 0) {
-      G.st->n_tiles[par ^ 1] = 0u;  // the next step's slot
-      G.st->n_work = 0u;            // filled by the k_flux that follows
-    }
+    DevState* st = G.st;
     // k_flux visits the tiles with active cells that have a wet or
     // source-ring cell within two cells: elsewhere every face stencil is dry,
     // so every flux and wave speed is zero, and the (dry, all-zero) cells get
     // a zero right-hand side that k_update does not need
     if (s_any && s_hot) {
-      const unsigned int slot = atomicAdd(&G.st->n_tiles[par], 1u);
+      const unsigned int slot = atomicAdd(&st->n_tiles[par], 1u);
       G.tile_list[(size_t)par * G.ntx * G.nty + slot] = tile;
     }
-    if (s_cnt) atomicAdd(&G.st->active, (unsigned long long)s_cnt);
+    if (s_hot) {
+      const int tx = tile % G.ntx, ty = tile / G.ntx;
+      const unsigned int nslot = (seq + 1u) % 3u;
+      for (int dy = -1; dy <= 1; dy++)
+        for (int dx = -1; dx <= 1; dx++) {
+          const int ux = tx + dx, uy = ty + dy;
+          if (ux < 0 || ux >= G.ntx || uy < 0 || uy >= G.nty) continue;
+          const int u = ux + G.ntx * uy;
+          if (atomicExch(&G.cand_stamp[u], seq + 1u) != seq + 1u) {
+            const unsigned int slot = atomicAdd(&st->n_cand[nslot], 1u);
+            G.cand[(size_t)nslot * G.ntx * G.nty + slot] = u;
+          }
+        }
+    }
+    // active cells of the tile beyond its frame band (counted by block 0)
+    const long long extra = (long long)s_cnt - (long long)G.tile_static[tile];
+    if (extra) atomicAdd(&st->active, (unsigned long long)extra);
+  }
+}
+
+// full: every tile (the first pass after an upload, and after a pass with
+// solve_all); otherwise the candidate tiles listed by the previous pass.  A
+// tile outside the candidates had no wet or ring cell within ten cells at the
+// previous step, so none within two now (fluxes move mass two cells at most
+// per step): its mask bytes are the frame band's and stay as stored.
+__global__ __launch_bounds__(NTHR) void k_mask(Params P, Grid G, int solve_all, int par,
+                                               unsigned int seq, int full) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    // slots no block of this launch touches
+    G.st->n_tiles[par ^ 1] = 0u;            // the next step's k_flux list
+    G.st->n_work = 0u;                      // filled by the k_flux that follows
+    G.st->n_cand[(seq + 2u) % 3u] = 0u;     // the list the next pass fills
+    // the frame band of every tile; each visited tile adds what it has beyond
+    atomicAdd(&G.st->active, G.static_active);
+  }
+  if (full) {
+    mask_tile(P, G, xcd_tile(blockIdx.x, gridDim.x), solve_all, par, seq);
+  } else {
+    const unsigned int slot = seq % 3u;
+    const int n = (int)G.st->n_cand[slot];
+    const int* list = G.cand + (size_t)slot * G.ntx * G.nty;
+    for (int it = blockIdx.x; it < n; it += gridDim.x) mask_tile(P, G, list[it], solve_all, par, seq);
   }
 }
 
@@ -668,25 +709,23 @@ __global__ __launch_bounds__(NTHR) void k_flux(Params P, Grid G, int mode, doubl
   }
 }
 
-// The CFL time step from the blocks' minimum, then (mode 1) the driver's clip
-// to t_end and growth limit, src/IMEX_SfloW2D.f90:417-439.  One thread; runs
+// The CFL time step from the blocks' minimum (timestep's result,
+// src/solver_2d.f90:873-1006), for the host-driven step.  One thread; runs
 // after k_flux, so the kernel boundary orders it after every block's atomic.
-__global__ void k_dt(Params P, Grid G, int mode, int step_idx) {
+__global__ void k_dt(Params P, Grid G) {
   DevState* st = G.st;
-  const double dtmin = __longlong_as_double((long long)st->dtmin_bits);
-  double dt = P.max_dt;
-  dt = fmin_f(dt, dtmin);
-  st->dt_cfl = dt;
-  if (mode == 1) {
-    const double t = st->t;
-    if (t + dt > st->t_end) dt = st->t_end - t;
-    dt = fmin_f(dt, 1.1 * 0.5 * (st->dt_old + st->dt_old_old));
-    st->dt_old_old = st->dt_old;
-    st->dt_old = dt;
-    st->dt = dt;
-    if (G.dts) G.dts[step_idx] = dt;
-  }
+  st->dt_cfl = fmin_f(P.max_dt, __longlong_as_double((long long)st->dtmin_bits));
   st->dtmin_bits = 0x7ff0000000000000ULL;
+}
+
+// The device-side driver's dt: the CFL step clipped to t_end and limited to
+// 1.1 * (dt_old + dt_old_old) / 2, src/IMEX_SfloW2D.f90:417-439.  Reads only
+// fields no kernel of the step writes before the sedimentation launch.
+__device__ __forceinline__ double driver_dt(const Params& P, const DevState* st) {
+  double dt = fmin_f(P.max_dt, __longlong_as_double((long long)st->dtmin_bits));
+  const double t = st->t;
+  if (t + dt > st->t_end) dt = st->t_end - t;
+  return fmin_f(dt, 1.1 * 0.5 * (st->dt_old + st->dt_old_old));
 }
 
 // ===========================================================================
@@ -1159,8 +1198,12 @@ __device__ void epilogue(const Params& P, const double* q, double& hmax, double&
 // epilogue advances t (device-side driver).  Each PARTS is its own kernel so
 // that the register budget of the Newton solve does not bound the occupancy
 // of the sedimentation and epilogue.
+// mode 1 (device-side driver): the stage-2 launch computes the step's dt
+// (every thread the same, driver_dt) and publishes it in st->dt; the launch
+// with the epilogue ends the step: dt history, t += dt, CFL minimum reset.
 template <int NS, int PARTS>
-__global__ __launch_bounds__(NTHR) void k_update(Params P, Grid G, double dt_arg, int mode) {
+__global__ __launch_bounds__(NTHR) void k_update(Params P, Grid G, double dt_arg, int mode,
+                                                 int step_idx) {
   constexpr int NV = 4 + NS;
   constexpr bool ALL = (PARTS & 8) != 0;
   // lanes per cell: the stage-2 launch evaluates the two Jacobian columns of
@@ -1171,7 +1214,19 @@ __global__ __launch_bounds__(NTHR) void k_update(Params P, Grid G, double dt_arg
   const int lane = (LANES == 2) ? (int)(threadIdx.x & 1u) : 0;
   const bool lead = lane == 0;
   const int N = G.N;
-  const double dt = (mode == 1) ? G.st->dt : dt_arg;
+  double dt = dt_arg;
+  if (mode == 1) {
+    if constexpr ((PARTS & 1) != 0) {
+      dt = driver_dt(P, G.st);
+      if (blockIdx.x == 0 && threadIdx.x == 0) {
+        G.st->dt = dt;
+        G.st->dt_cfl = fmin_f(P.max_dt, __longlong_as_double((long long)G.st->dtmin_bits));
+        if (G.dts) G.dts[step_idx] = dt;
+      }
+    } else {
+      dt = G.st->dt;
+    }
+  }
   const int n = ALL ? N : (int)G.st->n_work;
   unsigned int err = 0;
   int dried = 0;
@@ -1226,7 +1281,13 @@ __global__ __launch_bounds__(NTHR) void k_update(Params P, Grid G, double dt_arg
   }
   if (err && lead) atomicOr(&G.st->err, err);
   if (dried) atomicAdd(&G.st->dried, (unsigned long long)dried);
-  if ((PARTS & 4) && mode == 1 && blockIdx.x == 0 && threadIdx.x == 0) G.st->t = G.st->t + dt;
+  if ((PARTS & 4) && mode == 1 && blockIdx.x == 0 && threadIdx.x == 0) {
+    DevState* st = G.st;
+    st->dt_old_old = st->dt_old;
+    st->dt_old = dt;
+    st->t = st->t + dt;
+    st->dtmin_bits = 0x7ff0000000000000ULL;
+  }
 }
 
 // ===========================================================================
@@ -1388,6 +1449,8 @@ struct imex_handle {
   int device = 0;
   int ntiles = 0;
   int par = 0;  // slot of the active-tile list filled by the latest k_mask
+  unsigned int mask_seq = 0;  // k_mask launches so far (candidate-list slots)
+  bool mask_full_next = true;  // the next k_mask visits every tile
   size_t lds = 0;
   std::vector<void*> allocs;
   double* d_tmp = nullptr;  // N*nv scratch for layout conversion
@@ -1429,6 +1492,29 @@ static size_t lds_for(int ns) {
     case 2: return flux_lds<2>();
     default: return flux_lds<3>();
   }
+}
+
+// Active cells of each tile when no cell is wet: the frame and the interior
+// cells within Manhattan distance 2 of it (the closest frame cell is straight
+// along an axis), minus source-disc cells.  k_mask adds the total once per
+// pass and each visited tile its cells beyond these.
+static int frame_band(imex_handle* h, const signed char* scell) {
+  const Grid& G = h->G;
+  std::vector<int> cnt((size_t)h->ntiles, 0);
+  unsigned long long total = 0;
+  for (int k = 0; k < G.ny; k++)
+    for (int j = 0; j < G.nx; j++) {
+      const bool band = j <= 2 || j >= G.nx - 3 || k <= 2 || k >= G.ny - 3;
+      if (!band) continue;
+      if (h->P.radial && scell && scell[j + (size_t)G.nx * k] == 1) continue;
+      cnt[(size_t)(j / TX) + (size_t)G.ntx * (k / TY)]++;
+      total++;
+    }
+  HCHK(h, hipMemcpyAsync(h->G.tile_static, cnt.data(), 4 * cnt.size(), hipMemcpyHostToDevice,
+                         h->stream));
+  HCHK(h, hipStreamSynchronize(h->stream));
+  h->G.static_active = total;
+  return 0;
 }
 
 extern "C" int imex_create(const imex_config* cfg, imex_handle** out) {
@@ -1537,6 +1623,9 @@ extern "C" int imex_create(const imex_config* cfg, imex_handle** out) {
   if ((rc = dalloc(h, &G.mask, N))) return rc;
   if ((rc = dalloc(h, &G.tile_list, 2 * (size_t)h->ntiles))) return rc;
   if ((rc = dalloc(h, &G.work, N))) return rc;
+  if ((rc = dalloc(h, &G.cand, 3 * (size_t)h->ntiles))) return rc;
+  if ((rc = dalloc(h, &G.cand_stamp, (size_t)h->ntiles))) return rc;
+  if ((rc = dalloc(h, &G.tile_static, (size_t)h->ntiles))) return rc;
   double *bx, *by, *ap, *ax, *ay, *xc, *xs, *yc, *ys;
   signed char* sc;
   unsigned char* sd;
@@ -1576,6 +1665,7 @@ extern "C" int imex_create(const imex_config* cfg, imex_handle** out) {
     HCHK(h, hipMemcpyAsync(st, &s0, sizeof(s0), hipMemcpyHostToDevice, h->stream));
     HCHK(h, hipStreamSynchronize(h->stream));
   }
+  if ((rc = frame_band(h, nullptr))) return rc;
   h->lds = lds_for(ns);
   switch (ns) {
     case 1: HCHK(h, hipFuncSetAttribute((const void*)k_flux<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds)); break;
@@ -1605,7 +1695,9 @@ extern "C" int imex_set_static(imex_handle* h, const imex_static* s) {
     HCHK(h, hipMemcpyAsync((void*)h->G.arcy, s->arc_n_y, 8 * N, hipMemcpyHostToDevice, h->stream));
   HCHK(h, hipMemcpyAsync((void*)h->G.scell, sc.data(), N, hipMemcpyHostToDevice, h->stream));
   HCHK(h, hipMemcpyAsync((void*)h->G.sides, sd.data(), N, hipMemcpyHostToDevice, h->stream));
+  if (int rc = frame_band(h, sc.data())) return rc;
   HCHK(h, hipStreamSynchronize(h->stream));
+  h->mask_full_next = true;
   return 0;
 }
 
@@ -1616,26 +1708,29 @@ static int grid_for(const imex_handle* h, int per_cu) {
 }
 
 template <int NS>
-static void launch_update(imex_handle* h, int parts, double dt, int mode) {
+static void launch_update(imex_handle* h, int parts, double dt, int mode, int step_idx) {
   const int nb = (parts & 8) ? (h->G.N + NTHR - 1) / NTHR : grid_for(h, 4);
   const dim3 g(nb), b(NTHR);
+  const Params& P = h->P;
+  const Grid& G = h->G;
+  hipStream_t s = h->stream;
   switch (parts) {
-    case 1: hipLaunchKernelGGL((k_update<NS, 1>), g, b, 0, h->stream, h->P, h->G, dt, mode); break;
-    case 2: hipLaunchKernelGGL((k_update<NS, 2>), g, b, 0, h->stream, h->P, h->G, dt, mode); break;
-    case 4: hipLaunchKernelGGL((k_update<NS, 4>), g, b, 0, h->stream, h->P, h->G, dt, mode); break;
-    case 6: hipLaunchKernelGGL((k_update<NS, 6>), g, b, 0, h->stream, h->P, h->G, dt, mode); break;
-    default: hipLaunchKernelGGL((k_update<NS, 8>), g, b, 0, h->stream, h->P, h->G, dt, mode); break;
+    case 1: hipLaunchKernelGGL((k_update<NS, 1>), g, b, 0, s, P, G, dt, mode, step_idx); break;
+    case 2: hipLaunchKernelGGL((k_update<NS, 2>), g, b, 0, s, P, G, dt, mode, step_idx); break;
+    case 4: hipLaunchKernelGGL((k_update<NS, 4>), g, b, 0, s, P, G, dt, mode, step_idx); break;
+    case 6: hipLaunchKernelGGL((k_update<NS, 6>), g, b, 0, s, P, G, dt, mode, step_idx); break;
+    default: hipLaunchKernelGGL((k_update<NS, 8>), g, b, 0, s, P, G, dt, mode, step_idx); break;
   }
 }
 // parts: 1, 2, 4, 8 as the host-driven entry points call them, or 1|2|4 for
-// a device-driven step, which runs as two launches (1, then 2|4).
-static int update(imex_handle* h, int parts, double dt, int mode) {
+// a device-driven step (mode 1), which runs as two launches (1, then 2|4).
+static int update(imex_handle* h, int parts, double dt, int mode, int step_idx = 0) {
   const int seq[2] = {(parts == 7) ? 1 : parts, (parts == 7) ? 6 : 0};
   for (int k = 0; k < 2 && seq[k]; k++) {
     switch (h->P.ns) {
-      case 1: launch_update<1>(h, seq[k], dt, mode); break;
-      case 2: launch_update<2>(h, seq[k], dt, mode); break;
-      default: launch_update<3>(h, seq[k], dt, mode); break;
+      case 1: launch_update<1>(h, seq[k], dt, mode, step_idx); break;
+      case 2: launch_update<2>(h, seq[k], dt, mode, step_idx); break;
+      default: launch_update<3>(h, seq[k], dt, mode, step_idx); break;
     }
     HCHK(h, hipGetLastError());
   }
@@ -1643,8 +1738,12 @@ static int update(imex_handle* h, int parts, double dt, int mode) {
 }
 static int mask(imex_handle* h, int solve_all) {
   h->par ^= 1;
-  hipLaunchKernelGGL(k_mask, dim3(h->ntiles), dim3(NTHR), 0, h->stream, h->P, h->G, solve_all,
-                     h->par);
+  h->mask_seq++;
+  const int full = (solve_all || h->mask_full_next) ? 1 : 0;
+  h->mask_full_next = solve_all != 0;  // the pass after solve_all redoes every tile
+  const int nb = full ? h->ntiles : grid_for(h, 4);
+  hipLaunchKernelGGL(k_mask, dim3(nb), dim3(NTHR), 0, h->stream, h->P, h->G, solve_all, h->par,
+                     h->mask_seq, full);
   HCHK(h, hipGetLastError());
   return 0;
 }
@@ -1656,8 +1755,11 @@ static int flux(imex_handle* h, int mode, double t, int step_idx) {
     default: hipLaunchKernelGGL(k_flux<3>, dim3(nb), dim3(NTHR), h->lds, h->stream, h->P, h->G, mode, t, h->par); break;
   }
   HCHK(h, hipGetLastError());
-  hipLaunchKernelGGL(k_dt, dim3(1), dim3(1), 0, h->stream, h->P, h->G, mode, step_idx);
-  HCHK(h, hipGetLastError());
+  (void)step_idx;
+  if (mode == 0) {  // the device-side driver takes its dt in k_update instead
+    hipLaunchKernelGGL(k_dt, dim3(1), dim3(1), 0, h->stream, h->P, h->G);
+    HCHK(h, hipGetLastError());
+  }
   return 0;
 }
 
@@ -1809,7 +1911,7 @@ extern "C" int imex_run_steps(imex_handle* h, int nsteps, double t_end, double* 
   for (int n = 0; n < nsteps; n++) {
     if ((rc = mask(h, 0))) return rc;
     if ((rc = flux(h, 1, 0.0, n))) return rc;
-    if ((rc = update(h, 1 | 2 | 4, 0.0, 1))) return rc;
+    if ((rc = update(h, 1 | 2 | 4, 0.0, 1, n))) return rc;
   }
   if (dts && nsteps > 0)
     HCHK(h, hipMemcpyAsync(dts, h->G.dts, 8 * (size_t)nsteps, hipMemcpyDeviceToHost, h->stream));
@@ -1844,7 +1946,7 @@ extern "C" int imex_run_steps_timed(imex_handle* h, int nsteps, double t_end, do
     HCHK(h, hipEventRecord(ev[4 * n + 1], h->stream));
     if ((rc = flux(h, 1, 0.0, n))) break;
     HCHK(h, hipEventRecord(ev[4 * n + 2], h->stream));
-    if ((rc = update(h, 1 | 2 | 4, 0.0, 1))) break;
+    if ((rc = update(h, 1 | 2 | 4, 0.0, 1, n))) break;
     HCHK(h, hipEventRecord(ev[4 * n + 3], h->stream));
   }
   HCHK(h, hipStreamSynchronize(h->stream));

@@ -4,24 +4,29 @@
 // driver loop):
 //   k_mask    active set (check_solve): seeds = wet cells, ring cells and the
 //             boundary frame, dilated by the diamond of radius n_RK = 2; one
-//             byte per cell, and the list of tiles holding active cells.
-//   k_flux    persistent blocks over the active-tile list; per TX x TY tile
-//             with a two-cell halo in LDS: primitive states -> MUSCL/minmod
-//             face states -> local speeds -> Kurganov-Tadmor face fluxes ->
-//             flux divergence minus explicit terms (the stage-1 right-hand
-//             side, stored per active cell), and the CFL time-step candidates
-//             reduced per block into one atomic minimum.
-//   k_dt      one thread: the CFL time step and the driver's clip and growth
-//             limit (device-side dt).
-//   k_update  persistent blocks over the active-tile list, one thread per
-//             cell, state in registers: stage-2 semi-implicit Coulomb
-//             friction, Newton solve of the turbulent friction (complex-step
-//             Jacobian, 2x2 Cramer, backtracking line search), assembly
-//             checks, sedimentation / entrainment, and the driver epilogue
-//             (running maxima, vulnerability table).
-// The active-tile list is double-buffered: the k_mask of a step fills slot
-// `par` and clears the other slot's counter for the next step, so no kernel
-// waits on another through device-scope fences.
+//             byte per cell.  Visits the candidate tiles the previous pass
+//             listed (hot tiles and their neighbours) -- every other tile
+//             holds only its frame band -- and lists the "hot" active tiles
+//             (a wet or source-ring cell within two cells) for k_flux.
+//   k_flux    persistent blocks over the hot-tile list; per TX x TY tile with
+//             a two-cell halo in LDS: primitive states -> MUSCL/minmod face
+//             states -> local speeds -> Kurganov-Tadmor face fluxes -> flux
+//             divergence minus explicit terms (the stage-1 right-hand side),
+//             the CFL candidates reduced per block into one atomic minimum,
+//             and the list of work cells (active cells whose state or
+//             right-hand side is not all zero).
+//   k_dt      one thread, host-driven steps only: the CFL time step.
+//   k_update  persistent blocks over the work cells, two launches per step:
+//             <1> stage-2 semi-implicit Coulomb friction, Newton solve of the
+//             turbulent friction (complex-step Jacobian columns on a lane
+//             pair, 2x2 Cramer, backtracking line search), assembly checks;
+//             in the device-side driver also the step's dt (clip to t_end,
+//             growth limit);  <6> sedimentation / entrainment (settling
+//             classes on a lane pair) and the driver epilogue (running
+//             maxima, vulnerability table), then t += dt.
+// Lists are double- or triple-buffered by pass parity so that no kernel waits
+// on another through device-scope fences: each launch only clears counters of
+// slots no block of that launch reads or fills.
 #pragma once
 
 #include "imex_physics.h"
@@ -42,6 +47,8 @@ struct DevState {
   unsigned int n_tiles[2];  // active tiles listed by k_mask, per list slot
   unsigned int err;
   unsigned int n_work;      // cells k_flux listed for k_update
+  unsigned int n_cand[3];   // candidate tiles of k_mask passes, three slots
+  unsigned int pad;
   unsigned long long dried;
   unsigned long long active;
 };
@@ -57,6 +64,10 @@ struct Grid {
   unsigned char* mask;       // [N]
   int* tile_list;            // [2][ntiles] active tiles, two slots
   int* work;                 // [N] active cells with a non-zero state or right-hand side
+  int* cand;                 // [3][ntiles] tiles the next k_mask pass visits
+  unsigned int* cand_stamp;  // [ntiles] pass that last listed the tile
+  int* tile_static;          // [ntiles] frame-band active cells of the tile
+  unsigned long long static_active;  // their sum
   const double* bx;          // [N] B_prime_x_geom
   const double* by;          // [N] B_prime_y_geom
   const signed char* scell;  // [N] source_cell
