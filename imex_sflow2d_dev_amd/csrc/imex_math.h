@@ -404,6 +404,80 @@ __host__ __device__ __forceinline__ double exp_dd(dd z) {
   return ldexp(f, -1074);
 }
 
+// The double-double x**y above, kept out of line: pow_fast settles all but a
+// small fraction of the calls.
+__host__ __device__ __noinline__ inline double pow_slow(double x, double y) {
+  return exp_dd(mul_d(log_dd(x), y));
+}
+
+// ln2 = LN2H + LN2L with LN2H of 42 bits (e * LN2H is exact for |e| <= 2048);
+// ln2/128 = LN2_128H + LN2_128L with LN2_128H of 36 bits (n * LN2_128H is
+// exact for |n| < 2**17)
+constexpr double LN2H = 0x1.62e42fefa3800p-1;
+constexpr double LN2L = 0x1.ef35793c76730p-45;
+constexpr double LN2_128H = 0x1.62e42fefa0000p-8;
+constexpr double LN2_128L = 0x1.cf79abc9e3b3ap-47;
+
+// Ziv's strategy, first step: x**y to a relative error below
+// eps = (|y log x| + |y|/128 + 2) 2**-67 with the same tables and mostly
+// plain double arithmetic (error-free sums and products only where the
+// leading terms need them).  Returns false unless the result rounds the same
+// way at both ends of the error interval; the caller then takes pow_slow.
+//   log x:  2 atanh(s) = 2 s + 2 s**3/3 + 2 s**5/5 + 2 s**7/7 with s as an
+//           unevaluated pair sh + sl (|s| < 0.0028, truncation < 2**-71 |2s|);
+//   exp r:  r + r**2/2 + ... + r**6/720 (|r| < 0.0028, truncation < 2**-72).
+__host__ __device__ __forceinline__ bool pow_fast(double x, double y, double& out) {
+  int e = 0;
+  double m = frexp(x, &e);
+  if (m < SQRT_HALF) {
+    m = 2.0 * m;
+    e -= 1;
+  }
+  const int k = (int)rint((m - 1.0) * 128.0);
+  const double c = 1.0 + k * 0x1.0p-7;
+  const double num = m - c;  // exact
+  const dd den = two_sum(m, c);
+  const double rden = 1.0 / den.hi;
+  const double sh = num * rden;
+  const double sl = (fma(-sh, den.hi, num) - sh * den.lo) * rden;
+  const double t = sh * sh;
+  const double poly = sh * t * (0x1.5555555555555p-1 + t * (0x1.999999999999ap-2 + t * 0x1.2492492492492p-2));
+  const dd lc = log_table(k);
+  const double ed = (double)e;
+  const dd u = two_sum(ed * LN2H, lc.hi);
+  const dd v = two_sum(u.hi, 2.0 * sh);
+  const double lo = v.lo + u.lo + (ed * LN2L + lc.lo + 2.0 * sl + poly);
+  const dd l = fast_two_sum(v.hi, lo);
+  dd z = two_prod(y, l.hi);
+  z = fast_two_sum(z.hi, z.lo + y * l.lo);
+  if (!(fabs(z.hi) < 700.0)) return false;
+  const double nd = rint(z.hi * 0x1.71547652b82fep+7);  // z 128 / ln2
+  const int n = (int)nd;
+  const int j = n & 127;
+  const int e2 = (n - j) / 128;
+  // r = z - n ln2/128 as the pair r1 + r2 (|r2| <= ulp(r1)/2)
+  const dd r = two_sum(z.hi - nd * LN2_128H, z.lo - nd * LN2_128L);  // first term exact
+  const double r1 = r.hi;
+  // exp(r1 + r2) - 1 - r1 = r2 + r1**2/2 + ... + r1**6/720 (r1 r2 < 2**-106)
+  const double rest =
+      r.lo + (r1 * r1) * (0.5 + r1 * (0x1.5555555555555p-3 + r1 * (0x1.5555555555555p-5 +
+                                      r1 * (0x1.1111111111111p-7 + r1 * 0x1.6c16c16c16c17p-10))));
+  const dd T = exp2_table(j);
+  const dd mp = two_prod(T.hi, r1);
+  const dd sm = two_sum(T.hi, mp.hi);
+  const double corr = sm.lo + mp.lo + T.lo + T.lo * r1 + T.hi * rest;
+  const dd R = fast_two_sum(sm.hi, corr);
+  const double eps = (fabs(z.hi) + fabs(y) * 0x1.0p-7 + 2.0) * 0x1.0p-67;
+  int ex = 0;
+  const double fr = frexp(R.hi, &ex);  // R.hi = fr 2**ex, |fr| in [0.5, 1)
+  double half_ulp = ldexp(1.0, ex - 54);
+  if (fabs(fr) == 0.5) half_ulp *= 0.5;  // a power of two: the gap below is halved
+  if (!(fabs(R.lo) + eps * fabs(R.hi) < 0.9999 * half_ulp)) return false;
+  if (ex + e2 < -1020 || ex + e2 > 1023) return false;  // leave the range ends to pow_slow
+  out = ldexp(R.hi, e2);
+  return true;
+}
+
 }  // namespace ddm
 
 // x**y correctly rounded; the special cases follow C's pow
@@ -411,7 +485,9 @@ __host__ __device__ __forceinline__ double pow_cr(double x, double y) {
   if (y == 0.0 || x == 1.0) return 1.0;
   if (x == 0.0) return (y > 0.0) ? 0.0 : HUGE_VAL;
   if (!(x > 0.0) || !(x < HUGE_VAL) || !(fabs(y) < HUGE_VAL)) return pow(x, y);
-  return ddm::exp_dd(ddm::mul_d(ddm::log_dd(x), y));
+  double r;
+  if (ddm::pow_fast(x, y, r)) return r;
+  return ddm::pow_slow(x, y);
 }
 
 }  // namespace imex
