@@ -45,22 +45,32 @@ sys.path.insert(0, HERE)
 HBM_PEAK_GBS = 8000.0
 
 
-def kernel_bytes_per_cell(cfg):
-    """Algorithmic HBM bytes per active cell per launch of each kernel.
+# the four kernels of a device-side driver step, in launch order, as
+# imex_run_steps_timed times them (rocprofv3 names in brackets)
+KERNELS = ["k_mask", "k_flux", "k_update_stage2", "k_update_settle_epilogue"]
 
-    k_mask:   read q(1), write the mask byte                      9
-    k_flux:   read q (n_v) and the bed slopes B'_x, B'_y          8 (n_v + 2)
-    k_update: read and write q (2 n_v), deposit read and write
-              with settling (16 n_s), hmax / pdynmax / mod_vel_max
-              read and write (48)                                 16 n_v + 16 n_s S + 48
-    Their sum is the whole-step figure 8 (3 n_v + 2) + 16 n_s S + 57.
+
+def kernel_bytes_per_cell(cfg):
+    """Algorithmic HBM bytes per active cell per launch of each kernel: the
+    whole-step figure 8 (3 n_v + 2) + 16 n_s S + 57 split over the kernels.
+
+    k_mask [k_mask]:          read q(1), write the mask byte        9
+    k_flux [k_flux<NS>]:      read q (n_v), bed slopes B'_x, B'_y   8 (n_v + 2)
+    k_update_stage2 [k_update<NS,1>]:
+                              read and write q                      16 n_v
+    k_update_settle_epilogue [k_update<NS,6>]:
+                              deposit read and write with
+                              settling (16 n_s), hmax / pdynmax /
+                              mod_vel_max read and write (48)       16 n_s S + 48
     Intermediates that a fused single pass would not need (the flux
-    divergence handed from k_flux to k_update, the tile flags) are not
-    counted: they show up in the measured traffic instead.
+    divergence handed from k_flux to the stage-2 kernel, q read again by the
+    sedimentation kernel, the tile and work lists) are not counted: they
+    show up in the measured traffic instead.
     """
     nv, ns = cfg.n_vars, cfg.n_solid
     s = 1 if cfg.settling_flag else 0
-    per = {"k_mask": 9, "k_flux": 8 * (nv + 2), "k_update": 16 * nv + 16 * ns * s + 48}
+    per = {"k_mask": 9, "k_flux": 8 * (nv + 2), "k_update_stage2": 16 * nv,
+           "k_update_settle_epilogue": 16 * ns * s + 48}
     per["step"] = sum(per.values())
     return per
 
@@ -163,7 +173,7 @@ def main():
     s = Solver(cfg, static, q0)
     total_cells_all = 0
     if args.warmup:
-        k = np.zeros(3)
+        k = np.zeros(len(KERNELS))
         c = np.zeros(1, np.int64)
         s.H.check(s.L.imex_run_steps_timed(s.h, args.warmup, ctypes.c_double(cfg.t_end), None,
                                            native._dp(k), _i64p(c)), "warmup")
@@ -174,7 +184,7 @@ def main():
     # timed region: two halves, with the mid-run state copied out between
     # them (outside the clock) for the CPU sample
     halves = [args.steps // 2, args.steps - args.steps // 2]
-    kernel_ms = np.zeros(3)
+    kernel_ms = np.zeros(len(KERNELS))
     cells = 0
     wall = 0.0
     mid = None
@@ -183,7 +193,7 @@ def main():
     for i, n in enumerate(halves):
         if n == 0:
             continue
-        k = np.zeros(3)
+        k = np.zeros(len(KERNELS))
         c = np.zeros(1, np.int64)
         dts = np.zeros(n)
         s_time(s)                               # synchronises the stream
@@ -215,8 +225,8 @@ def main():
     K = args.steps
     value = 2.0 * cells / wall
     per = kernel_bytes_per_cell(cfg)
-    names = ["k_mask", "k_flux", "k_update"]
-    dom = 1 + int(np.argmax(kernel_ms[1:]))
+    names = KERNELS
+    dom = int(np.argmax(kernel_ms))
     dname = names[dom]
     mean_launch_s = kernel_ms[dom] * 1e-3 / K
     cells_per_launch = cells / K

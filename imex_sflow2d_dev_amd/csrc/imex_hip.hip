@@ -1937,30 +1937,34 @@ extern "C" int imex_run_steps_timed(imex_handle* h, int nsteps, double t_end, do
   s.t_end = t_end;
   const unsigned long long act0 = s.active;
   HCHK(h, hipMemcpyAsync(h->G.st, &s, sizeof(s), hipMemcpyHostToDevice, h->stream));
-  std::vector<hipEvent_t> ev((size_t)nsteps * 4);
+  constexpr int NK = 4;  // k_mask, k_flux, k_update stage 2, k_update sedimentation + epilogue
+  std::vector<hipEvent_t> ev((size_t)nsteps * (NK + 1));
   for (auto& e : ev) HCHK(h, hipEventCreate(&e));
   int rc = 0;
   for (int n = 0; n < nsteps && !rc; n++) {
-    HCHK(h, hipEventRecord(ev[4 * n + 0], h->stream));
+    hipEvent_t* e = &ev[(size_t)(NK + 1) * n];
+    HCHK(h, hipEventRecord(e[0], h->stream));
     if ((rc = mask(h, 0))) break;
-    HCHK(h, hipEventRecord(ev[4 * n + 1], h->stream));
+    HCHK(h, hipEventRecord(e[1], h->stream));
     if ((rc = flux(h, 1, 0.0, n))) break;
-    HCHK(h, hipEventRecord(ev[4 * n + 2], h->stream));
-    if ((rc = update(h, 1 | 2 | 4, 0.0, 1, n))) break;
-    HCHK(h, hipEventRecord(ev[4 * n + 3], h->stream));
+    HCHK(h, hipEventRecord(e[2], h->stream));
+    if ((rc = update(h, 1, 0.0, 1, n))) break;
+    HCHK(h, hipEventRecord(e[3], h->stream));
+    if ((rc = update(h, 2 | 4, 0.0, 1, n))) break;
+    HCHK(h, hipEventRecord(e[4], h->stream));
   }
   HCHK(h, hipStreamSynchronize(h->stream));
-  double ms[3] = {0, 0, 0};
+  double ms[NK] = {0, 0, 0, 0};
   for (int n = 0; n < nsteps && !rc; n++)
-    for (int k = 0; k < 3; k++) {
+    for (int k = 0; k < NK; k++) {
       float f = 0.f;
-      HCHK(h, hipEventElapsedTime(&f, ev[4 * n + k], ev[4 * n + k + 1]));
+      HCHK(h, hipEventElapsedTime(&f, ev[(size_t)(NK + 1) * n + k], ev[(size_t)(NK + 1) * n + k + 1]));
       ms[k] += f;
     }
   for (auto& e : ev) (void)hipEventDestroy(e);
   if (rc) return rc;
   if (kernel_ms)
-    for (int k = 0; k < 3; k++) kernel_ms[k] = ms[k];
+    for (int k = 0; k < NK; k++) kernel_ms[k] = ms[k];
   if (dts && nsteps > 0)
     HCHK(h, hipMemcpyAsync(dts, h->G.dts, 8 * (size_t)nsteps, hipMemcpyDeviceToHost, h->stream));
   if (read_state(h, &s)) return -1;

@@ -116,9 +116,10 @@ int imex_post_step(imex_handle *h, double t_new, imex_step_stats *out);
  * epilogue.  dts (may be NULL) receives each step's dt at the end. */
 int imex_run_steps(imex_handle *h, int nsteps, double t_end, double *dts);
 /* imex_run_steps with HIP events around every kernel launch on the handle's
- * stream; kernel_ms[0..2] receive the summed durations of the active-set,
- * flux and update kernels over the nsteps steps, cells[0] the active cells
- * summed over the steps. */
+ * stream; kernel_ms[0..3] receive the summed durations over the nsteps steps
+ * of the active-set kernel, the flux kernel, the stage-2 update kernel and
+ * the sedimentation + epilogue kernel; cells[0] the active cells summed over
+ * the steps. */
 int imex_run_steps_timed(imex_handle *h, int nsteps, double t_end, double *dts,
                          double *kernel_ms, int64_t *cells);
 /* Byte-counter calibration: copies n_doubles doubles between two device
