@@ -173,12 +173,8 @@ def main():
     s = Solver(cfg, static, q0)
     total_cells_all = 0
     if args.warmup:
-        k = np.zeros(len(KERNELS))
-        c = np.zeros(1, np.int64)
-        s.H.check(s.L.imex_run_steps_timed(s.h, args.warmup, ctypes.c_double(cfg.t_end), None,
-                                           native._dp(k), _i64p(c)), "warmup")
-        total_cells_all += int(c[0])
-    s.t, s.dt = s_time(s)
+        _, _, c = s.run_steps_timed(args.warmup, record_dt=False)
+        total_cells_all += c
     print(f"[bench] warmup {args.warmup} steps done, t={s.t:.4f}", file=sys.stderr, flush=True)
 
     # timed region: two halves, with the mid-run state copied out between
@@ -190,36 +186,29 @@ def main():
     mid = None
     dts_all = []
     err_bits = 0
+    s.raise_on_error = False                    # physics error bits are reported, not raised
     for i, n in enumerate(halves):
         if n == 0:
             continue
-        k = np.zeros(len(KERNELS))
-        c = np.zeros(1, np.int64)
-        dts = np.zeros(n)
         s_time(s)                               # synchronises the stream
         t0 = time.perf_counter()
-        rc = s.L.imex_run_steps_timed(s.h, n, ctypes.c_double(cfg.t_end), native._dp(dts),
-                                      native._dp(k), _i64p(c))
-        t1 = time.perf_counter()                # the call returns after a stream sync
-        s.H.check(rc, "imex_run_steps_timed")
-        err_bits |= rc
+        dts, k, c = s.run_steps_timed(n)        # returns after a stream sync
+        t1 = time.perf_counter()
+        err_bits |= s.err
         wall += t1 - t0
         kernel_ms += k
-        cells += int(c[0])
+        cells += c
         dts_all.append(dts)
         print(f"[bench] timed part {i + 1}: {n} steps, {t1 - t0:.3f} s, "
-              f"{int(c[0])} active cells", file=sys.stderr, flush=True)
+              f"{c} active cells", file=sys.stderr, flush=True)
         if i == 0:
-            t, _ = s_time(s)
-            dd = np.concatenate(dts_all)
-            mid = dict(q=s.q, t=t, dt_old=dd[-1] if len(dd) else cfg.dt0,
-                       dt_old_old=dd[-2] if len(dd) > 1 else cfg.dt0,
+            mid = dict(q=s.q, t=s.t, dt_old=s.dt_old, dt_old_old=s.dt_old_old,
                        step=args.warmup + n)
     total_cells_all += cells
     if mid is None:
-        t, _ = s_time(s)
-        mid = dict(q=s.q, t=t, dt_old=cfg.dt0, dt_old_old=cfg.dt0, step=args.warmup + args.steps)
-    t_end_phys, _ = s_time(s)
+        mid = dict(q=s.q, t=s.t, dt_old=s.dt_old, dt_old_old=s.dt_old_old,
+                   step=args.warmup + args.steps)
+    t_end_phys = s.t
     s.close()
 
     K = args.steps
@@ -282,10 +271,6 @@ def main():
         print("[bench] CPU reference sample ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(args.workload, mid, args.cpu_seconds)
     print(json.dumps(out), flush=True)
-
-
-def _i64p(a):
-    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
 
 
 def s_time(s):

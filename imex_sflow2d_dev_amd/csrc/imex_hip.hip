@@ -737,15 +737,20 @@ template <int NS>
 __device__ void eval_f(const Params& P, double dt, const double* qj, const double* q0,
                        const double* Rni, double* f, double& scal) {
   constexpr int NV = 4 + NS;
-  cx qc[NV];
+  double n1, n2;
+  if (!implicit_r<NS>(P, qj, n1, n2)) {
+    cx qc[NV];
 #pragma unroll
-  for (int i = 0; i < NV; i++) qc[i] = C(qj[i], 0.0);
-  cx n1, n2;
-  implicit_c<NS>(P, qc, n1, n2);
+    for (int i = 0; i < NV; i++) qc[i] = C(qj[i], 0.0);
+    cx c1, c2;
+    implicit_c<NS>(P, qc, c1, c2);
+    n1 = c1.re;
+    n2 = c2.re;
+  }
   double s = 0.0;
 #pragma unroll
   for (int i = 0; i < NV; i++) {
-    const double nh = (i == 1) ? n1.re : ((i == 2) ? n2.re : 0.0);
+    const double nh = (i == 1) ? n1 : ((i == 2) ? n2 : 0.0);
     const double Rj = Rni[i] - 1.0 * nh;
     f[i] = qj[i] - q0[i] + dt * Rj;
     f[i] = 1.0 * f[i];
@@ -1387,6 +1392,27 @@ __global__ void kt_implicit_cplx(Params P, int n, const double* qre, const doubl
     }
   }
 }
+// the residual's implicit terms as eval_f computes them (real path, complex
+// fallback); fast[i] = 1 when the real path was taken
+template <int NS>
+__global__ void kt_implicit_real(Params P, int n, const double* q, double* out, int* fast) {
+  constexpr int NV = 4 + NS;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    double n1, n2;
+    const bool ok = implicit_r<NS>(P, q + (size_t)NV * i, n1, n2);
+    if (!ok) {
+      cx qc[NV];
+      for (int m = 0; m < NV; m++) qc[m] = C(q[(size_t)NV * i + m], 0.0);
+      cx c1, c2;
+      implicit_c<NS>(P, qc, c1, c2);
+      n1 = c1.re;
+      n2 = c2.re;
+    }
+    for (int m = 0; m < NV; m++) out[(size_t)NV * i + m] = (m == 1) ? n1 : ((m == 2) ? n2 : 0.0);
+    fast[i] = ok ? 1 : 0;
+  }
+}
 template <int NS>
 __global__ void kt_semi_impl(Params P, int n, const double* qp, double* si) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1456,6 +1482,7 @@ struct imex_handle {
   double* d_tmp = nullptr;  // N*nv scratch for layout conversion
   double* d_diag = nullptr;
   int diag_blocks = 0;
+  double* dts_buf = nullptr;  // device copy of the per-step dt of a run_steps call
   int dts_cap = 0;
   std::string err;
 };
@@ -1480,6 +1507,25 @@ static int dalloc(imex_handle* h, T** p, size_t n) {
   HCHK(h, hipMemsetAsync(v, 0, n ? n * sizeof(T) : 16, h->stream));
   h->allocs.push_back(v);
   *p = (T*)v;
+  return 0;
+}
+
+// Points the kernels' per-step dt output at the handle's device buffer (grown
+// to nsteps when needed) when the caller wants the dt sequence, else at NULL.
+static int bind_dts(imex_handle* h, const double* dts, int nsteps) {
+  if (dts && nsteps > h->dts_cap) {
+    double* p = nullptr;
+    if (int rc = dalloc(h, &p, (size_t)nsteps)) return rc;
+    if (h->dts_buf) {
+      HCHK(h, hipStreamSynchronize(h->stream));
+      for (auto& a : h->allocs)
+        if (a == h->dts_buf) a = nullptr;
+      HCHK(h, hipFree(h->dts_buf));
+    }
+    h->dts_buf = p;
+    h->dts_cap = nsteps;
+  }
+  h->G.dts = dts ? h->dts_buf : nullptr;
   return 0;
 }
 
@@ -1893,14 +1939,7 @@ extern "C" int imex_post_step(imex_handle* h, double t_new, imex_step_stats* out
 extern "C" int imex_run_steps(imex_handle* h, int nsteps, double t_end, double* dts) {
   if (!h || nsteps < 0) return -2;
   HCHK(h, hipSetDevice(h->device));
-  if (dts && nsteps > h->dts_cap) {
-    double* p = nullptr;
-    int rc = dalloc(h, &p, (size_t)nsteps);
-    if (rc) return rc;
-    h->G.dts = p;
-    h->dts_cap = nsteps;
-  }
-  if (!dts) h->G.dts = nullptr;
+  if (int rc = bind_dts(h, dts, nsteps)) return rc;
   {
     DevState s;
     if (read_state(h, &s)) return -1;
@@ -1924,14 +1963,7 @@ extern "C" int imex_run_steps_timed(imex_handle* h, int nsteps, double t_end, do
                                     double* kernel_ms, int64_t* cells) {
   if (!h || nsteps < 0) return -2;
   HCHK(h, hipSetDevice(h->device));
-  if (dts && nsteps > h->dts_cap) {
-    double* p = nullptr;
-    int rc = dalloc(h, &p, (size_t)nsteps);
-    if (rc) return rc;
-    h->G.dts = p;
-    h->dts_cap = nsteps;
-  }
-  if (!dts) h->G.dts = nullptr;
+  if (int rc = bind_dts(h, dts, nsteps)) return rc;
   DevState s;
   if (read_state(h, &s)) return -1;
   s.t_end = t_end;
@@ -1995,6 +2027,17 @@ extern "C" int imex_calib_copy(imex_handle* h, int64_t n, int reps) {
   (void)hipFree(a);
   (void)hipFree(b);
   if (e != hipSuccess) return set_err(h, "k_copy", e);
+  return 0;
+}
+
+extern "C" int imex_get_time_history(imex_handle* h, double* t, double* dt_old,
+                                     double* dt_old_old) {
+  if (!h) return -2;
+  DevState s;
+  if (read_state(h, &s)) return -1;
+  if (t) *t = s.t;
+  if (dt_old) *dt_old = s.dt_old;
+  if (dt_old_old) *dt_old_old = s.dt_old_old;
   return 0;
 }
 
@@ -2075,26 +2118,42 @@ extern "C" void imex_destroy(imex_handle* h) {
 }
 
 // ---- per-function batch entry points -------------------------------------
+// Device buffers of one test-hook call; `bad` is set when an allocation or
+// copy failed, and the hook then returns before launching anything.
 struct DevBufs {
   imex_handle* h;
   std::vector<void*> p;
+  bool bad = false;
   ~DevBufs() {
     for (void* x : p) (void)hipFree(x);
   }
   double* up(const double* src, size_t n) {
     void* d = nullptr;
-    if (hipMalloc(&d, n ? 8 * n : 16) != hipSuccess) return nullptr;
+    if (hipMalloc(&d, n ? 8 * n : 16) != hipSuccess) {
+      bad = true;
+      return nullptr;
+    }
     p.push_back(d);
-    if (src && hipMemcpy(d, src, 8 * n, hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+    if (src && hipMemcpy(d, src, 8 * n, hipMemcpyHostToDevice) != hipSuccess) bad = true;
     return (double*)d;
   }
   int* upi(size_t n) {
     void* d = nullptr;
-    if (hipMalloc(&d, n ? 4 * n : 16) != hipSuccess) return nullptr;
+    if (hipMalloc(&d, n ? 4 * n : 16) != hipSuccess) {
+      bad = true;
+      return nullptr;
+    }
     p.push_back(d);
     return (int*)d;
   }
 };
+#define BUFS_OK(h, b) \
+  do {                \
+    if ((b).bad) {     \
+      (h)->err = "test hook: device buffer allocation or upload failed"; \
+      return -1;      \
+    }                 \
+  } while (0)
 
 #define NSDISPATCH(h, KER, ...)                                                             \
   do {                                                                                      \
@@ -2121,6 +2180,7 @@ extern "C" int imex_test_qc_to_qp(imex_handle* h, int n, const double* qc, doubl
   const int nv = h->P.nv;
   DevBufs b{h};
   double *dq = b.up(qc, (size_t)n * nv), *dqp = b.up(nullptr, (size_t)n * (nv + 2)), *dp = b.up(nullptr, n);
+  BUFS_OK(h, b);
   NSDISPATCH(h, kt_qc_to_qp, n, dq, dqp, dp);
   if (fin(h)) return -1;
   if (down(h, qp, dqp, (size_t)n * (nv + 2)) || down(h, pdyn, dp, n)) return -1;
@@ -2131,6 +2191,7 @@ extern "C" int imex_test_qp_to_qc(imex_handle* h, int n, const double* qp, doubl
   const int nv = h->P.nv;
   DevBufs b{h};
   double *dqp = b.up(qp, (size_t)n * (nv + 2)), *dq = b.up(nullptr, (size_t)n * nv);
+  BUFS_OK(h, b);
   NSDISPATCH(h, kt_qp_to_qc, n, dqp, dq);
   if (fin(h)) return -1;
   return down(h, qc, dq, (size_t)n * nv);
@@ -2141,6 +2202,7 @@ extern "C" int imex_test_fluxes(imex_handle* h, int n, int dir, const double* qc
   const int nv = h->P.nv;
   DevBufs b{h};
   double *dq = b.up(qc, (size_t)n * nv), *dqp = b.up(qp, (size_t)n * (nv + 2)), *df = b.up(nullptr, (size_t)n * nv);
+  BUFS_OK(h, b);
   NSDISPATCH(h, kt_fluxes, n, dir, dq, dqp, df);
   if (fin(h)) return -1;
   return down(h, flux, df, (size_t)n * nv);
@@ -2150,6 +2212,7 @@ extern "C" int imex_test_speeds(imex_handle* h, int n, int dir, const double* qp
   const int nv = h->P.nv;
   DevBufs b{h};
   double *dqp = b.up(qp, (size_t)n * (nv + 2)), *d = b.up(nullptr, (size_t)n * 2);
+  BUFS_OK(h, b);
   NSDISPATCH(h, kt_speeds, n, dir, dqp, d);
   if (fin(h)) return -1;
   return down(h, out2, d, (size_t)n * 2);
@@ -2161,9 +2224,24 @@ extern "C" int imex_test_implicit_cplx(imex_handle* h, int n, const double* qre,
   DevBufs b{h};
   double *a = b.up(qre, (size_t)n * nv), *c = b.up(qim, (size_t)n * nv);
   double *o1 = b.up(nullptr, (size_t)n * nv), *o2 = b.up(nullptr, (size_t)n * nv);
+  BUFS_OK(h, b);
   NSDISPATCH(h, kt_implicit_cplx, n, a, c, o1, o2);
   if (fin(h)) return -1;
   if (down(h, ore, o1, (size_t)n * nv) || down(h, oim, o2, (size_t)n * nv)) return -1;
+  return 0;
+}
+extern "C" int imex_test_implicit_real(imex_handle* h, int n, const double* q, double* out,
+                                       int* fast) {
+  if (!h || n <= 0) return -2;
+  const int nv = h->P.nv;
+  DevBufs b{h};
+  double *a = b.up(q, (size_t)n * nv), *o = b.up(nullptr, (size_t)n * nv);
+  int* f = b.upi((size_t)n);
+  BUFS_OK(h, b);
+  NSDISPATCH(h, kt_implicit_real, n, a, o, f);
+  if (fin(h)) return -1;
+  if (down(h, out, o, (size_t)n * nv)) return -1;
+  HCHK(h, hipMemcpy(fast, f, 4 * (size_t)n, hipMemcpyDeviceToHost));
   return 0;
 }
 extern "C" int imex_test_semi_impl(imex_handle* h, int n, const double* qp, double* si) {
@@ -2171,6 +2249,7 @@ extern "C" int imex_test_semi_impl(imex_handle* h, int n, const double* qp, doub
   const int nv = h->P.nv;
   DevBufs b{h};
   double *dqp = b.up(qp, (size_t)n * (nv + 2)), *d = b.up(nullptr, (size_t)n * nv);
+  BUFS_OK(h, b);
   NSDISPATCH(h, kt_semi_impl, n, dqp, d);
   if (fin(h)) return -1;
   return down(h, si, d, (size_t)n * nv);
@@ -2182,6 +2261,7 @@ extern "C" int imex_test_expl(imex_handle* h, int n, const double* bx, const dou
   DevBufs b{h};
   double *dbx = b.up(bx, n), *dby = b.up(by, n), *dqp = b.up(qp, (size_t)n * (nv + 2));
   double* d = b.up(nullptr, (size_t)n * nv);
+  BUFS_OK(h, b);
   NSDISPATCH(h, kt_expl, n, dbx, dby, t, dqp, d);
   if (fin(h)) return -1;
   return down(h, expl, d, (size_t)n * nv);
@@ -2191,6 +2271,7 @@ extern "C" int imex_test_settling(imex_handle* h, int n, const double* diam, con
   if (!h || n <= 0) return -2;
   DevBufs b{h};
   double *a = b.up(diam, n), *c = b.up(rhos, n), *d = b.up(rhoc, n), *e = b.up(ikv, n), *o = b.up(nullptr, n);
+  BUFS_OK(h, b);
   hipLaunchKernelGGL(kt_settling, dim3((n + 127) / 128), dim3(128), 0, 0, h->P, n, a, c, d, e, o);
   if (fin(h)) return -1;
   return down(h, out, o, n);
@@ -2201,6 +2282,7 @@ extern "C" int imex_test_mass_exchange(imex_handle* h, int n, const double* qp, 
   const int nv = h->P.nv, ns = h->P.ns;
   DevBufs b{h};
   double *dqp = b.up(qp, (size_t)n * (nv + 2)), *dd = b.up(nullptr, (size_t)n * ns), *de = b.up(nullptr, (size_t)n * nv);
+  BUFS_OK(h, b);
   NSDISPATCH(h, kt_mass_exchange, n, dqp, dt, dd, de);
   if (fin(h)) return -1;
   if (down(h, dep, dd, (size_t)n * ns) || down(h, eq, de, (size_t)n * nv)) return -1;
@@ -2214,6 +2296,7 @@ extern "C" int imex_test_newton(imex_handle* h, int n, double dt, const double* 
   double *a = b.up(q_si, (size_t)n * nv), *c = b.up(q0, (size_t)n * nv), *r = b.up(R, (size_t)n * nv);
   double* o = b.up(nullptr, (size_t)n * nv);
   int* it = b.upi(n);
+  BUFS_OK(h, b);
   NSDISPATCH(h, kt_newton, n, dt, a, c, r, o, it);
   if (fin(h)) return -1;
   if (down(h, q, o, (size_t)n * nv)) return -1;
@@ -2224,6 +2307,7 @@ extern "C" int imex_test_pow(imex_handle* h, int n, const double* x, const doubl
   if (!h || n <= 0) return -2;
   DevBufs b{h};
   double *dx = b.up(x, n), *dy = b.up(y, n), *o = b.up(nullptr, n);
+  BUFS_OK(h, b);
   hipLaunchKernelGGL(kt_pow, dim3((n + 127) / 128), dim3(128), 0, 0, n, dx, dy, o);
   if (fin(h)) return -1;
   return down(h, out, o, (size_t)n);
@@ -2237,6 +2321,7 @@ extern "C" int imex_test_arith(imex_handle* h, int n, const double* a, const dou
   if (!h || n <= 0) return -2;
   DevBufs b{h};
   double *da = b.up(a, n), *db = b.up(bb, n), *o = b.up(nullptr, (size_t)4 * n);
+  BUFS_OK(h, b);
   hipLaunchKernelGGL(kt_arith, dim3((n + 127) / 128), dim3(128), 0, 0, n, da, db, o);
   if (fin(h)) return -1;
   return down(h, out, o, (size_t)4 * n);

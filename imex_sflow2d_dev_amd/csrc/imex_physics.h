@@ -557,6 +557,117 @@ __device__ inline void implicit_c(const Params& P, const cx* qj, cx& nh1, cx& nh
   }
 }
 
+// The residual evaluation of the Newton solve calls the implicit terms at a
+// real state: the reference converts r_qj to complex (imaginary parts +0) and
+// runs the complex code (src/constitutive_2d.f90:3513-3683).  With imaginary
+// parts that are all zeros, each complex operation's real part is the real
+// operation on the real parts plus or minus a signed zero:
+//   (a + 0i)(b + 0i)  real part  a*b - (+-0)            = a*b
+//   (a + 0i)/(c + 0i) (Annex G)  scalbn((a*c')/(c'*c'), -k),  k = logb|c|,
+//                                c' = scalbn(c, -k)
+//   sqrt(x + 0i), x > 0          sqrt(x)
+// and every imaginary part stays a signed zero, provided no real
+// intermediate is zero (the signed zeros could then decide the sign of a
+// zero), infinite or NaN (a NaN imaginary part would reach the real parts).
+// implicit_r evaluates the real parts that way and reports whether that
+// proviso held for every intermediate; when it did not, the caller takes the
+// complex path.  The result is then bit for bit the complex path's real part.
+__device__ __forceinline__ bool nz_finite(double v) {
+  return (fabs(v) > 0.0) & (fabs(v) < INFINITY);
+}
+// real part of cdiv(C(a, +-0), C(c, +-0)); ok cleared unless a, c and the
+// quotient are non-zero and finite
+__device__ __forceinline__ double cdiv_re(double a, double c, bool& ok) {
+  ok = ok & nz_finite(a) & nz_finite(c);
+  const int k = ilogb(c);  // logb(fmax(|c|, 0)) for a finite non-zero c
+  const double cs = scalbn(c, -k);
+  const double x = scalbn((a * cs) / (cs * cs), -k);
+  ok = ok & nz_finite(x);
+  return x;
+}
+template <int NS>
+__device__ inline bool implicit_r(const Params& P, const double* q, double& nh1, double& nh2) {
+  nh1 = 0.0;
+  nh2 = 0.0;
+  // dry: u = v = 0 and mod_vel = 0 on the complex path, both terms +0
+  if (!(q[0] > kEps)) return true;
+  if (!(q[0] > P.eps_sing)) return false;  // thin-layer velocity regularisation: complex path
+  bool ok = nz_finite(q[0]);
+  const double inv = cdiv_re(1.0, q[0], ok);
+  double xs[NS];
+  double xs_tot = 0.0;
+#pragma unroll
+  for (int i = 0; i < NS; i++) {
+    xs[i] = q[4 + i] * inv;
+    ok = ok & nz_finite(xs[i]);
+    xs_tot = xs_tot + xs[i];
+  }
+  const double xc = 1.0 - xs_tot;
+  double t = (xc - 0.0) * P.cp_a;
+  ok = ok & nz_finite(t);
+  const double sp_heat_c = cdiv_re(t + 0.0, xc, ok);
+  double dot = 0.0;
+#pragma unroll
+  for (int i = 0; i < NS; i++) {
+    const double m = xs[i] * P.cp_s[i];
+    ok = ok & nz_finite(m);
+    dot = dot + m;
+  }
+  const double m2 = xc * sp_heat_c;
+  ok = ok & nz_finite(m2);
+  const double sp_heat_mix = dot + m2;
+  const double den = q[0] * sp_heat_mix;
+  ok = ok & nz_finite(den);
+  double T;
+  if (P.energy) {
+    const double a1 = q[1] * q[1], a2 = q[2] * q[2];
+    ok = ok & nz_finite(a1) & nz_finite(a2);
+    double ke = 0.5 * (a1 + a2);
+    ok = ok & nz_finite(ke);
+    ke = ke * inv;
+    ok = ok & nz_finite(ke);
+    T = cdiv_re(q[3] - ke, den, ok);
+  } else {
+    T = cdiv_re(q[3], den, ok);
+  }
+  if (T <= 0.0) T = P.T_amb;
+  t = (xc - 0.0) * P.R_a;
+  ok = ok & nz_finite(t);
+  const double Rc = cdiv_re(t + 0.0, xc, ok);
+  double irc = Rc * T;
+  ok = ok & nz_finite(irc);
+  irc = irc * P.inv_pres;
+  ok = ok & nz_finite(irc);
+  double d2 = 0.0;
+#pragma unroll
+  for (int i = 0; i < NS; i++) {
+    const double m = xs[i] * P.inv_rho_s[i];
+    ok = ok & nz_finite(m);
+    d2 = d2 + m;
+  }
+  const double m3 = xc * irc;
+  ok = ok & nz_finite(m3);
+  const double rho_m = cdiv_re(1.0, d2 + m3, ok);
+  const double u = q[1] * inv, v = q[2] * inv;
+  ok = ok & nz_finite(u) & nz_finite(v);
+  const double red_grav = cdiv_re(rho_m - P.rho_a_amb, rho_m, ok) * P.grav;
+  ok = ok & nz_finite(red_grav);
+  const double uu = u * u, vv = v * v;
+  ok = ok & nz_finite(uu) & nz_finite(vv);
+  const double mod_vel2 = (uu + vv) + 0.0;
+  const double mod_vel = sqrt(mod_vel2);
+  ok = ok & nz_finite(mod_vel);
+  const double rg = rho_m * red_grav;
+  ok = ok & nz_finite(rg);
+  const double ts = cdiv_re(rg, P.xi, ok) * mod_vel2;
+  ok = ok & nz_finite(ts);
+  const double p1 = ts * cdiv_re(u, mod_vel, ok), p2 = ts * cdiv_re(v, mod_vel, ok);
+  ok = ok & nz_finite(p1) & nz_finite(p2);
+  nh1 = 0.0 - p1;
+  nh2 = 0.0 - p2;
+  return ok;
+}
+
 // settling_velocity (Schiller-Naumann): src/constitutive_2d.f90:5326-5404
 __device__ inline double settling_velocity(const Params& P, double diam, double rhos,
                                            double rhoc, double ikv) {

@@ -33,9 +33,9 @@ EXPORTS = [
     "imex_create", "imex_set_static", "imex_upload_state", "imex_check_solve",
     "imex_timestep", "imex_rk_step", "imex_erosion_deposition", "imex_post_step",
     "imex_run_steps", "imex_run_steps_timed", "imex_calib_copy", "imex_get_time",
-    "imex_set_time", "imex_download", "imex_last_error", "imex_destroy",
+    "imex_get_time_history", "imex_set_time", "imex_download", "imex_last_error", "imex_destroy",
     "imex_test_qc_to_qp", "imex_test_qp_to_qc", "imex_test_fluxes", "imex_test_speeds",
-    "imex_test_implicit_cplx", "imex_test_semi_impl", "imex_test_expl",
+    "imex_test_implicit_cplx", "imex_test_implicit_real", "imex_test_semi_impl", "imex_test_expl",
     "imex_test_settling", "imex_test_mass_exchange", "imex_test_newton", "imex_test_arith",
     "imex_test_pow", "imex_host_pow",
 ]

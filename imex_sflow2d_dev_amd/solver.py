@@ -48,6 +48,7 @@ class Solver:
         self.dt_old = cfg.dt0
         self.dt_old_old = cfg.dt0
         self.err = 0
+        self._host_driven = False  # host-driven steps since the device last had t
         self._set_static(static)
         self._upload(q0)
 
@@ -127,6 +128,7 @@ class Solver:
         dt = min(dt, 1.1 * 0.5 * (self.dt_old + self.dt_old_old))
         self.dt_old_old = self.dt_old
         self.dt_old = dt
+        self._host_driven = True
         self.dt = dt
         self.imex_RK_solver()
         self.update_erosion_deposition_cell(dt)
@@ -142,24 +144,44 @@ class Solver:
         rc = self.L.imex_run_steps(self.h, ctypes.c_int(nsteps), ctypes.c_double(t_end),
                                    native._dp(dts) if record_dt else None)
         self.H.check(rc, "imex_run_steps")
-        t = ctypes.c_double()
-        dt = ctypes.c_double()
-        self.H.check(self.L.imex_get_time(self.h, ctypes.byref(t), ctypes.byref(dt)), "get_time")
-        self.t, self.dt = t.value, dt.value
-        if record_dt and nsteps > 0:
-            self.dt_old_old = dts[nsteps - 2] if nsteps > 1 else self.dt_old
-            self.dt_old = dts[nsteps - 1]
+        self._pull_time()
         self._raise(rc)
         return dts[:nsteps] if record_dt else None
+
+    def run_steps_timed(self, nsteps: int, t_end: float = None, record_dt: bool = True):
+        """run_steps with HIP events around every launch: returns (dts or
+        None, kernel_ms[4], active cells summed over the steps) -- the four
+        kernels are the active set, the fluxes, stage 2 and the
+        sedimentation + epilogue."""
+        t_end = self.cfg.t_end if t_end is None else t_end
+        self._sync_time_to_device()
+        dts = np.zeros(max(nsteps, 1)) if record_dt else None
+        kms = np.zeros(4)
+        cells = np.zeros(1, np.int64)
+        rc = self.L.imex_run_steps_timed(self.h, ctypes.c_int(nsteps), ctypes.c_double(t_end),
+                                         native._dp(dts) if record_dt else None, native._dp(kms),
+                                         cells.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
+        self.H.check(rc, "imex_run_steps_timed")
+        self._pull_time()
+        self._raise(rc)
+        return (dts[:nsteps] if record_dt else None), kms, int(cells[0])
 
     def _sync_time_to_device(self):
         # the device-side driver keeps t and the dt history in device memory;
         # host-driven steps advance them on the host, so push them first
-        self.H.check(self.L.imex_set_time(self.h, ctypes.c_double(self.t),
-                                          ctypes.c_double(self.dt_old),
-                                          ctypes.c_double(self.dt_old_old)), "imex_set_time")
+        if self._host_driven:
+            self.H.check(self.L.imex_set_time(self.h, ctypes.c_double(self.t),
+                                              ctypes.c_double(self.dt_old),
+                                              ctypes.c_double(self.dt_old_old)), "imex_set_time")
+            self._host_driven = False
 
-    # ---- lazy host mirrors -----------------------------------------------------
+    def _pull_time(self):
+        t, d1, d2, dt = (ctypes.c_double() for _ in range(4))
+        self.H.check(self.L.imex_get_time_history(self.h, ctypes.byref(t), ctypes.byref(d1),
+                                                  ctypes.byref(d2)), "imex_get_time_history")
+        self.H.check(self.L.imex_get_time(self.h, None, ctypes.byref(dt)), "imex_get_time")
+        self.t, self.dt_old, self.dt_old_old, self.dt = t.value, d1.value, d2.value, dt.value
+
     def _download(self, what, **kw):
         c = self.cfg
         bufs = dict(q=None, hmax=None, pdynmax=None, mvel=None, dep=None, vuln=None, mask=None)
@@ -263,6 +285,18 @@ class KernelProbe:
         self._run("imex_test_implicit_cplx", ctypes.c_int(n), native._dp(qre), native._dp(qim),
                   native._dp(ore), native._dp(oim))
         return ore, oim
+
+    def implicit_real(self, q):
+        """Implicit terms at real states as the Newton residual evaluates
+        them; returns (terms, fast) with fast[i] True where the real-arithmetic
+        evaluation was used."""
+        q = np.ascontiguousarray(q, np.float64)
+        n = q.shape[0]
+        out = np.zeros((n, self.nv))
+        fast = np.zeros(n, np.int32)
+        self._run("imex_test_implicit_real", ctypes.c_int(n), native._dp(q), native._dp(out),
+                  native._ip(fast))
+        return out, fast.astype(bool)
 
     def semi_impl(self, qp):
         qp = np.ascontiguousarray(qp, np.float64)

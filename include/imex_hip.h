@@ -131,6 +131,9 @@ int imex_get_time(imex_handle *h, double *t, double *dt);
 /* set t and the dt history (dt_old, dt_old_old) of the device-side driver,
  * e.g. after host-driven steps or a restart */
 int imex_set_time(imex_handle *h, double t, double dt_old, double dt_old_old);
+/* t and the dt history of the device-side driver after device-driven steps
+ * (the counterpart of imex_set_time) */
+int imex_get_time_history(imex_handle *h, double *t, double *dt_old, double *dt_old_old);
 /* Lazy host mirrors: copies the selected device arrays (any pointer may be
  * NULL when not selected).  q in Fortran order; deposit(nx,ny,n_solid);
  * vuln as one int32 per (level, cell) in Fortran order; mask one int32 per cell. */
@@ -150,6 +153,11 @@ int imex_test_fluxes(imex_handle *h, int n, int dir, const double *qc, const dou
 int imex_test_speeds(imex_handle *h, int n, int dir, const double *qp, double *out2);
 int imex_test_implicit_cplx(imex_handle *h, int n, const double *qre, const double *qim,
                             double *ore, double *oim);
+/* the implicit terms as the Newton residual evaluates them at real states
+ * (src/solver_2d.f90:2271 calls src/constitutive_2d.f90:3513 with r_qj): a
+ * real-arithmetic evaluation equal to the complex one's real part, with the
+ * complex evaluation as fallback; fast[i] = 1 where the real one was used */
+int imex_test_implicit_real(imex_handle *h, int n, const double *q, double *out, int *fast);
 int imex_test_semi_impl(imex_handle *h, int n, const double *qp, double *si);
 int imex_test_expl(imex_handle *h, int n, const double *bx, const double *by, double t,
                    const double *qp, double *expl);

@@ -140,3 +140,69 @@ def test_functions_match_reference_fixtures(name):
     np.testing.assert_allclose(dep, g["mx_dep"], rtol=1e-5, atol=0)
     np.testing.assert_allclose(eq, g["mx_eq"], rtol=1e-5, atol=1e-12 * np.abs(g["mx_eq"]).max())
     kp.close()
+
+
+def _bits_eq(a, b, what):
+    """Equal bit patterns (signed zeros told apart), any NaN equal to any NaN."""
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    same = (a.view(np.int64) == b.view(np.int64)) | (np.isnan(a) & np.isnan(b))
+    if not same.all():
+        i = tuple(np.argwhere(~same)[0])
+        raise AssertionError(f"{what}: {int((~same).sum())} mismatches, first at {i}: "
+                             f"{a[i]!r} vs {b[i]!r}")
+
+
+def _edge_states(cfg, wet, rng):
+    """Wet states with the zeros, extremes and non-finite values that make
+    the real evaluation of the implicit terms hand over to the complex one."""
+    nv, ns = cfg.n_vars, cfg.n_solid
+    base = wet[rng.choice(len(wet), 64)]
+    out = [base]
+    for col in range(nv):
+        for val in (0.0, -0.0, 5e-324, 1e-310, 1e300, -1e300, np.inf, -np.inf, np.nan):
+            s = base[:4].copy()
+            s[:, col] = val
+            out.append(s)
+    s = base[:8].copy()
+    s[:, 4:4 + ns] = s[:, [0]] / ns            # solids only: x_c = 0
+    out.append(s)
+    s = base[:8].copy()
+    s[:, 3] = -s[:, 3]                         # T <= 0 fallback to T_ambient
+    out.append(s)
+    s = base[:8].copy()
+    s[:, 0] = cfg.eps_sing * np.array([0.5, 1.0, 1.5, 2.0, 0.1, 0.9, 3.0, 1e-3])
+    out.append(s)                               # thin layers around eps_sing
+    s = base[:8].copy()
+    s[:, 0] = [0.0, -1.0, 1e-17, 2.220446049250313e-16, 3e-16, np.nan, 1e-16, 5e-324]
+    out.append(s)                               # dry and below EPSILON
+    return np.concatenate(out)
+
+
+def test_implicit_real_path_equals_complex_real_part(case):
+    """The Newton residual's implicit terms, evaluated in real arithmetic
+    where that is exact, bit for bit (signed zeros included) against the
+    complex evaluation's real part, on flow states and on edge states; the
+    real evaluation must carry the ordinary flow states."""
+    cfg, B = case
+    wet = _states(cfg, B, 150, 5)
+    rng = np.random.default_rng(11)
+    q = np.concatenate([wet, _edge_states(cfg, wet, rng)])
+    kp = KernelProbe(cfg)
+    got, fast = kp.implicit_real(q)
+    re, _ = kp.implicit_cplx(q, np.zeros_like(q))
+    _bits_eq(got, re, "implicit terms, real path vs complex")
+    o = oraclepy.Oracle(cfg)
+    _eq(got, o.implicit(q), "implicit terms vs oracle")
+    assert fast[:len(wet)].mean() > 0.8, fast[:len(wet)].mean()
+    assert not fast[len(wet):].all()
+    kp.close()
+
+
+@pytest.mark.parametrize("name", ["c1_48", "c2_64"])
+def test_implicit_real_path_matches_reference_fixture(name):
+    cfg, _ = cases.CASES[name]()
+    g = dict(np.load(golden_path(f"functions_{name}.npz"), allow_pickle=False))
+    kp = KernelProbe(cfg)
+    got, _ = kp.implicit_real(g["qc"])
+    _eq(got, g["implicit"], "implicit terms at real states")
+    kp.close()

@@ -67,3 +67,31 @@ def test_active_set_rule_and_finite_state(c2):
         assert (q[wet, 0] > 0.0).all() and (q[~wet] == 0.0).all()
         assert (q[:, :, 4:] >= 0.0).all()
     s.close()
+
+
+def test_dt_output_after_a_timed_run_without_it():
+    """imex_run_steps_timed with no dt output between two imex_run_steps
+    calls that ask for it: the later call still returns the dt sequence, and
+    the run equals one driven by imex_run_steps alone."""
+    import ctypes
+    from imex_sflow2d_dev_amd import native
+    import cases
+    cfg, B = cases.CASES["c2_96"]()
+    st, q0 = init.setup(cfg, B)
+    a = Solver(cfg, st, q0)
+    b = Solver(cfg, st, q0)
+    a.run_steps(40)
+    b.run_steps(40)
+    k = np.zeros(4)
+    c = np.zeros(1, np.int64)
+    rc = a.L.imex_run_steps_timed(a.h, 20, ctypes.c_double(cfg.t_end), None, native._dp(k),
+                                  c.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
+    assert rc == 0
+    assert (k > 0.0).all() and c[0] > 0
+    b.run_steps(20)
+    da = a.run_steps(30)
+    db = b.run_steps(30)
+    assert np.array_equal(da, db) and (da > 0.0).all()
+    assert np.array_equal(a.q, b.q)
+    a.close()
+    b.close()
