@@ -137,7 +137,7 @@ __global__ __launch_bounds__(NTHR) void k_mask(Params P, Grid G, int solve_all, 
 // ===========================================================================
 // reconstruction of one cell in one direction: src/solver_2d.f90:3095-3924
 // ===========================================================================
-// LDS primitive-state tile: sq[(i*RY + lk)*RX + lj]
+// LDS primitive states of a half tile: sq[(i*RY + lk)*FRX + lj]
 struct TileView {
   const double* sq;
   const signed char* ssc;
@@ -145,7 +145,7 @@ struct TileView {
   bool tsvar;  // timestep variant: cells of the source disc carry T = T_ambient
   double T_amb;
   __device__ __forceinline__ double at(int i, int idx) const {
-    double v = sq[i * RX * RY + idx];
+    double v = sq[i * FRX * RY + idx];
     if (tsvar && i == 3 && ssc[idx] == 1) v = T_amb;
     return v;
   }
@@ -296,7 +296,7 @@ __device__ __forceinline__ void recon_y(const Params& P, const Grid& G, const Ti
     } else {
       ys0 = G.yc[k - 1];
       ys2 = G.yc[k + 1];
-      double v0 = T.at(i, idx - RX), v2 = T.at(i, idx + RX);
+      double v0 = T.at(i, idx - FRX), v2 = T.at(i, idx + FRX);
       if (src2) {
         if (sS) {
           ys0 = G.ys[k];
@@ -322,14 +322,14 @@ __device__ __forceinline__ void recon_y(const Params& P, const Grid& G, const Ti
     if (k == 0) {
       qp_to_qp2(S, q2u, q2v);
       v0 = ii ? q2v : q2u;
-      v2 = T.at(i, idx + RX);
+      v2 = T.at(i, idx + FRX);
     } else if (k == ny - 1) {
       qp_to_qp2(Nn, q2u, q2v);
-      v0 = T.at(i, idx - RX);
+      v0 = T.at(i, idx - FRX);
       v2 = ii ? q2v : q2u;
     } else {
-      v0 = T.at(i, idx - RX);
-      v2 = T.at(i, idx + RX);
+      v0 = T.at(i, idx - FRX);
+      v2 = T.at(i, idx + FRX);
       if (src2) {
         if (sS) {
           ys0 = G.ys[k];
@@ -488,10 +488,10 @@ __device__ __forceinline__ void kt_flux(const Params& P, int dir, const double* 
 template <int NS>
 constexpr size_t flux_lds_bytes() {
   constexpr int NV = 4 + NS, NP = NV + 2;
-  return sizeof(double) * ((size_t)NP * RY * RX + (size_t)NV * (TX + 1) * TY +
-                           (size_t)NV * TX * (TY + 1) + (size_t)(TX + 1) * TY +
-                           (size_t)TX * (TY + 1)) +
-         2 * RY * RX + TX * TY;
+  return sizeof(double) * ((size_t)NP * RY * FRX + (size_t)NV * (FX + 1) * TY +
+                           (size_t)NV * FX * (TY + 1) + (size_t)(FX + 1) * TY +
+                           (size_t)FX * (TY + 1)) +
+         2 * RY * FRX + FX * TY;
 }
 
 template <int NS>
@@ -500,13 +500,13 @@ __global__ __launch_bounds__(NTHR) void k_flux(Params P, Grid G, int mode, doubl
   constexpr int NV = 4 + NS, NP = NV + 2;
   extern __shared__ double smem[];
   double* sq = smem;
-  double* sHx = sq + NP * RY * RX;
-  double* sHy = sHx + NV * (TX + 1) * TY;
-  double* sAx = sHy + NV * TX * (TY + 1);
-  double* sAy = sAx + (TX + 1) * TY;
-  signed char* ssc = (signed char*)(sAy + TX * (TY + 1));
-  unsigned char* ssd = (unsigned char*)(ssc + RY * RX);
-  unsigned char* sact = ssd + RY * RX;
+  double* sHx = sq + NP * RY * FRX;
+  double* sHy = sHx + NV * (FX + 1) * TY;
+  double* sAx = sHy + NV * FX * (TY + 1);
+  double* sAy = sAx + (FX + 1) * TY;
+  signed char* ssc = (signed char*)(sAy + FX * (TY + 1));
+  unsigned char* ssd = (unsigned char*)(ssc + RY * FRX);
+  unsigned char* sact = ssd + RY * FRX;
   __shared__ double red[NTHR / 64];
 
   const int tid = threadIdx.x;
@@ -516,13 +516,13 @@ __global__ __launch_bounds__(NTHR) void k_flux(Params P, Grid G, int mode, doubl
   const int* list = G.tile_list + (size_t)par * G.ntx * G.nty;
   const double t = (mode == 1) ? G.st->t : t_arg;
 
-  for (int it = blockIdx.x; it < n_list; it += gridDim.x) {
-    const int tile = list[it];
-    const int j0 = (tile % G.ntx) * TX, k0 = (tile / G.ntx) * TY;
+  for (int it = blockIdx.x; it < 2 * n_list; it += gridDim.x) {
+    const int tile = list[it >> 1];
+    const int j0 = (tile % G.ntx) * TX + (it & 1) * FX, k0 = (tile / G.ntx) * TY;
     __syncthreads();  // the previous tile's LDS reads are done
     // ---- primitive states of the tile and its two-cell halo ----------------
-    for (int e = tid; e < RX * RY; e += NTHR) {
-      const int lj = e % RX, lk = e / RX;
+    for (int e = tid; e < FRX * RY; e += NTHR) {
+      const int lj = e % FRX, lk = e / FRX;
       const int j = j0 + lj - HALO, k = k0 + lk - HALO;
       double qp[NP];
       signed char sc = 0;
@@ -546,27 +546,27 @@ __global__ __launch_bounds__(NTHR) void k_flux(Params P, Grid G, int mode, doubl
         }
       }
 #pragma unroll
-      for (int i = 0; i < NP; i++) sq[i * RX * RY + e] = qp[i];
+      for (int i = 0; i < NP; i++) sq[i * FRX * RY + e] = qp[i];
       ssc[e] = sc;
       ssd[e] = sd;
     }
-    for (int e = tid; e < TX * TY; e += NTHR) {
-      const int j = j0 + e % TX, k = k0 + e / TX;
+    for (int e = tid; e < FX * TY; e += NTHR) {
+      const int j = j0 + e % FX, k = k0 + e / FX;
       sact[e] = (j < nx && k < ny) ? G.mask[j + nx * k] : 0;
     }
     __syncthreads();
     TileView T1{sq, ssc, ssd, false, P.T_amb};
     TileView T0{sq, ssc, ssd, true, P.T_amb};
-    // ---- x faces: (TX+1) x TY ---------------------------------------------
-    for (int e = tid; e < (TX + 1) * TY; e += NTHR) {
-      const int fl = e % (TX + 1), fk = e / (TX + 1);
+    // ---- x faces: (FX+1) x TY ---------------------------------------------
+    for (int e = tid; e < (FX + 1) * TY; e += NTHR) {
+      const int fl = e % (FX + 1), fk = e / (FX + 1);
       const int jf = j0 + fl, k = k0 + fk;
       bool need = (k < ny) && (jf <= nx) &&
-                  ((fl > 0 && sact[fk * TX + fl - 1]) || (fl < TX && jf < nx && sact[fk * TX + fl]));
+                  ((fl > 0 && sact[fk * FX + fl - 1]) || (fl < FX && jf < nx && sact[fk * FX + fl]));
       // a face whose four stencil cells are dry has zero states on both
       // sides: H = 0 and no wave speed (the minmod slope of a dry cell is 0)
       if (need) {
-        const int b = (fk + HALO) * RX + fl + HALO;
+        const int b = (fk + HALO) * FRX + fl + HALO;
         need = (sq[b - 2] != 0.0) || (sq[b - 1] != 0.0) || (sq[b] != 0.0) || (sq[b + 1] != 0.0);
       }
       double H[NV];
@@ -575,7 +575,7 @@ __global__ __launch_bounds__(NTHR) void k_flux(Params P, Grid G, int mode, doubl
       for (int i = 0; i < NV; i++) H[i] = 0.0;
       if (need) {
         const int lk = fk + HALO, ljb = fl + HALO;
-        const int idxB = lk * RX + ljb, idxA = idxB - 1;
+        const int idxB = lk * FRX + ljb, idxA = idxB - 1;
         const bool hasA = jf - 1 >= 0, hasB = jf < nx;
         double qpA[NP], qpB[NP], qA[NV], qB[NV], an, ap;
         face_states<NS>(P, G, T1, 1, idxA, idxB, jf - 1, k, jf, k, hasA, hasB, qpA, qpB, qA, qB);
@@ -584,7 +584,7 @@ __global__ __launch_bounds__(NTHR) void k_flux(Params P, Grid G, int mode, doubl
         if (P.radial) {
           for (int d = -2; d <= 1; d++) {
             const int jj = jf + d;
-            if (jj >= 0 && jj < nx && ssc[lk * RX + ljb + d] == 1) disc = true;
+            if (jj >= 0 && jj < nx && ssc[lk * FRX + ljb + d] == 1) disc = true;
           }
         }
         if (disc) {
@@ -595,19 +595,19 @@ __global__ __launch_bounds__(NTHR) void k_flux(Params P, Grid G, int mode, doubl
         amax = fmax_f(ap, -an);
       }
 #pragma unroll
-      for (int i = 0; i < NV; i++) sHx[i * (TX + 1) * TY + e] = H[i];
+      for (int i = 0; i < NV; i++) sHx[i * (FX + 1) * TY + e] = H[i];
       sAx[e] = amax;
     }
-    // ---- y faces: TX x (TY+1) ---------------------------------------------
-    for (int e = tid; e < TX * (TY + 1); e += NTHR) {
-      const int fj = e % TX, fl = e / TX;
+    // ---- y faces: FX x (TY+1) ---------------------------------------------
+    for (int e = tid; e < FX * (TY + 1); e += NTHR) {
+      const int fj = e % FX, fl = e / FX;
       const int j = j0 + fj, kf = k0 + fl;
       bool need = (j < nx) && (kf <= ny) &&
-                  ((fl > 0 && sact[(fl - 1) * TX + fj]) || (fl < TY && kf < ny && sact[fl * TX + fj]));
+                  ((fl > 0 && sact[(fl - 1) * FX + fj]) || (fl < TY && kf < ny && sact[fl * FX + fj]));
       if (need) {
-        const int b = (fl + HALO) * RX + fj + HALO;
-        need = (sq[b - 2 * RX] != 0.0) || (sq[b - RX] != 0.0) || (sq[b] != 0.0) ||
-               (sq[b + RX] != 0.0);
+        const int b = (fl + HALO) * FRX + fj + HALO;
+        need = (sq[b - 2 * FRX] != 0.0) || (sq[b - FRX] != 0.0) || (sq[b] != 0.0) ||
+               (sq[b + FRX] != 0.0);
       }
       double H[NV];
       double amax = 0.0;
@@ -615,7 +615,7 @@ __global__ __launch_bounds__(NTHR) void k_flux(Params P, Grid G, int mode, doubl
       for (int i = 0; i < NV; i++) H[i] = 0.0;
       if (need) {
         const int lj = fj + HALO, lkb = fl + HALO;
-        const int idxB = lkb * RX + lj, idxA = idxB - RX;
+        const int idxB = lkb * FRX + lj, idxA = idxB - FRX;
         const bool hasA = kf - 1 >= 0, hasB = kf < ny;
         double qpA[NP], qpB[NP], qA[NV], qB[NV], an, ap;
         face_states<NS>(P, G, T1, 2, idxA, idxB, j, kf - 1, j, kf, hasA, hasB, qpA, qpB, qA, qB);
@@ -624,7 +624,7 @@ __global__ __launch_bounds__(NTHR) void k_flux(Params P, Grid G, int mode, doubl
         if (P.radial) {
           for (int d = -2; d <= 1; d++) {
             const int kk = kf + d;
-            if (kk >= 0 && kk < ny && ssc[(lkb + d) * RX + lj] == 1) disc = true;
+            if (kk >= 0 && kk < ny && ssc[(lkb + d) * FRX + lj] == 1) disc = true;
           }
         }
         if (disc) {
@@ -635,23 +635,23 @@ __global__ __launch_bounds__(NTHR) void k_flux(Params P, Grid G, int mode, doubl
         amax = fmax_f(ap, -an);
       }
 #pragma unroll
-      for (int i = 0; i < NV; i++) sHy[i * TX * (TY + 1) + e] = H[i];
+      for (int i = 0; i < NV; i++) sHy[i * FX * (TY + 1) + e] = H[i];
       sAy[e] = amax;
     }
     __syncthreads();
     // ---- cells: divergence, explicit terms, CFL candidate -----------------
     {
-      const int lj = tid % TX, lk = tid / TX;
+      const int lj = tid % FX, lk = tid / FX;
       const int j = j0 + lj, k = k0 + lk;
       bool work = false;
       const int c = j + nx * k;
-      if (j < nx && k < ny && sact[tid]) {
-        const int ew = lk * (TX + 1) + lj;
-        const int es = lk * TX + lj;
+      if (tid < FX * TY && j < nx && k < ny && sact[tid]) {
+        const int ew = lk * (FX + 1) + lj;
+        const int es = lk * FX + lj;
         double qp[NP], ex[NV];
-        const int idx = (lk + HALO) * RX + lj + HALO;
+        const int idx = (lk + HALO) * FRX + lj + HALO;
 #pragma unroll
-        for (int i = 0; i < NP; i++) qp[i] = sq[i * RX * RY + idx];
+        for (int i = 0; i < NP; i++) qp[i] = sq[i * FRX * RY + idx];
         const signed char sc = ssc[idx];
         double ap = 0.0, ax = 0.0, ay = 0.0;
         if (sc == 2) {
@@ -663,8 +663,8 @@ __global__ __launch_bounds__(NTHR) void k_flux(Params P, Grid G, int mode, doubl
 #pragma unroll
         for (int i = 0; i < NV; i++) {
           double d = 0.0;
-          d = d + (sHx[i * (TX + 1) * TY + ew + 1] - sHx[i * (TX + 1) * TY + ew]) * P.one_by_dx;
-          d = d + (sHy[i * TX * (TY + 1) + es + TX] - sHy[i * TX * (TY + 1) + es]) * P.one_by_dy;
+          d = d + (sHx[i * (FX + 1) * TY + ew + 1] - sHx[i * (FX + 1) * TY + ew]) * P.one_by_dx;
+          d = d + (sHy[i * FX * (TY + 1) + es + FX] - sHy[i * FX * (TY + 1) + es]) * P.one_by_dy;
           const double r = d - ex[i];
           G.R[(size_t)i * N + c] = r;
           work = work || (r != 0.0);
@@ -678,7 +678,7 @@ __global__ __launch_bounds__(NTHR) void k_flux(Params P, Grid G, int mode, doubl
         work = work || (P.T_amb < 273.0);
         double ma = fmax_f(sAx[ew], sAx[ew + 1]);
         const double dtx = (ma > 0.0) ? P.cfl * P.dx / ma : DINF;
-        ma = fmax_f(sAy[es], sAy[es + TX]);
+        ma = fmax_f(sAy[es], sAy[es + FX]);
         const double dty = (ma > 0.0) ? P.cfl * P.dy / ma : DINF;
         const double dtc = fmin_f(dtx, dty);
         my_dt = (dtc < my_dt) ? dtc : my_dt;
@@ -1473,6 +1473,7 @@ struct imex_handle {
   Grid G;
   hipStream_t stream = nullptr;
   int device = 0;
+  int n_cu = 256;  // compute units of the device (persistent grid sizes)
   int ntiles = 0;
   int par = 0;  // slot of the active-tile list filled by the latest k_mask
   unsigned int mask_seq = 0;  // k_mask launches so far (candidate-list slots)
@@ -1587,6 +1588,10 @@ extern "C" int imex_create(const imex_config* cfg, imex_handle** out) {
       *out = h;
       return -1;
     }
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->device) == hipSuccess &&
+        ncu > 0)
+      h->n_cu = ncu;
   }
   Params& P = h->P;
   memset(&P, 0, sizeof(P));
@@ -1749,8 +1754,8 @@ extern "C" int imex_set_static(imex_handle* h, const imex_static* s) {
 
 // Persistent grids: enough blocks to fill the 256 CUs at the kernels'
 // occupancy; each block walks the active-tile list with a stride of the grid.
-static int grid_for(const imex_handle* h, int per_cu) {
-  return std::max(1, std::min(h->ntiles, 256 * per_cu));
+static int grid_for(const imex_handle* h, int per_cu, int items = -1) {
+  return std::max(1, std::min(items < 0 ? h->ntiles : items, h->n_cu * per_cu));
 }
 
 template <int NS>
@@ -1794,7 +1799,7 @@ static int mask(imex_handle* h, int solve_all) {
   return 0;
 }
 static int flux(imex_handle* h, int mode, double t, int step_idx) {
-  const int nb = grid_for(h, 4);
+  const int nb = grid_for(h, 4, 2 * h->ntiles);  // work items are half tiles
   switch (h->P.ns) {
     case 1: hipLaunchKernelGGL(k_flux<1>, dim3(nb), dim3(NTHR), h->lds, h->stream, h->P, h->G, mode, t, h->par); break;
     case 2: hipLaunchKernelGGL(k_flux<2>, dim3(nb), dim3(NTHR), h->lds, h->stream, h->P, h->G, mode, t, h->par); break;

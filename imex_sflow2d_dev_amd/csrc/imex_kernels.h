@@ -8,8 +8,8 @@
 //             listed (hot tiles and their neighbours) -- every other tile
 //             holds only its frame band -- and lists the "hot" active tiles
 //             (a wet or source-ring cell within two cells) for k_flux.
-//   k_flux    persistent blocks over the hot-tile list; per TX x TY tile with
-//             a two-cell halo in LDS: primitive states -> MUSCL/minmod face
+//   k_flux    persistent blocks over the halves of the hot tiles; per
+//             FX x TY half tile with a two-cell halo in LDS: primitive states -> MUSCL/minmod face
 //             states -> local speeds -> Kurganov-Tadmor face fluxes -> flux
 //             divergence minus explicit terms (the stage-1 right-hand side),
 //             the CFL candidates reduced per block into one atomic minimum,
@@ -39,6 +39,10 @@ constexpr int HALO = 2;
 constexpr int RX = TX + 2 * HALO;
 constexpr int RY = TY + 2 * HALO;
 constexpr int NTHR = 256;
+// k_flux works on half tiles (FX x TY cells): the halo states, the x faces,
+// the y faces and the cells then each take one pass of the NTHR threads
+constexpr int FX = TX / 2;
+constexpr int FRX = FX + 2 * HALO;
 
 // device-side driver state (one per handle)
 struct DevState {
