@@ -425,6 +425,65 @@ __device__ __forceinline__ void face_states(const Params& P, const Grid& G, cons
   }
 }
 
+// The primitive face states of face_states alone (no conservative states),
+// for the timestep variant whose fluxes are not used, only its wave speeds.
+template <int NS>
+__device__ __forceinline__ void face_prims(const Params& P, const Grid& G, const TileView& T, int dir,
+                                           int idxA, int idxB, int ja, int ka, int jb, int kb,
+                                           bool hasA, bool hasB, double* qpA, double* qpB) {
+  constexpr int NV = 4 + NS, NP = NV + 2;
+  double tmp[NP];
+  if (hasA) {
+    if (T.ssc[idxA] == 1) {
+#pragma unroll
+      for (int i = 0; i < NP; i++) qpA[i] = 0.0;
+    } else if (dir == 1) {
+      recon_x<NS>(P, G, T, idxA, ja, ka, tmp, qpA);
+    } else {
+      recon_y<NS>(P, G, T, idxA, ja, ka, tmp, qpA);
+    }
+  }
+  if (hasB) {
+    if (T.ssc[idxB] == 1) {
+#pragma unroll
+      for (int i = 0; i < NP; i++) qpB[i] = 0.0;
+    } else if (dir == 1) {
+      recon_x<NS>(P, G, T, idxB, jb, kb, qpB, tmp);
+    } else {
+      recon_y<NS>(P, G, T, idxB, jb, kb, qpB, tmp);
+    }
+  }
+  const int vel = (dir == 1) ? NV : NV + 1;
+  const unsigned char bitA = (dir == 1) ? 1 : 4;
+  const unsigned char bitB = (dir == 1) ? 2 : 8;
+  if (!hasA) {
+#pragma unroll
+    for (int i = 0; i < NP; i++) qpA[i] = qpB[i];
+  } else if (!hasB) {
+#pragma unroll
+    for (int i = 0; i < NP; i++) qpB[i] = qpA[i];
+  } else if (T.ssc[idxA] == 2 && (T.ssd[idxA] & bitA)) {
+#pragma unroll
+    for (int i = 0; i < NP; i++) qpB[i] = qpA[i];
+    qpB[vel] = -qpA[vel];
+  } else if (T.ssc[idxB] == 2 && (T.ssd[idxB] & bitB)) {
+#pragma unroll
+    for (int i = 0; i < NP; i++) qpA[i] = qpB[i];
+    qpA[vel] = -qpB[vel];
+  }
+}
+
+// eval_speeds for one face (the a- / a+ of kt_flux)
+template <int NS>
+__device__ __forceinline__ void kt_speeds(const Params& P, int dir, const double* qpA,
+                                          const double* qpB, double& an, double& ap) {
+  double amin, amax, bmin, bmax;
+  local_speeds<NS>(P, qpA, dir, amin, amax);
+  local_speeds<NS>(P, qpB, dir, bmin, bmax);
+  an = fmin_f(fmin_f(amin, bmin), 0.0);
+  ap = fmax_f(fmax_f(amax, bmax), 0.0);
+}
+
 // eval_speeds + eval_flux_KT + average_KT for one face:
 // src/solver_2d.f90:3937-4014, :2834-3002, :3019-3049
 template <int NS>
@@ -587,10 +646,9 @@ __global__ __launch_bounds__(NTHR) void k_flux(Params P, Grid G, int mode, doubl
             if (jj >= 0 && jj < nx && ssc[lk * FRX + ljb + d] == 1) disc = true;
           }
         }
-        if (disc) {
-          double H0[NV];
-          face_states<NS>(P, G, T0, 1, idxA, idxB, jf - 1, k, jf, k, hasA, hasB, qpA, qpB, qA, qB);
-          kt_flux<NS>(P, 1, qpA, qpB, qA, qB, H0, an, ap);
+        if (disc) {  // timestep's wave speeds see T_ambient in the source disc
+          face_prims<NS>(P, G, T0, 1, idxA, idxB, jf - 1, k, jf, k, hasA, hasB, qpA, qpB);
+          kt_speeds<NS>(P, 1, qpA, qpB, an, ap);
         }
         amax = fmax_f(ap, -an);
       }
@@ -627,10 +685,9 @@ __global__ __launch_bounds__(NTHR) void k_flux(Params P, Grid G, int mode, doubl
             if (kk >= 0 && kk < ny && ssc[(lkb + d) * FRX + lj] == 1) disc = true;
           }
         }
-        if (disc) {
-          double H0[NV];
-          face_states<NS>(P, G, T0, 2, idxA, idxB, j, kf - 1, j, kf, hasA, hasB, qpA, qpB, qA, qB);
-          kt_flux<NS>(P, 2, qpA, qpB, qA, qB, H0, an, ap);
+        if (disc) {  // timestep's wave speeds see T_ambient in the source disc
+          face_prims<NS>(P, G, T0, 2, idxA, idxB, j, kf - 1, j, kf, hasA, hasB, qpA, qpB);
+          kt_speeds<NS>(P, 2, qpA, qpB, an, ap);
         }
         amax = fmax_f(ap, -an);
       }
