@@ -736,16 +736,29 @@ __device__ inline bool mass_exchange(const Params& P, const double* qp, double d
 #pragma unroll
   for (int i = 0; i < NS; i++) {
     sv[i] = 0.0;
-    const bool settles = (alphas[i] > 0.0) && P.settling;
-    any = any || settles;
-    if (settles && (LANES == 1 || (i & 1) == lane))
-      sv[i] = settling_velocity(P, P.diam_s[i], P.rho_s[i], m.rho_c, ikv);
+    any = any || ((alphas[i] > 0.0) && P.settling);
   }
-  if constexpr (LANES == 2) {
+  if constexpr (LANES == 1) {
 #pragma unroll
-    for (int i = 0; i < NS; i++) {
-      const double o = __shfl_xor(sv[i], 1);
-      if ((i & 1) != lane) sv[i] = o;
+    for (int i = 0; i < NS; i++)
+      if ((alphas[i] > 0.0) && P.settling)
+        sv[i] = settling_velocity(P, P.diam_s[i], P.rho_s[i], m.rho_c, ikv);
+  } else {
+    // classes i0 and i0 + 1 go through ONE call on the pair (each lane with
+    // its own class's constants), so the two fixed-point iterations run side
+    // by side rather than one after the other with half the lanes masked
+#pragma unroll
+    for (int i0 = 0; i0 < NS; i0 += 2) {
+      const bool two = i0 + 1 < NS;
+      const bool mine = (lane == 0) || two;
+      const double a = (lane == 0 || !two) ? alphas[i0] : alphas[two ? i0 + 1 : i0];
+      const double d = (lane == 0 || !two) ? P.diam_s[i0] : P.diam_s[two ? i0 + 1 : i0];
+      const double r = (lane == 0 || !two) ? P.rho_s[i0] : P.rho_s[two ? i0 + 1 : i0];
+      double s = 0.0;
+      if (mine && (a > 0.0) && P.settling) s = settling_velocity(P, d, r, m.rho_c, ikv);
+      const double o = __shfl_xor(s, 1);
+      sv[i0] = (lane == 0) ? s : o;
+      if (two) sv[i0 + 1] = (lane == 0) ? o : s;
     }
   }
   const double hindered = any ? pow_cr(1.0 - fmin_f(1.0, at / 0.6), 4.65) : 0.0;
