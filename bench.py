@@ -16,7 +16,8 @@ Metric: cell-stage updates/s = 2 * (active cells summed over the timed steps)
 
 The JSON line also carries
   roofline:     the dominant kernel's algorithmic HBM bytes per launch over
-                its mean launch time (HIP events on the launch stream), vs
+                its mean launch time (HIP events on the launch stream around
+                the launches of every E-th timed step, --event-every), vs
                 8 TB/s; traffic = measured HBM bytes per launch when a
                 rocprofv3 FETCH_SIZE/WRITE_SIZE summary of that kernel is
                 committed under profiles/, else null;
@@ -130,6 +131,9 @@ def main():
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the reference CPU sample")
+    ap.add_argument("--event-every", type=int, default=10, metavar="E",
+                    help="HIP events around the kernel launches of every E-th timed step "
+                         "(an event record costs the stream a few microseconds)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="N",
                     help="only run the plain copy kernel over N doubles 20 times (the "
                          "reference point for rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
@@ -173,7 +177,7 @@ def main():
     s = Solver(cfg, static, q0)
     total_cells_all = 0
     if args.warmup:
-        _, _, c = s.run_steps_timed(args.warmup, record_dt=False)
+        _, _, _, c = s.run_steps_timed(args.warmup, record_dt=False, sample_every=args.warmup)
         total_cells_all += c
     print(f"[bench] warmup {args.warmup} steps done, t={s.t:.4f}", file=sys.stderr, flush=True)
 
@@ -181,6 +185,7 @@ def main():
     # them (outside the clock) for the CPU sample
     halves = [args.steps // 2, args.steps - args.steps // 2]
     kernel_ms = np.zeros(len(KERNELS))
+    sampled = 0
     cells = 0
     wall = 0.0
     mid = None
@@ -192,11 +197,13 @@ def main():
             continue
         s_time(s)                               # synchronises the stream
         t0 = time.perf_counter()
-        dts, k, c = s.run_steps_timed(n)        # returns after a stream sync
+        # returns after a stream sync
+        dts, k, ns, c = s.run_steps_timed(n, sample_every=args.event_every)
         t1 = time.perf_counter()
         err_bits |= s.err
         wall += t1 - t0
         kernel_ms += k
+        sampled += ns
         cells += c
         dts_all.append(dts)
         print(f"[bench] timed part {i + 1}: {n} steps, {t1 - t0:.3f} s, "
@@ -217,7 +224,7 @@ def main():
     names = KERNELS
     dom = int(np.argmax(kernel_ms))
     dname = names[dom]
-    mean_launch_s = kernel_ms[dom] * 1e-3 / K
+    mean_launch_s = kernel_ms[dom] * 1e-3 / sampled
     cells_per_launch = cells / K
     alg_bytes = per[dname] * cells_per_launch
     achieved = alg_bytes / mean_launch_s / 1e9
@@ -229,6 +236,7 @@ def main():
         "kernel": dname,
         "alg_bytes_per_active_cell": per[dname],
         "mean_launch_us": mean_launch_s * 1e6,
+        "event_timed_launches": sampled,
         "mean_active_cells_per_launch": cells_per_launch,
     }
     step_gbs = per["step"] * cells / wall / 1e9
@@ -256,7 +264,8 @@ def main():
             "t_physical_end_s": t_end_phys,
             "step_hbm_alg_GBs": step_gbs,
             "step_alg_bytes_per_active_cell": per["step"],
-            "kernel_ms": dict(zip(names, [float(x) for x in kernel_ms])),
+            "kernel_ms_event_timed_steps": dict(zip(names, [float(x) for x in kernel_ms])),
+            "event_timed_steps": sampled,
             "physics_error_bits": err_bits,
         },
         "roofline": roofline,
@@ -265,7 +274,9 @@ def main():
     if args.info:
         with open(args.info, "w") as f:
             json.dump({"steps_total": args.warmup + K + 3, "active_cells_total": total_cells_all,
-                       "active_cells_timed": cells, "kernel_ms": out["config"]["kernel_ms"],
+                       "active_cells_timed": cells,
+                       "kernel_ms_event_timed_steps": out["config"]["kernel_ms_event_timed_steps"],
+                       "event_timed_steps": sampled,
                        "alg_bytes": per}, f, indent=1)
     if not args.no_cpu:
         print("[bench] CPU reference sample ...", file=sys.stderr, flush=True)

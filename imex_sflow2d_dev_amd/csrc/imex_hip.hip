@@ -2022,8 +2022,9 @@ extern "C" int imex_run_steps(imex_handle* h, int nsteps, double t_end, double* 
 }
 
 extern "C" int imex_run_steps_timed(imex_handle* h, int nsteps, double t_end, double* dts,
-                                    double* kernel_ms, int64_t* cells) {
-  if (!h || nsteps < 0) return -2;
+                                    int sample_every, double* kernel_ms, int* n_sampled,
+                                    int64_t* cells) {
+  if (!h || nsteps < 0 || sample_every < 1) return -2;
   HCHK(h, hipSetDevice(h->device));
   if (int rc = bind_dts(h, dts, nsteps)) return rc;
   DevState s;
@@ -2032,33 +2033,47 @@ extern "C" int imex_run_steps_timed(imex_handle* h, int nsteps, double t_end, do
   const unsigned long long act0 = s.active;
   HCHK(h, hipMemcpyAsync(h->G.st, &s, sizeof(s), hipMemcpyHostToDevice, h->stream));
   constexpr int NK = 4;  // k_mask, k_flux, k_update stage 2, k_update sedimentation + epilogue
-  std::vector<hipEvent_t> ev((size_t)nsteps * (NK + 1));
-  for (auto& e : ev) HCHK(h, hipEventCreate(&e));
+  const int ns = (nsteps + sample_every - 1) / sample_every;  // steps n with n % sample_every == 0
+  std::vector<hipEvent_t> ev((size_t)ns * (NK + 1), nullptr);
   int rc = 0;
+  for (auto& e : ev)
+    if (hipEventCreate(&e) != hipSuccess) rc = set_err(h, "hipEventCreate", hipErrorOutOfMemory);
+  // an event record costs a few microseconds of stream time: only the sampled
+  // steps carry them, the others run as imex_run_steps does
+  auto rec = [&](hipEvent_t* e, int k) {
+    if (e && !rc) {
+      const hipError_t x = hipEventRecord(e[k], h->stream);
+      if (x != hipSuccess) rc = set_err(h, "hipEventRecord", x);
+    }
+  };
   for (int n = 0; n < nsteps && !rc; n++) {
-    hipEvent_t* e = &ev[(size_t)(NK + 1) * n];
-    HCHK(h, hipEventRecord(e[0], h->stream));
-    if ((rc = mask(h, 0))) break;
-    HCHK(h, hipEventRecord(e[1], h->stream));
-    if ((rc = flux(h, 1, 0.0, n))) break;
-    HCHK(h, hipEventRecord(e[2], h->stream));
-    if ((rc = update(h, 1, 0.0, 1, n))) break;
-    HCHK(h, hipEventRecord(e[3], h->stream));
-    if ((rc = update(h, 2 | 4, 0.0, 1, n))) break;
-    HCHK(h, hipEventRecord(e[4], h->stream));
+    hipEvent_t* e = (n % sample_every == 0) ? &ev[(size_t)(NK + 1) * (n / sample_every)] : nullptr;
+    rec(e, 0);
+    if (!rc) rc = mask(h, 0);
+    rec(e, 1);
+    if (!rc) rc = flux(h, 1, 0.0, n);
+    rec(e, 2);
+    if (!rc) rc = update(h, 1, 0.0, 1, n);
+    rec(e, 3);
+    if (!rc) rc = update(h, 2 | 4, 0.0, 1, n);
+    rec(e, 4);
   }
-  HCHK(h, hipStreamSynchronize(h->stream));
+  const hipError_t sy = hipStreamSynchronize(h->stream);
+  if (!rc && sy != hipSuccess) rc = set_err(h, "hipStreamSynchronize", sy);
   double ms[NK] = {0, 0, 0, 0};
-  for (int n = 0; n < nsteps && !rc; n++)
-    for (int k = 0; k < NK; k++) {
+  for (int m = 0; m < ns && !rc; m++)
+    for (int k = 0; k < NK && !rc; k++) {
       float f = 0.f;
-      HCHK(h, hipEventElapsedTime(&f, ev[(size_t)(NK + 1) * n + k], ev[(size_t)(NK + 1) * n + k + 1]));
+      const hipError_t x = hipEventElapsedTime(&f, ev[(size_t)(NK + 1) * m + k], ev[(size_t)(NK + 1) * m + k + 1]);
+      if (x != hipSuccess) rc = set_err(h, "hipEventElapsedTime", x);
       ms[k] += f;
     }
-  for (auto& e : ev) (void)hipEventDestroy(e);
+  for (auto& e : ev)
+    if (e) (void)hipEventDestroy(e);
   if (rc) return rc;
   if (kernel_ms)
     for (int k = 0; k < NK; k++) kernel_ms[k] = ms[k];
+  if (n_sampled) *n_sampled = ns;
   if (dts && nsteps > 0)
     HCHK(h, hipMemcpyAsync(dts, h->G.dts, 8 * (size_t)nsteps, hipMemcpyDeviceToHost, h->stream));
   if (read_state(h, &s)) return -1;

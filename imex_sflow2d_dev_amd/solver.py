@@ -148,23 +148,28 @@ class Solver:
         self._raise(rc)
         return dts[:nsteps] if record_dt else None
 
-    def run_steps_timed(self, nsteps: int, t_end: float = None, record_dt: bool = True):
-        """run_steps with HIP events around every launch: returns (dts or
-        None, kernel_ms[4], active cells summed over the steps) -- the four
-        kernels are the active set, the fluxes, stage 2 and the
-        sedimentation + epilogue."""
+    def run_steps_timed(self, nsteps: int, t_end: float = None, record_dt: bool = True,
+                        sample_every: int = 1):
+        """run_steps with HIP events around the launches of every
+        sample_every-th step: returns (dts or None, kernel_ms[4] summed over
+        the sampled steps, number of sampled steps, active cells summed over
+        all steps) -- the four kernels are the active set, the fluxes, stage
+        2 and the sedimentation + epilogue."""
         t_end = self.cfg.t_end if t_end is None else t_end
         self._sync_time_to_device()
         dts = np.zeros(max(nsteps, 1)) if record_dt else None
         kms = np.zeros(4)
+        ns = ctypes.c_int()
         cells = np.zeros(1, np.int64)
         rc = self.L.imex_run_steps_timed(self.h, ctypes.c_int(nsteps), ctypes.c_double(t_end),
-                                         native._dp(dts) if record_dt else None, native._dp(kms),
+                                         native._dp(dts) if record_dt else None,
+                                         ctypes.c_int(sample_every), native._dp(kms),
+                                         ctypes.byref(ns),
                                          cells.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
         self.H.check(rc, "imex_run_steps_timed")
         self._pull_time()
         self._raise(rc)
-        return (dts[:nsteps] if record_dt else None), kms, int(cells[0])
+        return (dts[:nsteps] if record_dt else None), kms, ns.value, int(cells[0])
 
     def _sync_time_to_device(self):
         # the device-side driver keeps t and the dt history in device memory;

@@ -115,13 +115,16 @@ int imex_post_step(imex_handle *h, double t_new, imex_step_stats *out);
  * (src/IMEX_SfloW2D.f90:417-439), imex_RK_solver, sedimentation, t += dt,
  * epilogue.  dts (may be NULL) receives each step's dt at the end. */
 int imex_run_steps(imex_handle *h, int nsteps, double t_end, double *dts);
-/* imex_run_steps with HIP events around every kernel launch on the handle's
- * stream; kernel_ms[0..3] receive the summed durations over the nsteps steps
- * of the active-set kernel, the flux kernel, the stage-2 update kernel and
- * the sedimentation + epilogue kernel; cells[0] the active cells summed over
- * the steps. */
+/* imex_run_steps with HIP events around the kernel launches of every
+ * sample_every-th step (steps n with n % sample_every == 0; an event record
+ * costs the stream a few microseconds, so the other steps carry none) on the
+ * handle's stream: kernel_ms[0..3] receive the summed durations over the
+ * sampled steps of the active-set kernel, the flux kernel, the stage-2 update
+ * kernel and the sedimentation + epilogue kernel, n_sampled their number;
+ * cells[0] the active cells summed over all nsteps steps.  Has no
+ * counterpart in the reference (measurement only). */
 int imex_run_steps_timed(imex_handle *h, int nsteps, double t_end, double *dts,
-                         double *kernel_ms, int64_t *cells);
+                         int sample_every, double *kernel_ms, int *n_sampled, int64_t *cells);
 /* Byte-counter calibration: copies n_doubles doubles between two device
  * buffers reps times (reads and writes 8*n_doubles bytes per launch). */
 int imex_calib_copy(imex_handle *h, int64_t n_doubles, int reps);

@@ -84,10 +84,12 @@ def test_dt_output_after_a_timed_run_without_it():
     b.run_steps(40)
     k = np.zeros(4)
     c = np.zeros(1, np.int64)
-    rc = a.L.imex_run_steps_timed(a.h, 20, ctypes.c_double(cfg.t_end), None, native._dp(k),
+    ns = ctypes.c_int()
+    rc = a.L.imex_run_steps_timed(a.h, 20, ctypes.c_double(cfg.t_end), None, ctypes.c_int(3),
+                                  native._dp(k), ctypes.byref(ns),
                                   c.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
     assert rc == 0
-    assert (k > 0.0).all() and c[0] > 0
+    assert (k > 0.0).all() and c[0] > 0 and ns.value == 7
     b.run_steps(20)
     da = a.run_steps(30)
     db = b.run_steps(30)
