@@ -865,13 +865,27 @@ __device__ __forceinline__ double Lrc(int r, int c, const double* qorg, double L
   return (r == c) ? 1.0 * qorg[c] : 0.0;
 }
 
-// Returns true when rt and f_out are the residual at the returned x (the
-// caller then need not evaluate it again: eval_f is a pure function).
-template <int NS>
+// lnsrch, src/solver_2d.f90:2021-2226.  Returns true when rt and f_out are
+// the residual at the returned x (the caller then need not evaluate it
+// again: eval_f is a pure function).
+//
+// The reference's step sequence: alam_0 = 1; after a trial at alam that
+// neither accepts (f <= 0.9 f_old) nor ends the search (alam < alamin), the
+// next alam is MAX(tmplam, alam/2), where tmplam is the quadratic minimiser
+// when alam == 1 and otherwise a cubic minimiser that the reference first
+// caps at alam/2 -- so every trial after one at alam != 1 is at exactly half
+// the previous alam (a NaN tmplam loses both comparisons and gives alam/2
+// too).  The trials of a search are thus known in advance except after a
+// trial at alam == 1.  LANES = 2: the lanes of the pair evaluate the next two
+// trials of the sequence at once, the second assuming the halving; the pair
+// then walks the two results in order with the reference's rules and keeps
+// the second only when its alam is the one those rules give.  Extra
+// evaluations past the deciding trial are discarded.
+template <int NS, int LANES>
 __device__ bool lnsrch(const Params& P, double dt, const double* xold, const double* qorg,
                        const double* q0, double fold, const double* g, double* p, double* x,
                        double& f_out, double* rt, double stpmax, const double* Rni,
-                       unsigned int& err) {
+                       unsigned int& err, int lane) {
   constexpr int NV = 4 + NS;
   const double TOLX = kEps;
   double sum = 0.0;
@@ -897,61 +911,75 @@ __device__ bool lnsrch(const Params& P, double dt, const double* xold, const dou
     for (int i = 0; i < NV; i++) x[i] = xold[i];
     return true;  // rt and f_out still hold the residual at xold
   }
-  bool at_x = true;
-  double alam = 1.0, alam2 = 0.0, f2 = 0.0, tmplam = 0.0, f = 0.0;
-  double qj[NV];
+  // value of the pair's lane l (the own value when LANES == 1)
+  auto from = [&](double v, int l) {
+    if constexpr (LANES == 2) {
+      const double o = __shfl_xor(v, 1);
+      return (l == lane) ? v : o;
+    } else {
+      return v;
+    }
+  };
+  double base = 1.0;  // alam of the round's first trial
+  double xt[NV], qj[NV], rtt[NV], ft = 0.0;
   // The reference loop ends by construction: alam at least halves after the
   // first trial, so it drops below any positive alamin within
   // log2(DBL_MAX / DBL_TRUE_MIN) < 2100 trials.  Only a NaN alamin (a NaN
   // step) would loop for ever; the bound turns that into an error.
-  int trial;
-  for (trial = 0; trial < 2200; trial++) {
+  int trials = 0;
+  while (true) {
+    const double a = (lane == 0) ? base : 0.5 * base;
 #pragma unroll
-    for (int i = 0; i < NV; i++) x[i] = xold[i] + alam * p[i];
+    for (int i = 0; i < NV; i++) xt[i] = xold[i] + a * p[i];
 #pragma unroll
-    for (int i = 0; i < NV; i++) qj[i] = x[i] * qorg[i];
-    eval_f<NS>(P, dt, qj, q0, Rni, rt, f);
-    if (f <= 0.9 * fold) {
-      break;
-    } else if (alam < alamin) {
+    for (int i = 0; i < NV; i++) qj[i] = xt[i] * qorg[i];
+    eval_f<NS>(P, dt, qj, q0, Rni, rtt, ft);
+    double next = base;
+    int decided = -1;
+    bool accept = false;
+#pragma unroll
+    for (int l = 0; l < LANES; l++) {
+      const double al = from(a, l), fl = from(ft, l);
+      if (al != next) break;  // the halving guess of lane l does not hold
+      trials++;
+      if (fl <= 0.9 * fold) {
+        decided = l;
+        accept = true;
+        break;
+      }
+      if (al < alamin) {
+        decided = l;
+        break;
+      }
+      next = (al == 1.0) ? fmax_f(-(slope / (2.0 * (fl - fold - slope))), 0.5 * al) : 0.5 * al;
+    }
+    if (decided >= 0) {
+      const double al = from(a, decided);
+#pragma unroll
+      for (int i = 0; i < NV; i++) rt[i] = from(rtt[i], decided);
+      if (accept) {
+#pragma unroll
+        for (int i = 0; i < NV; i++) x[i] = xold[i] + al * p[i];
+        f_out = from(ft, decided);
+        return true;
+      }
 #pragma unroll
       for (int i = 0; i < NV; i++) x[i] = xold[i];
-      f = fold;
-      at_x = false;  // rt is the last trial's residual
-      break;
-    } else {
-      if (alam == 1.0) {
-        tmplam = -(slope / (2.0 * (f - fold - slope)));
-      } else {
-        const double rhs1 = f - fold - alam * slope;
-        const double rhs2 = f2 - fold - alam2 * slope;
-        const double a = (rhs1 / (alam * alam) - rhs2 / (alam2 * alam2)) / (alam - alam2);
-        const double b =
-            (-(alam2 * rhs1 / (alam * alam)) + alam * rhs2 / (alam2 * alam2)) / (alam - alam2);
-        if (a == 0.0) {
-          tmplam = -(slope / (2.0 * b));
-        } else {
-          const double disc = b * b - 3.0 * a * slope;
-          if (disc < 0.0)
-            tmplam = 0.5 * alam;
-          else if (b <= 0.0)
-            tmplam = (-b + sqrt(disc)) / (3.0 * a);
-          else
-            tmplam = -(slope / (b + sqrt(disc)));
-        }
-        if (tmplam > 0.5 * alam) tmplam = 0.5 * alam;
-      }
+      f_out = fold;
+      return false;  // rt is the last trial's residual
     }
-    alam2 = alam;
-    f2 = f;
-    alam = fmax_f(tmplam, 0.5 * alam);
+    if (trials >= 2200) {  // lane 0's trial of the round, so that the pair stays in step
+      err |= IMEX_ERR_NAN;
+#pragma unroll
+      for (int i = 0; i < NV; i++) {
+        x[i] = from(xt[i], 0);
+        rt[i] = from(rtt[i], 0);
+      }
+      f_out = from(ft, 0);
+      return false;
+    }
+    base = next;
   }
-  if (trial == 2200) {
-    err |= IMEX_ERR_NAN;
-    at_x = false;
-  }
-  f_out = f;
-  return at_x;
 }
 
 template <int NS, int LANES>
@@ -1025,8 +1053,8 @@ __device__ int solve_rk_step(const Params& P, double dt, double* qj, const doubl
         for (int r = 0; r < NV; r++) s += rt[r] * Lrc(r, i, qorg, L11, L21, L12, L22);
         grad[i] = s;
       }
-      fresh = lnsrch<NS>(P, dt, xold, qorg, q0, scal_f_old, grad, p, qrel, scal_f, rt, stpmax,
-                         Rni, err);
+      fresh = lnsrch<NS, LANES>(P, dt, xold, qorg, q0, scal_f_old, grad, p, qrel, scal_f, rt,
+                                stpmax, Rni, err, lane);
     } else {
 #pragma unroll
       for (int i = 0; i < NV; i++) qrel[i] = xold[i] + p[i];
