@@ -808,6 +808,11 @@ __device__ inline bool mass_exchange(const Params& P, const double* qp, double d
   return err;
 }
 
+// x / |x| as the correctly rounded quotient gives it
+__device__ __forceinline__ double sign_quot(double x) {
+  return (fabs(x) > 0.0 && fabs(x) < INFINITY) ? copysign(1.0, x) : __builtin_nan("");
+}
+
 // limit with the minmod limiter: src/geometry_2d.f90:1776-1806, :1855-1876
 __device__ __forceinline__ double limit_minmod(double v0, double v1, double v2, double z0,
                                                double z1, double z2) {
@@ -815,7 +820,9 @@ __device__ __forceinline__ double limit_minmod(double v0, double v1, double v2, 
   double b = (v1 - v0) / (z1 - z0);
   double m = fmin_f(fabs(a), fabs(b));
   if (m <= 0.0) return 0.0;
-  double sa = a / fabs(a), sb = b / fabs(b);
+  // a / |a| without the division: exactly +-1 for a finite non-zero a and
+  // NaN for 0, +-Inf and NaN (0/0, Inf/Inf, NaN), as the quotient is
+  double sa = sign_quot(a), sb = sign_quot(b);
   return 0.5 * (sa + sb) * m;
 }
 
