@@ -19,8 +19,8 @@ The JSON line also carries
                 its mean launch time (HIP events on the launch stream around
                 the launches of every E-th timed step, --event-every), vs
                 8 TB/s; traffic = measured HBM bytes per launch when a
-                rocprofv3 FETCH_SIZE/WRITE_SIZE summary of that kernel is
-                committed under profiles/, else null;
+                rocprofv3 FETCH_SIZE/WRITE_SIZE summary of the step's
+                kernels is committed under profiles/, else null;
   cpu_baseline: the reference OpenMP Fortran build (compiled from its own
                 sources into oracle/_ref) timed on this host's cores over a
                 bounded sample of steps that starts from the GPU state at the
@@ -77,13 +77,14 @@ def kernel_bytes_per_cell(cfg):
 
 
 def read_traffic(kernel: str, workload: str):
-    """Measured HBM bytes per active cell of one kernel, from the committed
-    rocprofv3 PMC summary profiles/<kernel>_<workload>_traffic.md (line
-    'bytes_per_active_cell: X'), or None."""
-    p = os.path.join(HERE, "profiles", f"{kernel}_{workload}_traffic.md")
+    """Measured HBM bytes per active cell of one kernel of the step, from the
+    committed rocprofv3 PMC summary profiles/<workload>_step_kernels_traffic.md
+    (line 'bytes_per_active_cell <kernel>: X'), or None."""
+    p = os.path.join(HERE, "profiles", f"{workload}_step_kernels_traffic.md")
     if not os.path.exists(p):
         return None
-    m = re.search(r"^bytes_per_active_cell:\s*([0-9.eE+-]+)", open(p).read(), re.M)
+    m = re.search(rf"^bytes_per_active_cell {re.escape(kernel)}:\s*([0-9.eE+-]+)", open(p).read(),
+                  re.M)
     return float(m.group(1)) if m else None
 
 
@@ -174,7 +175,10 @@ def main():
     s.run_steps(3)
     s.close()
 
+    t_up = time.perf_counter()                  # handle, static fields and q0 from host memory
     s = Solver(cfg, static, q0)
+    s_time(s)
+    t_up = time.perf_counter() - t_up
     total_cells_all = 0
     if args.warmup:
         _, _, _, c = s.run_steps_timed(args.warmup, record_dt=False, sample_every=args.warmup)
@@ -216,6 +220,10 @@ def main():
         mid = dict(q=s.q, t=s.t, dt_old=s.dt_old, dt_old_old=s.dt_old_old,
                    step=args.warmup + args.steps)
     t_end_phys = s.t
+    t_down = time.perf_counter()                # the state and the output fields to host memory
+    s.q
+    s.fields()
+    t_down = time.perf_counter() - t_down
     s.close()
 
     K = args.steps
@@ -267,6 +275,9 @@ def main():
             "kernel_ms_event_timed_steps": dict(zip(names, [float(x) for x in kernel_ms])),
             "event_timed_steps": sampled,
             "physics_error_bits": err_bits,
+            "host_upload_s": t_up,
+            "host_download_s": t_down,
+            "host_buffers_inclusive_cell_stage_updates_per_s": 2.0 * cells / (wall + t_up + t_down),
         },
         "roofline": roofline,
         "cpu_baseline": None,
