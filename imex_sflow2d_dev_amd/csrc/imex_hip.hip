@@ -74,35 +74,36 @@ __device__ __forceinline__ void mask_tile(const Params& P, const Grid& G, int ti
     G.mask[c] = m;
     act = m;
   }
-  if (act) {
+  // active cells of the tile: one shared-memory atomic per wavefront
+  const unsigned long long bal = __ballot(act != 0);
+  if ((tid & 63) == 0 && bal) {
     s_any = 1;
-    atomicAdd(&s_cnt, 1);
+    atomicAdd(&s_cnt, (int)__popcll(bal));
   }
   __syncthreads();
-  if (tid == 0) {
-    DevState* st = G.st;
+  // the tile's global atomics, each on a thread of its own so that their
+  // round trips overlap
+  DevState* st = G.st;
+  if (tid < 9 && s_hot) {
+    // a hot tile makes itself and its neighbours candidates of the next pass;
+    // the stamp lists each candidate once
+    const int ux = tile % G.ntx + tid % 3 - 1, uy = tile / G.ntx + tid / 3 - 1;
+    if (ux >= 0 && ux < G.ntx && uy >= 0 && uy < G.nty) {
+      const int u = ux + G.ntx * uy;
+      const unsigned int nslot = (seq + 1u) % 3u;
+      if (atomicExch(&G.cand_stamp[u], seq + 1u) != seq + 1u) {
+        const unsigned int slot = atomicAdd(&st->n_cand[nslot], 1u);
+        G.cand[(size_t)nslot * G.ntx * G.nty + slot] = u;
+      }
+    }
+  } else if (tid == 9 && s_any && s_hot) {
     // k_flux visits the tiles with active cells that have a wet or
     // source-ring cell within two cells: elsewhere every face stencil is dry,
     // so every flux and wave speed is zero, and the (dry, all-zero) cells get
     // a zero right-hand side that k_update does not need
-    if (s_any && s_hot) {
-      const unsigned int slot = atomicAdd(&st->n_tiles[par], 1u);
-      G.tile_list[(size_t)par * G.ntx * G.nty + slot] = tile;
-    }
-    if (s_hot) {
-      const int tx = tile % G.ntx, ty = tile / G.ntx;
-      const unsigned int nslot = (seq + 1u) % 3u;
-      for (int dy = -1; dy <= 1; dy++)
-        for (int dx = -1; dx <= 1; dx++) {
-          const int ux = tx + dx, uy = ty + dy;
-          if (ux < 0 || ux >= G.ntx || uy < 0 || uy >= G.nty) continue;
-          const int u = ux + G.ntx * uy;
-          if (atomicExch(&G.cand_stamp[u], seq + 1u) != seq + 1u) {
-            const unsigned int slot = atomicAdd(&st->n_cand[nslot], 1u);
-            G.cand[(size_t)nslot * G.ntx * G.nty + slot] = u;
-          }
-        }
-    }
+    const unsigned int slot = atomicAdd(&st->n_tiles[par], 1u);
+    G.tile_list[(size_t)par * G.ntx * G.nty + slot] = tile;
+  } else if (tid == 10) {
     // active cells of the tile beyond its frame band (counted by block 0)
     const long long extra = (long long)s_cnt - (long long)G.tile_static[tile];
     if (extra) atomicAdd(&st->active, (unsigned long long)extra);
@@ -361,25 +362,22 @@ __device__ __forceinline__ void recon_y(const Params& P, const Grid& G, const Ti
   }
 }
 
-// The two states of one face.  dir 1: face between local cells idxA (west) and
-// idxB (east), global cell columns ja = jf-1, jb = jf; dir 2: south/north.
-// A side owned by a source-disc cell is the ring mirror when the ring cell on
-// the other side faces the disc across this face, else the zero state
-// (a face state the reference never writes).
+// The two states of one face from the primitive side states of its cells.
+// dir 1: face between local cells idxA (west) and idxB (east); dir 2:
+// south/north.  qpA / qpB hold on entry the reconstructed side states (E of
+// A, W of B, or N of A, S of B) of the cells that exist (hasA / hasB); a side
+// owned by a source-disc cell is the ring mirror when the ring cell on the
+// other side faces the disc across this face, else the zero state (a face
+// state the reference never writes).
 template <int NS>
-__device__ __forceinline__ void face_states(const Params& P, const Grid& G, const TileView& T, int dir, int idxA,
-                            int idxB, int ja, int ka, int jb, int kb, bool hasA, bool hasB,
-                            double* qpA, double* qpB, double* qA, double* qB) {
+__device__ __forceinline__ void face_finish(const Params& P, const TileView& T, int dir, int idxA,
+                                            int idxB, bool hasA, bool hasB, double* qpA,
+                                            double* qpB, double* qA, double* qB) {
   constexpr int NV = 4 + NS, NP = NV + 2;
-  double tmp[NP];
   if (hasA) {
     if (T.ssc[idxA] == 1) {
 #pragma unroll
       for (int i = 0; i < NP; i++) qpA[i] = 0.0;
-    } else if (dir == 1) {
-      recon_x<NS>(P, G, T, idxA, ja, ka, tmp, qpA);
-    } else {
-      recon_y<NS>(P, G, T, idxA, ja, ka, tmp, qpA);
     }
     qp_to_qc<NS>(P, qpA, qA);
   }
@@ -387,10 +385,6 @@ __device__ __forceinline__ void face_states(const Params& P, const Grid& G, cons
     if (T.ssc[idxB] == 1) {
 #pragma unroll
       for (int i = 0; i < NP; i++) qpB[i] = 0.0;
-    } else if (dir == 1) {
-      recon_x<NS>(P, G, T, idxB, jb, kb, qpB, tmp);
-    } else {
-      recon_y<NS>(P, G, T, idxB, jb, kb, qpB, tmp);
     }
     qp_to_qc<NS>(P, qpB, qB);
   }
@@ -544,12 +538,18 @@ __device__ __forceinline__ void kt_flux(const Params& P, int dir, const double* 
 // mode 0: finisher stores dt_cfl only; mode 1: also the driver's clip and
 // growth limit (src/IMEX_SfloW2D.f90:417-439) into st->dt.
 // ===========================================================================
+// side states of the half tile's cells, computed once and read by both faces
+// of each cell: x (W, E) for columns -1..FX of the TY rows, y (S, N) for rows
+// -1..TY of the FX columns
+constexpr int NRX = (FX + 2) * TY;
+constexpr int NRY = FX * (TY + 2);
+
 template <int NS>
 constexpr size_t flux_lds_bytes() {
   constexpr int NV = 4 + NS, NP = NV + 2;
   return sizeof(double) * ((size_t)NP * RY * FRX + (size_t)NV * (FX + 1) * TY +
                            (size_t)NV * FX * (TY + 1) + (size_t)(FX + 1) * TY +
-                           (size_t)FX * (TY + 1)) +
+                           (size_t)FX * (TY + 1) + 2 * (size_t)NP * (NRX + NRY)) +
          2 * RY * FRX + FX * TY;
 }
 
@@ -563,7 +563,11 @@ __global__ __launch_bounds__(NTHR) void k_flux(Params P, Grid G, int mode, doubl
   double* sHy = sHx + NV * (FX + 1) * TY;
   double* sAx = sHy + NV * FX * (TY + 1);
   double* sAy = sAx + (FX + 1) * TY;
-  signed char* ssc = (signed char*)(sAy + FX * (TY + 1));
+  double* sRW = sAy + FX * (TY + 1);
+  double* sRE = sRW + NP * NRX;
+  double* sRS = sRE + NP * NRX;
+  double* sRN = sRS + NP * NRY;
+  signed char* ssc = (signed char*)(sRN + NP * NRY);
   unsigned char* ssd = (unsigned char*)(ssc + RY * FRX);
   unsigned char* sact = ssd + RY * FRX;
   __shared__ double red[NTHR / 64];
@@ -616,6 +620,38 @@ __global__ __launch_bounds__(NTHR) void k_flux(Params P, Grid G, int mode, doubl
     __syncthreads();
     TileView T1{sq, ssc, ssd, false, P.T_amb};
     TileView T0{sq, ssc, ssd, true, P.T_amb};
+    // ---- side states of the cells (cells of the source disc have none) -----
+    for (int e = tid; e < NRX + NRY; e += NTHR) {
+      if (e < NRX) {
+        const int lj = e % (FX + 2) - 1, lk = e / (FX + 2);
+        const int j = j0 + lj, k = k0 + lk;
+        const int idx = (lk + HALO) * FRX + lj + HALO;
+        if (j >= 0 && j < nx && k < ny && ssc[idx] != 1) {
+          double W[NP], E[NP];
+          recon_x<NS>(P, G, T1, idx, j, k, W, E);
+#pragma unroll
+          for (int i = 0; i < NP; i++) {
+            sRW[i * NRX + e] = W[i];
+            sRE[i * NRX + e] = E[i];
+          }
+        }
+      } else {
+        const int e2 = e - NRX;
+        const int lj = e2 % FX, lk = e2 / FX - 1;
+        const int j = j0 + lj, k = k0 + lk;
+        const int idx = (lk + HALO) * FRX + lj + HALO;
+        if (k >= 0 && k < ny && j < nx && ssc[idx] != 1) {
+          double S[NP], Nn[NP];
+          recon_y<NS>(P, G, T1, idx, j, k, S, Nn);
+#pragma unroll
+          for (int i = 0; i < NP; i++) {
+            sRS[i * NRY + e2] = S[i];
+            sRN[i * NRY + e2] = Nn[i];
+          }
+        }
+      }
+    }
+    __syncthreads();
     // ---- x faces: (FX+1) x TY ---------------------------------------------
     for (int e = tid; e < (FX + 1) * TY; e += NTHR) {
       const int fl = e % (FX + 1), fk = e / (FX + 1);
@@ -636,8 +672,14 @@ __global__ __launch_bounds__(NTHR) void k_flux(Params P, Grid G, int mode, doubl
         const int lk = fk + HALO, ljb = fl + HALO;
         const int idxB = lk * FRX + ljb, idxA = idxB - 1;
         const bool hasA = jf - 1 >= 0, hasB = jf < nx;
+        const int ra = fk * (FX + 2) + fl;  // side states of cell A; B is ra + 1
         double qpA[NP], qpB[NP], qA[NV], qB[NV], an, ap;
-        face_states<NS>(P, G, T1, 1, idxA, idxB, jf - 1, k, jf, k, hasA, hasB, qpA, qpB, qA, qB);
+#pragma unroll
+        for (int i = 0; i < NP; i++) {
+          qpA[i] = sRE[i * NRX + ra];
+          qpB[i] = sRW[i * NRX + ra + 1];
+        }
+        face_finish<NS>(P, T1, 1, idxA, idxB, hasA, hasB, qpA, qpB, qA, qB);
         kt_flux<NS>(P, 1, qpA, qpB, qA, qB, H, an, ap);
         bool disc = false;
         if (P.radial) {
@@ -675,8 +717,14 @@ __global__ __launch_bounds__(NTHR) void k_flux(Params P, Grid G, int mode, doubl
         const int lj = fj + HALO, lkb = fl + HALO;
         const int idxB = lkb * FRX + lj, idxA = idxB - FRX;
         const bool hasA = kf - 1 >= 0, hasB = kf < ny;
+        const int ra = fl * FX + fj;  // side states of cell A; B is ra + FX
         double qpA[NP], qpB[NP], qA[NV], qB[NV], an, ap;
-        face_states<NS>(P, G, T1, 2, idxA, idxB, j, kf - 1, j, kf, hasA, hasB, qpA, qpB, qA, qB);
+#pragma unroll
+        for (int i = 0; i < NP; i++) {
+          qpA[i] = sRN[i * NRY + ra];
+          qpB[i] = sRS[i * NRY + ra + FX];
+        }
+        face_finish<NS>(P, T1, 2, idxA, idxB, hasA, hasB, qpA, qpB, qA, qB);
         kt_flux<NS>(P, 2, qpA, qpB, qA, qB, H, an, ap);
         bool disc = false;
         if (P.radial) {
@@ -1803,6 +1851,16 @@ extern "C" int imex_create(const imex_config* cfg, imex_handle** out) {
   }
   if ((rc = frame_band(h, nullptr))) return rc;
   h->lds = lds_for(ns);
+  {
+    int max_lds = 0;
+    HCHK(h, hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device));
+    if ((size_t)max_lds < h->lds) {
+      h->err = "k_flux needs " + std::to_string(h->lds) + " bytes of LDS per block, the device allows " +
+               std::to_string(max_lds);
+      *out = h;
+      return -1;
+    }
+  }
   switch (ns) {
     case 1: HCHK(h, hipFuncSetAttribute((const void*)k_flux<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds)); break;
     case 2: HCHK(h, hipFuncSetAttribute((const void*)k_flux<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds)); break;
