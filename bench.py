@@ -46,9 +46,9 @@ sys.path.insert(0, HERE)
 HBM_PEAK_GBS = 8000.0
 
 
-# the four kernels of a device-side driver step, in launch order, as
+# the three kernels of a device-side driver step, in launch order, as
 # imex_run_steps_timed times them (rocprofv3 names in brackets)
-KERNELS = ["k_mask", "k_flux", "k_update_stage2", "k_update_settle_epilogue"]
+KERNELS = ["k_mask", "k_flux", "k_update"]
 
 
 def kernel_bytes_per_cell(cfg):
@@ -57,21 +57,17 @@ def kernel_bytes_per_cell(cfg):
 
     k_mask [k_mask]:          read q(1), write the mask byte        9
     k_flux [k_flux<NS>]:      read q (n_v), bed slopes B'_x, B'_y   8 (n_v + 2)
-    k_update_stage2 [k_update<NS,1>]:
-                              read and write q                      16 n_v
-    k_update_settle_epilogue [k_update<NS,6>]:
-                              deposit read and write with
-                              settling (16 n_s), hmax / pdynmax /
-                              mod_vel_max read and write (48)       16 n_s S + 48
+    k_update [k_update<NS,7>]: read and write q (16 n_v), deposit
+                              read and write with settling
+                              (16 n_s), hmax / pdynmax /
+                              mod_vel_max read and write (48)       16 n_v + 16 n_s S + 48
     Intermediates that a fused single pass would not need (the flux
-    divergence handed from k_flux to the stage-2 kernel, q read again by the
-    sedimentation kernel, the tile and work lists) are not counted: they
-    show up in the measured traffic instead.
+    divergence handed from k_flux to k_update, the tile and work lists) are
+    not counted: they show up in the measured traffic instead.
     """
     nv, ns = cfg.n_vars, cfg.n_solid
     s = 1 if cfg.settling_flag else 0
-    per = {"k_mask": 9, "k_flux": 8 * (nv + 2), "k_update_stage2": 16 * nv,
-           "k_update_settle_epilogue": 16 * ns * s + 48}
+    per = {"k_mask": 9, "k_flux": 8 * (nv + 2), "k_update": 16 * nv + 16 * ns * s + 48}
     per["step"] = sum(per.values())
     return per
 

@@ -115,9 +115,24 @@ __device__ __forceinline__ void mask_tile(const Params& P, const Grid& G, int ti
 // tile outside the candidates had no wet or ring cell within ten cells at the
 // previous step, so none within two now (fluxes move mass two cells at most
 // per step): its mask bytes are the frame band's and stay as stored.
+// The end of a device-driven step: dt history, t += dt, CFL minimum reset.
+// The update launch of the step leaves it pending; the next launch on the
+// stream, which no block of that update launch can overtake, applies it.
+__device__ __forceinline__ void commit_step(DevState* st) {
+  if (st->pending) {
+    st->dt_old_old = st->dt_old;
+    st->dt_old = st->dt;
+    st->t = st->t + st->dt;
+    st->dtmin_bits = 0x7ff0000000000000ULL;
+    st->pending = 0u;
+  }
+}
+__global__ void k_commit(Grid G) { commit_step(G.st); }
+
 __global__ __launch_bounds__(NTHR) void k_mask(Params P, Grid G, int solve_all, int par,
                                                unsigned int seq, int full) {
   if (blockIdx.x == 0 && threadIdx.x == 0) {
+    commit_step(G.st);
     // slots no block of this launch touches
     G.st->n_tiles[par ^ 1] = 0u;            // the next step's k_flux list
     G.st->n_work = 0u;                      // filled by the k_flux that follows
@@ -1360,6 +1375,9 @@ __global__ __launch_bounds__(NTHR) void k_update(Params P, Grid G, double dt_arg
         G.st->dt = dt;
         G.st->dt_cfl = fmin_f(P.max_dt, __longlong_as_double((long long)G.st->dtmin_bits));
         if (G.dts) G.dts[step_idx] = dt;
+        // the whole step in this launch: other blocks may still be reading
+        // t and the dt history, so the commit waits for the next launch
+        if constexpr ((PARTS & 4) != 0) G.st->pending = 1u;
       }
     } else {
       dt = G.st->dt;
@@ -1419,7 +1437,7 @@ __global__ __launch_bounds__(NTHR) void k_update(Params P, Grid G, double dt_arg
   }
   if (err && lead) atomicOr(&G.st->err, err);
   if (dried) atomicAdd(&G.st->dried, (unsigned long long)dried);
-  if ((PARTS & 4) && mode == 1 && blockIdx.x == 0 && threadIdx.x == 0) {
+  if ((PARTS & 4) && !(PARTS & 1) && mode == 1 && blockIdx.x == 0 && threadIdx.x == 0) {
     DevState* st = G.st;
     st->dt_old_old = st->dt_old;
     st->dt_old = dt;
@@ -1913,21 +1931,25 @@ static void launch_update(imex_handle* h, int parts, double dt, int mode, int st
     case 2: hipLaunchKernelGGL((k_update<NS, 2>), g, b, 0, s, P, G, dt, mode, step_idx); break;
     case 4: hipLaunchKernelGGL((k_update<NS, 4>), g, b, 0, s, P, G, dt, mode, step_idx); break;
     case 6: hipLaunchKernelGGL((k_update<NS, 6>), g, b, 0, s, P, G, dt, mode, step_idx); break;
+    case 7: hipLaunchKernelGGL((k_update<NS, 7>), g, b, 0, s, P, G, dt, mode, step_idx); break;
     default: hipLaunchKernelGGL((k_update<NS, 8>), g, b, 0, s, P, G, dt, mode, step_idx); break;
   }
 }
 // parts: 1, 2, 4, 8 as the host-driven entry points call them, or 1|2|4 for
 // a device-driven step (mode 1), which runs as two launches (1, then 2|4).
 static int update(imex_handle* h, int parts, double dt, int mode, int step_idx = 0) {
-  const int seq[2] = {(parts == 7) ? 1 : parts, (parts == 7) ? 6 : 0};
-  for (int k = 0; k < 2 && seq[k]; k++) {
-    switch (h->P.ns) {
-      case 1: launch_update<1>(h, seq[k], dt, mode, step_idx); break;
-      case 2: launch_update<2>(h, seq[k], dt, mode, step_idx); break;
-      default: launch_update<3>(h, seq[k], dt, mode, step_idx); break;
-    }
-    HCHK(h, hipGetLastError());
+  switch (h->P.ns) {
+    case 1: launch_update<1>(h, parts, dt, mode, step_idx); break;
+    case 2: launch_update<2>(h, parts, dt, mode, step_idx); break;
+    default: launch_update<3>(h, parts, dt, mode, step_idx); break;
   }
+  HCHK(h, hipGetLastError());
+  return 0;
+}
+// applies a pending end of step (after the last device-driven step of a call)
+static int commit(imex_handle* h) {
+  hipLaunchKernelGGL(k_commit, dim3(1), dim3(1), 0, h->stream, h->G);
+  HCHK(h, hipGetLastError());
   return 0;
 }
 static int mask(imex_handle* h, int solve_all) {
@@ -2100,6 +2122,7 @@ extern "C" int imex_run_steps(imex_handle* h, int nsteps, double t_end, double* 
     if ((rc = flux(h, 1, 0.0, n))) return rc;
     if ((rc = update(h, 1 | 2 | 4, 0.0, 1, n))) return rc;
   }
+  if (nsteps > 0 && (rc = commit(h))) return rc;
   if (dts && nsteps > 0)
     HCHK(h, hipMemcpyAsync(dts, h->G.dts, 8 * (size_t)nsteps, hipMemcpyDeviceToHost, h->stream));
   unsigned err = 0;
@@ -2118,7 +2141,7 @@ extern "C" int imex_run_steps_timed(imex_handle* h, int nsteps, double t_end, do
   s.t_end = t_end;
   const unsigned long long act0 = s.active;
   HCHK(h, hipMemcpyAsync(h->G.st, &s, sizeof(s), hipMemcpyHostToDevice, h->stream));
-  constexpr int NK = 4;  // k_mask, k_flux, k_update stage 2, k_update sedimentation + epilogue
+  constexpr int NK = 3;  // k_mask, k_flux, k_update
   const int ns = (nsteps + sample_every - 1) / sample_every;  // steps n with n % sample_every == 0
   std::vector<hipEvent_t> ev((size_t)ns * (NK + 1), nullptr);
   int rc = 0;
@@ -2139,14 +2162,13 @@ extern "C" int imex_run_steps_timed(imex_handle* h, int nsteps, double t_end, do
     rec(e, 1);
     if (!rc) rc = flux(h, 1, 0.0, n);
     rec(e, 2);
-    if (!rc) rc = update(h, 1, 0.0, 1, n);
+    if (!rc) rc = update(h, 1 | 2 | 4, 0.0, 1, n);
     rec(e, 3);
-    if (!rc) rc = update(h, 2 | 4, 0.0, 1, n);
-    rec(e, 4);
   }
+  if (!rc && nsteps > 0) rc = commit(h);
   const hipError_t sy = hipStreamSynchronize(h->stream);
   if (!rc && sy != hipSuccess) rc = set_err(h, "hipStreamSynchronize", sy);
-  double ms[NK] = {0, 0, 0, 0};
+  double ms[NK] = {0, 0, 0};
   for (int m = 0; m < ns && !rc; m++)
     for (int k = 0; k < NK && !rc; k++) {
       float f = 0.f;

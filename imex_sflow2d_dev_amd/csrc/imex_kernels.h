@@ -16,14 +16,18 @@
 //             and the list of work cells (active cells whose state or
 //             right-hand side is not all zero).
 //   k_dt      one thread, host-driven steps only: the CFL time step.
-//   k_update  persistent blocks over the work cells, two launches per step:
-//             <1> stage-2 semi-implicit Coulomb friction, Newton solve of the
-//             turbulent friction (complex-step Jacobian columns on a lane
-//             pair, 2x2 Cramer, backtracking line search), assembly checks;
-//             in the device-side driver also the step's dt (clip to t_end,
-//             growth limit);  <6> sedimentation / entrainment (settling
-//             classes on a lane pair) and the driver epilogue (running
-//             maxima, vulnerability table), then t += dt.
+//   k_update  persistent blocks over the work cells, an aligned lane pair
+//             per cell: stage-2 semi-implicit Coulomb friction, Newton solve
+//             of the turbulent friction (complex-step Jacobian columns and
+//             line-search trials on the pair, 2x2 Cramer), assembly checks,
+//             sedimentation / entrainment (settling classes on the pair) and
+//             the driver epilogue (running maxima, vulnerability table).  The
+//             device-side driver runs all of it in one launch <7> and takes
+//             the step's dt (clip to t_end, growth limit) at its start; the
+//             next launch (k_mask, or k_commit after the last step) commits
+//             the dt history and t += dt.  Host-driven steps launch the parts
+//             <1>, <2>, <4> from imex_rk_step, imex_erosion_deposition and
+//             imex_post_step.
 // Lists are double- or triple-buffered by pass parity so that no kernel waits
 // on another through device-scope fences: each launch only clears counters of
 // slots no block of that launch reads or fills.
@@ -52,7 +56,7 @@ struct DevState {
   unsigned int err;
   unsigned int n_work;      // cells k_flux listed for k_update
   unsigned int n_cand[3];   // candidate tiles of k_mask passes, three slots
-  unsigned int pad;
+  unsigned int pending;      // 1: the last step's dt history and t are still to commit
   unsigned long long dried;
   unsigned long long active;
 };

@@ -151,14 +151,14 @@ class Solver:
     def run_steps_timed(self, nsteps: int, t_end: float = None, record_dt: bool = True,
                         sample_every: int = 1):
         """run_steps with HIP events around the launches of every
-        sample_every-th step: returns (dts or None, kernel_ms[4] summed over
+        sample_every-th step: returns (dts or None, kernel_ms[3] summed over
         the sampled steps, number of sampled steps, active cells summed over
-        all steps) -- the four kernels are the active set, the fluxes, stage
-        2 and the sedimentation + epilogue."""
+        all steps) -- the three kernels are the active set, the fluxes and
+        the update (stage 2, sedimentation, epilogue)."""
         t_end = self.cfg.t_end if t_end is None else t_end
         self._sync_time_to_device()
         dts = np.zeros(max(nsteps, 1)) if record_dt else None
-        kms = np.zeros(4)
+        kms = np.zeros(3)
         ns = ctypes.c_int()
         cells = np.zeros(1, np.int64)
         rc = self.L.imex_run_steps_timed(self.h, ctypes.c_int(nsteps), ctypes.c_double(t_end),

@@ -118,9 +118,9 @@ int imex_run_steps(imex_handle *h, int nsteps, double t_end, double *dts);
 /* imex_run_steps with HIP events around the kernel launches of every
  * sample_every-th step (steps n with n % sample_every == 0; an event record
  * costs the stream a few microseconds, so the other steps carry none) on the
- * handle's stream: kernel_ms[0..3] receive the summed durations over the
- * sampled steps of the active-set kernel, the flux kernel, the stage-2 update
- * kernel and the sedimentation + epilogue kernel, n_sampled their number;
+ * handle's stream: kernel_ms[0..2] receive the summed durations over the
+ * sampled steps of the active-set kernel, the flux kernel and the update
+ * kernel (stage 2, sedimentation, epilogue), n_sampled their number;
  * cells[0] the active cells summed over all nsteps steps.  Has no
  * counterpart in the reference (measurement only). */
 int imex_run_steps_timed(imex_handle *h, int nsteps, double t_end, double *dts,

@@ -22,5 +22,5 @@ def test_algorithmic_bytes_per_active_cell():
     per = bench.kernel_bytes_per_cell(c2)
     assert per["step"] == 249
     assert per["k_flux"] == 64 and per["k_mask"] == 9
-    assert per["k_update_stage2"] == 96 and per["k_update_settle_epilogue"] == 80
+    assert per["k_update"] == 176
     assert set(per) == set(bench.KERNELS) | {"step"}

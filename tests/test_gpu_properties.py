@@ -82,7 +82,7 @@ def test_dt_output_after_a_timed_run_without_it():
     b = Solver(cfg, st, q0)
     a.run_steps(40)
     b.run_steps(40)
-    k = np.zeros(4)
+    k = np.zeros(3)
     c = np.zeros(1, np.int64)
     ns = ctypes.c_int()
     rc = a.L.imex_run_steps_timed(a.h, 20, ctypes.c_double(cfg.t_end), None, ctypes.c_int(3),
