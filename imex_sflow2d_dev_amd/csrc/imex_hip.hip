@@ -118,6 +118,8 @@ __device__ __forceinline__ void mask_tile(const Params& P, const Grid& G, int ti
 // The end of a device-driven step: dt history, t += dt, CFL minimum reset.
 // The update launch of the step leaves it pending; the next launch on the
 // stream, which no block of that update launch can overtake, applies it.
+// A member of a batch stops once t reaches its t_end (the driver loop's
+// condition t < t_end, src/IMEX_SfloW2D.f90:395).
 __device__ __forceinline__ void commit_step(DevState* st) {
   if (st->pending) {
     st->dt_old_old = st->dt_old;
@@ -125,12 +127,13 @@ __device__ __forceinline__ void commit_step(DevState* st) {
     st->t = st->t + st->dt;
     st->dtmin_bits = 0x7ff0000000000000ULL;
     st->pending = 0u;
+    st->done = (st->t < st->t_end) ? 0u : 1u;
   }
 }
 __global__ void k_commit(Grid G) { commit_step(G.st); }
 
-__global__ __launch_bounds__(NTHR) void k_mask(Params P, Grid G, int solve_all, int par,
-                                               unsigned int seq, int full) {
+__device__ __forceinline__ void mask_body(const Params& P, const Grid& G, int solve_all, int par,
+                                          unsigned int seq, int full) {
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     commit_step(G.st);
     // slots no block of this launch touches
@@ -141,13 +144,18 @@ __global__ __launch_bounds__(NTHR) void k_mask(Params P, Grid G, int solve_all, 
     atomicAdd(&G.st->active, G.static_active);
   }
   if (full) {
-    mask_tile(P, G, xcd_tile(blockIdx.x, gridDim.x), solve_all, par, seq);
+    const int nt = G.ntx * G.nty;
+    for (int b = blockIdx.x; b < nt; b += gridDim.x) mask_tile(P, G, xcd_tile(b, nt), solve_all, par, seq);
   } else {
     const unsigned int slot = seq % 3u;
     const int n = (int)G.st->n_cand[slot];
     const int* list = G.cand + (size_t)slot * G.ntx * G.nty;
     for (int it = blockIdx.x; it < n; it += gridDim.x) mask_tile(P, G, list[it], solve_all, par, seq);
   }
+}
+__global__ __launch_bounds__(NTHR) void k_mask(Params P, Grid G, int solve_all, int par,
+                                               unsigned int seq, int full) {
+  mask_body(P, G, solve_all, par, seq, full);
 }
 
 // ===========================================================================
@@ -569,8 +577,8 @@ constexpr size_t flux_lds_bytes() {
 }
 
 template <int NS>
-__global__ __launch_bounds__(NTHR) void k_flux(Params P, Grid G, int mode, double t_arg,
-                                               int par) {
+__device__ __forceinline__ void flux_body(const Params& P, const Grid& G, int mode, double t_arg,
+                                          int par) {
   constexpr int NV = 4 + NS, NP = NV + 2;
   extern __shared__ double smem[];
   double* sq = smem;
@@ -827,6 +835,12 @@ __global__ __launch_bounds__(NTHR) void k_flux(Params P, Grid G, int mode, doubl
     for (int w = 1; w < NTHR / 64; w++) b = (red[w] < b) ? red[w] : b;
     if (b < DINF) atomicMin(&G.st->dtmin_bits, (unsigned long long)__double_as_longlong(b));
   }
+}
+
+template <int NS>
+__global__ __launch_bounds__(NTHR) void k_flux(Params P, Grid G, int mode, double t_arg,
+                                               int par) {
+  flux_body<NS>(P, G, mode, t_arg, par);
 }
 
 // The CFL time step from the blocks' minimum (timestep's result,
@@ -1355,8 +1369,8 @@ __device__ void epilogue(const Params& P, const double* q, double& hmax, double&
 // (every thread the same, driver_dt) and publishes it in st->dt; the launch
 // with the epilogue ends the step: dt history, t += dt, CFL minimum reset.
 template <int NS, int PARTS>
-__global__ __launch_bounds__(NTHR) void k_update(Params P, Grid G, double dt_arg, int mode,
-                                                 int step_idx) {
+__device__ __forceinline__ void update_body(const Params& P, const Grid& G, double dt_arg, int mode,
+                                            int step_idx) {
   constexpr int NV = 4 + NS;
   constexpr bool ALL = (PARTS & 8) != 0;
   // lanes per cell: the stage-2 launch evaluates the two Jacobian columns of
@@ -1444,6 +1458,12 @@ __global__ __launch_bounds__(NTHR) void k_update(Params P, Grid G, double dt_arg
     st->t = st->t + dt;
     st->dtmin_bits = 0x7ff0000000000000ULL;
   }
+}
+
+template <int NS, int PARTS>
+__global__ __launch_bounds__(NTHR) void k_update(Params P, Grid G, double dt_arg, int mode,
+                                                 int step_idx) {
+  update_body<NS, PARTS>(P, G, dt_arg, mode, step_idx);
 }
 
 // ===========================================================================
@@ -2010,6 +2030,8 @@ extern "C" int imex_upload_state(imex_handle* h, const double* q_aos, double t, 
   s.dt_old_old = dt0;
   s.dtmin_bits = 0x7ff0000000000000ULL;
   s.err = 0;
+  s.pending = 0;
+  s.done = 0;
   HCHK(h, hipMemcpyAsync(h->G.st, &s, sizeof(s), hipMemcpyHostToDevice, h->stream));
   int rc;
   if ((rc = mask(h, 1))) return rc;
@@ -2187,6 +2209,278 @@ extern "C" int imex_run_steps_timed(imex_handle* h, int nsteps, double t_end, do
   if (read_state(h, &s)) return -1;
   if (cells) cells[0] = (int64_t)(s.active - act0);
   return (int)(s.err & ~IMEX_WARN_RESCALE);
+}
+
+// ===========================================================================
+// Batches: independent runs (ensemble members, served requests) advanced
+// together.  One launch per kernel and step covers every member: blockIdx.y
+// selects the member, whose constants and buffers come from a table in
+// device memory, and the member's blocks walk its own lists exactly as the
+// single-run kernels do, so a member's arithmetic -- and result -- is the
+// same alone and in a batch.  Each member keeps its own dt, t and t_end; a
+// member whose t has reached t_end does nothing.  A batch step is k_mask,
+// k_flux, k_update<7> and a commit launch that closes the step of every
+// member (dt history, t += dt, the done flag) before the next step's
+// launches read them.
+// ===========================================================================
+struct Member {
+  Params P;
+  Grid G;
+};
+
+__global__ __launch_bounds__(NTHR) void kb_mask(const Member* M, int par, unsigned int seq) {
+  const Member& m = M[blockIdx.y];
+  const DevState* st = m.G.st;
+  if (st->done) return;
+  mask_body(m.P, m.G, 0, par, seq, st->full_next ? 1 : 0);
+}
+template <int NS>
+__global__ __launch_bounds__(NTHR) void kb_flux(const Member* M, int par) {
+  const Member& m = M[blockIdx.y];
+  if (m.G.st->done) return;
+  flux_body<NS>(m.P, m.G, 1, 0.0, par);
+}
+template <int NS>
+__global__ __launch_bounds__(NTHR) void kb_update(const Member* M) {
+  const Member& m = M[blockIdx.y];
+  if (m.G.st->done) return;
+  update_body<NS, 7>(m.P, m.G, 0.0, 1, 0);
+}
+__global__ void kb_commit(const Member* M, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    DevState* st = M[i].G.st;
+    if (!st->done) {
+      commit_step(st);
+      st->full_next = 0u;
+    }
+  }
+}
+__global__ void kb_status(const Member* M, int n, unsigned int* done, double* t) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    done[i] = M[i].G.st->done;
+    t[i] = M[i].G.st->t;
+  }
+}
+
+struct imex_batch {
+  int ns = 0;
+  int device = 0;
+  int n_cu = 256;
+  int gx = 64;                       // blocks per member and launch
+  hipStream_t stream = nullptr;
+  std::vector<imex_handle*> mem;
+  Member* d_mem = nullptr;
+  int cap = 0;
+  bool dirty = true;
+  int par = 0;
+  unsigned int seq = 0;
+  size_t lds = 0;
+  unsigned int* d_done = nullptr;
+  double* d_t = nullptr;
+  int status_cap = 0;
+  std::string err;
+};
+
+static int bset_err(imex_batch* b, const std::string& what, hipError_t e) {
+  b->err = what + ": " + hipGetErrorString(e);
+  return -1;
+}
+#define BCHK(b, call)                                   \
+  do {                                                  \
+    hipError_t e_ = (call);                             \
+    if (e_ != hipSuccess) return bset_err((b), #call, e_); \
+  } while (0)
+
+// Empties a run's tile, candidate and work lists and their stamps, so that
+// they can follow another sequence of pass numbers (joining or leaving a
+// batch); the next active-set pass visits every tile.  Sets t_end and the
+// batch flags (done, full pass next).
+static int reset_lists(imex_handle* h, double t_end, unsigned int full_next) {
+  HCHK(h, hipMemsetAsync(h->G.cand_stamp, 0, sizeof(unsigned int) * (size_t)h->ntiles, h->stream));
+  DevState s;
+  if (read_state(h, &s)) return -1;
+  s.n_tiles[0] = s.n_tiles[1] = 0u;
+  s.n_cand[0] = s.n_cand[1] = s.n_cand[2] = 0u;
+  s.n_work = 0u;
+  s.t_end = t_end;
+  s.done = (s.t < t_end) ? 0u : 1u;
+  s.full_next = full_next;
+  HCHK(h, hipMemcpyAsync(h->G.st, &s, sizeof(s), hipMemcpyHostToDevice, h->stream));
+  HCHK(h, hipStreamSynchronize(h->stream));
+  h->mask_full_next = true;
+  h->mask_seq = 0;
+  return 0;
+}
+
+extern "C" int imex_batch_create(int n_solid, imex_batch** out) {
+  if (!out || n_solid < 1 || n_solid > 3) return -2;
+  imex_batch* b = new imex_batch();
+  *out = b;
+  b->ns = n_solid;
+  const char* dev = getenv("IMEX_DEVICE");
+  b->device = dev ? atoi(dev) : 0;
+  BCHK(b, hipSetDevice(b->device));
+  BCHK(b, hipStreamCreate(&b->stream));
+  int ncu = 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, b->device) == hipSuccess && ncu > 0)
+    b->n_cu = ncu;
+  b->lds = lds_for(n_solid);
+  switch (n_solid) {
+    case 1: BCHK(b, hipFuncSetAttribute((const void*)kb_flux<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b->lds)); break;
+    case 2: BCHK(b, hipFuncSetAttribute((const void*)kb_flux<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b->lds)); break;
+    default: BCHK(b, hipFuncSetAttribute((const void*)kb_flux<3>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b->lds)); break;
+  }
+  return 0;
+}
+
+extern "C" int imex_batch_set_blocks(imex_batch* b, int blocks_per_member) {
+  if (!b || blocks_per_member < 1 || blocks_per_member > 4096) return -2;
+  b->gx = blocks_per_member;
+  return 0;
+}
+
+extern "C" int imex_batch_add(imex_batch* b, imex_handle* h, double t_end) {
+  if (!b || !h || h->P.ns != b->ns || h->device != b->device) return -2;
+  for (imex_handle* x : b->mem)
+    if (x == h) return -2;
+  // the member's own stream is idle from here on; its state joins the batch
+  // with a full active-set pass at the batch's next step
+  if (commit(h)) {
+    b->err = h->err;
+    return -1;
+  }
+  if (reset_lists(h, t_end, 1u)) {
+    b->err = h->err;
+    return -1;
+  }
+  b->mem.push_back(h);
+  b->dirty = true;
+  return 0;
+}
+
+extern "C" int imex_batch_remove(imex_batch* b, imex_handle* h) {
+  if (!b || !h) return -2;
+  for (size_t i = 0; i < b->mem.size(); i++)
+    if (b->mem[i] == h) {
+      BCHK(b, hipStreamSynchronize(b->stream));
+      b->mem.erase(b->mem.begin() + (long)i);
+      b->dirty = true;
+      // back to its own pass counters: fresh lists, a full pass first
+      DevState s;
+      if (read_state(h, &s) || reset_lists(h, s.t_end, 0u)) {
+        b->err = h->err;
+        return -1;
+      }
+      return 0;
+    }
+  return -2;
+}
+
+static int batch_table(imex_batch* b) {
+  if (!b->dirty) return 0;
+  const int n = (int)b->mem.size();
+  if (n > b->cap) {
+    BCHK(b, hipStreamSynchronize(b->stream));
+    if (b->d_mem) BCHK(b, hipFree(b->d_mem));
+    b->d_mem = nullptr;
+    const int cap = std::max(n, 2 * b->cap);
+    BCHK(b, hipMalloc(&b->d_mem, sizeof(Member) * (size_t)cap));
+    b->cap = cap;
+  }
+  std::vector<Member> host((size_t)n);
+  for (int i = 0; i < n; i++) {
+    host[i].P = b->mem[i]->P;
+    host[i].G = b->mem[i]->G;
+    host[i].G.dts = nullptr;
+  }
+  if (n > 0) {
+    BCHK(b, hipMemcpyAsync(b->d_mem, host.data(), sizeof(Member) * (size_t)n, hipMemcpyHostToDevice, b->stream));
+    BCHK(b, hipStreamSynchronize(b->stream));  // the host vector goes out of scope
+  }
+  b->dirty = false;
+  return 0;
+}
+
+// nsteps batch steps; members whose t reaches t_end stop on the way.
+extern "C" int imex_batch_run(imex_batch* b, int nsteps) {
+  if (!b || nsteps < 0) return -2;
+  BCHK(b, hipSetDevice(b->device));
+  if (int rc = batch_table(b)) return rc;
+  const int n = (int)b->mem.size();
+  if (n == 0 || nsteps == 0) return 0;
+  if (n > 65535) return -2;
+  const dim3 g(b->gx, n), blk(NTHR);
+  for (int k = 0; k < nsteps; k++) {
+    b->par ^= 1;
+    b->seq++;
+    hipLaunchKernelGGL(kb_mask, g, blk, 0, b->stream, b->d_mem, b->par, b->seq);
+    switch (b->ns) {
+      case 1:
+        hipLaunchKernelGGL(kb_flux<1>, g, blk, b->lds, b->stream, b->d_mem, b->par);
+        hipLaunchKernelGGL(kb_update<1>, g, blk, 0, b->stream, b->d_mem);
+        break;
+      case 2:
+        hipLaunchKernelGGL(kb_flux<2>, g, blk, b->lds, b->stream, b->d_mem, b->par);
+        hipLaunchKernelGGL(kb_update<2>, g, blk, 0, b->stream, b->d_mem);
+        break;
+      default:
+        hipLaunchKernelGGL(kb_flux<3>, g, blk, b->lds, b->stream, b->d_mem, b->par);
+        hipLaunchKernelGGL(kb_update<3>, g, blk, 0, b->stream, b->d_mem);
+        break;
+    }
+    hipLaunchKernelGGL(kb_commit, dim3((n + 63) / 64), dim3(64), 0, b->stream, b->d_mem, n);
+    BCHK(b, hipGetLastError());
+  }
+  return 0;
+}
+
+// done flag and t of every member, in the order they were added (members
+// removed since are not listed); synchronises the batch stream
+extern "C" int imex_batch_status(imex_batch* b, int n, int32_t* done, double* t) {
+  if (!b || n != (int)b->mem.size()) return -2;
+  if (n == 0) return 0;
+  BCHK(b, hipSetDevice(b->device));
+  if (int rc = batch_table(b)) return rc;
+  if (n > b->status_cap) {
+    BCHK(b, hipStreamSynchronize(b->stream));
+    if (b->d_done) BCHK(b, hipFree(b->d_done));
+    if (b->d_t) BCHK(b, hipFree(b->d_t));
+    b->d_done = nullptr;
+    b->d_t = nullptr;
+    BCHK(b, hipMalloc(&b->d_done, sizeof(unsigned int) * (size_t)n));
+    BCHK(b, hipMalloc(&b->d_t, sizeof(double) * (size_t)n));
+    b->status_cap = n;
+  }
+  hipLaunchKernelGGL(kb_status, dim3((n + 63) / 64), dim3(64), 0, b->stream, b->d_mem, n, b->d_done, b->d_t);
+  BCHK(b, hipGetLastError());
+  std::vector<unsigned int> dn((size_t)n);
+  BCHK(b, hipMemcpyAsync(dn.data(), b->d_done, sizeof(unsigned int) * (size_t)n, hipMemcpyDeviceToHost, b->stream));
+  if (t) BCHK(b, hipMemcpyAsync(t, b->d_t, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, b->stream));
+  BCHK(b, hipStreamSynchronize(b->stream));
+  if (done)
+    for (int i = 0; i < n; i++) done[i] = (int32_t)dn[i];
+  return 0;
+}
+
+extern "C" int imex_batch_last_error(imex_batch* b, char* buf, size_t n) {
+  if (!b || !buf || n == 0) return -2;
+  const size_t k = std::min(n - 1, b->err.size());
+  memcpy(buf, b->err.data(), k);
+  buf[k] = 0;
+  return 0;
+}
+
+extern "C" void imex_batch_destroy(imex_batch* b) {
+  if (!b) return;
+  (void)hipSetDevice(b->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->d_mem) (void)hipFree(b->d_mem);
+  if (b->d_done) (void)hipFree(b->d_done);
+  if (b->d_t) (void)hipFree(b->d_t);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
 }
 
 __global__ void k_copy(const double* __restrict__ a, double* __restrict__ b, long long n) {

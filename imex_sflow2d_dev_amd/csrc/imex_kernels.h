@@ -57,6 +57,8 @@ struct DevState {
   unsigned int n_work;      // cells k_flux listed for k_update
   unsigned int n_cand[3];   // candidate tiles of k_mask passes, three slots
   unsigned int pending;      // 1: the last step's dt history and t are still to commit
+  unsigned int done;         // batch member: t reached t_end, its kernels do nothing
+  unsigned int full_next;    // batch member: the next active-set pass visits every tile
   unsigned long long dried;
   unsigned long long active;
 };

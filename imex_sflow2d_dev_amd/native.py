@@ -34,6 +34,8 @@ EXPORTS = [
     "imex_timestep", "imex_rk_step", "imex_erosion_deposition", "imex_post_step",
     "imex_run_steps", "imex_run_steps_timed", "imex_calib_copy", "imex_get_time",
     "imex_get_time_history", "imex_set_time", "imex_download", "imex_last_error", "imex_destroy",
+    "imex_batch_create", "imex_batch_set_blocks", "imex_batch_add", "imex_batch_remove",
+    "imex_batch_run", "imex_batch_status", "imex_batch_last_error", "imex_batch_destroy",
     "imex_test_qc_to_qp", "imex_test_qp_to_qc", "imex_test_fluxes", "imex_test_speeds",
     "imex_test_implicit_cplx", "imex_test_implicit_real", "imex_test_semi_impl", "imex_test_expl",
     "imex_test_settling", "imex_test_mass_exchange", "imex_test_newton", "imex_test_arith",
@@ -91,7 +93,7 @@ def lib():
         L = ctypes.CDLL(LIB_PATH)
         for name in EXPORTS:
             f = getattr(L, name)
-            f.restype = None if name == "imex_destroy" else ctypes.c_int
+            f.restype = None if name in ("imex_destroy", "imex_batch_destroy") else ctypes.c_int
         L.imex_create.argtypes = [ctypes.POINTER(ImexConfig), ctypes.POINTER(ctypes.c_void_p)]
         _LIB = L
     return _LIB

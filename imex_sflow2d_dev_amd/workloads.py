@@ -116,6 +116,46 @@ def config2(nx: int = 2048, ny: int = 2048, dx: float = 5.0, seed_member: int = 
     return cfg, np.ascontiguousarray(B)
 
 
+def config4_member(member: int, min_side: int = 128, max_side: int = 1024,
+                   t_end: float = 30.0):
+    """One member of the batched ensemble: a grid side log-uniform in
+    [min_side, max_side] (dx = 10 m), a seeded fractal DEM (exponent -2.5,
+    RMS 10 m) on a 5-degree slope, and per member an initial volume
+    U[1e5, 1e7] m3 realised as a cylinder of thickness 20 m (radius
+    sqrt(V / (pi 20))) at rest upslope, mu U[0.05, 0.3], xi U[200, 2000],
+    initial T U[400, 900] K and particle fraction U[0.01, 0.1]; the config-1
+    physics otherwise (one class, no settling).  Each member runs to t_end.
+    Returns (cfg, B[k, j])."""
+    rng = _rng(4, member)
+    n = int(round(math.exp(rng.uniform(math.log(min_side), math.log(max_side)))))
+    dx = 10.0
+    cfg = RunConfig(nx=n, ny=n, dx=dx, name=f"C4m{member}")
+    xc, yc, _, _ = cell_centres(cfg)
+    lx = n * dx
+    X, _ = np.meshgrid(xc, yc)
+    frac = fractal_surface(n, dx, -2.5, 10.0, rng)
+    B = 50.0 + (lx - X) * math.tan(math.radians(5.0)) + frac
+    B = np.maximum(B, 0.0)
+    vol = rng.uniform(1.0e5, 1.0e7)
+    cfg.h_collapse = 20.0
+    cfg.r_collapse = min(math.sqrt(vol / (math.pi * cfg.h_collapse)), 0.2 * lx)
+    # off the cell faces' symmetry lines (see config2)
+    cfg.x_collapse = 0.3 * lx + 0.31 * dx
+    cfg.y_collapse = 0.5 * lx - 0.43 * dx
+    cfg.mu = rng.uniform(0.05, 0.3)
+    cfg.xi = rng.uniform(200.0, 2000.0)
+    cfg.T_collapse = rng.uniform(400.0, 900.0)
+    cfg.alphas_collapse = [rng.uniform(0.01, 0.1)]
+    cfg.t_end = t_end
+    cfg.validate()
+    return cfg, np.ascontiguousarray(B)
+
+
+def config4(n_members: int = 256, **kw):
+    """The batched ensemble: [(cfg, B), ...] for members 0..n_members-1."""
+    return [config4_member(m, **kw) for m in range(n_members)]
+
+
 def small_config2(nx: int = 96, ny: int = 96, dx: float = 20.0):
     """A CPU-sized version of the production workload (same physics and
     source, coarser and smaller grid) for parity tests against the oracle."""

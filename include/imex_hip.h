@@ -146,6 +146,31 @@ int imex_download(imex_handle *h, unsigned what, double *q_aos, double *hmax,
 int imex_last_error(imex_handle *h, char *buf, size_t len);
 void imex_destroy(imex_handle *h);
 
+/* ---- batches of independent runs -----------------------------------------
+ * No single counterpart in the reference: the reference runs an ensemble as
+ * independent processes, one driver loop each (EXAMPLES/
+ * EXAMPLE_ENSEMBLE_ETNA/launch_jobs.py).  A batch advances every member run
+ * (a handle, created, given its static fields and uploaded as above) by
+ * device-driven driver steps (imex_run_steps) in one launch per kernel for
+ * all members; each member keeps its own dt, t and t_end and stops once t
+ * reaches t_end.  A member's results equal those of imex_run_steps on the
+ * same handle.  All members have the same n_solid and device. */
+typedef struct imex_batch imex_batch;
+int imex_batch_create(int n_solid, imex_batch **b);
+/* blocks per member and launch (default 64) */
+int imex_batch_set_blocks(imex_batch *b, int blocks_per_member);
+/* the handle joins at the next step and runs until its t reaches t_end; the
+ * handle's own calls must not be used while it is a member */
+int imex_batch_add(imex_batch *b, imex_handle *h, double t_end);
+int imex_batch_remove(imex_batch *b, imex_handle *h);
+/* nsteps steps of every member that has not reached its t_end (asynchronous) */
+int imex_batch_run(imex_batch *b, int nsteps);
+/* per member, in the order added: done flag (t reached t_end) and t;
+ * n = number of members; synchronises */
+int imex_batch_status(imex_batch *b, int n, int32_t *done, double *t);
+int imex_batch_last_error(imex_batch *b, char *buf, size_t n);
+void imex_batch_destroy(imex_batch *b);
+
 /* Per-function batch entry points (device evaluation of single reference
  * routines on n host-provided states; for known-answer tests).  States are
  * n contiguous vectors of n_vars (qc) or n_vars+2 (qp) values. */

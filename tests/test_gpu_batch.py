@@ -1,0 +1,98 @@
+"""Batches of independent runs (imex_batch_*): every member's result is the
+same, bit for bit, as the same run alone through the single-run device
+driver -- members of different grid sizes and parameters, one that joins
+the batch after it started, and two-class members with settling and the
+radial source."""
+import numpy as np
+import pytest
+
+from imex_sflow2d_dev_amd import init, workloads
+from imex_sflow2d_dev_amd.ensemble import Batch, run_alone
+from imex_sflow2d_dev_amd.solver import Solver
+
+pytestmark = pytest.mark.gpu
+
+
+def _bits(a, b, what):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    same = (a.view(np.int64) == b.view(np.int64)) | (np.isnan(a) & np.isnan(b))
+    assert same.all(), f"{what}: {int((~same).sum())} mismatches"
+
+
+def _same_run(s, r, what):
+    assert s.t == r.t, what
+    _bits(s.q, r.q, what + " q")
+    fs, fr = s.fields(), r.fields()
+    for k in ("hmax", "pdynmax", "mod_vel_max", "deposit"):
+        _bits(fs[k], fr[k], f"{what} {k}")
+    assert np.array_equal(s.mask(), r.mask()), what + " active set"
+
+
+def _members(n, t_end, sides=(40, 90)):
+    out = []
+    for m in range(n):
+        cfg, B = workloads.config4_member(m, min_side=sides[0], max_side=sides[1], t_end=t_end)
+        st, q0 = init.setup(cfg, B)
+        out.append((cfg, st, q0))
+    return out
+
+
+def test_members_match_runs_alone():
+    mem = _members(6, t_end=4.0)
+    alone = []
+    for cfg, st, q0 in mem:
+        s = Solver(cfg, st, q0)
+        run_alone(s, cfg.t_end)
+        alone.append(s)
+    b = Batch(n_solid=1, blocks_per_member=8)
+    inb = []
+    for cfg, st, q0 in mem:
+        s = Solver(cfg, st, q0)
+        b.add(s, cfg.t_end)
+        inb.append(s)
+    steps = b.run_to_end(chunk=25)
+    done, t = b.status()
+    assert done.all() and steps > 0
+    for s in list(inb):
+        b.remove(s)
+    for i, (s, r) in enumerate(zip(inb, alone)):
+        assert s.t >= mem[i][0].t_end
+        _same_run(s, r, f"member {i}")
+    b.close()
+
+
+def test_member_joining_later_matches_alone():
+    mem = _members(3, t_end=3.0)
+    b = Batch(n_solid=1, blocks_per_member=16)
+    first = [Solver(c, st, q0) for c, st, q0 in mem[:2]]
+    for s, (c, _, _) in zip(first, mem[:2]):
+        b.add(s, c.t_end)
+    b.run(40)
+    late = Solver(*mem[2])
+    b.add(late, mem[2][0].t_end)
+    b.run_to_end(chunk=20)
+    b.remove(late)
+    ref = Solver(*mem[2])
+    run_alone(ref, mem[2][0].t_end)
+    _same_run(late, ref, "late member")
+    b.close()
+
+
+def test_two_class_radial_source_members():
+    runs = []
+    for nx, dx in ((64, 25.0), (80, 20.0)):
+        cfg, B = workloads.small_config2(nx=nx, ny=nx, dx=dx)
+        cfg.t_end = 6.0
+        st, q0 = init.setup(cfg, B)
+        runs.append((cfg, st, q0))
+    b = Batch(n_solid=2, blocks_per_member=8)
+    inb = [Solver(*r) for r in runs]
+    for s, r in zip(inb, runs):
+        b.add(s, r[0].t_end)
+    b.run_to_end(chunk=30)
+    for s, r in zip(list(inb), runs):
+        b.remove(s)
+        ref = Solver(*r)
+        run_alone(ref, r[0].t_end)
+        _same_run(s, ref, f"two-class {r[0].nx}")
+    b.close()
