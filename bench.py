@@ -125,7 +125,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5000)
     ap.add_argument("--warmup", type=int, default=0)
-    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--workload", default="c2",
+                    help="c2 (default), c1, c2-small, c3, or c4 (the batched ensemble)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the reference CPU sample")
     ap.add_argument("--event-every", type=int, default=10, metavar="E",
@@ -134,6 +135,8 @@ def main():
     ap.add_argument("--calib-copy", type=int, default=0, metavar="N",
                     help="only run the plain copy kernel over N doubles 20 times (the "
                          "reference point for rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
+    ap.add_argument("--members", type=int, default=256, help="c4: ensemble members")
+    ap.add_argument("--t-end", type=float, default=30.0, help="c4: physical end time of a member (s)")
     ap.add_argument("--info", default=None,
                     help="also write per-kernel launch counts and active cells to this JSON file")
     args = ap.parse_args()
@@ -158,6 +161,10 @@ def main():
         print(json.dumps({"calib_copy_doubles": args.calib_copy, "launches": 20,
                           "bytes_read_per_launch": 8 * args.calib_copy,
                           "bytes_written_per_launch": 8 * args.calib_copy}), flush=True)
+        return
+
+    if args.workload == "c4":
+        bench_c4(args)
         return
 
     t_host = time.perf_counter()
@@ -288,6 +295,31 @@ def main():
     if not args.no_cpu:
         print("[bench] CPU reference sample ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(args.workload, mid, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+def bench_c4(args):
+    """The batched ensemble (256 members of 128^2..1024^2 cells, each to its
+    own t_end) against the same runs one after the other on the same GPU."""
+    from imex_sflow2d_dev_amd import ensemble
+    r = ensemble.bench_ensemble(n_members=args.members, t_end=args.t_end,
+                                log=lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True))
+    out = {
+        "metric": "ensemble runs/hr",
+        "value": r["runs_per_hour_batched"],
+        "unit": "runs/hr",
+        "n_gpus": 1,
+        "steps": r["member_steps_total"],
+        "warmup": 0,
+        "ms_per_step": None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded PCG64 fractal DEMs and member parameters, generated in-process)",
+        "config": dict({"workload": (f"c4: {r['members']} members, sides log-uniform in 128..1024 "
+                                     f"cells (dx 10 m), each to t_end = {r['t_end_s']} s, FP64")}, **r),
+    }
     print(json.dumps(out), flush=True)
 
 

@@ -128,6 +128,7 @@ __device__ __forceinline__ void commit_step(DevState* st) {
     st->dtmin_bits = 0x7ff0000000000000ULL;
     st->pending = 0u;
     st->done = (st->t < st->t_end) ? 0u : 1u;
+    st->steps = st->steps + 1u;
   }
 }
 __global__ void k_commit(Grid G) { commit_step(G.st); }
@@ -2032,6 +2033,7 @@ extern "C" int imex_upload_state(imex_handle* h, const double* q_aos, double t, 
   s.err = 0;
   s.pending = 0;
   s.done = 0;
+  s.steps = 0;
   HCHK(h, hipMemcpyAsync(h->G.st, &s, sizeof(s), hipMemcpyHostToDevice, h->stream));
   int rc;
   if ((rc = mask(h, 1))) return rc;
@@ -2259,7 +2261,8 @@ __global__ void kb_commit(const Member* M, int n) {
 __global__ void kb_status(const Member* M, int n, unsigned int* done, double* t) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
-    done[i] = M[i].G.st->done;
+    done[2 * i] = M[i].G.st->done;
+    done[2 * i + 1] = M[i].G.st->steps;
     t[i] = M[i].G.st->t;
   }
 }
@@ -2438,7 +2441,7 @@ extern "C" int imex_batch_run(imex_batch* b, int nsteps) {
 
 // done flag and t of every member, in the order they were added (members
 // removed since are not listed); synchronises the batch stream
-extern "C" int imex_batch_status(imex_batch* b, int n, int32_t* done, double* t) {
+extern "C" int imex_batch_status(imex_batch* b, int n, int32_t* done, int32_t* steps, double* t) {
   if (!b || n != (int)b->mem.size()) return -2;
   if (n == 0) return 0;
   BCHK(b, hipSetDevice(b->device));
@@ -2449,18 +2452,20 @@ extern "C" int imex_batch_status(imex_batch* b, int n, int32_t* done, double* t)
     if (b->d_t) BCHK(b, hipFree(b->d_t));
     b->d_done = nullptr;
     b->d_t = nullptr;
-    BCHK(b, hipMalloc(&b->d_done, sizeof(unsigned int) * (size_t)n));
+    BCHK(b, hipMalloc(&b->d_done, 2 * sizeof(unsigned int) * (size_t)n));
     BCHK(b, hipMalloc(&b->d_t, sizeof(double) * (size_t)n));
     b->status_cap = n;
   }
   hipLaunchKernelGGL(kb_status, dim3((n + 63) / 64), dim3(64), 0, b->stream, b->d_mem, n, b->d_done, b->d_t);
   BCHK(b, hipGetLastError());
-  std::vector<unsigned int> dn((size_t)n);
-  BCHK(b, hipMemcpyAsync(dn.data(), b->d_done, sizeof(unsigned int) * (size_t)n, hipMemcpyDeviceToHost, b->stream));
+  std::vector<unsigned int> dn(2 * (size_t)n);
+  BCHK(b, hipMemcpyAsync(dn.data(), b->d_done, 2 * sizeof(unsigned int) * (size_t)n, hipMemcpyDeviceToHost, b->stream));
   if (t) BCHK(b, hipMemcpyAsync(t, b->d_t, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, b->stream));
   BCHK(b, hipStreamSynchronize(b->stream));
-  if (done)
-    for (int i = 0; i < n; i++) done[i] = (int32_t)dn[i];
+  for (int i = 0; i < n; i++) {
+    if (done) done[i] = (int32_t)dn[2 * i];
+    if (steps) steps[i] = (int32_t)dn[2 * i + 1];
+  }
   return 0;
 }
 

@@ -59,6 +59,7 @@ struct DevState {
   unsigned int pending;      // 1: the last step's dt history and t are still to commit
   unsigned int done;         // batch member: t reached t_end, its kernels do nothing
   unsigned int full_next;    // batch member: the next active-set pass visits every tile
+  unsigned int steps;        // device-driven steps committed since the upload
   unsigned long long dried;
   unsigned long long active;
 };

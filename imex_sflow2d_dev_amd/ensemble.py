@@ -59,19 +59,22 @@ class Batch:
         self._check(self.L.imex_batch_run(self.b, ctypes.c_int(nsteps)), "imex_batch_run")
 
     def status(self):
-        """(done[bool], t[float]) per member, in self.members order; synchronises."""
+        """(done[bool], steps[int], t[float]) per member, in self.members
+        order; synchronises."""
         n = len(self.members)
         done = np.zeros(max(n, 1), np.int32)
+        steps = np.zeros(max(n, 1), np.int32)
         t = np.zeros(max(n, 1))
         self._check(self.L.imex_batch_status(self.b, ctypes.c_int(n), native._ip(done),
-                                             native._dp(t)), "imex_batch_status")
-        return done[:n].astype(bool), t[:n]
+                                             native._ip(steps), native._dp(t)),
+                    "imex_batch_status")
+        return done[:n].astype(bool), steps[:n], t[:n]
 
     def run_to_end(self, chunk: int = 50, max_steps: int = 10 ** 7) -> int:
         """Steps until every member is done; returns the batch steps taken."""
         steps = 0
         while steps < max_steps:
-            done, _ = self.status()
+            done, _, _ = self.status()
             if done.all():
                 break
             self.run(chunk)
@@ -98,3 +101,87 @@ def run_alone(s: Solver, t_end: float, max_steps: int = 10 ** 7) -> int:
         s.run_steps(1, t_end=t_end, record_dt=False)
         n += 1
     return n
+
+
+def autotune_blocks(batch: Batch, candidates=(8, 16, 32, 64, 128), steps: int = 8):
+    """Picks the blocks per member (the waves per CU of a batch launch) on the
+    batch itself: each candidate times `steps` batch steps of the members,
+    which advance meanwhile (useful work, nothing is undone), and the fastest
+    is kept.  Returns (best, {candidate: seconds per step})."""
+    import time
+    table = {}
+    for gx in candidates:
+        batch._check(batch.L.imex_batch_set_blocks(batch.b, ctypes.c_int(gx)), "imex_batch_set_blocks")
+        batch.status()
+        t0 = time.perf_counter()
+        batch.run(steps)
+        batch.status()
+        table[gx] = (time.perf_counter() - t0) / steps
+    best = min(table, key=table.get)
+    batch._check(batch.L.imex_batch_set_blocks(batch.b, ctypes.c_int(best)), "imex_batch_set_blocks")
+    return best, table
+
+
+def bench_ensemble(n_members: int = 256, t_end: float = 30.0, chunk: int = 25,
+                   min_side: int = 128, max_side: int = 1024, log=None):
+    """The batched ensemble against the same runs one after the other through
+    the single-run device driver.  Set-up (DEMs, static fields, uploads) is
+    outside both clocks.  Returns a dict of the measurements."""
+    import time
+    from . import init, workloads
+    say = log or (lambda *a: None)
+    t_set = time.perf_counter()
+    specs = []
+    for m in range(n_members):
+        cfg, B = workloads.config4_member(m, min_side=min_side, max_side=max_side, t_end=t_end)
+        st, q0 = init.setup(cfg, B)
+        specs.append((cfg, st, q0))
+    say(f"host set-up of {n_members} members: {time.perf_counter() - t_set:.1f} s")
+    cells = np.array([c.nx * c.ny for c, _, _ in specs])
+
+    # batched
+    sol = [Solver(*sp) for sp in specs]
+    b = Batch(n_solid=1)
+    for s, (cfg, _, _) in zip(sol, specs):
+        b.add(s, cfg.t_end)
+    b.status()
+    t0 = time.perf_counter()
+    best, table = autotune_blocks(b)
+    steps = b.run_to_end(chunk=chunk)
+    done, nsteps, tfin = b.status()
+    wall_batch = time.perf_counter() - t0
+    assert done.all()
+    say(f"batched: {wall_batch:.3f} s, blocks per member {best}, max member steps {nsteps.max()}")
+    for s in list(sol):
+        b.remove(s)
+    b.close()
+    q_batch = {i: sol[i].q for i in (0, n_members // 2, n_members - 1)}
+    for s in sol:
+        s.close()
+
+    # sequential: the same member runs, each to its step count, one by one
+    wall_seq = 0.0
+    same = True
+    for i, sp in enumerate(specs):
+        s = Solver(*sp)
+        s.run_steps(0)
+        t1 = time.perf_counter()
+        s.run_steps(int(nsteps[i]), t_end=sp[0].t_end, record_dt=False)
+        wall_seq += time.perf_counter() - t1
+        if i in q_batch:
+            same = same and np.array_equal(s.q.view(np.int64), q_batch[i].view(np.int64)) \
+                and s.t == tfin[i]
+        s.close()
+    say(f"sequential: {wall_seq:.3f} s")
+    return {
+        "members": n_members, "t_end_s": t_end, "cells_total": int(cells.sum()),
+        "cells_min": int(cells.min()), "cells_max": int(cells.max()),
+        "member_steps_total": int(nsteps.sum()), "member_steps_max": int(nsteps.max()),
+        "batch_wall_s": wall_batch, "sequential_wall_s": wall_seq,
+        "runs_per_hour_batched": n_members / wall_batch * 3600.0,
+        "runs_per_hour_sequential": n_members / wall_seq * 3600.0,
+        "speedup": wall_seq / wall_batch,
+        "blocks_per_member": best,
+        "autotune_s_per_step": {str(k): v for k, v in table.items()},
+        "batch_equals_alone_checked_members": sorted(q_batch), "batch_equals_alone": bool(same),
+    }

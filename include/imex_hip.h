@@ -165,9 +165,10 @@ int imex_batch_add(imex_batch *b, imex_handle *h, double t_end);
 int imex_batch_remove(imex_batch *b, imex_handle *h);
 /* nsteps steps of every member that has not reached its t_end (asynchronous) */
 int imex_batch_run(imex_batch *b, int nsteps);
-/* per member, in the order added: done flag (t reached t_end) and t;
- * n = number of members; synchronises */
-int imex_batch_status(imex_batch *b, int n, int32_t *done, double *t);
+/* per member, in the order added: done flag (t reached t_end), device-driven
+ * steps since its upload, and t; n = number of members; any output may be
+ * NULL; synchronises */
+int imex_batch_status(imex_batch *b, int n, int32_t *done, int32_t *steps, double *t);
 int imex_batch_last_error(imex_batch *b, char *buf, size_t n);
 void imex_batch_destroy(imex_batch *b);
 

@@ -51,8 +51,8 @@ def test_members_match_runs_alone():
         b.add(s, cfg.t_end)
         inb.append(s)
     steps = b.run_to_end(chunk=25)
-    done, t = b.status()
-    assert done.all() and steps > 0
+    done, nst, t = b.status()
+    assert done.all() and steps > 0 and (nst > 0).all()
     for s in list(inb):
         b.remove(s)
     for i, (s, r) in enumerate(zip(inb, alone)):
