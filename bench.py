@@ -126,7 +126,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5000)
     ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--workload", default="c2",
-                    help="c2 (default), c1, c2-small, c3, or c4 (the batched ensemble)")
+                    help="c2 (default), c1, c2-small, c3, c4 (the batched ensemble) or c5 "
+                         "(serving)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the reference CPU sample")
     ap.add_argument("--event-every", type=int, default=10, metavar="E",
@@ -137,6 +138,9 @@ def main():
                          "reference point for rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
     ap.add_argument("--members", type=int, default=256, help="c4: ensemble members")
     ap.add_argument("--t-end", type=float, default=30.0, help="c4: physical end time of a member (s)")
+    ap.add_argument("--rates", default="2,8,32", help="c5: arrival rates per second")
+    ap.add_argument("--window", type=float, default=20.0, help="c5: arrival window (wall s)")
+    ap.add_argument("--serve-t-end", default="60,600", help="c5: t_end range of a request (s)")
     ap.add_argument("--info", default=None,
                     help="also write per-kernel launch counts and active cells to this JSON file")
     args = ap.parse_args()
@@ -165,6 +169,9 @@ def main():
 
     if args.workload == "c4":
         bench_c4(args)
+        return
+    if args.workload == "c5":
+        bench_c5(args)
         return
 
     t_host = time.perf_counter()
@@ -319,6 +326,41 @@ def bench_c4(args):
         "data": "synthetic (seeded PCG64 fractal DEMs and member parameters, generated in-process)",
         "config": dict({"workload": (f"c4: {r['members']} members, sides log-uniform in 128..1024 "
                                      f"cells (dx 10 m), each to t_end = {r['t_end_s']} s, FP64")}, **r),
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_c5(args):
+    """Serving: Poisson arrivals at each rate of --rates over --window seconds
+    of wall time, admitted by device-memory footprint into one running batch;
+    time-to-result percentiles and cell-stage updates/s beside a FIFO server
+    of plain runs."""
+    from imex_sflow2d_dev_amd import serving
+    t_end = tuple(float(x) for x in args.serve_t_end.split(","))
+    res = []
+    for rate in [float(x) for x in args.rates.split(",")]:
+        r = serving.bench_serving(rate, args.window, t_end=t_end,
+                                  log=lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True))
+        print(f"[bench] rate {rate}/s: {json.dumps(r)}", file=sys.stderr, flush=True)
+        res.append(r)
+    top = res[-1]
+    out = {
+        "metric": "p50/p99 time-to-result (s)",
+        "value": top["p99_s"],
+        "unit": "s (p99 time-to-result at the highest rate)",
+        "n_gpus": 1,
+        "steps": sum(r["steps_total"] for r in res),
+        "warmup": 0,
+        "ms_per_step": None,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded PCG64 request stream, DEMs and parameters, generated in-process)",
+        "config": {"workload": (f"c5: Poisson arrivals at {args.rates} per s over {args.window} s, sides "
+                                f"log-uniform in 64..2048 cells (dx 10 m), t_end U[{args.serve_t_end}] s, "
+                                f"200 GB admission budget, FP64"),
+                   "rates": res},
     }
     print(json.dumps(out), flush=True)
 

@@ -2514,6 +2514,15 @@ extern "C" int imex_calib_copy(imex_handle* h, int64_t n, int reps) {
   return 0;
 }
 
+extern "C" int imex_get_counters(imex_handle* h, int64_t* active_cells, int64_t* steps) {
+  if (!h) return -2;
+  DevState s;
+  if (read_state(h, &s)) return -1;
+  if (active_cells) *active_cells = (int64_t)s.active;
+  if (steps) *steps = (int64_t)s.steps;
+  return 0;
+}
+
 extern "C" int imex_get_time_history(imex_handle* h, double* t, double* dt_old,
                                      double* dt_old_old) {
   if (!h) return -2;

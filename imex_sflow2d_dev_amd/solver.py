@@ -226,6 +226,15 @@ class Solver:
         self._download(native.DL_MASK, mask=m)
         return m
 
+    def counters(self):
+        """(active cells summed over every check_solve since create,
+        device-driven steps since the upload)."""
+        a = ctypes.c_int64()
+        n = ctypes.c_int64()
+        self.H.check(self.L.imex_get_counters(self.h, ctypes.byref(a), ctypes.byref(n)),
+                     "imex_get_counters")
+        return a.value, n.value
+
     def stats(self):
         st = native.ImexStepStats()
         self.H.check(self.L.imex_post_step(self.h, ctypes.c_double(self.t), ctypes.byref(st)),

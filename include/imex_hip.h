@@ -137,6 +137,10 @@ int imex_set_time(imex_handle *h, double t, double dt_old, double dt_old_old);
 /* t and the dt history of the device-side driver after device-driven steps
  * (the counterpart of imex_set_time) */
 int imex_get_time_history(imex_handle *h, double *t, double *dt_old, double *dt_old_old);
+/* active cells summed over every check_solve since create, and the
+ * device-driven steps since the last upload (measurement; no counterpart in
+ * the reference) */
+int imex_get_counters(imex_handle *h, int64_t *active_cells, int64_t *steps);
 /* Lazy host mirrors: copies the selected device arrays (any pointer may be
  * NULL when not selected).  q in Fortran order; deposit(nx,ny,n_solid);
  * vuln as one int32 per (level, cell) in Fortran order; mask one int32 per cell. */

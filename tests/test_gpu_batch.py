@@ -96,3 +96,20 @@ def test_two_class_radial_source_members():
         run_alone(ref, r[0].t_end)
         _same_run(s, ref, f"two-class {r[0].nx}")
     b.close()
+
+
+def test_served_results_equal_runs_alone():
+    """The serving loop (arrivals, admission under a small memory budget so
+    that requests wait and join as others leave): every served result equals
+    the same request run alone."""
+    from imex_sflow2d_dev_amd import serving
+    reqs = serving.make_requests(rate=40.0, duration=0.3, seed=9, t_end=(2.0, 5.0), side=(40, 90))
+    assert len(reqs) >= 4
+    budget = 2.5 * max(r.footprint for r in reqs)
+    serving.serve(reqs, budget_bytes=budget, chunk=7, keep=True)
+    for r in reqs:
+        s = Solver(r.cfg, r.static, r.q0)
+        run_alone(s, r.t_end)
+        assert s.t == r.t, r.idx
+        _bits(s.q, r.q, f"request {r.idx}")
+        s.close()
