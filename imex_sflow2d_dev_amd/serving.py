@@ -153,7 +153,7 @@ def bench_serving(rate: float, duration: float, t_end=(60.0, 600.0), side=(64, 2
     svc = fifo_service_times(reqs)
     cf = fifo_completion(arr, svc)
     ttr_f = cf - arr
-    wall_f = float(cf.max() - arr.min()) if len(reqs) else 0.0
+    wall_f = float(cf.max()) if len(reqs) else 0.0
     return {
         "rate_per_s": rate, "window_s": duration, "requests": len(reqs),
         "t_end_range_s": list(t_end), "side_range": list(side),
