@@ -2271,7 +2271,7 @@ struct imex_batch {
   int ns = 0;
   int device = 0;
   int n_cu = 256;
-  int gx = 64;                       // blocks per member and launch
+  int gx = 0;                        // blocks per member and launch; 0: from the member count
   hipStream_t stream = nullptr;
   std::vector<imex_handle*> mem;
   Member* d_mem = nullptr;
@@ -2339,7 +2339,7 @@ extern "C" int imex_batch_create(int n_solid, imex_batch** out) {
 }
 
 extern "C" int imex_batch_set_blocks(imex_batch* b, int blocks_per_member) {
-  if (!b || blocks_per_member < 1 || blocks_per_member > 4096) return -2;
+  if (!b || blocks_per_member < 0 || blocks_per_member > 4096) return -2;
   b->gx = blocks_per_member;
   return 0;
 }
@@ -2414,7 +2414,9 @@ extern "C" int imex_batch_run(imex_batch* b, int nsteps) {
   const int n = (int)b->mem.size();
   if (n == 0 || nsteps == 0) return 0;
   if (n > 65535) return -2;
-  const dim3 g(b->gx, n), blk(NTHR);
+  // automatic: about 16 blocks per CU in all, at least 8 and at most 1024 per member
+  const int gx = b->gx ? b->gx : std::min(1024, std::max(8, 16 * b->n_cu / n));
+  const dim3 g(gx, n), blk(NTHR);
   for (int k = 0; k < nsteps; k++) {
     b->par ^= 1;
     b->seq++;

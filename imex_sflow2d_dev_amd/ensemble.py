@@ -19,7 +19,7 @@ class Batch:
     fields and initial state uploaded); while in the batch their own step
     calls must not be used."""
 
-    def __init__(self, n_solid: int, blocks_per_member: int = 64):
+    def __init__(self, n_solid: int, blocks_per_member: int = 0):
         self.L = native.lib()
         self.n_solid = n_solid
         b = ctypes.c_void_p()
@@ -103,9 +103,10 @@ def run_alone(s: Solver, t_end: float, max_steps: int = 10 ** 7) -> int:
     return n
 
 
-def autotune_blocks(batch: Batch, candidates=(8, 16, 32, 64, 128), steps: int = 8):
-    """Picks the blocks per member (the waves per CU of a batch launch) on the
-    batch itself: each candidate times `steps` batch steps of the members,
+def autotune_blocks(batch: Batch, candidates=(0, 4, 8, 16, 32, 64), steps: int = 8):
+    """Picks the blocks per member (the waves per CU of a batch launch; 0 =
+    the library's choice from the member count) on the batch itself: each
+    candidate times `steps` batch steps of the members,
     which advance meanwhile (useful work, nothing is undone), and the fastest
     is kept.  Returns (best, {candidate: seconds per step})."""
     import time

@@ -69,7 +69,7 @@ def serve(reqs: List[Request], budget_bytes: float = 200e9, chunk: int = 20, log
     first arrival to the last result; fills admit/finish/steps per request
     (and the final state and t when keep)."""
     say = log or (lambda *a: None)
-    b = Batch(n_solid=1, blocks_per_member=8)
+    b = Batch(n_solid=1)
     waiting, running = [], []
     pending = sorted(reqs, key=lambda r: r.arrival)
     used = 0.0

@@ -161,7 +161,8 @@ void imex_destroy(imex_handle *h);
  * same handle.  All members have the same n_solid and device. */
 typedef struct imex_batch imex_batch;
 int imex_batch_create(int n_solid, imex_batch **b);
-/* blocks per member and launch (default 64) */
+/* blocks per member and launch; 0 (the default): about 16 blocks per CU
+ * shared by the members, 8 to 1024 per member */
 int imex_batch_set_blocks(imex_batch *b, int blocks_per_member);
 /* the handle joins at the next step and runs until its t reaches t_end; the
  * handle's own calls must not be used while it is a member */
