@@ -71,6 +71,8 @@ class RunConfig:
     thickness_levels: List[float] = dataclasses.field(default_factory=list)
     dyn_pres_levels: List[float] = dataclasses.field(default_factory=list)
     name: str = "run"
+    # 0: FP64 drop-in; 1: FP32 form (state, reconstruction and fluxes in FP32)
+    precision: int = 0
 
     # ---- derived quantities ------------------------------------------------
     @property

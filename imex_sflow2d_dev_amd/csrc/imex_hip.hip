@@ -27,6 +27,7 @@ using namespace imex;
 // lists: the tile goes to k_flux's list when it is active and "hot" (a wet
 // or ring cell within two cells), and a hot tile makes itself and its eight
 // neighbours candidates of the next step's k_mask.
+template <typename S>
 __device__ __forceinline__ void mask_tile(const Params& P, const Grid& G, int tile, int solve_all,
                                           int par, unsigned int seq) {
   __shared__ unsigned char seed[RY][RX];
@@ -47,7 +48,7 @@ __device__ __forceinline__ void mask_tile(const Params& P, const Grid& G, int ti
     if (j >= 0 && j < G.nx && k >= 0 && k < G.ny) {
       const int c = j + G.nx * k;
       const bool ring = (j == 0 || j == G.nx - 1 || k == 0 || k == G.ny - 1);
-      const bool wet = G.q[c] > 0.0, src = P.radial && G.scell[c] == 2;
+      const bool wet = state_q<S>(G)[c] > S(0.0), src = P.radial && G.scell[c] == 2;
       if (ring || solve_all || wet || src) s = 1;
       if (wet || src) s_hot = 1;
     }
@@ -133,6 +134,7 @@ __device__ __forceinline__ void commit_step(DevState* st) {
 }
 __global__ void k_commit(Grid G) { commit_step(G.st); }
 
+template <typename S>
 __device__ __forceinline__ void mask_body(const Params& P, const Grid& G, int solve_all, int par,
                                           unsigned int seq, int full) {
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -146,41 +148,45 @@ __device__ __forceinline__ void mask_body(const Params& P, const Grid& G, int so
   }
   if (full) {
     const int nt = G.ntx * G.nty;
-    for (int b = blockIdx.x; b < nt; b += gridDim.x) mask_tile(P, G, xcd_tile(b, nt), solve_all, par, seq);
+    for (int b = blockIdx.x; b < nt; b += gridDim.x)
+      mask_tile<S>(P, G, xcd_tile(b, nt), solve_all, par, seq);
   } else {
     const unsigned int slot = seq % 3u;
     const int n = (int)G.st->n_cand[slot];
     const int* list = G.cand + (size_t)slot * G.ntx * G.nty;
-    for (int it = blockIdx.x; it < n; it += gridDim.x) mask_tile(P, G, list[it], solve_all, par, seq);
+    for (int it = blockIdx.x; it < n; it += gridDim.x) mask_tile<S>(P, G, list[it], solve_all, par, seq);
   }
 }
+template <typename S>
 __global__ __launch_bounds__(NTHR) void k_mask(Params P, Grid G, int solve_all, int par,
                                                unsigned int seq, int full) {
-  mask_body(P, G, solve_all, par, seq, full);
+  mask_body<S>(P, G, solve_all, par, seq, full);
 }
 
 // ===========================================================================
 // reconstruction of one cell in one direction: src/solver_2d.f90:3095-3924
 // ===========================================================================
 // LDS primitive states of a half tile: sq[(i*RY + lk)*FRX + lj]
+template <typename F>
 struct TileView {
-  const double* sq;
+  const F* sq;
   const signed char* ssc;
   const unsigned char* ssd;
   bool tsvar;  // timestep variant: cells of the source disc carry T = T_ambient
-  double T_amb;
-  __device__ __forceinline__ double at(int i, int idx) const {
-    double v = sq[i * FRX * RY + idx];
+  F T_amb;
+  __device__ __forceinline__ F at(int i, int idx) const {
+    F v = sq[i * FRX * RY + idx];
     if (tsvar && i == 3 && ssc[idx] == 1) v = T_amb;
     return v;
   }
 };
 
-__device__ __forceinline__ void qp_to_qp2(const double* qp, double& u, double& v) {
+template <typename F>
+__device__ __forceinline__ void qp_to_qp2(const F* qp, F& u, F& v) {
   // src/constitutive_2d.f90:2164-2188 (qp2(1) = h + B is not needed here)
-  if (qp[0] <= 0.0) {
-    u = 0.0;
-    v = 0.0;
+  if (qp[0] <= F(0.0)) {
+    u = F(0.0);
+    v = F(0.0);
   } else {
     u = qp[1] / qp[0];
     v = qp[2] / qp[0];
@@ -188,58 +194,58 @@ __device__ __forceinline__ void qp_to_qp2(const double* qp, double& u, double& v
 }
 
 // x direction: W and E face states (primitive) of cell (j,k) at local index idx.
-template <int NS>
-__device__ __forceinline__ void recon_x(const Params& P, const Grid& G, const TileView& T, int idx, int j, int k,
-                        double* W, double* E) {
+template <int NS, typename F>
+__device__ __forceinline__ void recon_x(const Params& P, const Grid& G, const TileView<F>& T, int idx,
+                                        int j, int k, F* W, F* E) {
   constexpr int NV = 4 + NS;
   const int nx = G.nx;
   const signed char sc = T.ssc[idx];
   const unsigned char sd = T.ssd[idx];
   const bool src2 = (sc == 2);
   const bool sE = sd & 1, sW = sd & 2, sN = sd & 4, sS = sd & 8;
-  const double xs1 = G.xc[j];
-  double xs0, xs2;
+  const F xs1 = F(G.xc[j]);
+  F xs0, xs2;
 #pragma unroll
   for (int i = 0; i < NV; i++) {
-    const double v1 = T.at(i, idx);
-    double pr;
+    const F v1 = T.at(i, idx);
+    F pr;
     if (j == 0) {
-      xs0 = G.xs[0];
-      xs2 = G.xc[1];
-      pr = src2 ? limit_minmod(v1, v1, T.at(i, idx + 1), xs0, xs1, xs2) : 0.0;
+      xs0 = F(G.xs[0]);
+      xs2 = F(G.xc[1]);
+      pr = src2 ? limit_minmod(v1, v1, T.at(i, idx + 1), xs0, xs1, xs2) : F(0.0);
     } else if (j == nx - 1) {
-      xs2 = G.xs[nx];
-      xs0 = G.xc[nx - 2];
-      pr = src2 ? limit_minmod(T.at(i, idx - 1), v1, v1, xs0, xs1, xs2) : 0.0;
+      xs2 = F(G.xs[nx]);
+      xs0 = F(G.xc[nx - 2]);
+      pr = src2 ? limit_minmod(T.at(i, idx - 1), v1, v1, xs0, xs1, xs2) : F(0.0);
     } else {
-      xs0 = G.xc[j - 1];
-      xs2 = G.xc[j + 1];
-      double v0 = T.at(i, idx - 1), v2 = T.at(i, idx + 1);
+      xs0 = F(G.xc[j - 1]);
+      xs2 = F(G.xc[j + 1]);
+      F v0 = T.at(i, idx - 1), v2 = T.at(i, idx + 1);
       if (src2) {
         if (sE) {
-          xs2 = G.xs[j + 1];
+          xs2 = F(G.xs[j + 1]);
           v2 = v1;
         } else if (sW) {
-          xs0 = G.xs[j];
+          xs0 = F(G.xs[j]);
           v0 = v1;
         }
       }
       pr = limit_minmod(v0, v1, v2, xs0, xs1, xs2);
     }
-    const double dq = 1.0 * P.dx2 * pr;
+    const F dq = F(1.0) * F(P.dx2) * pr;
     W[i] = v1 - dq;
     E[i] = v1 + dq;
     if (j == 0) {
       if (src2)
         W[i] = v1;
       else if (i == 1)
-        W[i] = fmin_f(W[i], 0.0);
+        W[i] = fmin_f(W[i], F(0.0));
     }
     if (j == nx - 1) {
       if (src2)
         E[i] = v1;
       else if (i == 1)
-        E[i] = fmax_f(E[i], 0.0);
+        E[i] = fmax_f(E[i], F(0.0));
     }
   }
   // u and v with the limiters of hu and hv; the y part of u may have stored
@@ -248,9 +254,9 @@ __device__ __forceinline__ void recon_x(const Params& P, const Grid& G, const Ti
 #pragma unroll
   for (int ii = 0; ii < 2; ii++) {
     const int i = NV + ii;
-    const double v1 = T.at(i, idx);
-    if (ii == 1 && typo) xs2 = G.ys[k + 1];
-    double v0, v2, q2u, q2v;
+    const F v1 = T.at(i, idx);
+    if (ii == 1 && typo) xs2 = F(G.ys[k + 1]);
+    F v0, v2, q2u, q2v;
     if (j == 0) {
       qp_to_qp2(W, q2u, q2v);
       v0 = ii ? q2v : q2u;
@@ -264,16 +270,16 @@ __device__ __forceinline__ void recon_x(const Params& P, const Grid& G, const Ti
       v2 = T.at(i, idx + 1);
       if (src2) {
         if (sE) {
-          xs2 = G.xs[j + 1];
+          xs2 = F(G.xs[j + 1]);
           v2 = v1;
         } else if (sW) {
-          xs0 = G.xs[j];
+          xs0 = F(G.xs[j]);
           v0 = v1;
         }
       }
     }
-    const double pr = limit_minmod(v0, v1, v2, xs0, xs1, xs2);
-    const double dq = 1.0 * P.dx2 * pr;
+    const F pr = limit_minmod(v0, v1, v2, xs0, xs1, xs2);
+    const F dq = F(1.0) * F(P.dx2) * pr;
     W[i] = v1 - dq;
     E[i] = v1 + dq;
     if (j == 0) {
@@ -295,55 +301,55 @@ __device__ __forceinline__ void recon_x(const Params& P, const Grid& G, const Ti
 }
 
 // y direction: S and N face states of cell (j,k).
-template <int NS>
-__device__ __forceinline__ void recon_y(const Params& P, const Grid& G, const TileView& T, int idx, int j, int k,
-                        double* S, double* Nn) {
+template <int NS, typename F>
+__device__ __forceinline__ void recon_y(const Params& P, const Grid& G, const TileView<F>& T, int idx,
+                                        int j, int k, F* S, F* Nn) {
   constexpr int NV = 4 + NS;
   const int ny = G.ny;
   const signed char sc = T.ssc[idx];
   const unsigned char sd = T.ssd[idx];
   const bool src2 = (sc == 2);
   const bool sN = sd & 4, sS = sd & 8;
-  const double ys1 = G.yc[k];
-  double ys0, ys2;
+  const F ys1 = F(G.yc[k]);
+  F ys0, ys2;
 #pragma unroll
   for (int i = 0; i < NV; i++) {
-    const double v1 = T.at(i, idx);
-    double pr;
+    const F v1 = T.at(i, idx);
+    F pr;
     if (k == 0) {
-      ys0 = G.ys[0];
-      ys2 = G.yc[1];
-      pr = 0.0;
+      ys0 = F(G.ys[0]);
+      ys2 = F(G.yc[1]);
+      pr = F(0.0);
     } else if (k == ny - 1) {
-      ys0 = G.yc[ny - 2];
-      ys2 = G.ys[ny];
-      pr = 0.0;
+      ys0 = F(G.yc[ny - 2]);
+      ys2 = F(G.ys[ny]);
+      pr = F(0.0);
     } else {
-      ys0 = G.yc[k - 1];
-      ys2 = G.yc[k + 1];
-      double v0 = T.at(i, idx - FRX), v2 = T.at(i, idx + FRX);
+      ys0 = F(G.yc[k - 1]);
+      ys2 = F(G.yc[k + 1]);
+      F v0 = T.at(i, idx - FRX), v2 = T.at(i, idx + FRX);
       if (src2) {
         if (sS) {
-          ys0 = G.ys[k];
+          ys0 = F(G.ys[k]);
           v0 = v1;
         } else if (sN) {
-          ys2 = G.ys[k + 1];
+          ys2 = F(G.ys[k + 1]);
           v2 = v1;
         }
       }
       pr = limit_minmod(v0, v1, v2, ys0, ys1, ys2);
     }
-    const double dq = 1.0 * P.dy2 * pr;
+    const F dq = F(1.0) * F(P.dy2) * pr;
     S[i] = v1 - dq;
     Nn[i] = v1 + dq;
-    if (k == 0 && i == 2) S[i] = fmin_f(S[i], 0.0);
-    if (k == ny - 1 && i == 2) Nn[i] = fmax_f(Nn[i], 0.0);
+    if (k == 0 && i == 2) S[i] = fmin_f(S[i], F(0.0));
+    if (k == ny - 1 && i == 2) Nn[i] = fmax_f(Nn[i], F(0.0));
   }
 #pragma unroll
   for (int ii = 0; ii < 2; ii++) {
     const int i = NV + ii;
-    const double v1 = T.at(i, idx);
-    double v0, v2, q2u, q2v;
+    const F v1 = T.at(i, idx);
+    F v0, v2, q2u, q2v;
     if (k == 0) {
       qp_to_qp2(S, q2u, q2v);
       v0 = ii ? q2v : q2u;
@@ -357,15 +363,15 @@ __device__ __forceinline__ void recon_y(const Params& P, const Grid& G, const Ti
       v2 = T.at(i, idx + FRX);
       if (src2) {
         if (sS) {
-          ys0 = G.ys[k];
+          ys0 = F(G.ys[k]);
           v0 = v1;
         } else if (sN) {
           v2 = v1;
         }
       }
     }
-    const double pr = limit_minmod(v0, v1, v2, ys0, ys1, ys2);
-    const double dq = 1.0 * P.dy2 * pr;
+    const F pr = limit_minmod(v0, v1, v2, ys0, ys1, ys2);
+    const F dq = F(1.0) * F(P.dy2) * pr;
     S[i] = v1 - dq;
     Nn[i] = v1 + dq;
     if (k == 0) {
@@ -393,24 +399,24 @@ __device__ __forceinline__ void recon_y(const Params& P, const Grid& G, const Ti
 // owned by a source-disc cell is the ring mirror when the ring cell on the
 // other side faces the disc across this face, else the zero state (a face
 // state the reference never writes).
-template <int NS>
-__device__ __forceinline__ void face_finish(const Params& P, const TileView& T, int dir, int idxA,
-                                            int idxB, bool hasA, bool hasB, double* qpA,
-                                            double* qpB, double* qA, double* qB) {
+template <int NS, typename F>
+__device__ __forceinline__ void face_finish(const Params& P, const TileView<F>& T, int dir, int idxA,
+                                            int idxB, bool hasA, bool hasB, F* qpA, F* qpB, F* qA,
+                                            F* qB) {
   constexpr int NV = 4 + NS, NP = NV + 2;
   if (hasA) {
     if (T.ssc[idxA] == 1) {
 #pragma unroll
-      for (int i = 0; i < NP; i++) qpA[i] = 0.0;
+      for (int i = 0; i < NP; i++) qpA[i] = F(0.0);
     }
-    qp_to_qc<NS>(P, qpA, qA);
+    qp_to_qc<NS, F>(P, qpA, qA);
   }
   if (hasB) {
     if (T.ssc[idxB] == 1) {
 #pragma unroll
-      for (int i = 0; i < NP; i++) qpB[i] = 0.0;
+      for (int i = 0; i < NP; i++) qpB[i] = F(0.0);
     }
-    qp_to_qc<NS>(P, qpB, qB);
+    qp_to_qc<NS, F>(P, qpB, qB);
   }
   const int mom = dir;               // hu (1) or hv (2)
   const int vel = (dir == 1) ? NV : NV + 1;
@@ -445,30 +451,30 @@ __device__ __forceinline__ void face_finish(const Params& P, const TileView& T, 
 
 // The primitive face states of face_states alone (no conservative states),
 // for the timestep variant whose fluxes are not used, only its wave speeds.
-template <int NS>
-__device__ __forceinline__ void face_prims(const Params& P, const Grid& G, const TileView& T, int dir,
+template <int NS, typename F>
+__device__ __forceinline__ void face_prims(const Params& P, const Grid& G, const TileView<F>& T, int dir,
                                            int idxA, int idxB, int ja, int ka, int jb, int kb,
-                                           bool hasA, bool hasB, double* qpA, double* qpB) {
+                                           bool hasA, bool hasB, F* qpA, F* qpB) {
   constexpr int NV = 4 + NS, NP = NV + 2;
-  double tmp[NP];
+  F tmp[NP];
   if (hasA) {
     if (T.ssc[idxA] == 1) {
 #pragma unroll
-      for (int i = 0; i < NP; i++) qpA[i] = 0.0;
+      for (int i = 0; i < NP; i++) qpA[i] = F(0.0);
     } else if (dir == 1) {
-      recon_x<NS>(P, G, T, idxA, ja, ka, tmp, qpA);
+      recon_x<NS, F>(P, G, T, idxA, ja, ka, tmp, qpA);
     } else {
-      recon_y<NS>(P, G, T, idxA, ja, ka, tmp, qpA);
+      recon_y<NS, F>(P, G, T, idxA, ja, ka, tmp, qpA);
     }
   }
   if (hasB) {
     if (T.ssc[idxB] == 1) {
 #pragma unroll
-      for (int i = 0; i < NP; i++) qpB[i] = 0.0;
+      for (int i = 0; i < NP; i++) qpB[i] = F(0.0);
     } else if (dir == 1) {
-      recon_x<NS>(P, G, T, idxB, jb, kb, qpB, tmp);
+      recon_x<NS, F>(P, G, T, idxB, jb, kb, qpB, tmp);
     } else {
-      recon_y<NS>(P, G, T, idxB, jb, kb, qpB, tmp);
+      recon_y<NS, F>(P, G, T, idxB, jb, kb, qpB, tmp);
     }
   }
   const int vel = (dir == 1) ? NV : NV + 1;
@@ -492,65 +498,65 @@ __device__ __forceinline__ void face_prims(const Params& P, const Grid& G, const
 }
 
 // eval_speeds for one face (the a- / a+ of kt_flux)
-template <int NS>
-__device__ __forceinline__ void kt_speeds(const Params& P, int dir, const double* qpA,
-                                          const double* qpB, double& an, double& ap) {
-  double amin, amax, bmin, bmax;
-  local_speeds<NS>(P, qpA, dir, amin, amax);
-  local_speeds<NS>(P, qpB, dir, bmin, bmax);
-  an = fmin_f(fmin_f(amin, bmin), 0.0);
-  ap = fmax_f(fmax_f(amax, bmax), 0.0);
+template <int NS, typename F>
+__device__ __forceinline__ void kt_speeds(const Params& P, int dir, const F* qpA, const F* qpB,
+                                          F& an, F& ap) {
+  F amin, amax, bmin, bmax;
+  local_speeds<NS, F>(P, qpA, dir, amin, amax);
+  local_speeds<NS, F>(P, qpB, dir, bmin, bmax);
+  an = fmin_f(fmin_f(amin, bmin), F(0.0));
+  ap = fmax_f(fmax_f(amax, bmax), F(0.0));
 }
 
 // eval_speeds + eval_flux_KT + average_KT for one face:
 // src/solver_2d.f90:3937-4014, :2834-3002, :3019-3049
-template <int NS>
-__device__ __forceinline__ void kt_flux(const Params& P, int dir, const double* qpA, const double* qpB,
-                        const double* qA, const double* qB, double* H, double& an, double& ap) {
+template <int NS, typename F>
+__device__ __forceinline__ void kt_flux(const Params& P, int dir, const F* qpA, const F* qpB,
+                                        const F* qA, const F* qB, F* H, F& an, F& ap) {
   constexpr int NV = 4 + NS;
-  double amin, amax, bmin, bmax;
-  local_speeds<NS>(P, qpA, dir, amin, amax);
-  local_speeds<NS>(P, qpB, dir, bmin, bmax);
-  an = fmin_f(fmin_f(amin, bmin), 0.0);
-  ap = fmax_f(fmax_f(amax, bmax), 0.0);
-  double fA[NV], fB[NV];
-  phys_flux<NS>(P, qA, qpA, dir, fA);
-  phys_flux<NS>(P, qB, qpB, dir, fB);
+  F amin, amax, bmin, bmax;
+  local_speeds<NS, F>(P, qpA, dir, amin, amax);
+  local_speeds<NS, F>(P, qpB, dir, bmin, bmax);
+  an = fmin_f(fmin_f(amin, bmin), F(0.0));
+  ap = fmax_f(fmax_f(amax, bmax), F(0.0));
+  F fA[NV], fB[NV];
+  phys_flux<NS, F>(P, qA, qpA, dir, fA);
+  phys_flux<NS, F>(P, qB, qpB, dir, fB);
 #pragma unroll
   for (int i = 0; i < NV; i++) {
     if (an == ap) {
-      H[i] = 0.0;
+      H[i] = F(0.0);
     } else {
-      const double avg = (ap * fA[i] - an * fB[i]) / (ap - an);
+      const F avg = (ap * fA[i] - an * fB[i]) / (ap - an);
       H[i] = avg + (ap * an) / (ap - an) * (qB[i] - qA[i]);
     }
   }
-  double S = 0.0;
+  F S = F(0.0);
 #pragma unroll
   for (int i = 0; i < NS; i++) S += H[4 + i];
   if (dir == 1) {
-    if ((S / H[0]) >= 1.0) {
-      double S2 = 0.0;
+    if ((S / H[0]) >= F(1.0)) {
+      F S2 = F(0.0);
 #pragma unroll
       for (int i = 0; i < NS; i++) S2 += H[4 + i] / H[0];
 #pragma unroll
       for (int i = 0; i < NS; i++) H[4 + i] = H[4 + i] / S2;
     }
-    if ((qpA[1] == 0.0) && (qpB[1] == 0.0)) {
-      H[0] = 0.0;
+    if ((qpA[1] == F(0.0)) && (qpB[1] == F(0.0))) {
+      H[0] = F(0.0);
 #pragma unroll
-      for (int i = 3; i < NV; i++) H[i] = 0.0;
+      for (int i = 3; i < NV; i++) H[i] = F(0.0);
     }
   } else {
-    if ((S / H[0]) > 1.0) {
-      const double S2 = S / H[0];
+    if ((S / H[0]) > F(1.0)) {
+      const F S2 = S / H[0];
 #pragma unroll
       for (int i = 0; i < NS; i++) H[4 + i] = H[4 + i] / S2;
     }
-    if ((qA[2] == 0.0) && (qB[2] == 0.0)) {
-      H[0] = 0.0;
+    if ((qA[2] == F(0.0)) && (qB[2] == F(0.0))) {
+      H[0] = F(0.0);
 #pragma unroll
-      for (int i = 3; i < NV; i++) H[i] = 0.0;
+      for (int i = 3; i < NV; i++) H[i] = F(0.0);
     }
   }
 }
@@ -568,29 +574,34 @@ __device__ __forceinline__ void kt_flux(const Params& P, int dir, const double* 
 constexpr int NRX = (FX + 2) * TY;
 constexpr int NRY = FX * (TY + 2);
 
-template <int NS>
+template <int NS, typename F = double>
 constexpr size_t flux_lds_bytes() {
   constexpr int NV = 4 + NS, NP = NV + 2;
-  return sizeof(double) * ((size_t)NP * RY * FRX + (size_t)NV * (FX + 1) * TY +
+  return sizeof(F) * ((size_t)NP * RY * FRX + (size_t)NV * (FX + 1) * TY +
                            (size_t)NV * FX * (TY + 1) + (size_t)(FX + 1) * TY +
                            (size_t)FX * (TY + 1) + 2 * (size_t)NP * (NRX + NRY)) +
          2 * RY * FRX + FX * TY;
 }
 
-template <int NS>
+// F: the arithmetic and storage type of the state, the side states and the
+// fluxes (double on the FP64 path; float in the FP32 form, where thin
+// layers, h < 1e-3 m, are converted to primitive variables in double)
+template <int NS, typename F = double>
 __device__ __forceinline__ void flux_body(const Params& P, const Grid& G, int mode, double t_arg,
                                           int par) {
   constexpr int NV = 4 + NS, NP = NV + 2;
   extern __shared__ double smem[];
-  double* sq = smem;
-  double* sHx = sq + NP * RY * FRX;
-  double* sHy = sHx + NV * (FX + 1) * TY;
-  double* sAx = sHy + NV * FX * (TY + 1);
-  double* sAy = sAx + (FX + 1) * TY;
-  double* sRW = sAy + FX * (TY + 1);
-  double* sRE = sRW + NP * NRX;
-  double* sRS = sRE + NP * NRX;
-  double* sRN = sRS + NP * NRY;
+  F* sq = reinterpret_cast<F*>(smem);
+  F* sHx = sq + NP * RY * FRX;
+  F* sHy = sHx + NV * (FX + 1) * TY;
+  F* sAx = sHy + NV * FX * (TY + 1);
+  F* sAy = sAx + (FX + 1) * TY;
+  F* sRW = sAy + FX * (TY + 1);
+  F* sRE = sRW + NP * NRX;
+  F* sRS = sRE + NP * NRX;
+  F* sRN = sRS + NP * NRY;
+  const F* gq = state_q<F>(G);
+  F* gR = state_R<F>(G);
   signed char* ssc = (signed char*)(sRN + NP * NRY);
   unsigned char* ssd = (unsigned char*)(ssc + RY * FRX);
   unsigned char* sact = ssd + RY * FRX;
@@ -611,24 +622,34 @@ __device__ __forceinline__ void flux_body(const Params& P, const Grid& G, int mo
     for (int e = tid; e < FRX * RY; e += NTHR) {
       const int lj = e % FRX, lk = e / FRX;
       const int j = j0 + lj - HALO, k = k0 + lk - HALO;
-      double qp[NP];
+      F qp[NP];
       signed char sc = 0;
       unsigned char sd = 0;
 #pragma unroll
-      for (int i = 0; i < NP; i++) qp[i] = 0.0;
+      for (int i = 0; i < NP; i++) qp[i] = F(0.0);
       if (j >= 0 && j < nx && k >= 0 && k < ny) {
         const int c = j + nx * k;
         sc = G.scell[c];
         sd = G.sides[c];
         if (sc != 1) {
-          double q[NV];
+          F q[NV];
 #pragma unroll
-          for (int i = 0; i < NV; i++) q[i] = G.q[(size_t)i * N + c];
-          if (q[0] > 0.0) {
-            double pd;
-            qc_to_qp<NS>(P, q, qp, pd);
+          for (int i = 0; i < NV; i++) q[i] = gq[(size_t)i * N + c];
+          if (q[0] > F(0.0)) {
+            F pd;
+            qc_to_qp<NS, F>(P, q, qp, pd);
+            if constexpr (sizeof(F) < sizeof(double)) {
+              if (qp[0] < F(1.0e-3)) {  // thin layer: the division by h in double
+                double qd[NV], qpd[NP], pdd;
+#pragma unroll
+                for (int i = 0; i < NV; i++) qd[i] = (double)q[i];
+                qc_to_qp<NS, double>(P, qd, qpd, pdd);
+#pragma unroll
+                for (int i = 0; i < NP; i++) qp[i] = F(qpd[i]);
+              }
+            }
           } else {
-            qp[3] = P.T_amb;
+            qp[3] = F(P.T_amb);
           }
         }
       }
@@ -642,8 +663,8 @@ __device__ __forceinline__ void flux_body(const Params& P, const Grid& G, int mo
       sact[e] = (j < nx && k < ny) ? G.mask[j + nx * k] : 0;
     }
     __syncthreads();
-    TileView T1{sq, ssc, ssd, false, P.T_amb};
-    TileView T0{sq, ssc, ssd, true, P.T_amb};
+    TileView<F> T1{sq, ssc, ssd, false, F(P.T_amb)};
+    TileView<F> T0{sq, ssc, ssd, true, F(P.T_amb)};
     // ---- side states of the cells (cells of the source disc have none) -----
     for (int e = tid; e < NRX + NRY; e += NTHR) {
       if (e < NRX) {
@@ -651,8 +672,8 @@ __device__ __forceinline__ void flux_body(const Params& P, const Grid& G, int mo
         const int j = j0 + lj, k = k0 + lk;
         const int idx = (lk + HALO) * FRX + lj + HALO;
         if (j >= 0 && j < nx && k < ny && ssc[idx] != 1) {
-          double W[NP], E[NP];
-          recon_x<NS>(P, G, T1, idx, j, k, W, E);
+          F W[NP], E[NP];
+          recon_x<NS, F>(P, G, T1, idx, j, k, W, E);
 #pragma unroll
           for (int i = 0; i < NP; i++) {
             sRW[i * NRX + e] = W[i];
@@ -665,8 +686,8 @@ __device__ __forceinline__ void flux_body(const Params& P, const Grid& G, int mo
         const int j = j0 + lj, k = k0 + lk;
         const int idx = (lk + HALO) * FRX + lj + HALO;
         if (k >= 0 && k < ny && j < nx && ssc[idx] != 1) {
-          double S[NP], Nn[NP];
-          recon_y<NS>(P, G, T1, idx, j, k, S, Nn);
+          F S[NP], Nn[NP];
+          recon_y<NS, F>(P, G, T1, idx, j, k, S, Nn);
 #pragma unroll
           for (int i = 0; i < NP; i++) {
             sRS[i * NRY + e2] = S[i];
@@ -686,25 +707,26 @@ __device__ __forceinline__ void flux_body(const Params& P, const Grid& G, int mo
       // sides: H = 0 and no wave speed (the minmod slope of a dry cell is 0)
       if (need) {
         const int b = (fk + HALO) * FRX + fl + HALO;
-        need = (sq[b - 2] != 0.0) || (sq[b - 1] != 0.0) || (sq[b] != 0.0) || (sq[b + 1] != 0.0);
+        need = (sq[b - 2] != F(0.0)) || (sq[b - 1] != F(0.0)) || (sq[b] != F(0.0)) ||
+               (sq[b + 1] != F(0.0));
       }
-      double H[NV];
-      double amax = 0.0;
+      F H[NV];
+      F amax = F(0.0);
 #pragma unroll
-      for (int i = 0; i < NV; i++) H[i] = 0.0;
+      for (int i = 0; i < NV; i++) H[i] = F(0.0);
       if (need) {
         const int lk = fk + HALO, ljb = fl + HALO;
         const int idxB = lk * FRX + ljb, idxA = idxB - 1;
         const bool hasA = jf - 1 >= 0, hasB = jf < nx;
         const int ra = fk * (FX + 2) + fl;  // side states of cell A; B is ra + 1
-        double qpA[NP], qpB[NP], qA[NV], qB[NV], an, ap;
+        F qpA[NP], qpB[NP], qA[NV], qB[NV], an, ap;
 #pragma unroll
         for (int i = 0; i < NP; i++) {
           qpA[i] = sRE[i * NRX + ra];
           qpB[i] = sRW[i * NRX + ra + 1];
         }
-        face_finish<NS>(P, T1, 1, idxA, idxB, hasA, hasB, qpA, qpB, qA, qB);
-        kt_flux<NS>(P, 1, qpA, qpB, qA, qB, H, an, ap);
+        face_finish<NS, F>(P, T1, 1, idxA, idxB, hasA, hasB, qpA, qpB, qA, qB);
+        kt_flux<NS, F>(P, 1, qpA, qpB, qA, qB, H, an, ap);
         bool disc = false;
         if (P.radial) {
           for (int d = -2; d <= 1; d++) {
@@ -713,8 +735,8 @@ __device__ __forceinline__ void flux_body(const Params& P, const Grid& G, int mo
           }
         }
         if (disc) {  // timestep's wave speeds see T_ambient in the source disc
-          face_prims<NS>(P, G, T0, 1, idxA, idxB, jf - 1, k, jf, k, hasA, hasB, qpA, qpB);
-          kt_speeds<NS>(P, 1, qpA, qpB, an, ap);
+          face_prims<NS, F>(P, G, T0, 1, idxA, idxB, jf - 1, k, jf, k, hasA, hasB, qpA, qpB);
+          kt_speeds<NS, F>(P, 1, qpA, qpB, an, ap);
         }
         amax = fmax_f(ap, -an);
       }
@@ -730,26 +752,26 @@ __device__ __forceinline__ void flux_body(const Params& P, const Grid& G, int mo
                   ((fl > 0 && sact[(fl - 1) * FX + fj]) || (fl < TY && kf < ny && sact[fl * FX + fj]));
       if (need) {
         const int b = (fl + HALO) * FRX + fj + HALO;
-        need = (sq[b - 2 * FRX] != 0.0) || (sq[b - FRX] != 0.0) || (sq[b] != 0.0) ||
-               (sq[b + FRX] != 0.0);
+        need = (sq[b - 2 * FRX] != F(0.0)) || (sq[b - FRX] != F(0.0)) || (sq[b] != F(0.0)) ||
+               (sq[b + FRX] != F(0.0));
       }
-      double H[NV];
-      double amax = 0.0;
+      F H[NV];
+      F amax = F(0.0);
 #pragma unroll
-      for (int i = 0; i < NV; i++) H[i] = 0.0;
+      for (int i = 0; i < NV; i++) H[i] = F(0.0);
       if (need) {
         const int lj = fj + HALO, lkb = fl + HALO;
         const int idxB = lkb * FRX + lj, idxA = idxB - FRX;
         const bool hasA = kf - 1 >= 0, hasB = kf < ny;
         const int ra = fl * FX + fj;  // side states of cell A; B is ra + FX
-        double qpA[NP], qpB[NP], qA[NV], qB[NV], an, ap;
+        F qpA[NP], qpB[NP], qA[NV], qB[NV], an, ap;
 #pragma unroll
         for (int i = 0; i < NP; i++) {
           qpA[i] = sRN[i * NRY + ra];
           qpB[i] = sRS[i * NRY + ra + FX];
         }
-        face_finish<NS>(P, T1, 2, idxA, idxB, hasA, hasB, qpA, qpB, qA, qB);
-        kt_flux<NS>(P, 2, qpA, qpB, qA, qB, H, an, ap);
+        face_finish<NS, F>(P, T1, 2, idxA, idxB, hasA, hasB, qpA, qpB, qA, qB);
+        kt_flux<NS, F>(P, 2, qpA, qpB, qA, qB, H, an, ap);
         bool disc = false;
         if (P.radial) {
           for (int d = -2; d <= 1; d++) {
@@ -758,8 +780,8 @@ __device__ __forceinline__ void flux_body(const Params& P, const Grid& G, int mo
           }
         }
         if (disc) {  // timestep's wave speeds see T_ambient in the source disc
-          face_prims<NS>(P, G, T0, 2, idxA, idxB, j, kf - 1, j, kf, hasA, hasB, qpA, qpB);
-          kt_speeds<NS>(P, 2, qpA, qpB, an, ap);
+          face_prims<NS, F>(P, G, T0, 2, idxA, idxB, j, kf - 1, j, kf, hasA, hasB, qpA, qpB);
+          kt_speeds<NS, F>(P, 2, qpA, qpB, an, ap);
         }
         amax = fmax_f(ap, -an);
       }
@@ -777,37 +799,37 @@ __device__ __forceinline__ void flux_body(const Params& P, const Grid& G, int mo
       if (tid < FX * TY && j < nx && k < ny && sact[tid]) {
         const int ew = lk * (FX + 1) + lj;
         const int es = lk * FX + lj;
-        double qp[NP], ex[NV];
+        F qp[NP], ex[NV];
         const int idx = (lk + HALO) * FRX + lj + HALO;
 #pragma unroll
         for (int i = 0; i < NP; i++) qp[i] = sq[i * FRX * RY + idx];
         const signed char sc = ssc[idx];
-        double ap = 0.0, ax = 0.0, ay = 0.0;
+        F ap = F(0.0), ax = F(0.0), ay = F(0.0);
         if (sc == 2) {
-          ap = G.arcp[c];
-          ax = G.arcx[c];
-          ay = G.arcy[c];
+          ap = F(G.arcp[c]);
+          ax = F(G.arcx[c]);
+          ay = F(G.arcy[c]);
         }
-        expl_terms<NS>(P, G.bx[c], G.by[c], qp, ex, t, ap, ax, ay);
+        expl_terms<NS, F>(P, F(G.bx[c]), F(G.by[c]), qp, ex, t, ap, ax, ay);
 #pragma unroll
         for (int i = 0; i < NV; i++) {
-          double d = 0.0;
-          d = d + (sHx[i * (FX + 1) * TY + ew + 1] - sHx[i * (FX + 1) * TY + ew]) * P.one_by_dx;
-          d = d + (sHy[i * FX * (TY + 1) + es + FX] - sHy[i * FX * (TY + 1) + es]) * P.one_by_dy;
-          const double r = d - ex[i];
-          G.R[(size_t)i * N + c] = r;
-          work = work || (r != 0.0);
+          F d = F(0.0);
+          d = d + (sHx[i * (FX + 1) * TY + ew + 1] - sHx[i * (FX + 1) * TY + ew]) * F(P.one_by_dx);
+          d = d + (sHy[i * FX * (TY + 1) + es + FX] - sHy[i * FX * (TY + 1) + es]) * F(P.one_by_dy);
+          const F r = d - ex[i];
+          gR[(size_t)i * N + c] = r;
+          work = work || (r != F(0.0));
         }
         // A cell whose state and right-hand side are all zero stays exactly
         // zero through stage 2, the assembly, sedimentation and the epilogue,
         // and raises no error unless T_ambient < 273 K: only the other cells
         // go to k_update.
 #pragma unroll
-        for (int i = 0; i < NV; i++) work = work || (G.q[(size_t)i * N + c] != 0.0);
+        for (int i = 0; i < NV; i++) work = work || (gq[(size_t)i * N + c] != F(0.0));
         work = work || (P.T_amb < 273.0);
-        double ma = fmax_f(sAx[ew], sAx[ew + 1]);
+        double ma = (double)fmax_f(sAx[ew], sAx[ew + 1]);
         const double dtx = (ma > 0.0) ? P.cfl * P.dx / ma : DINF;
-        ma = fmax_f(sAy[es], sAy[es + FX]);
+        ma = (double)fmax_f(sAy[es], sAy[es + FX]);
         const double dty = (ma > 0.0) ? P.cfl * P.dy / ma : DINF;
         const double dtc = fmin_f(dtx, dty);
         my_dt = (dtc < my_dt) ? dtc : my_dt;
@@ -838,10 +860,10 @@ __device__ __forceinline__ void flux_body(const Params& P, const Grid& G, int mo
   }
 }
 
-template <int NS>
+template <int NS, typename F>
 __global__ __launch_bounds__(NTHR) void k_flux(Params P, Grid G, int mode, double t_arg,
                                                int par) {
-  flux_body<NS>(P, G, mode, t_arg, par);
+  flux_body<NS, F>(P, G, mode, t_arg, par);
 }
 
 // The CFL time step from the blocks' minimum (timestep's result,
@@ -1369,7 +1391,9 @@ __device__ void epilogue(const Params& P, const double* q, double& hmax, double&
 // mode 1 (device-side driver): the stage-2 launch computes the step's dt
 // (every thread the same, driver_dt) and publishes it in st->dt; the launch
 // with the epilogue ends the step: dt history, t += dt, CFL minimum reset.
-template <int NS, int PARTS>
+// S: the storage type of the state and right-hand side (float in the FP32
+// form); the update itself runs in double.
+template <int NS, int PARTS, typename S = double>
 __device__ __forceinline__ void update_body(const Params& P, const Grid& G, double dt_arg, int mode,
                                             int step_idx) {
   constexpr int NV = 4 + NS;
@@ -1406,7 +1430,7 @@ __device__ __forceinline__ void update_body(const Params& P, const Grid& G, doub
     const int c = ALL ? i : G.work[i];
     double q[NV];
 #pragma unroll
-    for (int v = 0; v < NV; v++) q[v] = G.q[(size_t)v * N + c];
+    for (int v = 0; v < NV; v++) q[v] = (double)state_q<S>(G)[(size_t)v * N + c];
     if constexpr ((PARTS & 1) != 0) {
       // remember "wet at the start of the step" in bit 1 of the mask byte for
       // the dried-cell count of the sedimentation launch
@@ -1414,7 +1438,7 @@ __device__ __forceinline__ void update_body(const Params& P, const Grid& G, doub
       if (lead) G.mask[c] = (unsigned char)(1u | (wet0 ? 2u : 0u));
       double X[NV];
 #pragma unroll
-      for (int v = 0; v < NV; v++) X[v] = G.R[(size_t)v * N + c];
+      for (int v = 0; v < NV; v++) X[v] = (double)state_R<S>(G)[(size_t)v * N + c];
       stage2<NS, LANES>(P, dt, q, X, err, lane);
     }
     if constexpr ((PARTS & 2) != 0) {
@@ -1435,7 +1459,7 @@ __device__ __forceinline__ void update_body(const Params& P, const Grid& G, doub
     if constexpr ((PARTS & 3) != 0) {
       if (lead) {
 #pragma unroll
-        for (int v = 0; v < NV; v++) G.q[(size_t)v * N + c] = q[v];
+        for (int v = 0; v < NV; v++) state_q<S>(G)[(size_t)v * N + c] = S(q[v]);
       }
     }
     if constexpr ((PARTS & 12) != 0) {
@@ -1461,21 +1485,23 @@ __device__ __forceinline__ void update_body(const Params& P, const Grid& G, doub
   }
 }
 
-template <int NS, int PARTS>
+template <int NS, int PARTS, typename S>
 __global__ __launch_bounds__(NTHR) void k_update(Params P, Grid G, double dt_arg, int mode,
                                                  int step_idx) {
-  update_body<NS, PARTS>(P, G, dt_arg, mode, step_idx);
+  update_body<NS, PARTS, S>(P, G, dt_arg, mode, step_idx);
 }
 
 // ===========================================================================
 // layout conversion and diagnostics
 // ===========================================================================
-__global__ void k_aos_to_soa(const double* aos, double* soa, int nv, int N) {
+template <typename S>
+__global__ void k_aos_to_soa(const double* aos, S* soa, int nv, int N) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c < N)
     for (int i = 0; i < nv; i++) soa[(size_t)i * N + c] = aos[(size_t)c * nv + i];
 }
-__global__ void k_soa_to_aos(const double* soa, double* aos, int nv, int N) {
+template <typename S>
+__global__ void k_soa_to_aos(const S* soa, double* aos, int nv, int N) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c < N)
     for (int i = 0; i < nv; i++) aos[(size_t)c * nv + i] = soa[(size_t)i * N + c];
@@ -1483,7 +1509,7 @@ __global__ void k_soa_to_aos(const double* soa, double* aos, int nv, int N) {
 
 // Neumaier-compensated partial sums of the stdout diagnostics, one
 // (sum, compensation) pair per block and quantity; finished on the host.
-template <int NS>
+template <int NS, typename S>
 __global__ __launch_bounds__(256) void k_diag(Params P, Grid G, double* part) {
   constexpr int NV = 4 + NS, NP = NV + 2;
   __shared__ double sh[4][256];
@@ -1492,7 +1518,7 @@ __global__ __launch_bounds__(256) void k_diag(Params P, Grid G, double* part) {
   for (int c = blockIdx.x * blockDim.x + tid; c < G.N; c += gridDim.x * blockDim.x) {
     double q[NV];
 #pragma unroll
-    for (int i = 0; i < NV; i++) q[i] = G.q[(size_t)i * G.N + c];
+    for (int i = 0; i < NV; i++) q[i] = (double)state_q<S>(G)[(size_t)i * G.N + c];
     double h = 0.0;
     if (q[0] > 0.0) {
       double qp[NP], pd;
@@ -1645,6 +1671,7 @@ struct imex_handle {
   Grid G;
   hipStream_t stream = nullptr;
   int device = 0;
+  bool fp32 = false;  // the FP32 form: float state, right-hand side and fluxes
   int n_cu = 256;  // compute units of the device (persistent grid sizes)
   int ntiles = 0;
   int par = 0;  // slot of the active-tile list filled by the latest k_mask
@@ -1703,13 +1730,15 @@ static int bind_dts(imex_handle* h, const double* dts, int nsteps) {
 }
 
 template <int NS>
-static size_t flux_lds() { return flux_lds_bytes<NS>(); }
+static size_t flux_lds(bool fp32) {
+  return fp32 ? flux_lds_bytes<NS, float>() : flux_lds_bytes<NS, double>();
+}
 
-static size_t lds_for(int ns) {
+static size_t lds_for(int ns, bool fp32 = false) {
   switch (ns) {
-    case 1: return flux_lds<1>();
-    case 2: return flux_lds<2>();
-    default: return flux_lds<3>();
+    case 1: return flux_lds<1>(fp32);
+    case 2: return flux_lds<2>(fp32);
+    default: return flux_lds<3>(fp32);
   }
 }
 
@@ -1741,10 +1770,11 @@ extern "C" int imex_create(const imex_config* cfg, imex_handle** out) {
   *out = nullptr;
   if (cfg->nx < 5 || cfg->ny < 5 || cfg->n_solid < 1 || cfg->n_solid > 3 ||
       cfg->n_thickness_levels < 0 || cfg->n_thickness_levels > 8 || cfg->n_dyn_pres_levels < 0 ||
-      cfg->n_dyn_pres_levels > 8 || cfg->precision != 0)
+      cfg->n_dyn_pres_levels > 8 || cfg->precision < 0 || cfg->precision > 1)
     return -3;
   imex_handle* h = new imex_handle();
   h->cfg = *cfg;
+  h->fp32 = cfg->precision == 1;
   const char* dev = getenv("IMEX_DEVICE");
   h->device = dev ? atoi(dev) : 0;
   {
@@ -1835,8 +1865,13 @@ extern "C" int imex_create(const imex_config* cfg, imex_handle** out) {
   const int nv = P.nv, ns = P.ns;
   int rc = 0;
   *out = h;
-  if ((rc = dalloc(h, &G.q, N * nv))) return rc;
-  if ((rc = dalloc(h, &G.R, N * nv))) return rc;
+  if (h->fp32) {
+    if ((rc = dalloc(h, &G.qf, N * nv))) return rc;
+    if ((rc = dalloc(h, &G.Rf, N * nv))) return rc;
+  } else {
+    if ((rc = dalloc(h, &G.q, N * nv))) return rc;
+    if ((rc = dalloc(h, &G.R, N * nv))) return rc;
+  }
   if ((rc = dalloc(h, &G.hmax, N))) return rc;
   if ((rc = dalloc(h, &G.pdyn, N))) return rc;
   if ((rc = dalloc(h, &G.mvel, N))) return rc;
@@ -1889,7 +1924,7 @@ extern "C" int imex_create(const imex_config* cfg, imex_handle** out) {
     HCHK(h, hipStreamSynchronize(h->stream));
   }
   if ((rc = frame_band(h, nullptr))) return rc;
-  h->lds = lds_for(ns);
+  h->lds = lds_for(ns, h->fp32);
   {
     int max_lds = 0;
     HCHK(h, hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device));
@@ -1901,9 +1936,9 @@ extern "C" int imex_create(const imex_config* cfg, imex_handle** out) {
     }
   }
   switch (ns) {
-    case 1: HCHK(h, hipFuncSetAttribute((const void*)k_flux<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds)); break;
-    case 2: HCHK(h, hipFuncSetAttribute((const void*)k_flux<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds)); break;
-    default: HCHK(h, hipFuncSetAttribute((const void*)k_flux<3>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds)); break;
+    case 1: HCHK(h, hipFuncSetAttribute(h->fp32 ? (const void*)k_flux<1, float> : (const void*)k_flux<1, double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds)); break;
+    case 2: HCHK(h, hipFuncSetAttribute(h->fp32 ? (const void*)k_flux<2, float> : (const void*)k_flux<2, double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds)); break;
+    default: HCHK(h, hipFuncSetAttribute(h->fp32 ? (const void*)k_flux<3, float> : (const void*)k_flux<3, double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lds)); break;
   }
   return 0;
 }
@@ -1940,7 +1975,7 @@ static int grid_for(const imex_handle* h, int per_cu, int items = -1) {
   return std::max(1, std::min(items < 0 ? h->ntiles : items, h->n_cu * per_cu));
 }
 
-template <int NS>
+template <int NS, typename S>
 static void launch_update(imex_handle* h, int parts, double dt, int mode, int step_idx) {
   const int nb = (parts & 8) ? (h->G.N + NTHR - 1) / NTHR : grid_for(h, 4);
   const dim3 g(nb), b(NTHR);
@@ -1948,21 +1983,30 @@ static void launch_update(imex_handle* h, int parts, double dt, int mode, int st
   const Grid& G = h->G;
   hipStream_t s = h->stream;
   switch (parts) {
-    case 1: hipLaunchKernelGGL((k_update<NS, 1>), g, b, 0, s, P, G, dt, mode, step_idx); break;
-    case 2: hipLaunchKernelGGL((k_update<NS, 2>), g, b, 0, s, P, G, dt, mode, step_idx); break;
-    case 4: hipLaunchKernelGGL((k_update<NS, 4>), g, b, 0, s, P, G, dt, mode, step_idx); break;
-    case 6: hipLaunchKernelGGL((k_update<NS, 6>), g, b, 0, s, P, G, dt, mode, step_idx); break;
-    case 7: hipLaunchKernelGGL((k_update<NS, 7>), g, b, 0, s, P, G, dt, mode, step_idx); break;
-    default: hipLaunchKernelGGL((k_update<NS, 8>), g, b, 0, s, P, G, dt, mode, step_idx); break;
+    case 1: hipLaunchKernelGGL((k_update<NS, 1, S>), g, b, 0, s, P, G, dt, mode, step_idx); break;
+    case 2: hipLaunchKernelGGL((k_update<NS, 2, S>), g, b, 0, s, P, G, dt, mode, step_idx); break;
+    case 4: hipLaunchKernelGGL((k_update<NS, 4, S>), g, b, 0, s, P, G, dt, mode, step_idx); break;
+    case 6: hipLaunchKernelGGL((k_update<NS, 6, S>), g, b, 0, s, P, G, dt, mode, step_idx); break;
+    case 7: hipLaunchKernelGGL((k_update<NS, 7, S>), g, b, 0, s, P, G, dt, mode, step_idx); break;
+    default: hipLaunchKernelGGL((k_update<NS, 8, S>), g, b, 0, s, P, G, dt, mode, step_idx); break;
   }
 }
 // parts: 1, 2, 4, 8 as the host-driven entry points call them, or 1|2|4 for
 // a device-driven step (mode 1), which runs as two launches (1, then 2|4).
 static int update(imex_handle* h, int parts, double dt, int mode, int step_idx = 0) {
   switch (h->P.ns) {
-    case 1: launch_update<1>(h, parts, dt, mode, step_idx); break;
-    case 2: launch_update<2>(h, parts, dt, mode, step_idx); break;
-    default: launch_update<3>(h, parts, dt, mode, step_idx); break;
+    case 1:
+      if (h->fp32) launch_update<1, float>(h, parts, dt, mode, step_idx);
+      else launch_update<1, double>(h, parts, dt, mode, step_idx);
+      break;
+    case 2:
+      if (h->fp32) launch_update<2, float>(h, parts, dt, mode, step_idx);
+      else launch_update<2, double>(h, parts, dt, mode, step_idx);
+      break;
+    default:
+      if (h->fp32) launch_update<3, float>(h, parts, dt, mode, step_idx);
+      else launch_update<3, double>(h, parts, dt, mode, step_idx);
+      break;
   }
   HCHK(h, hipGetLastError());
   return 0;
@@ -1979,7 +2023,7 @@ static int mask(imex_handle* h, int solve_all) {
   const int full = (solve_all || h->mask_full_next) ? 1 : 0;
   h->mask_full_next = solve_all != 0;  // the pass after solve_all redoes every tile
   const int nb = full ? h->ntiles : grid_for(h, 4);
-  hipLaunchKernelGGL(k_mask, dim3(nb), dim3(NTHR), 0, h->stream, h->P, h->G, solve_all, h->par,
+  hipLaunchKernelGGL((h->fp32 ? k_mask<float> : k_mask<double>), dim3(nb), dim3(NTHR), 0, h->stream, h->P, h->G, solve_all, h->par,
                      h->mask_seq, full);
   HCHK(h, hipGetLastError());
   return 0;
@@ -1987,9 +2031,9 @@ static int mask(imex_handle* h, int solve_all) {
 static int flux(imex_handle* h, int mode, double t, int step_idx) {
   const int nb = grid_for(h, 4, 2 * h->ntiles);  // work items are half tiles
   switch (h->P.ns) {
-    case 1: hipLaunchKernelGGL(k_flux<1>, dim3(nb), dim3(NTHR), h->lds, h->stream, h->P, h->G, mode, t, h->par); break;
-    case 2: hipLaunchKernelGGL(k_flux<2>, dim3(nb), dim3(NTHR), h->lds, h->stream, h->P, h->G, mode, t, h->par); break;
-    default: hipLaunchKernelGGL(k_flux<3>, dim3(nb), dim3(NTHR), h->lds, h->stream, h->P, h->G, mode, t, h->par); break;
+    case 1: hipLaunchKernelGGL((h->fp32 ? k_flux<1, float> : k_flux<1, double>), dim3(nb), dim3(NTHR), h->lds, h->stream, h->P, h->G, mode, t, h->par); break;
+    case 2: hipLaunchKernelGGL((h->fp32 ? k_flux<2, float> : k_flux<2, double>), dim3(nb), dim3(NTHR), h->lds, h->stream, h->P, h->G, mode, t, h->par); break;
+    default: hipLaunchKernelGGL((h->fp32 ? k_flux<3, float> : k_flux<3, double>), dim3(nb), dim3(NTHR), h->lds, h->stream, h->P, h->G, mode, t, h->par); break;
   }
   HCHK(h, hipGetLastError());
   (void)step_idx;
@@ -2021,7 +2065,10 @@ extern "C" int imex_upload_state(imex_handle* h, const double* q_aos, double t, 
           break;
         }
   HCHK(h, hipMemcpyAsync(h->d_tmp, q_aos, 8 * (size_t)N * nv, hipMemcpyHostToDevice, h->stream));
-  hipLaunchKernelGGL(k_aos_to_soa, dim3((N + 255) / 256), dim3(256), 0, h->stream, h->d_tmp, h->G.q, nv, N);
+  if (h->fp32)
+    hipLaunchKernelGGL(k_aos_to_soa<float>, dim3((N + 255) / 256), dim3(256), 0, h->stream, h->d_tmp, h->G.qf, nv, N);
+  else
+    hipLaunchKernelGGL(k_aos_to_soa<double>, dim3((N + 255) / 256), dim3(256), 0, h->stream, h->d_tmp, h->G.q, nv, N);
   HCHK(h, hipGetLastError());
   DevState s;
   if (read_state(h, &s)) return -1;
@@ -2091,9 +2138,9 @@ extern "C" int imex_post_step(imex_handle* h, double t_new, imex_step_stats* out
   HCHK(h, hipMemcpyAsync(h->G.st, &s, sizeof(s), hipMemcpyHostToDevice, h->stream));
   if (out) {
     switch (h->P.ns) {
-      case 1: hipLaunchKernelGGL(k_diag<1>, dim3(h->diag_blocks), dim3(256), 0, h->stream, h->P, h->G, h->d_diag); break;
-      case 2: hipLaunchKernelGGL(k_diag<2>, dim3(h->diag_blocks), dim3(256), 0, h->stream, h->P, h->G, h->d_diag); break;
-      default: hipLaunchKernelGGL(k_diag<3>, dim3(h->diag_blocks), dim3(256), 0, h->stream, h->P, h->G, h->d_diag); break;
+      case 1: hipLaunchKernelGGL((h->fp32 ? k_diag<1, float> : k_diag<1, double>), dim3(h->diag_blocks), dim3(256), 0, h->stream, h->P, h->G, h->d_diag); break;
+      case 2: hipLaunchKernelGGL((h->fp32 ? k_diag<2, float> : k_diag<2, double>), dim3(h->diag_blocks), dim3(256), 0, h->stream, h->P, h->G, h->d_diag); break;
+      default: hipLaunchKernelGGL((h->fp32 ? k_diag<3, float> : k_diag<3, double>), dim3(h->diag_blocks), dim3(256), 0, h->stream, h->P, h->G, h->d_diag); break;
     }
     HCHK(h, hipGetLastError());
     std::vector<double> part((size_t)h->diag_blocks * 8);
@@ -2234,19 +2281,19 @@ __global__ __launch_bounds__(NTHR) void kb_mask(const Member* M, int par, unsign
   const Member& m = M[blockIdx.y];
   const DevState* st = m.G.st;
   if (st->done) return;
-  mask_body(m.P, m.G, 0, par, seq, st->full_next ? 1 : 0);
+  mask_body<double>(m.P, m.G, 0, par, seq, st->full_next ? 1 : 0);
 }
 template <int NS>
 __global__ __launch_bounds__(NTHR) void kb_flux(const Member* M, int par) {
   const Member& m = M[blockIdx.y];
   if (m.G.st->done) return;
-  flux_body<NS>(m.P, m.G, 1, 0.0, par);
+  flux_body<NS, double>(m.P, m.G, 1, 0.0, par);
 }
 template <int NS>
 __global__ __launch_bounds__(NTHR) void kb_update(const Member* M) {
   const Member& m = M[blockIdx.y];
   if (m.G.st->done) return;
-  update_body<NS, 7>(m.P, m.G, 0.0, 1, 0);
+  update_body<NS, 7, double>(m.P, m.G, 0.0, 1, 0);
 }
 __global__ void kb_commit(const Member* M, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2345,7 +2392,7 @@ extern "C" int imex_batch_set_blocks(imex_batch* b, int blocks_per_member) {
 }
 
 extern "C" int imex_batch_add(imex_batch* b, imex_handle* h, double t_end) {
-  if (!b || !h || h->P.ns != b->ns || h->device != b->device) return -2;
+  if (!b || !h || h->P.ns != b->ns || h->device != b->device || h->fp32) return -2;
   for (imex_handle* x : b->mem)
     if (x == h) return -2;
   // the member's own stream is idle from here on; its state joins the batch
@@ -2565,7 +2612,10 @@ extern "C" int imex_download(imex_handle* h, unsigned what, double* q_aos, doubl
   HCHK(h, hipSetDevice(h->device));
   const int N = h->G.N, nv = h->P.nv, ns = h->P.ns;
   if ((what & IMEX_DL_Q) && q_aos) {
-    hipLaunchKernelGGL(k_soa_to_aos, dim3((N + 255) / 256), dim3(256), 0, h->stream, h->G.q, h->d_tmp, nv, N);
+    if (h->fp32)
+      hipLaunchKernelGGL(k_soa_to_aos<float>, dim3((N + 255) / 256), dim3(256), 0, h->stream, h->G.qf, h->d_tmp, nv, N);
+    else
+      hipLaunchKernelGGL(k_soa_to_aos<double>, dim3((N + 255) / 256), dim3(256), 0, h->stream, h->G.q, h->d_tmp, nv, N);
     HCHK(h, hipGetLastError());
     HCHK(h, hipMemcpyAsync(q_aos, h->d_tmp, 8 * (size_t)N * nv, hipMemcpyDeviceToHost, h->stream));
   }

@@ -65,8 +65,10 @@ struct DevState {
 };
 
 struct Grid {
-  double* q;     // [nv][N] conservative state
-  double* R;     // [nv][N] stage-1 right-hand side divF - expl
+  double* q;     // [nv][N] conservative state (FP64 path)
+  double* R;     // [nv][N] stage-1 right-hand side divF - expl (FP64 path)
+  float* qf;     // [nv][N] the state in the FP32 form (q unused there)
+  float* Rf;     // [nv][N] the right-hand side in the FP32 form
   double* hmax;  // [N]
   double* pdyn;  // [N]
   double* mvel;  // [N]
@@ -94,6 +96,20 @@ struct Grid {
   double* dts;               // per-step dt record of the device-side driver
   int nx, ny, N, ntx, nty;
 };
+
+// the state and right-hand side arrays of the storage type S
+template <typename S>
+__device__ __forceinline__ S* state_q(const Grid& G);
+template <>
+__device__ __forceinline__ double* state_q<double>(const Grid& G) { return G.q; }
+template <>
+__device__ __forceinline__ float* state_q<float>(const Grid& G) { return G.qf; }
+template <typename S>
+__device__ __forceinline__ S* state_R(const Grid& G);
+template <>
+__device__ __forceinline__ double* state_R<double>(const Grid& G) { return G.R; }
+template <>
+__device__ __forceinline__ float* state_R<float>(const Grid& G) { return G.Rf; }
 
 // Block index -> tile index so that each XCD (blocks are dispatched round-robin
 // over the 8 XCDs) works on a contiguous band of tiles and finds its halo

@@ -44,8 +44,10 @@ struct Params {
 // Fortran MIN(a, b) / MAX(a, b) as the reference build evaluates them:
 // (a < b) ? a : b and (a > b) ? a : b, so a NaN or a signed-zero tie
 // returns the second argument.
-__device__ __forceinline__ double fmin_f(double a, double b) { return (a < b) ? a : b; }
-__device__ __forceinline__ double fmax_f(double a, double b) { return (a > b) ? a : b; }
+template <typename T>
+__device__ __forceinline__ T fmin_f(T a, T b) { return (a < b) ? a : b; }
+template <typename T>
+__device__ __forceinline__ T fmax_f(T a, T b) { return (a > b) ? a : b; }
 
 // ---------------------------------------------------------------------------
 // complex arithmetic as the reference build evaluates it
@@ -162,22 +164,28 @@ __device__ inline cx csqrt_c(cx z) {
 // constitutive_2d
 // ===========================================================================
 
+// The functions of the explicit part (primitive variables, local speeds,
+// physical fluxes, explicit terms, reconstruction) take the arithmetic type T
+// as a template parameter: double everywhere on the FP64 path (then every
+// T(...) below is an identity and the operations are the reference's), float
+// in the FP32 form's flux kernel.
+
 // r_phys_var (configured branch) + qc_to_qp:
 // src/constitutive_2d.f90:550-976 and :1637-1690.  Also returns p_dyn.
-template <int NS>
-__device__ inline void qc_to_qp(const Params& P, const double* qc, double* qp, double& p_dyn) {
+template <int NS, typename T = double>
+__device__ inline void qc_to_qp(const Params& P, const T* qc, T* qp, T& p_dyn) {
   constexpr int NV = 4 + NS;
-  double xs[NS];
-  double inv_qj1;
-  if (qc[0] > kEps) {
-    inv_qj1 = 1.0 / qc[0];
+  T xs[NS];
+  T inv_qj1;
+  if (qc[0] > T(kEps)) {
+    inv_qj1 = T(1.0) / qc[0];
 #pragma unroll
     for (int i = 0; i < NS; i++) xs[i] = qc[4 + i] * inv_qj1;
-    double s = 0.0;
+    T s = T(0.0);
 #pragma unroll
     for (int i = 0; i < NS; i++) s += qc[4 + i];
     if (s == qc[0]) {
-      double sx = 0.0;
+      T sx = T(0.0);
 #pragma unroll
       for (int i = 0; i < NS; i++) sx += xs[i];
 #pragma unroll
@@ -185,171 +193,173 @@ __device__ inline void qc_to_qp(const Params& P, const double* qc, double* qp, d
     }
   } else {
 #pragma unroll
-    for (int i = 0; i < NV + 2; i++) qp[i] = 0.0;
-    qp[3] = P.T_amb;
-    p_dyn = 0.0;
+    for (int i = 0; i < NV + 2; i++) qp[i] = T(0.0);
+    qp[3] = T(P.T_amb);
+    p_dyn = T(0.0);
     return;
   }
-  double xs_tot = 0.0;
+  T xs_tot = T(0.0);
 #pragma unroll
   for (int i = 0; i < NS; i++) xs_tot += xs[i];
-  double xc = 1.0 - xs_tot;
-  double cpc_by_xc = (xc - 0.0) * P.cp_a + 0.0;
-  double dot = 0.0;
+  T xc = T(1.0) - xs_tot;
+  T cpc_by_xc = (xc - T(0.0)) * T(P.cp_a) + T(0.0);
+  T dot = T(0.0);
 #pragma unroll
-  for (int i = 0; i < NS; i++) dot += xs[i] * P.cp_s[i];
-  double cp_mix = dot + cpc_by_xc;
-  double T;
-  if (qc[0] > P.eps_sing) {
+  for (int i = 0; i < NS; i++) dot += xs[i] * T(P.cp_s[i]);
+  T cp_mix = dot + cpc_by_xc;
+  T Tm;
+  if (qc[0] > T(P.eps_sing)) {
     if (P.energy)
-      T = (qc[3] - 0.5 * (qc[1] * qc[1] + qc[2] * qc[2]) * inv_qj1) / (qc[0] * cp_mix);
+      Tm = (qc[3] - T(0.5) * (qc[1] * qc[1] + qc[2] * qc[2]) * inv_qj1) / (qc[0] * cp_mix);
     else
-      T = qc[3] / (qc[0] * cp_mix);
-    if (T <= 0.0) T = P.T_amb;
+      Tm = qc[3] / (qc[0] * cp_mix);
+    if (Tm <= T(0.0)) Tm = T(P.T_amb);
   } else {
-    T = P.T_amb;
+    Tm = T(P.T_amb);
   }
-  double Rc;
-  if (xc > kEps)
-    Rc = ((xc - 0.0) * P.R_a + 0.0) / xc;
+  T Rc;
+  if (xc > T(kEps))
+    Rc = ((xc - T(0.0)) * T(P.R_a) + T(0.0)) / xc;
   else
-    Rc = P.R_a;
-  double inv_rho_c = Rc * T * P.inv_pres;
-  double inv_rhom = 0.0;
+    Rc = T(P.R_a);
+  T inv_rho_c = Rc * Tm * T(P.inv_pres);
+  T inv_rhom = T(0.0);
 #pragma unroll
-  for (int i = 0; i < NS; i++) inv_rhom += xs[i] * P.inv_rho_s[i];
+  for (int i = 0; i < NS; i++) inv_rhom += xs[i] * T(P.inv_rho_s[i]);
   inv_rhom = inv_rhom + xc * inv_rho_c;
-  double rm = 1.0 / inv_rhom;
-  double h = qc[0] * inv_rhom;
-  double u, v;
-  if (qc[0] > P.eps_sing) {
+  T rm = T(1.0) / inv_rhom;
+  T h = qc[0] * inv_rhom;
+  T u, v;
+  if (qc[0] > T(P.eps_sing)) {
     u = qc[1] * inv_qj1;
     v = qc[2] * inv_qj1;
   } else {
-    double q1 = qc[0];
-    double den = sqrt(q1 * (q1 * (q1 * q1)) + P.eps_sing4);
-    u = kSqrt2 * q1 * qc[1] / den;
-    v = kSqrt2 * q1 * qc[2] / den;
+    T q1 = qc[0];
+    T den = sqrt(q1 * (q1 * (q1 * q1)) + T(P.eps_sing4));
+    u = T(kSqrt2) * q1 * qc[1] / den;
+    v = T(kSqrt2) * q1 * qc[2] / den;
   }
-  p_dyn = 0.5 * rm * (u * u + v * v);
+  p_dyn = T(0.5) * rm * (u * u + v * v);
   qp[0] = h;
   qp[1] = h * u;
   qp[2] = h * v;
-  qp[3] = T;
+  qp[3] = Tm;
 #pragma unroll
-  for (int i = 0; i < NS; i++) qp[4 + i] = xs[i] * rm * P.inv_rho_s[i] * h;
+  for (int i = 0; i < NS; i++) qp[4 + i] = xs[i] * rm * T(P.inv_rho_s[i]) * h;
   qp[NV] = u;
   qp[NV + 1] = v;
 }
 
 // qp_to_qc (configured branch): src/constitutive_2d.f90:1718-2149
-template <int NS>
-__device__ inline void qp_to_qc(const Params& P, const double* qp, double* qc) {
+template <int NS, typename T = double>
+__device__ inline void qp_to_qc(const Params& P, const T* qp, T* qc) {
   constexpr int NV = 4 + NS;
-  double h = qp[0];
-  if (!(h > kEps)) {
+  T h = qp[0];
+  if (!(h > T(kEps))) {
 #pragma unroll
-    for (int i = 0; i < NV; i++) qc[i] = 0.0;
+    for (int i = 0; i < NV; i++) qc[i] = T(0.0);
     return;
   }
-  double hu = qp[1], hv = qp[2], u = qp[NV], v = qp[NV + 1], T = qp[3];
-  double alphas[NS], ar[NS];
+  T hu = qp[1], hv = qp[2], u = qp[NV], v = qp[NV + 1], Tm = qp[3];
+  T alphas[NS], ar[NS];
 #pragma unroll
   for (int i = 0; i < NS; i++) alphas[i] = qp[4 + i] / qp[0];
-  double at = 0.0;
+  T at = T(0.0);
 #pragma unroll
   for (int i = 0; i < NS; i++) at += alphas[i];
 #pragma unroll
-  for (int i = 0; i < NS; i++) ar[i] = alphas[i] * P.rho_s[i];
-  double rho_a = P.pres / (P.R_a * T);
-  double rho_c = (((1.0 - 0.0) - at - 0.0) * rho_a + 0.0) / ((1.0 - 0.0) - at);
-  double alphac = 1.0 - at;
-  double sar = 0.0;
+  for (int i = 0; i < NS; i++) ar[i] = alphas[i] * T(P.rho_s[i]);
+  T rho_a = T(P.pres) / (T(P.R_a) * Tm);
+  T rho_c = (((T(1.0) - T(0.0)) - at - T(0.0)) * rho_a + T(0.0)) / ((T(1.0) - T(0.0)) - at);
+  T alphac = T(1.0) - at;
+  T sar = T(0.0);
 #pragma unroll
   for (int i = 0; i < NS; i++) sar += ar[i];
-  double rho_m = alphac * rho_c + sar;
-  double inv_rhom = 1.0 / rho_m;
-  double xs[NS];
+  T rho_m = alphac * rho_c + sar;
+  T inv_rhom = T(1.0) / rho_m;
+  T xs[NS];
 #pragma unroll
   for (int i = 0; i < NS; i++) xs[i] = ar[i] * inv_rhom;
-  double xc = alphac * rho_c * inv_rhom;
-  double cp_c;
-  if (xc > kEps)
-    cp_c = ((xc - 0.0) * P.cp_a + 0.0) / xc;
+  T xc = alphac * rho_c * inv_rhom;
+  T cp_c;
+  if (xc > T(kEps))
+    cp_c = ((xc - T(0.0)) * T(P.cp_a) + T(0.0)) / xc;
   else
-    cp_c = P.cp_a;
-  double dot = 0.0;
+    cp_c = T(P.cp_a);
+  T dot = T(0.0);
 #pragma unroll
-  for (int i = 0; i < NS; i++) dot += xs[i] * P.cp_s[i];
-  double cp_mix = dot + xc * cp_c;
+  for (int i = 0; i < NS; i++) dot += xs[i] * T(P.cp_s[i]);
+  T cp_mix = dot + xc * cp_c;
   qc[0] = rho_m * h;
   qc[1] = rho_m * hu;
   qc[2] = rho_m * hv;
   if (P.energy) {
-    if (h > 0.0)
-      qc[3] = h * rho_m * (cp_mix * T + 0.5 * (u * u + v * v));
+    if (h > T(0.0))
+      qc[3] = h * rho_m * (cp_mix * Tm + T(0.5) * (u * u + v * v));
     else
-      qc[3] = 0.0;
+      qc[3] = T(0.0);
   } else {
-    qc[3] = h * rho_m * cp_mix * T;
+    qc[3] = h * rho_m * cp_mix * Tm;
   }
 #pragma unroll
   for (int i = 0; i < NS; i++) qc[4 + i] = xs[i] * qc[0];
 }
 
-struct Mixt {
-  double Ri, rho_m, rho_c, red_grav, cp_c, cp_mix;
+template <typename T = double>
+struct MixtT {
+  T Ri, rho_m, rho_c, red_grav, cp_c, cp_mix;
 };
+using Mixt = MixtT<double>;
 
 // mixt_var (+ eval_sp_heat when sp_heat): src/constitutive_2d.f90:1342-1607
-template <int NS>
-__device__ inline Mixt mixt_var(const Params& P, const double* qp, bool sp_heat) {
+template <int NS, typename T = double>
+__device__ inline MixtT<T> mixt_var(const Params& P, const T* qp, bool sp_heat) {
   constexpr int NV = 4 + NS;
-  Mixt m;
-  double h = qp[0];
-  if (qp[0] <= kEps) {
-    m.red_grav = 0.0;
-    m.rho_m = P.rho_a_amb;
-    m.rho_c = P.rho_a_amb;
-    m.cp_c = P.cp_a;
-    m.cp_mix = P.cp_a;
-    m.Ri = 0.0;
+  MixtT<T> m;
+  T h = qp[0];
+  if (qp[0] <= T(kEps)) {
+    m.red_grav = T(0.0);
+    m.rho_m = T(P.rho_a_amb);
+    m.rho_c = T(P.rho_a_amb);
+    m.cp_c = T(P.cp_a);
+    m.cp_mix = T(P.cp_a);
+    m.Ri = T(0.0);
     return m;
   }
-  double u = qp[NV], v = qp[NV + 1], T = qp[3];
-  double alphas[NS];
+  T u = qp[NV], v = qp[NV + 1], Tm = qp[3];
+  T alphas[NS];
 #pragma unroll
   for (int i = 0; i < NS; i++) alphas[i] = qp[4 + i] / qp[0];
-  double at = 0.0;
+  T at = T(0.0);
 #pragma unroll
   for (int i = 0; i < NS; i++) at += alphas[i];
-  double rho_a = P.pres / (P.R_a * T);
-  double rc = (((1.0 - 0.0) - at - 0.0) * rho_a + 0.0) / ((1.0 - 0.0) - at);
-  double dot = 0.0;
+  T rho_a = T(P.pres) / (T(P.R_a) * Tm);
+  T rc = (((T(1.0) - T(0.0)) - at - T(0.0)) * rho_a + T(0.0)) / ((T(1.0) - T(0.0)) - at);
+  T dot = T(0.0);
 #pragma unroll
-  for (int i = 0; i < NS; i++) dot += alphas[i] * P.rho_s[i];
-  double rm = (1.0 - at) * rc + dot;
-  double inv_rhom = 1.0 / rm;
-  double rg = (rm - P.rho_a_amb) / rm * P.grav;
-  double u2 = u * u + v * v;
-  m.Ri = (u2 > 0.0) ? fmin_f(1.0e15, rg * h / u2) : 0.0;
+  for (int i = 0; i < NS; i++) dot += alphas[i] * T(P.rho_s[i]);
+  T rm = (T(1.0) - at) * rc + dot;
+  T inv_rhom = T(1.0) / rm;
+  T rg = (rm - T(P.rho_a_amb)) / rm * T(P.grav);
+  T u2 = u * u + v * v;
+  m.Ri = (u2 > T(0.0)) ? fmin_f(T(1.0e15), rg * h / u2) : T(0.0);
   m.rho_m = rm;
   m.rho_c = rc;
   m.red_grav = rg;
-  m.cp_c = 0.0;
-  m.cp_mix = 0.0;
+  m.cp_c = T(0.0);
+  m.cp_mix = T(0.0);
   if (sp_heat) {
-    double xs[NS];
+    T xs[NS];
 #pragma unroll
-    for (int i = 0; i < NS; i++) xs[i] = alphas[i] * P.rho_s[i] * inv_rhom;
-    double sxs = 0.0;
+    for (int i = 0; i < NS; i++) xs[i] = alphas[i] * T(P.rho_s[i]) * inv_rhom;
+    T sxs = T(0.0);
 #pragma unroll
     for (int i = 0; i < NS; i++) sxs += xs[i];
-    double xc = 1.0 - sxs;
-    double c = (xc > kEps) ? ((xc - 0.0) * P.cp_a + 0.0) / xc : P.cp_a;
-    double d2 = 0.0;
+    T xc = T(1.0) - sxs;
+    T c = (xc > T(kEps)) ? ((xc - T(0.0)) * T(P.cp_a) + T(0.0)) / xc : T(P.cp_a);
+    T d2 = T(0.0);
 #pragma unroll
-    for (int i = 0; i < NS; i++) d2 += xs[i] * P.cp_s[i];
+    for (int i = 0; i < NS; i++) d2 += xs[i] * T(P.cp_s[i]);
     m.cp_c = c;
     m.cp_mix = d2 + xc * c;
   }
@@ -357,77 +367,76 @@ __device__ inline Mixt mixt_var(const Params& P, const double* qp, bool sp_heat)
 }
 
 // eval_local_speeds_x / _y with grav_coeff = 1: src/constitutive_2d.f90:2203-2307
-template <int NS>
-__device__ inline void local_speeds(const Params& P, const double* qp, int dir, double& vmin,
-                                    double& vmax) {
+template <int NS, typename T = double>
+__device__ inline void local_speeds(const Params& P, const T* qp, int dir, T& vmin, T& vmax) {
   constexpr int NV = 4 + NS;
-  Mixt m = mixt_var<NS>(P, qp, false);
-  double h = qp[0];
-  double w = (dir == 1) ? qp[NV] : qp[NV + 1];
-  double rg = m.red_grav;
-  if (rg * h < 0.0) {
+  MixtT<T> m = mixt_var<NS, T>(P, qp, false);
+  T h = qp[0];
+  T w = (dir == 1) ? qp[NV] : qp[NV + 1];
+  T rg = m.red_grav;
+  if (rg * h < T(0.0)) {
     vmin = w;
     vmax = w;
   } else {
-    double cel = (dir == 1) ? sqrt(rg * h * 1.0) : sqrt(1.0 * rg * h);
+    T cel = (dir == 1) ? sqrt(rg * h * T(1.0)) : sqrt(T(1.0) * rg * h);
     vmin = w - cel;
     vmax = w + cel;
   }
 }
 
 // eval_fluxes with grav_coeff = 1: src/constitutive_2d.f90:2325-2518
-template <int NS>
-__device__ inline void phys_flux(const Params& P, const double* qc, const double* qp, int dir,
-                                 double* f) {
+template <int NS, typename T = double>
+__device__ inline void phys_flux(const Params& P, const T* qc, const T* qp, int dir, T* f) {
   constexpr int NV = 4 + NS;
-  if (qp[0] > kEps) {
-    double h = qp[0];
-    double w = (dir == 1) ? qp[NV] : qp[NV + 1];
-    Mixt m = mixt_var<NS>(P, qp, false);
-    double Pr = 0.5 * m.rho_m * 1.0 * m.red_grav * (h * h);
-    f[0] = w * qc[0] * 1.0;
+  if (qp[0] > T(kEps)) {
+    T h = qp[0];
+    T w = (dir == 1) ? qp[NV] : qp[NV + 1];
+    MixtT<T> m = mixt_var<NS, T>(P, qp, false);
+    T Pr = T(0.5) * m.rho_m * T(1.0) * m.red_grav * (h * h);
+    f[0] = w * qc[0] * T(1.0);
     if (dir == 1) {
-      f[1] = w * qc[1] * 1.0 + Pr;
-      f[2] = w * qc[2] * 1.0;
+      f[1] = w * qc[1] * T(1.0) + Pr;
+      f[2] = w * qc[2] * T(1.0);
     } else {
-      f[1] = w * qc[1] * 1.0;
-      f[2] = w * qc[2] * 1.0 + Pr;
+      f[1] = w * qc[1] * T(1.0);
+      f[2] = w * qc[2] * T(1.0) + Pr;
     }
-    f[3] = P.energy ? w * (qc[3] * 1.0 + Pr) : w * qc[3] * 1.0;
+    f[3] = P.energy ? w * (qc[3] * T(1.0) + Pr) : w * qc[3] * T(1.0);
 #pragma unroll
-    for (int i = 0; i < NS; i++) f[4 + i] = w * qc[4 + i] * 1.0;
-    double S = 0.0;
+    for (int i = 0; i < NS; i++) f[4 + i] = w * qc[4 + i] * T(1.0);
+    T S = T(0.0);
 #pragma unroll
     for (int i = 0; i < NS; i++) S += f[4 + i];
-    if ((f[0] > 0.0) && (S / f[0] > 1.0)) {
+    if ((f[0] > T(0.0)) && (S / f[0] > T(1.0))) {
 #pragma unroll
       for (int i = 0; i < NS; i++) f[4 + i] = f[4 + i] / S * f[0];
     }
   } else {
 #pragma unroll
-    for (int i = 0; i < NV; i++) f[i] = 0.0;
+    for (int i = 0; i < NV; i++) f[i] = T(0.0);
   }
 }
 
 // eval_expl_terms (gravity, lateral radial volume source), grav_coeff = 1,
-// no curvature, d_grav_coeff = 0: src/constitutive_2d.f90:3024-3431
-template <int NS>
-__device__ inline void expl_terms(const Params& P, double Bx, double By, const double* qp,
-                                  double* ex, double time, double arc_p, double arc_x,
-                                  double arc_y) {
+// no curvature, d_grav_coeff = 0: src/constitutive_2d.f90:3024-3431.  The
+// source's time factor and inflow state are computed in double and rounded
+// to T.
+template <int NS, typename T = double>
+__device__ inline void expl_terms(const Params& P, T Bx, T By, const T* qp, T* ex, double time,
+                                  T arc_p, T arc_x, T arc_y) {
   constexpr int NV = 4 + NS;
 #pragma unroll
-  for (int i = 0; i < NV; i++) ex[i] = 0.0;
-  if ((qp[0] <= kEps) && (arc_p == 0.0)) return;
-  if (qp[0] > kEps) {
-    double h = qp[0], u = qp[NV], v = qp[NV + 1];
-    Mixt m = mixt_var<NS>(P, qp, true);
-    double tg = m.red_grav + 0.0;
-    ex[1] = -(1.0 * m.rho_m * tg * h * Bx) + 0.5 * m.rho_m * m.red_grav * (h * h) * 0.0;
-    ex[2] = -(1.0 * m.rho_m * tg * h * By) + 0.5 * m.rho_m * m.red_grav * (h * h) * 0.0;
-    ex[3] = P.energy ? ex[1] * u + ex[2] * v : 0.0;
+  for (int i = 0; i < NV; i++) ex[i] = T(0.0);
+  if ((qp[0] <= T(kEps)) && (arc_p == T(0.0))) return;
+  if (qp[0] > T(kEps)) {
+    T h = qp[0], u = qp[NV], v = qp[NV + 1];
+    MixtT<T> m = mixt_var<NS, T>(P, qp, true);
+    T tg = m.red_grav + T(0.0);
+    ex[1] = -(T(1.0) * m.rho_m * tg * h * Bx) + T(0.5) * m.rho_m * m.red_grav * (h * h) * T(0.0);
+    ex[2] = -(T(1.0) * m.rho_m * tg * h * By) + T(0.5) * m.rho_m * m.red_grav * (h * h) * T(0.0);
+    ex[3] = P.energy ? ex[1] * u + ex[2] * v : T(0.0);
   }
-  if (P.radial && (arc_p > 0.0) && (P.h_source > 0.0)) {
+  if (P.radial && (arc_p > T(0.0)) && (P.h_source > 0.0)) {
     const double* tp = P.time_param;
     double t_coeff;
     if (time >= tp[3]) {
@@ -458,14 +467,16 @@ __device__ inline void expl_terms(const Params& P, double Bx, double By, const d
 #pragma unroll
       for (int i = 0; i < NS; i++) qs[4 + i] = P.alphas_source[i] * qs[0];
       Mixt m = mixt_var<NS>(P, qs, true);
-      double h_dot = P.h_source * P.vel_source * arc_p / (P.dx * P.dy);
-      ex[0] = ex[0] + t_coeff * h_dot * m.rho_m;
-      ex[1] = ex[1] + t_coeff * h_dot * m.rho_m * P.vel_source * arc_x;
-      ex[2] = ex[2] + t_coeff * h_dot * m.rho_m * P.vel_source * arc_y;
-      ex[3] = ex[3] + t_coeff * h_dot * m.rho_m * m.cp_mix * P.T_source;
+      const T tc = T(t_coeff);
+      const T h_dot = T(P.h_source) * T(P.vel_source) * arc_p / (T(P.dx) * T(P.dy));
+      const T rmx = T(m.rho_m);
+      ex[0] = ex[0] + tc * h_dot * rmx;
+      ex[1] = ex[1] + tc * h_dot * rmx * T(P.vel_source) * arc_x;
+      ex[2] = ex[2] + tc * h_dot * rmx * T(P.vel_source) * arc_y;
+      ex[3] = ex[3] + tc * h_dot * rmx * T(m.cp_mix) * T(P.T_source);
 #pragma unroll
       for (int i = 0; i < NS; i++)
-        ex[4 + i] = ex[4 + i] + t_coeff * h_dot * P.alphas_source[i] * P.rho_s[i];
+        ex[4 + i] = ex[4 + i] + tc * h_dot * T(P.alphas_source[i]) * T(P.rho_s[i]);
     }
   }
 }
@@ -809,21 +820,22 @@ __device__ inline bool mass_exchange(const Params& P, const double* qp, double d
 }
 
 // x / |x| as the correctly rounded quotient gives it
-__device__ __forceinline__ double sign_quot(double x) {
-  return (fabs(x) > 0.0 && fabs(x) < INFINITY) ? copysign(1.0, x) : __builtin_nan("");
+template <typename T>
+__device__ __forceinline__ T sign_quot(T x) {
+  return (fabs(x) > T(0.0) && fabs(x) < T(INFINITY)) ? copysign(T(1.0), x) : T(__builtin_nan(""));
 }
 
 // limit with the minmod limiter: src/geometry_2d.f90:1776-1806, :1855-1876
-__device__ __forceinline__ double limit_minmod(double v0, double v1, double v2, double z0,
-                                               double z1, double z2) {
-  double a = (v2 - v1) / (z2 - z1);
-  double b = (v1 - v0) / (z1 - z0);
-  double m = fmin_f(fabs(a), fabs(b));
-  if (m <= 0.0) return 0.0;
+template <typename T>
+__device__ __forceinline__ T limit_minmod(T v0, T v1, T v2, T z0, T z1, T z2) {
+  T a = (v2 - v1) / (z2 - z1);
+  T b = (v1 - v0) / (z1 - z0);
+  T m = fmin_f(fabs(a), fabs(b));
+  if (m <= T(0.0)) return T(0.0);
   // a / |a| without the division: exactly +-1 for a finite non-zero a and
   // NaN for 0, +-Inf and NaN (0/0, Inf/Inf, NaN), as the quotient is
-  double sa = sign_quot(a), sb = sign_quot(b);
-  return 0.5 * (sa + sb) * m;
+  T sa = sign_quot(a), sb = sign_quot(b);
+  return T(0.5) * (sa + sb) * m;
 }
 
 }  // namespace imex

@@ -113,7 +113,7 @@ def make_config(cfg: RunConfig) -> ImexConfig:
     c.radial_source_flag = int(cfg.radial_source_flag)
     c.n_thickness_levels = len(cfg.thickness_levels)
     c.n_dyn_pres_levels = len(cfg.dyn_pres_levels)
-    c.precision = 0
+    c.precision = int(getattr(cfg, "precision", 0))
     c.dx, c.x0, c.y0 = cfg.dx, cfg.x0, cfg.y0
     c.cfl, c.max_dt = cfg.cfl, cfg.max_dt
     c.grav, c.mu, c.xi, c.pres = cfg.grav, cfg.mu, cfg.xi, cfg.pres

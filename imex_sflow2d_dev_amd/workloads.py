@@ -177,6 +177,7 @@ def config3(nx: int = 4096, ny: int = 4096, dx: float = 2.5, seed_member: int = 
     cfg.diam_s = [1.0e-2, 1.0e-3, 1.0e-4]
     cfg.sp_heat_s = [1200.0, 1200.0, 1200.0]
     cfg.alphas_collapse = [0.02, 0.02, 0.01]
+    cfg.alphas_source = [0.0, 0.0, 0.0]
     cfg.settling_flag = True
     cfg.r_collapse = 0.03 * nx * dx
     cfg.validate()

@@ -44,7 +44,10 @@ typedef struct imex_config {
   int32_t radial_source_flag;
   int32_t n_thickness_levels;  /* 0..8 */
   int32_t n_dyn_pres_levels;   /* 0..8 */
-  int32_t precision;           /* 0 = FP64 drop-in (only value accepted) */
+  int32_t precision;           /* 0 = FP64 drop-in; 1 = FP32 form: state, right-hand side,
+                                  reconstruction and fluxes in FP32, the Newton solve,
+                                  thin-layer (h < 1e-3 m) primitive variables, deposit,
+                                  maxima and the mass budget in FP64 (not in batches) */
   double dx;                   /* cell_size (dx = dy) */
   double x0, y0;
   double cfl, max_dt;

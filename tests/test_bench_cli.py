@@ -24,3 +24,12 @@ def test_algorithmic_bytes_per_active_cell():
     assert per["k_flux"] == 64 and per["k_mask"] == 9
     assert per["k_update"] == 176
     assert set(per) == set(bench.KERNELS) | {"step"}
+
+
+def test_every_named_workload_validates():
+    for name in workloads.WORKLOADS:
+        fn = workloads.WORKLOADS[name]
+        cfg, B = fn(nx=32, ny=32) if name != "c2-small" else fn(nx=32, ny=32)
+        assert B.shape == (cfg.ny, cfg.nx)
+    cfg, B = workloads.config4_member(3, min_side=40, max_side=60)
+    assert B.shape == (cfg.ny, cfg.nx)

@@ -1,0 +1,24 @@
+"""The FP32 form (state, reconstruction and fluxes in FP32; Newton, thin
+layers, deposit and maxima in FP64) against the FP64 drop-in on the same
+run: runout area, maximum thickness, deposit and mass agree to the stated
+tolerance, and the FP32 run keeps the mass budget."""
+import numpy as np
+import pytest
+
+from imex_sflow2d_dev_amd import init, precision, workloads
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("n, dx, steps", [(96, 20.0, 400), (160, 12.0, 500)])
+def test_fp32_form_against_fp64(n, dx, steps):
+    cfg, B = workloads.config3(nx=n, ny=n, dx=dx)
+    st, q0 = init.setup(cfg, B)
+    r = precision.compare(cfg, st, q0, steps)
+    f64, f32 = r["fp64"], r["fp32"]
+    assert f64["err_bits"] == 0 and f32["err_bits"] == 0
+    assert f32["t_end_s"] == pytest.approx(f64["t_end_s"], rel=1e-3)
+    rel = r["relative_difference"]
+    # the stated bar of the precision form: 1e-3 relative on these quantities
+    for k in ("runout_area_m2", "max_thickness_m", "deposit_volume_m3", "total_mass_kg"):
+        assert rel[k] < 1e-3, (k, rel[k])
