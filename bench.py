@@ -173,6 +173,9 @@ def main():
     if args.workload == "c5":
         bench_c5(args)
         return
+    if args.workload == "c3":
+        bench_c3(args)
+        return
 
     t_host = time.perf_counter()
     cfg, B = workloads.by_name(args.workload)
@@ -326,6 +329,47 @@ def bench_c4(args):
         "data": "synthetic (seeded PCG64 fractal DEMs and member parameters, generated in-process)",
         "config": dict({"workload": (f"c4: {r['members']} members, sides log-uniform in 128..1024 "
                                      f"cells (dx 10 m), each to t_end = {r['t_end_s']} s, FP64")}, **r),
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_c3(args):
+    """The precision form: the 4096^2, three-class run for --steps steps
+    (3000 by default here) in FP64 and in the FP32 form; value = the FP32
+    form's cell-stage updates/s, beside the FP64 rate and the relative
+    differences of runout area, maximum thickness, deposit and mass."""
+    from imex_sflow2d_dev_amd import init, precision, workloads
+    steps = args.steps if args.steps != 5000 else 3000
+    say = lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True)  # noqa: E731
+    cfg, B = workloads.config3()
+    static, q0 = init.setup(cfg, B)
+    r = precision.compare(cfg, static, q0, steps, log=say)
+    # the same run with sedimentation on, for the deposit (the 1 cm class
+    # settles within a few hundred steps, so this variant is not timed)
+    cfg_s, _ = workloads.config3()
+    cfg_s.settling_flag = True
+    rs = precision.compare(cfg_s, static, q0, steps, log=say)
+    out = {
+        "metric": "cell-stage updates/s (FP32 form; FP64 beside it)",
+        "value": r["fp32"]["cell_stage_updates_per_s"],
+        "unit": "cell-stage updates/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": 0,
+        "ms_per_step": r["fp32"]["wall_s"] * 1e3 / steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (state, reconstruction, fluxes) / f64 (Newton, thin layers, deposit, budget)",
+        "data": "synthetic (seeded PCG64 DEM, generated in-process)",
+        "config": {"workload": (f"c3: {cfg.nx}x{cfg.ny} cells dx={cfg.dx} m, {cfg.n_solid} solid classes, "
+                                f"settling={cfg.settling_flag}, driver steps 1..{steps}"),
+                   "fp64": r["fp64"], "fp32": r["fp32"],
+                   "relative_difference_fp32_vs_fp64": r["relative_difference"],
+                   "speedup_fp32": r["speedup_fp32"],
+                   "with_sedimentation": {"fp64": rs["fp64"], "fp32": rs["fp32"],
+                                          "relative_difference_fp32_vs_fp64":
+                                              rs["relative_difference"]}},
     }
     print(json.dumps(out), flush=True)
 

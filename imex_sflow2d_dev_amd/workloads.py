@@ -166,10 +166,10 @@ def small_config2(nx: int = 96, ny: int = 96, dx: float = 20.0):
 
 
 def config3(nx: int = 4096, ny: int = 4096, dx: float = 2.5, seed_member: int = 0):
-    """The config-1 physics on a larger grid with three solid classes
-    (1e-2/1e-3/1e-4 m).  The per-class fractions are not fixed by the workload
-    description: 0.02/0.02/0.01 is used, and the collapsing volume is scaled
-    with the domain (radius 3 % of the domain side)."""
+    """The config-1 physics (no sedimentation) on a larger grid with three
+    solid classes (1e-2/1e-3/1e-4 m).  The per-class fractions are not fixed
+    by the workload description: 0.02/0.02/0.01 is used, and the collapsing
+    volume is scaled with the domain (radius 3 % of the domain side)."""
     cfg, B = config1(nx=nx, ny=ny, dx=dx, seed_member=seed_member)
     cfg.name = "C3"
     cfg.n_solid = 3
@@ -178,7 +178,6 @@ def config3(nx: int = 4096, ny: int = 4096, dx: float = 2.5, seed_member: int = 
     cfg.sp_heat_s = [1200.0, 1200.0, 1200.0]
     cfg.alphas_collapse = [0.02, 0.02, 0.01]
     cfg.alphas_source = [0.0, 0.0, 0.0]
-    cfg.settling_flag = True
     cfg.r_collapse = 0.03 * nx * dx
     cfg.validate()
     return cfg, B
