@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIBDIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(LIBDIR, "libimexhip.so")
-SRC = os.path.join(HERE, "csrc", "imex_hip.hip")
+SRCS = [os.path.join(HERE, "csrc", f) for f in ("imex_hip.hip", "imex_output.hip")]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("IMEX_OFFLOAD_ARCH", "gfx950")
 
@@ -22,7 +22,7 @@ def compile_cmd(out: str = LIB):
     return [HIPCC, f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
             "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
             "-I", os.path.join(ROOT, "include"), "-I", os.path.join(HERE, "csrc"),
-            "-o", out, SRC]
+            "-o", out] + SRCS
 
 
 def build(verbose: bool = False) -> str:

@@ -1,0 +1,64 @@
+// Host-side handle of one run and the helpers the C-ABI translation units
+// share (imex_hip.hip: the step; imex_output.hip: the outputs).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "imex_hip.h"
+#include "imex_kernels.h"
+
+using namespace imex;
+
+struct imex_handle {
+  imex_config cfg;
+  Params P;
+  Grid G;
+  hipStream_t stream = nullptr;
+  int device = 0;
+  bool fp32 = false;  // the FP32 form: float state, right-hand side and fluxes
+  int n_cu = 256;  // compute units of the device (persistent grid sizes)
+  int ntiles = 0;
+  int par = 0;  // slot of the active-tile list filled by the latest k_mask
+  unsigned int mask_seq = 0;  // k_mask launches so far (candidate-list slots)
+  bool mask_full_next = true;  // the next k_mask visits every tile
+  size_t lds = 0;
+  std::vector<void*> allocs;
+  double* d_tmp = nullptr;  // N*nv scratch for layout conversion
+  double* d_diag = nullptr;
+  int diag_blocks = 0;
+  double* dts_buf = nullptr;  // device copy of the per-step dt of a run_steps call
+  int dts_cap = 0;
+  // outputs (imex_output.hip)
+  unsigned char* hpos_old = nullptr;  // [N] runout mask stored by the previous call
+  double* bcent = nullptr;            // [N] B_cent, once the caller set it
+  void* d_red = nullptr;              // per-block partials of the runout reductions
+  int red_cap = 0;
+  std::string err;
+};
+
+static inline int set_err(imex_handle* h, const char* what, hipError_t e) {
+  if (h) {
+    h->err = std::string(what) + ": " + hipGetErrorString(e);
+  }
+  return -1;
+}
+#define HCHK(h, call)                                   \
+  do {                                                  \
+    hipError_t e_ = (call);                             \
+    if (e_ != hipSuccess) return set_err((h), #call, e_); \
+  } while (0)
+
+template <typename T>
+static inline int dalloc(imex_handle* h, T** p, size_t n) {
+  void* v = nullptr;
+  hipError_t e = hipMalloc(&v, n ? n * sizeof(T) : 16);
+  if (e != hipSuccess) return set_err(h, "hipMalloc", e);
+  HCHK(h, hipMemsetAsync(v, 0, n ? n * sizeof(T) : 16, h->stream));
+  h->allocs.push_back(v);
+  *p = (T*)v;
+  return 0;
+}
+

@@ -134,9 +134,14 @@ __device__ __forceinline__ void commit_step(DevState* st) {
 }
 __global__ void k_commit(Grid G) { commit_step(G.st); }
 
+// driven: a step of the single-run device-side driver, which ends once t
+// reaches t_end: the pass after the last step (and every later one of the
+// call) commits that step and clears the slots, but builds no lists and
+// counts no active cells; the driver's next call starts with a full pass.
 template <typename S>
 __device__ __forceinline__ void mask_body(const Params& P, const Grid& G, int solve_all, int par,
-                                          unsigned int seq, int full) {
+                                          unsigned int seq, int full, int driven = 0) {
+  const bool stop = driven && G.st->stop_next;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     commit_step(G.st);
     // slots no block of this launch touches
@@ -144,8 +149,9 @@ __device__ __forceinline__ void mask_body(const Params& P, const Grid& G, int so
     G.st->n_work = 0u;                      // filled by the k_flux that follows
     G.st->n_cand[(seq + 2u) % 3u] = 0u;     // the list the next pass fills
     // the frame band of every tile; each visited tile adds what it has beyond
-    atomicAdd(&G.st->active, G.static_active);
+    if (!stop) atomicAdd(&G.st->active, G.static_active);
   }
+  if (stop) return;
   if (full) {
     const int nt = G.ntx * G.nty;
     for (int b = blockIdx.x; b < nt; b += gridDim.x)
@@ -159,8 +165,8 @@ __device__ __forceinline__ void mask_body(const Params& P, const Grid& G, int so
 }
 template <typename S>
 __global__ __launch_bounds__(NTHR) void k_mask(Params P, Grid G, int solve_all, int par,
-                                               unsigned int seq, int full) {
-  mask_body<S>(P, G, solve_all, par, seq, full);
+                                               unsigned int seq, int full, int driven) {
+  mask_body<S>(P, G, solve_all, par, seq, full, driven);
 }
 
 // ===========================================================================
@@ -1409,8 +1415,13 @@ __device__ __forceinline__ void update_body(const Params& P, const Grid& G, doub
   double dt = dt_arg;
   if (mode == 1) {
     if constexpr ((PARTS & 1) != 0) {
+      if (G.st->done) {  // t reached t_end at an earlier step of the call
+        if (blockIdx.x == 0 && threadIdx.x == 0 && G.dts) G.dts[step_idx] = 0.0;
+        return;
+      }
       dt = driver_dt(P, G.st);
       if (blockIdx.x == 0 && threadIdx.x == 0) {
+        G.st->stop_next = (G.st->t + dt < G.st->t_end) ? 0u : 1u;
         G.st->dt = dt;
         G.st->dt_cfl = fmin_f(P.max_dt, __longlong_as_double((long long)G.st->dtmin_bits));
         if (G.dts) G.dts[step_idx] = dt;
@@ -1665,50 +1676,7 @@ __global__ void kt_pow(int n, const double* x, const double* y, double* out) {
 // ===========================================================================
 // host side
 // ===========================================================================
-struct imex_handle {
-  imex_config cfg;
-  Params P;
-  Grid G;
-  hipStream_t stream = nullptr;
-  int device = 0;
-  bool fp32 = false;  // the FP32 form: float state, right-hand side and fluxes
-  int n_cu = 256;  // compute units of the device (persistent grid sizes)
-  int ntiles = 0;
-  int par = 0;  // slot of the active-tile list filled by the latest k_mask
-  unsigned int mask_seq = 0;  // k_mask launches so far (candidate-list slots)
-  bool mask_full_next = true;  // the next k_mask visits every tile
-  size_t lds = 0;
-  std::vector<void*> allocs;
-  double* d_tmp = nullptr;  // N*nv scratch for layout conversion
-  double* d_diag = nullptr;
-  int diag_blocks = 0;
-  double* dts_buf = nullptr;  // device copy of the per-step dt of a run_steps call
-  int dts_cap = 0;
-  std::string err;
-};
-
-static int set_err(imex_handle* h, const char* what, hipError_t e) {
-  if (h) {
-    h->err = std::string(what) + ": " + hipGetErrorString(e);
-  }
-  return -1;
-}
-#define HCHK(h, call)                                   \
-  do {                                                  \
-    hipError_t e_ = (call);                             \
-    if (e_ != hipSuccess) return set_err((h), #call, e_); \
-  } while (0)
-
-template <typename T>
-static int dalloc(imex_handle* h, T** p, size_t n) {
-  void* v = nullptr;
-  hipError_t e = hipMalloc(&v, n ? n * sizeof(T) : 16);
-  if (e != hipSuccess) return set_err(h, "hipMalloc", e);
-  HCHK(h, hipMemsetAsync(v, 0, n ? n * sizeof(T) : 16, h->stream));
-  h->allocs.push_back(v);
-  *p = (T*)v;
-  return 0;
-}
+#include "imex_handle.h"
 
 // Points the kernels' per-step dt output at the handle's device buffer (grown
 // to nsteps when needed) when the caller wants the dt sequence, else at NULL.
@@ -2017,14 +1985,14 @@ static int commit(imex_handle* h) {
   HCHK(h, hipGetLastError());
   return 0;
 }
-static int mask(imex_handle* h, int solve_all) {
+static int mask(imex_handle* h, int solve_all, int driven = 0) {
   h->par ^= 1;
   h->mask_seq++;
   const int full = (solve_all || h->mask_full_next) ? 1 : 0;
   h->mask_full_next = solve_all != 0;  // the pass after solve_all redoes every tile
   const int nb = full ? h->ntiles : grid_for(h, 4);
   hipLaunchKernelGGL((h->fp32 ? k_mask<float> : k_mask<double>), dim3(nb), dim3(NTHR), 0, h->stream, h->P, h->G, solve_all, h->par,
-                     h->mask_seq, full);
+                     h->mask_seq, full, driven);
   HCHK(h, hipGetLastError());
   return 0;
 }
@@ -2080,6 +2048,7 @@ extern "C" int imex_upload_state(imex_handle* h, const double* q_aos, double t, 
   s.err = 0;
   s.pending = 0;
   s.done = 0;
+  s.stop_next = 0;
   s.steps = 0;
   HCHK(h, hipMemcpyAsync(h->G.st, &s, sizeof(s), hipMemcpyHostToDevice, h->stream));
   int rc;
@@ -2177,19 +2146,30 @@ extern "C" int imex_post_step(imex_handle* h, double t_new, imex_step_stats* out
   return (int)(err & ~IMEX_WARN_RESCALE);
 }
 
+// Start of a device-driven call: the call's t_end, and the end condition of
+// the driver loop (t < t_end, src/IMEX_SfloW2D.f90:395) for its first step.
+// The passes skipped after the previous call reached its t_end built no
+// lists, so the next pass visits every tile.
+static int begin_driven(imex_handle* h, double t_end, DevState* s) {
+  if (read_state(h, s)) return -1;
+  if (s->stop_next) h->mask_full_next = true;
+  s->t_end = t_end;
+  s->done = s->stop_next = (s->t < t_end) ? 0u : 1u;
+  HCHK(h, hipMemcpyAsync(h->G.st, s, sizeof(*s), hipMemcpyHostToDevice, h->stream));
+  return 0;
+}
+
 extern "C" int imex_run_steps(imex_handle* h, int nsteps, double t_end, double* dts) {
   if (!h || nsteps < 0) return -2;
   HCHK(h, hipSetDevice(h->device));
   if (int rc = bind_dts(h, dts, nsteps)) return rc;
   {
     DevState s;
-    if (read_state(h, &s)) return -1;
-    s.t_end = t_end;
-    HCHK(h, hipMemcpyAsync(h->G.st, &s, sizeof(s), hipMemcpyHostToDevice, h->stream));
+    if (int rc = begin_driven(h, t_end, &s)) return rc;
   }
   int rc;
   for (int n = 0; n < nsteps; n++) {
-    if ((rc = mask(h, 0))) return rc;
+    if ((rc = mask(h, 0, 1))) return rc;
     if ((rc = flux(h, 1, 0.0, n))) return rc;
     if ((rc = update(h, 1 | 2 | 4, 0.0, 1, n))) return rc;
   }
@@ -2208,10 +2188,8 @@ extern "C" int imex_run_steps_timed(imex_handle* h, int nsteps, double t_end, do
   HCHK(h, hipSetDevice(h->device));
   if (int rc = bind_dts(h, dts, nsteps)) return rc;
   DevState s;
-  if (read_state(h, &s)) return -1;
-  s.t_end = t_end;
+  if (int r = begin_driven(h, t_end, &s)) return r;
   const unsigned long long act0 = s.active;
-  HCHK(h, hipMemcpyAsync(h->G.st, &s, sizeof(s), hipMemcpyHostToDevice, h->stream));
   constexpr int NK = 3;  // k_mask, k_flux, k_update
   const int ns = (nsteps + sample_every - 1) / sample_every;  // steps n with n % sample_every == 0
   std::vector<hipEvent_t> ev((size_t)ns * (NK + 1), nullptr);
@@ -2229,7 +2207,7 @@ extern "C" int imex_run_steps_timed(imex_handle* h, int nsteps, double t_end, do
   for (int n = 0; n < nsteps && !rc; n++) {
     hipEvent_t* e = (n % sample_every == 0) ? &ev[(size_t)(NK + 1) * (n / sample_every)] : nullptr;
     rec(e, 0);
-    if (!rc) rc = mask(h, 0);
+    if (!rc) rc = mask(h, 0, 1);
     rec(e, 1);
     if (!rc) rc = flux(h, 1, 0.0, n);
     rec(e, 2);

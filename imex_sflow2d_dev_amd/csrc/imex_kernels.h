@@ -60,6 +60,8 @@ struct DevState {
   unsigned int done;         // batch member: t reached t_end, its kernels do nothing
   unsigned int full_next;    // batch member: the next active-set pass visits every tile
   unsigned int steps;        // device-driven steps committed since the upload
+  unsigned int stop_next;    // single run, device-driven: the step taken last reaches t_end,
+                             // so the following active-set passes do no tile work
   unsigned long long dried;
   unsigned long long active;
 };

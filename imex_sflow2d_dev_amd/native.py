@@ -40,7 +40,10 @@ EXPORTS = [
     "imex_test_implicit_cplx", "imex_test_implicit_real", "imex_test_semi_impl", "imex_test_expl",
     "imex_test_settling", "imex_test_mass_exchange", "imex_test_newton", "imex_test_arith",
     "imex_test_pow", "imex_host_pow",
+    "imex_set_topography", "imex_runout_reduce", "imex_probes",
 ]
+
+RUNOUT_STORE, RUNOUT_B = 1, 2
 
 
 class ImexConfig(ctypes.Structure):
@@ -75,6 +78,20 @@ class ImexStepStats(ctypes.Structure):
                 ("volume", ctypes.c_double), ("area", ctypes.c_double),
                 ("solid_mass", ctypes.c_double), ("active_cells", ctypes.c_int64),
                 ("dried_cells", ctypes.c_int64), ("err", ctypes.c_uint32)]
+
+
+class ImexRunoutRed(ctypes.Structure):
+    _fields_ = [("max_h", ctypes.c_double), ("sum_q1", ctypes.c_double),
+                ("sum_xq1", ctypes.c_double), ("sum_yq1", ctypes.c_double),
+                ("n_hpos", ctypes.c_int64), ("n_hpos_old", ctypes.c_int64),
+                ("n_new", ctypes.c_int64), ("imax", ctypes.c_int64), ("imax_x", ctypes.c_int64),
+                ("imax_y", ctypes.c_int64), ("dist_max", ctypes.c_double),
+                ("dist_x_max", ctypes.c_double), ("dist_y_max", ctypes.c_double),
+                ("dist_at_imax_x", ctypes.c_double), ("dist_at_imax_y", ctypes.c_double),
+                ("ib", ctypes.c_int64), ("b_max", ctypes.c_double)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
 
 
 _LIB = None

@@ -235,6 +235,35 @@ class Solver:
                      "imex_get_counters")
         return a.value, n.value
 
+    # ---- device-side outputs (the grid work of output_runout / output_probes)
+    def set_topography(self, B: np.ndarray):
+        """B_cent as [k, j]; output_runout reads it when the runout origin is
+        not set (src/inpout_2d.f90:7293)."""
+        b = np.ascontiguousarray(B, dtype=np.float64)
+        assert b.shape == (self.cfg.ny, self.cfg.nx)
+        self.H.check(self.L.imex_set_topography(self.h, native._dp(b)), "imex_set_topography")
+
+    def runout_reduce(self, x0: float, y0: float, alphas_threshold: float = 0.0,
+                      store: bool = False, with_b: bool = False) -> dict:
+        """The whole-array expressions of one output_runout call on the device
+        state (imex_runout_reduce); store=True then sets hpos_old := hpos."""
+        r = native.ImexRunoutRed()
+        flags = (native.RUNOUT_STORE if store else 0) | (native.RUNOUT_B if with_b else 0)
+        self.H.check(self.L.imex_runout_reduce(
+            self.h, ctypes.c_double(x0), ctypes.c_double(y0), ctypes.c_double(alphas_threshold),
+            ctypes.c_uint(flags), ctypes.byref(r)), "imex_runout_reduce")
+        return r.as_dict()
+
+    def probes(self, xy) -> np.ndarray:
+        """output_probes' values per probe ([n, 6 + n_solid (+ n_solid with
+        settling)]) at the points xy ([n, 2])."""
+        xy = np.ascontiguousarray(xy, dtype=np.float64).reshape(-1, 2)
+        c = self.cfg
+        out = np.zeros((len(xy), 6 + c.n_solid + (c.n_solid if c.settling_flag else 0)))
+        self.H.check(self.L.imex_probes(self.h, ctypes.c_int(len(xy)), native._dp(xy),
+                                        native._dp(out)), "imex_probes")
+        return out
+
     def stats(self):
         st = native.ImexStepStats()
         self.H.check(self.L.imex_post_step(self.h, ctypes.c_double(self.t), ctypes.byref(st)),

@@ -113,10 +113,13 @@ int imex_rk_step(imex_handle *h, double dt);
 int imex_erosion_deposition(imex_handle *h, double dt);
 /* driver epilogue (src/IMEX_SfloW2D.f90:449-505) and the stdout diagnostics. */
 int imex_post_step(imex_handle *h, double t_new, imex_step_stats *out);
-/* nsteps full driver steps on the device, no host synchronisation inside:
- * check_solve, timestep, the driver's dt clip to t_end and growth limit
- * (src/IMEX_SfloW2D.f90:417-439), imex_RK_solver, sedimentation, t += dt,
- * epilogue.  dts (may be NULL) receives each step's dt at the end. */
+/* Up to nsteps full driver steps on the device, no host synchronisation
+ * inside: check_solve, timestep, the driver's dt clip to t_end and growth
+ * limit (src/IMEX_SfloW2D.f90:417-439), imex_RK_solver, sedimentation,
+ * t += dt, epilogue.  The steps end once t reaches t_end (the driver loop's
+ * condition t < t_end, :395): the remaining steps of the call do nothing,
+ * so a call with t_end at the next output time stops exactly there.  dts
+ * (may be NULL) receives each step's dt at the end (0 for a step not taken). */
 int imex_run_steps(imex_handle *h, int nsteps, double t_end, double *dts);
 /* imex_run_steps with HIP events around the kernel launches of every
  * sample_every-th step (steps n with n % sample_every == 0; an event record
@@ -152,6 +155,54 @@ int imex_download(imex_handle *h, unsigned what, double *q_aos, double *hmax,
                   int32_t *mask);
 int imex_last_error(imex_handle *h, char *buf, size_t len);
 void imex_destroy(imex_handle *h);
+
+/* ---- device-side outputs ---------------------------------------------------
+ * The grid work of the driver's periodic outputs, on the device state; the
+ * caller formats and writes the files.  Primitive variables are those of the
+ * driver's module array qp: qc_to_qp(q) where q(1) > 0, else zero with
+ * T = T_ambient (src/IMEX_SfloW2D.f90:318-360, :456-503).  Cell indices are
+ * 0-based, linear j + nx*k (Fortran array order). */
+
+/* B_cent (nx*ny, x fastest), which output_runout reads when the runout
+ * origin is not set (src/inpout_2d.f90:7293) */
+int imex_set_topography(imex_handle *h, const double *B_cent);
+
+/* The whole-array expressions of one output_runout call
+ * (src/inpout_2d.f90:7170-7427) over hpos = (qp(1) > 1e-5) .AND.
+ * (SUM(qp(5:4+n_solid)) >= alphas_threshold): counts, sums, MAXVAL and
+ * MAXLOC (ties to the first index, as MAXLOC).  dist, dist_x, dist_y are
+ * 0 outside hpos.  hpos_old is a device mask that the call with
+ * IMEX_RUNOUT_STORE overwrites with hpos after counting; it starts all
+ * false (the reference leaves it uninitialised until its first call sets
+ * it).  Sums are double-double accumulations rounded once (the reference
+ * sums in array order). */
+typedef struct imex_runout_red {
+  double max_h;                     /* MAXVAL(qp(1,:,:)) */
+  double sum_q1, sum_xq1, sum_yq1;  /* SUM(q(1,:,:)), SUM(X*q(1,:,:)), SUM(Y*q(1,:,:)) */
+  int64_t n_hpos;                   /* COUNT(hpos) */
+  int64_t n_hpos_old;               /* COUNT(hpos_old) */
+  int64_t n_new;                    /* COUNT(hpos .AND. .NOT. hpos_old) */
+  int64_t imax, imax_x, imax_y;     /* MAXLOC(dist), MAXLOC(dist_x), MAXLOC(dist_y) */
+  double dist_max, dist_x_max, dist_y_max; /* the three maxima */
+  double dist_at_imax_x, dist_at_imax_y;   /* dist(imax_x), dist(imax_y) (:7315, :7323) */
+  int64_t ib;                       /* with IMEX_RUNOUT_B: MAXLOC of B_cent where
+                                       qp(1) > 1e-5, else 0 (:7293-7294); -1 otherwise */
+  double b_max;
+} imex_runout_red;
+#define IMEX_RUNOUT_STORE 1u  /* hpos_old := hpos after counting */
+#define IMEX_RUNOUT_B 2u      /* also the MAXLOC of the wet cells' B_cent */
+int imex_runout_reduce(imex_handle *h, double x0_runout, double y0_runout,
+                       double alphas_threshold, unsigned flags, imex_runout_red *out);
+
+/* output_probes (src/inpout_2d.f90:6926-7155) for n probes at xy[2*p],
+ * xy[2*p+1]: per probe the values the reference writes after the time
+ * column, in its order -- h, rho_m, T, u, v, alpha_s(1..n_solid) (h*alpha
+ * over h), deposit(1..n_solid) when settling is on, p_dyn -- from the
+ * bilinear interp_2d_scalarB (src/geometry_2d.f90:1535-1599); out holds
+ * n * (6 + n_solid + settling*n_solid) values.  A point left of the first
+ * cell centre reads 0 (the reference reads x_comp(0) there, outside the
+ * array). */
+int imex_probes(imex_handle *h, int n, const double *xy, double *out);
 
 /* ---- batches of independent runs -----------------------------------------
  * No single counterpart in the reference: the reference runs an ensemble as

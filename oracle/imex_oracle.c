@@ -1822,6 +1822,11 @@ void oracle_set_time(oracle_t *o, double t, double dt_old, double dt_old_old)
     o->dt_old_old = dt_old_old;
 }
 
+/* The end time of the driver's dt clip (an output time, src/IMEX_SfloW2D.f90:417-433). */
+void oracle_set_t_end(oracle_t *o, double t_end) { o->t_end = t_end; }
+
+double oracle_get_t(const oracle_t *o) { return o->t; }
+
 void oracle_get_counters(const oracle_t *o, long *dried, long *stale)
 {
     *dried = o->dried;

@@ -45,6 +45,8 @@ void oracle_set_pow_mode(oracle_t *o, int mode);
 double oracle_pow_cr(double x, double y);
 /* driver time and dt history (restart from a saved state) */
 void oracle_set_time(oracle_t *o, double t, double dt_old, double dt_old_old);
+void oracle_set_t_end(oracle_t *o, double t_end);
+double oracle_get_t(const oracle_t *o);
 /* number of cells that went from wet (q1 > 0) to exactly dry in a step, and
  * reads of a face or stage state left from an earlier step that differ from
  * the state a fresh evaluation gives (the reference's stale-state quirk) */

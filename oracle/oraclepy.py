@@ -52,6 +52,10 @@ def _lib():
         lib.oracle_pow_cr.restype = ctypes.c_double
         lib.oracle_pow_cr.argtypes = [ctypes.c_double, ctypes.c_double]
         lib.oracle_set_pow_mode.restype = None
+        lib.oracle_set_t_end.restype = None
+        lib.oracle_set_t_end.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        lib.oracle_get_t.restype = ctypes.c_double
+        lib.oracle_get_t.argtypes = [ctypes.c_void_p]
         _LIB = lib
     return _LIB
 
@@ -168,6 +172,13 @@ class Oracle:
     def set_time(self, t, dt_old, dt_old_old):
         self.lib.oracle_set_time(self.h, ctypes.c_double(t), ctypes.c_double(dt_old),
                                  ctypes.c_double(dt_old_old))
+
+    def set_t_end(self, t_end):
+        self.lib.oracle_set_t_end(self.h, ctypes.c_double(t_end))
+
+    @property
+    def t(self):
+        return self.lib.oracle_get_t(self.h)
 
     def counters(self):
         d = ctypes.c_long()

@@ -1,0 +1,263 @@
+"""numpy restatement of the grid expressions of the driver's periodic
+outputs: output_runout (src/inpout_2d.f90:7170-7427) and output_probes
+(src/inpout_2d.f90:6926-7155, interp_2d_scalarB src/geometry_2d.f90:
+1535-1599), over whole arrays as the reference writes them.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/ as the checker.  Product code
+never imports it.
+
+Arrays are [k, j] (C order of the Fortran (j, k) arrays), so a flat index is
+j + nx*k, Fortran's array order; np.argmax returns the first maximum, as
+MAXLOC.  SUM is restated in the reference's array order (a sequential
+running sum, np.cumsum) and, separately, correctly rounded (math.fsum).
+
+Pinned by tests/test_outputs_oracle.py: module_qp against the reference
+solver's own qp array after driver steps; interp_2d_scalarB against the
+reference's compiled routine; the runout reductions against closed forms
+(a disc of known radius, symmetric mass centres) and brute-force MAXLOC
+ties on small grids.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+
+def module_qp(cfg, q, qc_to_qp):
+    """The driver's qp array from q ([k, j, var]): qc_to_qp where q(1) > 0,
+    else zero with T = T_ambient (src/IMEX_SfloW2D.f90:318-360, :456-503).
+    qc_to_qp maps an [n, n_vars] array to ([n, n_vars + 2], p_dyn)."""
+    ny, nx, nv = q.shape
+    qp = np.zeros((ny, nx, nv + 2))
+    qp[:, :, 3] = cfg.T_ambient
+    wet = q[:, :, 0] > 0.0
+    if wet.any():
+        out = qc_to_qp(q[wet])
+        qp[wet] = out[0] if isinstance(out, tuple) else out
+    return qp
+
+
+def naive_sum(a) -> float:
+    """Fortran SUM in array order (a running sum)."""
+    a = np.asarray(a, dtype=np.float64).reshape(-1)
+    return float(np.cumsum(a)[-1]) if a.size else 0.0
+
+
+def runout_red(cfg, q, qp, hpos_old, x0, y0, thr, B=None):
+    """The whole-array expressions of one output_runout call.  Returns a dict
+    with the fields of imex_runout_red (0-based flat indices), the sums both
+    in array order ('*_naive') and correctly rounded, and hpos."""
+    ny, nx = q.shape[:2]
+    x_comp = 0.5 * ((cfg.x0 + np.arange(nx) * cfg.dx) + (cfg.x0 + np.arange(1, nx + 1) * cfg.dx))
+    y_comp = 0.5 * ((cfg.y0 + np.arange(ny) * cfg.dx) + (cfg.y0 + np.arange(1, ny + 1) * cfg.dx))
+    X = np.broadcast_to(x_comp[None, :], (ny, nx))
+    Y = np.broadcast_to(y_comp[:, None], (ny, nx))
+    ns = cfg.n_solid
+    alphas_tot = np.zeros((ny, nx))
+    for i in range(ns):                          # SUM(qp(5:4+n_solid,:,:), DIM=1)
+        alphas_tot = alphas_tot + qp[:, :, 4 + i]
+    hpos = (qp[:, :, 0] > 1.0e-5) & (alphas_tot >= thr)
+    q1 = q[:, :, 0]
+    ax, ay = X - x0, Y - y0                      # (X - x0_runout)**2 is a product
+    dist = np.where(hpos, np.sqrt(ax * ax + ay * ay), 0.0)
+    dist_x = np.where(hpos, np.sqrt(ax * ax), 0.0)
+    dist_y = np.where(hpos, np.sqrt(ay * ay), 0.0)
+    imax = int(np.argmax(dist.reshape(-1)))
+    imax_x = int(np.argmax(dist_x.reshape(-1)))
+    imax_y = int(np.argmax(dist_y.reshape(-1)))
+    old = np.asarray(hpos_old, dtype=bool)
+    r = dict(
+        max_h=float(qp[:, :, 0].max()),
+        sum_q1=math.fsum(q1.reshape(-1)),
+        sum_xq1=math.fsum((X * q1).reshape(-1)),
+        sum_yq1=math.fsum((Y * q1).reshape(-1)),
+        sum_q1_naive=naive_sum(q1), sum_xq1_naive=naive_sum(X * q1),
+        sum_yq1_naive=naive_sum(Y * q1),
+        n_hpos=int(hpos.sum()), n_hpos_old=int(old.sum()), n_new=int((hpos & ~old).sum()),
+        imax=imax, imax_x=imax_x, imax_y=imax_y,
+        dist_max=float(dist.reshape(-1)[imax]), dist_x_max=float(dist_x.reshape(-1)[imax_x]),
+        dist_y_max=float(dist_y.reshape(-1)[imax_y]),
+        dist_at_imax_x=float(dist.reshape(-1)[imax_x]),
+        dist_at_imax_y=float(dist.reshape(-1)[imax_y]),
+        X=X, Y=Y, hpos=hpos)
+    if B is not None:
+        d = np.where(qp[:, :, 0] > 1.0e-5, B, 0.0)
+        r["ib"] = int(np.argmax(d.reshape(-1)))
+        r["b_max"] = float(d.reshape(-1)[r["ib"]])
+    return r
+
+
+def interp_2d_scalarB(x1, y1, f1, x2, y2):
+    """interp_2d_scalarB (src/geometry_2d.f90:1535-1599) with f1 as [k, j]
+    and 1-based indices as in the reference.  A point left of the first cell
+    centre (ix clamped to 0) returns 0: the reference reads x1(0) there,
+    outside the array."""
+    nx, ny = len(x1), len(y1)
+    if nx > 1:
+        ix = math.floor((x2 - x1[0]) / (x1[1] - x1[0])) + 1
+        ix = max(0, min(ix, nx - 1))
+        if ix == 0:
+            return 0.0
+        ax = (x1[ix] - x2) / (x1[ix] - x1[ix - 1])
+    else:
+        ix, ax = 1, 0.0
+    if ny > 1:
+        iy = math.floor((y2 - y1[0]) / (y1[1] - y1[0])) + 1
+        iy = max(1, min(iy, ny - 1))
+        ay = (y1[iy] - y2) / (y1[iy] - y1[iy - 1])
+    else:
+        iy, ay = 1, 0.0
+    if ax < 0.0 or ax > 1.0 or ay < 0.0 or ay > 1.0:
+        return 0.0
+
+    def f(j, k):  # 1-based f1(j, k)
+        return float(f1[k - 1, j - 1])
+
+    if nx == 1:
+        return ay * f(ix, iy) + (1.0 - ay) * f(ix, iy + 1)
+    if ny == 1:
+        return ax * f(ix, iy) + (1.0 - ax) * f(ix + 1, iy)
+    return (ax * (ay * f(ix, iy) + (1.0 - ay) * f(ix, iy + 1))
+            + (1.0 - ax) * (ay * f(ix + 1, iy) + (1.0 - ay) * f(ix + 1, iy + 1)))
+
+
+def probes(cfg, q, qp, deposit, xy):
+    """The values output_probes writes after the time column for each probe
+    (src/inpout_2d.f90:7040-7147): h, rho_m, T, u, v, alpha_s per class,
+    deposit per class when settling is on, p_dyn.  deposit is [class, k, j]."""
+    ny, nx, nv = q.shape
+    ns = cfg.n_solid
+    x_comp = 0.5 * ((cfg.x0 + np.arange(nx) * cfg.dx) + (cfg.x0 + np.arange(1, nx + 1) * cfg.dx))
+    y_comp = 0.5 * ((cfg.y0 + np.arange(ny) * cfg.dx) + (cfg.y0 + np.arange(1, ny + 1) * cfg.dx))
+    rows = []
+    for x2, y2 in np.asarray(xy, dtype=np.float64).reshape(-1, 2):
+        it = lambda fld: interp_2d_scalarB(x_comp, y_comp, fld, x2, y2)  # noqa: E731
+        h = it(qp[:, :, 0])
+        row = [h]
+        if h > 1.0e-5:
+            rhom = it(q[:, :, 0]) / h
+            T = it(qp[:, :, 3])
+            u = it(qp[:, :, nv])
+            v = it(qp[:, :, nv + 1])
+            row += [rhom, T, u, v] + [it(qp[:, :, 4 + i]) / h for i in range(ns)]
+            pdyn = 0.5 * rhom * (u * u + v * v)
+        else:
+            row += [0.0, cfg.T_ambient, 0.0, 0.0] + [0.0] * ns
+            pdyn = 0.0
+        if cfg.settling_flag:
+            row += [it(deposit[i]) for i in range(ns)]
+        row.append(pdyn)
+        rows.append(row)
+    return np.array(rows, dtype=np.float64)
+
+
+class RunoutRef:
+    """output_runout (src/inpout_2d.f90:7170-7427) restated over whole arrays
+    with its state between calls, writing the reference's three files; the
+    state q of each call is given (q as [k, j, var]).  Lines are written with
+    the edit descriptors of imex_sflow2d_dev_amd.fortran_format, which
+    tests/test_fortran_format.py pins to the Fortran runtime."""
+
+    def __init__(self, cfg, qc_to_qp, directory, run_name="run", dt_runout=60.0, eps_stop=0.0,
+                 alphas_threshold=0.0, x0_runout=-1.0, y0_runout=-1.0, B=None):
+        import os
+        self.cfg, self.qc_to_qp = cfg, qc_to_qp
+        if x0_runout == -1.0 and y0_runout == -1.0:          # :4817-4838
+            if cfg.collapsing_volume_flag:
+                x0_runout, y0_runout = cfg.x_collapse, cfg.y_collapse
+            if cfg.radial_source_flag:
+                x0_runout, y0_runout = cfg.x_source, cfg.y_source
+        self.x0, self.y0, self.B = x0_runout, y0_runout, B
+        self.dt_runout, self.eps_stop, self.thr = dt_runout, eps_stop, alphas_threshold
+        self.f_runout = os.path.join(directory, f"{run_name}_runout.csv")
+        self.f_mc = os.path.join(directory, f"{run_name}_mass_center.csv")
+        self.f_txt = os.path.join(directory, "runout.txt")
+        open(self.f_runout, "w").close()
+        open(self.f_mc, "w").close()
+        self.hpos = np.zeros((cfg.ny, cfg.nx), bool)
+        self.hpos_old = np.zeros((cfg.ny, cfg.nx), bool)
+        self.init_runout = self.init_runout_x = self.init_runout_y = 0.0
+
+    def __call__(self, time, q, stop_flag):
+        from imex_sflow2d_dev_amd.fortran_format import a_fmt, f_fmt
+        c = self.cfg
+        qp = module_qp(c, q, self.qc_to_qp)
+        r = runout_red(c, q, qp, self.hpos_old, self.x0, self.y0, self.thr, B=self.B)
+        X, Y, mask = r["X"], r["Y"], r["hpos"]
+        q1 = q[:, :, 0]
+        with np.errstate(divide="ignore", invalid="ignore"):
+            xmc_now = np.float64(naive_sum(X * q1)) / np.float64(naive_sum(q1))
+            ymc_now = np.float64(naive_sum(Y * q1)) / np.float64(naive_sum(q1))
+        if time == c.t_start:
+            with open(self.f_runout, "a") as f:
+                f.write(a_fmt("Time (s),", 13) + a_fmt("Runout (m),", 13) + a_fmt(" Area (m2),", 13)
+                        + a_fmt("X_runout (m),", 17) + a_fmt("Y_runout (m)", 17) + "\n")
+            with open(self.f_mc, "a") as f:
+                f.write("".join(a_fmt(t, 18) for t in (
+                    "Time (s),", "Barycenter-x (m),", "Barycenter-y (m),", "Baricenter-dx (m),",
+                    "Baricenter-dy (m)")) + "\n")
+            if qp[:, :, 0].max() == 0.0:
+                xmc = ymc = 0.0
+                if c.collapsing_volume_flag:
+                    xmc, ymc = c.x_collapse, c.y_collapse
+                if c.radial_source_flag:
+                    xmc, ymc = c.x_source, c.y_source
+            else:
+                xmc, ymc = float(xmc_now), float(ymc_now)
+                self.hpos = mask
+            self.hpos_old = mask
+            self.xmc_old, self.ymc_old = xmc, ymc
+            if self.x0 == -1.0 and self.y0 == -1.0:
+                d = np.where(qp[:, :, 0] > 1.0e-5, self.B, 0.0).reshape(-1)
+                i = int(np.argmax(d))
+                self.x0, self.y0 = float(X.reshape(-1)[i]), float(Y.reshape(-1)[i])
+                ax, ay = X - self.x0, Y - self.y0
+                dist = np.where(self.hpos, np.sqrt(ax * ax + ay * ay), 0.0).reshape(-1)
+                dist_x = np.where(self.hpos, np.sqrt(ax * ax), 0.0).reshape(-1)
+                dist_y = np.where(self.hpos, np.sqrt(ay * ay), 0.0).reshape(-1)
+                self.init_runout = float(dist[np.argmax(dist)])
+                self.init_runout_x = float(dist[np.argmax(dist_x)])
+                self.init_runout_y = float(dist[np.argmax(dist_y)])
+        else:
+            if qp[:, :, 0].max() == 0.0:
+                xmc, ymc = self.xmc_old, self.ymc_old
+            else:
+                xmc, ymc = float(xmc_now), float(ymc_now)
+            self.hpos = mask
+        ax, ay = X - self.x0, Y - self.y0
+        dist = np.where(self.hpos, np.sqrt(ax * ax + ay * ay), 0.0).reshape(-1)
+        dist_x = np.where(self.hpos, np.sqrt(ax * ax), 0.0).reshape(-1)
+        dist_y = np.where(self.hpos, np.sqrt(ay * ay), 0.0).reshape(-1)
+        imax, imax_x, imax_y = int(np.argmax(dist)), int(np.argmax(dist_x)), int(np.argmax(dist_y))
+        if time > c.t_start:
+            with open(self.f_txt) as f:
+                old_runout = float(f.readline()[18:30])
+        else:
+            old_runout = 0.0
+        now = dist[imax] - self.init_runout
+        with open(self.f_txt, "w") as f:
+            f.write(a_fmt("maximum runout =", 18) + f_fmt(old_runout if old_runout > now else now, 12, 3)
+                    + "\n" + a_fmt("final runout =", 18) + f_fmt(now, 12, 3) + "\n")
+        area_old = c.dx * c.dx * int(self.hpos_old.sum())
+        area = c.dx * c.dx * int(self.hpos.sum())
+        with open(self.f_runout, "a") as f:
+            f.write(f_fmt(time, 12, 3) + "," + f_fmt(now, 12, 3) + "," + f_fmt(area, 14, 3) + ","
+                    + f_fmt(float(X.reshape(-1)[imax]), 14, 3) + ","
+                    + f_fmt(float(Y.reshape(-1)[imax]), 14, 3) + "\n")
+        with open(self.f_mc, "a") as f:
+            f.write(f_fmt(time, 17, 3) + "," + f_fmt(xmc, 17, 3) + "," + f_fmt(ymc, 17, 3) + ","
+                    + f_fmt(float(dist_x[imax_x]) - self.init_runout_x, 17, 3) + ","
+                    + f_fmt(float(dist_y[imax_y]) - self.init_runout_y, 17, 3) + "\n")
+        if time > c.t_start:
+            vel = math.sqrt((self.xmc_old - xmc) ** 2 + (self.ymc_old - ymc) ** 2) / self.dt_runout
+            with np.errstate(divide="ignore", invalid="ignore"):
+                new_rel = float(np.float64(c.dx * c.dx * int((self.hpos & ~self.hpos_old).sum()))
+                                / np.float64(int(self.hpos.sum())))
+            self.xmc_old, self.ymc_old = xmc, ymc
+            self.hpos_old = self.hpos
+            a, b = vel, new_rel / self.dt_runout
+            if (a if a > b else b) < self.eps_stop and not stop_flag:
+                stop_flag = True
+        del area_old
+        return stop_flag

@@ -504,6 +504,26 @@ CONTAINS
     qout = q
   END SUBROUTINE imexref_get_q
 
+  ! The driver's module array qp (primitive variables kept by the epilogue).
+  SUBROUTINE imexref_get_qp(qpo) BIND(C, name='imexref_get_qp')
+    REAL(c_double), INTENT(OUT) :: qpo(n_vars+2, comp_cells_x, comp_cells_y)
+    qpo = qp
+  END SUBROUTINE imexref_get_qp
+
+  ! interp_2d_scalarB (src/geometry_2d.f90:1535-1599) of the field f on the
+  ! cell centres x_comp, y_comp at n points xy(1:2, p) (output_probes).
+  SUBROUTINE imexref_interp(n, xy, f, out) BIND(C, name='imexref_interp')
+    USE geometry_2d, ONLY : x_comp, y_comp, interp_2d_scalarB
+    INTEGER(c_int), VALUE :: n
+    REAL(c_double), INTENT(IN) :: xy(2, n)
+    REAL(c_double), INTENT(IN) :: f(comp_cells_x, comp_cells_y)
+    REAL(c_double), INTENT(OUT) :: out(n)
+    INTEGER :: i
+    DO i = 1, n
+       CALL interp_2d_scalarB( x_comp, y_comp, f, xy(1,i), xy(2,i), out(i) )
+    END DO
+  END SUBROUTINE imexref_interp
+
   SUBROUTINE imexref_set_q(qin) BIND(C, name='imexref_set_q')
     REAL(c_double), INTENT(IN) :: qin(n_vars, comp_cells_x, comp_cells_y)
     q = qin

@@ -138,6 +138,21 @@ class RefRun:
         assert a.shape == (self.cfg.ny, self.cfg.nx, self.nv)
         self.lib.imexref_set_q(_dp(a))
 
+    def qp(self):
+        """The driver's module array qp as [k, j, var] (n_vars + 2 variables)."""
+        out = np.zeros((self.cfg.ny, self.cfg.nx, self.nv + 2), np.float64)
+        self.lib.imexref_get_qp(_dp(out))
+        return out
+
+    def interp(self, xy, f):
+        """interp_2d_scalarB of the field f ([k, j]) at the points xy ([n, 2])."""
+        xy = np.ascontiguousarray(xy, dtype=np.float64).reshape(-1, 2)
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        assert f.shape == (self.cfg.ny, self.cfg.nx)
+        out = np.zeros(len(xy), np.float64)
+        self.lib.imexref_interp(ctypes.c_int(len(xy)), _dp(xy), _dp(f), _dp(out))
+        return out
+
     def fields(self):
         c = self.cfg
         hm = np.zeros((c.ny, c.nx))
