@@ -36,8 +36,19 @@ struct imex_handle {
   double* bcent = nullptr;            // [N] B_cent, once the caller set it
   void* d_red = nullptr;              // per-block partials of the runout reductions
   int red_cap = 0;
+  // snapshots of output_solution's records (imex_output.hip): device staging
+  // per record kind, copied to the host on their own stream
+  hipStream_t copy_stream = nullptr;
+  hipEvent_t ev_staged = nullptr, ev_copied = nullptr;
+  bool copy_inflight = false;
+  double* stage[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  size_t stage_cap[5] = {0, 0, 0, 0, 0};
+  unsigned int* d_flag = nullptr;
   std::string err;
 };
+
+// releases what imex_output.hip created (streams, events); called by imex_destroy
+void imex_output_release(imex_handle* h);
 
 static inline int set_err(imex_handle* h, const char* what, hipError_t e) {
   if (h) {

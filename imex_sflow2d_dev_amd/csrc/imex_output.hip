@@ -95,6 +95,8 @@ __device__ void block_reduce(RunoutPart& mine, RunoutPart* out) {
   if (tid == 0) *out = sh[0];
 }
 
+__global__ void k_fill_u32(unsigned int* p, unsigned int v) { *p = v; }
+
 // The primitive thickness and SUM(qp(5:4+n_solid)) of one cell, as the
 // driver's module array qp holds them.
 template <int NS, typename S>
@@ -294,7 +296,254 @@ __global__ void k_probes(Params P, Grid G, int n, const double* xy, double* out)
   o[nout - 1] = pdyn;
 }
 
+// ---------------------------------------------------------------------------
+// output_solution's records (src/inpout_2d.f90:5934-6423), one cell per
+// thread, in the reference's order of operations: the .q_2d pass zeroes
+// |q| < 1e-20 in the state (:6043-6054) and writes x, y, q; the _deposit_
+// file gets the deposit as it is; the .p_2d pass (:6169-6402) computes
+// qc_to_qp and mixt_var of the (zeroed) state, zeroes |deposit| < 1e-20 in
+// place and writes its columns.  Configured path: no added gases, liquid,
+// stochastic or pore-pressure variables, no erosion (erosion 0, erodible 0
+// with porosity 0), Voellmy rheology (model 1): shear velocity, Rouse
+// numbers and inertial number 0.  mu_eff reads pore_pres(1) of a zero-size
+// array in the reference; here the excess pore pressure is 0, so mu_eff =
+// mu * MAX(0, 1 - 0 / (rho_m h g')) on wet cells.
+// ---------------------------------------------------------------------------
+__host__ __device__ constexpr int p2d_cols(int ns) { return 19 + 4 * ns; }
+
+template <int NS, typename S>
+__global__ void k_snapshot(Params P, Grid G, const double* bcent, unsigned what, double* sq,
+                           double* sq2d, double* sdep2d, double* sp2d, double* sfld,
+                           unsigned int* flag) {
+  constexpr int NV = 4 + NS;
+  const int N = G.N;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < N; c += gridDim.x * blockDim.x) {
+    const int j = c % G.nx, k = c / G.nx;
+    const double X = G.xc[j], Y = G.yc[k];
+    double q[NV];
+#pragma unroll
+    for (int i = 0; i < NV; i++) q[i] = (double)state_q<S>(G)[(size_t)i * N + c];
+    if (what & IMEX_SNAP_Q2D) {
+      bool z = false, rest = false;
+#pragma unroll
+      for (int i = 0; i < NV; i++)
+        if (fabs(q[i]) < 1.0e-20) {  // -0.0 becomes +0.0 too; stores only changed values
+          if (q[i] != 0.0 || signbit(q[i])) {
+            state_q<S>(G)[(size_t)i * N + c] = S(0.0);
+            z = true;
+          }
+          q[i] = 0.0;
+        }
+      // a dry cell left with other non-zero components: the flux kernel must
+      // then visit every tile with active cells (as after such an upload)
+      if (z && !(q[0] > 0.0))
+        for (int i = 1; i < NV; i++) rest = rest || q[i] != 0.0;
+      if (rest) atomicOr(flag, 1u);
+      double* r = sq2d + (size_t)c * (2 + NV);
+      r[0] = X;
+      r[1] = Y;
+#pragma unroll
+      for (int i = 0; i < NV; i++) r[2 + i] = q[i];
+    }
+    double dep[NS];
+#pragma unroll
+    for (int i = 0; i < NS; i++) dep[i] = G.dep[(size_t)i * N + c];
+    if (what & IMEX_SNAP_DEP2D) {
+      double* r = sdep2d + (size_t)c * (2 + NS);
+      r[0] = X;
+      r[1] = Y;
+#pragma unroll
+      for (int i = 0; i < NS; i++) r[2 + i] = dep[i];
+    }
+    if (what & IMEX_SNAP_P2D) {
+      double qp[NV + 2], pd;
+      qc_to_qp<NS>(P, q, qp, pd);
+      const MixtT<double> m = mixt_var<NS>(P, qp, false);
+      double h = qp[0], u = qp[NV], v = qp[NV + 1], T = qp[3];
+      double rho_m = m.rho_m, rg = m.red_grav;
+      double al[NS];
+      double mu_eff = 0.0;
+      if (h > 0.0) {
+#pragma unroll
+        for (int i = 0; i < NS; i++) al[i] = qp[4 + i] / h;
+        const double excess = fmax_f(0.0, P.pres - P.pres);
+        mu_eff = P.mu * fmax_f(0.0, (1.0 - excess / (rho_m * h * rg)));
+      } else {
+#pragma unroll
+        for (int i = 0; i < NS; i++) al[i] = 0.0;
+      }
+      auto tiny = [](double& x) {
+        if (fabs(x) < 1.0e-20) x = 0.0;
+      };
+      tiny(h);
+      tiny(u);
+      tiny(v);
+      double B = bcent ? bcent[c] : 0.0;
+      tiny(B);
+#pragma unroll
+      for (int i = 0; i < NS; i++) {
+        tiny(al[i]);
+        if (fabs(dep[i]) < 1.0e-20) {
+          if (dep[i] != 0.0 || signbit(dep[i])) G.dep[(size_t)i * N + c] = 0.0;
+          dep[i] = 0.0;
+        }
+      }
+      tiny(T);
+      tiny(rho_m);
+      tiny(rg);
+      tiny(mu_eff);
+      double* r = sp2d + (size_t)c * p2d_cols(NS);
+      int o = 0;
+      r[o++] = X;
+      r[o++] = Y;
+      r[o++] = h;
+      r[o++] = u;
+      r[o++] = v;
+      r[o++] = B;
+      r[o++] = h + B;
+#pragma unroll
+      for (int i = 0; i < NS; i++) r[o++] = al[i];
+      r[o++] = T;
+      r[o++] = rho_m;
+      r[o++] = rg;
+#pragma unroll
+      for (int i = 0; i < NS; i++) r[o++] = dep[i];
+#pragma unroll
+      for (int i = 0; i < NS; i++) r[o++] = 0.0;  // erosion
+      r[o++] = 0.0 / (1.0 - 0.0);                  // SUM(erodible) / (1 - porosity)
+      r[o++] = 0.0;                                // alpha_l
+      r[o++] = 0.0;                                // shear velocity
+      r[o++] = m.Ri;
+#pragma unroll
+      for (int i = 0; i < NS; i++) r[o++] = 0.0;  // Rouse numbers
+      r[o++] = mu_eff;
+      r[o++] = G.hmax[c];
+      r[o++] = G.pdyn[c];
+      r[o++] = G.mvel[c];
+      r[o++] = 0.0;                                // inertial number
+    }
+    if (what & IMEX_SNAP_Q) {
+#pragma unroll
+      for (int i = 0; i < NV; i++) sq[(size_t)c * NV + i] = q[i];
+    }
+    if (what & IMEX_SNAP_FIELDS) {
+#pragma unroll
+      for (int i = 0; i < NS; i++) sfld[(size_t)i * N + c] = dep[i];
+      sfld[(size_t)NS * N + c] = G.hmax[c];
+      sfld[(size_t)(NS + 1) * N + c] = G.pdyn[c];
+      sfld[(size_t)(NS + 2) * N + c] = G.mvel[c];
+    }
+  }
+}
+
 }  // namespace
+
+// values per cell of each snapshot kind, in the order of the bits
+static int snap_cols(const imex_handle* h, int kind) {
+  const int nv = h->P.nv, ns = h->P.ns;
+  switch (kind) {
+    case 0: return nv;
+    case 1: return 2 + nv;
+    case 2: return 2 + ns;
+    case 3: return p2d_cols(ns);
+    default: return ns + 3;
+  }
+}
+
+extern "C" int imex_snapshot_cols(imex_handle* h, unsigned kind) {
+  if (!h) return -2;
+  for (int b = 0; b < 5; b++)
+    if (kind == (1u << b)) return snap_cols(h, b);
+  return -2;
+}
+
+extern "C" int imex_snapshot_begin(imex_handle* h, unsigned what, double* q, double* q2d,
+                                   double* dep2d, double* p2d, double* fields) {
+  if (!h || (what & ~31u)) return -2;
+  double* outs[5] = {q, q2d, dep2d, p2d, fields};
+  for (int b = 0; b < 5; b++)
+    if ((what & (1u << b)) && !outs[b]) return -2;
+  HCHK(h, hipSetDevice(h->device));
+  if (!h->copy_stream) {
+    HCHK(h, hipStreamCreateWithFlags(&h->copy_stream, hipStreamNonBlocking));
+    HCHK(h, hipEventCreateWithFlags(&h->ev_staged, hipEventDisableTiming));
+    HCHK(h, hipEventCreateWithFlags(&h->ev_copied, hipEventDisableTiming));
+    if (int rc = dalloc(h, &h->d_flag, 1)) return rc;
+  }
+  const size_t N = (size_t)h->G.N;
+  for (int b = 0; b < 5; b++) {
+    if (!(what & (1u << b))) continue;
+    const size_t need = N * snap_cols(h, b);
+    if (h->stage_cap[b] < need) {
+      if (h->copy_inflight) HCHK(h, hipEventSynchronize(h->ev_copied));
+      if (int rc = dalloc(h, &h->stage[b], need)) return rc;
+      h->stage_cap[b] = need;
+    }
+  }
+  // the staging of the previous snapshot may still be on its way to the host
+  if (h->copy_inflight) HCHK(h, hipStreamWaitEvent(h->stream, h->ev_copied, 0));
+  const int nb = std::max(1, std::min(8 * h->n_cu, (int)((N + 255) / 256)));
+  const Params& P = h->P;
+  const Grid& G = h->G;
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(1), 0, h->stream, h->d_flag, 0u);
+#define SNAP_LAUNCH(NS)                                                                       \
+  hipLaunchKernelGGL((h->fp32 ? k_snapshot<NS, float> : k_snapshot<NS, double>), dim3(nb),   \
+                     dim3(256), 0, h->stream, P, G, h->bcent, what, h->stage[0], h->stage[1], \
+                     h->stage[2], h->stage[3], h->stage[4], h->d_flag)
+  switch (P.ns) {
+    case 1: SNAP_LAUNCH(1); break;
+    case 2: SNAP_LAUNCH(2); break;
+    default: SNAP_LAUNCH(3); break;
+  }
+#undef SNAP_LAUNCH
+  HCHK(h, hipGetLastError());
+  if (what & IMEX_SNAP_Q2D) {
+    unsigned int f = 0;
+    HCHK(h, hipMemcpyAsync(&f, h->d_flag, sizeof(f), hipMemcpyDeviceToHost, h->stream));
+    HCHK(h, hipStreamSynchronize(h->stream));
+    if (f) h->P.all_tiles = 1;
+  }
+  HCHK(h, hipEventRecord(h->ev_staged, h->stream));
+  HCHK(h, hipStreamWaitEvent(h->copy_stream, h->ev_staged, 0));
+  for (int b = 0; b < 5; b++)
+    if (what & (1u << b))
+      HCHK(h, hipMemcpyAsync(outs[b], h->stage[b], 8 * N * snap_cols(h, b), hipMemcpyDeviceToHost,
+                             h->copy_stream));
+  HCHK(h, hipEventRecord(h->ev_copied, h->copy_stream));
+  h->copy_inflight = true;
+  return 0;
+}
+
+extern "C" int imex_snapshot_wait(imex_handle* h) {
+  if (!h) return -2;
+  if (!h->copy_inflight) return 0;
+  HCHK(h, hipSetDevice(h->device));
+  HCHK(h, hipEventSynchronize(h->ev_copied));
+  h->copy_inflight = false;
+  return 0;
+}
+
+extern "C" int imex_host_alloc(size_t bytes, void** p) {
+  if (!p) return -2;
+  *p = nullptr;
+  return hipHostMalloc(p, bytes ? bytes : 16, hipHostMallocDefault) == hipSuccess ? 0 : -1;
+}
+
+extern "C" int imex_host_free(void* p) {
+  if (!p) return 0;
+  return hipHostFree(p) == hipSuccess ? 0 : -1;
+}
+
+void imex_output_release(imex_handle* h) {
+  if (h->copy_stream) {
+    (void)hipStreamSynchronize(h->copy_stream);
+    (void)hipStreamDestroy(h->copy_stream);
+    h->copy_stream = nullptr;
+  }
+  if (h->ev_staged) (void)hipEventDestroy(h->ev_staged);
+  if (h->ev_copied) (void)hipEventDestroy(h->ev_copied);
+  h->ev_staged = h->ev_copied = nullptr;
+}
 
 extern "C" int imex_set_topography(imex_handle* h, const double* B_cent) {
   if (!h || !B_cent) return -2;

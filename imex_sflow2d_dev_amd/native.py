@@ -41,9 +41,13 @@ EXPORTS = [
     "imex_test_settling", "imex_test_mass_exchange", "imex_test_newton", "imex_test_arith",
     "imex_test_pow", "imex_host_pow",
     "imex_set_topography", "imex_runout_reduce", "imex_probes",
+    "imex_snapshot_cols", "imex_snapshot_begin", "imex_snapshot_wait", "imex_host_alloc",
+    "imex_host_free", "imex_write_records", "imex_write_restart", "imex_format_value",
 ]
 
 RUNOUT_STORE, RUNOUT_B = 1, 2
+SNAP_Q, SNAP_Q2D, SNAP_DEP2D, SNAP_P2D, SNAP_FIELDS = 1, 2, 4, 8, 16
+FMT_E20_12, FMT_ES15_7, FMT_ES14_7 = 0, 1, 2
 
 
 class ImexConfig(ctypes.Structure):
@@ -112,6 +116,13 @@ def lib():
             f = getattr(L, name)
             f.restype = None if name in ("imex_destroy", "imex_batch_destroy") else ctypes.c_int
         L.imex_create.argtypes = [ctypes.POINTER(ImexConfig), ctypes.POINTER(ctypes.c_void_p)]
+        L.imex_host_alloc.argtypes = [ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)]
+        L.imex_host_free.argtypes = [ctypes.c_void_p]
+        L.imex_write_records.argtypes = [ctypes.c_char_p, _D, ctypes.c_int64, ctypes.c_int,
+                                         ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.imex_write_restart.argtypes = [ctypes.c_char_p] + [ctypes.c_int] * 4 + \
+            [ctypes.c_double] * 2 + [_D] * 8
+        L.imex_format_value.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_char_p]
         _LIB = L
     return _LIB
 
@@ -196,3 +207,55 @@ class Handle:
             self.close()
         except Exception:
             pass
+
+
+class PinnedArray:
+    """A page-locked host buffer (imex_host_alloc) viewed as a float64 array."""
+
+    def __init__(self, n: int):
+        self.L = lib()
+        self.p = ctypes.c_void_p()
+        if self.L.imex_host_alloc(ctypes.c_size_t(8 * max(int(n), 1)), ctypes.byref(self.p)) != 0:
+            raise NativeError("imex_host_alloc failed")
+        self.n = int(n)
+        self.a = np.ctypeslib.as_array(ctypes.cast(self.p, _D), shape=(max(self.n, 1),))[:self.n]
+
+    def free(self):
+        if self.p is not None and self.p.value:
+            self.a = None
+            self.L.imex_host_free(self.p)
+            self.p = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def write_records(path: str, rec: np.ndarray, nx: int, fmt: int, n_threads: int = 0):
+    """The reference's formatted record files (.q_2d: FMT_E20_12, .p_2d:
+    FMT_ES15_7) from records [n_cells, n_col]."""
+    rec = np.ascontiguousarray(rec, dtype=np.float64)
+    rc = lib().imex_write_records(path.encode(), _dp(rec), ctypes.c_int64(rec.shape[0]),
+                                  ctypes.c_int(rec.shape[1]), ctypes.c_int(nx), ctypes.c_int(fmt),
+                                  ctypes.c_int(n_threads))
+    if rc != 0:
+        raise NativeError(f"imex_write_records failed ({rc}) for {path}")
+
+
+def write_restart(path: str, nx, ny, n_vars, n_solid, t, dt, q_aos, B_cent, erodible, deposit,
+                  erosion, hmax, pdynmax, mod_vel_max):
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in
+            (q_aos, B_cent, erodible, deposit, erosion, hmax, pdynmax, mod_vel_max)]
+    rc = lib().imex_write_restart(path.encode(), nx, ny, n_vars, n_solid, t, dt,
+                                  *[_dp(a) for a in arrs])
+    if rc != 0:
+        raise NativeError(f"imex_write_restart failed ({rc}) for {path}")
+
+
+def format_value(fmt: int, x: float) -> str:
+    buf = ctypes.create_string_buffer(32)
+    if lib().imex_format_value(fmt, x, buf) != 0:
+        raise NativeError("imex_format_value failed")
+    return buf.value.decode()

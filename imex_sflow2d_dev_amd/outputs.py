@@ -192,27 +192,139 @@ class ProbesOutput:
         self.t_probes = time + self.dt_probes
 
 
+class SolutionOutput:
+    """output_solution's .q_2d, _deposit_ .q_2d and .p_2d files
+    (src/inpout_2d.f90:5934-6423) and write_restart_file's restart.bin
+    (:8037-8084).  The device assembles the records (Solver.snapshot_begin)
+    and copies them into page-locked buffers on its own stream; a host
+    thread waits for the copy and writes the files while the next steps
+    run.  A new output first waits for the previous one's files."""
+
+    def __init__(self, solver, run_name: str = "run", directory: str = ".", dt_output: float = 60.0,
+                 cons: bool = True, phys: bool = True, restart: bool = True, B=None,
+                 n_threads: int = 0, overlap: bool = True):
+        from . import native
+        self.native = native
+        self.s = solver
+        c = solver.cfg
+        self.cfg = c
+        self.dir, self.run_name = directory, run_name
+        self.dt_output = dt_output
+        self.cons, self.phys, self.restart = cons, phys, restart
+        self.n_threads, self.overlap = n_threads, overlap
+        self.B = None if B is None else np.ascontiguousarray(B, dtype=np.float64)
+        if B is not None:
+            solver.set_topography(self.B)
+        self.output_idx = 0
+        self.t_output = c.t_start + dt_output if dt_output > 0 else c.t_end
+        N = c.nx * c.ny
+        self.what = 0
+        if cons:
+            self.what |= native.SNAP_Q2D | (native.SNAP_DEP2D if c.settling_flag else 0)
+        if phys:
+            self.what |= native.SNAP_P2D
+        if restart:
+            self.what |= native.SNAP_Q | native.SNAP_FIELDS
+        self.bufs = {}
+        for kind in (native.SNAP_Q, native.SNAP_Q2D, native.SNAP_DEP2D, native.SNAP_P2D,
+                     native.SNAP_FIELDS):
+            if self.what & kind:
+                self.bufs[kind] = native.PinnedArray(N * solver.snapshot_cols(kind))
+        self._thread = None
+        self._error = None
+        self.timings = []   # per output: (seconds in the caller, seconds of the writer thread)
+
+    def _files(self, idx: str, time: float, dt: float, with_restart: bool):
+        import time as _t
+        nat, c, s = self.native, self.cfg, self.s
+        t0 = _t.perf_counter()
+        try:
+            s.snapshot_wait()
+            N, nx = c.nx * c.ny, c.nx
+            if self.cons:
+                rec = self.bufs[nat.SNAP_Q2D].a.reshape(N, -1)
+                nat.write_records(os.path.join(self.dir, f"{self.run_name}_{idx}.q_2d"), rec, nx,
+                                  nat.FMT_E20_12, self.n_threads)
+                if c.settling_flag:
+                    rec = self.bufs[nat.SNAP_DEP2D].a.reshape(N, -1)
+                    nat.write_records(os.path.join(self.dir, f"{self.run_name}_deposit_{idx}.q_2d"),
+                                      rec, nx, nat.FMT_E20_12, self.n_threads)
+            if self.phys:
+                rec = self.bufs[nat.SNAP_P2D].a.reshape(N, -1)
+                nat.write_records(os.path.join(self.dir, f"{self.run_name}_{idx}.p_2d"), rec, nx,
+                                  nat.FMT_ES15_7, self.n_threads)
+            if with_restart and self.restart:
+                ns = c.n_solid
+                fld = self.bufs[nat.SNAP_FIELDS].a.reshape(ns + 3, N)
+                zeros = np.zeros(N * ns)
+                B = self.B if self.B is not None else np.zeros(N)
+                nat.write_restart(os.path.join(self.dir, "restart.bin"), c.nx, c.ny, c.n_vars, ns,
+                                  time, dt, self.bufs[nat.SNAP_Q].a, B, zeros, fld[:ns], zeros,
+                                  fld[ns], fld[ns + 1], fld[ns + 2])
+        except Exception as e:  # reported by the next call or by finish()
+            self._error = e
+        self.timings[-1][1] = _t.perf_counter() - t0
+
+    def __call__(self, time: float, restart: bool = True):
+        import threading
+        import time as _t
+        t0 = _t.perf_counter()
+        self.finish()
+        idx = lettera(self.output_idx)
+        self.output_idx += 1
+        what = self.what if restart else self.what & ~(self.native.SNAP_Q | self.native.SNAP_FIELDS)
+        self.s.snapshot_begin(what, {k: b.a for k, b in self.bufs.items()})
+        self.timings.append([0.0, 0.0])
+        args = (idx, time, self.s.dt, restart)
+        if self.overlap:
+            self._thread = threading.Thread(target=self._files, args=args, daemon=True)
+            self._thread.start()
+        else:
+            self._files(*args)
+        self.t_output = time + self.dt_output
+        self.timings[-1][0] = _t.perf_counter() - t0
+
+    def finish(self):
+        if self._thread is not None:
+            self._thread.join()
+            self._thread = None
+        if self._error is not None:
+            e, self._error = self._error, None
+            raise e
+
+    def close(self):
+        self.finish()
+        for b in self.bufs.values():
+            b.free()
+        self.bufs = {}
+
+
 def run_with_outputs(solver, t_end: float, runout: RunoutOutput = None, probes: ProbesOutput = None,
-                     chunk: int = 64, max_steps: int = 10 ** 8):
-    """The driver loop's output schedule (src/IMEX_SfloW2D.f90:370-375,
+                     solution: SolutionOutput = None, chunk: int = 64, max_steps: int = 10 ** 8):
+    """The driver loop's output schedule (src/IMEX_SfloW2D.f90:370-377,
     :395-591) over device-driven steps: the initial outputs at t_start, then
     steps up to the next output time (the dt clip of :417-433 makes the
     step that reaches it end exactly there), the outputs due, and the steady
     stop (t_steady).  Returns (steps taken, stop_flag)."""
-    c = solver.cfg
     stop_flag = False
     t_steady = t_end
     if runout is not None:
         stop_flag = runout(solver.t, stop_flag)
+    if solution is not None:
+        solution(solver.t, restart=False)
     if probes is not None:
         probes(solver.t)
     steps = 0
     while solver.t < t_end and solver.t < t_steady and steps < max_steps:
+        if solution is not None and t_end - solution.t_output < 1.0e-7:
+            solution.t_output = t_end
         if runout is not None and t_end - runout.t_runout < 1.0e-7:
             runout.t_runout = t_end
         if probes is not None and t_end - probes.t_probes < 1.0e-7:
             probes.t_probes = t_end
         target = t_end
+        if solution is not None:
+            target = min(target, solution.t_output)
         if runout is not None:
             target = min(target, runout.t_runout)
         if probes is not None:
@@ -225,7 +337,12 @@ def run_with_outputs(solver, t_end: float, runout: RunoutOutput = None, probes: 
             old = stop_flag
             stop_flag = runout(solver.t, stop_flag)
             if stop_flag and not old:
-                t_steady = t_end
+                t_out = solution.t_output if solution is not None else t_end
+                t_steady = min(t_end, t_out)
         if probes is not None and solver.t >= probes.t_probes:
             probes(solver.t)
+        if solution is not None and (solver.t >= solution.t_output or solver.t >= t_end):
+            solution(solver.t)
+    if solution is not None:
+        solution.finish()
     return steps, stop_flag

@@ -264,6 +264,30 @@ class Solver:
                                         native._dp(out)), "imex_probes")
         return out
 
+    def snapshot_cols(self, kind: int) -> int:
+        return self.H.check(self.L.imex_snapshot_cols(self.h, ctypes.c_uint(kind)),
+                            "imex_snapshot_cols")
+
+    def snapshot_begin(self, what: int, bufs: dict):
+        """Starts a snapshot (imex_snapshot_begin) into the host arrays of
+        `bufs` (keys native.SNAP_*: float64 arrays of N * cols values,
+        page-locked for the copy to overlap the next steps)."""
+        N = self.cfg.nx * self.cfg.ny
+        ptrs = []
+        for kind in (native.SNAP_Q, native.SNAP_Q2D, native.SNAP_DEP2D, native.SNAP_P2D,
+                     native.SNAP_FIELDS):
+            a = bufs.get(kind) if what & kind else None
+            if a is not None:
+                assert a.dtype == np.float64 and a.size >= N * self.snapshot_cols(kind)
+                ptrs.append(native._dp(a))
+            else:
+                ptrs.append(None)
+        self.H.check(self.L.imex_snapshot_begin(self.h, ctypes.c_uint(what), *ptrs),
+                     "imex_snapshot_begin")
+
+    def snapshot_wait(self):
+        self.H.check(self.L.imex_snapshot_wait(self.h), "imex_snapshot_wait")
+
     def stats(self):
         st = native.ImexStepStats()
         self.H.check(self.L.imex_post_step(self.h, ctypes.c_double(self.t), ctypes.byref(st)),

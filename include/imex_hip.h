@@ -204,6 +204,54 @@ int imex_runout_reduce(imex_handle *h, double x0_runout, double y0_runout,
  * array). */
 int imex_probes(imex_handle *h, int n, const double *xy, double *out);
 
+/* Snapshots for output_solution (src/inpout_2d.f90:5934-6423) and
+ * write_restart_file (:8037-8084).  imex_snapshot_begin assembles the
+ * selected records on the device (in stream order after the steps already
+ * issued) and copies them to the caller's host buffers on a second stream;
+ * it returns without waiting, so the next steps run while the copy (and the
+ * caller's writing of the previous snapshot) proceeds.  The buffers must
+ * stay valid, and be page-locked for the copy to overlap (imex_host_alloc),
+ * until imex_snapshot_wait returns.  Like the reference, the .q_2d pass
+ * zeroes |q| < 1e-20 in the device state and the .p_2d pass zeroes
+ * |deposit| < 1e-20.  Records are one row per cell in Fortran order (x
+ * fastest); imex_snapshot_cols gives the values per cell of one kind. */
+#define IMEX_SNAP_Q 1u      /* q(n_vars, nx, ny), after the .q_2d zeroing when selected */
+#define IMEX_SNAP_Q2D 2u    /* x, y, q(1:n_vars) (src/inpout_2d.f90:6043-6058) */
+#define IMEX_SNAP_DEP2D 4u  /* x, y, deposit(j,k,1:n_solid) (:6071-6075) */
+#define IMEX_SNAP_P2D 8u    /* x, y, h, u, v, B, h+B, alpha_s(n_solid), T, rho_m, g', deposit,
+                               erosion, erodible, alpha_l, u*, Ri, Rouse(n_solid), mu_eff,
+                               hmax, pdynmax, mod_vel_max, I (:6390-6396): 19 + 4 n_solid
+                               values; B from imex_set_topography (0 without it) */
+#define IMEX_SNAP_FIELDS 16u /* deposit(nx,ny,n_solid), hmax, pdynmax, mod_vel_max */
+int imex_snapshot_cols(imex_handle *h, unsigned kind);
+int imex_snapshot_begin(imex_handle *h, unsigned what, double *q, double *q2d, double *dep2d,
+                        double *p2d, double *fields);
+int imex_snapshot_wait(imex_handle *h);
+/* page-locked host memory for snapshot buffers */
+int imex_host_alloc(size_t bytes, void **p);
+int imex_host_free(void *p);
+
+/* Host writers (no device work).  imex_write_records writes n_rows records
+ * of n_col values as the reference's formatted output: one line per record
+ * with fmt IMEX_FMT_E20_12 ('(2e20.12,100(e20.12))', the .q_2d files) or
+ * IMEX_FMT_ES15_7 ('(100ES15.7E2)', .p_2d), a list-directed blank record
+ * after every nx records and two at the end; n_threads formatting threads
+ * (0: one per host core).  imex_write_restart writes restart.bin as
+ * write_restart_file's unformatted sequential records (4-byte markers;
+ * records of 2 GiB or more are rejected).  imex_format_value formats one
+ * value (fmt also IMEX_FMT_ES14_7, the probe files' ES14.7E2) into out
+ * (at least 21 bytes). */
+#define IMEX_FMT_E20_12 0
+#define IMEX_FMT_ES15_7 1
+#define IMEX_FMT_ES14_7 2
+int imex_write_records(const char *path, const double *rec, int64_t n_rows, int n_col, int nx,
+                       int fmt, int n_threads);
+int imex_write_restart(const char *path, int nx, int ny, int n_vars, int n_solid, double t,
+                       double dt, const double *q_aos, const double *B_cent,
+                       const double *erodible, const double *deposit, const double *erosion,
+                       const double *hmax, const double *pdynmax, const double *mod_vel_max);
+int imex_format_value(int fmt, double x, char *out);
+
 /* ---- batches of independent runs -----------------------------------------
  * No single counterpart in the reference: the reference runs an ensemble as
  * independent processes, one driver loop each (EXAMPLES/

@@ -261,3 +261,57 @@ class RunoutRef:
                 stop_flag = True
         del area_old
         return stop_flag
+
+
+def zero_tiny(a):
+    """IF (ABS(x) .LT. 1.0E-20_wp) x = 0.0_wp, elementwise (signed zeros
+    become +0.0), as output_solution does to q and DEPOSIT."""
+    a = np.array(a, dtype=np.float64, copy=True)
+    a[np.abs(a) < 1.0e-20] = 0.0
+    return a
+
+
+def p2d_records(cfg, q, deposit, B, hmax, pdynmax, mod_vel_max, oracle):
+    """output_solution's .p_2d columns per cell (src/inpout_2d.f90:6169-6396)
+    for the configured path (Voellmy rheology, model 1; no added gas, liquid,
+    stochastic or pore-pressure variables; no erosion, erodible 0 with
+    porosity 0).  The reference's mu_eff reads pore_pres(1) of a zero-size
+    array; here the excess pore pressure is 0.  q is [k, j, var], deposit
+    [class, k, j]; returns ([N, 19 + 4 n_solid] records, the zeroed
+    deposit)."""
+    ny, nx, nv = q.shape
+    ns = cfg.n_solid
+    N = nx * ny
+    qc = q.reshape(N, nv)
+    qp, _ = oracle.qc_to_qp(qc)
+    mx = oracle.mixt_var(qp)                 # Ri, rho_m, rho_c, red_grav, cp_c, cp_mix
+    x_comp = 0.5 * ((cfg.x0 + np.arange(nx) * cfg.dx) + (cfg.x0 + np.arange(1, nx + 1) * cfg.dx))
+    y_comp = 0.5 * ((cfg.y0 + np.arange(ny) * cfg.dx) + (cfg.y0 + np.arange(1, ny + 1) * cfg.dx))
+    dep = deposit.reshape(ns, N).copy()
+    out = np.zeros((N, 19 + 4 * ns))
+    z = lambda x: 0.0 if abs(x) < 1.0e-20 else x  # noqa: E731
+    Bf = np.asarray(B, dtype=np.float64).reshape(N)
+    for c in range(N):
+        j, k = c % nx, c // nx
+        h, u, v, T = qp[c, 0], qp[c, nv], qp[c, nv + 1], qp[c, 3]
+        Ri, rho_m, rg = mx[c, 0], mx[c, 1], mx[c, 3]
+        if h > 0.0:
+            al = [qp[c, 4 + i] / h for i in range(ns)]
+            excess = 0.0                       # MAX(0, pore_pres(1) - pres) with pore_pres(1) = pres
+            with np.errstate(divide="ignore", invalid="ignore"):
+                d = float(1.0 - np.float64(excess) / np.float64(rho_m * h * rg))
+            mu_eff = cfg.mu * (0.0 if 0.0 > d else d)   # Fortran MAX(0.0, d)
+        else:
+            al = [0.0] * ns
+            mu_eff = 0.0
+        h, u, v = z(h), z(u), z(v)
+        Bo = z(Bf[c])
+        al = [z(a) for a in al]
+        for i in range(ns):
+            dep[i, c] = z(dep[i, c])
+        T, rho_m, rg, mu_eff = z(T), z(rho_m), z(rg), z(mu_eff)
+        row = [x_comp[j], y_comp[k], h, u, v, Bo, h + Bo] + al + [T, rho_m, rg] \
+            + [dep[i, c] for i in range(ns)] + [0.0] * ns + [0.0, 0.0, 0.0, Ri] + [0.0] * ns \
+            + [mu_eff, hmax.reshape(N)[c], pdynmax.reshape(N)[c], mod_vel_max.reshape(N)[c], 0.0]
+        out[c] = row
+    return out, dep.reshape(deposit.shape)
