@@ -126,8 +126,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5000)
     ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--workload", default="c2",
-                    help="c2 (default), c1, c2-small, c3, c4 (the batched ensemble) or c5 "
-                         "(serving)")
+                    help="c2 (default), c1, c2-small, c3, c4 (the batched ensemble), c5 "
+                         "(serving) or c2-outputs (the run with periodic outputs)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the reference CPU sample")
     ap.add_argument("--event-every", type=int, default=10, metavar="E",
@@ -141,6 +141,10 @@ def main():
     ap.add_argument("--rates", default="2,8,32", help="c5: arrival rates per second")
     ap.add_argument("--window", type=float, default=20.0, help="c5: arrival window (wall s)")
     ap.add_argument("--serve-t-end", default="60,600", help="c5: t_end range of a request (s)")
+    ap.add_argument("--out-t-end", type=float, default=200.0,
+                    help="c2-outputs: physical end time (s)")
+    ap.add_argument("--out-dt", type=float, default=100.0,
+                    help="c2-outputs: interval of the solution outputs (s)")
     ap.add_argument("--info", default=None,
                     help="also write per-kernel launch counts and active cells to this JSON file")
     args = ap.parse_args()
@@ -175,6 +179,9 @@ def main():
         return
     if args.workload == "c3":
         bench_c3(args)
+        return
+    if args.workload == "c2-outputs":
+        bench_outputs(args)
         return
 
     t_host = time.perf_counter()
@@ -407,6 +414,196 @@ def bench_c5(args):
                    "rates": res},
     }
     print(json.dumps(out), flush=True)
+
+
+def bench_outputs(args):
+    """The radial-source flow at 2048^2 (c2) to t = --out-t-end physical
+    seconds with the reference's periodic outputs on the device state:
+    output_runout every 10 s, 16 probes every 5 s, output_solution (.q_2d,
+    deposit .q_2d, .p_2d) and restart.bin every --out-dt s, against the same
+    run without outputs.  value = cell-stage updates/s with the outputs;
+    per-output costs and the writers' throughput beside it.  cpu_baseline:
+    the Fortran runtime writing the same .p_2d records single-threaded
+    (oracle/_ref/records_writer, the reference's own format statement) and
+    the whole-array runout expressions in numpy, on a sample."""
+    import shutil
+    from imex_sflow2d_dev_amd import init, native, outputs, workloads
+    from imex_sflow2d_dev_amd.solver import Solver
+    say = lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True)  # noqa: E731
+    cfg, B = workloads.by_name("c2")
+    static, q0 = init.setup(cfg, B)
+    t_end, dt_out = args.out_t_end, args.out_dt
+    # the file system with the most free space; one solution output of this
+    # grid writes about 3.3 GB
+    cands = [d for d in ("/dev/shm", tempfile.gettempdir()) if os.path.isdir(d)]
+    base = max(cands, key=lambda d: shutil.disk_usage(d).free)
+    free = shutil.disk_usage(base).free
+    if 3.5e9 * (int(t_end // dt_out) + 1) > 0.8 * free:
+        dt_out = t_end
+    if 3.5e9 * 2 > 0.8 * free:
+        sys.exit(f"c2-outputs needs about 7 GB free for its files; {base} has {free / 1e9:.1f} GB")
+    say(f"files in {base} ({free / 1e9:.0f} GB free), solution outputs every {dt_out} s")
+    work = tempfile.mkdtemp(prefix="imex_outputs_", dir=base)
+    res = {}
+    try:
+        # the same run without outputs
+        s = Solver(cfg, static, q0, raise_on_error=False)
+        s.run_steps(3, t_end=cfg.t_start)      # loads the code objects, takes no step
+        a0, _ = s.counters()
+        t0 = time.perf_counter()
+        n_plain = 0
+        while s.t < t_end:
+            d = s.run_steps(256, t_end=t_end)
+            n_plain += int(np.count_nonzero(d))
+        wall_plain = time.perf_counter() - t0
+        cells_plain = s.counters()[0] - a0
+        s.close()
+        say(f"no outputs: {n_plain} steps to t={t_end} in {wall_plain:.3f} s")
+
+        s = Solver(cfg, static, q0, raise_on_error=False)
+        s.run_steps(3, t_end=cfg.t_start)
+        xs, ys = cfg.x_source, cfg.y_source
+        ang = np.linspace(0.0, 2.0 * np.pi, 16, endpoint=False)
+        coords = np.stack([xs + 800.0 * np.cos(ang), ys + 800.0 * np.sin(ang)], 1)
+        ro = outputs.RunoutOutput(s, "c2", work, dt_runout=10.0, B=B)
+        po = outputs.ProbesOutput(s, coords, "c2", work, dt_probes=5.0)
+        so = outputs.SolutionOutput(s, "c2", work, dt_output=dt_out, B=B, n_threads=16)
+        calls = {"runout": [], "probes": []}
+
+        def timed(fn, key):
+            def call(*a):
+                t = time.perf_counter()
+                r = fn(*a)
+                calls[key].append(time.perf_counter() - t)
+                return r
+            return call
+
+        class W:  # a writer's calls timed, its state shared
+            def __init__(self, inner, key):
+                object.__setattr__(self, "_i", inner)
+                object.__setattr__(self, "_c", timed(inner.__call__, key))
+
+            def __call__(self, *a):
+                return self._c(*a)
+
+            def __getattr__(self, n):
+                return getattr(self._i, n)
+
+            def __setattr__(self, n, v):
+                setattr(self._i, n, v)
+
+        a0, _ = s.counters()
+        t0 = time.perf_counter()
+        steps, _ = outputs.run_with_outputs(s, t_end, runout=W(ro, "runout"), probes=W(po, "probes"),
+                                            solution=so, chunk=256)
+        wall_out = time.perf_counter() - t0
+        cells_out = s.counters()[0] - a0
+        sizes = {f: os.path.getsize(os.path.join(work, f)) for f in os.listdir(work)}
+        sol_bytes = sum(v for f, v in sizes.items() if f.endswith((".q_2d", ".p_2d", ".bin")))
+        say(f"with outputs: {steps} steps in {wall_out:.3f} s; {len(calls['runout'])} runout, "
+            f"{len(calls['probes'])} probe, {so.output_idx} solution outputs")
+        # device costs of one call, on the final state
+        t = time.perf_counter()
+        for _ in range(20):
+            s.runout_reduce(ro.x0, ro.y0, 0.0)
+        red_s = (time.perf_counter() - t) / 20
+        t = time.perf_counter()
+        for _ in range(20):
+            s.probes(coords)
+        probe_s = (time.perf_counter() - t) / 20
+        N = cfg.nx * cfg.ny
+        p2d = so.bufs[native.SNAP_P2D].a.reshape(N, -1)
+        # writer throughput on one .p_2d file: 16 threads and 1 thread
+        wt = {}
+        for nt in (16, 1):
+            path = os.path.join(work, f"tp_{nt}.p_2d")
+            t = time.perf_counter()
+            native.write_records(path, p2d, cfg.nx, native.FMT_ES15_7, nt)
+            wt[nt] = time.perf_counter() - t
+            p2d_bytes = os.path.getsize(path)
+            os.remove(path)
+        res = {
+            "wall_s": wall_out, "steps": steps, "cells": cells_out, "physics_error_bits": s.err,
+            "cell_stage_updates_per_s": 2.0 * cells_out / wall_out,
+            "no_outputs": {"wall_s": wall_plain, "steps": n_plain,
+                           "cell_stage_updates_per_s": 2.0 * cells_plain / wall_plain},
+            "outputs": {"runout_calls": len(calls["runout"]),
+                        "runout_call_ms_mean": 1e3 * float(np.mean(calls["runout"])),
+                        "runout_reduce_ms": 1e3 * red_s,
+                        "probe_calls": len(calls["probes"]),
+                        "probe_call_ms_mean": 1e3 * float(np.mean(calls["probes"])),
+                        "probes_device_ms": 1e3 * probe_s, "n_probes": len(coords),
+                        "solution_outputs": so.output_idx,
+                        "solution_caller_ms": [1e3 * a for a, _ in so.timings],
+                        "solution_writer_s": [b for _, b in so.timings],
+                        "solution_bytes_total": sol_bytes,
+                        "files": len(sizes)},
+            "p2d_file_bytes": p2d_bytes,
+            "p2d_write_s": {"16_threads": wt[16], "1_thread": wt[1]},
+            "p2d_write_MBps_16_threads": p2d_bytes / wt[16] / 1e6,
+        }
+        sample = np.array(p2d[:128 * cfg.nx])     # the page-locked buffers go with so.close()
+        so.close()
+        if not args.no_cpu:
+            res["cpu_baseline"] = outputs_cpu_baseline(cfg, s, sample, work, say)
+        s.close()
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+    out = {
+        "metric": "cell-stage updates/s with periodic outputs",
+        "value": res["cell_stage_updates_per_s"],
+        "unit": "cell-stage updates/s",
+        "n_gpus": 1,
+        "steps": res["steps"],
+        "warmup": 0,
+        "ms_per_step": res["wall_s"] * 1e3 / res["steps"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded PCG64 DEM and source, generated in-process)",
+        "config": {"workload": (f"c2-outputs: {cfg.nx}x{cfg.ny} cells, radial source, to t = {t_end} s "
+                                f"with output_runout every 10 s, 16 probes every 5 s, output_solution "
+                                f"(.q_2d, deposit .q_2d, .p_2d) and restart.bin every {dt_out} s, "
+                                f"files in {base}"),
+                   **{k: v for k, v in res.items() if k != "cpu_baseline"}},
+    }
+    if "cpu_baseline" in res:
+        out["cpu_baseline"] = res["cpu_baseline"]
+    print(json.dumps(out), flush=True)
+
+
+def outputs_cpu_baseline(cfg, s, p2d, work, say):
+    """The Fortran runtime writing .p_2d records single-threaded (the
+    reference's format statement, oracle/_ref/records_writer) on a sample
+    of rows of cells (p2d), and the whole-array runout expressions of
+    output_runout in numpy (oracle/outputs_ref.py) on the full grid."""
+    from oracle import oraclepy, outputs_ref
+    exe = os.path.join(HERE, "oracle", "_ref", "records_writer")
+    out = {"kind": "port", "cores": 1}
+    if os.path.exists(exe):
+        rec = np.ascontiguousarray(p2d)
+        rows = rec.shape[0] // cfg.nx
+        raw = os.path.join(work, "sample.raw")
+        rec.tofile(raw)
+        dst = os.path.join(work, "sample.p_2d")
+        t = time.perf_counter()
+        subprocess.run([exe, "p2d", raw, dst, str(cfg.nx), str(rows), str(rec.shape[1])], check=True)
+        wf = time.perf_counter() - t
+        nb = os.path.getsize(dst)
+        out["p2d_write_MBps"] = nb / wf / 1e6
+        out["p2d_sample"] = f"{rows} rows of {cfg.nx} cells, {nb} bytes, {wf:.2f} s"
+        say(f"Fortran runtime .p_2d writer: {nb / wf / 1e6:.1f} MB/s")
+    q = s.q
+    o = oraclepy.Oracle(cfg)
+    t = time.perf_counter()
+    qp = outputs_ref.module_qp(cfg, q, o.qc_to_qp)
+    outputs_ref.runout_red(cfg, q, qp, np.zeros((cfg.ny, cfg.nx), bool), cfg.x_source, cfg.y_source, 0.0)
+    out["runout_numpy_s"] = time.perf_counter() - t
+    out["value"] = out.get("p2d_write_MBps")
+    out["unit"] = "MB/s (.p_2d text, one thread)"
+    out["sample"] = out.get("p2d_sample", "")
+    return out
 
 
 def s_time(s):
