@@ -67,6 +67,16 @@ class RunConfig:
     h_source: float = 0.0             # derived from mfr_source when > 0
     time_param: List[float] = dataclasses.field(default_factory=lambda: [0.0, 0.0, 0.0, 0.0])
     alphas_source: List[float] = dataclasses.field(default_factory=lambda: [0.0])
+    # erosion of an erodible layer (SOLID_TRANSPORT_PARAMETERS,
+    # src/inpout_2d.f90:1813-1900); the layer's thickness per class is
+    # erodible_fract * (1 - erodible_porosity) * erodible_thickness
+    # (src/inpout_2d.f90:1982-1986), uniform here
+    erosion_coeff: float = 0.0
+    erodible_porosity: float = 0.0
+    erodible_fract: List[float] = dataclasses.field(default_factory=list)
+    T_erodible: float = 300.0
+    erodible_deposit_flag: bool = False
+    erodible_thickness: float = 0.0
     # vulnerability table (VULNERABILITY_TABLE_PARAMETERS)
     thickness_levels: List[float] = dataclasses.field(default_factory=list)
     dyn_pres_levels: List[float] = dataclasses.field(default_factory=list)
@@ -101,6 +111,26 @@ class RunConfig:
     def n_cells(self) -> int:
         return self.nx * self.ny
 
+    def erodible_fractions(self) -> List[float]:
+        """erodible_fract as read_param leaves it (src/inpout_2d.f90:1827-1866):
+        1/n_solid without erosion, 1 for one class, else the given fractions
+        divided by their sum when it is not one."""
+        ns = self.n_solid
+        if self.erosion_coeff == 0.0:
+            return [1.0 / ns] * ns
+        if ns == 1:
+            return [1.0]
+        f = [float(x) for x in self.erodible_fract[:ns]]
+        s = sum(f)
+        return [x / s for x in f] if s != 1.0 else f
+
+    def effective_T_erodible(self) -> float:
+        """T_erodible: 300 K without erosion (src/inpout_2d.f90:1831)."""
+        return 300.0 if self.erosion_coeff == 0.0 else self.T_erodible
+
+    def effective_erodible_porosity(self) -> float:
+        return 0.0 if self.erosion_coeff == 0.0 else self.erodible_porosity
+
     def validate(self) -> None:
         if self.nx < 5 or self.ny < 5:
             raise ValueError("the 5-point bed-slope stencil needs nx, ny >= 5")
@@ -118,6 +148,16 @@ class RunConfig:
             raise ValueError("only the minmod limiter (1) is on this path")
         if self.bc_flag != 1:
             raise ValueError("only Neumann (flag 1) boundaries are on this path")
+        if self.erosion_coeff < 0.0:
+            raise ValueError("erosion_coeff must be >= 0")
+        if self.erosion_coeff > 0.0:
+            if not (0.0 <= self.erodible_porosity < 1.0):
+                raise ValueError("erodible_porosity must be in [0, 1)")
+            if self.n_solid > 1 and (len(self.erodible_fract) < self.n_solid
+                                     or min(self.erodible_fract[:self.n_solid]) < 0.0):
+                raise ValueError("erodible_fract needs n_solid non-negative entries")
+            if self.T_erodible < 0.0:
+                raise ValueError("T_erodible must be >= 0")
 
 
 def solid_arrays(cfg: RunConfig):

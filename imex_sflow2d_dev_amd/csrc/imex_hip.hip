@@ -1299,11 +1299,13 @@ __device__ void stage2(const Params& P, double dt, double* q, const double* X,
 }
 
 // update_erosion_deposition_cell for one cell: src/solver_2d.f90:2429-2593
+// erod / ero: the cell's erodible layer and eroded thickness (updated; the
+// caller passes zeros and ignores them when erosion is off)
 template <int NS, int LANES>
 __device__ void erosion_deposition(const Params& P, double dt, double* q, double* depo,
-                                   unsigned int& err, int lane) {
+                                   double* erod, double* ero, unsigned int& err, int lane) {
   constexpr int NV = 4 + NS, NP = NV + 2;
-  double qp[NP], pd, dep[NS], eq[NV];
+  double qp[NP], pd, dep[NS], eq[NV], ers[NS];
   if (q[0] > 0.0) {
     qc_to_qp<NS>(P, q, qp, pd);
   } else {
@@ -1311,13 +1313,21 @@ __device__ void erosion_deposition(const Params& P, double dt, double* q, double
     for (int i = 0; i < NP; i++) qp[i] = 0.0;
     qp[3] = P.T_amb;
   }
-  if (mass_exchange<NS, LANES>(P, qp, dt, dep, eq, lane)) err |= IMEX_ERR_NEG_DEP;
+  if (mass_exchange<NS, LANES>(P, qp, erod, dt, dep, eq, ers, lane)) err |= IMEX_ERR_NEG_DEP;
 #pragma unroll
   for (int i = 0; i < NV; i++) q[i] = q[i] + dt * eq[i];
 #pragma unroll
   for (int i = 0; i < NS; i++) q[4 + i] = fmax_f(0.0, q[4 + i]);
 #pragma unroll
   for (int i = 0; i < NS; i++) depo[i] = depo[i] + dt * dep[i];
+#pragma unroll
+  for (int i = 0; i < NS; i++) ero[i] = ero[i] + dt * ers[i];
+#pragma unroll
+  for (int i = 0; i < NS; i++) erod[i] = erod[i] - dt * ers[i];
+  if (P.erodible_deposit) {
+#pragma unroll
+    for (int i = 0; i < NS; i++) erod[i] = erod[i] + dt * dep[i];
+  }
 #pragma unroll
   for (int i = 0; i < NS; i++)
     if (q[4 + i] < 0.0) err |= IMEX_ERR_NEG_SOLID;
@@ -1453,14 +1463,27 @@ __device__ __forceinline__ void update_body(const Params& P, const Grid& G, doub
       stage2<NS, LANES>(P, dt, q, X, err, lane);
     }
     if constexpr ((PARTS & 2) != 0) {
-      if (P.settling || P.entrainment) {
-        double depo[NS];
+      if (P.settling || P.entrainment || P.erosion_coeff > 0.0) {
+        double depo[NS], erod[NS], ero[NS];
 #pragma unroll
         for (int v = 0; v < NS; v++) depo[v] = G.dep[(size_t)v * N + c];
-        erosion_deposition<NS, LANES>(P, dt, q, depo, err, lane);
+        const bool er = G.erodible != nullptr;
+#pragma unroll
+        for (int v = 0; v < NS; v++) {
+          erod[v] = er ? G.erodible[(size_t)v * N + c] : 0.0;
+          ero[v] = er ? G.erosion[(size_t)v * N + c] : 0.0;
+        }
+        erosion_deposition<NS, LANES>(P, dt, q, depo, erod, ero, err, lane);
         if (lead) {
 #pragma unroll
           for (int v = 0; v < NS; v++) G.dep[(size_t)v * N + c] = depo[v];
+          if (er) {
+#pragma unroll
+            for (int v = 0; v < NS; v++) {
+              G.erodible[(size_t)v * N + c] = erod[v];
+              G.erosion[(size_t)v * N + c] = ero[v];
+            }
+          }
         }
       }
       // wet at the step start (flag left by the stage-2 launch), exactly dry
@@ -1639,10 +1662,19 @@ __global__ void kt_settling(Params P, int n, const double* d, const double* rs, 
   if (i < n) out[i] = settling_velocity(P, d[i], rs[i], rc[i], ikv[i]);
 }
 template <int NS>
-__global__ void kt_mass_exchange(Params P, int n, const double* qp, double dt, double* dep,
-                                 double* eq) {
+__global__ void kt_mass_exchange(Params P, int n, const double* qp, const double* erod, double dt,
+                                 double* dep, double* eq, double* ers) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) mass_exchange<NS>(P, qp + (size_t)(6 + NS) * i, dt, dep + (size_t)NS * i, eq + (size_t)(4 + NS) * i);
+  if (i < n) {
+    double e[NS], r[NS];
+#pragma unroll
+    for (int k = 0; k < NS; k++) e[k] = erod ? erod[(size_t)NS * i + k] : 0.0;
+    mass_exchange<NS>(P, qp + (size_t)(6 + NS) * i, e, dt, dep + (size_t)NS * i,
+                      eq + (size_t)(4 + NS) * i, r);
+    if (ers)
+#pragma unroll
+      for (int k = 0; k < NS; k++) ers[(size_t)NS * i + k] = r[k];
+  }
 }
 template <int NS>
 __global__ void kt_newton(Params P, int n, double dt, const double* qsi, const double* q0,
@@ -1807,7 +1839,22 @@ extern "C" int imex_create(const imex_config* cfg, imex_handle** out) {
     P.cp_s[i] = cfg->sp_heat_s[i];
     P.alphas_source[i] = cfg->alphas_source[i];
   }
-  P.T_erodible = 300.0;
+  // erosion parameters as read_param leaves them (src/inpout_2d.f90:1827-1900)
+  P.erosion_coeff = cfg->erosion_coeff;
+  if (P.erosion_coeff > 0.0) {
+    P.erodible_porosity = cfg->erodible_porosity;
+    P.coeff_porosity = P.erodible_porosity / (1.0 - P.erodible_porosity);
+    for (int i = 0; i < 3; i++) P.erodible_fract[i] = (cfg->n_solid == 1) ? (i == 0 ? 1.0 : 0.0)
+                                                                           : cfg->erodible_fract[i];
+    P.T_erodible = cfg->T_erodible;
+    P.erodible_deposit = cfg->erodible_deposit_flag;
+  } else {
+    P.erodible_porosity = 0.0;
+    P.coeff_porosity = 0.0;
+    for (int i = 0; i < 3; i++) P.erodible_fract[i] = 1.0 / cfg->n_solid;
+    P.T_erodible = 300.0;
+    P.erodible_deposit = cfg->erodible_deposit_flag;
+  }
   P.rho_c_sub = cfg->pres / (cfg->sp_gas_const_a * P.T_erodible);
   P.vel_source = cfg->vel_source;
   P.T_source = cfg->T_source;
@@ -1844,6 +1891,10 @@ extern "C" int imex_create(const imex_config* cfg, imex_handle** out) {
   if ((rc = dalloc(h, &G.pdyn, N))) return rc;
   if ((rc = dalloc(h, &G.mvel, N))) return rc;
   if ((rc = dalloc(h, &G.dep, N * ns))) return rc;
+  if (P.erosion_coeff > 0.0 || P.erodible_deposit) {
+    if ((rc = dalloc(h, &G.erodible, N * ns))) return rc;
+    if ((rc = dalloc(h, &G.erosion, N * ns))) return rc;
+  }
   if (P.n_thk * P.n_pdyn > 0)
     if ((rc = dalloc(h, &G.vuln, N))) return rc;
   if ((rc = dalloc(h, &G.mask, N))) return rc;
@@ -2625,6 +2676,46 @@ extern "C" int imex_download(imex_handle* h, unsigned what, double* q_aos, doubl
   return 0;
 }
 
+extern "C" int imex_set_erodible(imex_handle* h, const double* erodible) {
+  if (!h || !erodible) return -2;
+  if (!h->G.erodible) {
+    h->err = "imex_set_erodible: the run has no erosion (erosion_coeff = 0)";
+    return -2;
+  }
+  HCHK(h, hipSetDevice(h->device));
+  const size_t N = (size_t)h->G.N;
+  const int ns = h->P.ns;
+  // erodible(n_solid, nx, ny) -> [class][cell]
+  std::vector<double> t(N * ns);
+  for (size_t c = 0; c < N; c++)
+    for (int i = 0; i < ns; i++) t[(size_t)i * N + c] = erodible[c * ns + i];
+  HCHK(h, hipMemcpyAsync(h->G.erodible, t.data(), 8 * N * ns, hipMemcpyHostToDevice, h->stream));
+  HCHK(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+extern "C" int imex_get_erosion(imex_handle* h, double* erosion, double* erodible) {
+  if (!h) return -2;
+  HCHK(h, hipSetDevice(h->device));
+  const size_t N = (size_t)h->G.N;
+  const int ns = h->P.ns;
+  if (!h->G.erodible) {
+    if (erosion) std::fill(erosion, erosion + N * ns, 0.0);
+    if (erodible) std::fill(erodible, erodible + N * ns, 0.0);
+    return 0;
+  }
+  if (erosion)
+    HCHK(h, hipMemcpyAsync(erosion, h->G.erosion, 8 * N * ns, hipMemcpyDeviceToHost, h->stream));
+  std::vector<double> t(erodible ? N * ns : 0);
+  if (erodible)
+    HCHK(h, hipMemcpyAsync(t.data(), h->G.erodible, 8 * N * ns, hipMemcpyDeviceToHost, h->stream));
+  HCHK(h, hipStreamSynchronize(h->stream));
+  if (erodible)
+    for (size_t c = 0; c < N; c++)
+      for (int i = 0; i < ns; i++) erodible[c * ns + i] = t[(size_t)i * N + c];
+  return 0;
+}
+
 extern "C" int imex_last_error(imex_handle* h, char* buf, size_t len) {
   if (!h || !buf || len == 0) return -2;
   snprintf(buf, len, "%s", h->err.c_str());
@@ -2807,9 +2898,26 @@ extern "C" int imex_test_mass_exchange(imex_handle* h, int n, const double* qp, 
   DevBufs b{h};
   double *dqp = b.up(qp, (size_t)n * (nv + 2)), *dd = b.up(nullptr, (size_t)n * ns), *de = b.up(nullptr, (size_t)n * nv);
   BUFS_OK(h, b);
-  NSDISPATCH(h, kt_mass_exchange, n, dqp, dt, dd, de);
+  NSDISPATCH(h, kt_mass_exchange, n, dqp, (const double*)nullptr, dt, dd, de, (double*)nullptr);
   if (fin(h)) return -1;
   if (down(h, dep, dd, (size_t)n * ns) || down(h, eq, de, (size_t)n * nv)) return -1;
+  return 0;
+}
+extern "C" int imex_test_mass_exchange_ers(imex_handle* h, int n, const double* qp,
+                                           const double* erod, double dt, double* dep, double* eq,
+                                           double* ers) {
+  if (!h || n <= 0) return -2;
+  const int nv = h->P.nv, ns = h->P.ns;
+  DevBufs b{h};
+  double *dqp = b.up(qp, (size_t)n * (nv + 2)), *der = b.up(erod, (size_t)n * ns),
+         *dd = b.up(nullptr, (size_t)n * ns), *de = b.up(nullptr, (size_t)n * nv),
+         *drs = b.up(nullptr, (size_t)n * ns);
+  BUFS_OK(h, b);
+  NSDISPATCH(h, kt_mass_exchange, n, dqp, (const double*)der, dt, dd, de, drs);
+  if (fin(h)) return -1;
+  if (down(h, dep, dd, (size_t)n * ns) || down(h, eq, de, (size_t)n * nv) ||
+      down(h, ers, drs, (size_t)n * ns))
+    return -1;
   return 0;
 }
 extern "C" int imex_test_newton(imex_handle* h, int n, double dt, const double* q_si,

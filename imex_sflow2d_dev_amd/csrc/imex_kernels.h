@@ -75,6 +75,8 @@ struct Grid {
   double* pdyn;  // [N]
   double* mvel;  // [N]
   double* dep;   // [ns][N]
+  double* erodible;  // [ns][N] erodible layer (erosion runs only)
+  double* erosion;   // [ns][N] eroded thickness (erosion runs only)
   unsigned long long* vuln;  // [N] bit (i_thk*n_pdyn + i_pdyn)
   unsigned char* mask;       // [N]
   int* tile_list;            // [2][ntiles] active tiles, two slots

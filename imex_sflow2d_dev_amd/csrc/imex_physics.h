@@ -36,6 +36,9 @@ struct Params {
   double eps_sing, eps_sing4;
   double rho_s[3], inv_rho_s[3], diam_s[3], cp_s[3], alphas_source[3];
   double rho_c_sub, T_erodible;
+  // erosion of the erodible layer (src/inpout_2d.f90:1813-1900)
+  double erosion_coeff, erodible_porosity, coeff_porosity, erodible_fract[3];
+  int erodible_deposit;
   double vel_source, T_source, h_source, time_param[4], pi_g;
   double thk[8], pdyn[8];
   double hcs, one_by_hcs;  // complex step n_vars*eps and its reciprocal
@@ -700,18 +703,19 @@ __device__ inline double settling_velocity(const Params& P, double diam, double 
   return sv;
 }
 
-// eval_mass_exchange_terms without erosion / pore pressure / vaporisation:
-// src/constitutive_2d.f90:4574-5153.  Returns true when the reference would
-// pause on a negative deposition rate.
+// eval_mass_exchange_terms without pore pressure / vaporisation:
+// src/constitutive_2d.f90:4574-5153 (erosion :4731-4758).  erod: the cell's
+// erodible layer per class; ers receives the erosion rates.  Returns true
+// when the reference would pause on a negative deposition rate.
 // LANES = 2: the two lanes of an aligned pair evaluate the same cell and
 // must call this together; lane `lane` computes the settling velocities of
 // the classes i with i % 2 == lane and the pair swaps them, so the slowest
 // part (the drag fixed point) runs on both lanes at once.
 template <int NS, int LANES = 1>
-__device__ inline bool mass_exchange(const Params& P, const double* qp, double dt, double* dep,
-                                     double* eq, int lane = 0) {
+__device__ inline bool mass_exchange(const Params& P, const double* qp, const double* erod,
+                                     double dt, double* dep, double* eq, double* ers,
+                                     int lane = 0) {
   constexpr int NV = 4 + NS;
-  double ers[NS];
 #pragma unroll
   for (int i = 0; i < NS; i++) {
     ers[i] = 0.0;
@@ -729,12 +733,19 @@ __device__ inline bool mass_exchange(const Params& P, const double* qp, double d
   for (int i = 0; i < NS; i++) at += alphas[i];
   double w = 0.0;
   double mod_vel2 = u * u + v * v + w * w;
+  if (P.erosion_coeff > 0.0) {
+    // Fagents & Baloga (2006) with the solid volume fraction
+    const double tot_erosion = P.erosion_coeff * sqrt(mod_vel2) * h * (1.0 - at);
+    const double tse0 = tot_erosion * (1.0 - P.erodible_porosity);
 #pragma unroll
-  for (int i = 0; i < NS; i++) ers[i] = fmax_f(0.0, fmin_f(ers[i], 0.0 / dt));
+    for (int i = 0; i < NS; i++) ers[i] = P.erodible_fract[i] * tse0;
+  }
+#pragma unroll
+  for (int i = 0; i < NS; i++) ers[i] = fmax_f(0.0, fmin_f(ers[i], erod[i] / dt));
   double tse = 0.0;
 #pragma unroll
   for (int i = 0; i < NS; i++) tse += ers[i];
-  double cpe = (tse / (1.0 - 0.0)) * 0.0;
+  double cpe = (tse / (1.0 - P.erodible_porosity)) * P.erodible_porosity;
   if (at <= 0.0) return false;
   double T = qp[3];
   Mixt m = mixt_var<NS>(P, qp, true);
@@ -785,11 +796,14 @@ __device__ inline bool mass_exchange(const Params& P, const double* qp, double d
   double sdep = 0.0;
 #pragma unroll
   for (int i = 0; i < NS; i++) sdep += dep[i];
-  double cpl = 0.0 + 0.0 * sdep;
+  double cpl = 0.0 + P.coeff_porosity * sdep;
   // the continuous-phase loss is limited by the maximum solid packing (0.64)
   // and by the continuous phase present (src/constitutive_2d.f90:5034-5041)
   cpl = fmin_f(cpl, h * fmax_f(0.0, 0.64 - at) / dt);
-  cpl = fmin_f(cpl, (h * (1.0 - at) - 0.0 * (h * at - dt * sdep)) / dt);
+  cpl = fmin_f(cpl, (h * (1.0 - at) - P.coeff_porosity * (h * at - dt * sdep)) / dt);
+  // with erodible_deposit_flag the reference sets T_erodible to the cell's T
+  // before using it (src/constitutive_2d.f90:5044)
+  const double T_er = P.erodible_deposit ? T : P.T_erodible;
   double air = 0.0;
   if (P.entrainment && (h > 0.0) && (m.Ri > 0.0)) {
     double ec = 0.075 / sqrt(1.0 + 718.0 * pow_cr(m.Ri, 2.4));
@@ -810,9 +824,9 @@ __device__ inline bool mass_exchange(const Params& P, const double* qp, double d
   eq[2] = -(v * (rdt + rc * cpl));
   if (P.energy)
     eq[3] = -(T * (A + rc * cpc * cpl)) - 0.5 * mod_vel2 * (rdt + rc * cpl) +
-            P.T_erodible * (D + P.rho_c_sub * cpc * cpe) + P.T_amb * P.cp_a * P.rho_a_amb * air;
+            T_er * (D + P.rho_c_sub * cpc * cpe) + P.T_amb * P.cp_a * P.rho_a_amb * air;
   else
-    eq[3] = -(T * (A + rc * cpc * cpl)) + P.T_erodible * (D + P.rho_c_sub * cpc * cpe) +
+    eq[3] = -(T * (A + rc * cpc * cpl)) + T_er * (D + P.rho_c_sub * cpc * cpe) +
             P.T_amb * P.cp_a * P.rho_a_amb * air;
 #pragma unroll
   for (int i = 0; i < NS; i++) eq[4 + i] = P.rho_s[i] * (ers[i] - dep[i]);

@@ -107,4 +107,10 @@ def setup(cfg: RunConfig, B: np.ndarray):
         if cfg.mfr_source > 0.0:
             cfg.h_source = source_thickness(cfg)
     st["h_source"] = cfg.h_source
+    if cfg.erosion_coeff > 0.0:
+        # erodible(i,:,:) = erodible_fract(i) * (1 - erodible_porosity) *
+        # erodible_init (src/inpout_2d.f90:1982-1986), a uniform layer here
+        por = cfg.effective_erodible_porosity()
+        st["erodible"] = np.stack([np.full((cfg.ny, cfg.nx), f * (1.0 - por) * cfg.erodible_thickness)
+                                   for f in cfg.erodible_fractions()])
     return st, initial_state(cfg)

@@ -43,6 +43,7 @@ EXPORTS = [
     "imex_set_topography", "imex_runout_reduce", "imex_probes",
     "imex_snapshot_cols", "imex_snapshot_begin", "imex_snapshot_wait", "imex_host_alloc",
     "imex_host_free", "imex_write_records", "imex_write_restart", "imex_format_value",
+    "imex_set_erodible", "imex_get_erosion", "imex_test_mass_exchange_ers",
 ]
 
 RUNOUT_STORE, RUNOUT_B = 1, 2
@@ -69,6 +70,9 @@ class ImexConfig(ctypes.Structure):
         ("h_source", ctypes.c_double), ("time_param", ctypes.c_double * 4),
         ("alphas_source", ctypes.c_double * 3),
         ("thickness_levels", ctypes.c_double * 8), ("dyn_pres_levels", ctypes.c_double * 8),
+        ("erosion_coeff", ctypes.c_double), ("erodible_porosity", ctypes.c_double),
+        ("T_erodible", ctypes.c_double), ("erodible_fract", ctypes.c_double * 3),
+        ("erodible_deposit_flag", ctypes.c_int32),
     ]
 
 
@@ -159,6 +163,12 @@ def make_config(cfg: RunConfig) -> ImexConfig:
         c.thickness_levels[i] = v
     for i, v in enumerate(cfg.dyn_pres_levels):
         c.dyn_pres_levels[i] = v
+    c.erosion_coeff = cfg.erosion_coeff
+    c.erodible_porosity = cfg.effective_erodible_porosity()
+    c.T_erodible = cfg.effective_T_erodible()
+    for i, v in enumerate(cfg.erodible_fractions()):
+        c.erodible_fract[i] = v
+    c.erodible_deposit_flag = int(cfg.erodible_deposit_flag)
     return c
 
 

@@ -78,6 +78,24 @@ class Solver:
                 ptr = native._ip(arr) if arr.dtype == np.int32 else native._dp(arr)
                 setattr(s, name, ptr)
         self.H.check(self.L.imex_set_static(self.h, ctypes.byref(s)), "imex_set_static")
+        if st.get("erodible") is not None and (self.cfg.erosion_coeff > 0.0
+                                               or self.cfg.erodible_deposit_flag):
+            self.set_erodible(st["erodible"])
+
+    def set_erodible(self, e: np.ndarray):
+        """The erodible layer as [class, k, j]."""
+        a = np.ascontiguousarray(np.transpose(np.asarray(e, dtype=np.float64), (1, 2, 0)))
+        assert a.shape == (self.cfg.ny, self.cfg.nx, self.cfg.n_solid)
+        self.H.check(self.L.imex_set_erodible(self.h, native._dp(a)), "imex_set_erodible")
+
+    def erosion(self):
+        """(erosion, erodible) as [class, k, j]."""
+        c = self.cfg
+        ers = np.zeros((c.n_solid, c.ny, c.nx))
+        erd = np.zeros((c.ny, c.nx, c.n_solid))
+        self.H.check(self.L.imex_get_erosion(self.h, native._dp(ers), native._dp(erd)),
+                     "imex_get_erosion")
+        return ers, np.transpose(erd, (2, 0, 1)).copy()
 
     def _upload(self, q0: np.ndarray):
         a = np.ascontiguousarray(q0, dtype=np.float64)
@@ -398,6 +416,19 @@ class KernelProbe:
         self._run("imex_test_mass_exchange", ctypes.c_int(n), native._dp(qp), ctypes.c_double(dt),
                   native._dp(dep), native._dp(eq))
         return dep, eq
+
+    def mass_exchange_ers(self, qp, erod, dt):
+        """(deposition, eqns_term, erosion) rates with the cells' erodible
+        layers erod [n, n_solid]."""
+        qp = np.ascontiguousarray(qp, np.float64)
+        erod = np.ascontiguousarray(erod, np.float64)
+        n = qp.shape[0]
+        dep = np.zeros((n, self.cfg.n_solid))
+        eq = np.zeros((n, self.nv))
+        ers = np.zeros((n, self.cfg.n_solid))
+        self._run("imex_test_mass_exchange_ers", ctypes.c_int(n), native._dp(qp), native._dp(erod),
+                  ctypes.c_double(dt), native._dp(dep), native._dp(eq), native._dp(ers))
+        return dep, eq, ers
 
     def newton(self, dt, q_si, q0, R):
         a = [np.ascontiguousarray(x, np.float64) for x in (q_si, q0, R)]

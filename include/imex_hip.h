@@ -55,6 +55,12 @@ typedef struct imex_config {
   double rho_s[3], diam_s[3], sp_heat_s[3];
   double vel_source, T_source, h_source, time_param[4], alphas_source[3];
   double thickness_levels[8], dyn_pres_levels[8];
+  /* erosion of an erodible layer (SOLID_TRANSPORT_PARAMETERS,
+   * src/inpout_2d.f90:1813-1900); with erosion_coeff = 0 the others are
+   * ignored as the reference ignores them.  erodible_fract as read_param
+   * leaves them (summing to one; 1 for one class). */
+  double erosion_coeff, erodible_porosity, T_erodible, erodible_fract[3];
+  int32_t erodible_deposit_flag;
 } imex_config;
 
 /* Per-cell static fields computed once by the host (src/geometry_2d.f90:555-895
@@ -147,6 +153,12 @@ int imex_get_time_history(imex_handle *h, double *t, double *dt_old, double *dt_
  * device-driven steps since the last upload (measurement; no counterpart in
  * the reference) */
 int imex_get_counters(imex_handle *h, int64_t *active_cells, int64_t *steps);
+/* The erodible layer erodible(n_solid, nx, ny) (Fortran order; read_erodible
+ * and src/inpout_2d.f90:1982-1986) of a run with erosion. */
+int imex_set_erodible(imex_handle *h, const double *erodible);
+/* erosion(nx, ny, n_solid) and erodible(n_solid, nx, ny) (either may be
+ * NULL); zeros for a run without erosion. */
+int imex_get_erosion(imex_handle *h, double *erosion, double *erodible);
 /* Lazy host mirrors: copies the selected device arrays (any pointer may be
  * NULL when not selected).  q in Fortran order; deposit(nx,ny,n_solid);
  * vuln as one int32 per (level, cell) in Fortran order; mask one int32 per cell. */
@@ -299,6 +311,8 @@ int imex_test_expl(imex_handle *h, int n, const double *bx, const double *by, do
                    const double *qp, double *expl);
 int imex_test_settling(imex_handle *h, int n, const double *diam, const double *rhos,
                        const double *rhoc, const double *ikv, double *out);
+int imex_test_mass_exchange_ers(imex_handle *h, int n, const double *qp, const double *erod,
+                                double dt, double *dep, double *eq, double *ers);
 int imex_test_mass_exchange(imex_handle *h, int n, const double *qp, double dt,
                             double *dep, double *eq);
 int imex_test_newton(imex_handle *h, int n, double dt, const double *q_si, const double *q0,

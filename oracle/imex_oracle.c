@@ -51,6 +51,9 @@ struct oracle_s {
     double eps_sing, eps_sing4;
     double rho_s[3], inv_rho_s[3], diam_s[3], cp_s[3], alphas_collapse[3], alphas_source[3];
     double rho_c_sub, T_erodible;
+    /* erosion (src/inpout_2d.f90:1813-1900) */
+    double erosion_coeff, erodible_porosity, coeff_porosity, erodible_fract[3];
+    int erodible_deposit;
     double vel_source, T_source, h_source, time_param[4], pi_g;
     double thk[8], pdyn[8];
     double hcs, one_by_hcs; /* complex step h = n_vars*eps (src/solver_2d.f90:274-275) */
@@ -69,6 +72,7 @@ struct oracle_s {
     int *jc, *kc, *jsx, *ksx, *jsy, *ksy;
     int ncells, nfx, nfy;
     double *hmax, *pdynmax, *modvelmax, *deposit;
+    double *erodible, *erosion; /* [class][cell] */
     unsigned char *vuln;
     double t, dt, dt_old, dt_old_old;
     long dried, stale;
@@ -591,14 +595,15 @@ static double settling_velocity(const oracle_t *o, double diam, double rhos, dou
     return sv;
 }
 
-/* eval_mass_exchange_terms, no erosion / pore pressure / vaporisation:
- * src/constitutive_2d.f90:4574-5153 (:4718-4870 deposition, :5046-5112 terms)
- * Returns 1 when the reference would pause (negative deposition). */
-static int eval_mass_exchange_terms(const oracle_t *o, const double *qp, double dt,
-                                    double *dep, double *eq)
+/* eval_mass_exchange_terms, no pore pressure / vaporisation:
+ * src/constitutive_2d.f90:4574-5153 (:4731-4758 erosion, :4718-4870
+ * deposition, :5046-5112 terms).  erod: the cell's erodible layer per class;
+ * ers receives the erosion rates.  Returns 1 when the reference would pause
+ * (negative deposition). */
+static int eval_mass_exchange_terms(const oracle_t *o, const double *qp, const double *erod,
+                                    double dt, double *dep, double *eq, double *ers)
 {
     const int ns = o->ns, nv = o->nv;
-    double ers[3];
     for (int i = 0; i < ns; i++) { ers[i] = 0.0; dep[i] = 0.0; }
     for (int i = 0; i < nv; i++) eq[i] = 0.0;
     if (qp[0] <= EPS) return 0;
@@ -609,12 +614,17 @@ static int eval_mass_exchange_terms(const oracle_t *o, const double *qp, double 
     for (int i = 0; i < ns; i++) at += alphas[i];
     double w = 0.0;
     double mod_vel2 = u * u + v * v + w * w;
-    double erodible = 0.0;
-    for (int i = 0; i < ns; i++) ers[i] = fmax_f(0.0, fmin_f(ers[i], erodible / dt));
+    double mod_vel = sqrt(mod_vel2);
+    if (o->erosion_coeff > 0.0) {
+        double tot_erosion = o->erosion_coeff * mod_vel * h * (1.0 - at);
+        double tse = tot_erosion * (1.0 - o->erodible_porosity);
+        for (int i = 0; i < ns; i++) ers[i] = o->erodible_fract[i] * tse;
+    }
+    for (int i = 0; i < ns; i++) ers[i] = fmax_f(0.0, fmin_f(ers[i], erod[i] / dt));
     double tse = 0.0;
     for (int i = 0; i < ns; i++) tse += ers[i];
-    double tot_erosion = tse / (1.0 - 0.0);
-    double cpe = tot_erosion * 0.0;
+    double tot_erosion = tse / (1.0 - o->erodible_porosity);
+    double cpe = tot_erosion * o->erodible_porosity;
     if (at <= 0.0) return 0; /* alphastot_min = 0 */
     double T = qp[3];
     double Ri, rm, rc, rg, cpc, cpm;
@@ -634,11 +644,12 @@ static int eval_mass_exchange_terms(const oracle_t *o, const double *qp, double 
     double sdep = 0.0;
     for (int i = 0; i < ns; i++) sdep += dep[i];
     double cpl = 0.0;
-    cpl = cpl + 0.0 * sdep; /* coeff_porosity = 0 */
+    cpl = cpl + o->coeff_porosity * sdep;
     /* limits by the maximum solid packing (0.64, src/inpout_2d.f90:854) and
      * by the continuous phase present: src/constitutive_2d.f90:5034-5041 */
     cpl = fmin_f(cpl, h * fmax_f(0.0, 0.64 - at) / dt);
-    cpl = fmin_f(cpl, (h * (1.0 - at) - 0.0 * (h * at - dt * sdep)) / dt);
+    cpl = fmin_f(cpl, (h * (1.0 - at) - o->coeff_porosity * (h * at - dt * sdep)) / dt);
+    const double T_erodible = o->erodible_deposit ? T : o->T_erodible;
     double air = 0.0;
     if (o->entrainment && (h > 0.0) && (Ri > 0.0)) {
         double ec = 0.075 / sqrt(1.0 + 718.0 * rpow(o, Ri, 2.4));
@@ -655,10 +666,10 @@ static int eval_mass_exchange_terms(const oracle_t *o, const double *qp, double 
     for (int i = 0; i < ns; i++) D += o->rho_s[i] * o->cp_s[i] * ers[i];
     if (o->energy)
         eq[3] = -(T * (A + rc * cpc * cpl)) - 0.5 * mod_vel2 * (rdt + rc * cpl) +
-                o->T_erodible * (D + o->rho_c_sub * cpc * cpe) +
+                T_erodible * (D + o->rho_c_sub * cpc * cpe) +
                 o->T_amb * o->cp_a * o->rho_a_amb * air;
     else
-        eq[3] = -(T * (A + rc * cpc * cpl)) + o->T_erodible * (D + o->rho_c_sub * cpc * cpe) +
+        eq[3] = -(T * (A + rc * cpc * cpl)) + T_erodible * (D + o->rho_c_sub * cpc * cpe) +
                 o->T_amb * o->cp_a * o->rho_a_amb * air;
     for (int i = 0; i < ns; i++) eq[4 + i] = o->rho_s[i] * (ers[i] - dep[i]);
     return err;
@@ -1528,7 +1539,7 @@ static void update_erosion_deposition_cell(oracle_t *o, double dt)
 {
     const int nv = o->nv, np = nv + 2, ns = o->ns;
     const size_t N = (size_t)o->nx * o->ny;
-    if (!o->settling && !o->entrainment) return;
+    if (o->erosion_coeff == 0.0 && !o->settling && !o->entrainment) return;
     for (int l = 0; l < o->ncells; l++) {
         const size_t c = CELL(o, o->jc[l], o->kc[l]);
         double *qq = o->q + nv * c, *qpc = o->qp + np * c, pd;
@@ -1538,11 +1549,17 @@ static void update_erosion_deposition_cell(oracle_t *o, double dt)
             for (int i = 0; i < np; i++) qpc[i] = 0.0;
             qpc[3] = o->T_amb;
         }
-        double dep[3], eq[8];
-        if (eval_mass_exchange_terms(o, qpc, dt, dep, eq)) o->err |= 16;
+        double dep[3], eq[8], ers[3], erod[3];
+        for (int i = 0; i < ns; i++) erod[i] = o->erodible[c + N * i];
+        if (eval_mass_exchange_terms(o, qpc, erod, dt, dep, eq, ers)) o->err |= 16;
         for (int i = 0; i < nv; i++) qq[i] = qq[i] + dt * eq[i];
         for (int i = 0; i < ns; i++) qq[4 + i] = fmax_f(0.0, qq[4 + i]);
         for (int i = 0; i < ns; i++) o->deposit[c + N * i] = o->deposit[c + N * i] + dt * dep[i];
+        for (int i = 0; i < ns; i++) o->erosion[c + N * i] = o->erosion[c + N * i] + dt * ers[i];
+        for (int i = 0; i < ns; i++) o->erodible[c + N * i] = o->erodible[c + N * i] - dt * ers[i];
+        if (o->erodible_deposit)
+            for (int i = 0; i < ns; i++)
+                o->erodible[c + N * i] = o->erodible[c + N * i] + dt * dep[i];
         for (int i = 0; i < ns; i++)
             if (qq[4 + i] < 0.0) o->err |= 4;
         if (qq[0] <= 0.0) {
@@ -1663,7 +1680,19 @@ oracle_t *oracle_create(const int *ip, const double *rp)
         o->inv_rho_s[i] = 1.0 / o->rho_s[i];
     }
     for (int i = 0; i < 8; i++) { o->thk[i] = rp[46 + i]; o->pdyn[i] = rp[54 + i]; }
-    o->T_erodible = 300.0;
+    o->erodible_deposit = ip[12];
+    o->erosion_coeff = rp[63];
+    if (o->erosion_coeff == 0.0) { /* src/inpout_2d.f90:1827-1832 */
+        o->erodible_porosity = 0.0;
+        o->coeff_porosity = 0.0;
+        for (int i = 0; i < o->ns; i++) o->erodible_fract[i] = 1.0 / o->ns;
+        o->T_erodible = 300.0;
+    } else {               /* :1835-1890, the fractions as read_param leaves them */
+        o->erodible_porosity = rp[64];
+        o->coeff_porosity = o->erodible_porosity / (1.0 - o->erodible_porosity);
+        for (int i = 0; i < o->ns; i++) o->erodible_fract[i] = rp[66 + i];
+        o->T_erodible = rp[65];
+    }
     o->rho_c_sub = o->pres / (o->R_a * o->T_erodible);
     o->pi_g = 4.0 * atan(1.0);
     o->hcs = o->nv * EPS;
@@ -1700,6 +1729,8 @@ oracle_t *oracle_create(const int *ip, const double *rp)
     o->jsy = zalloc(sizeof(int) * NFY); o->ksy = zalloc(sizeof(int) * NFY);
     o->hmax = zalloc(8 * N); o->pdynmax = zalloc(8 * N); o->modvelmax = zalloc(8 * N);
     o->deposit = zalloc(8 * N * o->ns);
+    o->erodible = zalloc(8 * N * o->ns);
+    o->erosion = zalloc(8 * N * o->ns);
     o->vuln = zalloc(N * (size_t)(o->n_thk * o->n_pdyn > 0 ? o->n_thk * o->n_pdyn : 1));
     o->stL = zalloc(sizeof(int) * NFX); o->stR = zalloc(sizeof(int) * NFX);
     o->stB = zalloc(sizeof(int) * NFY); o->stT = zalloc(sizeof(int) * NFY);
@@ -1722,7 +1753,8 @@ void oracle_destroy(oracle_t *o)
                     o->qT, o->qpL, o->qpR, o->qpB, o->qpT, o->axn, o->axp, o->ayn, o->ayp,
                     o->axmax, o->aymax, o->Hx, o->Hy, o->mask, o->mtmp, o->mx, o->my, o->jc,
                     o->kc, o->jsx, o->ksx, o->jsy, o->ksy, o->hmax, o->pdynmax, o->modvelmax,
-                    o->deposit, o->vuln, o->stL, o->stR, o->stB, o->stT, o->stQP};
+                    o->deposit, o->vuln, o->stL, o->stR, o->stB, o->stT, o->stQP,
+                    o->erodible, o->erosion};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
     free(o);
 }
@@ -1893,8 +1925,30 @@ void oracle_settling(const oracle_t *o, int n, const double *diam, const double 
 void oracle_mass_exchange(const oracle_t *o, int n, const double *qp, double dt, double *dep,
                           double *eq)
 {
+    const double zero[3] = {0.0, 0.0, 0.0};
+    double ers[3];
     for (int i = 0; i < n; i++)
-        eval_mass_exchange_terms(o, qp + (size_t)(o->nv + 2) * i, dt, dep + (size_t)o->ns * i, eq + (size_t)o->nv * i);
+        eval_mass_exchange_terms(o, qp + (size_t)(o->nv + 2) * i, zero, dt, dep + (size_t)o->ns * i,
+                                 eq + (size_t)o->nv * i, ers);
+}
+/* with the cells' erodible layers erod[n][n_solid]; ers[n][n_solid] the erosion rates */
+void oracle_mass_exchange_ers(const oracle_t *o, int n, const double *qp, const double *erod,
+                              double dt, double *dep, double *eq, double *ers)
+{
+    for (int i = 0; i < n; i++)
+        eval_mass_exchange_terms(o, qp + (size_t)(o->nv + 2) * i, erod + (size_t)o->ns * i, dt,
+                                 dep + (size_t)o->ns * i, eq + (size_t)o->nv * i,
+                                 ers + (size_t)o->ns * i);
+}
+/* erodible[class][cell] (cells x fastest) */
+void oracle_set_erodible(oracle_t *o, const double *e)
+{
+    memcpy(o->erodible, e, 8 * (size_t)o->nx * o->ny * o->ns);
+}
+void oracle_get_erosion(const oracle_t *o, double *ers, double *erd)
+{
+    memcpy(ers, o->erosion, 8 * (size_t)o->nx * o->ny * o->ns);
+    memcpy(erd, o->erodible, 8 * (size_t)o->nx * o->ny * o->ns);
 }
 double oracle_limit_minmod(const double *v3, const double *z3) { return limit_minmod(v3, z3); }
 

@@ -46,6 +46,10 @@ double oracle_pow_cr(double x, double y);
 /* driver time and dt history (restart from a saved state) */
 void oracle_set_time(oracle_t *o, double t, double dt_old, double dt_old_old);
 void oracle_set_t_end(oracle_t *o, double t_end);
+void oracle_mass_exchange_ers(const oracle_t *o, int n, const double *qp, const double *erod,
+                              double dt, double *dep, double *eq, double *ers);
+void oracle_set_erodible(oracle_t *o, const double *e);
+void oracle_get_erosion(const oracle_t *o, double *ers, double *erd);
 double oracle_get_t(const oracle_t *o);
 /* number of cells that went from wet (q1 > 0) to exactly dry in a step, and
  * reads of a face or stage state left from an earlier step that differ from

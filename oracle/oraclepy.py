@@ -52,6 +52,8 @@ def _lib():
         lib.oracle_pow_cr.restype = ctypes.c_double
         lib.oracle_pow_cr.argtypes = [ctypes.c_double, ctypes.c_double]
         lib.oracle_set_pow_mode.restype = None
+        for name in ("oracle_mass_exchange_ers", "oracle_set_erodible", "oracle_get_erosion"):
+            getattr(lib, name).restype = None
         lib.oracle_set_t_end.restype = None
         lib.oracle_set_t_end.argtypes = [ctypes.c_void_p, ctypes.c_double]
         lib.oracle_get_t.restype = ctypes.c_double
@@ -92,6 +94,8 @@ class Oracle:
         self.nv = cfg.n_vars
         if static is not None:
             self.set_static(static)
+            if static.get("erodible") is not None:
+                self.set_erodible(static["erodible"])
         if q0 is not None:
             self.init_state(q0)
 
@@ -172,6 +176,33 @@ class Oracle:
     def set_time(self, t, dt_old, dt_old_old):
         self.lib.oracle_set_time(self.h, ctypes.c_double(t), ctypes.c_double(dt_old),
                                  ctypes.c_double(dt_old_old))
+
+    def mass_exchange_ers(self, qp, erod, dt):
+        """(deposition, eqns_term, erosion) rates with the cells' erodible
+        layers erod [n, n_solid]."""
+        qp = np.ascontiguousarray(qp, np.float64)
+        erod = np.ascontiguousarray(erod, np.float64)
+        n = qp.shape[0]
+        dep = np.zeros((n, self.cfg.n_solid))
+        eq = np.zeros((n, self.nv))
+        ers = np.zeros((n, self.cfg.n_solid))
+        self.lib.oracle_mass_exchange_ers(self.h, ctypes.c_int(n), _dp(qp), _dp(erod),
+                                          ctypes.c_double(dt), _dp(dep), _dp(eq), _dp(ers))
+        return dep, eq, ers
+
+    def set_erodible(self, e):
+        """erodible as [class, k, j]."""
+        a = np.ascontiguousarray(e, np.float64)
+        assert a.shape == (self.cfg.n_solid, self.cfg.ny, self.cfg.nx)
+        self.lib.oracle_set_erodible(self.h, _dp(a))
+
+    def erosion(self):
+        """(erosion, erodible) as [class, k, j]."""
+        c = self.cfg
+        ers = np.zeros((c.n_solid, c.ny, c.nx))
+        erd = np.zeros((c.n_solid, c.ny, c.nx))
+        self.lib.oracle_get_erosion(self.h, _dp(ers), _dp(erd))
+        return ers, erd
 
     def set_t_end(self, t_end):
         self.lib.oracle_set_t_end(self.h, ctypes.c_double(t_end))

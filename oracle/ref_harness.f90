@@ -77,8 +77,8 @@ CONTAINS
   !   (x index fastest).
   !----------------------------------------------------------------------------
   INTEGER(c_int) FUNCTION imexref_setup(ip, rp, bdem) BIND(C, name='imexref_setup')
-    INTEGER(c_int), INTENT(IN) :: ip(0:11)
-    REAL(c_double), INTENT(IN) :: rp(0:62)
+    INTEGER(c_int), INTENT(IN) :: ip(0:12)
+    REAL(c_double), INTENT(IN) :: rp(0:68)
     REAL(c_double), INTENT(IN) :: bdem(*)
 
     INTEGER :: j, k, i, nx, ny
@@ -118,7 +118,7 @@ CONTAINS
     topo_change_flag = .FALSE.
     lateral_source_flag = .FALSE.
     bottom_radial_source_flag = .FALSE.
-    erodible_deposit_flag = .FALSE.
+    erodible_deposit_flag = ( ip(12) .NE. 0 )
     liquid_vaporization_flag = .FALSE.
     radial_source_flag = ( ip(6) .NE. 0 )
     collapsing_volume_flag = ( ip(7) .NE. 0 )
@@ -223,14 +223,26 @@ CONTAINS
     collective_settling_flag = .FALSE.
     IF ( .NOT. ALLOCATED(erosion_coeff) ) ALLOCATE( erosion_coeff )
     IF ( .NOT. ALLOCATED(loss_rate) ) ALLOCATE( loss_rate )
-    erosion_coeff = 0.0_wp
+    erosion_coeff = rp(63)
     loss_rate = 0.0_wp
-    ! erosion_coeff = 0 branch of src/inpout_2d.f90:1827-1832 and :1896
-    erodible_porosity = 0.0_wp
-    erodible_fract(1:n_solid) = 1.0_wp / n_solid
-    T_erodible = 300.0_wp
+    IF ( erosion_coeff .EQ. 0.0_wp ) THEN
+       ! src/inpout_2d.f90:1827-1832
+       erodible_porosity = 0.0_wp
+       erodible_fract(1:n_solid) = 1.0_wp / n_solid
+       T_erodible = 300.0_wp
+       coeff_porosity = 0.0_wp
+    ELSE
+       ! src/inpout_2d.f90:1835-1890 (fractions given summing to one)
+       erodible_porosity = rp(64)
+       coeff_porosity = erodible_porosity / ( 1.0_wp - erodible_porosity )
+       IF ( n_solid .EQ. 1 ) THEN
+          erodible_fract(1) = 1.0_wp
+       ELSE
+          erodible_fract(1:n_solid) = rp(66:65+n_solid)
+       END IF
+       T_erodible = rp(65)
+    END IF
     rho_c_sub = pres / ( sp_gas_const_a * T_erodible )
-    coeff_porosity = 0.0_wp
     alphastot_min = 0.0_wp
     maximum_solid_packing = 0.64_wp
 
@@ -503,6 +515,20 @@ CONTAINS
     REAL(c_double), INTENT(OUT) :: qout(n_vars, comp_cells_x, comp_cells_y)
     qout = q
   END SUBROUTINE imexref_get_q
+
+  ! The erodible layer erodible(n_solid, nx, ny) (read_erodible's result).
+  SUBROUTINE imexref_set_erodible(e) BIND(C, name='imexref_set_erodible')
+    REAL(c_double), INTENT(IN) :: e(n_solid, comp_cells_x, comp_cells_y)
+    erodible = e
+  END SUBROUTINE imexref_set_erodible
+
+  ! erosion(nx, ny, n_solid) and erodible(n_solid, nx, ny)
+  SUBROUTINE imexref_get_erosion(ers, erd) BIND(C, name='imexref_get_erosion')
+    REAL(c_double), INTENT(OUT) :: ers(comp_cells_x, comp_cells_y, n_solid)
+    REAL(c_double), INTENT(OUT) :: erd(n_solid, comp_cells_x, comp_cells_y)
+    ers = erosion
+    erd = erodible
+  END SUBROUTINE imexref_get_erosion
 
   ! The driver's module array qp (primitive variables kept by the epilogue).
   SUBROUTINE imexref_get_qp(qpo) BIND(C, name='imexref_get_qp')

@@ -50,8 +50,9 @@ def _ip(a):
 
 
 def pack_params(cfg):
-    """ip[12] / rp[63] parameter vectors read by imexref_setup."""
-    ip = np.zeros(12, dtype=np.int32)
+    """ip[13] / rp[69] parameter vectors read by imexref_setup (and by the
+    C restatement's oracle_create)."""
+    ip = np.zeros(13, dtype=np.int32)
     ip[0] = cfg.nx
     ip[1] = cfg.ny
     ip[2] = cfg.n_solid
@@ -64,7 +65,8 @@ def pack_params(cfg):
     ip[9] = len(cfg.dyn_pres_levels)
     ip[10] = cfg.limiter
     ip[11] = cfg.bc_flag
-    rp = np.zeros(63, dtype=np.float64)
+    ip[12] = int(getattr(cfg, "erodible_deposit_flag", False))
+    rp = np.zeros(69, dtype=np.float64)
     rp[0:16] = [cfg.dx, cfg.x0, cfg.y0, cfg.cfl, cfg.dt0, cfg.max_dt, cfg.t_start,
                 cfg.t_end, cfg.grav, cfg.mu, cfg.xi, cfg.pres, cfg.T_ambient,
                 cfg.sp_heat_a, cfg.sp_gas_const_a, cfg.kin_visc_a]
@@ -84,6 +86,11 @@ def pack_params(cfg):
     for i, v in enumerate(cfg.dyn_pres_levels):
         rp[54 + i] = v
     rp[62] = cfg.mfr_source
+    rp[63] = getattr(cfg, "erosion_coeff", 0.0)
+    rp[64] = getattr(cfg, "erodible_porosity", 0.0)
+    rp[65] = getattr(cfg, "T_erodible", 300.0)
+    for i, v in enumerate(cfg.erodible_fractions()):
+        rp[66 + i] = v
     return ip, rp
 
 
@@ -137,6 +144,20 @@ class RefRun:
         a = np.ascontiguousarray(q, dtype=np.float64)
         assert a.shape == (self.cfg.ny, self.cfg.nx, self.nv)
         self.lib.imexref_set_q(_dp(a))
+
+    def set_erodible(self, e):
+        """erodible as [class, k, j]."""
+        a = np.ascontiguousarray(np.transpose(np.asarray(e, dtype=np.float64), (1, 2, 0)))
+        assert a.shape == (self.cfg.ny, self.cfg.nx, self.cfg.n_solid)
+        self.lib.imexref_set_erodible(_dp(a))
+
+    def erosion(self):
+        """(erosion [class, k, j], erodible [class, k, j])."""
+        c = self.cfg
+        ers = np.zeros((c.n_solid, c.ny, c.nx))
+        erd = np.zeros((c.ny, c.nx, c.n_solid))
+        self.lib.imexref_get_erosion(_dp(ers), _dp(erd))
+        return ers, np.transpose(erd, (2, 0, 1)).copy()
 
     def qp(self):
         """The driver's module array qp as [k, j, var] (n_vars + 2 variables)."""

@@ -33,4 +33,26 @@ def c2_96():
     return workloads.small_config2(nx=96, ny=96, dx=20.0)
 
 
-CASES = {"c1_48": c1_48, "c1_64": c1_64, "c2_64": c2_64, "c2_96": c2_96}
+def _erosion(cfg, fract):
+    cfg.erosion_coeff = 2.0e-4
+    cfg.erodible_porosity = 0.3
+    cfg.erodible_fract = list(fract)
+    cfg.T_erodible = 350.0
+    cfg.erodible_thickness = 0.4
+    return cfg
+
+
+def c1e_48():
+    """c1_48 over an erodible layer of 0.4 m (porosity 0.3, 350 K)."""
+    cfg, B = c1_48()
+    return _erosion(cfg, [1.0]), B
+
+
+def c2e_64():
+    """c2_64 over an erodible layer of 0.4 m, fractions 0.7 / 0.3."""
+    cfg, B = c2_64()
+    return _erosion(cfg, [0.7, 0.3]), B
+
+
+CASES = {"c1_48": c1_48, "c1_64": c1_64, "c2_64": c2_64, "c2_96": c2_96,
+         "c1e_48": c1e_48, "c2e_64": c2e_64}

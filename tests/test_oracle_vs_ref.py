@@ -74,3 +74,30 @@ def test_multistep_bit_exact(ref_lib, name, steps):
     np.testing.assert_array_equal(o.mask(), run.mask())
     assert o.counters()[1] == 0          # no stale face or stage state was read
     run.close()
+
+
+@pytest.mark.parametrize("name, steps", [("c1e_48", 300), ("c2e_64", 300)])
+def test_erosion_multistep_bit_exact(ref_lib, name, steps):
+    """The erosion branch (src/constitutive_2d.f90:4731-4758, the erodible and
+    erosion updates of src/solver_2d.f90:2478-2491) against the reference."""
+    from imex_sflow2d_dev_amd import init
+    cfg, B = cases.CASES[name]()
+    run = ref_lib.RefRun(cfg, B)
+    st = run.static()
+    q0 = run.q()
+    er0 = init.setup(cfg, B)[0]["erodible"]
+    run.set_erodible(er0)
+    st["erodible"] = er0
+    o = oraclepy.Oracle(cfg, st, q0)
+    for block in range(steps // 100):
+        d1, n1 = run.run(100)
+        d2, n2, err = o.run(100)
+        np.testing.assert_array_equal(d2, d1)
+        np.testing.assert_array_equal(o.q(), run.q())
+    ers_r, erd_r = run.erosion()
+    ers_o, erd_o = o.erosion()
+    np.testing.assert_array_equal(ers_o, ers_r)
+    np.testing.assert_array_equal(erd_o, erd_r)
+    np.testing.assert_array_equal(o.fields()["deposit"], run.fields()["deposit"])
+    assert ers_r.max() > 1e-4 and (erd_r < er0).any()
+    run.close()
