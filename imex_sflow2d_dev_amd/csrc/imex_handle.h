@@ -41,8 +41,8 @@ struct imex_handle {
   hipStream_t copy_stream = nullptr;
   hipEvent_t ev_staged = nullptr, ev_copied = nullptr;
   bool copy_inflight = false;
-  double* stage[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
-  size_t stage_cap[5] = {0, 0, 0, 0, 0};
+  double* stage[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  size_t stage_cap[7] = {0, 0, 0, 0, 0, 0, 0};
   unsigned int* d_flag = nullptr;
   std::string err;
 };

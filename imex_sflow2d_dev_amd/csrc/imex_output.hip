@@ -302,10 +302,12 @@ __global__ void k_probes(Params P, Grid G, int n, const double* xy, double* out)
 // |q| < 1e-20 in the state (:6043-6054) and writes x, y, q; the _deposit_
 // file gets the deposit as it is; the .p_2d pass (:6169-6402) computes
 // qc_to_qp and mixt_var of the (zeroed) state, zeroes |deposit| < 1e-20 in
-// place and writes its columns.  Configured path: no added gases, liquid,
-// stochastic or pore-pressure variables, no erosion (erosion 0, erodible 0
-// with porosity 0), Voellmy rheology (model 1): shear velocity, Rouse
-// numbers and inertial number 0.  mu_eff reads pore_pres(1) of a zero-size
+// place and writes its columns; with erosion the _erosion_ and _erodible_
+// files get x, y and the layers (:6060-6069) and the .p_2d pass zeroes
+// |erosion| < 1e-20 in place too.  Configured path: no added gases, liquid,
+// stochastic or pore-pressure variables, Voellmy rheology (model 1): shear
+// velocity, Rouse numbers and inertial number 0; without erosion the erosion
+// and erodible columns are 0.  mu_eff reads pore_pres(1) of a zero-size
 // array in the reference; here the excess pore pressure is 0, so mu_eff =
 // mu * MAX(0, 1 - 0 / (rho_m h g')) on wet cells.
 // ---------------------------------------------------------------------------
@@ -314,7 +316,7 @@ __host__ __device__ constexpr int p2d_cols(int ns) { return 19 + 4 * ns; }
 template <int NS, typename S>
 __global__ void k_snapshot(Params P, Grid G, const double* bcent, unsigned what, double* sq,
                            double* sq2d, double* sdep2d, double* sp2d, double* sfld,
-                           unsigned int* flag) {
+                           double* sero2d, double* serd2d, unsigned int* flag) {
   constexpr int NV = 4 + NS;
   const int N = G.N;
   for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < N; c += gridDim.x * blockDim.x) {
@@ -355,6 +357,27 @@ __global__ void k_snapshot(Params P, Grid G, const double* bcent, unsigned what,
 #pragma unroll
       for (int i = 0; i < NS; i++) r[2 + i] = dep[i];
     }
+    const bool er = G.erosion != nullptr;
+    double ero[NS], erd[NS];
+#pragma unroll
+    for (int i = 0; i < NS; i++) {
+      ero[i] = er ? G.erosion[(size_t)i * N + c] : 0.0;
+      erd[i] = er ? G.erodible[(size_t)i * N + c] : 0.0;
+    }
+    if (what & IMEX_SNAP_ERO2D) {
+      double* r = sero2d + (size_t)c * (2 + NS);
+      r[0] = X;
+      r[1] = Y;
+#pragma unroll
+      for (int i = 0; i < NS; i++) r[2 + i] = ero[i];
+    }
+    if (what & IMEX_SNAP_ERD2D) {
+      double* r = serd2d + (size_t)c * (2 + NS);
+      r[0] = X;
+      r[1] = Y;
+#pragma unroll
+      for (int i = 0; i < NS; i++) r[2 + i] = erd[i];
+    }
     if (what & IMEX_SNAP_P2D) {
       double qp[NV + 2], pd;
       qc_to_qp<NS>(P, q, qp, pd);
@@ -387,7 +410,14 @@ __global__ void k_snapshot(Params P, Grid G, const double* bcent, unsigned what,
           if (dep[i] != 0.0 || signbit(dep[i])) G.dep[(size_t)i * N + c] = 0.0;
           dep[i] = 0.0;
         }
+        if (fabs(ero[i]) < 1.0e-20) {
+          if (er && (ero[i] != 0.0 || signbit(ero[i]))) G.erosion[(size_t)i * N + c] = 0.0;
+          ero[i] = 0.0;
+        }
       }
+      double erd_sum = 0.0;
+#pragma unroll
+      for (int i = 0; i < NS; i++) erd_sum += erd[i];
       tiny(T);
       tiny(rho_m);
       tiny(rg);
@@ -409,8 +439,8 @@ __global__ void k_snapshot(Params P, Grid G, const double* bcent, unsigned what,
 #pragma unroll
       for (int i = 0; i < NS; i++) r[o++] = dep[i];
 #pragma unroll
-      for (int i = 0; i < NS; i++) r[o++] = 0.0;  // erosion
-      r[o++] = 0.0 / (1.0 - 0.0);                  // SUM(erodible) / (1 - porosity)
+      for (int i = 0; i < NS; i++) r[o++] = ero[i];
+      r[o++] = erd_sum / (1.0 - P.erodible_porosity);
       r[o++] = 0.0;                                // alpha_l
       r[o++] = 0.0;                                // shear velocity
       r[o++] = m.Ri;
@@ -432,6 +462,11 @@ __global__ void k_snapshot(Params P, Grid G, const double* bcent, unsigned what,
       sfld[(size_t)NS * N + c] = G.hmax[c];
       sfld[(size_t)(NS + 1) * N + c] = G.pdyn[c];
       sfld[(size_t)(NS + 2) * N + c] = G.mvel[c];
+#pragma unroll
+      for (int i = 0; i < NS; i++) {
+        sfld[(size_t)(NS + 3 + i) * N + c] = ero[i];
+        sfld[(size_t)(2 * NS + 3 + i) * N + c] = erd[i];
+      }
     }
   }
 }
@@ -446,22 +481,25 @@ static int snap_cols(const imex_handle* h, int kind) {
     case 1: return 2 + nv;
     case 2: return 2 + ns;
     case 3: return p2d_cols(ns);
-    default: return ns + 3;
+    case 4: return 3 * ns + 3;
+    default: return 2 + ns;
   }
 }
+constexpr int NKIND = 7;
 
 extern "C" int imex_snapshot_cols(imex_handle* h, unsigned kind) {
   if (!h) return -2;
-  for (int b = 0; b < 5; b++)
+  for (int b = 0; b < NKIND; b++)
     if (kind == (1u << b)) return snap_cols(h, b);
   return -2;
 }
 
 extern "C" int imex_snapshot_begin(imex_handle* h, unsigned what, double* q, double* q2d,
-                                   double* dep2d, double* p2d, double* fields) {
-  if (!h || (what & ~31u)) return -2;
-  double* outs[5] = {q, q2d, dep2d, p2d, fields};
-  for (int b = 0; b < 5; b++)
+                                   double* dep2d, double* p2d, double* fields, double* ero2d,
+                                   double* erd2d) {
+  if (!h || (what & ~127u)) return -2;
+  double* outs[NKIND] = {q, q2d, dep2d, p2d, fields, ero2d, erd2d};
+  for (int b = 0; b < NKIND; b++)
     if ((what & (1u << b)) && !outs[b]) return -2;
   HCHK(h, hipSetDevice(h->device));
   if (!h->copy_stream) {
@@ -471,7 +509,7 @@ extern "C" int imex_snapshot_begin(imex_handle* h, unsigned what, double* q, dou
     if (int rc = dalloc(h, &h->d_flag, 1)) return rc;
   }
   const size_t N = (size_t)h->G.N;
-  for (int b = 0; b < 5; b++) {
+  for (int b = 0; b < NKIND; b++) {
     if (!(what & (1u << b))) continue;
     const size_t need = N * snap_cols(h, b);
     if (h->stage_cap[b] < need) {
@@ -489,7 +527,8 @@ extern "C" int imex_snapshot_begin(imex_handle* h, unsigned what, double* q, dou
 #define SNAP_LAUNCH(NS)                                                                       \
   hipLaunchKernelGGL((h->fp32 ? k_snapshot<NS, float> : k_snapshot<NS, double>), dim3(nb),   \
                      dim3(256), 0, h->stream, P, G, h->bcent, what, h->stage[0], h->stage[1], \
-                     h->stage[2], h->stage[3], h->stage[4], h->d_flag)
+                     h->stage[2], h->stage[3], h->stage[4], h->stage[5], h->stage[6],        \
+                     h->d_flag)
   switch (P.ns) {
     case 1: SNAP_LAUNCH(1); break;
     case 2: SNAP_LAUNCH(2); break;
@@ -505,7 +544,7 @@ extern "C" int imex_snapshot_begin(imex_handle* h, unsigned what, double* q, dou
   }
   HCHK(h, hipEventRecord(h->ev_staged, h->stream));
   HCHK(h, hipStreamWaitEvent(h->copy_stream, h->ev_staged, 0));
-  for (int b = 0; b < 5; b++)
+  for (int b = 0; b < NKIND; b++)
     if (what & (1u << b))
       HCHK(h, hipMemcpyAsync(outs[b], h->stage[b], 8 * N * snap_cols(h, b), hipMemcpyDeviceToHost,
                              h->copy_stream));

@@ -47,7 +47,8 @@ EXPORTS = [
 ]
 
 RUNOUT_STORE, RUNOUT_B = 1, 2
-SNAP_Q, SNAP_Q2D, SNAP_DEP2D, SNAP_P2D, SNAP_FIELDS = 1, 2, 4, 8, 16
+SNAP_Q, SNAP_Q2D, SNAP_DEP2D, SNAP_P2D, SNAP_FIELDS, SNAP_ERO2D, SNAP_ERD2D = 1, 2, 4, 8, 16, 32, 64
+SNAP_KINDS = (SNAP_Q, SNAP_Q2D, SNAP_DEP2D, SNAP_P2D, SNAP_FIELDS, SNAP_ERO2D, SNAP_ERD2D)
 FMT_E20_12, FMT_ES15_7, FMT_ES14_7 = 0, 1, 2
 
 

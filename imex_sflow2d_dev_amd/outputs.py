@@ -221,13 +221,14 @@ class SolutionOutput:
         self.what = 0
         if cons:
             self.what |= native.SNAP_Q2D | (native.SNAP_DEP2D if c.settling_flag else 0)
+            if c.erosion_coeff > 0.0:
+                self.what |= native.SNAP_ERO2D | native.SNAP_ERD2D
         if phys:
             self.what |= native.SNAP_P2D
         if restart:
             self.what |= native.SNAP_Q | native.SNAP_FIELDS
         self.bufs = {}
-        for kind in (native.SNAP_Q, native.SNAP_Q2D, native.SNAP_DEP2D, native.SNAP_P2D,
-                     native.SNAP_FIELDS):
+        for kind in native.SNAP_KINDS:
             if self.what & kind:
                 self.bufs[kind] = native.PinnedArray(N * solver.snapshot_cols(kind))
         self._thread = None
@@ -245,22 +246,25 @@ class SolutionOutput:
                 rec = self.bufs[nat.SNAP_Q2D].a.reshape(N, -1)
                 nat.write_records(os.path.join(self.dir, f"{self.run_name}_{idx}.q_2d"), rec, nx,
                                   nat.FMT_E20_12, self.n_threads)
-                if c.settling_flag:
-                    rec = self.bufs[nat.SNAP_DEP2D].a.reshape(N, -1)
-                    nat.write_records(os.path.join(self.dir, f"{self.run_name}_deposit_{idx}.q_2d"),
-                                      rec, nx, nat.FMT_E20_12, self.n_threads)
+                for kind, tag, on in ((nat.SNAP_ERO2D, "erosion", c.erosion_coeff > 0.0),
+                                      (nat.SNAP_ERD2D, "erodible", c.erosion_coeff > 0.0),
+                                      (nat.SNAP_DEP2D, "deposit", c.settling_flag)):
+                    if on:
+                        rec = self.bufs[kind].a.reshape(N, -1)
+                        nat.write_records(os.path.join(self.dir, f"{self.run_name}_{tag}_{idx}.q_2d"),
+                                          rec, nx, nat.FMT_E20_12, self.n_threads)
             if self.phys:
                 rec = self.bufs[nat.SNAP_P2D].a.reshape(N, -1)
                 nat.write_records(os.path.join(self.dir, f"{self.run_name}_{idx}.p_2d"), rec, nx,
                                   nat.FMT_ES15_7, self.n_threads)
             if with_restart and self.restart:
                 ns = c.n_solid
-                fld = self.bufs[nat.SNAP_FIELDS].a.reshape(ns + 3, N)
-                zeros = np.zeros(N * ns)
+                fld = self.bufs[nat.SNAP_FIELDS].a.reshape(3 * ns + 3, N)
                 B = self.B if self.B is not None else np.zeros(N)
+                erodible = np.ascontiguousarray(fld[2 * ns + 3:].T)   # erodible(n_solid, nx, ny)
                 nat.write_restart(os.path.join(self.dir, "restart.bin"), c.nx, c.ny, c.n_vars, ns,
-                                  time, dt, self.bufs[nat.SNAP_Q].a, B, zeros, fld[:ns], zeros,
-                                  fld[ns], fld[ns + 1], fld[ns + 2])
+                                  time, dt, self.bufs[nat.SNAP_Q].a, B, erodible, fld[:ns],
+                                  fld[ns + 3:2 * ns + 3], fld[ns], fld[ns + 1], fld[ns + 2])
         except Exception as e:  # reported by the next call or by finish()
             self._error = e
         self.timings[-1][1] = _t.perf_counter() - t0

@@ -292,8 +292,7 @@ class Solver:
         page-locked for the copy to overlap the next steps)."""
         N = self.cfg.nx * self.cfg.ny
         ptrs = []
-        for kind in (native.SNAP_Q, native.SNAP_Q2D, native.SNAP_DEP2D, native.SNAP_P2D,
-                     native.SNAP_FIELDS):
+        for kind in native.SNAP_KINDS:
             a = bufs.get(kind) if what & kind else None
             if a is not None:
                 assert a.dtype == np.float64 and a.size >= N * self.snapshot_cols(kind)

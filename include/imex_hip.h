@@ -234,10 +234,13 @@ int imex_probes(imex_handle *h, int n, const double *xy, double *out);
                                erosion, erodible, alpha_l, u*, Ri, Rouse(n_solid), mu_eff,
                                hmax, pdynmax, mod_vel_max, I (:6390-6396): 19 + 4 n_solid
                                values; B from imex_set_topography (0 without it) */
-#define IMEX_SNAP_FIELDS 16u /* deposit(nx,ny,n_solid), hmax, pdynmax, mod_vel_max */
+#define IMEX_SNAP_FIELDS 16u /* deposit(nx,ny,n_solid), hmax, pdynmax, mod_vel_max,
+                               erosion(nx,ny,n_solid), erodible as [class][cell] planes */
+#define IMEX_SNAP_ERO2D 32u /* x, y, erosion(j,k,1:n_solid) (:6060-6063), erosion runs */
+#define IMEX_SNAP_ERD2D 64u /* x, y, erodible(1:n_solid,j,k) (:6065-6066), erosion runs */
 int imex_snapshot_cols(imex_handle *h, unsigned kind);
 int imex_snapshot_begin(imex_handle *h, unsigned what, double *q, double *q2d, double *dep2d,
-                        double *p2d, double *fields);
+                        double *p2d, double *fields, double *ero2d, double *erd2d);
 int imex_snapshot_wait(imex_handle *h);
 /* page-locked host memory for snapshot buffers */
 int imex_host_alloc(size_t bytes, void **p);

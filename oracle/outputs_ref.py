@@ -271,14 +271,15 @@ def zero_tiny(a):
     return a
 
 
-def p2d_records(cfg, q, deposit, B, hmax, pdynmax, mod_vel_max, oracle):
+def p2d_records(cfg, q, deposit, B, hmax, pdynmax, mod_vel_max, oracle, erosion=None,
+                erodible=None):
     """output_solution's .p_2d columns per cell (src/inpout_2d.f90:6169-6396)
     for the configured path (Voellmy rheology, model 1; no added gas, liquid,
-    stochastic or pore-pressure variables; no erosion, erodible 0 with
-    porosity 0).  The reference's mu_eff reads pore_pres(1) of a zero-size
-    array; here the excess pore pressure is 0.  q is [k, j, var], deposit
-    [class, k, j]; returns ([N, 19 + 4 n_solid] records, the zeroed
-    deposit)."""
+    stochastic or pore-pressure variables; without erosion the erosion and
+    erodible columns are 0).  The reference's mu_eff reads pore_pres(1) of
+    a zero-size array; here the excess pore pressure is 0.  q is [k, j, var],
+    deposit, erosion and erodible [class, k, j]; returns ([N, 19 + 4 n_solid]
+    records, the zeroed deposit, the zeroed erosion)."""
     ny, nx, nv = q.shape
     ns = cfg.n_solid
     N = nx * ny
@@ -288,6 +289,9 @@ def p2d_records(cfg, q, deposit, B, hmax, pdynmax, mod_vel_max, oracle):
     x_comp = 0.5 * ((cfg.x0 + np.arange(nx) * cfg.dx) + (cfg.x0 + np.arange(1, nx + 1) * cfg.dx))
     y_comp = 0.5 * ((cfg.y0 + np.arange(ny) * cfg.dx) + (cfg.y0 + np.arange(1, ny + 1) * cfg.dx))
     dep = deposit.reshape(ns, N).copy()
+    ero = np.zeros((ns, N)) if erosion is None else erosion.reshape(ns, N).copy()
+    erd = np.zeros((ns, N)) if erodible is None else erodible.reshape(ns, N)
+    por = cfg.effective_erodible_porosity()
     out = np.zeros((N, 19 + 4 * ns))
     z = lambda x: 0.0 if abs(x) < 1.0e-20 else x  # noqa: E731
     Bf = np.asarray(B, dtype=np.float64).reshape(N)
@@ -309,9 +313,14 @@ def p2d_records(cfg, q, deposit, B, hmax, pdynmax, mod_vel_max, oracle):
         al = [z(a) for a in al]
         for i in range(ns):
             dep[i, c] = z(dep[i, c])
+            ero[i, c] = z(ero[i, c])
         T, rho_m, rg, mu_eff = z(T), z(rho_m), z(rg), z(mu_eff)
+        esum = 0.0
+        for i in range(ns):
+            esum = esum + erd[i, c]
         row = [x_comp[j], y_comp[k], h, u, v, Bo, h + Bo] + al + [T, rho_m, rg] \
-            + [dep[i, c] for i in range(ns)] + [0.0] * ns + [0.0, 0.0, 0.0, Ri] + [0.0] * ns \
+            + [dep[i, c] for i in range(ns)] + [ero[i, c] for i in range(ns)] \
+            + [esum / (1.0 - por), 0.0, 0.0, Ri] + [0.0] * ns \
             + [mu_eff, hmax.reshape(N)[c], pdynmax.reshape(N)[c], mod_vel_max.reshape(N)[c], 0.0]
         out[c] = row
-    return out, dep.reshape(deposit.shape)
+    return out, dep.reshape(deposit.shape), ero.reshape((ns, ny, nx))

@@ -14,17 +14,16 @@ from oracle import oraclepy, outputs_ref
 
 pytestmark = pytest.mark.gpu
 
-ALL = native.SNAP_Q | native.SNAP_Q2D | native.SNAP_DEP2D | native.SNAP_P2D | native.SNAP_FIELDS
+ALL = native.SNAP_Q | native.SNAP_Q2D | native.SNAP_DEP2D | native.SNAP_P2D | native.SNAP_FIELDS \
+    | native.SNAP_ERO2D | native.SNAP_ERD2D
 
 
 def _bufs(s, what):
     N = s.cfg.nx * s.cfg.ny
-    return {k: native.PinnedArray(N * s.snapshot_cols(k)) for k in
-            (native.SNAP_Q, native.SNAP_Q2D, native.SNAP_DEP2D, native.SNAP_P2D, native.SNAP_FIELDS)
-            if what & k}
+    return {k: native.PinnedArray(N * s.snapshot_cols(k)) for k in native.SNAP_KINDS if what & k}
 
 
-@pytest.mark.parametrize("name, steps", [("c1_64", 200), ("c2_64", 200)])
+@pytest.mark.parametrize("name, steps", [("c1_64", 200), ("c2_64", 200), ("c2e_64", 200)])
 def test_snapshot_records_match_oracle(name, steps):
     print(f"snapshot {name} steps={steps}", flush=True)
     cfg, B = cases.CASES[name]()
@@ -48,6 +47,7 @@ def test_snapshot_records_match_oracle(name, steps):
         s.L.imex_set_time(s.h, ctypes.c_double(t), ctypes.c_double(d1), ctypes.c_double(d2))
         s.set_topography(B)
         f0 = s.fields()
+        ero0, erd0 = s.erosion()
         N = cfg.nx * cfg.ny
         bufs = _bufs(s, ALL)
         s.snapshot_begin(ALL, {k: b.a for k, b in bufs.items()})
@@ -66,20 +66,33 @@ def test_snapshot_records_match_oracle(name, steps):
         dep2d = bufs[native.SNAP_DEP2D].a.reshape(N, -1)
         np.testing.assert_array_equal(
             dep2d, np.concatenate([xy, f0["deposit"].reshape(cfg.n_solid, N).T], 1))
-        want, depz = outputs_ref.p2d_records(cfg, qz, f0["deposit"], B, f0["hmax"], f0["pdynmax"],
-                                             f0["mod_vel_max"], o)
+        want, depz, eroz = outputs_ref.p2d_records(cfg, qz, f0["deposit"], B, f0["hmax"],
+                                                   f0["pdynmax"], f0["mod_vel_max"], o, ero0, erd0)
         got = bufs[native.SNAP_P2D].a.reshape(N, -1)
         np.testing.assert_array_equal(got, want)
         np.testing.assert_array_equal(bufs[native.SNAP_Q].a, qz.reshape(-1))
-        fld = bufs[native.SNAP_FIELDS].a.reshape(cfg.n_solid + 3, N)
-        np.testing.assert_array_equal(fld[:cfg.n_solid], depz.reshape(cfg.n_solid, N))
-        np.testing.assert_array_equal(fld[cfg.n_solid], f0["hmax"].reshape(N))
+        ns = cfg.n_solid
+        fld = bufs[native.SNAP_FIELDS].a.reshape(3 * ns + 3, N)
+        np.testing.assert_array_equal(fld[:ns], depz.reshape(ns, N))
+        np.testing.assert_array_equal(fld[ns], f0["hmax"].reshape(N))
         np.testing.assert_array_equal(s.fields()["deposit"], depz)
+        # erosion runs: the _erosion_ / _erodible_ rows before the .p_2d pass,
+        # the zeroed erosion after it
+        for kind, arr in ((native.SNAP_ERO2D, ero0), (native.SNAP_ERD2D, erd0)):
+            np.testing.assert_array_equal(bufs[kind].a.reshape(N, -1),
+                                          np.concatenate([xy, arr.reshape(ns, N).T], 1))
+        np.testing.assert_array_equal(fld[ns + 3:2 * ns + 3], eroz.reshape(ns, N))
+        np.testing.assert_array_equal(fld[2 * ns + 3:], erd0.reshape(ns, N))
+        np.testing.assert_array_equal(s.erosion()[0], eroz)
+        if cfg.erosion_coeff > 0.0:
+            assert (erd0 < st["erodible"]).any() and (eroz > 0).any()
         # the run goes on from the zeroed state as the oracle does
         # a fresh start from the zeroed state, as after the upload
         o2 = oraclepy.Oracle(cfg, st, qz)
         o2.set_pow_mode(1)
         o2.set_time(t, d1, d2)
+        if cfg.erosion_coeff > 0.0:
+            o2.set_erodible(erd0)
         s.run_steps(30)
         o2.run(30)
         np.testing.assert_array_equal(s.q, o2.q())
