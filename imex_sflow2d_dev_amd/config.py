@@ -77,6 +77,9 @@ class RunConfig:
     T_erodible: float = 300.0
     erodible_deposit_flag: bool = False
     erodible_thickness: float = 0.0
+    # B_cent follows erosion and deposition (topo_change_flag), with the bed
+    # slopes recomputed every step (src/IMEX_SfloW2D.f90:445)
+    topo_change_flag: bool = False
     # vulnerability table (VULNERABILITY_TABLE_PARAMETERS)
     thickness_levels: List[float] = dataclasses.field(default_factory=list)
     dyn_pres_levels: List[float] = dataclasses.field(default_factory=list)

@@ -33,11 +33,11 @@ struct imex_handle {
   int dts_cap = 0;
   // outputs (imex_output.hip)
   unsigned char* hpos_old = nullptr;  // [N] runout mask stored by the previous call
-  double* bcent = nullptr;            // [N] B_cent, once the caller set it
   void* d_red = nullptr;              // per-block partials of the runout reductions
   int red_cap = 0;
   // snapshots of output_solution's records (imex_output.hip): device staging
   // per record kind, copied to the host on their own stream
+  bool bcent_set = false;  // imex_set_topography was called
   hipStream_t copy_stream = nullptr;
   hipEvent_t ev_staged = nullptr, ev_copied = nullptr;
   bool copy_inflight = false;

@@ -1303,7 +1303,8 @@ __device__ void stage2(const Params& P, double dt, double* q, const double* X,
 // caller passes zeros and ignores them when erosion is off)
 template <int NS, int LANES>
 __device__ void erosion_deposition(const Params& P, double dt, double* q, double* depo,
-                                   double* erod, double* ero, unsigned int& err, int lane) {
+                                   double* erod, double* ero, double& topo, unsigned int& err,
+                                   int lane) {
   constexpr int NV = 4 + NS, NP = NV + 2;
   double qp[NP], pd, dep[NS], eq[NV], ers[NS];
   if (q[0] > 0.0) {
@@ -1313,7 +1314,7 @@ __device__ void erosion_deposition(const Params& P, double dt, double* q, double
     for (int i = 0; i < NP; i++) qp[i] = 0.0;
     qp[3] = P.T_amb;
   }
-  if (mass_exchange<NS, LANES>(P, qp, erod, dt, dep, eq, ers, lane)) err |= IMEX_ERR_NEG_DEP;
+  if (mass_exchange<NS, LANES>(P, qp, erod, dt, dep, eq, ers, topo, lane)) err |= IMEX_ERR_NEG_DEP;
 #pragma unroll
   for (int i = 0; i < NV; i++) q[i] = q[i] + dt * eq[i];
 #pragma unroll
@@ -1473,10 +1474,12 @@ __device__ __forceinline__ void update_body(const Params& P, const Grid& G, doub
           erod[v] = er ? G.erodible[(size_t)v * N + c] : 0.0;
           ero[v] = er ? G.erosion[(size_t)v * N + c] : 0.0;
         }
-        erosion_deposition<NS, LANES>(P, dt, q, depo, erod, ero, err, lane);
+        double topo;
+        erosion_deposition<NS, LANES>(P, dt, q, depo, erod, ero, topo, err, lane);
         if (lead) {
 #pragma unroll
           for (int v = 0; v < NS; v++) G.dep[(size_t)v * N + c] = depo[v];
+          if (P.topo_change) G.bcent[c] = G.bcent[c] + dt * topo;
           if (er) {
 #pragma unroll
             for (int v = 0; v < NS; v++) {
@@ -1669,8 +1672,9 @@ __global__ void kt_mass_exchange(Params P, int n, const double* qp, const double
     double e[NS], r[NS];
 #pragma unroll
     for (int k = 0; k < NS; k++) e[k] = erod ? erod[(size_t)NS * i + k] : 0.0;
+    double topo;
     mass_exchange<NS>(P, qp + (size_t)(6 + NS) * i, e, dt, dep + (size_t)NS * i,
-                      eq + (size_t)(4 + NS) * i, r);
+                      eq + (size_t)(4 + NS) * i, r, topo);
     if (ers)
 #pragma unroll
       for (int k = 0; k < NS; k++) ers[(size_t)NS * i + k] = r[k];
@@ -1855,6 +1859,7 @@ extern "C" int imex_create(const imex_config* cfg, imex_handle** out) {
     P.T_erodible = 300.0;
     P.erodible_deposit = cfg->erodible_deposit_flag;
   }
+  P.topo_change = cfg->topo_change_flag;
   P.rho_c_sub = cfg->pres / (cfg->sp_gas_const_a * P.T_erodible);
   P.vel_source = cfg->vel_source;
   P.T_source = cfg->T_source;
@@ -1895,6 +1900,8 @@ extern "C" int imex_create(const imex_config* cfg, imex_handle** out) {
     if ((rc = dalloc(h, &G.erodible, N * ns))) return rc;
     if ((rc = dalloc(h, &G.erosion, N * ns))) return rc;
   }
+  if (P.topo_change)
+    if ((rc = dalloc(h, &G.bcent, N))) return rc;
   if (P.n_thk * P.n_pdyn > 0)
     if ((rc = dalloc(h, &G.vuln, N))) return rc;
   if ((rc = dalloc(h, &G.mask, N))) return rc;
@@ -2030,6 +2037,42 @@ static int update(imex_handle* h, int parts, double dt, int mode, int step_idx =
   HCHK(h, hipGetLastError());
   return 0;
 }
+// topography_reconstruction's bed slopes (src/geometry_2d.f90:744-836) from
+// B_cent, every cell: the 5-point stencil (-2, -1, 0, 1, 2) / (10 dx) summed
+// in the reference's order at interior cells, and the border copies, which
+// give each cell the value of the nearest interior cell (x, then y).
+__global__ void k_topo(Params P, Grid G) {
+  const double c1[5] = {-2.0, -1.0, 0.0, 1.0, 2.0};
+  const int nx = G.nx, ny = G.ny;
+  const double nxn = 10.0 * P.dx, nyn = 10.0 * P.dy;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < G.N; c += gridDim.x * blockDim.x) {
+    const int j = c % nx, k = c / nx;
+    const int js = min(max(j, 2), nx - 3), ks = min(max(k, 2), ny - 3);
+    double sx = 0.0, sy = 0.0;
+#pragma unroll
+    for (int m = 0; m < 5; m++) sx = sx + c1[m] * G.bcent[(js + m - 2) + (size_t)nx * ks];
+#pragma unroll
+    for (int m = 0; m < 5; m++) sy = sy + c1[m] * G.bcent[js + (size_t)nx * (ks + m - 2)];
+    G.bx[c] = sx / nxn;
+    G.by[c] = sy / nyn;
+  }
+}
+// a run with topography change steps only once B_cent is on the device
+static int need_topography(imex_handle* h) {
+  if (h->P.topo_change && !h->bcent_set) {
+    h->err = "topo_change_flag: imex_set_topography must give B_cent before the first step";
+    return -2;
+  }
+  return 0;
+}
+static int topo(imex_handle* h) {
+  if (!h->P.topo_change) return 0;
+  const int nb = std::max(1, std::min(8 * h->n_cu, (h->G.N + 255) / 256));
+  hipLaunchKernelGGL(k_topo, dim3(nb), dim3(256), 0, h->stream, h->P, h->G);
+  HCHK(h, hipGetLastError());
+  return 0;
+}
+
 // applies a pending end of step (after the last device-driven step of a call)
 static int commit(imex_handle* h) {
   hipLaunchKernelGGL(k_commit, dim3(1), dim3(1), 0, h->stream, h->G);
@@ -2141,6 +2184,7 @@ extern "C" int imex_rk_step(imex_handle* h, double dt) {
 
 extern "C" int imex_erosion_deposition(imex_handle* h, double dt) {
   if (!h) return -2;
+  if (int rc = need_topography(h)) return rc;
   HCHK(h, hipSetDevice(h->device));
   return update(h, 2, dt, 0);
 }
@@ -2212,6 +2256,7 @@ static int begin_driven(imex_handle* h, double t_end, DevState* s) {
 
 extern "C" int imex_run_steps(imex_handle* h, int nsteps, double t_end, double* dts) {
   if (!h || nsteps < 0) return -2;
+  if (int rc = need_topography(h)) return rc;
   HCHK(h, hipSetDevice(h->device));
   if (int rc = bind_dts(h, dts, nsteps)) return rc;
   {
@@ -2223,6 +2268,7 @@ extern "C" int imex_run_steps(imex_handle* h, int nsteps, double t_end, double* 
     if ((rc = mask(h, 0, 1))) return rc;
     if ((rc = flux(h, 1, 0.0, n))) return rc;
     if ((rc = update(h, 1 | 2 | 4, 0.0, 1, n))) return rc;
+    if ((rc = topo(h))) return rc;
   }
   if (nsteps > 0 && (rc = commit(h))) return rc;
   if (dts && nsteps > 0)
@@ -2236,6 +2282,7 @@ extern "C" int imex_run_steps_timed(imex_handle* h, int nsteps, double t_end, do
                                     int sample_every, double* kernel_ms, int* n_sampled,
                                     int64_t* cells) {
   if (!h || nsteps < 0 || sample_every < 1) return -2;
+  if (int rc = need_topography(h)) return rc;
   HCHK(h, hipSetDevice(h->device));
   if (int rc = bind_dts(h, dts, nsteps)) return rc;
   DevState s;
@@ -2263,6 +2310,7 @@ extern "C" int imex_run_steps_timed(imex_handle* h, int nsteps, double t_end, do
     if (!rc) rc = flux(h, 1, 0.0, n);
     rec(e, 2);
     if (!rc) rc = update(h, 1 | 2 | 4, 0.0, 1, n);
+    if (!rc) rc = topo(h);
     rec(e, 3);
   }
   if (!rc && nsteps > 0) rc = commit(h);
@@ -2421,7 +2469,9 @@ extern "C" int imex_batch_set_blocks(imex_batch* b, int blocks_per_member) {
 }
 
 extern "C" int imex_batch_add(imex_batch* b, imex_handle* h, double t_end) {
-  if (!b || !h || h->P.ns != b->ns || h->device != b->device || h->fp32) return -2;
+  // members step together with no per-member slope pass: no topography change
+  if (!b || !h || h->P.ns != b->ns || h->device != b->device || h->fp32 || h->P.topo_change)
+    return -2;
   for (imex_handle* x : b->mem)
     if (x == h) return -2;
   // the member's own stream is idle from here on; its state joins the batch
@@ -2674,6 +2724,13 @@ extern "C" int imex_download(imex_handle* h, unsigned what, double* q_aos, doubl
   if (!mb.empty())
     for (int c = 0; c < N; c++) maskout[c] = mb[c] & 1;
   return 0;
+}
+
+extern "C" int imex_topography_reconstruction(imex_handle* h) {
+  if (!h) return -2;
+  if (int rc = need_topography(h)) return rc;
+  HCHK(h, hipSetDevice(h->device));
+  return topo(h);
 }
 
 extern "C" int imex_set_erodible(imex_handle* h, const double* erodible) {

@@ -85,8 +85,9 @@ struct Grid {
   unsigned int* cand_stamp;  // [ntiles] pass that last listed the tile
   int* tile_static;          // [ntiles] frame-band active cells of the tile
   unsigned long long static_active;  // their sum
-  const double* bx;          // [N] B_prime_x_geom
-  const double* by;          // [N] B_prime_y_geom
+  double* bx;                // [N] B_prime_x_geom (recomputed each step with topography change)
+  double* by;                // [N] B_prime_y_geom
+  double* bcent;             // [N] B_cent, once the caller set it (outputs, topography change)
   const signed char* scell;  // [N] source_cell
   const unsigned char* sides;  // [N] sourceE/W/N/S bits
   const double* arcp;        // [N]

@@ -526,7 +526,7 @@ extern "C" int imex_snapshot_begin(imex_handle* h, unsigned what, double* q, dou
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(1), 0, h->stream, h->d_flag, 0u);
 #define SNAP_LAUNCH(NS)                                                                       \
   hipLaunchKernelGGL((h->fp32 ? k_snapshot<NS, float> : k_snapshot<NS, double>), dim3(nb),   \
-                     dim3(256), 0, h->stream, P, G, h->bcent, what, h->stage[0], h->stage[1], \
+                     dim3(256), 0, h->stream, P, G, h->G.bcent, what, h->stage[0], h->stage[1], \
                      h->stage[2], h->stage[3], h->stage[4], h->stage[5], h->stage[6],        \
                      h->d_flag)
   switch (P.ns) {
@@ -587,9 +587,22 @@ void imex_output_release(imex_handle* h) {
 extern "C" int imex_set_topography(imex_handle* h, const double* B_cent) {
   if (!h || !B_cent) return -2;
   HCHK(h, hipSetDevice(h->device));
-  if (!h->bcent)
-    if (int rc = dalloc(h, &h->bcent, (size_t)h->G.N)) return rc;
-  HCHK(h, hipMemcpyAsync(h->bcent, B_cent, 8 * (size_t)h->G.N, hipMemcpyHostToDevice, h->stream));
+  if (!h->G.bcent)
+    if (int rc = dalloc(h, &h->G.bcent, (size_t)h->G.N)) return rc;
+  HCHK(h, hipMemcpyAsync(h->G.bcent, B_cent, 8 * (size_t)h->G.N, hipMemcpyHostToDevice, h->stream));
+  HCHK(h, hipStreamSynchronize(h->stream));
+  h->bcent_set = true;
+  return 0;
+}
+
+extern "C" int imex_get_topography(imex_handle* h, double* B_cent) {
+  if (!h || !B_cent) return -2;
+  if (!h->G.bcent) {
+    h->err = "imex_get_topography: B_cent was not set";
+    return -2;
+  }
+  HCHK(h, hipSetDevice(h->device));
+  HCHK(h, hipMemcpyAsync(B_cent, h->G.bcent, 8 * (size_t)h->G.N, hipMemcpyDeviceToHost, h->stream));
   HCHK(h, hipStreamSynchronize(h->stream));
   return 0;
 }
@@ -597,7 +610,7 @@ extern "C" int imex_set_topography(imex_handle* h, const double* B_cent) {
 extern "C" int imex_runout_reduce(imex_handle* h, double x0_runout, double y0_runout,
                                   double alphas_threshold, unsigned flags, imex_runout_red* out) {
   if (!h || !out) return -2;
-  if ((flags & IMEX_RUNOUT_B) && !h->bcent) {
+  if ((flags & IMEX_RUNOUT_B) && !h->G.bcent) {
     h->err = "imex_runout_reduce: IMEX_RUNOUT_B needs imex_set_topography first";
     return -2;
   }
@@ -616,7 +629,7 @@ extern "C" int imex_runout_reduce(imex_handle* h, double x0_runout, double y0_ru
   const Grid& G = h->G;
 #define RUNOUT_LAUNCH(NS)                                                                   \
   hipLaunchKernelGGL((h->fp32 ? k_runout<NS, float> : k_runout<NS, double>), dim3(nb),     \
-                     dim3(RB), 0, h->stream, P, G, h->bcent, h->hpos_old, x0_runout,        \
+                     dim3(RB), 0, h->stream, P, G, h->G.bcent, h->hpos_old, x0_runout,        \
                      y0_runout, alphas_threshold, flags, part)
   switch (P.ns) {
     case 1: RUNOUT_LAUNCH(1); break;

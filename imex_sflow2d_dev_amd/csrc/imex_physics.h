@@ -39,6 +39,7 @@ struct Params {
   // erosion of the erodible layer (src/inpout_2d.f90:1813-1900)
   double erosion_coeff, erodible_porosity, coeff_porosity, erodible_fract[3];
   int erodible_deposit;
+  int topo_change;  // B_cent follows erosion and deposition (topo_change_flag)
   double vel_source, T_source, h_source, time_param[4], pi_g;
   double thk[8], pdyn[8];
   double hcs, one_by_hcs;  // complex step n_vars*eps and its reciprocal
@@ -714,8 +715,9 @@ __device__ inline double settling_velocity(const Params& P, double diam, double 
 template <int NS, int LANES = 1>
 __device__ inline bool mass_exchange(const Params& P, const double* qp, const double* erod,
                                      double dt, double* dep, double* eq, double* ers,
-                                     int lane = 0) {
+                                     double& topo, int lane = 0) {
   constexpr int NV = 4 + NS;
+  topo = 0.0;
 #pragma unroll
   for (int i = 0; i < NS; i++) {
     ers[i] = 0.0;
@@ -830,6 +832,13 @@ __device__ inline bool mass_exchange(const Params& P, const double* qp, const do
             P.T_amb * P.cp_a * P.rho_a_amb * air;
 #pragma unroll
   for (int i = 0; i < NS; i++) eq[4 + i] = P.rho_s[i] * (ers[i] - dep[i]);
+  // rate of change of the topography (src/constitutive_2d.f90:5149)
+  double dep_tot = 0.0, ers_tot = 0.0;
+#pragma unroll
+  for (int i = 0; i < NS; i++) dep_tot += dep[i];
+#pragma unroll
+  for (int i = 0; i < NS; i++) ers_tot += ers[i];
+  topo = (dep_tot - ers_tot) / (1.0 - P.erodible_porosity);
   return err;
 }
 

@@ -44,6 +44,7 @@ EXPORTS = [
     "imex_snapshot_cols", "imex_snapshot_begin", "imex_snapshot_wait", "imex_host_alloc",
     "imex_host_free", "imex_write_records", "imex_write_restart", "imex_format_value",
     "imex_set_erodible", "imex_get_erosion", "imex_test_mass_exchange_ers",
+    "imex_topography_reconstruction", "imex_get_topography",
 ]
 
 RUNOUT_STORE, RUNOUT_B = 1, 2
@@ -73,7 +74,7 @@ class ImexConfig(ctypes.Structure):
         ("thickness_levels", ctypes.c_double * 8), ("dyn_pres_levels", ctypes.c_double * 8),
         ("erosion_coeff", ctypes.c_double), ("erodible_porosity", ctypes.c_double),
         ("T_erodible", ctypes.c_double), ("erodible_fract", ctypes.c_double * 3),
-        ("erodible_deposit_flag", ctypes.c_int32),
+        ("erodible_deposit_flag", ctypes.c_int32), ("topo_change_flag", ctypes.c_int32),
     ]
 
 
@@ -170,6 +171,7 @@ def make_config(cfg: RunConfig) -> ImexConfig:
     for i, v in enumerate(cfg.erodible_fractions()):
         c.erodible_fract[i] = v
     c.erodible_deposit_flag = int(cfg.erodible_deposit_flag)
+    c.topo_change_flag = int(cfg.topo_change_flag)
     return c
 
 

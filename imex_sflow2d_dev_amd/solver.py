@@ -81,12 +81,22 @@ class Solver:
         if st.get("erodible") is not None and (self.cfg.erosion_coeff > 0.0
                                                or self.cfg.erodible_deposit_flag):
             self.set_erodible(st["erodible"])
+        if self.cfg.topo_change_flag:
+            if st.get("B_cent") is None:
+                raise ValueError("topo_change_flag needs B_cent in the static fields")
+            self.set_topography(st["B_cent"])
 
     def set_erodible(self, e: np.ndarray):
         """The erodible layer as [class, k, j]."""
         a = np.ascontiguousarray(np.transpose(np.asarray(e, dtype=np.float64), (1, 2, 0)))
         assert a.shape == (self.cfg.ny, self.cfg.nx, self.cfg.n_solid)
         self.H.check(self.L.imex_set_erodible(self.h, native._dp(a)), "imex_set_erodible")
+
+    def topography(self) -> np.ndarray:
+        """B_cent as [k, j] (evolves with topo_change_flag)."""
+        b = np.zeros((self.cfg.ny, self.cfg.nx))
+        self.H.check(self.L.imex_get_topography(self.h, native._dp(b)), "imex_get_topography")
+        return b
 
     def erosion(self):
         """(erosion, erodible) as [class, k, j]."""
@@ -150,6 +160,8 @@ class Solver:
         self.dt = dt
         self.imex_RK_solver()
         self.update_erosion_deposition_cell(dt)
+        if self.cfg.topo_change_flag:
+            self.H.check(self.L.imex_topography_reconstruction(self.h), "topography_reconstruction")
         self.t = self.t + dt
         self.post_step()
         return dt

@@ -61,6 +61,9 @@ typedef struct imex_config {
    * leaves them (summing to one; 1 for one class). */
   double erosion_coeff, erodible_porosity, T_erodible, erodible_fract[3];
   int32_t erodible_deposit_flag;
+  int32_t topo_change_flag;    /* B_cent += dt * topo_term (src/solver_2d.f90:2493-2497) and the
+                                  bed slopes recomputed after each device-driven step; needs
+                                  imex_set_topography before the first step */
 } imex_config;
 
 /* Per-cell static fields computed once by the host (src/geometry_2d.f90:555-895
@@ -153,6 +156,12 @@ int imex_get_time_history(imex_handle *h, double *t, double *dt_old, double *dt_
  * device-driven steps since the last upload (measurement; no counterpart in
  * the reference) */
 int imex_get_counters(imex_handle *h, int64_t *active_cells, int64_t *steps);
+/* topography_reconstruction (src/geometry_2d.f90:555-895, called at
+ * src/IMEX_SfloW2D.f90:445 when topo_change_flag): the bed slopes
+ * B_prime_x_geom / B_prime_y_geom from the device's B_cent, for host-driven
+ * steps (the device-driven steps do it themselves); no-op without
+ * topo_change_flag. */
+int imex_topography_reconstruction(imex_handle *h);
 /* The erodible layer erodible(n_solid, nx, ny) (Fortran order; read_erodible
  * and src/inpout_2d.f90:1982-1986) of a run with erosion. */
 int imex_set_erodible(imex_handle *h, const double *erodible);
@@ -175,9 +184,11 @@ void imex_destroy(imex_handle *h);
  * T = T_ambient (src/IMEX_SfloW2D.f90:318-360, :456-503).  Cell indices are
  * 0-based, linear j + nx*k (Fortran array order). */
 
-/* B_cent (nx*ny, x fastest), which output_runout reads when the runout
- * origin is not set (src/inpout_2d.f90:7293) */
+/* B_cent (nx*ny, x fastest): output_runout reads it when the runout origin
+ * is not set (src/inpout_2d.f90:7293), the .p_2d records write it, and a run
+ * with topo_change_flag evolves it (required there before the first step). */
 int imex_set_topography(imex_handle *h, const double *B_cent);
+int imex_get_topography(imex_handle *h, double *B_cent);
 
 /* The whole-array expressions of one output_runout call
  * (src/inpout_2d.f90:7170-7427) over hpos = (qp(1) > 1e-5) .AND.

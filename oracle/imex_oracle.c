@@ -54,6 +54,8 @@ struct oracle_s {
     /* erosion (src/inpout_2d.f90:1813-1900) */
     double erosion_coeff, erodible_porosity, coeff_porosity, erodible_fract[3];
     int erodible_deposit;
+    int topo_change;        /* topo_change_flag */
+    double *bcent;          /* B_cent (topography change) */
     double vel_source, T_source, h_source, time_param[4], pi_g;
     double thk[8], pdyn[8];
     double hcs, one_by_hcs; /* complex step h = n_vars*eps (src/solver_2d.f90:274-275) */
@@ -601,9 +603,10 @@ static double settling_velocity(const oracle_t *o, double diam, double rhos, dou
  * ers receives the erosion rates.  Returns 1 when the reference would pause
  * (negative deposition). */
 static int eval_mass_exchange_terms(const oracle_t *o, const double *qp, const double *erod,
-                                    double dt, double *dep, double *eq, double *ers)
+                                    double dt, double *dep, double *eq, double *ers, double *topo)
 {
     const int ns = o->ns, nv = o->nv;
+    *topo = 0.0;
     for (int i = 0; i < ns; i++) { ers[i] = 0.0; dep[i] = 0.0; }
     for (int i = 0; i < nv; i++) eq[i] = 0.0;
     if (qp[0] <= EPS) return 0;
@@ -672,6 +675,11 @@ static int eval_mass_exchange_terms(const oracle_t *o, const double *qp, const d
         eq[3] = -(T * (A + rc * cpc * cpl)) + T_erodible * (D + o->rho_c_sub * cpc * cpe) +
                 o->T_amb * o->cp_a * o->rho_a_amb * air;
     for (int i = 0; i < ns; i++) eq[4 + i] = o->rho_s[i] * (ers[i] - dep[i]);
+    /* topo_term: src/constitutive_2d.f90:5149 */
+    double dep_tot = 0.0, ers_tot = 0.0;
+    for (int i = 0; i < ns; i++) dep_tot += dep[i];
+    for (int i = 0; i < ns; i++) ers_tot += ers[i];
+    *topo = (dep_tot - ers_tot) / (1.0 - o->erodible_porosity);
     return err;
 }
 
@@ -1549,9 +1557,9 @@ static void update_erosion_deposition_cell(oracle_t *o, double dt)
             for (int i = 0; i < np; i++) qpc[i] = 0.0;
             qpc[3] = o->T_amb;
         }
-        double dep[3], eq[8], ers[3], erod[3];
+        double dep[3], eq[8], ers[3], erod[3], topo;
         for (int i = 0; i < ns; i++) erod[i] = o->erodible[c + N * i];
-        if (eval_mass_exchange_terms(o, qpc, erod, dt, dep, eq, ers)) o->err |= 16;
+        if (eval_mass_exchange_terms(o, qpc, erod, dt, dep, eq, ers, &topo)) o->err |= 16;
         for (int i = 0; i < nv; i++) qq[i] = qq[i] + dt * eq[i];
         for (int i = 0; i < ns; i++) qq[4 + i] = fmax_f(0.0, qq[4 + i]);
         for (int i = 0; i < ns; i++) o->deposit[c + N * i] = o->deposit[c + N * i] + dt * dep[i];
@@ -1560,6 +1568,7 @@ static void update_erosion_deposition_cell(oracle_t *o, double dt)
         if (o->erodible_deposit)
             for (int i = 0; i < ns; i++)
                 o->erodible[c + N * i] = o->erodible[c + N * i] + dt * dep[i];
+        if (o->topo_change) o->bcent[c] = o->bcent[c] + dt * topo; /* src/solver_2d.f90:2493-2497 */
         for (int i = 0; i < ns; i++)
             if (qq[4 + i] < 0.0) o->err |= 4;
         if (qq[0] <= 0.0) {
@@ -1592,6 +1601,40 @@ static void update_erosion_deposition_cell(oracle_t *o, double dt)
         }
         if (rg <= 0.0)
             for (int i = 0; i < nv; i++) qq[i] = 0.0;
+    }
+}
+
+/* topography_reconstruction's bed slopes (src/geometry_2d.f90:744-836):
+ * the 5-point stencil at the interior cells (1-based 3..n-2), then the
+ * copies to the two-cell border, columns first, then rows. */
+static void topography_reconstruction(oracle_t *o)
+{
+    static const double c1[5] = {-2.0, -1.0, 0.0, 1.0, 2.0};
+    const int nx = o->nx, ny = o->ny;
+    const double n1x = 10.0 * o->dx, n1y = 10.0 * o->dy;
+    for (int k = 2; k <= ny - 3; k++)
+        for (int j = 2; j <= nx - 3; j++) {
+            double sx = 0.0, sy = 0.0;
+            for (int m = 0; m < 5; m++) sx = sx + c1[m] * o->bcent[CELL(o, j + m - 2, k)];
+            for (int m = 0; m < 5; m++) sy = sy + c1[m] * o->bcent[CELL(o, j, k + m - 2)];
+            o->bx[CELL(o, j, k)] = sx / n1x;
+            o->by[CELL(o, j, k)] = sy / n1y;
+        }
+    double *a[2] = {o->bx, o->by};
+    for (int m = 0; m < 2; m++) {
+        double *b = a[m];
+        for (int k = 0; k < ny; k++) {
+            b[CELL(o, 0, k)] = b[CELL(o, 2, k)];
+            b[CELL(o, 1, k)] = b[CELL(o, 2, k)];
+            b[CELL(o, nx - 2, k)] = b[CELL(o, nx - 3, k)];
+            b[CELL(o, nx - 1, k)] = b[CELL(o, nx - 3, k)];
+        }
+        for (int j = 0; j < nx; j++) {
+            b[CELL(o, j, 0)] = b[CELL(o, j, 2)];
+            b[CELL(o, j, 1)] = b[CELL(o, j, 2)];
+            b[CELL(o, j, ny - 2)] = b[CELL(o, j, ny - 3)];
+            b[CELL(o, j, ny - 1)] = b[CELL(o, j, ny - 3)];
+        }
     }
 }
 
@@ -1681,6 +1724,7 @@ oracle_t *oracle_create(const int *ip, const double *rp)
     }
     for (int i = 0; i < 8; i++) { o->thk[i] = rp[46 + i]; o->pdyn[i] = rp[54 + i]; }
     o->erodible_deposit = ip[12];
+    o->topo_change = ip[13];
     o->erosion_coeff = rp[63];
     if (o->erosion_coeff == 0.0) { /* src/inpout_2d.f90:1827-1832 */
         o->erodible_porosity = 0.0;
@@ -1730,6 +1774,7 @@ oracle_t *oracle_create(const int *ip, const double *rp)
     o->hmax = zalloc(8 * N); o->pdynmax = zalloc(8 * N); o->modvelmax = zalloc(8 * N);
     o->deposit = zalloc(8 * N * o->ns);
     o->erodible = zalloc(8 * N * o->ns);
+    o->bcent = zalloc(8 * N);
     o->erosion = zalloc(8 * N * o->ns);
     o->vuln = zalloc(N * (size_t)(o->n_thk * o->n_pdyn > 0 ? o->n_thk * o->n_pdyn : 1));
     o->stL = zalloc(sizeof(int) * NFX); o->stR = zalloc(sizeof(int) * NFX);
@@ -1754,7 +1799,7 @@ void oracle_destroy(oracle_t *o)
                     o->axmax, o->aymax, o->Hx, o->Hy, o->mask, o->mtmp, o->mx, o->my, o->jc,
                     o->kc, o->jsx, o->ksx, o->jsy, o->ksy, o->hmax, o->pdynmax, o->modvelmax,
                     o->deposit, o->vuln, o->stL, o->stR, o->stB, o->stT, o->stQP,
-                    o->erodible, o->erosion};
+                    o->erodible, o->erosion, o->bcent};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
     free(o);
 }
@@ -1809,6 +1854,7 @@ int oracle_run(oracle_t *o, int n, double *dts, int *ncells)
         if (dts) dts[s] = o->dt;
         imex_RK_solver(o);
         update_erosion_deposition_cell(o, o->dt);
+        if (o->topo_change) topography_reconstruction(o);
         o->t = o->t + o->dt;
         epilogue(o, 0);
         for (size_t c = 0; c < N; c++)
@@ -1927,18 +1973,31 @@ void oracle_mass_exchange(const oracle_t *o, int n, const double *qp, double dt,
 {
     const double zero[3] = {0.0, 0.0, 0.0};
     double ers[3];
-    for (int i = 0; i < n; i++)
+    for (int i = 0; i < n; i++) {
+        double topo;
         eval_mass_exchange_terms(o, qp + (size_t)(o->nv + 2) * i, zero, dt, dep + (size_t)o->ns * i,
-                                 eq + (size_t)o->nv * i, ers);
+                                 eq + (size_t)o->nv * i, ers, &topo);
+    }
 }
 /* with the cells' erodible layers erod[n][n_solid]; ers[n][n_solid] the erosion rates */
 void oracle_mass_exchange_ers(const oracle_t *o, int n, const double *qp, const double *erod,
                               double dt, double *dep, double *eq, double *ers)
 {
-    for (int i = 0; i < n; i++)
+    for (int i = 0; i < n; i++) {
+        double topo;
         eval_mass_exchange_terms(o, qp + (size_t)(o->nv + 2) * i, erod + (size_t)o->ns * i, dt,
                                  dep + (size_t)o->ns * i, eq + (size_t)o->nv * i,
-                                 ers + (size_t)o->ns * i);
+                                 ers + (size_t)o->ns * i, &topo);
+    }
+}
+/* B_cent (cells x fastest), evolved with topo_change_flag */
+void oracle_set_bcent(oracle_t *o, const double *b)
+{
+    memcpy(o->bcent, b, 8 * (size_t)o->nx * o->ny);
+}
+void oracle_get_bcent(const oracle_t *o, double *b)
+{
+    memcpy(b, o->bcent, 8 * (size_t)o->nx * o->ny);
 }
 /* erodible[class][cell] (cells x fastest) */
 void oracle_set_erodible(oracle_t *o, const double *e)

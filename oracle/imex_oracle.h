@@ -49,6 +49,8 @@ void oracle_set_t_end(oracle_t *o, double t_end);
 void oracle_mass_exchange_ers(const oracle_t *o, int n, const double *qp, const double *erod,
                               double dt, double *dep, double *eq, double *ers);
 void oracle_set_erodible(oracle_t *o, const double *e);
+void oracle_set_bcent(oracle_t *o, const double *b);
+void oracle_get_bcent(const oracle_t *o, double *b);
 void oracle_get_erosion(const oracle_t *o, double *ers, double *erd);
 double oracle_get_t(const oracle_t *o);
 /* number of cells that went from wet (q1 > 0) to exactly dry in a step, and

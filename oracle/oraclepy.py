@@ -52,7 +52,8 @@ def _lib():
         lib.oracle_pow_cr.restype = ctypes.c_double
         lib.oracle_pow_cr.argtypes = [ctypes.c_double, ctypes.c_double]
         lib.oracle_set_pow_mode.restype = None
-        for name in ("oracle_mass_exchange_ers", "oracle_set_erodible", "oracle_get_erosion"):
+        for name in ("oracle_mass_exchange_ers", "oracle_set_erodible", "oracle_get_erosion",
+                     "oracle_set_bcent", "oracle_get_bcent"):
             getattr(lib, name).restype = None
         lib.oracle_set_t_end.restype = None
         lib.oracle_set_t_end.argtypes = [ctypes.c_void_p, ctypes.c_double]
@@ -96,6 +97,9 @@ class Oracle:
             self.set_static(static)
             if static.get("erodible") is not None:
                 self.set_erodible(static["erodible"])
+            if static.get("B_cent") is not None:
+                b = np.ascontiguousarray(static["B_cent"], np.float64)
+                self.lib.oracle_set_bcent(self.h, _dp(b))
         if q0 is not None:
             self.init_state(q0)
 
@@ -203,6 +207,12 @@ class Oracle:
         erd = np.zeros((c.n_solid, c.ny, c.nx))
         self.lib.oracle_get_erosion(self.h, _dp(ers), _dp(erd))
         return ers, erd
+
+    def topography(self):
+        """B_cent as [k, j]."""
+        b = np.zeros((self.cfg.ny, self.cfg.nx))
+        self.lib.oracle_get_bcent(self.h, _dp(b))
+        return b
 
     def set_t_end(self, t_end):
         self.lib.oracle_set_t_end(self.h, ctypes.c_double(t_end))

@@ -57,7 +57,7 @@ MODULE imexref_harness
        n_topography_profile_y, init_grid, init_source, deposit, erosion,         &
        erodible, B_cent, B_prime_x_geom, B_prime_y_geom, source_cell, sourceE,   &
        sourceW, sourceN, sourceS, cell_arc_perim, cell_arc_n_x, cell_arc_n_y,    &
-       x_comp, x_stag, y_comp, y_stag, dx, dy, pi_g
+       x_comp, x_stag, y_comp, y_stag, dx, dy, pi_g, topography_reconstruction
   USE solver_2d, ONLY : allocate_solver_variables, check_solve, timestep,       &
        imex_RK_solver, update_erosion_deposition_cell, q, qp, t, dt, hmax,       &
        pdynmax, mod_vel_max, vuln_table, thck_table, pdyn_table, solve_mask,     &
@@ -72,12 +72,12 @@ CONTAINS
 
   !----------------------------------------------------------------------------
   ! Set up the reference modules for one run.
-  !   ip(0:11)  integer parameters, rp(0:62) real parameters (layout in
+  !   ip(0:13)  integer parameters, rp(0:68) real parameters (layout in
   !   oracle/refpy.py), bdem(nx*ny) bed elevation at the cell centres
   !   (x index fastest).
   !----------------------------------------------------------------------------
   INTEGER(c_int) FUNCTION imexref_setup(ip, rp, bdem) BIND(C, name='imexref_setup')
-    INTEGER(c_int), INTENT(IN) :: ip(0:12)
+    INTEGER(c_int), INTENT(IN) :: ip(0:13)
     REAL(c_double), INTENT(IN) :: rp(0:68)
     REAL(c_double), INTENT(IN) :: bdem(*)
 
@@ -115,7 +115,7 @@ CONTAINS
     pore_pressure_flag = .FALSE.
     gas_loss_flag = .FALSE.
     sutherland_flag = .FALSE.
-    topo_change_flag = .FALSE.
+    topo_change_flag = ( ip(13) .NE. 0 )
     lateral_source_flag = .FALSE.
     bottom_radial_source_flag = .FALSE.
     erodible_deposit_flag = ( ip(12) .NE. 0 )
@@ -469,6 +469,7 @@ CONTAINS
 
        CALL imex_RK_solver
        CALL update_erosion_deposition_cell(dt)
+       IF ( topo_change_flag ) CALL topography_reconstruction
 
        t = t + dt
 
@@ -521,6 +522,12 @@ CONTAINS
     REAL(c_double), INTENT(IN) :: e(n_solid, comp_cells_x, comp_cells_y)
     erodible = e
   END SUBROUTINE imexref_set_erodible
+
+  ! B_cent(nx, ny)
+  SUBROUTINE imexref_get_bcent(b) BIND(C, name='imexref_get_bcent')
+    REAL(c_double), INTENT(OUT) :: b(comp_cells_x, comp_cells_y)
+    b = B_cent
+  END SUBROUTINE imexref_get_bcent
 
   ! erosion(nx, ny, n_solid) and erodible(n_solid, nx, ny)
   SUBROUTINE imexref_get_erosion(ers, erd) BIND(C, name='imexref_get_erosion')

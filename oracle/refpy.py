@@ -50,9 +50,9 @@ def _ip(a):
 
 
 def pack_params(cfg):
-    """ip[13] / rp[69] parameter vectors read by imexref_setup (and by the
+    """ip[14] / rp[69] parameter vectors read by imexref_setup (and by the
     C restatement's oracle_create)."""
-    ip = np.zeros(13, dtype=np.int32)
+    ip = np.zeros(14, dtype=np.int32)
     ip[0] = cfg.nx
     ip[1] = cfg.ny
     ip[2] = cfg.n_solid
@@ -66,6 +66,7 @@ def pack_params(cfg):
     ip[10] = cfg.limiter
     ip[11] = cfg.bc_flag
     ip[12] = int(getattr(cfg, "erodible_deposit_flag", False))
+    ip[13] = int(getattr(cfg, "topo_change_flag", False))
     rp = np.zeros(69, dtype=np.float64)
     rp[0:16] = [cfg.dx, cfg.x0, cfg.y0, cfg.cfl, cfg.dt0, cfg.max_dt, cfg.t_start,
                 cfg.t_end, cfg.grav, cfg.mu, cfg.xi, cfg.pres, cfg.T_ambient,
@@ -150,6 +151,12 @@ class RefRun:
         a = np.ascontiguousarray(np.transpose(np.asarray(e, dtype=np.float64), (1, 2, 0)))
         assert a.shape == (self.cfg.ny, self.cfg.nx, self.cfg.n_solid)
         self.lib.imexref_set_erodible(_dp(a))
+
+    def topography(self):
+        """B_cent as [k, j]."""
+        b = np.zeros((self.cfg.ny, self.cfg.nx))
+        self.lib.imexref_get_bcent(_dp(b))
+        return b
 
     def erosion(self):
         """(erosion [class, k, j], erodible [class, k, j])."""

@@ -14,13 +14,17 @@ from oracle import oraclepy
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("name, steps, driver, deposit_flag", [
-    ("c1e_48", 300, "device", False), ("c1e_48", 120, "host", False),
-    ("c2e_64", 300, "device", False), ("c2e_64", 200, "device", True)])
-def test_erosion_run_matches_oracle(name, steps, driver, deposit_flag):
-    print(f"erosion {name} steps={steps} {driver} erodible_deposit={deposit_flag}", flush=True)
+@pytest.mark.parametrize("name, steps, driver, deposit_flag, topo", [
+    ("c1e_48", 300, "device", False, False), ("c1e_48", 120, "host", False, False),
+    ("c2e_64", 300, "device", False, False), ("c2e_64", 200, "device", True, False),
+    ("c1e_48", 300, "device", False, True), ("c1e_48", 120, "host", False, True),
+    ("c2e_64", 300, "device", True, True)])
+def test_erosion_run_matches_oracle(name, steps, driver, deposit_flag, topo):
+    print(f"erosion {name} steps={steps} {driver} erodible_deposit={deposit_flag} "
+          f"topo_change={topo}", flush=True)
     cfg, B = cases.CASES[name]()
     cfg.erodible_deposit_flag = deposit_flag
+    cfg.topo_change_flag = topo
     st, q0 = init.setup(cfg, B)
     s = Solver(cfg, st, q0)
     o = oraclepy.Oracle(cfg, st, q0)
@@ -48,6 +52,11 @@ def test_erosion_run_matches_oracle(name, steps, driver, deposit_flag):
         print(f"stale reads in the oracle: {o.counters()[1]}, erosion cells differing: "
               f"{int(np.count_nonzero(ers_d != ers_o))}", flush=True)
         assert ers_o.max() > 1e-4
+        if topo:
+            # B_cent moves by the (deposit - erosion) of each step: the same
+            # cells' sub-1e-25 m differences bound it as they bound the layers
+            np.testing.assert_allclose(s.topography(), o.topography(), rtol=0, atol=1e-25)
+            assert not np.array_equal(o.topography(), B)
     finally:
         s.close()
 

@@ -76,12 +76,16 @@ def test_multistep_bit_exact(ref_lib, name, steps):
     run.close()
 
 
-@pytest.mark.parametrize("name, steps", [("c1e_48", 300), ("c2e_64", 300)])
-def test_erosion_multistep_bit_exact(ref_lib, name, steps):
+@pytest.mark.parametrize("name, steps, topo", [("c1e_48", 300, False), ("c2e_64", 300, False),
+                                               ("c1e_48", 300, True), ("c2e_64", 300, True)])
+def test_erosion_multistep_bit_exact(ref_lib, name, steps, topo):
     """The erosion branch (src/constitutive_2d.f90:4731-4758, the erodible and
-    erosion updates of src/solver_2d.f90:2478-2491) against the reference."""
+    erosion updates of src/solver_2d.f90:2478-2491) against the reference;
+    with topo, B_cent follows erosion and deposition (:2493-2497) and the
+    bed slopes are rebuilt every step (src/IMEX_SfloW2D.f90:445)."""
     from imex_sflow2d_dev_amd import init
     cfg, B = cases.CASES[name]()
+    cfg.topo_change_flag = topo
     run = ref_lib.RefRun(cfg, B)
     st = run.static()
     q0 = run.q()
@@ -100,4 +104,7 @@ def test_erosion_multistep_bit_exact(ref_lib, name, steps):
     np.testing.assert_array_equal(erd_o, erd_r)
     np.testing.assert_array_equal(o.fields()["deposit"], run.fields()["deposit"])
     assert ers_r.max() > 1e-4 and (erd_r < er0).any()
+    if topo:
+        np.testing.assert_array_equal(o.topography(), run.topography())
+        assert not np.array_equal(run.topography(), B)
     run.close()
