@@ -467,6 +467,7 @@ __global__ void k_snapshot(Params P, Grid G, const double* bcent, unsigned what,
         sfld[(size_t)(NS + 3 + i) * N + c] = ero[i];
         sfld[(size_t)(2 * NS + 3 + i) * N + c] = erd[i];
       }
+      sfld[(size_t)(3 * NS + 3) * N + c] = bcent ? bcent[c] : 0.0;
     }
   }
 }
@@ -481,7 +482,7 @@ static int snap_cols(const imex_handle* h, int kind) {
     case 1: return 2 + nv;
     case 2: return 2 + ns;
     case 3: return p2d_cols(ns);
-    case 4: return 3 * ns + 3;
+    case 4: return 3 * ns + 4;
     default: return 2 + ns;
   }
 }

@@ -225,8 +225,10 @@ class SolutionOutput:
                 self.what |= native.SNAP_ERO2D | native.SNAP_ERD2D
         if phys:
             self.what |= native.SNAP_P2D
+        if restart or (cons and c.topo_change_flag):
+            self.what |= native.SNAP_FIELDS
         if restart:
-            self.what |= native.SNAP_Q | native.SNAP_FIELDS
+            self.what |= native.SNAP_Q
         self.bufs = {}
         for kind in native.SNAP_KINDS:
             if self.what & kind:
@@ -246,6 +248,14 @@ class SolutionOutput:
                 rec = self.bufs[nat.SNAP_Q2D].a.reshape(N, -1)
                 nat.write_records(os.path.join(self.dir, f"{self.run_name}_{idx}.q_2d"), rec, nx,
                                   nat.FMT_E20_12, self.n_threads)
+                if c.topo_change_flag:       # _B_ file (src/inpout_2d.f90:6077-6082)
+                    fld = self.bufs[nat.SNAP_FIELDS].a.reshape(3 * c.n_solid + 4, N)
+                    xs = c.x0 + np.arange(c.nx + 1) * c.dx
+                    ys = c.y0 + np.arange(c.ny + 1) * c.dx
+                    X, Y = np.meshgrid(0.5 * (xs[:-1] + xs[1:]), 0.5 * (ys[:-1] + ys[1:]))
+                    rec = np.stack([X.reshape(-1), Y.reshape(-1), fld[3 * c.n_solid + 3]], 1)
+                    nat.write_records(os.path.join(self.dir, f"{self.run_name}_B_{idx}.q_2d"), rec,
+                                      nx, nat.FMT_E20_12, self.n_threads)
                 for kind, tag, on in ((nat.SNAP_ERO2D, "erosion", c.erosion_coeff > 0.0),
                                       (nat.SNAP_ERD2D, "erodible", c.erosion_coeff > 0.0),
                                       (nat.SNAP_DEP2D, "deposit", c.settling_flag)):
@@ -259,9 +269,9 @@ class SolutionOutput:
                                   nat.FMT_ES15_7, self.n_threads)
             if with_restart and self.restart:
                 ns = c.n_solid
-                fld = self.bufs[nat.SNAP_FIELDS].a.reshape(3 * ns + 3, N)
-                B = self.B if self.B is not None else np.zeros(N)
-                erodible = np.ascontiguousarray(fld[2 * ns + 3:].T)   # erodible(n_solid, nx, ny)
+                fld = self.bufs[nat.SNAP_FIELDS].a.reshape(3 * ns + 4, N)
+                B = np.ascontiguousarray(fld[3 * ns + 3])
+                erodible = np.ascontiguousarray(fld[2 * ns + 3:3 * ns + 3].T)   # erodible(n_solid, nx, ny)
                 nat.write_restart(os.path.join(self.dir, "restart.bin"), c.nx, c.ny, c.n_vars, ns,
                                   time, dt, self.bufs[nat.SNAP_Q].a, B, erodible, fld[:ns],
                                   fld[ns + 3:2 * ns + 3], fld[ns], fld[ns + 1], fld[ns + 2])
@@ -276,7 +286,7 @@ class SolutionOutput:
         self.finish()
         idx = lettera(self.output_idx)
         self.output_idx += 1
-        what = self.what if restart else self.what & ~(self.native.SNAP_Q | self.native.SNAP_FIELDS)
+        what = self.what if restart else self.what & ~self.native.SNAP_Q
         self.s.snapshot_begin(what, {k: b.a for k, b in self.bufs.items()})
         self.timings.append([0.0, 0.0])
         args = (idx, time, self.s.dt, restart)

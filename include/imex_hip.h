@@ -246,7 +246,8 @@ int imex_probes(imex_handle *h, int n, const double *xy, double *out);
                                hmax, pdynmax, mod_vel_max, I (:6390-6396): 19 + 4 n_solid
                                values; B from imex_set_topography (0 without it) */
 #define IMEX_SNAP_FIELDS 16u /* deposit(nx,ny,n_solid), hmax, pdynmax, mod_vel_max,
-                               erosion(nx,ny,n_solid), erodible as [class][cell] planes */
+                               erosion(nx,ny,n_solid), erodible as [class][cell] planes,
+                               B_cent (0 without imex_set_topography): 3 n_solid + 4 */
 #define IMEX_SNAP_ERO2D 32u /* x, y, erosion(j,k,1:n_solid) (:6060-6063), erosion runs */
 #define IMEX_SNAP_ERD2D 64u /* x, y, erodible(1:n_solid,j,k) (:6065-6066), erosion runs */
 int imex_snapshot_cols(imex_handle *h, unsigned kind);

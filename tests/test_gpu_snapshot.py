@@ -72,7 +72,8 @@ def test_snapshot_records_match_oracle(name, steps):
         np.testing.assert_array_equal(got, want)
         np.testing.assert_array_equal(bufs[native.SNAP_Q].a, qz.reshape(-1))
         ns = cfg.n_solid
-        fld = bufs[native.SNAP_FIELDS].a.reshape(3 * ns + 3, N)
+        fld = bufs[native.SNAP_FIELDS].a.reshape(3 * ns + 4, N)
+        np.testing.assert_array_equal(fld[3 * ns + 3], B.reshape(N))
         np.testing.assert_array_equal(fld[:ns], depz.reshape(ns, N))
         np.testing.assert_array_equal(fld[ns], f0["hmax"].reshape(N))
         np.testing.assert_array_equal(s.fields()["deposit"], depz)
@@ -82,7 +83,7 @@ def test_snapshot_records_match_oracle(name, steps):
             np.testing.assert_array_equal(bufs[kind].a.reshape(N, -1),
                                           np.concatenate([xy, arr.reshape(ns, N).T], 1))
         np.testing.assert_array_equal(fld[ns + 3:2 * ns + 3], eroz.reshape(ns, N))
-        np.testing.assert_array_equal(fld[2 * ns + 3:], erd0.reshape(ns, N))
+        np.testing.assert_array_equal(fld[2 * ns + 3:3 * ns + 3], erd0.reshape(ns, N))
         np.testing.assert_array_equal(s.erosion()[0], eroz)
         if cfg.erosion_coeff > 0.0:
             assert (erd0 < st["erodible"]).any() and (eroz > 0).any()
@@ -103,9 +104,11 @@ def test_snapshot_records_match_oracle(name, steps):
         s.close()
 
 
-def test_solution_files_of_a_run(tmp_path):
-    print("run with solution outputs c2_64", flush=True)
-    cfg, B = cases.c2_64()
+@pytest.mark.parametrize("name, topo", [("c2_64", False), ("c2e_64", True)])
+def test_solution_files_of_a_run(tmp_path, name, topo):
+    print(f"run with solution outputs {name} topo_change={topo}", flush=True)
+    cfg, B = cases.CASES[name]()
+    cfg.topo_change_flag = topo
     st, q0 = init.setup(cfg, B)
     s = Solver(cfg, st, q0)
     sol = outputs.SolutionOutput(s, "run", str(tmp_path), dt_output=0.4, B=B, n_threads=4)
@@ -113,6 +116,8 @@ def test_solution_files_of_a_run(tmp_path):
         steps, _ = outputs.run_with_outputs(s, 1.2, solution=sol)
         q_end = s.q
         f_end = s.fields()
+        ero_end, erd_end = s.erosion()
+        b_end = s.topography() if topo else B
     finally:
         sol.close()
         s.close()
@@ -121,6 +126,12 @@ def test_solution_files_of_a_run(tmp_path):
         for suffix in (f"run_{i:04d}.q_2d", f"run_deposit_{i:04d}.q_2d", f"run_{i:04d}.p_2d"):
             assert suffix in names, (suffix, names)
     assert "restart.bin" in names and steps > 20
+    if cfg.erosion_coeff > 0.0:
+        for tag in ("erosion", "erodible") + (("B",) if topo else ()):
+            assert f"run_{tag}_0003.q_2d" in names, (tag, names)
+        if topo:
+            b2d = np.loadtxt(tmp_path / "run_B_0003.q_2d")
+            np.testing.assert_allclose(b2d[:, 2], b_end.reshape(-1), rtol=1e-11, atol=0)
     N, nv, ns = cfg.nx * cfg.ny, cfg.n_vars, cfg.n_solid
     # line layout: fixed-width records, a blank record after each row
     p2d = (tmp_path / "run_0003.p_2d").read_text().split("\n")
@@ -142,5 +153,9 @@ def test_solution_files_of_a_run(tmp_path):
     np.testing.assert_array_equal(np.frombuffer(recs[2]), q_end.reshape(-1))
     np.testing.assert_array_equal(np.frombuffer(recs[5]), f_end["deposit"].reshape(-1))
     np.testing.assert_array_equal(np.frombuffer(recs[7]), f_end["hmax"].reshape(-1))
+    np.testing.assert_array_equal(np.frombuffer(recs[3]), np.asarray(b_end).reshape(-1))
+    np.testing.assert_array_equal(np.frombuffer(recs[6]), ero_end.reshape(-1))
+    np.testing.assert_array_equal(np.frombuffer(recs[4]),
+                                  np.transpose(erd_end, (1, 2, 0)).reshape(-1))
     t, dt = np.frombuffer(recs[1])
     assert t == pytest.approx(1.2, abs=1e-12)
