@@ -130,7 +130,7 @@ def main():
                          "(serving) or c2-outputs (the run with periodic outputs)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the reference CPU sample")
-    ap.add_argument("--event-every", type=int, default=10, metavar="E",
+    ap.add_argument("--event-every", type=int, default=50, metavar="E",
                     help="HIP events around the kernel launches of every E-th timed step "
                          "(an event record costs the stream a few microseconds)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="N",

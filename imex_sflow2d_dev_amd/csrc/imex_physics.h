@@ -747,7 +747,9 @@ __device__ inline bool mass_exchange(const Params& P, const double* qp, const do
   double tse = 0.0;
 #pragma unroll
   for (int i = 0; i < NS; i++) tse += ers[i];
-  double cpe = (tse / (1.0 - P.erodible_porosity)) * P.erodible_porosity;
+  // tot_erosion = tse / (1 - porosity): exactly tse without erosion (porosity 0)
+  const double te = (P.erodible_porosity == 0.0) ? tse : tse / (1.0 - P.erodible_porosity);
+  double cpe = te * P.erodible_porosity;
   if (at <= 0.0) return false;
   double T = qp[3];
   Mixt m = mixt_var<NS>(P, qp, true);
@@ -832,13 +834,16 @@ __device__ inline bool mass_exchange(const Params& P, const double* qp, const do
             P.T_amb * P.cp_a * P.rho_a_amb * air;
 #pragma unroll
   for (int i = 0; i < NS; i++) eq[4 + i] = P.rho_s[i] * (ers[i] - dep[i]);
-  // rate of change of the topography (src/constitutive_2d.f90:5149)
-  double dep_tot = 0.0, ers_tot = 0.0;
+  // rate of change of the topography (src/constitutive_2d.f90:5149), used
+  // only with topography change
+  if (P.topo_change) {
+    double dep_tot = 0.0, ers_tot = 0.0;
 #pragma unroll
-  for (int i = 0; i < NS; i++) dep_tot += dep[i];
+    for (int i = 0; i < NS; i++) dep_tot += dep[i];
 #pragma unroll
-  for (int i = 0; i < NS; i++) ers_tot += ers[i];
-  topo = (dep_tot - ers_tot) / (1.0 - P.erodible_porosity);
+    for (int i = 0; i < NS; i++) ers_tot += ers[i];
+    topo = (dep_tot - ers_tot) / (1.0 - P.erodible_porosity);
+  }
   return err;
 }
 
