@@ -1,5 +1,5 @@
 """The FP32 form against the FP64 drop-in on the same run: the quantities
-the precision form is judged on -- runout area, maximum thickness, deposit
+the precision form is compared on -- runout area, maximum thickness, deposit
 and conserved mass -- from the final state and output fields of each."""
 from __future__ import annotations
 

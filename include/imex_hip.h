@@ -3,7 +3,7 @@
  *
  * Drop-in boundary for MODULE solver_2d of the reference (src/solver_2d.f90).
  * The Fortran driver keeps its time loop (src/IMEX_SfloW2D.f90:395-591) and
- * calls these entry points through ISO_C_BINDING (see INTEGRATION.md).  All
+ * calls these entry points through ISO_C_BINDING.  All
  * state lives on the GPU in SoA layout; host arrays are borrowed only for the
  * duration of a call.  Arrays crossing the boundary use the reference's
  * Fortran order: q(n_vars, nx, ny) -> C index (k*nx + j)*n_vars + i.
