@@ -1,5 +1,6 @@
 """Run configuration for the IMEX-RK stage of the gas-particle shallow-water
-system (gas carrier, Voellmy friction, KT fluxes, minmod MUSCL, IMEX RK2).
+system (gas carrier, Voellmy or mu(I) friction, KT fluxes, minmod MUSCL,
+IMEX RK2).
 
 The fields are the namelist values the reference reads for this path
 (src/inpout_2d.f90:326-391 namelists; defaults at :408-970) plus the derived
@@ -80,6 +81,15 @@ class RunConfig:
     # B_cent follows erosion and deposition (topo_change_flag), with the bed
     # slopes recomputed every step (src/IMEX_SfloW2D.f90:445)
     topo_change_flag: bool = False
+    # friction (RHEOLOGY_PARAMETERS; mu(I) defaults -1 at src/inpout_2d.f90:511-514,
+    # checks at :3592-3630):
+    # 1 = Voellmy-Salm (mu, xi); 11 = mu(I) for dense granular flows
+    # (mu_s, mu_2, muI_inf, I_0); 12 = mu(I) plus the Voellmy turbulent term
+    rheology_model: int = 1
+    mu_s: float = -1.0
+    mu_2: float = -1.0
+    muI_inf: float = -1.0
+    I_0: float = -1.0
     # vulnerability table (VULNERABILITY_TABLE_PARAMETERS)
     thickness_levels: List[float] = dataclasses.field(default_factory=list)
     dyn_pres_levels: List[float] = dataclasses.field(default_factory=list)
@@ -151,6 +161,12 @@ class RunConfig:
             raise ValueError("only the minmod limiter (1) is on this path")
         if self.bc_flag != 1:
             raise ValueError("only Neumann (flag 1) boundaries are on this path")
+        if self.rheology_model not in (1, 11, 12):
+            raise ValueError("rheology_model must be 1 (Voellmy-Salm), 11 or 12 (mu(I))")
+        if self.rheology_model in (11, 12) and min(self.mu_s, self.mu_2, self.I_0,
+                                                    self.muI_inf) < 0.0:
+            # src/inpout_2d.f90:3600-3611, :3622-3629
+            raise ValueError("mu(I) rheology needs mu_s, mu_2, I_0 and muI_inf >= 0")
         if self.erosion_coeff < 0.0:
             raise ValueError("erosion_coeff must be >= 0")
         if self.erosion_coeff > 0.0:

@@ -1776,6 +1776,12 @@ extern "C" int imex_create(const imex_config* cfg, imex_handle** out) {
       cfg->n_thickness_levels < 0 || cfg->n_thickness_levels > 8 || cfg->n_dyn_pres_levels < 0 ||
       cfg->n_dyn_pres_levels > 8 || cfg->precision < 0 || cfg->precision > 1)
     return -3;
+  // friction law: Voellmy-Salm (0 = 1) or mu(I) with non-negative parameters
+  // (src/inpout_2d.f90:3592-3630)
+  const int rheo = cfg->rheology_model == 0 ? 1 : cfg->rheology_model;
+  if (rheo != 1 && rheo != 11 && rheo != 12) return -3;
+  if (rheo != 1 && !(cfg->mu_s >= 0.0 && cfg->mu_2 >= 0.0 && cfg->I_0 >= 0.0 && cfg->muI_inf >= 0.0))
+    return -3;
   imex_handle* h = new imex_handle();
   h->cfg = *cfg;
   h->fp32 = cfg->precision == 1;
@@ -1860,6 +1866,11 @@ extern "C" int imex_create(const imex_config* cfg, imex_handle** out) {
     P.erodible_deposit = cfg->erodible_deposit_flag;
   }
   P.topo_change = cfg->topo_change_flag;
+  P.rheology = rheo;
+  P.mu_s = cfg->mu_s;
+  P.mu_2 = cfg->mu_2;
+  P.muI_inf = cfg->muI_inf;
+  P.I_0 = cfg->I_0;
   P.rho_c_sub = cfg->pres / (cfg->sp_gas_const_a * P.T_erodible);
   P.vel_source = cfg->vel_source;
   P.T_source = cfg->T_source;

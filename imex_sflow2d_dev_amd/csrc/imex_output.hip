@@ -305,11 +305,12 @@ __global__ void k_probes(Params P, Grid G, int n, const double* xy, double* out)
 // place and writes its columns; with erosion the _erosion_ and _erodible_
 // files get x, y and the layers (:6060-6069) and the .p_2d pass zeroes
 // |erosion| < 1e-20 in place too.  Configured path: no added gases, liquid,
-// stochastic or pore-pressure variables, Voellmy rheology (model 1): shear
-// velocity, Rouse numbers and inertial number 0; without erosion the erosion
-// and erodible columns are 0.  mu_eff reads pore_pres(1) of a zero-size
-// array in the reference; here the excess pore pressure is 0, so mu_eff =
-// mu * MAX(0, 1 - 0 / (rho_m h g')) on wet cells.
+// stochastic or pore-pressure variables: shear velocity and Rouse numbers
+// 0; without erosion the erosion and erodible columns are 0.  Voellmy
+// rheology (model 1): inertial number 0, and mu_eff reads pore_pres(1) of a
+// zero-size array in the reference; here the excess pore pressure is 0, so
+// mu_eff = mu * MAX(0, 1 - 0 / (rho_m h g')) on wet cells.  mu(I) models 11
+// and 12: mu_eff and the inertial number of :6277-6335.
 // ---------------------------------------------------------------------------
 __host__ __device__ constexpr int p2d_cols(int ns) { return 19 + 4 * ns; }
 
@@ -385,12 +386,32 @@ __global__ void k_snapshot(Params P, Grid G, const double* bcent, unsigned what,
       double h = qp[0], u = qp[NV], v = qp[NV + 1], T = qp[3];
       double rho_m = m.rho_m, rg = m.red_grav;
       double al[NS];
-      double mu_eff = 0.0;
+      double mu_eff = 0.0, inertial = 0.0;
       if (h > 0.0) {
 #pragma unroll
         for (int i = 0; i < NS; i++) al[i] = qp[4 + i] / h;
-        const double excess = fmax_f(0.0, P.pres - P.pres);
-        mu_eff = P.mu * fmax_f(0.0, (1.0 - excess / (rho_m * h * rg)));
+        if (P.rheology == 1) {
+          const double excess = fmax_f(0.0, P.pres - P.pres);
+          mu_eff = P.mu * fmax_f(0.0, (1.0 - excess / (rho_m * h * rg)));
+        } else {
+          // mu(I) models (:6277-6335): mu_s at rest, else mu(I) of the
+          // horizontal speed; the thickness column shows the EPSILON floor
+          const double mhv = sqrt(u * u + v * v);
+          if (mhv < kEps) {
+            mu_eff = P.mu_s;
+          } else {
+            if (h < kEps) h = kEps;
+            const double vse = rho_m * rg * h;
+            double diam = P.diam_s[0], rhop = P.rho_s[0];
+            if (NS > 1) mean_particle<NS>(P, al, diam, rhop);
+            const double shear = 5.0 / 2.0 * mhv / h;
+            double ens = fmax_f(0.0, vse * sqrt(1.0));
+            if (ens < kEps) ens = kEps;
+            inertial = diam * shear / sqrt(ens / rhop);
+            mu_eff = (P.mu_s * P.I_0 + P.mu_2 * inertial + P.muI_inf * (inertial * inertial)) /
+                     (P.I_0 + inertial);
+          }
+        }
       } else {
 #pragma unroll
         for (int i = 0; i < NS; i++) al[i] = 0.0;
@@ -422,6 +443,7 @@ __global__ void k_snapshot(Params P, Grid G, const double* bcent, unsigned what,
       tiny(rho_m);
       tiny(rg);
       tiny(mu_eff);
+      tiny(inertial);
       double* r = sp2d + (size_t)c * p2d_cols(NS);
       int o = 0;
       r[o++] = X;
@@ -450,7 +472,7 @@ __global__ void k_snapshot(Params P, Grid G, const double* bcent, unsigned what,
       r[o++] = G.hmax[c];
       r[o++] = G.pdyn[c];
       r[o++] = G.mvel[c];
-      r[o++] = 0.0;                                // inertial number
+      r[o++] = inertial;
     }
     if (what & IMEX_SNAP_Q) {
 #pragma unroll

@@ -40,6 +40,9 @@ struct Params {
   double erosion_coeff, erodible_porosity, coeff_porosity, erodible_fract[3];
   int erodible_deposit;
   int topo_change;  // B_cent follows erosion and deposition (topo_change_flag)
+  // friction law: 1 Voellmy-Salm, 11 mu(I), 12 mu(I) + Voellmy turbulent term
+  int rheology;
+  double mu_s, mu_2, muI_inf, I_0;
   double vel_source, T_source, h_source, time_param[4], pi_g;
   double thk[8], pdyn[8];
   double hcs, one_by_hcs;  // complex step n_vars*eps and its reciprocal
@@ -485,8 +488,47 @@ __device__ inline void expl_terms(const Params& P, T Bx, T By, const T* qp, T* e
   }
 }
 
-// eval_nh_semi_impl_terms, Voellmy Coulomb part, grav_coeff = 1:
-// src/constitutive_2d.f90:4027-4547 (:4164-4213)
+// sauter_diameter and average_density_solids of the solid volume fractions:
+// src/constitutive_2d.f90:387-416
+template <int NS>
+__device__ inline void mean_particle(const Params& P, const double* al, double& diam,
+                                     double& rho) {
+  double s = 0.0, t = 0.0, r = 0.0;
+#pragma unroll
+  for (int i = 0; i < NS; i++) s += al[i];
+  if (s < kEps) {
+#pragma unroll
+    for (int i = 0; i < NS; i++) {
+      t += P.diam_s[i];
+      r += P.rho_s[i];
+    }
+    diam = t / (double)NS;
+    rho = r / (double)NS;
+    return;
+  }
+#pragma unroll
+  for (int i = 0; i < NS; i++) {
+    t += al[i] / P.diam_s[i];
+    r += al[i] * P.rho_s[i];
+  }
+  diam = s / t;
+  rho = r / s;
+}
+
+// mu(I) friction coefficient at the base for a flow of thickness h (floored
+// at EPSILON), speed mod_vel and effective normal stress ens (floored at
+// EPSILON), with grav_coeff = 1: src/constitutive_2d.f90:4409-4446 (model 11)
+// and :4488-4525 (model 12) compute the same quantities.  Returns mu(I).
+__device__ inline double mu_of_I(const Params& P, double diam, double rhop, double h,
+                                 double mod_vel, double ens) {
+  const double shear = 5.0 / 2.0 * mod_vel / (h * sqrt(1.0));
+  const double In = diam * shear / sqrt(ens / rhop);
+  return (P.mu_s * P.I_0 + P.mu_2 * In + P.muI_inf * (In * In)) / (P.I_0 + In);
+}
+
+// eval_nh_semi_impl_terms, grav_coeff = 1: src/constitutive_2d.f90:4027-4547;
+// Voellmy Coulomb part (model 1, :4164-4213) or the mu(I) friction of models
+// 11 and 12 (:4384-4537)
 template <int NS>
 __device__ inline void semi_impl(const Params& P, const double* qp, double* si) {
   constexpr int NV = 4 + NS;
@@ -496,12 +538,33 @@ __device__ inline void semi_impl(const Params& P, const double* qp, double* si) 
   Mixt m = mixt_var<NS>(P, qp, false);
   double w = 0.0;
   double mod_vel = sqrt(u * u + v * v + w * w);
-  if (mod_vel > 0.0) {
-    double tt = P.mu * m.rho_m * (m.red_grav + 0.0) * h * sqrt(1.0);
-    tt = fmax_f(0.0, tt);
-    tt = fmax_f(0.0, tt);
-    si[1] = si[1] - tt * u / mod_vel;
-    si[2] = si[2] - tt * v / mod_vel;
+  if (P.rheology == 1) {
+    if (mod_vel > 0.0) {
+      double tt = P.mu * m.rho_m * (m.red_grav + 0.0) * h * sqrt(1.0);
+      tt = fmax_f(0.0, tt);
+      tt = fmax_f(0.0, tt);
+      si[1] = si[1] - tt * u / mod_vel;
+      si[2] = si[2] - tt * v / mod_vel;
+    }
+  } else if (mod_vel > 0.0) {
+    double diam, rhop;
+    if (NS > 1) {
+      double al[NS];
+#pragma unroll
+      for (int i = 0; i < NS; i++) al[i] = qp[4 + i] / qp[0];
+      mean_particle<NS>(P, al, diam, rhop);
+    } else {
+      diam = P.diam_s[0];
+      rhop = P.rho_s[0];
+    }
+    if (h < kEps) h = kEps;
+    double vse = m.rho_m * (m.red_grav + 0.0) * h;
+    vse = fmax_f(vse, 0.0);
+    double ens = fmax_f(0.0, vse * sqrt(1.0));
+    if (ens < kEps) ens = kEps;
+    const double tt = mu_of_I(P, diam, rhop, h, mod_vel, ens) * ens;
+    si[1] = si[1] - tt * (u / mod_vel);
+    si[2] = si[2] - tt * (v / mod_vel);
   }
 }
 
@@ -512,6 +575,8 @@ template <int NS>
 __device__ inline void implicit_c(const Params& P, const cx* qj, cx& nh1, cx& nh2) {
   nh1 = C(0.0, 0.0);
   nh2 = C(0.0, 0.0);
+  // mu(I) model 11 has no implicit friction term (:3660 takes models 1, 12)
+  if (P.rheology == 11) return;
   cx u, v, rho_m;
   if (qj[0].re > kEps) {
     cx inv_cqj1 = cdiv(C(1.0, 0.0), qj[0]);
@@ -604,6 +669,7 @@ template <int NS>
 __device__ inline bool implicit_r(const Params& P, const double* q, double& nh1, double& nh2) {
   nh1 = 0.0;
   nh2 = 0.0;
+  if (P.rheology == 11) return true;  // no implicit friction term
   // dry: u = v = 0 and mod_vel = 0 on the complex path, both terms +0
   if (!(q[0] > kEps)) return true;
   if (!(q[0] > P.eps_sing)) return false;  // thin-layer velocity regularisation: complex path

@@ -75,6 +75,9 @@ class ImexConfig(ctypes.Structure):
         ("erosion_coeff", ctypes.c_double), ("erodible_porosity", ctypes.c_double),
         ("T_erodible", ctypes.c_double), ("erodible_fract", ctypes.c_double * 3),
         ("erodible_deposit_flag", ctypes.c_int32), ("topo_change_flag", ctypes.c_int32),
+        ("rheology_model", ctypes.c_int32),
+        ("mu_s", ctypes.c_double), ("mu_2", ctypes.c_double), ("muI_inf", ctypes.c_double),
+        ("I_0", ctypes.c_double),
     ]
 
 
@@ -172,6 +175,8 @@ def make_config(cfg: RunConfig) -> ImexConfig:
         c.erodible_fract[i] = v
     c.erodible_deposit_flag = int(cfg.erodible_deposit_flag)
     c.topo_change_flag = int(cfg.topo_change_flag)
+    c.rheology_model = cfg.rheology_model
+    c.mu_s, c.mu_2, c.muI_inf, c.I_0 = cfg.mu_s, cfg.mu_2, cfg.muI_inf, cfg.I_0
     return c
 
 

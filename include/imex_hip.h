@@ -33,7 +33,7 @@ extern "C" {
 #define IMEX_WARN_RESCALE 64u    /* solid mass rescaled after the RK stage (reference warns, goes on) */
 
 /* Run configuration: the namelist values of the configured path (gas
- * carrier, Voellmy friction, KT, minmod, IMEX RK2, Neumann boundaries).
+ * carrier, Voellmy or mu(I) friction, KT, minmod, IMEX RK2, Neumann boundaries).
  * Replaces the module variables read_param sets (src/inpout_2d.f90:986-5134). */
 typedef struct imex_config {
   int32_t nx, ny;              /* comp_cells_x, comp_cells_y */
@@ -64,6 +64,13 @@ typedef struct imex_config {
   int32_t topo_change_flag;    /* B_cent += dt * topo_term (src/solver_2d.f90:2493-2497) and the
                                   bed slopes recomputed after each device-driven step; needs
                                   imex_set_topography before the first step */
+  /* friction law (RHEOLOGY_PARAMETERS): 0 or 1 = Voellmy-Salm (mu, xi);
+   * 11 = mu(I) for dense granular flows, src/constitutive_2d.f90:4384-4458,
+   * no implicit term; 12 = mu(I) plus the Voellmy turbulent term xi,
+   * :4461-4537 and :3660-3683.  The mu(I) parameters must be >= 0
+   * (src/inpout_2d.f90:3592-3630). */
+  int32_t rheology_model;
+  double mu_s, mu_2, muI_inf, I_0;
 } imex_config;
 
 /* Per-cell static fields computed once by the host (src/geometry_2d.f90:555-895

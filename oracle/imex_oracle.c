@@ -6,7 +6,8 @@
  * never calls it.
  *
  * It restates, for the configured branches (gas carrier, no extra gases, no
- * liquid, alpha_flag=F, Voellmy rheology_model=1, KT fluxes, minmod,
+ * liquid, alpha_flag=F, Voellmy rheology_model=1 or the mu(I) models 11
+ * and 12 without pore pressure, KT fluxes, minmod,
  * IMEX RK2, Neumann boundaries, optional lateral radial source, settling and
  * air entrainment), the reference routines named at each function below, in
  * the reference's own data layout (Fortran order, variable index fastest),
@@ -55,6 +56,8 @@ struct oracle_s {
     double erosion_coeff, erodible_porosity, coeff_porosity, erodible_fract[3];
     int erodible_deposit;
     int topo_change;        /* topo_change_flag */
+    int rheology;           /* rheology_model: 1, 11 or 12 */
+    double mu_s, mu_2, muI_inf, I_0;  /* mu(I) parameters (RHEOLOGY_PARAMETERS) */
     double *bcent;          /* B_cent (topography change) */
     double vel_source, T_source, h_source, time_param[4], pi_g;
     double thk[8], pdyn[8];
@@ -482,8 +485,34 @@ static void eval_expl_terms(const oracle_t *o, double Bx, double By, double G, d
     }
 }
 
-/* eval_nh_semi_impl_terms, Voellmy Coulomb part:
- * src/constitutive_2d.f90:4027-4547 (:4117-4213) */
+/* sauter_diameter and average_density_solids: src/constitutive_2d.f90:387-416 */
+static double sauter_diameter(const oracle_t *o, const double *al)
+{
+    double s = 0.0, t = 0.0;
+    for (int i = 0; i < o->ns; i++) s += al[i];
+    if (s < EPS) {
+        for (int i = 0; i < o->ns; i++) t += o->diam_s[i];
+        return t / (double)o->ns;
+    }
+    for (int i = 0; i < o->ns; i++) t += al[i] / o->diam_s[i];
+    return s / t;
+}
+
+static double average_density_solids(const oracle_t *o, const double *al)
+{
+    double s = 0.0, t = 0.0;
+    for (int i = 0; i < o->ns; i++) s += al[i];
+    if (s < EPS) {
+        for (int i = 0; i < o->ns; i++) t += o->rho_s[i];
+        return t / (double)o->ns;
+    }
+    for (int i = 0; i < o->ns; i++) t += al[i] * o->rho_s[i];
+    return t / s;
+}
+
+/* eval_nh_semi_impl_terms: src/constitutive_2d.f90:4027-4547; Voellmy
+ * Coulomb part (model 1, :4164-4213) and the mu(I) friction of models 11
+ * and 12 (:4384-4537, the two branches compute the same terms) */
 static void eval_nh_semi_impl_terms(const oracle_t *o, double G, const double *qp, double *si)
 {
     const int nv = o->nv;
@@ -493,18 +522,44 @@ static void eval_nh_semi_impl_terms(const oracle_t *o, double G, const double *q
     mixt_var(o, qp, &Ri, &rm, &rc, &rg, 0, &c1, &c2);
     double w = 0.0;
     double mod_vel = sqrt(u * u + v * v + w * w);
-    if (mod_vel > 0.0) {
+    if (o->rheology == 1) {
+        if (mod_vel > 0.0) {
+            double centr = 0.0;
+            double tt = o->mu * rm * (rg + centr) * h * sqrt(G);
+            tt = fmax_f(0.0, tt);
+            tt = fmax_f(0.0, tt);
+            si[1] = si[1] - tt * u / mod_vel;
+            si[2] = si[2] - tt * v / mod_vel;
+        }
+    } else if (mod_vel > 0.0) {
+        double diam, rhop;
+        if (o->ns > 1) {
+            double al[3];
+            for (int i = 0; i < o->ns; i++) al[i] = qp[4 + i] / qp[0];
+            diam = sauter_diameter(o, al);
+            rhop = average_density_solids(o, al);
+        } else {
+            diam = o->diam_s[0];
+            rhop = o->rho_s[0];
+        }
+        if (h < EPS) h = EPS;
         double centr = 0.0;
-        double tt = o->mu * rm * (rg + centr) * h * sqrt(G);
-        tt = fmax_f(0.0, tt);
-        tt = fmax_f(0.0, tt);
-        si[1] = si[1] - tt * u / mod_vel;
-        si[2] = si[2] - tt * v / mod_vel;
+        double vse = rm * (rg + centr) * h;
+        vse = fmax_f(vse, 0.0);
+        double ens = fmax_f(0.0, vse * sqrt(G));
+        if (ens < EPS) ens = EPS;
+        double shear = 5.0 / 2.0 * mod_vel / (h * sqrt(G));
+        double In = diam * shear / sqrt(ens / rhop);  /* inertial number I */
+        double mu_I = (o->mu_s * o->I_0 + o->mu_2 * In + o->muI_inf * (In * In)) / (o->I_0 + In);
+        double tt = mu_I * ens;
+        si[1] = si[1] - tt * (u / mod_vel);
+        si[2] = si[2] - tt * (v / mod_vel);
     }
 }
 
 /* c_phys_var (gas, no liquid, no extra gas): src/constitutive_2d.f90:1154-1322;
- * eval_implicit_terms, Voellmy turbulent part: :3513-3683, :3994-4009 */
+ * eval_implicit_terms, Voellmy turbulent part (models 1 and 12; model 11 has
+ * no implicit friction term): :3513-3683, :3994-4009 */
 static void eval_implicit_terms_c(const oracle_t *o, const cx *qj, cx *nh)
 {
     const int ns = o->ns, nv = o->nv;
@@ -561,7 +616,7 @@ static void eval_implicit_terms_c(const oracle_t *o, const cx *qj, cx *nh)
     cx w = C(0.0, 0.0);
     cx mod_vel2 = cadd(cadd(csq(u), csq(v)), csq(w));
     cx mod_vel = csqrt_c(mod_vel2);
-    if (mod_vel.re != 0.0) {
+    if ((o->rheology == 1 || o->rheology == 12) && mod_vel.re != 0.0) {
         cx ts = cmul(cdiv(cmul(rho_m, red_grav), C(o->xi, 0.0)), mod_vel2);
         nh[1] = csub(nh[1], cmul(ts, cdiv(u, mod_vel)));
         nh[2] = csub(nh[2], cmul(ts, cdiv(v, mod_vel)));
@@ -1725,6 +1780,8 @@ oracle_t *oracle_create(const int *ip, const double *rp)
     for (int i = 0; i < 8; i++) { o->thk[i] = rp[46 + i]; o->pdyn[i] = rp[54 + i]; }
     o->erodible_deposit = ip[12];
     o->topo_change = ip[13];
+    o->rheology = (ip[14] == 0) ? 1 : ip[14];
+    o->mu_s = rp[69]; o->mu_2 = rp[70]; o->muI_inf = rp[71]; o->I_0 = rp[72];
     o->erosion_coeff = rp[63];
     if (o->erosion_coeff == 0.0) { /* src/inpout_2d.f90:1827-1832 */
         o->erodible_porosity = 0.0;

@@ -12,7 +12,7 @@ extern "C" {
 #endif
 
 /* Integer and real parameter vectors share the layout of oracle/refpy.py
- * pack_params(): ip[12], rp[63]. */
+ * pack_params(): ip[15], rp[73]. */
 typedef struct oracle_s oracle_t;
 
 oracle_t *oracle_create(const int *ip, const double *rp);

@@ -23,6 +23,8 @@ import math
 
 import numpy as np
 
+EPS = 2.220446049250313e-16          # EPSILON(1.0_wp)
+
 
 def module_qp(cfg, q, qc_to_qp):
     """The driver's qp array from q ([k, j, var]): qc_to_qp where q(1) > 0,
@@ -274,10 +276,12 @@ def zero_tiny(a):
 def p2d_records(cfg, q, deposit, B, hmax, pdynmax, mod_vel_max, oracle, erosion=None,
                 erodible=None):
     """output_solution's .p_2d columns per cell (src/inpout_2d.f90:6169-6396)
-    for the configured path (Voellmy rheology, model 1; no added gas, liquid,
-    stochastic or pore-pressure variables; without erosion the erosion and
-    erodible columns are 0).  The reference's mu_eff reads pore_pres(1) of
-    a zero-size array; here the excess pore pressure is 0.  q is [k, j, var],
+    for the configured path (no added gas, liquid, stochastic or
+    pore-pressure variables; without erosion the erosion and erodible columns
+    are 0).  Voellmy rheology (model 1): the reference's mu_eff reads
+    pore_pres(1) of a zero-size array, here the excess pore pressure is 0.
+    mu(I) models 11 and 12: mu_eff and the inertial number of :6277-6335,
+    the thickness floored at EPSILON when moving.  q is [k, j, var],
     deposit, erosion and erodible [class, k, j]; returns ([N, 19 + 4 n_solid]
     records, the zeroed deposit, the zeroed erosion)."""
     ny, nx, nv = q.shape
@@ -299,12 +303,48 @@ def p2d_records(cfg, q, deposit, B, hmax, pdynmax, mod_vel_max, oracle, erosion=
         j, k = c % nx, c // nx
         h, u, v, T = qp[c, 0], qp[c, nv], qp[c, nv + 1], qp[c, 3]
         Ri, rho_m, rg = mx[c, 0], mx[c, 1], mx[c, 3]
+        inertial = 0.0
         if h > 0.0:
             al = [qp[c, 4 + i] / h for i in range(ns)]
-            excess = 0.0                       # MAX(0, pore_pres(1) - pres) with pore_pres(1) = pres
-            with np.errstate(divide="ignore", invalid="ignore"):
-                d = float(1.0 - np.float64(excess) / np.float64(rho_m * h * rg))
-            mu_eff = cfg.mu * (0.0 if 0.0 > d else d)   # Fortran MAX(0.0, d)
+            if cfg.rheology_model == 1:
+                excess = 0.0                   # MAX(0, pore_pres(1) - pres) with pore_pres(1) = pres
+                with np.errstate(divide="ignore", invalid="ignore"):
+                    d = float(1.0 - np.float64(excess) / np.float64(rho_m * h * rg))
+                mu_eff = cfg.mu * (0.0 if 0.0 > d else d)   # Fortran MAX(0.0, d)
+            else:
+                mhv = math.sqrt(u * u + v * v)
+                if mhv < EPS:
+                    mu_eff = cfg.mu_s
+                else:
+                    if h < EPS:
+                        h = EPS
+                    vse = rho_m * rg * h
+                    if ns > 1:                 # sauter_diameter, average_density_solids (:387-416)
+                        sa = 0.0
+                        for a in al:
+                            sa = sa + a
+                        t = r = 0.0
+                        for i in range(ns):
+                            t = t + al[i] / cfg.diam_s[i]
+                            r = r + al[i] * cfg.rho_s[i]
+                        if sa < EPS:
+                            t = r = 0.0
+                            for i in range(ns):
+                                t = t + cfg.diam_s[i]
+                                r = r + cfg.rho_s[i]
+                            diam, rhop = t / ns, r / ns
+                        else:
+                            diam, rhop = sa / t, r / sa
+                    else:
+                        diam, rhop = cfg.diam_s[0], cfg.rho_s[0]
+                    shear = 5.0 / 2.0 * mhv / h
+                    ens = vse * math.sqrt(1.0)
+                    ens = ens if ens > 0.0 else 0.0      # MAX(0.0, .)
+                    if ens < EPS:
+                        ens = EPS
+                    inertial = diam * shear / math.sqrt(ens / rhop)
+                    mu_eff = (cfg.mu_s * cfg.I_0 + cfg.mu_2 * inertial
+                              + cfg.muI_inf * (inertial * inertial)) / (cfg.I_0 + inertial)
         else:
             al = [0.0] * ns
             mu_eff = 0.0
@@ -314,13 +354,14 @@ def p2d_records(cfg, q, deposit, B, hmax, pdynmax, mod_vel_max, oracle, erosion=
         for i in range(ns):
             dep[i, c] = z(dep[i, c])
             ero[i, c] = z(ero[i, c])
-        T, rho_m, rg, mu_eff = z(T), z(rho_m), z(rg), z(mu_eff)
+        T, rho_m, rg, mu_eff, inertial = z(T), z(rho_m), z(rg), z(mu_eff), z(inertial)
         esum = 0.0
         for i in range(ns):
             esum = esum + erd[i, c]
         row = [x_comp[j], y_comp[k], h, u, v, Bo, h + Bo] + al + [T, rho_m, rg] \
             + [dep[i, c] for i in range(ns)] + [ero[i, c] for i in range(ns)] \
             + [esum / (1.0 - por), 0.0, 0.0, Ri] + [0.0] * ns \
-            + [mu_eff, hmax.reshape(N)[c], pdynmax.reshape(N)[c], mod_vel_max.reshape(N)[c], 0.0]
+            + [mu_eff, hmax.reshape(N)[c], pdynmax.reshape(N)[c], mod_vel_max.reshape(N)[c],
+               inertial]
         out[c] = row
     return out, dep.reshape(deposit.shape), ero.reshape((ns, ny, nx))

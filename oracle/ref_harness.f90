@@ -51,7 +51,8 @@ MODULE imexref_harness
        sp_gas_const_g, loss_rate, init_problem_param, qc_to_qp, qp_to_qc,        &
        mixt_var, eval_fluxes, eval_local_speeds_x, eval_local_speeds_y,          &
        eval_expl_terms, eval_nh_semi_impl_terms, eval_implicit_terms,            &
-       settling_velocity, eval_mass_exchange_terms, sphericity_s
+       settling_velocity, eval_mass_exchange_terms, sphericity_s, mu_s, mu_2,   &
+       muI_inf, I_0
   USE geometry_2d, ONLY : comp_cells_x, comp_cells_y, x0, y0, cell_size,        &
        topography_profile, nodata_topo, n_topography_profile_x,                  &
        n_topography_profile_y, init_grid, init_source, deposit, erosion,         &
@@ -72,13 +73,13 @@ CONTAINS
 
   !----------------------------------------------------------------------------
   ! Set up the reference modules for one run.
-  !   ip(0:13)  integer parameters, rp(0:68) real parameters (layout in
+  !   ip(0:14)  integer parameters, rp(0:72) real parameters (layout in
   !   oracle/refpy.py), bdem(nx*ny) bed elevation at the cell centres
   !   (x index fastest).
   !----------------------------------------------------------------------------
   INTEGER(c_int) FUNCTION imexref_setup(ip, rp, bdem) BIND(C, name='imexref_setup')
-    INTEGER(c_int), INTENT(IN) :: ip(0:13)
-    REAL(c_double), INTENT(IN) :: rp(0:68)
+    INTEGER(c_int), INTENT(IN) :: ip(0:14)
+    REAL(c_double), INTENT(IN) :: rp(0:72)
     REAL(c_double), INTENT(IN) :: bdem(*)
 
     INTEGER :: j, k, i, nx, ny
@@ -102,7 +103,9 @@ CONTAINS
     gas_flag = .TRUE.
     liquid_flag = .FALSE.
     rheology_flag = .TRUE.
-    rheology_model = 1
+    ! 1 = Voellmy-Salm, 11 = mu(I), 12 = mu(I) + Voellmy turbulent term
+    rheology_model = ip(14)
+    IF ( rheology_model .EQ. 0 ) rheology_model = 1
     energy_flag = ( ip(3) .NE. 0 )
     alpha_flag = .FALSE.
     slope_correction_flag = .FALSE.
@@ -187,6 +190,10 @@ CONTAINS
     inv_grav = 1.0_wp / grav
     mu = rp(9)
     xi = rp(10)
+    mu_s = rp(69)
+    mu_2 = rp(70)
+    muI_inf = rp(71)
+    I_0 = rp(72)
     pres = rp(11)
     inv_pres = 1.0_wp / pres
     T_ambient = rp(12)

@@ -50,9 +50,9 @@ def _ip(a):
 
 
 def pack_params(cfg):
-    """ip[14] / rp[69] parameter vectors read by imexref_setup (and by the
+    """ip[15] / rp[73] parameter vectors read by imexref_setup (and by the
     C restatement's oracle_create)."""
-    ip = np.zeros(14, dtype=np.int32)
+    ip = np.zeros(15, dtype=np.int32)
     ip[0] = cfg.nx
     ip[1] = cfg.ny
     ip[2] = cfg.n_solid
@@ -67,7 +67,8 @@ def pack_params(cfg):
     ip[11] = cfg.bc_flag
     ip[12] = int(getattr(cfg, "erodible_deposit_flag", False))
     ip[13] = int(getattr(cfg, "topo_change_flag", False))
-    rp = np.zeros(69, dtype=np.float64)
+    ip[14] = getattr(cfg, "rheology_model", 1)
+    rp = np.zeros(73, dtype=np.float64)
     rp[0:16] = [cfg.dx, cfg.x0, cfg.y0, cfg.cfl, cfg.dt0, cfg.max_dt, cfg.t_start,
                 cfg.t_end, cfg.grav, cfg.mu, cfg.xi, cfg.pres, cfg.T_ambient,
                 cfg.sp_heat_a, cfg.sp_gas_const_a, cfg.kin_visc_a]
@@ -92,6 +93,8 @@ def pack_params(cfg):
     rp[65] = getattr(cfg, "T_erodible", 300.0)
     for i, v in enumerate(cfg.erodible_fractions()):
         rp[66 + i] = v
+    rp[69:73] = [getattr(cfg, "mu_s", 0.0), getattr(cfg, "mu_2", 0.0),
+                 getattr(cfg, "muI_inf", 0.0), getattr(cfg, "I_0", 0.0)]
     return ip, rp
 
 

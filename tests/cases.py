@@ -54,5 +54,24 @@ def c2e_64():
     return _erosion(cfg, [0.7, 0.3]), B
 
 
+def c1m_64():
+    """c1_64 with the mu(I) rheology (model 11) and 5 cm particles, so the
+    inertial number spans the regularised range."""
+    cfg, B = c1_64()
+    cfg.rheology_model = 11
+    cfg.mu_s, cfg.mu_2, cfg.muI_inf, cfg.I_0 = 0.12, 0.45, 0.05, 0.279
+    cfg.diam_s = [0.05]
+    return cfg, B
+
+
+def c2m_64():
+    """c2_64 with mu(I) plus the Voellmy turbulent term (model 12); two
+    classes, so the Sauter diameter and the mean particle density enter."""
+    cfg, B = c2_64()
+    cfg.rheology_model = 12
+    cfg.mu_s, cfg.mu_2, cfg.muI_inf, cfg.I_0 = 0.08, 0.3, 0.05, 0.279
+    return cfg, B
+
+
 CASES = {"c1_48": c1_48, "c1_64": c1_64, "c2_64": c2_64, "c2_96": c2_96,
-         "c1e_48": c1e_48, "c2e_64": c2e_64}
+         "c1e_48": c1e_48, "c2e_64": c2e_64, "c1m_64": c1m_64, "c2m_64": c2m_64}

@@ -23,7 +23,8 @@ def _bufs(s, what):
     return {k: native.PinnedArray(N * s.snapshot_cols(k)) for k in native.SNAP_KINDS if what & k}
 
 
-@pytest.mark.parametrize("name, steps", [("c1_64", 200), ("c2_64", 200), ("c2e_64", 200)])
+@pytest.mark.parametrize("name, steps", [("c1_64", 200), ("c2_64", 200), ("c2e_64", 200),
+                                         ("c1m_64", 200), ("c2m_64", 200)])
 def test_snapshot_records_match_oracle(name, steps):
     print(f"snapshot {name} steps={steps}", flush=True)
     cfg, B = cases.CASES[name]()

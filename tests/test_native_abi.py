@@ -35,6 +35,29 @@ def test_binding_structs_match_header_sizes():
     assert ctypes.sizeof(native.ImexStatic) == 7 * ctypes.sizeof(ctypes.c_void_p)
 
 
+def test_config_layout_matches_compiled_header(tmp_path):
+    """sizeof and field offsets of imex_config as gcc lays out the header,
+    against the ctypes mirror."""
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    fields = [f[0] for f in native.ImexConfig._fields_]
+    src = tmp_path / "layout.c"
+    body = "".join(f'  printf("%zu\\n", offsetof(imex_config, {f}));\n' for f in fields)
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "imex_hip.h"\n'
+                   'int main(void) {\n  printf("%zu\\n", sizeof(imex_config));\n'
+                   + body + '  return 0;\n}\n')
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", inc, "-o", str(exe), str(src)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], check=True, capture_output=True,
+                                          text=True).stdout.split()]
+    want = [ctypes.sizeof(native.ImexConfig)] + [getattr(native.ImexConfig, f).offset
+                                                 for f in fields]
+    assert got == want
+
+
 @pytest.mark.parametrize("y, lo, hi", [(0.687, -8, 3), (4.65, -6, 0), (2.4, -12, 15),
                                        (None, -300, 300)])
 def test_host_pow_is_correctly_rounded(y, lo, hi):

@@ -21,7 +21,8 @@ def _states(cfg, run, rng, n_pert=2):
     return np.concatenate(out)
 
 
-@pytest.mark.parametrize("name, warm", [("c1_64", 150), ("c2_96", 200)])
+@pytest.mark.parametrize("name, warm", [("c1_64", 150), ("c2_96", 200), ("c1m_64", 150),
+                                        ("c2m_64", 200)])
 def test_functions_on_fresh_states(ref_lib, name, warm):
     cfg, B = cases.CASES[name]()
     run = ref_lib.RefRun(cfg, B)
@@ -53,7 +54,8 @@ def test_functions_on_fresh_states(ref_lib, name, warm):
     run.close()
 
 
-@pytest.mark.parametrize("name, steps", [("c1_64", 300), ("c2_96", 400)])
+@pytest.mark.parametrize("name, steps", [("c1_64", 300), ("c2_96", 400), ("c1m_64", 300),
+                                         ("c2m_64", 300)])
 def test_multistep_bit_exact(ref_lib, name, steps):
     cfg, B = cases.CASES[name]()
     run = ref_lib.RefRun(cfg, B)

@@ -137,3 +137,47 @@ def test_oracle_stops_at_t_end():
         o.run(1)
         n += 1
     assert o.t == T and n > 1
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("name, steps", [("c1m_64", 200), ("c2m_64", 200)])
+def test_p2d_mu_i_columns_against_reference_friction(ref_lib, name, steps):
+    """The .p_2d mu_eff column of the mu(I) models (src/inpout_2d.f90:
+    6277-6335) against the friction the reference's own semi-implicit terms
+    apply (src/constitutive_2d.f90:4384-4537): with no slope correction both
+    evaluate mu(I) of the same inertial number, so the momentum source is
+    -mu_eff * sigma_eff * u / |u|, sigma_eff = MAX(rho_m g' h, EPSILON).  At
+    rest mu_eff is mu_s and the inertial number 0."""
+    cfg, B = cases.CASES[name]()
+    run = ref_lib.RefRun(cfg, B)
+    try:
+        run.run(steps)
+        q = run.q()
+        N = cfg.nx * cfg.ny
+        qc = q.reshape(N, cfg.n_vars)
+        qp, _ = run.qc_to_qp(qc)
+        si = run.semi_impl(qc, qp)
+        mx = run.mixt_var(qp)
+    finally:
+        run.close()
+    o = oraclepy.Oracle(cfg)
+    z = np.zeros((cfg.n_solid, cfg.ny, cfg.nx))
+    rec, _, _ = outputs_ref.p2d_records(cfg, q, z, np.zeros((cfg.ny, cfg.nx)), z[0], z[0],
+                                        z[0], o)
+    mu_eff, inertial = rec[:, -5], rec[:, -1]
+    nv = cfg.n_vars
+    h, u, v = qp[:, 0], qp[:, nv], qp[:, nv + 1]
+    speed = np.sqrt(u * u + v * v)
+    eps = np.finfo(np.float64).eps
+    ens = np.maximum(mx[:, 1] * mx[:, 3] * np.maximum(h, eps), eps)
+    sel = (h > 1e-3) & (np.abs(u) > 0.1 * speed) & (speed > 1e-8)
+    assert sel.sum() > 100
+    mu_ref = -si[sel, 1] / (u[sel] / speed[sel]) / ens[sel]
+    np.testing.assert_allclose(mu_eff[sel], mu_ref, rtol=1e-12, atol=0)
+    # mu(I) rises from mu_s with the inertial number; the runs reach I well
+    # above I_0 / 1000
+    assert (mu_eff[sel] > cfg.mu_s).all() and inertial[sel].max() > 1e-4 * cfg.I_0
+    rest = (h > 0) & (speed < eps)
+    np.testing.assert_array_equal(mu_eff[rest], cfg.mu_s)
+    np.testing.assert_array_equal(inertial[rest], 0.0)
+    assert (mu_eff[h <= 0] == 0).all() and (inertial[h <= 0] == 0).all()
