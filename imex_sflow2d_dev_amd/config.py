@@ -90,6 +90,18 @@ class RunConfig:
     mu_2: float = -1.0
     muI_inf: float = -1.0
     I_0: float = -1.0
+    # stochastic friction (STOCHASTIC_PARAMETERS, defaults src/inpout_2d.f90:
+    # 526-533): a per-cell value Z shifts xi (models 1 and 12); the noise is
+    # not transported.  Z is 0 unless given (the reference's value outside
+    # an ASCII restart); stochastic.steady_state_noise gives the burn-in of
+    # getSteadyStateZ from std_max, tau_stochastic, sym_noise, noise_pow_val
+    # and length_spatial_corr
+    stochastic_flag: bool = False
+    std_max: float = -1.0
+    tau_stochastic: float = -1.0
+    sym_noise: float = 0.0
+    noise_pow_val: float = -1.0
+    length_spatial_corr: float = -1.0
     # vulnerability table (VULNERABILITY_TABLE_PARAMETERS)
     thickness_levels: List[float] = dataclasses.field(default_factory=list)
     dyn_pres_levels: List[float] = dataclasses.field(default_factory=list)

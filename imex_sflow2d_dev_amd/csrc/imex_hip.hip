@@ -897,16 +897,16 @@ __device__ __forceinline__ double driver_dt(const Params& P, const DevState* st)
 // assembly checks (:1342-1630)
 // ===========================================================================
 template <int NS>
-__device__ void eval_f(const Params& P, double dt, const double* qj, const double* q0,
+__device__ void eval_f(const Params& P, double xi, double dt, const double* qj, const double* q0,
                        const double* Rni, double* f, double& scal) {
   constexpr int NV = 4 + NS;
   double n1, n2;
-  if (!implicit_r<NS>(P, qj, n1, n2)) {
+  if (!implicit_r<NS>(P, qj, n1, n2, xi)) {
     cx qc[NV];
 #pragma unroll
     for (int i = 0; i < NV; i++) qc[i] = C(qj[i], 0.0);
     cx c1, c2;
-    implicit_c<NS>(P, qc, c1, c2);
+    implicit_c<NS>(P, qc, c1, c2, xi);
     n1 = c1.re;
     n2 = c2.re;
   }
@@ -928,7 +928,8 @@ __device__ void eval_f(const Params& P, double dt, const double* qj, const doubl
 // aligned pair evaluates column 1 + lane and the pair swaps the entries, so
 // the two complex evaluations run side by side; both lanes call together.
 template <int NS, int LANES>
-__device__ void eval_jacobian(const Params& P, double dt, const double* qrel, const double* qorg,
+__device__ void eval_jacobian(const Params& P, double xi, double dt, const double* qrel,
+                              const double* qorg,
                               double& L11, double& L21, double& L12, double& L22, int lane) {
   constexpr int NV = 4 + NS;
 #pragma unroll
@@ -939,7 +940,7 @@ __device__ void eval_jacobian(const Params& P, double dt, const double* qrel, co
     for (int m = 0; m < NV; m++)
       qc[m] = cmul(C(qrel[m], (m == col) ? P.hcs : 0.0), C(qorg[m], 0.0));
     cx n1, n2;
-    implicit_c<NS>(P, qc, n1, n2);
+    implicit_c<NS>(P, qc, n1, n2, xi);
     const double J1 = 1.0 * n1.im * P.one_by_hcs;
     const double J2 = 1.0 * n2.im * P.one_by_hcs;
     if (col == 1) {
@@ -988,7 +989,7 @@ __device__ __forceinline__ double Lrc(int r, int c, const double* qorg, double L
 // the second only when its alam is the one those rules give.  Extra
 // evaluations past the deciding trial are discarded.
 template <int NS, int LANES>
-__device__ bool lnsrch(const Params& P, double dt, const double* xold, const double* qorg,
+__device__ bool lnsrch(const Params& P, double xi, double dt, const double* xold, const double* qorg,
                        const double* q0, double fold, const double* g, double* p, double* x,
                        double& f_out, double* rt, double stpmax, const double* Rni,
                        unsigned int& err, int lane) {
@@ -1039,7 +1040,7 @@ __device__ bool lnsrch(const Params& P, double dt, const double* xold, const dou
     for (int i = 0; i < NV; i++) xt[i] = xold[i] + a * p[i];
 #pragma unroll
     for (int i = 0; i < NV; i++) qj[i] = xt[i] * qorg[i];
-    eval_f<NS>(P, dt, qj, q0, Rni, rtt, ft);
+    eval_f<NS>(P, xi, dt, qj, q0, Rni, rtt, ft);
     double next = base;
     int decided = -1;
     bool accept = false;
@@ -1089,7 +1090,7 @@ __device__ bool lnsrch(const Params& P, double dt, const double* xold, const dou
 }
 
 template <int NS, int LANES>
-__device__ int solve_rk_step(const Params& P, double dt, double* qj, const double* q0,
+__device__ int solve_rk_step(const Params& P, double xi, double dt, double* qj, const double* q0,
                              const double* Rni, unsigned int& err, int lane) {
   constexpr int NV = 4 + NS;
   const double TOLF = 1.0e-10, STPMX = 100.0;
@@ -1105,12 +1106,12 @@ __device__ int solve_rk_step(const Params& P, double dt, double* qj, const doubl
   int it;
   bool fresh = false;  // rt and scal_f already hold the residual at qj
   for (it = 1; it <= 100; it++) {
-    if (!fresh) eval_f<NS>(P, dt, qj, q0, Rni, rt, scal_f);
+    if (!fresh) eval_f<NS>(P, xi, dt, qj, q0, Rni, rt, scal_f);
     double mx = 0.0;
 #pragma unroll
     for (int i = 0; i < NV; i++) mx = fmax_f(mx, fabs(rt[i]));
     if (mx < TOLF) break;
-    eval_jacobian<NS, LANES>(P, dt, qrel, qorg, L11, L21, L12, L22, lane);
+    eval_jacobian<NS, LANES>(P, xi, dt, qrel, qorg, L11, L21, L12, L22, lane);
     double d1[NV - 2];
     int ne = 0;
 #pragma unroll
@@ -1159,14 +1160,14 @@ __device__ int solve_rk_step(const Params& P, double dt, double* qj, const doubl
         for (int r = 0; r < NV; r++) s += rt[r] * Lrc(r, i, qorg, L11, L21, L12, L22);
         grad[i] = s;
       }
-      fresh = lnsrch<NS, LANES>(P, dt, xold, qorg, q0, scal_f_old, grad, p, qrel, scal_f, rt,
+      fresh = lnsrch<NS, LANES>(P, xi, dt, xold, qorg, q0, scal_f_old, grad, p, qrel, scal_f, rt,
                                 stpmax, Rni, err, lane);
     } else {
 #pragma unroll
       for (int i = 0; i < NV; i++) qrel[i] = xold[i] + p[i];
 #pragma unroll
       for (int i = 0; i < NV; i++) qj[i] = qrel[i] * qorg[i];
-      eval_f<NS>(P, dt, qj, q0, Rni, rt, scal_f);
+      eval_f<NS>(P, xi, dt, qj, q0, Rni, rt, scal_f);
       fresh = true;
     }
 #pragma unroll
@@ -1188,7 +1189,7 @@ __device__ int solve_rk_step(const Params& P, double dt, double* qj, const doubl
 
 // stage 2 + assembly for one cell; q holds q0 on entry, the new state on exit
 template <int NS, int LANES>
-__device__ void stage2(const Params& P, double dt, double* q, const double* X,
+__device__ void stage2(const Params& P, double xi, double dt, double* q, const double* X,
                        unsigned int& err, int lane) {
   constexpr int NV = 4 + NS, NP = NV + 2;
   double q0[NV], qfv[NV], qp[NP], q_si[NV], qg[NV], si[NV], Rj[NV], pd;
@@ -1232,7 +1233,7 @@ __device__ void stage2(const Params& P, double dt, double* q, const double* X,
       const double B = 0.0 + (0.0 + 0.0) * 0.0;
       Rj[i] = (A - B) - 1.0 * si[i];
     }
-    solve_rk_step<NS, LANES>(P, dt, qg, q0, Rj, err, lane);
+    solve_rk_step<NS, LANES>(P, xi, dt, qg, q0, Rj, err, lane);
     if (q_si[1] * q_si[1] + q_si[2] * q_si[2] == 0.0) {
       qg[1] = 0.0;
       qg[2] = 0.0;
@@ -1461,7 +1462,7 @@ __device__ __forceinline__ void update_body(const Params& P, const Grid& G, doub
       double X[NV];
 #pragma unroll
       for (int v = 0; v < NV; v++) X[v] = (double)state_R<S>(G)[(size_t)v * N + c];
-      stage2<NS, LANES>(P, dt, q, X, err, lane);
+      stage2<NS, LANES>(P, cell_xi(P, G.noise, c), dt, q, X, err, lane);
     }
     if constexpr ((PARTS & 2) != 0) {
       if (P.settling || P.entrainment || P.erosion_coeff > 0.0) {
@@ -1620,7 +1621,7 @@ __global__ void kt_implicit_cplx(Params P, int n, const double* qre, const doubl
     cx q[NV];
     for (int m = 0; m < NV; m++) q[m] = C(qre[NV * i + m], qim[NV * i + m]);
     cx n1, n2;
-    implicit_c<NS>(P, q, n1, n2);
+    implicit_c<NS>(P, q, n1, n2, P.xi);
     for (int m = 0; m < NV; m++) {
       ore[NV * i + m] = (m == 1) ? n1.re : ((m == 2) ? n2.re : 0.0);
       oim[NV * i + m] = (m == 1) ? n1.im : ((m == 2) ? n2.im : 0.0);
@@ -1635,12 +1636,12 @@ __global__ void kt_implicit_real(Params P, int n, const double* q, double* out, 
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
     double n1, n2;
-    const bool ok = implicit_r<NS>(P, q + (size_t)NV * i, n1, n2);
+    const bool ok = implicit_r<NS>(P, q + (size_t)NV * i, n1, n2, P.xi);
     if (!ok) {
       cx qc[NV];
       for (int m = 0; m < NV; m++) qc[m] = C(q[(size_t)NV * i + m], 0.0);
       cx c1, c2;
-      implicit_c<NS>(P, qc, c1, c2);
+      implicit_c<NS>(P, qc, c1, c2, P.xi);
       n1 = c1.re;
       n2 = c2.re;
     }
@@ -1689,7 +1690,7 @@ __global__ void kt_newton(Params P, int n, double dt, const double* qsi, const d
     double qq[NV];
     for (int m = 0; m < NV; m++) qq[m] = qsi[NV * i + m];
     unsigned int err = 0;
-    its[i] = solve_rk_step<NS, 1>(P, dt, qq, q0 + (size_t)NV * i, R + (size_t)NV * i, err, 0);
+    its[i] = solve_rk_step<NS, 1>(P, P.xi, dt, qq, q0 + (size_t)NV * i, R + (size_t)NV * i, err, 0);
     for (int m = 0; m < NV; m++) q[NV * i + m] = qq[m];
   }
 }
@@ -1867,6 +1868,7 @@ extern "C" int imex_create(const imex_config* cfg, imex_handle** out) {
   }
   P.topo_change = cfg->topo_change_flag;
   P.rheology = rheo;
+  P.stochastic = cfg->stochastic_flag != 0;
   P.mu_s = cfg->mu_s;
   P.mu_2 = cfg->mu_2;
   P.muI_inf = cfg->muI_inf;
@@ -1913,6 +1915,8 @@ extern "C" int imex_create(const imex_config* cfg, imex_handle** out) {
   }
   if (P.topo_change)
     if ((rc = dalloc(h, &G.bcent, N))) return rc;
+  if (P.stochastic)  // Z = 0 until imex_set_noise (the reference's allocation value)
+    if ((rc = dalloc(h, &G.noise, N))) return rc;
   if (P.n_thk * P.n_pdyn > 0)
     if ((rc = dalloc(h, &G.vuln, N))) return rc;
   if ((rc = dalloc(h, &G.mask, N))) return rc;
@@ -2758,6 +2762,18 @@ extern "C" int imex_set_erodible(imex_handle* h, const double* erodible) {
   for (size_t c = 0; c < N; c++)
     for (int i = 0; i < ns; i++) t[(size_t)i * N + c] = erodible[c * ns + i];
   HCHK(h, hipMemcpyAsync(h->G.erodible, t.data(), 8 * N * ns, hipMemcpyHostToDevice, h->stream));
+  HCHK(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+extern "C" int imex_set_noise(imex_handle* h, const double* Z) {
+  if (!h || !Z) return -2;
+  if (!h->G.noise) {
+    h->err = "imex_set_noise: the run has no stochastic friction (stochastic_flag = 0)";
+    return -2;
+  }
+  HCHK(h, hipSetDevice(h->device));
+  HCHK(h, hipMemcpyAsync(h->G.noise, Z, 8 * (size_t)h->G.N, hipMemcpyHostToDevice, h->stream));
   HCHK(h, hipStreamSynchronize(h->stream));
   return 0;
 }

@@ -88,6 +88,7 @@ struct Grid {
   double* bx;                // [N] B_prime_x_geom (recomputed each step with topography change)
   double* by;                // [N] B_prime_y_geom
   double* bcent;             // [N] B_cent, once the caller set it (outputs, topography change)
+  double* noise;             // [N] stochastic value Z of each cell (stochastic friction only)
   const signed char* scell;  // [N] source_cell
   const unsigned char* sides;  // [N] sourceE/W/N/S bits
   const double* arcp;        // [N]

@@ -42,6 +42,8 @@ struct Params {
   int topo_change;  // B_cent follows erosion and deposition (topo_change_flag)
   // friction law: 1 Voellmy-Salm, 11 mu(I), 12 mu(I) + Voellmy turbulent term
   int rheology;
+  // stochastic friction: a per-cell value Z shifts xi (stochastic_flag)
+  int stochastic;
   double mu_s, mu_2, muI_inf, I_0;
   double vel_source, T_source, h_source, time_param[4], pi_g;
   double thk[8], pdyn[8];
@@ -568,11 +570,22 @@ __device__ inline void semi_impl(const Params& P, const double* qp, double* si) 
   }
 }
 
+// xi_temp of eval_implicit_terms for a cell with stochastic value Z: xi + Z
+// limited to [1, 1e5] with the stochastic friction, xi without
+// (src/constitutive_2d.f90:3663-3673)
+__device__ __forceinline__ double cell_xi(const Params& P, const double* Z, int c) {
+  if (!P.stochastic) return P.xi;
+  double x = P.xi + Z[c];
+  if (x < 1.0) x = 1.0;
+  if (x > 1e5) x = 1e5;
+  return x;
+}
+
 // c_phys_var + eval_implicit_terms (Voellmy turbulent part) in complex
 // arithmetic: src/constitutive_2d.f90:1154-1322, :3513-3683, :3994-4009.
 // Only the momentum rows are non-zero; they are returned in nh1, nh2.
 template <int NS>
-__device__ inline void implicit_c(const Params& P, const cx* qj, cx& nh1, cx& nh2) {
+__device__ inline void implicit_c(const Params& P, const cx* qj, cx& nh1, cx& nh2, double xi) {
   nh1 = C(0.0, 0.0);
   nh2 = C(0.0, 0.0);
   // mu(I) model 11 has no implicit friction term (:3660 takes models 1, 12)
@@ -631,7 +644,7 @@ __device__ inline void implicit_c(const Params& P, const cx* qj, cx& nh1, cx& nh
   cx mod_vel2 = cadd(cadd(csq(u), csq(v)), csq(w));
   cx mod_vel = csqrt_c(mod_vel2);
   if (mod_vel.re != 0.0) {
-    cx ts = cmul(cdiv(cmul(rho_m, red_grav), C(P.xi, 0.0)), mod_vel2);
+    cx ts = cmul(cdiv(cmul(rho_m, red_grav), C(xi, 0.0)), mod_vel2);
     nh1 = csub(nh1, cmul(ts, cdiv(u, mod_vel)));
     nh2 = csub(nh2, cmul(ts, cdiv(v, mod_vel)));
   }
@@ -666,7 +679,8 @@ __device__ __forceinline__ double cdiv_re(double a, double c, bool& ok) {
   return x;
 }
 template <int NS>
-__device__ inline bool implicit_r(const Params& P, const double* q, double& nh1, double& nh2) {
+__device__ inline bool implicit_r(const Params& P, const double* q, double& nh1, double& nh2,
+                                  double xi) {
   nh1 = 0.0;
   nh2 = 0.0;
   if (P.rheology == 11) return true;  // no implicit friction term
@@ -740,7 +754,7 @@ __device__ inline bool implicit_r(const Params& P, const double* q, double& nh1,
   ok = ok & nz_finite(mod_vel);
   const double rg = rho_m * red_grav;
   ok = ok & nz_finite(rg);
-  const double ts = cdiv_re(rg, P.xi, ok) * mod_vel2;
+  const double ts = cdiv_re(rg, xi, ok) * mod_vel2;
   ok = ok & nz_finite(ts);
   const double p1 = ts * cdiv_re(u, mod_vel, ok), p2 = ts * cdiv_re(v, mod_vel, ok);
   ok = ok & nz_finite(p1) & nz_finite(p2);

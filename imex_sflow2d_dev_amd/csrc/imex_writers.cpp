@@ -212,7 +212,8 @@ extern "C" int imex_write_restart(const char* path, int nx, int ny, int n_vars, 
                                   double t, double dt, const double* q_aos, const double* B_cent,
                                   const double* erodible, const double* deposit,
                                   const double* erosion, const double* hmax,
-                                  const double* pdynmax, const double* mod_vel_max) {
+                                  const double* pdynmax, const double* mod_vel_max,
+                                  const double* Z) {
   if (!path || !q_aos || !B_cent || !erodible || !deposit || !erosion || !hmax || !pdynmax ||
       !mod_vel_max || nx < 1 || ny < 1)
     return -2;
@@ -232,6 +233,7 @@ extern "C" int imex_write_restart(const char* path, int nx, int ny, int n_vars, 
   if (!r) r = fwrite_record(f, hmax, 8 * N);
   if (!r) r = fwrite_record(f, pdynmax, 8 * N);
   if (!r) r = fwrite_record(f, mod_vel_max, 8 * N);
+  if (!r && Z) r = fwrite_record(f, Z, 8 * N);  // stochastic_flag runs (:8078-8080)
   if (std::fclose(f)) r = -1;
   return r;
 }

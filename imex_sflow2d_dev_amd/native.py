@@ -43,7 +43,7 @@ EXPORTS = [
     "imex_set_topography", "imex_runout_reduce", "imex_probes",
     "imex_snapshot_cols", "imex_snapshot_begin", "imex_snapshot_wait", "imex_host_alloc",
     "imex_host_free", "imex_write_records", "imex_write_restart", "imex_format_value",
-    "imex_set_erodible", "imex_get_erosion", "imex_test_mass_exchange_ers",
+    "imex_set_erodible", "imex_get_erosion", "imex_set_noise", "imex_test_mass_exchange_ers",
     "imex_topography_reconstruction", "imex_get_topography",
 ]
 
@@ -77,7 +77,7 @@ class ImexConfig(ctypes.Structure):
         ("erodible_deposit_flag", ctypes.c_int32), ("topo_change_flag", ctypes.c_int32),
         ("rheology_model", ctypes.c_int32),
         ("mu_s", ctypes.c_double), ("mu_2", ctypes.c_double), ("muI_inf", ctypes.c_double),
-        ("I_0", ctypes.c_double),
+        ("I_0", ctypes.c_double), ("stochastic_flag", ctypes.c_int32),
     ]
 
 
@@ -130,7 +130,7 @@ def lib():
         L.imex_write_records.argtypes = [ctypes.c_char_p, _D, ctypes.c_int64, ctypes.c_int,
                                          ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.imex_write_restart.argtypes = [ctypes.c_char_p] + [ctypes.c_int] * 4 + \
-            [ctypes.c_double] * 2 + [_D] * 8
+            [ctypes.c_double] * 2 + [_D] * 9
         L.imex_format_value.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_char_p]
         _LIB = L
     return _LIB
@@ -177,6 +177,7 @@ def make_config(cfg: RunConfig) -> ImexConfig:
     c.topo_change_flag = int(cfg.topo_change_flag)
     c.rheology_model = cfg.rheology_model
     c.mu_s, c.mu_2, c.muI_inf, c.I_0 = cfg.mu_s, cfg.mu_2, cfg.muI_inf, cfg.I_0
+    c.stochastic_flag = int(cfg.stochastic_flag)
     return c
 
 
@@ -263,11 +264,14 @@ def write_records(path: str, rec: np.ndarray, nx: int, fmt: int, n_threads: int 
 
 
 def write_restart(path: str, nx, ny, n_vars, n_solid, t, dt, q_aos, B_cent, erodible, deposit,
-                  erosion, hmax, pdynmax, mod_vel_max):
+                  erosion, hmax, pdynmax, mod_vel_max, Z=None):
+    """restart.bin as write_restart_file writes it; Z (nx * ny) is the
+    stochastic_flag record, omitted when None."""
     arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in
             (q_aos, B_cent, erodible, deposit, erosion, hmax, pdynmax, mod_vel_max)]
+    z = None if Z is None else np.ascontiguousarray(Z, dtype=np.float64)
     rc = lib().imex_write_restart(path.encode(), nx, ny, n_vars, n_solid, t, dt,
-                                  *[_dp(a) for a in arrs])
+                                  *[_dp(a) for a in arrs], None if z is None else _dp(z))
     if rc != 0:
         raise NativeError(f"imex_write_restart failed ({rc}) for {path}")
 

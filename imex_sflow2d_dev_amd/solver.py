@@ -81,6 +81,8 @@ class Solver:
         if st.get("erodible") is not None and (self.cfg.erosion_coeff > 0.0
                                                or self.cfg.erodible_deposit_flag):
             self.set_erodible(st["erodible"])
+        if self.cfg.stochastic_flag and st.get("Z") is not None:
+            self.set_noise(st["Z"])
         if self.cfg.topo_change_flag:
             if st.get("B_cent") is None:
                 raise ValueError("topo_change_flag needs B_cent in the static fields")
@@ -91,6 +93,12 @@ class Solver:
         a = np.ascontiguousarray(np.transpose(np.asarray(e, dtype=np.float64), (1, 2, 0)))
         assert a.shape == (self.cfg.ny, self.cfg.nx, self.cfg.n_solid)
         self.H.check(self.L.imex_set_erodible(self.h, native._dp(a)), "imex_set_erodible")
+
+    def set_noise(self, Z: np.ndarray):
+        """The cells' stochastic values Z as [k, j] (stochastic_flag)."""
+        a = np.ascontiguousarray(Z, dtype=np.float64)
+        assert a.shape == (self.cfg.ny, self.cfg.nx)
+        self.H.check(self.L.imex_set_noise(self.h, native._dp(a)), "imex_set_noise")
 
     def topography(self) -> np.ndarray:
         """B_cent as [k, j] (evolves with topo_change_flag)."""

@@ -71,6 +71,11 @@ typedef struct imex_config {
    * (src/inpout_2d.f90:3592-3630). */
   int32_t rheology_model;
   double mu_s, mu_2, muI_inf, I_0;
+  /* stochastic friction (STOCHASTIC_PARAMETERS without transport of the
+   * noise): each cell's xi is xi + Z(j,k) limited to [1, 1e5] in the
+   * turbulent term of models 1 and 12 (src/constitutive_2d.f90:3663-3673);
+   * Z is 0 until imex_set_noise gives it */
+  int32_t stochastic_flag;
 } imex_config;
 
 /* Per-cell static fields computed once by the host (src/geometry_2d.f90:555-895
@@ -172,6 +177,12 @@ int imex_topography_reconstruction(imex_handle *h);
 /* The erodible layer erodible(n_solid, nx, ny) (Fortran order; read_erodible
  * and src/inpout_2d.f90:1982-1986) of a run with erosion. */
 int imex_set_erodible(imex_handle *h, const double *erodible);
+/* The stochastic value Z(nx, ny) of every cell (Fortran order, x fastest) of
+ * a run with stochastic_flag: the array solver_2d's Z holds after
+ * getSteadyStateZ (src/stochastic_module.f90:79-132) or read_restart_file
+ * (src/inpout_2d.f90:8153).  The value stays as given: without transport of
+ * the noise the reference does not update Z during the run. */
+int imex_set_noise(imex_handle *h, const double *Z);
 /* erosion(nx, ny, n_solid) and erodible(n_solid, nx, ny) (either may be
  * NULL); zeros for a run without erosion. */
 int imex_get_erosion(imex_handle *h, double *erosion, double *erodible);
@@ -272,7 +283,8 @@ int imex_host_free(void *p);
  * after every nx records and two at the end; n_threads formatting threads
  * (0: one per host core).  imex_write_restart writes restart.bin as
  * write_restart_file's unformatted sequential records (4-byte markers;
- * records of 2 GiB or more are rejected).  imex_format_value formats one
+ * records of 2 GiB or more are rejected; Z, when not NULL, is the
+ * stochastic_flag record).  imex_format_value formats one
  * value (fmt also IMEX_FMT_ES14_7, the probe files' ES14.7E2) into out
  * (at least 21 bytes). */
 #define IMEX_FMT_E20_12 0
@@ -283,7 +295,8 @@ int imex_write_records(const char *path, const double *rec, int64_t n_rows, int 
 int imex_write_restart(const char *path, int nx, int ny, int n_vars, int n_solid, double t,
                        double dt, const double *q_aos, const double *B_cent,
                        const double *erodible, const double *deposit, const double *erosion,
-                       const double *hmax, const double *pdynmax, const double *mod_vel_max);
+                       const double *hmax, const double *pdynmax, const double *mod_vel_max,
+                       const double *Z);
 int imex_format_value(int fmt, double x, char *out);
 
 /* ---- batches of independent runs -----------------------------------------

@@ -58,6 +58,9 @@ struct oracle_s {
     int topo_change;        /* topo_change_flag */
     int rheology;           /* rheology_model: 1, 11 or 12 */
     double mu_s, mu_2, muI_inf, I_0;  /* mu(I) parameters (RHEOLOGY_PARAMETERS) */
+    int stochastic;         /* stochastic_flag (no transport of the noise) */
+    double *Z;              /* solver_2d's Z(nx, ny): the cells' stochastic values */
+    double xi_cell;         /* xi_temp of the cell being solved (xi outside a step) */
     double *bcent;          /* B_cent (topography change) */
     double vel_source, T_source, h_source, time_param[4], pi_g;
     double thk[8], pdyn[8];
@@ -617,7 +620,7 @@ static void eval_implicit_terms_c(const oracle_t *o, const cx *qj, cx *nh)
     cx mod_vel2 = cadd(cadd(csq(u), csq(v)), csq(w));
     cx mod_vel = csqrt_c(mod_vel2);
     if ((o->rheology == 1 || o->rheology == 12) && mod_vel.re != 0.0) {
-        cx ts = cmul(cdiv(cmul(rho_m, red_grav), C(o->xi, 0.0)), mod_vel2);
+        cx ts = cmul(cdiv(cmul(rho_m, red_grav), C(o->xi_cell, 0.0)), mod_vel2);
         nh[1] = csub(nh[1], cmul(ts, cdiv(u, mod_vel)));
         nh[2] = csub(nh[2], cmul(ts, cdiv(v, mod_vel)));
     }
@@ -629,6 +632,17 @@ static void eval_implicit_terms_r(const oracle_t *o, const double *q, double *nh
     for (int i = 0; i < o->nv; i++) qj[i] = C(q[i], 0.0);
     eval_implicit_terms_c(o, qj, r);
     for (int i = 0; i < o->nv; i++) nh[i] = r[i].re;
+}
+
+/* xi_temp of eval_implicit_terms: xi + Z(j,k) limited to [1, 1e5] with the
+ * stochastic friction, xi without (src/constitutive_2d.f90:3663-3673) */
+static double cell_xi(const oracle_t *o, size_t c)
+{
+    if (!o->stochastic) return o->xi;
+    double x = o->xi + o->Z[c];
+    if (x < 1.0) x = 1.0;
+    if (x > 1e5) x = 1e5;
+    return x;
 }
 
 /* settling_velocity, Schiller-Naumann drag: src/constitutive_2d.f90:5326-5404 */
@@ -1467,6 +1481,7 @@ static void imex_RK_solver(oracle_t *o)
         for (int l = 0; l < o->ncells; l++) {
             const int j = o->jc[l], k = o->kc[l];
             const size_t c = CELL(o, j, k);
+            o->xi_cell = cell_xi(o, c);
             double q_guess[8], q_si[8], Rj[8], pd;
             double *q0 = o->q0 + nv * c, *qfv = o->q_fv + nv * c, *qpc = o->qp + np * c;
             double *SIs = stage ? SI1 + nv * c : SI0 + nv * c;
@@ -1595,6 +1610,7 @@ static void imex_RK_solver(oracle_t *o)
             for (int i = 0; i < ns; i++) qq[4 + i] = qq[4 + i] / S3 * qq[0];
         }
     }
+    o->xi_cell = o->xi;
 }
 
 /* update_erosion_deposition_cell: src/solver_2d.f90:2382-2597 */
@@ -1782,6 +1798,8 @@ oracle_t *oracle_create(const int *ip, const double *rp)
     o->topo_change = ip[13];
     o->rheology = (ip[14] == 0) ? 1 : ip[14];
     o->mu_s = rp[69]; o->mu_2 = rp[70]; o->muI_inf = rp[71]; o->I_0 = rp[72];
+    o->stochastic = ip[15];
+    o->xi_cell = o->xi;
     o->erosion_coeff = rp[63];
     if (o->erosion_coeff == 0.0) { /* src/inpout_2d.f90:1827-1832 */
         o->erodible_porosity = 0.0;
@@ -1832,6 +1850,7 @@ oracle_t *oracle_create(const int *ip, const double *rp)
     o->deposit = zalloc(8 * N * o->ns);
     o->erodible = zalloc(8 * N * o->ns);
     o->bcent = zalloc(8 * N);
+    o->Z = zalloc(8 * N);
     o->erosion = zalloc(8 * N * o->ns);
     o->vuln = zalloc(N * (size_t)(o->n_thk * o->n_pdyn > 0 ? o->n_thk * o->n_pdyn : 1));
     o->stL = zalloc(sizeof(int) * NFX); o->stR = zalloc(sizeof(int) * NFX);
@@ -1849,6 +1868,7 @@ oracle_t *oracle_create(const int *ip, const double *rp)
 void oracle_destroy(oracle_t *o)
 {
     if (!o) return;
+    free(o->Z);
     void *ptrs[] = {o->x_comp, o->x_stag, o->y_comp, o->y_stag, o->bx, o->by, o->arcp, o->arcx,
                     o->arcy, o->scell, o->sE, o->sW, o->sN, o->sS, o->q, o->qp, o->q0, o->q_fv,
                     o->q_rk, o->qp_rk, o->divF, o->NH, o->SI, o->expl, o->qL, o->qR, o->qB,
@@ -2060,6 +2080,11 @@ void oracle_get_bcent(const oracle_t *o, double *b)
 void oracle_set_erodible(oracle_t *o, const double *e)
 {
     memcpy(o->erodible, e, 8 * (size_t)o->nx * o->ny * o->ns);
+}
+/* Z(nx, ny), cells x fastest */
+void oracle_set_noise(oracle_t *o, const double *Z)
+{
+    memcpy(o->Z, Z, 8 * (size_t)o->nx * o->ny);
 }
 void oracle_get_erosion(const oracle_t *o, double *ers, double *erd)
 {

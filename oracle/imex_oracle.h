@@ -12,7 +12,7 @@ extern "C" {
 #endif
 
 /* Integer and real parameter vectors share the layout of oracle/refpy.py
- * pack_params(): ip[15], rp[73]. */
+ * pack_params(): ip[16], rp[73]. */
 typedef struct oracle_s oracle_t;
 
 oracle_t *oracle_create(const int *ip, const double *rp);
@@ -49,6 +49,7 @@ void oracle_set_t_end(oracle_t *o, double t_end);
 void oracle_mass_exchange_ers(const oracle_t *o, int n, const double *qp, const double *erod,
                               double dt, double *dep, double *eq, double *ers);
 void oracle_set_erodible(oracle_t *o, const double *e);
+void oracle_set_noise(oracle_t *o, const double *Z);
 void oracle_set_bcent(oracle_t *o, const double *b);
 void oracle_get_bcent(const oracle_t *o, double *b);
 void oracle_get_erosion(const oracle_t *o, double *ers, double *erd);

@@ -52,7 +52,8 @@ def _lib():
         lib.oracle_pow_cr.restype = ctypes.c_double
         lib.oracle_pow_cr.argtypes = [ctypes.c_double, ctypes.c_double]
         lib.oracle_set_pow_mode.restype = None
-        for name in ("oracle_mass_exchange_ers", "oracle_set_erodible", "oracle_get_erosion",
+        for name in ("oracle_mass_exchange_ers", "oracle_set_erodible", "oracle_set_noise",
+                     "oracle_get_erosion",
                      "oracle_set_bcent", "oracle_get_bcent"):
             getattr(lib, name).restype = None
         lib.oracle_set_t_end.restype = None
@@ -97,6 +98,8 @@ class Oracle:
             self.set_static(static)
             if static.get("erodible") is not None:
                 self.set_erodible(static["erodible"])
+            if static.get("Z") is not None and getattr(cfg, "stochastic_flag", False):
+                self.set_noise(static["Z"])
             if static.get("B_cent") is not None:
                 b = np.ascontiguousarray(static["B_cent"], np.float64)
                 self.lib.oracle_set_bcent(self.h, _dp(b))
@@ -199,6 +202,12 @@ class Oracle:
         a = np.ascontiguousarray(e, np.float64)
         assert a.shape == (self.cfg.n_solid, self.cfg.ny, self.cfg.nx)
         self.lib.oracle_set_erodible(self.h, _dp(a))
+
+    def set_noise(self, Z):
+        """The cells' stochastic values Z as [k, j]."""
+        a = np.ascontiguousarray(Z, np.float64)
+        assert a.shape == (self.cfg.ny, self.cfg.nx)
+        self.lib.oracle_set_noise(self.h, _dp(a))
 
     def erosion(self):
         """(erosion, erodible) as [class, k, j]."""

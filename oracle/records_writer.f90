@@ -6,18 +6,19 @@
 !            src/inpout_2d.f90:6056-6058, :6084-6109, :6111-6112)
 !   p2d      '(100ES15.7E2)' per cell, the same blank records (:6390-6406)
 !   restart  the unformatted sequential records of write_restart_file
-!            (:8059-8081)
+!            (:8059-8081); restart_z adds the stochastic_flag record Z
 ! Usage: records_writer q2d|p2d <in.raw> <out> <nx> <ny> <ncol>
-!        records_writer restart <in.raw> <out> <nx> <ny> <n_vars> <n_solid>
+!        records_writer restart|restart_z <in.raw> <out> <nx> <ny> <n_vars> <n_solid>
 ! in.raw holds the values as raw doubles: the records (ncol per cell, cells
 ! x fastest), or for restart t, dt, q, B_cent, erodible, deposit, erosion,
-! hmax, pdynmax, mod_vel_max in the reference's array orders.
+! hmax, pdynmax, mod_vel_max (and Z for restart_z) in the reference's array
+! orders.
 PROGRAM records_writer
   IMPLICIT NONE
   CHARACTER(LEN=512) :: mode, fin, fout, arg
   INTEGER :: nx, ny, ncol, nv, ns, j, k, i
   REAL(8), ALLOCATABLE :: rec(:, :, :), q(:, :, :), b(:, :), er(:, :, :), dep(:, :, :), &
-       ero(:, :, :), hm(:, :), pd(:, :), mv(:, :)
+       ero(:, :, :), hm(:, :), pd(:, :), mv(:, :), z(:, :)
   REAL(8) :: t, dt
   CALL GET_COMMAND_ARGUMENT(1, mode)
   CALL GET_COMMAND_ARGUMENT(2, fin)
@@ -27,14 +28,15 @@ PROGRAM records_writer
   CALL GET_COMMAND_ARGUMENT(5, arg)
   READ(arg, *) ny
   OPEN(10, FILE=TRIM(fin), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='OLD')
-  IF (TRIM(mode) == 'restart') THEN
+  IF (TRIM(mode) == 'restart' .OR. TRIM(mode) == 'restart_z') THEN
      CALL GET_COMMAND_ARGUMENT(6, arg)
      READ(arg, *) nv
      CALL GET_COMMAND_ARGUMENT(7, arg)
      READ(arg, *) ns
      ALLOCATE(q(nv, nx, ny), b(nx, ny), er(ns, nx, ny), dep(nx, ny, ns), ero(nx, ny, ns), &
-          hm(nx, ny), pd(nx, ny), mv(nx, ny))
+          hm(nx, ny), pd(nx, ny), mv(nx, ny), z(nx, ny))
      READ(10) t, dt, q, b, er, dep, ero, hm, pd, mv
+     IF (TRIM(mode) == 'restart_z') READ(10) z
      CLOSE(10)
      OPEN(11, FILE=TRIM(fout), FORM='UNFORMATTED', STATUS='REPLACE', ACTION='WRITE')
      WRITE(11) nx, ny, nv, ns
@@ -47,6 +49,7 @@ PROGRAM records_writer
      WRITE(11) hm
      WRITE(11) pd
      WRITE(11) mv
+     IF (TRIM(mode) == 'restart_z') WRITE(11) z
      CLOSE(11)
   ELSE
      CALL GET_COMMAND_ARGUMENT(6, arg)

@@ -30,6 +30,7 @@ MODULE imexref_harness
        energy_flag, rheology_flag, rheology_model, liquid_flag, gas_flag,        &
        alpha_flag, slope_correction_flag, curvature_term_flag,                   &
        vertical_profiles_flag, stochastic_flag, stoch_transport_flag,            &
+       length_spatial_corr,                                                      &
        pore_pressure_flag, sutherland_flag, topo_change_flag,                    &
        radial_source_flag, lateral_source_flag, bottom_radial_source_flag,       &
        erodible_deposit_flag, liquid_vaporization_flag, mean_field_flag,         &
@@ -62,7 +63,7 @@ MODULE imexref_harness
   USE solver_2d, ONLY : allocate_solver_variables, check_solve, timestep,       &
        imex_RK_solver, update_erosion_deposition_cell, q, qp, t, dt, hmax,       &
        pdynmax, mod_vel_max, vuln_table, thck_table, pdyn_table, solve_mask,     &
-       solve_cells, j_cent, k_cent
+       solve_cells, j_cent, k_cent, Z
   USE init_2d, ONLY : collapsing_volume
 
   IMPLICIT NONE
@@ -73,12 +74,12 @@ CONTAINS
 
   !----------------------------------------------------------------------------
   ! Set up the reference modules for one run.
-  !   ip(0:14)  integer parameters, rp(0:72) real parameters (layout in
+  !   ip(0:15)  integer parameters, rp(0:72) real parameters (layout in
   !   oracle/refpy.py), bdem(nx*ny) bed elevation at the cell centres
   !   (x index fastest).
   !----------------------------------------------------------------------------
   INTEGER(c_int) FUNCTION imexref_setup(ip, rp, bdem) BIND(C, name='imexref_setup')
-    INTEGER(c_int), INTENT(IN) :: ip(0:14)
+    INTEGER(c_int), INTENT(IN) :: ip(0:15)
     REAL(c_double), INTENT(IN) :: rp(0:72)
     REAL(c_double), INTENT(IN) :: bdem(*)
 
@@ -112,7 +113,10 @@ CONTAINS
     curvature_term_flag = .FALSE.
     vertical_profiles_flag = .FALSE.
     bottom_conc_flag = .FALSE.
-    stochastic_flag = .FALSE.
+    ! stochastic friction without transport of the noise: Z is set by
+    ! imexref_set_noise (0 after allocate_solver_variables)
+    stochastic_flag = ( ip(15) .NE. 0 )
+    length_spatial_corr = -1.0_wp
     stoch_transport_flag = .FALSE.
     mean_field_flag = .FALSE.
     pore_pressure_flag = .FALSE.
@@ -529,6 +533,12 @@ CONTAINS
     REAL(c_double), INTENT(IN) :: e(n_solid, comp_cells_x, comp_cells_y)
     erodible = e
   END SUBROUTINE imexref_set_erodible
+
+  ! solver_2d's stochastic values Z(nx, ny)
+  SUBROUTINE imexref_set_noise(zin) BIND(C, name='imexref_set_noise')
+    REAL(c_double), INTENT(IN) :: zin(comp_cells_x, comp_cells_y)
+    Z = zin
+  END SUBROUTINE imexref_set_noise
 
   ! B_cent(nx, ny)
   SUBROUTINE imexref_get_bcent(b) BIND(C, name='imexref_get_bcent')

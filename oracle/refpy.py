@@ -50,9 +50,9 @@ def _ip(a):
 
 
 def pack_params(cfg):
-    """ip[15] / rp[73] parameter vectors read by imexref_setup (and by the
+    """ip[16] / rp[73] parameter vectors read by imexref_setup (and by the
     C restatement's oracle_create)."""
-    ip = np.zeros(15, dtype=np.int32)
+    ip = np.zeros(16, dtype=np.int32)
     ip[0] = cfg.nx
     ip[1] = cfg.ny
     ip[2] = cfg.n_solid
@@ -68,6 +68,7 @@ def pack_params(cfg):
     ip[12] = int(getattr(cfg, "erodible_deposit_flag", False))
     ip[13] = int(getattr(cfg, "topo_change_flag", False))
     ip[14] = getattr(cfg, "rheology_model", 1)
+    ip[15] = int(getattr(cfg, "stochastic_flag", False))
     rp = np.zeros(73, dtype=np.float64)
     rp[0:16] = [cfg.dx, cfg.x0, cfg.y0, cfg.cfl, cfg.dt0, cfg.max_dt, cfg.t_start,
                 cfg.t_end, cfg.grav, cfg.mu, cfg.xi, cfg.pres, cfg.T_ambient,
@@ -154,6 +155,12 @@ class RefRun:
         a = np.ascontiguousarray(np.transpose(np.asarray(e, dtype=np.float64), (1, 2, 0)))
         assert a.shape == (self.cfg.ny, self.cfg.nx, self.cfg.n_solid)
         self.lib.imexref_set_erodible(_dp(a))
+
+    def set_noise(self, Z):
+        """solver_2d's Z as [k, j]."""
+        a = np.ascontiguousarray(Z, dtype=np.float64)
+        assert a.shape == (self.cfg.ny, self.cfg.nx)
+        self.lib.imexref_set_noise(_dp(a))
 
     def topography(self):
         """B_cent as [k, j]."""

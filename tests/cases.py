@@ -73,5 +73,32 @@ def c2m_64():
     return cfg, B
 
 
+def stochastic_noise(cfg):
+    """A sample of the stochastic values for a case: the host burn-in with
+    std_max 600 around xi = 1000 (so a few per cent of the cells take the
+    lower limit xi_temp = 1), plus cells planted beyond both limits."""
+    from imex_sflow2d_dev_amd import stochastic
+    cfg.std_max, cfg.tau_stochastic = 600.0, 1.0
+    Z = stochastic.steady_state_noise(cfg, seed=4)
+    Z[::7, ::5] = 2.5e5
+    Z[3::11, 2::9] = -4000.0
+    return Z
+
+
+def c1s_64():
+    """c1_64 with the stochastic friction (Voellmy xi shifted per cell)."""
+    cfg, B = c1_64()
+    cfg.stochastic_flag = True
+    return cfg, B
+
+
+def c2s_64():
+    """c2m_64 (model 12) with the stochastic friction."""
+    cfg, B = c2m_64()
+    cfg.stochastic_flag = True
+    return cfg, B
+
+
 CASES = {"c1_48": c1_48, "c1_64": c1_64, "c2_64": c2_64, "c2_96": c2_96,
-         "c1e_48": c1e_48, "c2e_64": c2e_64, "c1m_64": c1m_64, "c2m_64": c2m_64}
+         "c1e_48": c1e_48, "c2e_64": c2e_64, "c1m_64": c1m_64, "c2m_64": c2m_64,
+         "c1s_64": c1s_64, "c2s_64": c2s_64}

@@ -58,3 +58,41 @@ def test_mu_i_differs_from_voellmy():
     finally:
         s.close()
         s1.close()
+
+
+@pytest.mark.parametrize("name, steps, driver", [
+    ("c1s_64", 300, "device"), ("c1s_64", 120, "host"), ("c2s_64", 300, "device")])
+def test_stochastic_friction_run_matches_oracle(name, steps, driver):
+    """The stochastic friction: each cell's xi + Z limited to [1, 1e5]
+    (src/constitutive_2d.f90:3663-3673) with the Z given by imex_set_noise;
+    the oracle takes the same Z and tests/test_oracle_vs_ref.py pins it to
+    the reference."""
+    print(f"stochastic {name} steps={steps} {driver}", flush=True)
+    cfg, B = cases.CASES[name]()
+    Z = cases.stochastic_noise(cfg)
+    st, q0 = init.setup(cfg, B)
+    st["Z"] = Z
+    s = Solver(cfg, st, q0)
+    o = oraclepy.Oracle(cfg, st, q0)
+    o.set_pow_mode(1)
+    try:
+        if driver == "device":
+            d = s.run_steps(steps)
+        else:
+            d = np.array([s.step() for _ in range(steps)])
+        do, _, err = o.run(steps)
+        np.testing.assert_array_equal(d, do)
+        np.testing.assert_array_equal(s.q, o.q())
+    finally:
+        s.close()
+
+
+def test_set_noise_needs_the_stochastic_flag():
+    cfg, B = cases.CASES["c1_64"]()
+    st, q0 = init.setup(cfg, B)
+    s = Solver(cfg, st, q0)
+    try:
+        with pytest.raises(Exception, match="stochastic"):
+            s.set_noise(np.zeros((cfg.ny, cfg.nx)))
+    finally:
+        s.close()

@@ -110,3 +110,30 @@ def test_erosion_multistep_bit_exact(ref_lib, name, steps, topo):
         np.testing.assert_array_equal(o.topography(), run.topography())
         assert not np.array_equal(run.topography(), B)
     run.close()
+
+
+@pytest.mark.parametrize("name, steps", [("c1s_64", 300), ("c2s_64", 300)])
+def test_stochastic_friction_multistep_bit_exact(ref_lib, name, steps):
+    """The stochastic friction (xi + Z limited to [1, 1e5] in the turbulent
+    term, src/constitutive_2d.f90:3663-3673) with the same Z given to the
+    reference's solver_2d module and to the oracle; the noise changes the
+    run."""
+    cfg, B = cases.CASES[name]()
+    Z = cases.stochastic_noise(cfg)
+    run = ref_lib.RefRun(cfg, B)
+    run.set_noise(Z)
+    st = run.static()
+    st["Z"] = Z
+    q0 = run.q()
+    o = oraclepy.Oracle(cfg, st, q0)
+    cfg0, _ = cases.CASES[name]()
+    cfg0.stochastic_flag = False
+    o0 = oraclepy.Oracle(cfg0, st, q0)
+    for block in range(steps // 100):
+        d1, n1 = run.run(100)
+        d2, n2, err = o.run(100)
+        o0.run(100)
+        np.testing.assert_array_equal(d2, d1)
+        np.testing.assert_array_equal(o.q(), run.q())
+    assert not np.array_equal(o.q(), o0.q())
+    run.close()

@@ -79,3 +79,10 @@ def test_restart_file_matches_fortran_runtime(tmp_path):
     got = tmp_path / "restart.bin"
     native.write_restart(str(got), nx, ny, nv, ns, t, dt, q, B, er, dep, ero, hm, pd, mv)
     assert got.read_bytes() == want.read_bytes()
+    # a stochastic_flag run appends the Z record
+    Z = rng.standard_normal(N)
+    np.concatenate([[t, dt], q, B, er, dep, ero, hm, pd, mv, Z]).tofile(raw)
+    subprocess.run([WRITER, "restart_z", str(raw), str(want), str(nx), str(ny), str(nv), str(ns)],
+                   check=True)
+    native.write_restart(str(got), nx, ny, nv, ns, t, dt, q, B, er, dep, ero, hm, pd, mv, Z)
+    assert got.read_bytes() == want.read_bytes()
