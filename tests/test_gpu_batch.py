@@ -113,3 +113,35 @@ def test_served_results_equal_runs_alone():
         assert s.t == r.t, r.idx
         _bits(s.q, r.q, f"request {r.idx}")
         s.close()
+
+
+def test_members_with_different_friction_laws():
+    """Voellmy, mu(I) (model 11) and stochastic-friction members in one
+    batch: each member's friction law and noise travel with it."""
+    import cases
+    specs = []
+    for name in ("c1_64", "c1m_64", "c1s_64"):
+        cfg, B = cases.CASES[name]()
+        cfg.t_end = 3.0
+        st, q0 = init.setup(cfg, B)
+        if cfg.stochastic_flag:
+            st["Z"] = cases.stochastic_noise(cfg)
+        specs.append((name, cfg, st, q0))
+    alone = []
+    for name, cfg, st, q0 in specs:
+        s = Solver(cfg, st, q0)
+        run_alone(s, cfg.t_end)
+        alone.append(s)
+    b = Batch(n_solid=1, blocks_per_member=8)
+    inb = []
+    for name, cfg, st, q0 in specs:
+        s = Solver(cfg, st, q0)
+        b.add(s, cfg.t_end)
+        inb.append(s)
+    b.run_to_end(chunk=25)
+    for s in list(inb):
+        b.remove(s)
+    for (name, *_), s, r in zip(specs, inb, alone):
+        _same_run(s, r, name)
+    assert not np.array_equal(inb[0].q, inb[1].q) and not np.array_equal(inb[0].q, inb[2].q)
+    b.close()
