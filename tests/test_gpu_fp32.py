@@ -10,9 +10,13 @@ from imex_sflow2d_dev_amd import init, precision, workloads
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("n, dx, steps", [(96, 20.0, 400), (160, 12.0, 500)])
-def test_fp32_form_against_fp64(n, dx, steps):
+@pytest.mark.parametrize("n, dx, steps, rheology", [(96, 20.0, 400, 1), (160, 12.0, 500, 1),
+                                                    (96, 20.0, 400, 11)])
+def test_fp32_form_against_fp64(n, dx, steps, rheology):
     cfg, B = workloads.config3(nx=n, ny=n, dx=dx)
+    if rheology == 11:                  # mu(I) friction (FP64 in both forms)
+        cfg.rheology_model = 11
+        cfg.mu_s, cfg.mu_2, cfg.muI_inf, cfg.I_0 = 0.12, 0.45, 0.05, 0.279
     st, q0 = init.setup(cfg, B)
     r = precision.compare(cfg, st, q0, steps)
     f64, f32 = r["fp64"], r["fp32"]
