@@ -209,3 +209,29 @@ def test_implicit_real_path_matches_reference_fixture(name):
     got, _ = kp.implicit_real(g["qc"])
     _eq(got, g["implicit"], "implicit terms at real states")
     kp.close()
+
+
+def test_mu_i_semi_implicit_edge_states():
+    """Device mu(I) friction against the oracle (pinned to the reference on
+    the same kind of states) where the solids sum below EPSILON (the mean
+    diameter and density fall back to the classes' plain means), on layers
+    thinner than EPSILON and at rest."""
+    cfg, B = cases.CASES["c2m_64"]()
+    wet = _states(cfg, B, 150, 7)[:400]
+    dil = wet.copy()
+    dil[:, 4:] = 0.0
+    dil[1::2, 4] = 1e-18 * dil[1::2, 0]
+    rest = wet[:50].copy()
+    rest[:, 1:3] = 0.0
+    qc = np.concatenate([wet, dil, rest])
+    o = oraclepy.Oracle(cfg)
+    qp, _ = o.qc_to_qp(qc)
+    thin = qp[:60].copy()
+    thin[:, 0] = 1e-17
+    qp = np.concatenate([qp, thin])
+    qc = np.concatenate([qc, qc[:60]])
+    kp = KernelProbe(cfg)
+    want = o.semi_impl(qc, qp)
+    _eq(kp.semi_impl(qp), want, "mu(I) semi_impl edge states")
+    assert (want[len(wet):2 * len(wet), 1] != 0).any()
+    kp.close()

@@ -137,3 +137,31 @@ def test_stochastic_friction_multistep_bit_exact(ref_lib, name, steps):
         np.testing.assert_array_equal(o.q(), run.q())
     assert not np.array_equal(o.q(), o0.q())
     run.close()
+
+
+def test_mu_i_semi_implicit_on_solid_free_states(ref_lib):
+    """The mu(I) friction where the solid fractions sum below EPSILON: the
+    Sauter diameter and the mean particle density fall back to the plain
+    means of the classes (src/constitutive_2d.f90:387-416); also thin
+    layers below EPSILON (the thickness floor) and resting states."""
+    cfg, B = cases.CASES["c2m_64"]()
+    run = ref_lib.RefRun(cfg, B)
+    run.run(150)
+    q = run.q().reshape(-1, cfg.n_vars)
+    wet = q[q[:, 0] > 0.0][:400].copy()
+    dil = wet.copy()
+    dil[:, 4:] = 0.0
+    dil[1::2, 4] = 1e-18 * dil[1::2, 0]
+    rest = wet[:50].copy()
+    rest[:, 1:3] = 0.0
+    qc = np.concatenate([wet, dil, rest])
+    qp, _ = run.qc_to_qp(qc)
+    thin = qp[:60].copy()
+    thin[:, 0] = 1e-17
+    qp = np.concatenate([qp, thin])
+    qc = np.concatenate([qc, qc[:60]])
+    o = oraclepy.Oracle(cfg)
+    got, want = o.semi_impl(qc, qp), run.semi_impl(qc, qp)
+    np.testing.assert_array_equal(got, want)
+    assert (want[len(wet):2 * len(wet), 1] != 0).any()
+    run.close()
