@@ -1545,14 +1545,24 @@ __global__ void k_soa_to_aos(const S* soa, double* aos, int nv, int N) {
     for (int i = 0; i < nv; i++) aos[(size_t)c * nv + i] = soa[(size_t)i * N + c];
 }
 
-// Neumaier-compensated partial sums of the stdout diagnostics, one
-// (sum, compensation) pair per block and quantity; finished on the host.
+// Neumaier-compensated partial sums of the stdout diagnostics: each thread
+// accumulates its cells as a (sum, compensation) pair, the block combines
+// its threads' pairs in thread order, and the host combines the blocks'
+// pairs; one pair per block and quantity.
+__device__ __forceinline__ void neumaier_add(double& sum, double& comp, double x) {
+  const double t = sum + x;
+  if (fabs(sum) >= fabs(x))
+    comp += (sum - t) + x;
+  else
+    comp += (x - t) + sum;
+  sum = t;
+}
 template <int NS, typename S>
 __global__ __launch_bounds__(256) void k_diag(Params P, Grid G, double* part) {
   constexpr int NV = 4 + NS, NP = NV + 2;
-  __shared__ double sh[4][256];
+  __shared__ double sh[4][256], shc[4][256];
   const int tid = threadIdx.x;
-  double acc[4] = {0, 0, 0, 0};
+  double acc[4] = {0, 0, 0, 0}, cmp[4] = {0, 0, 0, 0};
   for (int c = blockIdx.x * blockDim.x + tid; c < G.N; c += gridDim.x * blockDim.x) {
     double q[NV];
 #pragma unroll
@@ -1566,23 +1576,21 @@ __global__ __launch_bounds__(256) void k_diag(Params P, Grid G, double* part) {
     double s = 0.0;
 #pragma unroll
     for (int i = 0; i < NS; i++) s += q[4 + i];
-    acc[0] += q[0];
-    acc[1] += h;
-    acc[2] += (q[0] > 1.0e-7) ? 1.0 : 0.0;
-    acc[3] += s;
+    neumaier_add(acc[0], cmp[0], q[0]);
+    neumaier_add(acc[1], cmp[1], h);
+    neumaier_add(acc[2], cmp[2], (q[0] > 1.0e-7) ? 1.0 : 0.0);
+    neumaier_add(acc[3], cmp[3], s);
   }
-  for (int m = 0; m < 4; m++) sh[m][tid] = acc[m];
+  for (int m = 0; m < 4; m++) {
+    sh[m][tid] = acc[m];
+    shc[m][tid] = cmp[m];
+  }
   __syncthreads();
   if (tid < 4) {
     double sum = 0.0, comp = 0.0;
     for (int i = 0; i < 256; i++) {
-      const double x = sh[tid][i];
-      const double t = sum + x;
-      if (fabs(sum) >= fabs(x))
-        comp += (sum - t) + x;
-      else
-        comp += (x - t) + sum;
-      sum = t;
+      neumaier_add(sum, comp, sh[tid][i]);
+      comp += shc[tid][i];
     }
     part[(blockIdx.x * 4 + tid) * 2] = sum;
     part[(blockIdx.x * 4 + tid) * 2 + 1] = comp;

@@ -85,6 +85,7 @@ struct oracle_s {
     double t, dt, dt_old, dt_old_old;
     long dried, stale;
     int err;
+    long nan_cell;          /* first cell (j + nx k) whose assembled state held a NaN, -1 none */
     /* reconstruction-call stamps of the face sides and of the cell primitive
      * states, used only to count reads of states left from an earlier call */
     int call_id;
@@ -94,11 +95,37 @@ struct oracle_s {
     int pow_mode;
 };
 
+/* Work counters of the stage-2 Newton solves and the settling iterations
+ * since the last oracle_work_reset (process-wide; the oracle runs one thread):
+ * they describe how long each cell's dependency chain is, which sets the
+ * duration of a one-thread-per-cell update kernel. */
+static struct {
+    long newton_calls, newton_its, newton_max_its;
+    long evalf, jac, ls_trials;
+    long cell_evals, cell_evals_max;   /* residual + 2 complex Jacobian columns per cell */
+    long settle_calls, settle_its, settle_max_its;
+    long hist[8];                      /* newton iterations: 1,2,3,4,5-8,9-16,17-32,>32 */
+} g_wk;
+
 /* x**y correctly rounded: powq in binary128 (113-bit significand) rounded
  * once to double; exact unless x**y lies within 2**-58 ulp of a rounding
- * boundary of the double result */
+ * boundary of the double result.  powl settles the rounding first where it
+ * can (see below): the same result, faster. */
 double oracle_pow_cr(double x, double y)
 {
+    /* fast path: powl (64-bit significand, error within one long-double
+     * ulp, i.e. 2**-11 ulp of the double result) decides the rounding unless
+     * it lies within 2**-8 ulp of a rounding boundary; powq otherwise */
+    const long double r = powl((long double)x, (long double)y);
+    const double d = (double)r;
+    if (isnormal(d) && isfinite(d)) {
+        const long double up = (long double)nextafter(d, INFINITY) - (long double)d;
+        const long double dn = (long double)d - (long double)nextafter(d, -INFINITY);
+        const long double m = (r >= (long double)d) ? (long double)d + 0.5L * up
+                                                    : (long double)d - 0.5L * dn;
+        const long double ulp = up < dn ? up : dn;
+        if (fabsl(r - m) > ldexpl(ulp, -8)) return d;
+    }
     return (double)powq((__float128)x, (__float128)y);
 }
 
@@ -654,11 +681,14 @@ static double settling_velocity(const oracle_t *o, double diam, double rhos, dou
     double const_part = sqrt(1.3333333333333333 * (rhos / rhoc - 1.0) * diam * o->grav);
     double sv = const_part * inv_sqrt_C_D;
     double Rey = diam * sv * ikv;
+    g_wk.settle_calls++;
     if (Rey <= 1000.0) {
         for (int i = 0; i < 20; i++) {
             double old = sv;
             inv_sqrt_C_D = sqrt(Rey / (24.0 * (1.0 + 0.15 * rpow(o, Rey, 0.687))));
             sv = const_part * inv_sqrt_C_D;
+            g_wk.settle_its++;
+            if (i + 1 > g_wk.settle_max_its) g_wk.settle_max_its = i + 1;
             if (fabs(old - sv) / old < 1.0e-6) return sv;
             Rey = diam * sv * ikv;
         }
@@ -1256,6 +1286,8 @@ static void eval_f(const oracle_t *o, double dt, const double *qj, const double 
 {
     const int nv = o->nv;
     double nh[8];
+    g_wk.evalf++;
+    g_wk.cell_evals++;
     eval_implicit_terms_r(o, qj, nh);
     double s = 0.0;
     for (int i = 0; i < nv; i++) {
@@ -1275,6 +1307,8 @@ static void eval_jacobian(const oracle_t *o, double dt, double a_diag, const dou
     const int nv = o->nv;
     for (int i = 0; i < nv * nv; i++) L[i] = 0.0;
     cx init[8], qc[8], nh[8];
+    g_wk.jac++;
+    g_wk.cell_evals += 2;
     for (int i = 0; i < nv; i++) init[i] = C(qrel[i], 0.0);
     for (int i = 0; i < nv; i++) {
         L[i + nv * i] = 1.0 * qorg[i];
@@ -1323,6 +1357,7 @@ static void lnsrch(const oracle_t *o, double dt, double a_diag, const double *xo
         for (int i = 0; i < nv; i++) x[i] = xold[i] + alam * p[i];
         for (int i = 0; i < nv; i++) qj[i] = x[i] * qorg[i];
         eval_f(o, dt, qj, qj_old, a_diag, Rni, rt, &f);
+        g_wk.ls_trials++;
         if (f <= 0.9 * fold) {
             break;
         } else if (alam < alamin) {
@@ -1444,6 +1479,14 @@ static int solve_rk_step(const oracle_t *o, double dt, double *qj, const double 
         }
         if (test < TOLX) break;
     }
+    {
+        const long n = it > 100 ? 100 : it;
+        g_wk.newton_calls++;
+        g_wk.newton_its += n;
+        if (n > g_wk.newton_max_its) g_wk.newton_max_its = n;
+        const int b = n <= 4 ? (int)n - 1 : (n <= 8 ? 4 : (n <= 16 ? 5 : (n <= 32 ? 6 : 7)));
+        g_wk.hist[b]++;
+    }
     return it;
 }
 
@@ -1522,7 +1565,9 @@ static void imex_RK_solver(oracle_t *o)
                         double B = 0.0 + (NH0[nv * c + i] + SI0[nv * c + i]) * a_d[0];
                         Rj[i] = (A - B) - a_diag * SIs[i];
                     }
+                    g_wk.cell_evals = 0;
                     solve_rk_step(o, dt, q_guess, q0, a_diag, Rj);
+                    if (g_wk.cell_evals > g_wk.cell_evals_max) g_wk.cell_evals_max = g_wk.cell_evals;
                     eval_implicit_terms_r(o, q_guess, NHs);
                     if (q_si[1] * q_si[1] + q_si[2] * q_si[2] == 0.0) {
                         q_guess[1] = 0.0;
@@ -1573,7 +1618,10 @@ static void imex_RK_solver(oracle_t *o)
         double *qq = o->q + nv * c, pd;
         for (int i = 0; i < nv; i++) qq[i] = q_rk1[nv * c + i];
         for (int i = 0; i < nv; i++)
-            if (isnan(qq[i])) o->err |= 1;
+            if (isnan(qq[i])) {
+                o->err |= 1;
+                if (o->nan_cell < 0) o->nan_cell = (long)c;
+            }
         if (qq[0] < EPS) {
             if (qq[0] > -1.0e-7) {
                 for (int i = 0; i < nv; i++) qq[i] = 0.0;
@@ -1768,6 +1816,7 @@ static void *zalloc(size_t n) { return calloc(n ? n : 1, 1); }
 oracle_t *oracle_create(const int *ip, const double *rp)
 {
     oracle_t *o = (oracle_t *)zalloc(sizeof(oracle_t));
+    o->nan_cell = -1;
     o->nx = ip[0]; o->ny = ip[1]; o->ns = ip[2]; o->nv = 4 + o->ns;
     o->energy = ip[3]; o->settling = ip[4]; o->entrainment = ip[5];
     o->radial = ip[6]; o->collapse = ip[7]; o->n_thk = ip[8]; o->n_pdyn = ip[9];
@@ -1919,6 +1968,7 @@ int oracle_run(oracle_t *o, int n, double *dts, int *ncells)
     const size_t N = (size_t)o->nx * o->ny;
     unsigned char *wet = zalloc(N);
     o->err = 0;
+    o->nan_cell = -1;
     for (int s = 0; s < n; s++) {
         for (size_t c = 0; c < N; c++) wet[c] = o->q[nv * c] > 0.0;
         check_solve(o, 0);
@@ -1981,6 +2031,20 @@ void oracle_set_time(oracle_t *o, double t, double dt_old, double dt_old_old)
 void oracle_set_t_end(oracle_t *o, double t_end) { o->t_end = t_end; }
 
 double oracle_get_t(const oracle_t *o) { return o->t; }
+
+long oracle_get_nan_cell(const oracle_t *o) { return o->nan_cell; }
+
+void oracle_work_reset(void) { memset(&g_wk, 0, sizeof g_wk); }
+
+void oracle_work_stats(long *out)
+{
+    const long v[18] = {g_wk.newton_calls, g_wk.newton_its, g_wk.newton_max_its, g_wk.evalf,
+                        g_wk.jac, g_wk.ls_trials, g_wk.cell_evals_max, g_wk.settle_calls,
+                        g_wk.settle_its, g_wk.settle_max_its, g_wk.hist[0], g_wk.hist[1],
+                        g_wk.hist[2], g_wk.hist[3], g_wk.hist[4], g_wk.hist[5], g_wk.hist[6],
+                        g_wk.hist[7]};
+    memcpy(out, v, sizeof v);
+}
 
 void oracle_get_counters(const oracle_t *o, long *dried, long *stale)
 {

@@ -58,6 +58,18 @@ double oracle_get_t(const oracle_t *o);
  * reads of a face or stage state left from an earlier step that differ from
  * the state a fresh evaluation gives (the reference's stale-state quirk) */
 void oracle_get_counters(const oracle_t *o, long *dried, long *stale_reads);
+/* first cell (j + nx*k) whose assembled state held a NaN in the last
+ * oracle_run call (the reference's print-and-pause at
+ * src/solver_2d.f90:1382-1397), -1 when none */
+long oracle_get_nan_cell(const oracle_t *o);
+/* work counters since oracle_work_reset (process-wide): out[18] = Newton
+ * solves, Newton iterations, max iterations of one solve, residual
+ * evaluations, Jacobians, line-search trials, max residual + Jacobian-column
+ * evaluations of one cell's solve, settling-velocity calls, settling
+ * iterations, max iterations of one settling call, and the histogram of
+ * Newton iterations per solve (1, 2, 3, 4, 5-8, 9-16, 17-32, > 32) */
+void oracle_work_reset(void);
+void oracle_work_stats(long *out);
 
 /* per-function entry points: n states of n_vars / n_vars+2 values each */
 void oracle_qc_to_qp(const oracle_t *o, int n, const double *qc, double *qp, double *pdyn);

@@ -60,8 +60,30 @@ def _lib():
         lib.oracle_set_t_end.argtypes = [ctypes.c_void_p, ctypes.c_double]
         lib.oracle_get_t.restype = ctypes.c_double
         lib.oracle_get_t.argtypes = [ctypes.c_void_p]
+        lib.oracle_get_nan_cell.restype = ctypes.c_long
+        lib.oracle_get_nan_cell.argtypes = [ctypes.c_void_p]
+        lib.oracle_work_reset.restype = None
+        lib.oracle_work_stats.restype = None
         _LIB = lib
     return _LIB
+
+
+WORK_FIELDS = ("newton_calls", "newton_its", "newton_max_its", "evalf", "jac", "ls_trials",
+               "cell_evals_max", "settle_calls", "settle_its", "settle_max_its",
+               "hist_1", "hist_2", "hist_3", "hist_4", "hist_5_8", "hist_9_16", "hist_17_32",
+               "hist_gt32")
+
+
+def work_reset() -> None:
+    """Zero the process-wide Newton / settling work counters."""
+    _lib().oracle_work_reset()
+
+
+def work_stats() -> dict:
+    """The work counters since work_reset() (see oracle_work_stats)."""
+    out = np.zeros(len(WORK_FIELDS), np.int64)
+    _lib().oracle_work_stats(out.ctypes.data_as(ctypes.POINTER(ctypes.c_long)))
+    return dict(zip(WORK_FIELDS, (int(v) for v in out)))
 
 
 def pow_cr(x: float, y: float) -> float:
@@ -229,6 +251,12 @@ class Oracle:
     @property
     def t(self):
         return self.lib.oracle_get_t(self.h)
+
+    def nan_cell(self):
+        """(j, k) of the first cell whose assembled state held a NaN in the
+        last run() call (the reference prints it and pauses), or None."""
+        c = self.lib.oracle_get_nan_cell(self.h)
+        return None if c < 0 else (int(c % self.cfg.nx), int(c // self.cfg.nx))
 
     def counters(self):
         d = ctypes.c_long()

@@ -28,6 +28,22 @@ def c2_64():
     return workloads.small_config2(nx=64, ny=64, dx=20.0)
 
 
+def c1d_96():
+    """A collapse of two solid classes (1 mm at 2500 kg/m3, 0.1 mm at
+    2000 kg/m3, fractions 0.03 / 0.02) with settling, in the middle of the
+    slope-with-bumps bed on 96 x 96 cells of 10 m: nothing reaches the
+    boundary, so each class's mass stays in the flow or in the deposit
+    until buoyant cells lift off."""
+    cfg, B = workloads.config1(nx=96, ny=96, dx=10.0)
+    cfg.x_collapse, cfg.y_collapse, cfg.r_collapse = 480.0, 480.0, 100.0
+    cfg.n_solid = 2
+    cfg.rho_s, cfg.diam_s, cfg.sp_heat_s = [2500.0, 2000.0], [1e-3, 1e-4], [1200.0, 1100.0]
+    cfg.alphas_collapse, cfg.alphas_source = [0.03, 0.02], [0.0, 0.0]
+    cfg.settling_flag = True
+    cfg.validate()
+    return cfg, B
+
+
 def c2_96():
     """As c2_64 on 96 x 96 cells."""
     return workloads.small_config2(nx=96, ny=96, dx=20.0)
@@ -99,6 +115,6 @@ def c2s_64():
     return cfg, B
 
 
-CASES = {"c1_48": c1_48, "c1_64": c1_64, "c2_64": c2_64, "c2_96": c2_96,
+CASES = {"c1_48": c1_48, "c1_64": c1_64, "c1d_96": c1d_96, "c2_64": c2_64, "c2_96": c2_96,
          "c1e_48": c1e_48, "c2e_64": c2e_64, "c1m_64": c1m_64, "c2m_64": c2m_64,
          "c1s_64": c1s_64, "c2s_64": c2s_64}
