@@ -50,6 +50,9 @@ HBM_PEAK_GBS = 8000.0
 # imex_run_steps_timed times them (rocprofv3 names in brackets)
 KERNELS = ["k_mask", "k_flux", "k_update"]
 
+# driver steps of each workload's run (the timed window is placed in its middle)
+RUN_STEPS = {"c1": 1000, "c2": 5000, "c2-small": 1000}
+
 
 def kernel_bytes_per_cell(cfg):
     """Algorithmic HBM bytes per active cell per launch of each kernel: the
@@ -72,34 +75,69 @@ def kernel_bytes_per_cell(cfg):
     return per
 
 
-def read_traffic(kernel: str, workload: str):
-    """Measured HBM bytes per active cell of one kernel of the step, from the
-    committed rocprofv3 PMC summary profiles/<workload>_step_kernels_traffic.md
-    (line 'bytes_per_active_cell <kernel>: X'), or None."""
+def read_traffic(kernel: str, workload: str, window):
+    """Measured HBM bytes per processed cell of one kernel of the step, from
+    the committed rocprofv3 PMC summary profiles/<workload>_step_kernels_traffic.md
+    (lines 'window: steps A..B' and 'bytes_per_cell <kernel>: X'), when its
+    window is the timed window (first, last) of this run; else None."""
     p = os.path.join(HERE, "profiles", f"{workload}_step_kernels_traffic.md")
     if not os.path.exists(p):
         return None
-    m = re.search(rf"^bytes_per_active_cell {re.escape(kernel)}:\s*([0-9.eE+-]+)", open(p).read(),
-                  re.M)
+    text = open(p).read()
+    w = re.search(r"^window: steps (\d+)\.\.(\d+)", text, re.M)
+    if not w or (int(w.group(1)), int(w.group(2))) != tuple(window):
+        return None
+    m = re.search(rf"^bytes_per_cell {re.escape(kernel)}:\s*([0-9.eE+-]+)", text, re.M)
     return float(m.group(1)) if m else None
 
 
-def cpu_baseline(workload: str, state: dict, seconds: float):
-    """Runs oracle/ref_timing.py in a child process on a bounded sample."""
+def host_cores(share: int = 16):
+    """Threads for the CPU baseline: the physical cores (lscpu's CORE,SOCKET
+    pairs) among the CPUs this process may run on, at most the host's CPU
+    share of one GPU; and lscpu's topology line."""
+    cpus = sorted(os.sched_getaffinity(0))
+    topo, core_of = "", {}
+    try:
+        out = subprocess.run(["lscpu", "-p=CPU,CORE,SOCKET"], capture_output=True, text=True,
+                             timeout=30).stdout
+        for ln in out.splitlines():
+            if ln and not ln.startswith("#"):
+                c, core, sock = (int(x) for x in ln.split(",")[:3])
+                core_of[c] = (sock, core)
+        info = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=30).stdout
+        def get(key):
+            m = re.search(r"^" + re.escape(key) + r":\s*(.+)$", info, re.M)
+            return m.group(1).strip() if m else "?"
+        topo = (get("Socket(s)") + " socket(s) x " + get("Core(s) per socket") + " cores x "
+                + get("Thread(s) per core") + " threads, " + get("Model name"))
+    except (OSError, subprocess.SubprocessError):
+        pass
+    phys = {core_of[c] for c in cpus if c in core_of} or set(cpus)
+    return max(1, min(len(phys), share)), topo
+
+
+def cpu_baseline(workload: str, state: dict, seconds: float, threads: int = 0,
+                 max_steps: int = 200):
+    """Runs oracle/ref_timing.py in a child process on a bounded sample, with
+    OMP_NUM_THREADS = the physical cores available (host_cores), bound
+    close to one thread per core."""
     from oracle import refpy
     if not refpy.available("perf"):
         return None
+    n, topo = host_cores()
+    if threads:
+        n = threads
     fd, path = tempfile.mkstemp(suffix=".npz", prefix="imex_state_")
     os.close(fd)
     try:
         np.savez(path, **state)
         env = dict(os.environ)
-        env.setdefault("OMP_NUM_THREADS", str(os.cpu_count() or 1))
+        env.update(OMP_NUM_THREADS=str(n), OMP_PROC_BIND="close", OMP_PLACES="cores")
         env.setdefault("OMP_STACKSIZE", "1G")
         cmd = [sys.executable, "-m", "oracle.ref_timing", "--workload", workload,
-               "--state", path, "--seconds", str(seconds)]
+               "--state", path, "--seconds", str(seconds), "--max-steps", str(max_steps)]
         r = subprocess.run(cmd, cwd=HERE, env=env, stdin=subprocess.DEVNULL,
-                           capture_output=True, text=True, timeout=600)
+                           capture_output=True, text=True, timeout=3000)
         if r.returncode != 0:
             print(f"cpu_baseline: reference run failed ({r.returncode}): {r.stderr[-2000:]}",
                   file=sys.stderr, flush=True)
@@ -115,8 +153,10 @@ def cpu_baseline(workload: str, state: dict, seconds: float):
         "sample": (f"{res['steps']} driver steps of the same run from its state at step "
                    f"{state['step']}, {res['active_cells']} active cells in total, "
                    f"{res['elapsed_s']:.1f} s; reference src/*.f90 built with "
-                   f"amdflang -O3 -ffast-math -fstack-arrays -funroll-loops -fopenmp, "
-                   f"OMP_NUM_THREADS={res['threads']}"),
+                   f"amdflang -O3 -ffast-math -fstack-arrays -funroll-loops -fopenmp (flang's "
+                   f"spelling of -Ofast; it has no -funroll-all-loops), "
+                   f"OMP_NUM_THREADS={res['threads']} OMP_PROC_BIND=close OMP_PLACES=cores; "
+                   f"host {topo}"),
     }
 
 
@@ -130,6 +170,13 @@ def main():
                          "(serving) or c2-outputs (the run with periodic outputs)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the reference CPU sample")
+    ap.add_argument("--skip-whole-run", action="store_true",
+                    help="do not repeat the workload's whole run after the timed window (for "
+                         "counter passes, which then see exactly the spin-up, warmup and timed "
+                         "launches)")
+    ap.add_argument("--cpu-full", action="store_true",
+                    help="also time the reference's whole run of the workload from its initial "
+                         "state on all physical cores and on one thread (long: minutes)")
     ap.add_argument("--event-every", type=int, default=50, metavar="E",
                     help="HIP events around the kernel launches of every E-th timed step "
                          "(an event record costs the stream a few microseconds)")
@@ -187,8 +234,13 @@ def main():
     t_host = time.perf_counter()
     cfg, B = workloads.by_name(args.workload)
     static, q0 = init.setup(cfg, B)
+    full = RUN_STEPS.get(args.workload, args.warmup + args.steps)
+    # the timed window sits in the middle of the workload's run (the flow of
+    # its first and last steps is not representative of the run)
+    spinup = max(0, (full - args.warmup - args.steps) // 2)
     print(f"[bench] workload {args.workload}: {cfg.nx}x{cfg.ny} n_solid={cfg.n_solid} "
-          f"host set-up {time.perf_counter() - t_host:.1f} s", file=sys.stderr, flush=True)
+          f"host set-up {time.perf_counter() - t_host:.1f} s; run of {full} steps, "
+          f"{spinup} spin-up steps before the warmup", file=sys.stderr, flush=True)
 
     # load code objects and touch every kernel once on a throw-away run
     s = Solver(cfg, static, q0)
@@ -199,11 +251,15 @@ def main():
     s = Solver(cfg, static, q0)
     s_time(s)
     t_up = time.perf_counter() - t_up
-    total_cells_all = 0
+    s.raise_on_error = False                    # physics error bits are reported, not raised
+    if spinup:
+        s.run_steps(spinup, record_dt=False)
     if args.warmup:
-        _, _, _, c = s.run_steps_timed(args.warmup, record_dt=False, sample_every=args.warmup)
-        total_cells_all += c
-    print(f"[bench] warmup {args.warmup} steps done, t={s.t:.4f}", file=sys.stderr, flush=True)
+        s.run_steps_timed(args.warmup, record_dt=False, sample_every=args.warmup)
+    print(f"[bench] spin-up {spinup} and warmup {args.warmup} steps done, t={s.t:.4f}",
+          file=sys.stderr, flush=True)
+    first = spinup + args.warmup + 1            # driver step numbers of the timed window
+    last = spinup + args.warmup + args.steps
 
     # timed region: two halves, with the mid-run state copied out between
     # them (outside the clock) for the CPU sample
@@ -213,32 +269,31 @@ def main():
     cells = 0
     wall = 0.0
     mid = None
-    dts_all = []
     err_bits = 0
-    s.raise_on_error = False                    # physics error bits are reported, not raised
+    work0, frame = s.work_counters()
+    work = 0
     for i, n in enumerate(halves):
         if n == 0:
             continue
         s_time(s)                               # synchronises the stream
+        w0 = s.work_counters()[0]
         t0 = time.perf_counter()
         # returns after a stream sync
-        dts, k, ns, c = s.run_steps_timed(n, sample_every=args.event_every)
+        _, k, ns, c = s.run_steps_timed(n, sample_every=args.event_every)
         t1 = time.perf_counter()
+        work += s.work_counters()[0] - w0
         err_bits |= s.err
         wall += t1 - t0
         kernel_ms += k
         sampled += ns
         cells += c
-        dts_all.append(dts)
         print(f"[bench] timed part {i + 1}: {n} steps, {t1 - t0:.3f} s, "
               f"{c} active cells", file=sys.stderr, flush=True)
         if i == 0:
             mid = dict(q=s.q, t=s.t, dt_old=s.dt_old, dt_old_old=s.dt_old_old,
-                       step=args.warmup + n)
-    total_cells_all += cells
+                       step=first - 1 + n)
     if mid is None:
-        mid = dict(q=s.q, t=s.t, dt_old=s.dt_old, dt_old_old=s.dt_old_old,
-                   step=args.warmup + args.steps)
+        mid = dict(q=s.q, t=s.t, dt_old=s.dt_old, dt_old_old=s.dt_old_old, step=last)
     t_end_phys = s.t
     t_down = time.perf_counter()                # the state and the output fields to host memory
     s.q
@@ -248,21 +303,55 @@ def main():
 
     K = args.steps
     value = 2.0 * cells / wall
+    # the whole run of the workload in the same process (wall clock, no events)
+    if args.skip_whole_run:
+        whole = dict(steps=K, wall_s=wall, active_cells=cells, work_cells=work,
+                     physics_error_bits=err_bits, note="timed window only (--skip-whole-run)")
+    elif (spinup, args.warmup, K) == (0, 0, full):
+        whole = dict(steps=K, wall_s=wall, active_cells=cells, work_cells=work,
+                     physics_error_bits=err_bits)
+    else:
+        s2 = Solver(cfg, static, q0)
+        s2.raise_on_error = False
+        s_time(s2)
+        a_init, _ = s2.counters()               # the upload's check_solve
+        t0 = time.perf_counter()
+        s2.run_steps(full, record_dt=False)
+        s_time(s2)
+        wall2 = time.perf_counter() - t0
+        a2, _ = s2.counters()
+        w2, _ = s2.work_counters()
+        whole = dict(steps=full, wall_s=wall2, active_cells=a2 - a_init, work_cells=w2,
+                     physics_error_bits=s2.err)
+        s2.close()
+    whole["ms_per_step"] = whole["wall_s"] * 1e3 / whole["steps"]
+    whole["cell_stage_updates_per_s"] = 2.0 * whole["active_cells"] / whole["wall_s"]
+    whole["work_cell_stage_updates_per_s"] = 2.0 * whole["work_cells"] / whole["wall_s"]
+
     per = kernel_bytes_per_cell(cfg)
     names = KERNELS
     dom = int(np.argmax(kernel_ms))
     dname = names[dom]
     mean_launch_s = kernel_ms[dom] * 1e-3 / sampled
     cells_per_launch = cells / K
-    alg_bytes = per[dname] * cells_per_launch
+    work_per_launch = work / K
+    # the update kernel visits only its work list; the others are measured on
+    # the active cells (their per-cell bytes are those of an active cell)
+    basis_cells = work_per_launch if dname == "k_update" else cells_per_launch
+    alg_bytes = per[dname] * basis_cells
     achieved = alg_bytes / mean_launch_s / 1e9
-    tb = read_traffic(dname, args.workload)
+    achieved_active = per[dname] * cells_per_launch / mean_launch_s / 1e9
+    tb = read_traffic(dname, args.workload, (first, last))
     roofline = {
         "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS,
-        "traffic": (tb * cells_per_launch) if tb is not None else None,
+        "traffic": (tb * basis_cells) if tb is not None else None,
         "kernel": dname,
-        "alg_bytes_per_active_cell": per[dname],
+        "cells_basis": "work cells (the cells the kernel processes)" if dname == "k_update"
+                       else "active cells",
+        "alg_bytes_per_cell": per[dname],
+        "cells_per_launch": basis_cells,
+        "achieved_on_active_cells": achieved_active,
         "mean_launch_us": mean_launch_s * 1e6,
         "event_timed_launches": sampled,
         "mean_active_cells_per_launch": cells_per_launch,
@@ -284,9 +373,16 @@ def main():
         "config": {
             "workload": (f"{args.workload}: {cfg.nx}x{cfg.ny} cells dx={cfg.dx} m, "
                          f"{cfg.n_solid} solid classes, settling={cfg.settling_flag}, "
-                         f"radial source={cfg.radial_source_flag}, FP64, driver steps "
-                         f"{args.warmup + 1}..{args.warmup + K}"),
+                         f"radial source={cfg.radial_source_flag}, FP64, timed driver steps "
+                         f"{first}..{last} of the {full}-step run"),
             "grid": [cfg.nx, cfg.ny],
+            "run_steps": full,
+            "spinup_steps": spinup,
+            "timed_steps": [first, last],
+            "active_cells_timed": cells,
+            "work_cells_timed": work,
+            "frame_cells_per_step": frame,
+            "work_cell_stage_updates_per_s": 2.0 * work / wall,
             "active_fraction": cells / (K * cfg.nx * cfg.ny),
             "grid_cell_stage_updates_per_s": 2.0 * cfg.nx * cfg.ny * K / wall,
             "t_physical_end_s": t_end_phys,
@@ -295,6 +391,7 @@ def main():
             "kernel_ms_event_timed_steps": dict(zip(names, [float(x) for x in kernel_ms])),
             "event_timed_steps": sampled,
             "physics_error_bits": err_bits,
+            "whole_run": whole,
             "host_upload_s": t_up,
             "host_download_s": t_down,
             "host_buffers_inclusive_cell_stage_updates_per_s": 2.0 * cells / (wall + t_up + t_down),
@@ -304,14 +401,23 @@ def main():
     }
     if args.info:
         with open(args.info, "w") as f:
-            json.dump({"steps_total": args.warmup + K + 3, "active_cells_total": total_cells_all,
-                       "active_cells_timed": cells,
+            json.dump({"spinup": spinup, "timed_steps": [first, last],
+                       "active_cells_timed": cells, "work_cells_timed": work,
                        "kernel_ms_event_timed_steps": out["config"]["kernel_ms_event_timed_steps"],
                        "event_timed_steps": sampled,
                        "alg_bytes": per}, f, indent=1)
     if not args.no_cpu:
         print("[bench] CPU reference sample ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(args.workload, mid, args.cpu_seconds)
+    if args.cpu_full:
+        init_state = dict(q=q0, t=cfg.t_start, dt_old=cfg.dt0, dt_old_old=cfg.dt0, step=0)
+        runs = []
+        for th in (host_cores()[0], 1):
+            print(f"[bench] reference whole run on {th} thread(s) ...", file=sys.stderr,
+                  flush=True)
+            runs.append(cpu_baseline(args.workload, init_state, 1e9, threads=th,
+                                     max_steps=full))
+        out["config"]["cpu_whole_run"] = runs
     print(json.dumps(out), flush=True)
 
 

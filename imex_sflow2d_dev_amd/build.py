@@ -1,4 +1,7 @@
-"""Builds the HIP library libimexhip.so for gfx950 in-tree.
+"""Builds the HIP library libimexhip.so for gfx950 in-tree, and the Fortran
+side of the boundary: the ISO_C_BINDING module imex_hip_binding
+(fortran/imex_hip_binding.f90 -> _lib/imex_hip_binding.o, .mod) and the
+Fortran host program imex_fdriver linked against the library.
 
 hipcc flags: -O3, -ffp-contract=off (no fused multiply-add: the FP64 path
 keeps the reference's rounding), packed FP32 instructions disabled.
@@ -15,6 +18,9 @@ LIBDIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(LIBDIR, "libimexhip.so")
 SRCS = [os.path.join(HERE, "csrc", f) for f in ("imex_hip.hip", "imex_output.hip", "imex_writers.cpp")]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+FC = os.environ.get("IMEX_FC", "/opt/rocm/bin/amdflang")
+FDIR = os.path.join(HERE, "fortran")
+FDRIVER = os.path.join(LIBDIR, "imex_fdriver")
 ARCH = os.environ.get("IMEX_OFFLOAD_ARCH", "gfx950")
 
 
@@ -36,7 +42,26 @@ def build(verbose: bool = False) -> str:
     if verbose:
         sys.stderr.write(r.stderr)
     os.replace(tmp, LIB)
+    build_fortran()
     return LIB
+
+
+def build_fortran() -> str:
+    """imex_hip_binding.o/.mod and imex_fdriver in _lib (rpath $ORIGIN)."""
+    obj = os.path.join(LIBDIR, "imex_hip_binding.o")
+    steps = [
+        [FC, "-O2", "-fPIC", "-c", os.path.join(FDIR, "imex_hip_binding.f90"), "-J", LIBDIR,
+         "-o", obj],
+        [FC, "-O2", os.path.join(FDIR, "imex_fdriver.f90"), obj, "-I", LIBDIR, "-L", LIBDIR,
+         "-limexhip", "-Wl,-rpath,$ORIGIN", "-o", FDRIVER + ".tmp"],
+    ]
+    for cmd in steps:
+        r = subprocess.run(cmd, capture_output=True, text=True, cwd=LIBDIR)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError("amdflang failed: " + " ".join(cmd))
+    os.replace(FDRIVER + ".tmp", FDRIVER)
+    return FDRIVER
 
 
 if __name__ == "__main__":

@@ -1446,6 +1446,9 @@ __device__ __forceinline__ void update_body(const Params& P, const Grid& G, doub
     }
   }
   const int n = ALL ? N : (int)G.st->n_work;
+  if constexpr ((PARTS & 1) != 0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) G.st->work += (unsigned long long)n;
+  }
   unsigned int err = 0;
   int dried = 0;
   const int stride = (int)(gridDim.x * NTHR) / LANES;
@@ -2264,13 +2267,15 @@ extern "C" int imex_post_step(imex_handle* h, double t_new, imex_step_stats* out
   return (int)(err & ~IMEX_WARN_RESCALE);
 }
 
-// Start of a device-driven call: the call's t_end, and the end condition of
-// the driver loop (t < t_end, src/IMEX_SfloW2D.f90:395) for its first step.
+// Start of a device-driven call: the call's t_end, the end condition of the
+// driver loop (t < t_end, src/IMEX_SfloW2D.f90:395) for its first step, and
+// cleared physics error bits.
 // The passes skipped after the previous call reached its t_end built no
 // lists, so the next pass visits every tile.
 static int begin_driven(imex_handle* h, double t_end, DevState* s) {
   if (read_state(h, s)) return -1;
   if (s->stop_next) h->mask_full_next = true;
+  s->err = 0u;  // the call returns the bits its own steps raise
   s->t_end = t_end;
   s->done = s->stop_next = (s->t < t_end) ? 0u : 1u;
   HCHK(h, hipMemcpyAsync(h->G.st, s, sizeof(*s), hipMemcpyHostToDevice, h->stream));
@@ -2671,6 +2676,15 @@ extern "C" int imex_get_counters(imex_handle* h, int64_t* active_cells, int64_t*
   if (read_state(h, &s)) return -1;
   if (active_cells) *active_cells = (int64_t)s.active;
   if (steps) *steps = (int64_t)s.steps;
+  return 0;
+}
+
+extern "C" int imex_get_work_counters(imex_handle* h, int64_t* work_cells, int64_t* frame_cells) {
+  if (!h) return -2;
+  DevState s;
+  if (read_state(h, &s)) return -1;
+  if (work_cells) *work_cells = (int64_t)s.work;
+  if (frame_cells) *frame_cells = (int64_t)h->G.static_active;
   return 0;
 }
 

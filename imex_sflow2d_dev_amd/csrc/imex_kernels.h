@@ -64,6 +64,7 @@ struct DevState {
                              // so the following active-set passes do no tile work
   unsigned long long dried;
   unsigned long long active;
+  unsigned long long work;   // work-list cells summed over the stage-2 launches since create
 };
 
 struct Grid {

@@ -19,10 +19,10 @@ def run_stats(s: Solver) -> dict:
     f = s.fields()
     q = s.q
     area = cfg.dx * cfg.dx
-    dep = f["deposit"]                       # [k, j, class] thickness
-    dep_tot = dep.sum(axis=-1) if dep.ndim == 3 else dep
+    dep = f["deposit"]                       # [class, k, j] thickness
+    dep_tot = dep.sum(axis=0)
     rho_s = np.asarray(cfg.rho_s[:cfg.n_solid])
-    dep_mass = (dep.reshape(-1, cfg.n_solid) * rho_s).sum() * area if dep.size else 0.0
+    dep_mass = float((dep.sum(axis=(1, 2)) * rho_s).sum()) * area
     return {
         "runout_area_m2": float(area * np.count_nonzero(f["hmax"] > 1.0e-3)),
         "max_thickness_m": float(f["hmax"].max()),

@@ -325,6 +325,14 @@ class Solver:
     def snapshot_wait(self):
         self.H.check(self.L.imex_snapshot_wait(self.h), "imex_snapshot_wait")
 
+    def work_counters(self):
+        """(cells the update kernel processed summed over the steps since
+        create, cells of the always-active boundary frame per check_solve)."""
+        w, f = ctypes.c_int64(), ctypes.c_int64()
+        self.H.check(self.L.imex_get_work_counters(self.h, ctypes.byref(w), ctypes.byref(f)),
+                     "imex_get_work_counters")
+        return w.value, f.value
+
     def stats(self):
         st = native.ImexStepStats()
         self.H.check(self.L.imex_post_step(self.h, ctypes.c_double(self.t), ctypes.byref(st)),

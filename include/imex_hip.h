@@ -140,7 +140,8 @@ int imex_post_step(imex_handle *h, double t_new, imex_step_stats *out);
  * t += dt, epilogue.  The steps end once t reaches t_end (the driver loop's
  * condition t < t_end, :395): the remaining steps of the call do nothing,
  * so a call with t_end at the next output time stops exactly there.  dts
- * (may be NULL) receives each step's dt at the end (0 for a step not taken). */
+ * (may be NULL) receives each step's dt at the end (0 for a step not taken).
+ * Returns the physics error bits raised by the call's own steps. */
 int imex_run_steps(imex_handle *h, int nsteps, double t_end, double *dts);
 /* imex_run_steps with HIP events around the kernel launches of every
  * sample_every-th step (steps n with n % sample_every == 0; an event record
@@ -168,6 +169,12 @@ int imex_get_time_history(imex_handle *h, double *t, double *dt_old, double *dt_
  * device-driven steps since the last upload (measurement; no counterpart in
  * the reference) */
 int imex_get_counters(imex_handle *h, int64_t *active_cells, int64_t *steps);
+/* cells the update kernel processed (its work list: active cells whose state
+ * or stage-1 right-hand side is not all zero) summed over the steps since
+ * create, and the cells of the always-active boundary frame that every
+ * check_solve adds to the active count (src/solver_2d.f90:351-354) while no
+ * kernel has work there (measurement; no counterpart in the reference) */
+int imex_get_work_counters(imex_handle *h, int64_t *work_cells, int64_t *frame_cells);
 /* topography_reconstruction (src/geometry_2d.f90:555-895, called at
  * src/IMEX_SfloW2D.f90:445 when topo_change_flag): the bed slopes
  * B_prime_x_geom / B_prime_y_geom from the device's B_cent, for host-driven

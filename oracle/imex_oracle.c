@@ -86,6 +86,10 @@ struct oracle_s {
     long dried, stale;
     int err;
     long nan_cell;          /* first cell (j + nx k) whose assembled state held a NaN, -1 none */
+    /* the two x faces of a watched cell: KT fluxes before the solid-flux cap
+     * and after the zero-velocity rule, at the last step that computed them */
+    int watch_j, watch_k, watch_seen[2];
+    double watch_pre[2][8], watch_post[2][8];
     /* reconstruction-call stamps of the face sides and of the cell primitive
      * states, used only to count reads of states left from an earlier call */
     int call_id;
@@ -495,7 +499,7 @@ static void eval_expl_terms(const oracle_t *o, double Bx, double By, double G, d
             }
         }
         if (t_coeff > 0.0) {
-            double qs[8];
+            double qs[10];  /* n_vars + 2 <= 9 */
             for (int i = 0; i < nv + 2; i++) qs[i] = 0.0;
             qs[0] = 1.0;
             qs[3] = o->T_source;
@@ -875,7 +879,7 @@ static inline void note_stale_qp(oracle_t *o, int c, const double *qpx)
 static void reconstruction(oracle_t *o, const double *q_expl, const double *qp_expl)
 {
     const int nx = o->nx, ny = o->ny, nv = o->nv, np = nv + 2;
-    double qrecW[8], qrecE[8], qrecS[8], qrecN[8], sbd[8];
+    double qrecW[10], qrecE[10], qrecS[10], qrecN[10], sbd[10];  /* n_vars + 2 <= 9 */
     double pr_x[8], pr_y[8], st[3], xs[3], ys[3], q2[3];
     (void)q_expl;
     o->call_id++;
@@ -1182,6 +1186,9 @@ static void eval_flux_KT(oracle_t *o)
                 H[i] = avg + (ap * an) / (ap - an) * (qr[i] - ql[i]);
             }
         }
+        const int wslot = (o->ksx[l] == o->watch_k) ? o->jsx[l] - o->watch_j : -1;
+        if (wslot == 0 || wslot == 1)
+            for (int i = 0; i < nv; i++) o->watch_pre[wslot][i] = H[i];
         double S = 0.0;
         for (int i = 0; i < ns; i++) S += H[4 + i];
         if ((S / H[0]) >= 1.0) {
@@ -1192,6 +1199,10 @@ static void eval_flux_KT(oracle_t *o)
         if ((pl[1] == 0.0) && (pr[1] == 0.0)) {
             H[0] = 0.0;
             for (int i = 3; i < nv; i++) H[i] = 0.0;
+        }
+        if (wslot == 0 || wslot == 1) {
+            for (int i = 0; i < nv; i++) o->watch_post[wslot][i] = H[i];
+            o->watch_seen[wslot] = 1;
         }
     }
     for (int l = 0; l < o->nfy; l++) {
@@ -1817,6 +1828,7 @@ oracle_t *oracle_create(const int *ip, const double *rp)
 {
     oracle_t *o = (oracle_t *)zalloc(sizeof(oracle_t));
     o->nan_cell = -1;
+    o->watch_j = o->watch_k = -10;
     o->nx = ip[0]; o->ny = ip[1]; o->ns = ip[2]; o->nv = 4 + o->ns;
     o->energy = ip[3]; o->settling = ip[4]; o->entrainment = ip[5];
     o->radial = ip[6]; o->collapse = ip[7]; o->n_thk = ip[8]; o->n_pdyn = ip[9];
@@ -2033,6 +2045,23 @@ void oracle_set_t_end(oracle_t *o, double t_end) { o->t_end = t_end; }
 double oracle_get_t(const oracle_t *o) { return o->t; }
 
 long oracle_get_nan_cell(const oracle_t *o) { return o->nan_cell; }
+
+void oracle_watch_cell(oracle_t *o, int j, int k)
+{
+    o->watch_j = j;
+    o->watch_k = k;
+    o->watch_seen[0] = o->watch_seen[1] = 0;
+}
+
+int oracle_watched_faces(const oracle_t *o, double *pre, double *post)
+{
+    for (int f = 0; f < 2; f++)
+        for (int i = 0; i < 8; i++) {
+            pre[8 * f + i] = o->watch_pre[f][i];
+            post[8 * f + i] = o->watch_post[f][i];
+        }
+    return o->watch_seen[0] + 2 * o->watch_seen[1];
+}
 
 void oracle_work_reset(void) { memset(&g_wk, 0, sizeof g_wk); }
 

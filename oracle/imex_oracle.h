@@ -62,6 +62,14 @@ void oracle_get_counters(const oracle_t *o, long *dried, long *stale_reads);
  * oracle_run call (the reference's print-and-pause at
  * src/solver_2d.f90:1382-1397), -1 when none */
 long oracle_get_nan_cell(const oracle_t *o);
+/* watch the west and east x faces of cell (j, k) (0-based): at every step
+ * that computes them, their KT fluxes before the x solid-flux cap
+ * (src/solver_2d.f90:2903-2912) and after the zero-velocity rule are kept;
+ * oracle_watched_faces copies them (pre[16], post[16]: face 0 then face 1,
+ * n_vars values each in slots of 8) and returns bit 0 / bit 1 for the faces
+ * computed since the watch was set */
+void oracle_watch_cell(oracle_t *o, int j, int k);
+int oracle_watched_faces(const oracle_t *o, double *pre, double *post);
 /* work counters since oracle_work_reset (process-wide): out[18] = Newton
  * solves, Newton iterations, max iterations of one solve, residual
  * evaluations, Jacobians, line-search trials, max residual + Jacobian-column

@@ -41,7 +41,7 @@ def _lib():
         for name in ("oracle_destroy", "oracle_set_static", "oracle_init_state",
                      "oracle_run", "oracle_get_q", "oracle_set_q", "oracle_get_fields",
                      "oracle_get_vuln", "oracle_get_mask", "oracle_get_counters",
-                     "oracle_set_time",
+                     "oracle_set_time", "oracle_watch_cell",
                      "oracle_qc_to_qp", "oracle_qp_to_qc", "oracle_mixt_var",
                      "oracle_fluxes", "oracle_speeds", "oracle_implicit",
                      "oracle_implicit_cplx", "oracle_semi_impl", "oracle_expl",
@@ -63,6 +63,7 @@ def _lib():
         lib.oracle_get_nan_cell.restype = ctypes.c_long
         lib.oracle_get_nan_cell.argtypes = [ctypes.c_void_p]
         lib.oracle_work_reset.restype = None
+        lib.oracle_watched_faces.restype = ctypes.c_int
         lib.oracle_work_stats.restype = None
         _LIB = lib
     return _LIB
@@ -257,6 +258,18 @@ class Oracle:
         last run() call (the reference prints it and pauses), or None."""
         c = self.lib.oracle_get_nan_cell(self.h)
         return None if c < 0 else (int(c % self.cfg.nx), int(c // self.cfg.nx))
+
+    def watch_cell(self, j, k):
+        """Keep the KT fluxes of the west and east x faces of cell (j, k)."""
+        self.lib.oracle_watch_cell(self.h, ctypes.c_int(j), ctypes.c_int(k))
+
+    def watched_faces(self):
+        """(pre-cap fluxes [2, n_vars], post fluxes [2, n_vars], faces seen bits)."""
+        pre = np.zeros(16)
+        post = np.zeros(16)
+        seen = self.lib.oracle_watched_faces(self.h, _dp(pre), _dp(post))
+        nv = self.nv
+        return pre.reshape(2, 8)[:, :nv], post.reshape(2, 8)[:, :nv], seen
 
     def counters(self):
         d = ctypes.c_long()

@@ -44,6 +44,17 @@ def c1d_96():
     return cfg, B
 
 
+def c3_64():
+    """The three-class form of the collapse (classes of 1e-2, 1e-3 and
+    1e-4 m at fractions 0.02/0.02/0.01) with settling, 64 x 64 cells of
+    10 m: every kernel instantiated for three classes, and a deposit."""
+    cfg, B = workloads.config3(nx=64, ny=64, dx=10.0)
+    cfg.x_collapse, cfg.y_collapse, cfg.r_collapse = 200.0, 320.0, 100.0
+    cfg.settling_flag = True
+    cfg.validate()
+    return cfg, B
+
+
 def c2_96():
     """As c2_64 on 96 x 96 cells."""
     return workloads.small_config2(nx=96, ny=96, dx=20.0)
@@ -115,6 +126,6 @@ def c2s_64():
     return cfg, B
 
 
-CASES = {"c1_48": c1_48, "c1_64": c1_64, "c1d_96": c1d_96, "c2_64": c2_64, "c2_96": c2_96,
+CASES = {"c1_48": c1_48, "c1_64": c1_64, "c1d_96": c1d_96, "c3_64": c3_64, "c2_64": c2_64, "c2_96": c2_96,
          "c1e_48": c1e_48, "c2e_64": c2e_64, "c1m_64": c1m_64, "c2m_64": c2m_64,
          "c1s_64": c1s_64, "c2s_64": c2s_64}

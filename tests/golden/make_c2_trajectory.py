@@ -6,11 +6,14 @@ from their own Fortran sources by oracle/Makefile, compute it with the C
 library's pow.
 
 Three steps, each a process of its own (the two long runs may go side by
-side; the oracle takes about half an hour, the reference longer):
+side; each takes about half an hour on one to four host cores):
 
     python tests/golden/make_c2_trajectory.py oracle [--workdir DIR]
     python tests/golden/make_c2_trajectory.py ref    [--workdir DIR]
     python tests/golden/make_c2_trajectory.py merge  [--workdir DIR]
+    python tests/golden/make_c2_trajectory.py cause-cr   [--workdir DIR]   (after merge)
+    python tests/golden/make_c2_trajectory.py cause-libm [--workdir DIR]
+    python tests/golden/make_c2_trajectory.py cause-merge [--workdir DIR]
 
 Per step the fixture holds dt, the active-cell count and the physics error
 bits; for every step whose assembled state held a NaN (the reference's
@@ -21,6 +24,13 @@ rounded sum of q(1) and, between the two runs, the relative L-infinity
 difference over wet cells (per variable scaled by its largest magnitude) and
 whether the wet masks agree.  The oracle also records its per-step work
 counters (the longest Newton solve and settling iteration of the step).
+
+The cause steps run the oracle again, once with correctly rounded x**y and
+once with the C library's pow (the reference's arithmetic; its dt sequence is
+checked against the reference's), and keep, at each step where the first run
+raised the NaN, the KT fluxes of the NaN cell's two x faces before the
+x-direction solid-flux cap (src/solver_2d.f90:2903-2912) in both runs, and
+the two runs' relative difference over wet cells at the step before.
 
 The reference pauses with READ(*,*) after printing a NaN cell's state; its
 standard input here is a file of empty lines, so each pause reads one and
@@ -48,7 +58,7 @@ sys.path.insert(0, ROOT)
 
 STEPS = 5000
 CHECKPOINTS = [150] + list(range(250, STEPS + 1, 250))
-FULL_STATES = (600, 2500, 5000)   # steps whose whole q is kept in the work dir
+FULL_STATES = (2500, 5000)   # steps whose whole q is kept in the work dir
 OUT = os.path.join(HERE, "c2_trajectory.npz")
 
 
@@ -183,13 +193,18 @@ def merge(wd):
     ev = _ref_nan_events(text, r["dts"])
     diff = {}
     for s in FULL_STATES:
+        if not (os.path.exists(os.path.join(wd, f"oracle_q_{s}.npy"))
+                and os.path.exists(os.path.join(wd, f"ref_q_{s}.npy"))):
+            continue
         qo = np.load(os.path.join(wd, f"oracle_q_{s}.npy"))
         qr = np.load(os.path.join(wd, f"ref_q_{s}.npy"))
         wo, wr = qo[:, :, 0] > 0.0, qr[:, :, 0] > 0.0
         same_mask = bool(np.array_equal(wo, wr))
         both = wo & wr
-        scale = np.maximum(np.abs(qo[both]).max(axis=0), 1e-300)
-        rel = float((np.abs(qo[both] - qr[both]) / scale).max()) if both.any() else 0.0
+        rel = 0.0
+        if both.any():
+            scale = np.maximum(np.abs(qo[both]).max(axis=0), 1e-300)
+            rel = float((np.abs(qo[both] - qr[both]) / scale).max())
         diff[str(s)] = dict(same_wet_mask=same_mask, rel_linf_wet=rel,
                             n_wet_oracle=int(wo.sum()), n_wet_ref=int(wr.sum()))
     dts_o, dts_r = o["dts"], r["dts"]
@@ -216,9 +231,67 @@ def merge(wd):
                           dt_max_rel=dt_rel), indent=1))
 
 
+def run_cause(wd, mode):
+    """Both oracle modes up to each NaN event of the correctly rounded run:
+    the watched faces' fluxes at the event steps and the state before."""
+    from oracle import oraclepy
+    g = np.load(OUT, allow_pickle=False)
+    events = [(int(a), int(b), int(c)) for a, b, c in zip(g["nan_step"], g["nan_j"], g["nan_k"])]
+    cfg, B, st, q0 = _setup()
+    o = oraclepy.Oracle(cfg, st, q0)
+    o.set_pow_mode(1 if mode == "cr" else 0)
+    done = 0
+    dts = []
+    out = {}
+    for step, j, k in events:
+        d, _, _ = o.run(step - 1 - done)
+        dts.append(d)
+        np.save(os.path.join(wd, f"cause_{mode}_q_{step - 1}.npy"), o.q())
+        o.watch_cell(j, k)
+        d, _, e = o.run(1)
+        dts.append(d)
+        done = step
+        pre, post, seen = o.watched_faces()
+        out[str(step)] = dict(cell=[j, k], err=int(e), seen=int(seen), pre=pre.tolist(),
+                              post=post.tolist())
+        print(f"{mode} step {step}: err {e}, faces seen {seen}", flush=True)
+    np.savez(os.path.join(wd, f"cause_{mode}.npz"), dts=np.concatenate(dts),
+             faces=json.dumps(out))
+
+
+def merge_cause(wd):
+    g = dict(np.load(OUT, allow_pickle=False))
+    cr = np.load(os.path.join(wd, "cause_cr.npz"), allow_pickle=False)
+    lm = np.load(os.path.join(wd, "cause_libm.npz"), allow_pickle=False)
+    n = len(lm["dts"])
+    assert np.array_equal(cr["dts"], g["dts"][:n]), "correctly rounded rerun differs"
+    fc, fl = json.loads(str(cr["faces"])), json.loads(str(lm["faces"]))
+    cause = {}
+    for step in fc:
+        s = int(step)
+        qa = np.load(os.path.join(wd, f"cause_cr_q_{s - 1}.npy"))
+        qb = np.load(os.path.join(wd, f"cause_libm_q_{s - 1}.npy"))
+        wet = (qa[:, :, 0] > 0.0) & (qb[:, :, 0] > 0.0)
+        scale = np.maximum(np.abs(qa[wet]).max(axis=0), 1e-300)
+        cause[step] = dict(cr=fc[step], libm=fl[step],
+                           rel_linf_wet_before=float((np.abs(qa[wet] - qb[wet]) / scale).max()),
+                           same_wet_mask_before=bool(np.array_equal(qa[:, :, 0] > 0.0,
+                                                                    qb[:, :, 0] > 0.0)))
+    # the libm oracle follows the reference's dt sequence up to the last event
+    g["libm_dts_equal_ref"] = np.bool_(np.array_equal(lm["dts"], g["ref_dts"][:n]))
+    g["libm_dts_max_rel_ref"] = np.float64(np.max(np.abs(lm["dts"] - g["ref_dts"][:n])
+                                                  / g["ref_dts"][:n]))
+    g["nan_cause"] = json.dumps(cause)
+    np.savez_compressed(OUT, **g)
+    print(json.dumps(dict(libm_equal_ref=bool(g["libm_dts_equal_ref"]),
+                          libm_dt_rel=float(g["libm_dts_max_rel_ref"]),
+                          cause=cause), indent=1))
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("mode", choices=["oracle", "ref", "ref-child", "merge"])
+    ap.add_argument("mode", choices=["oracle", "ref", "ref-child", "merge", "cause-cr",
+                                     "cause-libm", "cause-merge"])
     ap.add_argument("--workdir", default="/tmp/imex_c2_trajectory")
     ap.add_argument("--steps", type=int, default=STEPS)
     a = ap.parse_args()
@@ -229,8 +302,14 @@ def main():
         run_ref(a.workdir, a.steps)
     elif a.mode == "ref-child":
         run_ref_child(a.workdir, a.steps)
-    else:
+    elif a.mode == "merge":
         merge(a.workdir)
+    elif a.mode == "cause-cr":
+        run_cause(a.workdir, "cr")
+    elif a.mode == "cause-libm":
+        run_cause(a.workdir, "libm")
+    else:
+        merge_cause(a.workdir)
 
 
 if __name__ == "__main__":

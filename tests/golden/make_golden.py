@@ -2,7 +2,7 @@
 modules, compiled from their own Fortran sources into oracle/_ref by
 oracle/Makefile (IEEE-preserving flags).
 
-    make -C oracle ref && python tests/golden/make_golden.py
+    make -C oracle ref && python tests/golden/make_golden.py [case ...]
 
 Fixtures (numpy .npz, inputs and the reference's outputs):
   functions_<case>.npz  per-function vectors: qc_to_qp, qp_to_qc, mixt_var,
@@ -33,8 +33,8 @@ sys.path.insert(0, os.path.dirname(HERE))
 from oracle import refpy  # noqa: E402
 import cases  # noqa: E402
 
-RUN_STEPS = {"c1_48": 120, "c2_64": 150}
-FUNC_WARM = {"c1_48": 80, "c2_64": 100}
+RUN_STEPS = {"c1_48": 120, "c2_64": 150, "c3_64": 200}
+FUNC_WARM = {"c1_48": 80, "c2_64": 100, "c3_64": 120}
 
 
 def branch_states(cfg, eps_sing):
@@ -135,10 +135,13 @@ def run_fixture(name):
 def main():
     if not refpy.available("ieee"):
         sys.exit("oracle/_ref/libimexref.so is not built: make -C oracle ref")
+    only = set(sys.argv[1:])            # case names; all when none are given
     for name in FUNC_WARM:
-        functions_fixture(name)
+        if not only or name in only:
+            functions_fixture(name)
     for name in RUN_STEPS:
-        run_fixture(name)
+        if not only or name in only:
+            run_fixture(name)
 
 
 if __name__ == "__main__":

@@ -33,3 +33,19 @@ def test_every_named_workload_validates():
         assert B.shape == (cfg.ny, cfg.nx)
     cfg, B = workloads.config4_member(3, min_side=40, max_side=60)
     assert B.shape == (cfg.ny, cfg.nx)
+
+
+def test_traffic_is_read_only_for_the_profiled_window(tmp_path, monkeypatch):
+    (tmp_path / "profiles").mkdir()
+    (tmp_path / "profiles" / "c2_step_kernels_traffic.md").write_text(
+        "window: steps 2493..2512\nbytes_per_cell k_update: 180.5\n")
+    monkeypatch.setattr(bench, "HERE", str(tmp_path))
+    assert bench.read_traffic("k_update", "c2", (2493, 2512)) == 180.5
+    assert bench.read_traffic("k_update", "c2", (1, 5000)) is None
+    assert bench.read_traffic("k_flux", "c2", (2493, 2512)) is None
+
+
+def test_host_cores_counts_physical_cores():
+    n, topo = bench.host_cores()
+    assert 1 <= n <= 16
+    assert "socket" in topo

@@ -62,7 +62,7 @@ def _states(cfg, B, steps, seed):
     return np.concatenate([wet, wet * (1.0 + 1e-3 * rng.standard_normal(wet.shape))])
 
 
-@pytest.fixture(params=["c1_64", "c2_96", "c1m_64", "c2m_64"])
+@pytest.fixture(params=["c1_64", "c2_96", "c1m_64", "c2m_64", "c3_64"])
 def case(request):
     return cases.CASES[request.param]()
 
@@ -111,7 +111,7 @@ def test_functions_match_oracle(case):
     kp.close()
 
 
-@pytest.mark.parametrize("name", ["c1_48", "c2_64"])
+@pytest.mark.parametrize("name", ["c1_48", "c2_64", "c3_64"])
 def test_functions_match_reference_fixtures(name):
     """Against the reference's own outputs.  x**y enters only the settling
     velocity and the deposition / entrainment terms: there the reference's

@@ -42,6 +42,7 @@ def _compare_fields(s, o, what):
 
 
 @pytest.mark.parametrize("name, steps, mode", [("c1_64", 300, "host"), ("c1_64", 300, "device"),
+                                               ("c3_64", 250, "device"), ("c3_64", 150, "host"),
                                                ("c2_96", 400, "device"), ("c2_96", 100, "host")])
 def test_run_matches_oracle(name, steps, mode):
     cfg, B = cases.CASES[name]()

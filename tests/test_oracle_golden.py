@@ -10,8 +10,8 @@ import cases
 from oracle import oraclepy
 from imex_sflow2d_dev_amd import init
 
-FUNC_CASES = ["c1_48", "c2_64"]
-RUN_CASES = ["c1_48", "c2_64"]
+FUNC_CASES = ["c1_48", "c2_64", "c3_64"]
+RUN_CASES = ["c1_48", "c2_64", "c3_64"]
 
 
 def _load(name):

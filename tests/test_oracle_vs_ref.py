@@ -22,7 +22,7 @@ def _states(cfg, run, rng, n_pert=2):
 
 
 @pytest.mark.parametrize("name, warm", [("c1_64", 150), ("c2_96", 200), ("c1m_64", 150),
-                                        ("c2m_64", 200)])
+                                        ("c2m_64", 200), ("c3_64", 150)])
 def test_functions_on_fresh_states(ref_lib, name, warm):
     cfg, B = cases.CASES[name]()
     run = ref_lib.RefRun(cfg, B)
@@ -55,7 +55,7 @@ def test_functions_on_fresh_states(ref_lib, name, warm):
 
 
 @pytest.mark.parametrize("name, steps", [("c1_64", 300), ("c2_96", 400), ("c1m_64", 300),
-                                         ("c2m_64", 300)])
+                                         ("c2m_64", 300), ("c3_64", 200)])
 def test_multistep_bit_exact(ref_lib, name, steps):
     cfg, B = cases.CASES[name]()
     run = ref_lib.RefRun(cfg, B)
