@@ -4,27 +4,35 @@
 
 Workload (default "c2"): the production-size radial-source flow, 2048 x 2048
 cells of 5 m on a seeded fractal cone, two solid classes, sedimentation and
-air entrainment on, FP64.  One step is one driver step of the reference's time
-loop (src/IMEX_SfloW2D.f90:395-505): check_solve, timestep, imex_RK_solver
-(two RK stages), update_erosion_deposition_cell and the epilogue, all run on
-the device with no host synchronisation inside.  The run starts from the
-initial state, takes W untimed steps, then times exactly K steps (the default
-W=0, K=5000 times the whole 5000-step run).
+air entrainment on, FP64, a run of 5000 driver steps.  One step is one
+driver step of the reference's time loop (src/IMEX_SfloW2D.f90:395-505):
+check_solve, timestep, imex_RK_solver (two RK stages),
+update_erosion_deposition_cell and the epilogue, all run on the device with
+no host synchronisation inside.  The timed window sits in the middle of the
+run: untimed spin-up steps, W untimed warmup steps, then exactly K timed
+steps (the default W=0, K=5000 times the whole run).
 
 Metric: cell-stage updates/s = 2 * (active cells summed over the timed steps)
-/ wall time, where active cells are the check_solve set of each step.
+/ wall time, where active cells are the check_solve set of each step (the
+reference's definition, which includes the always-active boundary frame);
+config.work_cell_stage_updates_per_s counts only the cells the update kernel
+processes.  config.whole_run repeats the workload's whole run in the same
+process (wall clock, no events).
 
 The JSON line also carries
-  roofline:     the dominant kernel's algorithmic HBM bytes per launch over
-                its mean launch time (HIP events on the launch stream around
-                the launches of every E-th timed step, --event-every), vs
-                8 TB/s; traffic = measured HBM bytes per launch when a
-                rocprofv3 FETCH_SIZE/WRITE_SIZE summary of the step's
-                kernels is committed under profiles/, else null;
+  roofline:     the dominant kernel's algorithmic HBM bytes per launch (per
+                processed cell for the update kernel) over its mean launch
+                time from HIP events on the launch stream -- around the
+                launches of every E-th timed step for K >= 500
+                (--event-every), else around every launch of the K steps
+                that follow the timed window (events would perturb a short
+                window) -- vs 8 TB/s; traffic = measured HBM bytes per
+                launch when the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+                summary covers those same steps, else null;
   cpu_baseline: the reference OpenMP Fortran build (compiled from its own
-                sources into oracle/_ref) timed on this host's cores over a
-                bounded sample of steps that starts from the GPU state at the
-                middle of the timed region.
+                sources into oracle/_ref) timed on this host's physical
+                cores over a bounded sample of steps that starts from the
+                GPU state at the middle of the timed region.
 """
 from __future__ import annotations
 
@@ -272,15 +280,26 @@ def main():
     err_bits = 0
     work0, frame = s.work_counters()
     work = 0
+    # A short window carries no HIP events (each event record costs the
+    # stream microseconds, a sizeable share of a 20-step window): its kernel
+    # durations come from the same number of steps right after it, with
+    # events around every launch.  A long window samples every E-th step.
+    events_inside = args.steps >= 500
     for i, n in enumerate(halves):
         if n == 0:
             continue
         s_time(s)                               # synchronises the stream
+        a0, _ = s.counters()
         w0 = s.work_counters()[0]
         t0 = time.perf_counter()
-        # returns after a stream sync
-        _, k, ns, c = s.run_steps_timed(n, sample_every=args.event_every)
+        # both return after a stream sync
+        if events_inside:
+            _, k, ns, c = s.run_steps_timed(n, sample_every=args.event_every)
+        else:
+            s.run_steps(n, record_dt=False)
+            k, ns = np.zeros(len(KERNELS)), 0
         t1 = time.perf_counter()
+        c = s.counters()[0] - a0
         work += s.work_counters()[0] - w0
         err_bits |= s.err
         wall += t1 - t0
@@ -294,6 +313,17 @@ def main():
                        step=first - 1 + n)
     if mid is None:
         mid = dict(q=s.q, t=s.t, dt_old=s.dt_old, dt_old_old=s.dt_old_old, step=last)
+    if events_inside:
+        kwin = (first, last)
+        kcells, kwork = cells, work
+    else:
+        a0, _ = s.counters()
+        w0 = s.work_counters()[0]
+        _, kernel_ms, sampled, _ = s.run_steps_timed(args.steps, record_dt=False, sample_every=1)
+        kcells = s.counters()[0] - a0
+        kwork = s.work_counters()[0] - w0
+        err_bits |= s.err
+        kwin = (last + 1, last + args.steps)
     t_end_phys = s.t
     t_down = time.perf_counter()                # the state and the output fields to host memory
     s.q
@@ -333,15 +363,16 @@ def main():
     dom = int(np.argmax(kernel_ms))
     dname = names[dom]
     mean_launch_s = kernel_ms[dom] * 1e-3 / sampled
-    cells_per_launch = cells / K
-    work_per_launch = work / K
+    klaunches = kwin[1] - kwin[0] + 1
+    cells_per_launch = kcells / klaunches
+    work_per_launch = kwork / klaunches
     # the update kernel visits only its work list; the others are measured on
     # the active cells (their per-cell bytes are those of an active cell)
     basis_cells = work_per_launch if dname == "k_update" else cells_per_launch
     alg_bytes = per[dname] * basis_cells
     achieved = alg_bytes / mean_launch_s / 1e9
     achieved_active = per[dname] * cells_per_launch / mean_launch_s / 1e9
-    tb = read_traffic(dname, args.workload, (first, last))
+    tb = read_traffic(dname, args.workload, kwin)
     roofline = {
         "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS,
@@ -354,6 +385,7 @@ def main():
         "achieved_on_active_cells": achieved_active,
         "mean_launch_us": mean_launch_s * 1e6,
         "event_timed_launches": sampled,
+        "kernel_window_steps": list(kwin),
         "mean_active_cells_per_launch": cells_per_launch,
     }
     step_gbs = per["step"] * cells / wall / 1e9

@@ -196,7 +196,13 @@ def test_implicit_real_path_equals_complex_real_part(case):
     if cfg.rheology_model == 11:        # mu(I) alone: no implicit friction term
         assert fast.all() and not got.any()
     else:
-        assert fast[:len(wet)].mean() > 0.8, fast[:len(wet)].mean()
+        # flow states above the thin-layer threshold with every class present
+        # (a class that has settled out leaves zero intermediates, which the
+        # real evaluation hands to the complex one)
+        ns = cfg.n_solid
+        full = (wet[:, 4:4 + ns] > 0.0).all(axis=1) & (wet[:, 0] > cfg.eps_sing)
+        assert full.sum() > 50
+        assert fast[:len(wet)][full].mean() > 0.8, fast[:len(wet)][full].mean()
         assert not fast[len(wet):].all()
     kp.close()
 
