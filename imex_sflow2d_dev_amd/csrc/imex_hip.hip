@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1600,6 +1601,26 @@ __global__ __launch_bounds__(256) void k_diag(Params P, Grid G, double* part) {
   }
 }
 
+// The host's combination of k_diag's per-block pairs (imex_post_step), on
+// the device: thread m sums quantity m over the blocks in block order with
+// the same Neumaier steps; out[m] = sum + compensation.
+__global__ void k_diag_finish(const double* part, int nb, double* out) {
+  const int m = threadIdx.x;
+  if (m >= 4) return;
+  double sum = 0.0, comp = 0.0;
+  for (int b = 0; b < nb; b++) {
+    const double x = part[(b * 4 + m) * 2];
+    comp += part[(b * 4 + m) * 2 + 1];
+    const double t = sum + x;
+    if (fabs(sum) >= fabs(x))
+      comp += (sum - t) + x;
+    else
+      comp += (x - t) + sum;
+    sum = t;
+  }
+  out[m] = sum + comp;
+}
+
 // ===========================================================================
 // per-function batch kernels (known-answer tests)
 // ===========================================================================
@@ -1781,6 +1802,34 @@ static int frame_band(imex_handle* h, const signed char* scell) {
   return 0;
 }
 
+// Device scratch of imex_post_step_noalloc: the diagnostics kernel's
+// per-block (sum, compensation) pairs of the four quantities (the same
+// layout as the handle's own partials buffer) and the four finished sums.
+// One copy per GPU, held until the process ends; host code uses the address
+// hipGetSymbolAddress gives on each device, never the array's name.
+constexpr int kNoallocDiagBlocks = 512;
+static __device__ __align__(256) double scratch[kNoallocDiagBlocks * 8 + 8];
+constexpr int kMaxDevices = 64;
+static double* g_scratch_dev[kMaxDevices];
+static std::mutex g_scratch_mu;
+
+// the address of `scratch` on the current device h->device (looked up once)
+static int scratch_address(imex_handle* h, double** ptr) {
+  if (h->device < 0 || h->device >= kMaxDevices) {
+    h->err = "device index beyond the scratch table";
+    return -2;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[h->device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return set_err(h, "hipGetSymbolAddress(scratch)", e);
+    g_scratch_dev[h->device] = static_cast<double*>(p);
+  }
+  *ptr = g_scratch_dev[h->device];
+  return 0;
+}
+
 extern "C" int imex_create(const imex_config* cfg, imex_handle** out) {
   if (!cfg || !out) return -2;
   *out = nullptr;
@@ -1816,6 +1865,11 @@ extern "C" int imex_create(const imex_config* cfg, imex_handle** out) {
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->device) == hipSuccess &&
         ncu > 0)
       h->n_cu = ncu;
+    double* sp = nullptr;
+    if (scratch_address(h, &sp)) {
+      *out = h;
+      return -1;
+    }
   }
   Params& P = h->P;
   memset(&P, 0, sizeof(P));
@@ -2251,6 +2305,60 @@ extern "C" int imex_post_step(imex_handle* h, double t_new, imex_step_stats* out
       }
       tot[m] = sum + comp;
     }
+    const double a = h->P.dx * h->P.dy;
+    out->t = t_new;
+    out->dt = s.dt;
+    out->mass = a * tot[0];
+    out->volume = a * tot[1];
+    out->area = a * tot[2];
+    out->solid_mass = a * tot[3];
+    out->active_cells = (int64_t)s.active;
+    out->dried_cells = (int64_t)s.dried;
+    out->err = err;
+  } else {
+    HCHK(h, hipStreamSynchronize(h->stream));
+  }
+  return (int)(err & ~IMEX_WARN_RESCALE);
+}
+
+// imex_post_step without allocations: the diagnostics' partial sums and
+// their combination live in the file-scope device array `scratch` (at most
+// kNoallocDiagBlocks blocks of partials), finished on the device by
+// k_diag_finish; the host reads back the driver state and four sums into
+// stack memory.  Calls that overlap on one GPU share `scratch` and race.
+extern "C" int imex_post_step_noalloc(imex_handle* h, double t_new, imex_step_stats* out) {
+  if (!h) return -2;
+  HCHK(h, hipSetDevice(h->device));
+  double* sp = nullptr;
+  if (scratch_address(h, &sp)) return -1;
+  if (h->diag_blocks > kNoallocDiagBlocks) {
+    h->err = "imex_post_step_noalloc: " + std::to_string(h->diag_blocks) +
+                    " diagnostics blocks exceed the scratch of " +
+                    std::to_string(kNoallocDiagBlocks);
+    return -2;
+  }
+  int rc;
+  if ((rc = update(h, 4, 0.0, 0))) return rc;
+  DevState s;
+  if (read_state(h, &s)) return -1;
+  s.t = t_new;
+  const unsigned err = s.err;
+  s.err = 0;
+  HCHK(h, hipMemcpyAsync(h->G.st, &s, sizeof(s), hipMemcpyHostToDevice, h->stream));
+  if (out) {
+    const int nb = h->diag_blocks;
+    switch (h->P.ns) {
+      case 1: hipLaunchKernelGGL((h->fp32 ? k_diag<1, float> : k_diag<1, double>), dim3(nb), dim3(256), 0, h->stream, h->P, h->G, sp); break;
+      case 2: hipLaunchKernelGGL((h->fp32 ? k_diag<2, float> : k_diag<2, double>), dim3(nb), dim3(256), 0, h->stream, h->P, h->G, sp); break;
+      default: hipLaunchKernelGGL((h->fp32 ? k_diag<3, float> : k_diag<3, double>), dim3(nb), dim3(256), 0, h->stream, h->P, h->G, sp); break;
+    }
+    HCHK(h, hipGetLastError());
+    double* fin = sp + (size_t)kNoallocDiagBlocks * 8;
+    hipLaunchKernelGGL(k_diag_finish, dim3(1), dim3(64), 0, h->stream, sp, nb, fin);
+    HCHK(h, hipGetLastError());
+    double tot[4];
+    HCHK(h, hipMemcpyAsync(tot, fin, sizeof(tot), hipMemcpyDeviceToHost, h->stream));
+    HCHK(h, hipStreamSynchronize(h->stream));
     const double a = h->P.dx * h->P.dy;
     out->t = t_new;
     out->dt = s.dt;

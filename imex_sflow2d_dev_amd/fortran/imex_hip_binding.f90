@@ -147,6 +147,14 @@ MODULE imex_hip_binding
        REAL(c_double), VALUE :: t_new
        TYPE(c_ptr), VALUE :: stats              ! C_LOC(imex_step_stats) or c_null_ptr
      END FUNCTION imex_post_step
+     ! imex_post_step without allocations (no counterpart in the reference)
+     INTEGER(c_int) FUNCTION imex_post_step_noalloc(h, t_new, stats) &
+          BIND(C, name='imex_post_step_noalloc')
+       IMPORT :: c_int, c_ptr, c_double
+       TYPE(c_ptr), VALUE :: h
+       REAL(c_double), VALUE :: t_new
+       TYPE(c_ptr), VALUE :: stats
+     END FUNCTION imex_post_step_noalloc
      INTEGER(c_int) FUNCTION imex_run_steps(h, nsteps, t_end, dts) BIND(C, name='imex_run_steps')
        IMPORT :: c_int, c_ptr, c_double
        TYPE(c_ptr), VALUE :: h

@@ -45,6 +45,7 @@ EXPORTS = [
     "imex_host_free", "imex_write_records", "imex_write_restart", "imex_format_value",
     "imex_set_erodible", "imex_get_erosion", "imex_set_noise", "imex_test_mass_exchange_ers",
     "imex_topography_reconstruction", "imex_get_topography", "imex_get_work_counters",
+    "imex_post_step_noalloc",
 ]
 
 RUNOUT_STORE, RUNOUT_B = 1, 2

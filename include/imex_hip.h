@@ -134,6 +134,15 @@ int imex_rk_step(imex_handle *h, double dt);
 int imex_erosion_deposition(imex_handle *h, double dt);
 /* driver epilogue (src/IMEX_SfloW2D.f90:449-505) and the stdout diagnostics. */
 int imex_post_step(imex_handle *h, double t_new, imex_step_stats *out);
+/* imex_post_step without any allocation: no counterpart in the reference;
+ * for callers that cannot allocate (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc).  The diagnostics' per-block
+ * partial sums (at most 512 blocks, the handle's fixed count) and their
+ * combination live in a file-scope device array (32 KB per GPU, held until
+ * the process ends) instead of the handle's buffer and host memory; the
+ * result equals imex_post_step's bit for bit.  Calls that overlap on one GPU
+ * share that array and race. */
+int imex_post_step_noalloc(imex_handle *h, double t_new, imex_step_stats *out);
 /* Up to nsteps full driver steps on the device, no host synchronisation
  * inside: check_solve, timestep, the driver's dt clip to t_end and growth
  * limit (src/IMEX_SfloW2D.f90:417-439), imex_RK_solver, sedimentation,
