@@ -26,6 +26,7 @@ struct imex_handle {
   bool mask_full_next = true;  // the next k_mask visits every tile
   size_t lds = 0;
   std::vector<void*> allocs;
+  std::vector<size_t> alloc_sizes;  // bytes of each entry of allocs (0 once freed)
   double* d_tmp = nullptr;  // N*nv scratch for layout conversion
   double* d_diag = nullptr;
   int diag_blocks = 0;
@@ -43,7 +44,6 @@ struct imex_handle {
   bool copy_inflight = false;
   double* stage[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   size_t stage_cap[7] = {0, 0, 0, 0, 0, 0, 0};
-  unsigned int* d_flag = nullptr;
   std::string err;
 };
 
@@ -69,7 +69,21 @@ static inline int dalloc(imex_handle* h, T** p, size_t n) {
   if (e != hipSuccess) return set_err(h, "hipMalloc", e);
   HCHK(h, hipMemsetAsync(v, 0, n ? n * sizeof(T) : 16, h->stream));
   h->allocs.push_back(v);
+  h->alloc_sizes.push_back(n ? n * sizeof(T) : 16);
   *p = (T*)v;
+  return 0;
+}
+
+// frees one dalloc allocation (its user must be done with it: the caller
+// waits first) and drops it from the handle's list and footprint
+static inline int dfree(imex_handle* h, void* p) {
+  if (!p) return 0;
+  for (size_t i = 0; i < h->allocs.size(); i++)
+    if (h->allocs[i] == p) {
+      h->allocs[i] = nullptr;
+      h->alloc_sizes[i] = 0;
+    }
+  HCHK(h, hipFree(p));
   return 0;
 }
 

@@ -39,7 +39,7 @@ __device__ __forceinline__ void mask_tile(const Params& P, const Grid& G, int ti
   if (tid == 0) {
     s_any = 0;
     s_cnt = 0;
-    s_hot = P.all_tiles;
+    s_hot = P.all_tiles | (int)G.st->hot;
   }
   __syncthreads();
   for (int e = tid; e < RX * RY; e += NTHR) {
@@ -1755,9 +1755,7 @@ static int bind_dts(imex_handle* h, const double* dts, int nsteps) {
     if (int rc = dalloc(h, &p, (size_t)nsteps)) return rc;
     if (h->dts_buf) {
       HCHK(h, hipStreamSynchronize(h->stream));
-      for (auto& a : h->allocs)
-        if (a == h->dts_buf) a = nullptr;
-      HCHK(h, hipFree(h->dts_buf));
+      if (int rc = dfree(h, h->dts_buf)) return rc;
     }
     h->dts_buf = p;
     h->dts_cap = nsteps;
@@ -2224,6 +2222,7 @@ extern "C" int imex_upload_state(imex_handle* h, const double* q_aos, double t, 
   s.done = 0;
   s.stop_next = 0;
   s.steps = 0;
+  s.hot = 0;
   HCHK(h, hipMemcpyAsync(h->G.st, &s, sizeof(s), hipMemcpyHostToDevice, h->stream));
   int rc;
   if ((rc = mask(h, 1))) return rc;
@@ -2784,6 +2783,14 @@ extern "C" int imex_get_counters(imex_handle* h, int64_t* active_cells, int64_t*
   if (read_state(h, &s)) return -1;
   if (active_cells) *active_cells = (int64_t)s.active;
   if (steps) *steps = (int64_t)s.steps;
+  return 0;
+}
+
+extern "C" int imex_get_footprint(imex_handle* h, size_t* bytes) {
+  if (!h || !bytes) return -2;
+  size_t b = 0;
+  for (size_t x : h->alloc_sizes) b += x;
+  *bytes = b;
   return 0;
 }
 

@@ -62,6 +62,9 @@ struct DevState {
   unsigned int steps;        // device-driven steps committed since the upload
   unsigned int stop_next;    // single run, device-driven: the step taken last reaches t_end,
                              // so the following active-set passes do no tile work
+  unsigned int hot;          // set by a .q_2d snapshot that left a dry cell with other
+                             // non-zero components: every active-set pass then visits
+                             // every tile that holds one (cleared by an upload)
   unsigned long long dried;
   unsigned long long active;
   unsigned long long work;   // work-list cells summed over the stage-2 launches since create

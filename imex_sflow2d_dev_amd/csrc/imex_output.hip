@@ -95,8 +95,6 @@ __device__ void block_reduce(RunoutPart& mine, RunoutPart* out) {
   if (tid == 0) *out = sh[0];
 }
 
-__global__ void k_fill_u32(unsigned int* p, unsigned int v) { *p = v; }
-
 // The primitive thickness and SUM(qp(5:4+n_solid)) of one cell, as the
 // driver's module array qp holds them.
 template <int NS, typename S>
@@ -317,7 +315,7 @@ __host__ __device__ constexpr int p2d_cols(int ns) { return 19 + 4 * ns; }
 template <int NS, typename S>
 __global__ void k_snapshot(Params P, Grid G, const double* bcent, unsigned what, double* sq,
                            double* sq2d, double* sdep2d, double* sp2d, double* sfld,
-                           double* sero2d, double* serd2d, unsigned int* flag) {
+                           double* sero2d, double* serd2d) {
   constexpr int NV = 4 + NS;
   const int N = G.N;
   for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < N; c += gridDim.x * blockDim.x) {
@@ -337,11 +335,12 @@ __global__ void k_snapshot(Params P, Grid G, const double* bcent, unsigned what,
           }
           q[i] = 0.0;
         }
-      // a dry cell left with other non-zero components: the flux kernel must
-      // then visit every tile with active cells (as after such an upload)
+      // a dry cell left with other non-zero components: the active-set
+      // passes must then visit every tile with active cells (as after such
+      // an upload); the flag stays on the device, in stream order
       if (z && !(q[0] > 0.0))
         for (int i = 1; i < NV; i++) rest = rest || q[i] != 0.0;
-      if (rest) atomicOr(flag, 1u);
+      if (rest) atomicOr(&G.st->hot, 1u);
       double* r = sq2d + (size_t)c * (2 + NV);
       r[0] = X;
       r[1] = Y;
@@ -529,7 +528,6 @@ extern "C" int imex_snapshot_begin(imex_handle* h, unsigned what, double* q, dou
     HCHK(h, hipStreamCreateWithFlags(&h->copy_stream, hipStreamNonBlocking));
     HCHK(h, hipEventCreateWithFlags(&h->ev_staged, hipEventDisableTiming));
     HCHK(h, hipEventCreateWithFlags(&h->ev_copied, hipEventDisableTiming));
-    if (int rc = dalloc(h, &h->d_flag, 1)) return rc;
   }
   const size_t N = (size_t)h->G.N;
   for (int b = 0; b < NKIND; b++) {
@@ -537,6 +535,9 @@ extern "C" int imex_snapshot_begin(imex_handle* h, unsigned what, double* q, dou
     const size_t need = N * snap_cols(h, b);
     if (h->stage_cap[b] < need) {
       if (h->copy_inflight) HCHK(h, hipEventSynchronize(h->ev_copied));
+      HCHK(h, hipStreamSynchronize(h->stream));  // no earlier snapshot kernel still writes it
+      if (int rc = dfree(h, h->stage[b])) return rc;
+      h->stage[b] = nullptr;
       if (int rc = dalloc(h, &h->stage[b], need)) return rc;
       h->stage_cap[b] = need;
     }
@@ -546,12 +547,10 @@ extern "C" int imex_snapshot_begin(imex_handle* h, unsigned what, double* q, dou
   const int nb = std::max(1, std::min(8 * h->n_cu, (int)((N + 255) / 256)));
   const Params& P = h->P;
   const Grid& G = h->G;
-  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(1), 0, h->stream, h->d_flag, 0u);
 #define SNAP_LAUNCH(NS)                                                                       \
   hipLaunchKernelGGL((h->fp32 ? k_snapshot<NS, float> : k_snapshot<NS, double>), dim3(nb),   \
                      dim3(256), 0, h->stream, P, G, h->G.bcent, what, h->stage[0], h->stage[1], \
-                     h->stage[2], h->stage[3], h->stage[4], h->stage[5], h->stage[6],        \
-                     h->d_flag)
+                     h->stage[2], h->stage[3], h->stage[4], h->stage[5], h->stage[6])
   switch (P.ns) {
     case 1: SNAP_LAUNCH(1); break;
     case 2: SNAP_LAUNCH(2); break;
@@ -559,12 +558,6 @@ extern "C" int imex_snapshot_begin(imex_handle* h, unsigned what, double* q, dou
   }
 #undef SNAP_LAUNCH
   HCHK(h, hipGetLastError());
-  if (what & IMEX_SNAP_Q2D) {
-    unsigned int f = 0;
-    HCHK(h, hipMemcpyAsync(&f, h->d_flag, sizeof(f), hipMemcpyDeviceToHost, h->stream));
-    HCHK(h, hipStreamSynchronize(h->stream));
-    if (f) h->P.all_tiles = 1;
-  }
   HCHK(h, hipEventRecord(h->ev_staged, h->stream));
   HCHK(h, hipStreamWaitEvent(h->copy_stream, h->ev_staged, 0));
   for (int b = 0; b < NKIND; b++)

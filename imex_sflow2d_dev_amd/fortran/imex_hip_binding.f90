@@ -201,6 +201,11 @@ MODULE imex_hip_binding
        TYPE(c_ptr), VALUE :: h
        INTEGER(c_int64_t), INTENT(OUT) :: active_cells, steps
      END FUNCTION imex_get_counters
+     INTEGER(c_int) FUNCTION imex_get_footprint(h, bytes) BIND(C, name='imex_get_footprint')
+       IMPORT :: c_int, c_ptr, c_size_t
+       TYPE(c_ptr), VALUE :: h
+       INTEGER(c_size_t), INTENT(OUT) :: bytes
+     END FUNCTION imex_get_footprint
      INTEGER(c_int) FUNCTION imex_get_work_counters(h, work_cells, frame_cells) &
           BIND(C, name='imex_get_work_counters')
        IMPORT :: c_int, c_ptr, c_int64_t

@@ -45,7 +45,7 @@ EXPORTS = [
     "imex_host_free", "imex_write_records", "imex_write_restart", "imex_format_value",
     "imex_set_erodible", "imex_get_erosion", "imex_set_noise", "imex_test_mass_exchange_ers",
     "imex_topography_reconstruction", "imex_get_topography", "imex_get_work_counters",
-    "imex_post_step_noalloc",
+    "imex_post_step_noalloc", "imex_get_footprint",
 ]
 
 RUNOUT_STORE, RUNOUT_B = 1, 2
@@ -133,6 +133,7 @@ def lib():
         L.imex_write_restart.argtypes = [ctypes.c_char_p] + [ctypes.c_int] * 4 + \
             [ctypes.c_double] * 2 + [_D] * 9
         L.imex_format_value.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_char_p]
+        L.imex_get_footprint.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_size_t)]
         _LIB = L
     return _LIB
 

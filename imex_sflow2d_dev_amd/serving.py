@@ -23,15 +23,18 @@ from . import init, workloads
 from .ensemble import Batch
 from .solver import Solver
 
-# device bytes per cell of a one-class run: q, the right-hand side and the
-# layout buffer (3 x 5 doubles), maxima (3), deposit (1), bed slopes (2),
-# ring fields (3), mask, lists and source bytes (about 12 bytes)
+# first admission estimate, device bytes per cell of a one-class run: q, the
+# right-hand side and the layout buffer (3 x 5 doubles), maxima (3), deposit
+# (1), bed slopes (2), ring fields (3), mask, lists and source bytes (about
+# 12 bytes).  Once a run exists the server prices requests at the largest
+# bytes per cell a handle has reported (imex_get_footprint) and accounts each
+# admitted run at its handle's real footprint.
 BYTES_PER_CELL = 8 * (15 + 3 + 1 + 2 + 3) + 12
 
 
 class Request:
     __slots__ = ("idx", "arrival", "cfg", "static", "q0", "footprint", "admit", "finish",
-                 "solver", "cells_stage_updates", "steps", "t_end", "q", "t")
+                 "solver", "cells_stage_updates", "steps", "t_end", "q", "t", "over_budget")
 
     def __init__(self, idx, arrival, cfg, static, q0):
         self.idx, self.arrival, self.cfg, self.static, self.q0 = idx, arrival, cfg, static, q0
@@ -43,6 +46,7 @@ class Request:
         self.t_end = cfg.t_end
         self.q = None   # final state, when the server keeps results
         self.t = None
+        self.over_budget = False
 
 
 def make_requests(rate: float, duration: float, seed: int = 5, t_end=(60.0, 600.0),
@@ -67,12 +71,21 @@ def serve(reqs: List[Request], budget_bytes: float = 200e9, chunk: int = 20, log
           keep: bool = False):
     """Continuous batching at step boundaries.  Returns the wall time from the
     first arrival to the last result; fills admit/finish/steps per request
-    (and the final state and t when keep)."""
+    (and the final state and t when keep).
+
+    Admission is in arrival order (no request is overtaken, so none starves).
+    A request is admitted while the footprints of the running runs plus its
+    own fit the budget; a request larger than the whole budget is admitted
+    alone once nothing runs (r.over_budget is set), so the queue always
+    drains.  The head request is priced before its handle exists at the
+    largest bytes per cell seen so far; once created, the run is accounted at
+    its handle's real device footprint."""
     say = log or (lambda *a: None)
     b = Batch(n_solid=1)
     waiting, running = [], []
     pending = sorted(reqs, key=lambda r: r.arrival)
     used = 0.0
+    per_cell = float(BYTES_PER_CELL)
     t0 = time.perf_counter()
     nxt = 0
     while nxt < len(pending) or waiting or running:
@@ -81,9 +94,17 @@ def serve(reqs: List[Request], budget_bytes: float = 200e9, chunk: int = 20, log
             waiting.append(pending[nxt])
             nxt += 1
         # admission by footprint, in arrival order
-        while waiting and used + waiting[0].footprint <= budget_bytes:
-            r = waiting.pop(0)
+        while waiting:
+            r = waiting[0]
+            est = per_cell * r.cfg.nx * r.cfg.ny
+            if used + est > budget_bytes:
+                if running:
+                    break
+                r.over_budget = est > budget_bytes
+            waiting.pop(0)
             r.solver = Solver(r.cfg, r.static, r.q0)
+            r.footprint = r.solver.footprint()
+            per_cell = max(per_cell, r.footprint / (r.cfg.nx * r.cfg.ny))
             b.add(r.solver, r.t_end)
             r.admit = time.perf_counter() - t0
             used += r.footprint

@@ -325,6 +325,12 @@ class Solver:
     def snapshot_wait(self):
         self.H.check(self.L.imex_snapshot_wait(self.h), "imex_snapshot_wait")
 
+    def footprint(self) -> int:
+        """Bytes of device memory the run holds."""
+        b = ctypes.c_size_t()
+        self.H.check(self.L.imex_get_footprint(self.h, ctypes.byref(b)), "imex_get_footprint")
+        return int(b.value)
+
     def work_counters(self):
         """(cells the update kernel processed summed over the steps since
         create, cells of the always-active boundary frame per check_solve)."""

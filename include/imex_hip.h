@@ -184,6 +184,10 @@ int imex_get_counters(imex_handle *h, int64_t *active_cells, int64_t *steps);
  * check_solve adds to the active count (src/solver_2d.f90:351-354) while no
  * kernel has work there (measurement; no counterpart in the reference) */
 int imex_get_work_counters(imex_handle *h, int64_t *work_cells, int64_t *frame_cells);
+/* bytes of device memory the handle holds now (its state, static fields,
+ * lists, and any output staging allocated so far; no counterpart in the
+ * reference: admission of served runs by footprint) */
+int imex_get_footprint(imex_handle *h, size_t *bytes);
 /* topography_reconstruction (src/geometry_2d.f90:555-895, called at
  * src/IMEX_SfloW2D.f90:445 when topo_change_flag): the bed slopes
  * B_prime_x_geom / B_prime_y_geom from the device's B_cent, for host-driven
@@ -265,8 +269,10 @@ int imex_probes(imex_handle *h, int n, const double *xy, double *out);
  * write_restart_file (:8037-8084).  imex_snapshot_begin assembles the
  * selected records on the device (in stream order after the steps already
  * issued) and copies them to the caller's host buffers on a second stream;
- * it returns without waiting, so the next steps run while the copy (and the
- * caller's writing of the previous snapshot) proceeds.  The buffers must
+ * it returns without waiting (except at the first snapshot of a kind, or of a
+ * larger grid, which allocates its device staging), so the next steps run
+ * while the copy (and the caller's writing of the previous snapshot)
+ * proceeds.  The buffers must
  * stay valid, and be page-locked for the copy to overlap (imex_host_alloc),
  * until imex_snapshot_wait returns.  Like the reference, the .q_2d pass
  * zeroes |q| < 1e-20 in the device state and the .p_2d pass zeroes

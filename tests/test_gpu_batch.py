@@ -115,6 +115,34 @@ def test_served_results_equal_runs_alone():
         s.close()
 
 
+def test_serving_admits_an_over_budget_request_alone():
+    """Admission accounts each run at its handle's real device footprint
+    (imex_get_footprint); a request larger than the whole budget runs alone
+    once nothing else runs, and the queue drains."""
+    from imex_sflow2d_dev_amd import serving
+    reqs = serving.make_requests(rate=40.0, duration=0.2, seed=11, t_end=(1.0, 2.0),
+                                 side=(40, 70))
+    assert len(reqs) >= 2
+    s = Solver(reqs[0].cfg, reqs[0].static, reqs[0].q0)
+    fp = s.footprint()
+    s.close()
+    N = reqs[0].cfg.nx * reqs[0].cfg.ny
+    assert fp >= 8 * (2 * reqs[0].cfg.n_vars + 4) * N
+    budget = 0.5 * min(r.footprint for r in reqs)
+    serving.serve(reqs, budget_bytes=budget, chunk=5, keep=True)
+    assert all(r.over_budget and r.finish is not None for r in reqs)
+    assert reqs[0].footprint == fp
+    # one at a time: each admission follows the previous result
+    order = sorted(reqs, key=lambda r: r.admit)
+    for a, b in zip(order, order[1:]):
+        assert b.admit >= a.finish
+    for r in reqs:
+        s = Solver(r.cfg, r.static, r.q0)
+        run_alone(s, r.t_end)
+        _bits(s.q, r.q, f"request {r.idx}")
+        s.close()
+
+
 def test_members_with_different_friction_laws():
     """Voellmy, mu(I) (model 11) and stochastic-friction members in one
     batch: each member's friction law and noise travel with it."""
