@@ -200,6 +200,59 @@ __device__ __forceinline__ void qp_to_qp2(const F* qp, F& u, F& v) {
   }
 }
 
+// x direction, conservative-variable part: the W and E values of variable i
+// of cell (j,k) at local index idx, and the stencil abscissae xs0 / xs2 the
+// velocity part goes on with.
+template <typename F>
+__device__ __forceinline__ void recon_x_var(const Params& P, const Grid& G, const TileView<F>& T,
+                                            int idx, int j, int i, F& Wi, F& Ei, F& xs0, F& xs2) {
+  const int nx = G.nx;
+  const bool src2 = (T.ssc[idx] == 2);
+  const unsigned char sd = T.ssd[idx];
+  const bool sE = sd & 1, sW = sd & 2;
+  const F xs1 = F(G.xc[j]);
+  const F v1 = T.at(i, idx);
+  F pr;
+  if (j == 0) {
+    xs0 = F(G.xs[0]);
+    xs2 = F(G.xc[1]);
+    pr = src2 ? limit_minmod(v1, v1, T.at(i, idx + 1), xs0, xs1, xs2) : F(0.0);
+  } else if (j == nx - 1) {
+    xs2 = F(G.xs[nx]);
+    xs0 = F(G.xc[nx - 2]);
+    pr = src2 ? limit_minmod(T.at(i, idx - 1), v1, v1, xs0, xs1, xs2) : F(0.0);
+  } else {
+    xs0 = F(G.xc[j - 1]);
+    xs2 = F(G.xc[j + 1]);
+    F v0 = T.at(i, idx - 1), v2 = T.at(i, idx + 1);
+    if (src2) {
+      if (sE) {
+        xs2 = F(G.xs[j + 1]);
+        v2 = v1;
+      } else if (sW) {
+        xs0 = F(G.xs[j]);
+        v0 = v1;
+      }
+    }
+    pr = limit_minmod(v0, v1, v2, xs0, xs1, xs2);
+  }
+  const F dq = F(1.0) * F(P.dx2) * pr;
+  Wi = v1 - dq;
+  Ei = v1 + dq;
+  if (j == 0) {
+    if (src2)
+      Wi = v1;
+    else if (i == 1)
+      Wi = fmin_f(Wi, F(0.0));
+  }
+  if (j == nx - 1) {
+    if (src2)
+      Ei = v1;
+    else if (i == 1)
+      Ei = fmax_f(Ei, F(0.0));
+  }
+}
+
 // x direction: W and E face states (primitive) of cell (j,k) at local index idx.
 template <int NS, typename F>
 __device__ __forceinline__ void recon_x(const Params& P, const Grid& G, const TileView<F>& T, int idx,
@@ -213,48 +266,7 @@ __device__ __forceinline__ void recon_x(const Params& P, const Grid& G, const Ti
   const F xs1 = F(G.xc[j]);
   F xs0, xs2;
 #pragma unroll
-  for (int i = 0; i < NV; i++) {
-    const F v1 = T.at(i, idx);
-    F pr;
-    if (j == 0) {
-      xs0 = F(G.xs[0]);
-      xs2 = F(G.xc[1]);
-      pr = src2 ? limit_minmod(v1, v1, T.at(i, idx + 1), xs0, xs1, xs2) : F(0.0);
-    } else if (j == nx - 1) {
-      xs2 = F(G.xs[nx]);
-      xs0 = F(G.xc[nx - 2]);
-      pr = src2 ? limit_minmod(T.at(i, idx - 1), v1, v1, xs0, xs1, xs2) : F(0.0);
-    } else {
-      xs0 = F(G.xc[j - 1]);
-      xs2 = F(G.xc[j + 1]);
-      F v0 = T.at(i, idx - 1), v2 = T.at(i, idx + 1);
-      if (src2) {
-        if (sE) {
-          xs2 = F(G.xs[j + 1]);
-          v2 = v1;
-        } else if (sW) {
-          xs0 = F(G.xs[j]);
-          v0 = v1;
-        }
-      }
-      pr = limit_minmod(v0, v1, v2, xs0, xs1, xs2);
-    }
-    const F dq = F(1.0) * F(P.dx2) * pr;
-    W[i] = v1 - dq;
-    E[i] = v1 + dq;
-    if (j == 0) {
-      if (src2)
-        W[i] = v1;
-      else if (i == 1)
-        W[i] = fmin_f(W[i], F(0.0));
-    }
-    if (j == nx - 1) {
-      if (src2)
-        E[i] = v1;
-      else if (i == 1)
-        E[i] = fmax_f(E[i], F(0.0));
-    }
-  }
+  for (int i = 0; i < NV; i++) recon_x_var<F>(P, G, T, idx, j, i, W[i], E[i], xs0, xs2);
   // u and v with the limiters of hu and hv; the y part of u may have stored
   // the north ring-face point in the x stencil (src/solver_2d.f90:3607)
   const bool typo = src2 && sN && !sS && (k > 0) && (k < G.ny - 1);
@@ -308,6 +320,48 @@ __device__ __forceinline__ void recon_x(const Params& P, const Grid& G, const Ti
 }
 
 // y direction: S and N face states of cell (j,k).
+// y direction, conservative-variable part: the S and N values of variable i
+// of cell (j,k), and the stencil ordinates the velocity part goes on with.
+template <typename F>
+__device__ __forceinline__ void recon_y_var(const Params& P, const Grid& G, const TileView<F>& T,
+                                            int idx, int k, int i, F& Si, F& Ni, F& ys0, F& ys2) {
+  const int ny = G.ny;
+  const bool src2 = (T.ssc[idx] == 2);
+  const unsigned char sd = T.ssd[idx];
+  const bool sN = sd & 4, sS = sd & 8;
+  const F ys1 = F(G.yc[k]);
+  const F v1 = T.at(i, idx);
+  F pr;
+  if (k == 0) {
+    ys0 = F(G.ys[0]);
+    ys2 = F(G.yc[1]);
+    pr = F(0.0);
+  } else if (k == ny - 1) {
+    ys0 = F(G.yc[ny - 2]);
+    ys2 = F(G.ys[ny]);
+    pr = F(0.0);
+  } else {
+    ys0 = F(G.yc[k - 1]);
+    ys2 = F(G.yc[k + 1]);
+    F v0 = T.at(i, idx - FRX), v2 = T.at(i, idx + FRX);
+    if (src2) {
+      if (sS) {
+        ys0 = F(G.ys[k]);
+        v0 = v1;
+      } else if (sN) {
+        ys2 = F(G.ys[k + 1]);
+        v2 = v1;
+      }
+    }
+    pr = limit_minmod(v0, v1, v2, ys0, ys1, ys2);
+  }
+  const F dq = F(1.0) * F(P.dy2) * pr;
+  Si = v1 - dq;
+  Ni = v1 + dq;
+  if (k == 0 && i == 2) Si = fmin_f(Si, F(0.0));
+  if (k == ny - 1 && i == 2) Ni = fmax_f(Ni, F(0.0));
+}
+
 template <int NS, typename F>
 __device__ __forceinline__ void recon_y(const Params& P, const Grid& G, const TileView<F>& T, int idx,
                                         int j, int k, F* S, F* Nn) {
@@ -320,38 +374,7 @@ __device__ __forceinline__ void recon_y(const Params& P, const Grid& G, const Ti
   const F ys1 = F(G.yc[k]);
   F ys0, ys2;
 #pragma unroll
-  for (int i = 0; i < NV; i++) {
-    const F v1 = T.at(i, idx);
-    F pr;
-    if (k == 0) {
-      ys0 = F(G.ys[0]);
-      ys2 = F(G.yc[1]);
-      pr = F(0.0);
-    } else if (k == ny - 1) {
-      ys0 = F(G.yc[ny - 2]);
-      ys2 = F(G.ys[ny]);
-      pr = F(0.0);
-    } else {
-      ys0 = F(G.yc[k - 1]);
-      ys2 = F(G.yc[k + 1]);
-      F v0 = T.at(i, idx - FRX), v2 = T.at(i, idx + FRX);
-      if (src2) {
-        if (sS) {
-          ys0 = F(G.ys[k]);
-          v0 = v1;
-        } else if (sN) {
-          ys2 = F(G.ys[k + 1]);
-          v2 = v1;
-        }
-      }
-      pr = limit_minmod(v0, v1, v2, ys0, ys1, ys2);
-    }
-    const F dq = F(1.0) * F(P.dy2) * pr;
-    S[i] = v1 - dq;
-    Nn[i] = v1 + dq;
-    if (k == 0 && i == 2) S[i] = fmin_f(S[i], F(0.0));
-    if (k == ny - 1 && i == 2) Nn[i] = fmax_f(Nn[i], F(0.0));
-  }
+  for (int i = 0; i < NV; i++) recon_y_var<F>(P, G, T, idx, k, i, S[i], Nn[i], ys0, ys2);
 #pragma unroll
   for (int ii = 0; ii < 2; ii++) {
     const int i = NV + ii;
@@ -456,32 +479,66 @@ __device__ __forceinline__ void face_finish(const Params& P, const TileView<F>& 
   }
 }
 
-// The primitive face states of face_states alone (no conservative states),
-// for the timestep variant whose fluxes are not used, only its wave speeds.
+// eval_speeds for one face (the a- / a+ of kt_flux)
 template <int NS, typename F>
-__device__ __forceinline__ void face_prims(const Params& P, const Grid& G, const TileView<F>& T, int dir,
-                                           int idxA, int idxB, int ja, int ka, int jb, int kb,
-                                           bool hasA, bool hasB, F* qpA, F* qpB) {
+__device__ __forceinline__ void kt_speeds(const Params& P, int dir, const F* qpA, const F* qpB,
+                                          F& an, F& ap) {
+  F amin, amax, bmin, bmax;
+  local_speeds<NS, F>(P, qpA, dir, amin, amax);
+  local_speeds<NS, F>(P, qpB, dir, bmin, bmax);
+  an = fmin_f(fmin_f(amin, bmin), F(0.0));
+  ap = fmax_f(fmax_f(amax, bmax), F(0.0));
+}
+
+// The wave speeds of one face in timestep's variant (cells of the source disc
+// carry T = T_ambient): that variant's face states and kt_speeds, from the
+// side states the tile already holds.  Under that view a side state differs only in its
+// temperature (variable 3, next to a disc cell), which is recomputed here
+// with the reconstruction's own rules; qpA / qpB hold on entry the side
+// states in LDS (E or N of A, W or S of B).
+template <int NS, typename F>
+__device__ __forceinline__ void disc_speeds(const Params& P, const Grid& G, const TileView<F>& T0,
+                                            int dir, int idxA, int idxB, int ja, int ka, int jb,
+                                            int kb, bool hasA, bool hasB, F* qpA, F* qpB, F& an,
+                                            F& ap) {
   constexpr int NV = 4 + NS, NP = NV + 2;
-  F tmp[NP];
+  F lo, hi, c0, c2;
   if (hasA) {
-    if (T.ssc[idxA] == 1) {
+    if (T0.ssc[idxA] == 1) {
 #pragma unroll
       for (int i = 0; i < NP; i++) qpA[i] = F(0.0);
-    } else if (dir == 1) {
-      recon_x<NS, F>(P, G, T, idxA, ja, ka, tmp, qpA);
     } else {
-      recon_y<NS, F>(P, G, T, idxA, ja, ka, tmp, qpA);
+      const bool src2 = T0.ssc[idxA] == 2;
+      const unsigned char sd = T0.ssd[idxA];
+      if (dir == 1) {
+        recon_x_var<F>(P, G, T0, idxA, ja, 3, lo, hi, c0, c2);
+        // recon_x: a ring cell facing the disc eastwards keeps its own E state
+        if (ja != 0 && ja != G.nx - 1 && src2 && (sd & 1)) hi = T0.at(3, idxA);
+      } else {
+        recon_y_var<F>(P, G, T0, idxA, ka, 3, lo, hi, c0, c2);
+        // recon_y: ... northwards (and not southwards) keeps its own N state
+        if (ka != 0 && ka != G.ny - 1 && src2 && !(sd & 8) && (sd & 4)) hi = T0.at(3, idxA);
+      }
+      qpA[3] = hi;
     }
   }
   if (hasB) {
-    if (T.ssc[idxB] == 1) {
+    if (T0.ssc[idxB] == 1) {
 #pragma unroll
       for (int i = 0; i < NP; i++) qpB[i] = F(0.0);
-    } else if (dir == 1) {
-      recon_x<NS, F>(P, G, T, idxB, jb, kb, qpB, tmp);
     } else {
-      recon_y<NS, F>(P, G, T, idxB, jb, kb, qpB, tmp);
+      const bool src2 = T0.ssc[idxB] == 2;
+      const unsigned char sd = T0.ssd[idxB];
+      if (dir == 1) {
+        recon_x_var<F>(P, G, T0, idxB, jb, 3, lo, hi, c0, c2);
+        // ... westwards (and not eastwards) keeps its own W state
+        if (jb != 0 && jb != G.nx - 1 && src2 && !(sd & 1) && (sd & 2)) lo = T0.at(3, idxB);
+      } else {
+        recon_y_var<F>(P, G, T0, idxB, kb, 3, lo, hi, c0, c2);
+        // ... southwards keeps its own S state
+        if (kb != 0 && kb != G.ny - 1 && src2 && (sd & 8)) lo = T0.at(3, idxB);
+      }
+      qpB[3] = lo;
     }
   }
   const int vel = (dir == 1) ? NV : NV + 1;
@@ -493,26 +550,16 @@ __device__ __forceinline__ void face_prims(const Params& P, const Grid& G, const
   } else if (!hasB) {
 #pragma unroll
     for (int i = 0; i < NP; i++) qpB[i] = qpA[i];
-  } else if (T.ssc[idxA] == 2 && (T.ssd[idxA] & bitA)) {
+  } else if (T0.ssc[idxA] == 2 && (T0.ssd[idxA] & bitA)) {
 #pragma unroll
     for (int i = 0; i < NP; i++) qpB[i] = qpA[i];
     qpB[vel] = -qpA[vel];
-  } else if (T.ssc[idxB] == 2 && (T.ssd[idxB] & bitB)) {
+  } else if (T0.ssc[idxB] == 2 && (T0.ssd[idxB] & bitB)) {
 #pragma unroll
     for (int i = 0; i < NP; i++) qpA[i] = qpB[i];
     qpA[vel] = -qpB[vel];
   }
-}
-
-// eval_speeds for one face (the a- / a+ of kt_flux)
-template <int NS, typename F>
-__device__ __forceinline__ void kt_speeds(const Params& P, int dir, const F* qpA, const F* qpB,
-                                          F& an, F& ap) {
-  F amin, amax, bmin, bmax;
-  local_speeds<NS, F>(P, qpA, dir, amin, amax);
-  local_speeds<NS, F>(P, qpB, dir, bmin, bmax);
-  an = fmin_f(fmin_f(amin, bmin), F(0.0));
-  ap = fmax_f(fmax_f(amax, bmax), F(0.0));
+  kt_speeds<NS, F>(P, dir, qpA, qpB, an, ap);
 }
 
 // eval_speeds + eval_flux_KT + average_KT for one face:
@@ -742,8 +789,12 @@ __device__ __forceinline__ void flux_body(const Params& P, const Grid& G, int mo
           }
         }
         if (disc) {  // timestep's wave speeds see T_ambient in the source disc
-          face_prims<NS, F>(P, G, T0, 1, idxA, idxB, jf - 1, k, jf, k, hasA, hasB, qpA, qpB);
-          kt_speeds<NS, F>(P, 1, qpA, qpB, an, ap);
+#pragma unroll
+          for (int i = 0; i < NP; i++) {
+            qpA[i] = sRE[i * NRX + ra];
+            qpB[i] = sRW[i * NRX + ra + 1];
+          }
+          disc_speeds<NS, F>(P, G, T0, 1, idxA, idxB, jf - 1, k, jf, k, hasA, hasB, qpA, qpB, an, ap);
         }
         amax = fmax_f(ap, -an);
       }
@@ -787,8 +838,12 @@ __device__ __forceinline__ void flux_body(const Params& P, const Grid& G, int mo
           }
         }
         if (disc) {  // timestep's wave speeds see T_ambient in the source disc
-          face_prims<NS, F>(P, G, T0, 2, idxA, idxB, j, kf - 1, j, kf, hasA, hasB, qpA, qpB);
-          kt_speeds<NS, F>(P, 2, qpA, qpB, an, ap);
+#pragma unroll
+          for (int i = 0; i < NP; i++) {
+            qpA[i] = sRN[i * NRY + ra];
+            qpB[i] = sRS[i * NRY + ra + FX];
+          }
+          disc_speeds<NS, F>(P, G, T0, 2, idxA, idxB, j, kf - 1, j, kf, hasA, hasB, qpA, qpB, an, ap);
         }
         amax = fmax_f(ap, -an);
       }
@@ -1306,7 +1361,7 @@ __device__ void stage2(const Params& P, double xi, double dt, double* q, const d
 template <int NS, int LANES>
 __device__ void erosion_deposition(const Params& P, double dt, double* q, double* depo,
                                    double* erod, double* ero, double& topo, unsigned int& err,
-                                   int lane) {
+                                   const ddm::dd* tab, int lane) {
   constexpr int NV = 4 + NS, NP = NV + 2;
   double qp[NP], pd, dep[NS], eq[NV], ers[NS];
   if (q[0] > 0.0) {
@@ -1316,7 +1371,7 @@ __device__ void erosion_deposition(const Params& P, double dt, double* q, double
     for (int i = 0; i < NP; i++) qp[i] = 0.0;
     qp[3] = P.T_amb;
   }
-  if (mass_exchange<NS, LANES>(P, qp, erod, dt, dep, eq, ers, topo, lane)) err |= IMEX_ERR_NEG_DEP;
+  if (mass_exchange<NS, LANES>(P, qp, erod, dt, dep, eq, ers, topo, tab, lane)) err |= IMEX_ERR_NEG_DEP;
 #pragma unroll
   for (int i = 0; i < NV; i++) q[i] = q[i] + dt * eq[i];
 #pragma unroll
@@ -1453,38 +1508,77 @@ __device__ __forceinline__ void update_body(const Params& P, const Grid& G, doub
   unsigned int err = 0;
   int dried = 0;
   const int stride = (int)(gridDim.x * NTHR) / LANES;
+  // the x**y tables of the sedimentation in LDS, in blocks that have cells
+  // (the condition is the same for every thread of a block)
+  __shared__ ddm::dd s_pow[((PARTS & 2) != 0) ? ddm::POW_TAB_N : 1];
+  if constexpr ((PARTS & 2) != 0) {
+    if ((int)(blockIdx.x * NTHR) / LANES < n) pow_tables_to_lds(s_pow);
+  }
+  constexpr bool SED = (PARTS & 2) != 0;
+  const bool sed = SED && (P.settling || P.entrainment || P.erosion_coeff > 0.0);
+  const bool er = sed && G.erodible != nullptr;
   for (int i = (int)(blockIdx.x * NTHR + threadIdx.x) / LANES; i < n; i += stride) {
     const int c = ALL ? i : G.work[i];
-    double q[NV];
+    double q[NV], X[NV];
 #pragma unroll
     for (int v = 0; v < NV; v++) q[v] = (double)state_q<S>(G)[(size_t)v * N + c];
     if constexpr ((PARTS & 1) != 0) {
-      // remember "wet at the start of the step" in bit 1 of the mask byte for
-      // the dried-cell count of the sedimentation launch
-      const bool wet0 = q[0] > 0.0;
-      if (lead) G.mask[c] = (unsigned char)(1u | (wet0 ? 2u : 0u));
-      double X[NV];
 #pragma unroll
       for (int v = 0; v < NV; v++) X[v] = (double)state_R<S>(G)[(size_t)v * N + c];
-      stage2<NS, LANES>(P, cell_xi(P, G.noise, c), dt, q, X, err, lane);
     }
-    if constexpr ((PARTS & 2) != 0) {
-      if (P.settling || P.entrainment || P.erosion_coeff > 0.0) {
-        double depo[NS], erod[NS], ero[NS];
+    const bool wet0 = q[0] > 0.0;
+    if constexpr ((PARTS & 1) != 0) stage2<NS, LANES>(P, cell_xi(P, G.noise, c), dt, q, X, err, lane);
+    // The loads the rest of the cell's update needs are issued here, before
+    // the sedimentation (whose settling fixed point covers their latency),
+    // not where their values are first used: a load issued at its use would
+    // wait for its memory latency on the launch's critical path.  They are
+    // not issued before the Newton solve, whose registers are the kernel's
+    // peak.
+    double depo[NS], erod[NS], ero[NS], bc = 0.0;
+    unsigned char m0 = 0;
+    if constexpr (SED) {
 #pragma unroll
-        for (int v = 0; v < NS; v++) depo[v] = G.dep[(size_t)v * N + c];
-        const bool er = G.erodible != nullptr;
-#pragma unroll
-        for (int v = 0; v < NS; v++) {
-          erod[v] = er ? G.erodible[(size_t)v * N + c] : 0.0;
-          ero[v] = er ? G.erosion[(size_t)v * N + c] : 0.0;
-        }
+      for (int v = 0; v < NS; v++) {
+        depo[v] = sed ? G.dep[(size_t)v * N + c] : 0.0;
+        erod[v] = er ? G.erodible[(size_t)v * N + c] : 0.0;
+        ero[v] = er ? G.erosion[(size_t)v * N + c] : 0.0;
+      }
+      if (sed && P.topo_change) bc = G.bcent[c];
+      // without the stage-2 part in this launch, "wet at the step start" is
+      // bit 1 of the mask byte that launch left
+      if constexpr ((PARTS & 1) == 0) m0 = G.mask[c];
+    }
+    double hm = 0.0, pm = 0.0, mv = 0.0;
+    unsigned long long vb = 0ULL;
+    if constexpr ((PARTS & 12) != 0) {
+      hm = G.hmax[c];
+      pm = G.pdyn[c];
+      mv = G.mvel[c];
+      vb = G.vuln ? G.vuln[c] : 0ULL;
+    }
+    if constexpr (SED) {
+      if (sed) {
         double topo;
-        erosion_deposition<NS, LANES>(P, dt, q, depo, erod, ero, topo, err, lane);
-        if (lead) {
+        erosion_deposition<NS, LANES>(P, dt, q, depo, erod, ero, topo, err, s_pow, lane);
+        if (P.topo_change) bc = bc + dt * topo;
+      }
+      // wet at the step start, exactly dry after the step's last change of q
+      const bool was_wet = ((PARTS & 1) != 0) ? wet0 : ((m0 & 2u) != 0);
+      if (lead && was_wet && q[0] == 0.0) dried++;
+    }
+    if constexpr ((PARTS & 12) != 0) {
+      if (lead) epilogue<NS>(P, q, hm, pm, mv, vb, ALL);
+    }
+    // the stores, all after the arithmetic
+    if (lead) {
+      // "wet at the step start" in bit 1 of the mask byte, for the dried-cell
+      // count of a sedimentation launch of its own
+      if constexpr ((PARTS & 1) != 0) G.mask[c] = (unsigned char)(1u | (wet0 ? 2u : 0u));
+      if constexpr (SED) {
+        if (sed) {
 #pragma unroll
           for (int v = 0; v < NS; v++) G.dep[(size_t)v * N + c] = depo[v];
-          if (P.topo_change) G.bcent[c] = G.bcent[c] + dt * topo;
+          if (P.topo_change) G.bcent[c] = bc;
           if (er) {
 #pragma unroll
             for (int v = 0; v < NS; v++) {
@@ -1494,21 +1588,11 @@ __device__ __forceinline__ void update_body(const Params& P, const Grid& G, doub
           }
         }
       }
-      // wet at the step start (flag left by the stage-2 launch), exactly dry
-      // after the step's last change of q
-      if (lead && (G.mask[c] & 2u) && q[0] == 0.0) dried++;
-    }
-    if constexpr ((PARTS & 3) != 0) {
-      if (lead) {
+      if constexpr ((PARTS & 3) != 0) {
 #pragma unroll
         for (int v = 0; v < NV; v++) state_q<S>(G)[(size_t)v * N + c] = S(q[v]);
       }
-    }
-    if constexpr ((PARTS & 12) != 0) {
-      if (lead) {
-        double hm = G.hmax[c], pm = G.pdyn[c], mv = G.mvel[c];
-        unsigned long long vb = G.vuln ? G.vuln[c] : 0ULL;
-        epilogue<NS>(P, q, hm, pm, mv, vb, ALL);
+      if constexpr ((PARTS & 12) != 0) {
         G.hmax[c] = hm;
         G.pdyn[c] = pm;
         G.mvel[c] = mv;
@@ -1694,12 +1778,16 @@ __global__ void kt_expl(Params P, int n, const double* bx, const double* by, dou
 }
 __global__ void kt_settling(Params P, int n, const double* d, const double* rs, const double* rc,
                             const double* ikv, double* out) {
+  __shared__ ddm::dd tab[ddm::POW_TAB_N];
+  pow_tables_to_lds(tab);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = settling_velocity(P, d[i], rs[i], rc[i], ikv[i]);
+  if (i < n) out[i] = settling_velocity(P, d[i], rs[i], rc[i], ikv[i], tab);
 }
 template <int NS>
 __global__ void kt_mass_exchange(Params P, int n, const double* qp, const double* erod, double dt,
                                  double* dep, double* eq, double* ers) {
+  __shared__ ddm::dd tab[ddm::POW_TAB_N];
+  pow_tables_to_lds(tab);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
     double e[NS], r[NS];
@@ -1707,7 +1795,7 @@ __global__ void kt_mass_exchange(Params P, int n, const double* qp, const double
     for (int k = 0; k < NS; k++) e[k] = erod ? erod[(size_t)NS * i + k] : 0.0;
     double topo;
     mass_exchange<NS>(P, qp + (size_t)(6 + NS) * i, e, dt, dep + (size_t)NS * i,
-                      eq + (size_t)(4 + NS) * i, r, topo);
+                      eq + (size_t)(4 + NS) * i, r, topo, tab);
     if (ers)
 #pragma unroll
       for (int k = 0; k < NS; k++) ers[(size_t)NS * i + k] = r[k];
@@ -1737,9 +1825,13 @@ __global__ void kt_arith(int n, const double* a, const double* b, double* out) {
   }
 }
 
+// even elements through the tables in LDS, odd ones through those in global
+// memory: both lookups the kernels use
 __global__ void kt_pow(int n, const double* x, const double* y, double* out) {
+  __shared__ ddm::dd tab[ddm::POW_TAB_N];
+  pow_tables_to_lds(tab);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = pow_cr(x[i], y[i]);
+  if (i < n) out[i] = (i & 1) ? pow_cr(x[i], y[i]) : pow_cr(x[i], y[i], tab);
 }
 
 // ===========================================================================

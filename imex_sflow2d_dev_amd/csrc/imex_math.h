@@ -404,29 +404,66 @@ __host__ __device__ __forceinline__ double exp_dd(dd z) {
   return ldexp(f, -1074);
 }
 
-// The double-double x**y above, kept out of line: pow_fast settles all but a
-// small fraction of the calls.
+// The double-double x**y above, kept out of line: pow_fast settles all but
+// about one call in 10**8.
 __host__ __device__ __noinline__ inline double pow_slow(double x, double y) {
   return exp_dd(mul_d(log_dd(x), y));
 }
 
-// ln2 = LN2H + LN2L with LN2H of 42 bits (e * LN2H is exact for |e| <= 2048);
-// ln2/128 = LN2_128H + LN2_128L with LN2_128H of 36 bits (n * LN2_128H is
-// exact for |n| < 2**17)
+// ln2 = LN2H + LN2L (to 2**-102) with LN2H of 42 bits (e * LN2H is exact for
+// |e| <= 2048); ln2/128 = LN2_128H + LN2_128L + LN2_128LL (to 2**-154) with
+// LN2_128H of 36 bits (n * LN2_128H is exact for |n| < 2**17)
 constexpr double LN2H = 0x1.62e42fefa3800p-1;
 constexpr double LN2L = 0x1.ef35793c76730p-45;
 constexpr double LN2_128H = 0x1.62e42fefa0000p-8;
 constexpr double LN2_128L = 0x1.cf79abc9e3b3ap-47;
+constexpr double LN2_128LL = -0x1.ff0342542fc33p-101;
+// 2/3 and 1/6 as pairs (to 2**-108)
+constexpr double C3H = 0x1.5555555555555p-1, C3L = 0x1.5555555555555p-55;
+constexpr double C6H = 0x1.5555555555555p-3, C6L = 0x1.5555555555555p-57;
 
-// Ziv's strategy, first step: x**y to a relative error below
-// eps = (|y log x| + |y|/128 + 2) 2**-67 with the same tables and mostly
-// plain double arithmetic (error-free sums and products only where the
-// leading terms need them).  Returns false unless the result rounds the same
-// way at both ends of the error interval; the caller then takes pow_slow.
-//   log x:  2 atanh(s) = 2 s + 2 s**3/3 + 2 s**5/5 + 2 s**7/7 with s as an
-//           unevaluated pair sh + sl (|s| < 0.0028, truncation < 2**-71 |2s|);
-//   exp r:  r + r**2/2 + ... + r**6/720 (|r| < 0.0028, truncation < 2**-72).
-__host__ __device__ __forceinline__ bool pow_fast(double x, double y, double& out) {
+// 1/d to within 2**-51 relative for d in [1, 4]: the hardware estimate on
+// the device (v_rcp_f64) and two Newton steps, which square its error
+__host__ __device__ __forceinline__ double recip(double d) {
+#ifdef __HIP_DEVICE_COMPILE__
+  double r = __builtin_amdgcn_rcp(d);
+#else
+  double r = 1.0 / d;
+#endif
+  r = fma(r, fma(-d, r, 1.0), r);
+  return fma(r, fma(-d, r, 1.0), r);
+}
+
+// The two tables as one array of POW_TAB_N pairs: log(1 + (i - 38)/128)
+// for i < 92, then 2**((i - 92)/128).  A kernel that evaluates x**y often
+// copies them to LDS (pow_tables_to_lds) and passes that copy to pow_cr:
+// from global memory each lookup is a dependent load whose latency, when the
+// line has left the vector L1, is several times that of LDS.
+constexpr int POW_TAB_N = 92 + 128;
+__host__ __device__ __forceinline__ dd pow_tab_entry(int i) {
+  return (i < 92) ? log_table(i - 38) : exp2_table(i - 92);
+}
+
+// Ziv's strategy, first step: x**y as the pair R = R.hi + R.lo and a bound
+// eps on its relative error, with the same tables and double arithmetic
+// whose every rounding below 2**-85 of the result is made exact by
+// error-free sums and products.  Returns false when the argument of exp
+// leaves (-708, 708).
+//   log x = e ln2 + log c + 2 atanh(s), s = (m - c)/(m + c) = sh + sl,
+//           |s| < 2**-8.48:  2 atanh(s) = 2s + (2/3)s**3 + (2/5)s**5 +
+//           (2/7)s**7 + (2/9)s**9 (truncation < 2**-88 |2s|), the s**3 term
+//           and its sl part exact to 2**-100, the higher terms in double
+//           (< 2**-87 |2s|); |2s| < 2.4 |log x| wherever the table point c
+//           is not 1, so the logarithm is within 2**-85 |log x|;
+//   z = y log x as a pair (error < 2**-85 |z| + 2**-104 |z|);
+//   exp z = 2**e2 2**(j/128) exp(r), r = z - n ln2/128 = r1 + r2 exact to
+//           2**-104 |z|, |r1| < 2**-8.47, |r2| < 2**-62:
+//           exp(r) - 1 = r1 + r1**2/2 + r1**3 (1/6 + r1 P(r1)) + r2 (1 + r1
+//           + r1**2/2), P to r1**5/40320 (truncation < 2**-94), the first
+//           three terms exact to 2**-89;
+// so eps = 2**-84 |z| + 2**-87 holds with a margin of two.
+__host__ __device__ __forceinline__ bool pow_fast_pair(double x, double y, dd& R, double& eps,
+                                                       int& e2, const dd* tab = nullptr) {
   int e = 0;
   double m = frexp(x, &e);
   if (m < SQRT_HALF) {
@@ -435,44 +472,77 @@ __host__ __device__ __forceinline__ bool pow_fast(double x, double y, double& ou
   }
   const int k = (int)rint((m - 1.0) * 128.0);
   const double c = 1.0 + k * 0x1.0p-7;
-  const double num = m - c;  // exact
-  const dd den = two_sum(m, c);
-  const double rden = 1.0 / den.hi;
+  const double num = m - c;      // exact
+  const dd den = two_sum(m, c);  // den.hi in [1.4, 2.9]
+  const double rden = recip(den.hi);
   const double sh = num * rden;
   const double sl = (fma(-sh, den.hi, num) - sh * den.lo) * rden;
-  const double t = sh * sh;
-  const double poly = sh * t * (0x1.5555555555555p-1 + t * (0x1.999999999999ap-2 + t * 0x1.2492492492492p-2));
-  const dd lc = log_table(k);
+  const dd t = two_prod(sh, sh);
+  const dd s3 = two_prod(sh, t.hi);
+  const dd p3 = two_prod(C3H, s3.hi);
+  const double p3l = p3.lo + (C3H * (s3.lo + sh * t.lo) + C3L * s3.hi);
+  const double p5 = (s3.hi * t.hi) *
+                    (0x1.999999999999ap-2 + t.hi * (0x1.2492492492492p-2 + t.hi * 0x1.c71c71c71c71cp-3));
+  const dd lc = tab ? tab[k + 38] : log_table(k);
   const double ed = (double)e;
   const dd u = two_sum(ed * LN2H, lc.hi);
   const dd v = two_sum(u.hi, 2.0 * sh);
-  const double lo = v.lo + u.lo + (ed * LN2L + lc.lo + 2.0 * sl + poly);
-  const dd l = fast_two_sum(v.hi, lo);
+  const dd w = two_sum(v.hi, p3.hi);
+  const double lo =
+      u.lo + v.lo + w.lo + (ed * LN2L + lc.lo + (2.0 * sl + (p3l + (2.0 * t.hi * sl + p5))));
+  const dd l = fast_two_sum(w.hi, lo);
   dd z = two_prod(y, l.hi);
   z = fast_two_sum(z.hi, z.lo + y * l.lo);
-  if (!(fabs(z.hi) < 700.0)) return false;
-  const double nd = rint(z.hi * 0x1.71547652b82fep+7);  // z 128 / ln2
+  if (!(fabs(z.hi) < 708.0)) return false;
+  const double nd = rint(z.hi * 0x1.71547652b82fep+7);  // z 128 / ln2, |nd| < 2**17
   const int n = (int)nd;
   const int j = n & 127;
-  const int e2 = (n - j) / 128;
-  // r = z - n ln2/128 as the pair r1 + r2 (|r2| <= ulp(r1)/2)
-  const dd r = two_sum(z.hi - nd * LN2_128H, z.lo - nd * LN2_128L);  // first term exact
-  const double r1 = r.hi;
-  // exp(r1 + r2) - 1 - r1 = r2 + r1**2/2 + ... + r1**6/720 (r1 r2 < 2**-106)
-  const double rest =
-      r.lo + (r1 * r1) * (0.5 + r1 * (0x1.5555555555555p-3 + r1 * (0x1.5555555555555p-5 +
-                                      r1 * (0x1.1111111111111p-7 + r1 * 0x1.6c16c16c16c17p-10))));
-  const dd T = exp2_table(j);
+  e2 = (n - j) / 128;
+  // r = z - nd ln2/128: the first difference is exact (Sterbenz), the
+  // product with LN2_128L error-free
+  const double a = z.hi - nd * LN2_128H;
+  const dd b = two_prod(-nd, LN2_128L);
+  const dd r0 = two_sum(a, b.hi);
+  const dd r = two_sum(r0.hi, r0.lo + (z.lo + (b.lo - nd * LN2_128LL)));
+  const double r1 = r.hi, r2 = r.lo;
+  const dd q2 = two_prod(r1, r1);
+  const double q2h = 0.5 * q2.hi, q2l = 0.5 * q2.lo;
+  const dd c3 = two_prod(r1, q2.hi);
+  const double c3l = c3.lo + r1 * q2.lo;
+  const double P = 0x1.5555555555555p-5 +
+                   r1 * (0x1.1111111111111p-7 +
+                         r1 * (0x1.6c16c16c16c17p-10 + r1 * (0x1.a01a01a01a01ap-13 + r1 * 0x1.a01a01a01a01ap-16)));
+  const dd p = two_prod(c3.hi, C6H);
+  const double pl = p.lo + (c3.hi * C6L + c3l * C6H + c3.hi * (r1 * P));
+  const double small = q2l + pl + r2 * (1.0 + r1 + q2h);
+  const dd T = tab ? tab[92 + j] : exp2_table(j);
   const dd mp = two_prod(T.hi, r1);
-  const dd sm = two_sum(T.hi, mp.hi);
-  const double corr = sm.lo + mp.lo + T.lo + T.lo * r1 + T.hi * rest;
-  const dd R = fast_two_sum(sm.hi, corr);
-  const double eps = (fabs(z.hi) + fabs(y) * 0x1.0p-7 + 2.0) * 0x1.0p-67;
+  const dd mq = two_prod(T.hi, q2h);
+  const dd mpp = two_prod(T.hi, p.hi);
+  const dd a1 = two_sum(T.hi, mp.hi);
+  const dd a2 = two_sum(mq.hi, mpp.hi);
+  const dd a3 = two_sum(a1.hi, a2.hi);
+  const double rl = a1.lo + a3.lo + a2.lo +
+                    (mp.lo + mq.lo + mpp.lo + (T.hi * small + (T.lo + T.lo * (r1 + (q2h + p.hi)))));
+  R = fast_two_sum(a3.hi, rl);
+  eps = fabs(z.hi) * 0x1.0p-84 + 0x1.0p-87;
+  return true;
+}
+
+// x**y when R rounds the same way at both ends of its error interval;
+// otherwise false (the caller then takes pow_slow)
+__host__ __device__ __forceinline__ bool pow_fast(double x, double y, double& out,
+                                                  const dd* tab = nullptr) {
+  dd R;
+  double eps;
+  int e2;
+  if (!pow_fast_pair(x, y, R, eps, e2, tab)) return false;
   int ex = 0;
   const double fr = frexp(R.hi, &ex);  // R.hi = fr 2**ex, |fr| in [0.5, 1)
   double half_ulp = ldexp(1.0, ex - 54);
   if (fabs(fr) == 0.5) half_ulp *= 0.5;  // a power of two: the gap below is halved
-  if (!(fabs(R.lo) + eps * fabs(R.hi) < 0.9999 * half_ulp)) return false;
+  // the factor covers the rounding of the left-hand side
+  if (!(fabs(R.lo) + eps * fabs(R.hi) < (1.0 - 0x1.0p-50) * half_ulp)) return false;
   if (ex + e2 < -1020 || ex + e2 > 1023) return false;  // leave the range ends to pow_slow
   out = ldexp(R.hi, e2);
   return true;
@@ -480,14 +550,23 @@ __host__ __device__ __forceinline__ bool pow_fast(double x, double y, double& ou
 
 }  // namespace ddm
 
-// x**y correctly rounded; the special cases follow C's pow
-__host__ __device__ __forceinline__ double pow_cr(double x, double y) {
+// x**y correctly rounded; the special cases follow C's pow.  tab: the tables
+// in LDS (pow_tables_to_lds) or null for the ones in global memory.
+__host__ __device__ __forceinline__ double pow_cr(double x, double y,
+                                                  const ddm::dd* tab = nullptr) {
   if (y == 0.0 || x == 1.0) return 1.0;
   if (x == 0.0) return (y > 0.0) ? 0.0 : HUGE_VAL;
   if (!(x > 0.0) || !(x < HUGE_VAL) || !(fabs(y) < HUGE_VAL)) return pow(x, y);
   double r;
-  if (ddm::pow_fast(x, y, r)) return r;
+  if (ddm::pow_fast(x, y, r, tab)) return r;
   return ddm::pow_slow(x, y);
+}
+
+// the block's copy of the x**y tables in LDS; every thread of the block
+// calls this (it ends with a barrier)
+__device__ __forceinline__ void pow_tables_to_lds(ddm::dd* t) {
+  for (int i = (int)threadIdx.x; i < ddm::POW_TAB_N; i += (int)blockDim.x) t[i] = ddm::pow_tab_entry(i);
+  __syncthreads();
 }
 
 }  // namespace imex

@@ -765,7 +765,7 @@ __device__ inline bool implicit_r(const Params& P, const double* q, double& nh1,
 
 // settling_velocity (Schiller-Naumann): src/constitutive_2d.f90:5326-5404
 __device__ inline double settling_velocity(const Params& P, double diam, double rhos,
-                                           double rhoc, double ikv) {
+                                           double rhoc, double ikv, const ddm::dd* tab) {
   double inv_sqrt_C_D = 1.0;
   if (rhos <= rhoc) return 0.0;
   double const_part = sqrt(1.3333333333333333 * (rhos / rhoc - 1.0) * diam * P.grav);
@@ -775,7 +775,7 @@ __device__ inline double settling_velocity(const Params& P, double diam, double 
 #pragma unroll 1
     for (int i = 0; i < 20; i++) {
       double old = sv;
-      inv_sqrt_C_D = sqrt(Rey / (24.0 * (1.0 + 0.15 * pow_cr(Rey, 0.687))));
+      inv_sqrt_C_D = sqrt(Rey / (24.0 * (1.0 + 0.15 * pow_cr(Rey, 0.687, tab))));
       sv = const_part * inv_sqrt_C_D;
       if (fabs(old - sv) / old < 1.0e-6) return sv;
       Rey = diam * sv * ikv;
@@ -795,7 +795,7 @@ __device__ inline double settling_velocity(const Params& P, double diam, double 
 template <int NS, int LANES = 1>
 __device__ inline bool mass_exchange(const Params& P, const double* qp, const double* erod,
                                      double dt, double* dep, double* eq, double* ers,
-                                     double& topo, int lane = 0) {
+                                     double& topo, const ddm::dd* tab, int lane = 0) {
   constexpr int NV = 4 + NS;
   topo = 0.0;
 #pragma unroll
@@ -848,7 +848,7 @@ __device__ inline bool mass_exchange(const Params& P, const double* qp, const do
 #pragma unroll
     for (int i = 0; i < NS; i++)
       if ((alphas[i] > 0.0) && P.settling)
-        sv[i] = settling_velocity(P, P.diam_s[i], P.rho_s[i], m.rho_c, ikv);
+        sv[i] = settling_velocity(P, P.diam_s[i], P.rho_s[i], m.rho_c, ikv, tab);
   } else {
     // classes i0 and i0 + 1 go through ONE call on the pair (each lane with
     // its own class's constants), so the two fixed-point iterations run side
@@ -861,13 +861,13 @@ __device__ inline bool mass_exchange(const Params& P, const double* qp, const do
       const double d = (lane == 0 || !two) ? P.diam_s[i0] : P.diam_s[two ? i0 + 1 : i0];
       const double r = (lane == 0 || !two) ? P.rho_s[i0] : P.rho_s[two ? i0 + 1 : i0];
       double s = 0.0;
-      if (mine && (a > 0.0) && P.settling) s = settling_velocity(P, d, r, m.rho_c, ikv);
+      if (mine && (a > 0.0) && P.settling) s = settling_velocity(P, d, r, m.rho_c, ikv, tab);
       const double o = __shfl_xor(s, 1);
       sv[i0] = (lane == 0) ? s : o;
       if (two) sv[i0 + 1] = (lane == 0) ? o : s;
     }
   }
-  const double hindered = any ? pow_cr(1.0 - fmin_f(1.0, at / 0.6), 4.65) : 0.0;
+  const double hindered = any ? pow_cr(1.0 - fmin_f(1.0, at / 0.6), 4.65, tab) : 0.0;
 #pragma unroll
   for (int i = 0; i < NS; i++) {
     if ((alphas[i] > 0.0) && P.settling) {
@@ -890,7 +890,7 @@ __device__ inline bool mass_exchange(const Params& P, const double* qp, const do
   const double T_er = P.erodible_deposit ? T : P.T_erodible;
   double air = 0.0;
   if (P.entrainment && (h > 0.0) && (m.Ri > 0.0)) {
-    double ec = 0.075 / sqrt(1.0 + 718.0 * pow_cr(m.Ri, 2.4));
+    double ec = 0.075 / sqrt(1.0 + 718.0 * pow_cr(m.Ri, 2.4, tab));
     air = ec * sqrt(mod_vel2);
   }
   double rdt = 0.0, rert = 0.0, A = 0.0, D = 0.0;
