@@ -25,21 +25,22 @@ using namespace imex;
 // ===========================================================================
 // One 32x8 tile: seeds (frame, wet, source ring) of the tile and its two-cell
 // halo in LDS, the diamond dilation of radius 2, the mask bytes, and the
-// lists: the tile goes to k_flux's list when it is active and "hot" (a wet
-// or ring cell within two cells), and a hot tile makes itself and its eight
+// lists: each half of the tile (16 x 8 cells, entry 2 tile + half) goes to
+// k_flux's list when it has active cells and is "hot" (a wet or ring cell
+// within two cells of it), and a hot tile makes itself and its eight
 // neighbours candidates of the next step's k_mask.
 template <typename S>
 __device__ __forceinline__ void mask_tile(const Params& P, const Grid& G, int tile, int solve_all,
                                           int par, unsigned int seq) {
   __shared__ unsigned char seed[RY][RX];
-  __shared__ int s_any, s_cnt, s_hot;
+  __shared__ int s_any[2], s_cnt, s_hot[2];
   const int j0 = (tile % G.ntx) * TX, k0 = (tile / G.ntx) * TY;
   const int tid = threadIdx.x;
   __syncthreads();  // the previous tile's use of the shared flags is over
   if (tid == 0) {
-    s_any = 0;
+    s_any[0] = s_any[1] = 0;
     s_cnt = 0;
-    s_hot = P.all_tiles | (int)G.st->hot;
+    s_hot[0] = s_hot[1] = P.all_tiles | (int)G.st->hot;
   }
   __syncthreads();
   for (int e = tid; e < RX * RY; e += NTHR) {
@@ -51,7 +52,9 @@ __device__ __forceinline__ void mask_tile(const Params& P, const Grid& G, int ti
       const bool ring = (j == 0 || j == G.nx - 1 || k == 0 || k == G.ny - 1);
       const bool wet = state_q<S>(G)[c] > S(0.0), src = P.radial && G.scell[c] == 2;
       if (ring || solve_all || wet || src) s = 1;
-      if (wet || src) s_hot = 1;
+      // halo columns lj of half 0 (cells 0..FX-1): 0..FX+3; of half 1: FX..TX+3
+      if ((wet || src) && lj < FX + 2 * HALO) s_hot[0] = 1;
+      if ((wet || src) && lj >= FX) s_hot[1] = 1;
     }
     seed[lk][lj] = s;
   }
@@ -76,17 +79,17 @@ __device__ __forceinline__ void mask_tile(const Params& P, const Grid& G, int ti
     G.mask[c] = m;
     act = m;
   }
-  // active cells of the tile: one shared-memory atomic per wavefront
+  // active cells of the tile (one shared-memory atomic per wavefront) and of
+  // each half
   const unsigned long long bal = __ballot(act != 0);
-  if ((tid & 63) == 0 && bal) {
-    s_any = 1;
-    atomicAdd(&s_cnt, (int)__popcll(bal));
-  }
+  if ((tid & 63) == 0 && bal) atomicAdd(&s_cnt, (int)__popcll(bal));
+  if (act) s_any[lj >= FX ? 1 : 0] = 1;
   __syncthreads();
+  const bool hot = s_hot[0] || s_hot[1];
   // the tile's global atomics, each on a thread of its own so that their
   // round trips overlap
   DevState* st = G.st;
-  if (tid < 9 && s_hot) {
+  if (tid < 9 && hot) {
     // a hot tile makes itself and its neighbours candidates of the next pass;
     // the stamp lists each candidate once
     const int ux = tile % G.ntx + tid % 3 - 1, uy = tile / G.ntx + tid / 3 - 1;
@@ -98,13 +101,13 @@ __device__ __forceinline__ void mask_tile(const Params& P, const Grid& G, int ti
         G.cand[(size_t)nslot * G.ntx * G.nty + slot] = u;
       }
     }
-  } else if (tid == 9 && s_any && s_hot) {
-    // k_flux visits the tiles with active cells that have a wet or
+  } else if ((tid == 9 || tid == 11) && s_any[tid == 11] && s_hot[tid == 11]) {
+    // k_flux visits the half tiles with active cells that have a wet or
     // source-ring cell within two cells: elsewhere every face stencil is dry,
     // so every flux and wave speed is zero, and the (dry, all-zero) cells get
     // a zero right-hand side that k_update does not need
     const unsigned int slot = atomicAdd(&st->n_tiles[par], 1u);
-    G.tile_list[(size_t)par * G.ntx * G.nty + slot] = tile;
+    G.tile_list[(size_t)par * 2 * G.ntx * G.nty + slot] = 2 * tile + (tid == 11 ? 1 : 0);
   } else if (tid == 10) {
     // active cells of the tile beyond its frame band (counted by block 0)
     const long long extra = (long long)s_cnt - (long long)G.tile_static[tile];
@@ -628,24 +631,46 @@ __device__ __forceinline__ void kt_flux(const Params& P, int dir, const F* qpA, 
 constexpr int NRX = (FX + 2) * TY;
 constexpr int NRY = FX * (TY + 2);
 
-template <int NS, typename F = double>
+// faces of the half tile: x faces (FX+1) x TY, then y faces FX x (TY+1)
+constexpr int NFX = (FX + 1) * TY;
+constexpr int NFY = FX * (TY + 1);
+
+// Blocks of FLUX_THR threads (single runs) take each phase of a half tile in
+// one pass: the halo states with the active bytes, the side states, the faces
+// with the cells' explicit terms, then the cells.  Blocks of NTHR threads
+// (batch members, where many members keep the CUs busy) take two passes for
+// the side states and the faces.
+constexpr int FLUX_THR = 512;
+template <int THR>
+constexpr bool flux_one_pass() {
+  return THR >= NFX + NFY + FX * TY;
+}
+
+template <int NS, typename F = double, int THR = NTHR>
 constexpr size_t flux_lds_bytes() {
   constexpr int NV = 4 + NS, NP = NV + 2;
   return sizeof(F) * ((size_t)NP * RY * FRX + (size_t)NV * (FX + 1) * TY +
                            (size_t)NV * FX * (TY + 1) + (size_t)(FX + 1) * TY +
-                           (size_t)FX * (TY + 1) + 2 * (size_t)NP * (NRX + NRY)) +
+                           (size_t)FX * (TY + 1) + 2 * (size_t)NP * (NRX + NRY) +
+                           (flux_one_pass<THR>() ? (size_t)NV * FX * TY : 0)) +
          2 * RY * FRX + FX * TY;
 }
 
 // F: the arithmetic and storage type of the state, the side states and the
 // fluxes (double on the FP64 path; float in the FP32 form, where thin
 // layers, h < 1e-3 m, are converted to primitive variables in double)
-template <int NS, typename F = double>
-__device__ __forceinline__ void flux_body(const Params& P, const Grid& G, int mode, double t_arg,
-                                          int par) {
+// One half tile (list entry ht: tile ht / 2, columns FX (ht % 2) onward) with
+// SUB threads starting at thread tid0 of the block and the LDS set at base:
+// the primitive states of its cells and two-cell halo, the side states, the
+// faces, then the cells (divergence minus the explicit terms into R, the CFL
+// candidate, the work list).  Every thread of the block calls this the same
+// number of times, valid or not (the barriers are the block's); an invalid
+// call does no work.
+template <int NS, typename F, int SUB>
+__device__ __forceinline__ void flux_half(const Params& P, const Grid& G, F* base, int tid, int ht,
+                                          bool valid, double t, double& my_dt) {
   constexpr int NV = 4 + NS, NP = NV + 2;
-  extern __shared__ double smem[];
-  F* sq = reinterpret_cast<F*>(smem);
+  F* sq = base;
   F* sHx = sq + NP * RY * FRX;
   F* sHy = sHx + NV * (FX + 1) * TY;
   F* sAx = sHy + NV * FX * (TY + 1);
@@ -656,24 +681,20 @@ __device__ __forceinline__ void flux_body(const Params& P, const Grid& G, int mo
   F* sRN = sRS + NP * NRY;
   const F* gq = state_q<F>(G);
   F* gR = state_R<F>(G);
-  signed char* ssc = (signed char*)(sRN + NP * NRY);
+  F* sEx = sRN + NP * NRY;  // [NV][FX*TY] explicit terms (one-pass blocks)
+  constexpr bool ONE = flux_one_pass<SUB>();
+  signed char* ssc = (signed char*)(sEx + (ONE ? NV * FX * TY : 0));
   unsigned char* ssd = (unsigned char*)(ssc + RY * FRX);
   unsigned char* sact = ssd + RY * FRX;
-  __shared__ double red[NTHR / 64];
-
-  const int tid = threadIdx.x;
   const int nx = G.nx, ny = G.ny, N = G.N;
-  double my_dt = DINF;
-  const int n_list = (int)G.st->n_tiles[par];
-  const int* list = G.tile_list + (size_t)par * G.ntx * G.nty;
-  const double t = (mode == 1) ? G.st->t : t_arg;
-
-  for (int it = blockIdx.x; it < 2 * n_list; it += gridDim.x) {
-    const int tile = list[it >> 1];
-    const int j0 = (tile % G.ntx) * TX + (it & 1) * FX, k0 = (tile / G.ntx) * TY;
-    __syncthreads();  // the previous tile's LDS reads are done
+  const int tile = valid ? ht >> 1 : 0;
+  const int j0 = (tile % G.ntx) * TX + (ht & 1) * FX, k0 = (tile / G.ntx) * TY;
+  TileView<F> T1{sq, ssc, ssd, false, F(P.T_amb)};
+  TileView<F> T0{sq, ssc, ssd, true, F(P.T_amb)};
+  __syncthreads();  // the previous half tile's LDS reads are done
+  if (valid) {
     // ---- primitive states of the tile and its two-cell halo ----------------
-    for (int e = tid; e < FRX * RY; e += NTHR) {
+    for (int e = tid; e < FRX * RY; e += SUB) {
       const int lj = e % FRX, lk = e / FRX;
       const int j = j0 + lj - HALO, k = k0 + lk - HALO;
       F qp[NP];
@@ -712,15 +733,16 @@ __device__ __forceinline__ void flux_body(const Params& P, const Grid& G, int mo
       ssc[e] = sc;
       ssd[e] = sd;
     }
-    for (int e = tid; e < FX * TY; e += NTHR) {
+    // (one-pass blocks: the threads past the halo cells)
+    for (int e = ONE ? tid - FRX * RY : tid; e >= 0 && e < FX * TY; e += SUB) {
       const int j = j0 + e % FX, k = k0 + e / FX;
       sact[e] = (j < nx && k < ny) ? G.mask[j + nx * k] : 0;
     }
-    __syncthreads();
-    TileView<F> T1{sq, ssc, ssd, false, F(P.T_amb)};
-    TileView<F> T0{sq, ssc, ssd, true, F(P.T_amb)};
+  }
+  __syncthreads();
+  if (valid) {
     // ---- side states of the cells (cells of the source disc have none) -----
-    for (int e = tid; e < NRX + NRY; e += NTHR) {
+    for (int e = tid; e < NRX + NRY; e += SUB) {
       if (e < NRX) {
         const int lj = e % (FX + 2) - 1, lk = e / (FX + 2);
         const int j = j0 + lj, k = k0 + lk;
@@ -750,9 +772,35 @@ __device__ __forceinline__ void flux_body(const Params& P, const Grid& G, int mo
         }
       }
     }
-    __syncthreads();
-    // ---- x faces: (FX+1) x TY ---------------------------------------------
-    for (int e = tid; e < (FX + 1) * TY; e += NTHR) {
+  }
+  __syncthreads();
+  if (valid) {
+    // ---- faces: x faces (FX+1) x TY, then y faces FX x (TY+1) -------------
+    // (one-pass blocks: the threads past the faces evaluate the explicit
+    // terms of the cells, which depend on the cell's own state only)
+    for (int e0 = tid; e0 < NFX + NFY + (ONE ? FX * TY : 0); e0 += SUB) {
+     if (ONE && e0 >= NFX + NFY) {
+      const int ce = e0 - NFX - NFY;
+      const int lj = ce % FX, lk = ce / FX;
+      const int j = j0 + lj, k = k0 + lk;
+      if (j < nx && k < ny && sact[ce]) {
+        const int c = j + nx * k;
+        const int idx = (lk + HALO) * FRX + lj + HALO;
+        F qp[NP], ex[NV];
+#pragma unroll
+        for (int i = 0; i < NP; i++) qp[i] = sq[i * FRX * RY + idx];
+        F ap = F(0.0), ax = F(0.0), ay = F(0.0);
+        if (ssc[idx] == 2) {
+          ap = F(G.arcp[c]);
+          ax = F(G.arcx[c]);
+          ay = F(G.arcy[c]);
+        }
+        expl_terms<NS, F>(P, F(G.bx[c]), F(G.by[c]), qp, ex, t, ap, ax, ay);
+#pragma unroll
+        for (int i = 0; i < NV; i++) sEx[i * FX * TY + ce] = ex[i];
+      }
+     } else if (e0 < NFX) {
+      const int e = e0;
       const int fl = e % (FX + 1), fk = e / (FX + 1);
       const int jf = j0 + fl, k = k0 + fk;
       bool need = (k < ny) && (jf <= nx) &&
@@ -801,9 +849,8 @@ __device__ __forceinline__ void flux_body(const Params& P, const Grid& G, int mo
 #pragma unroll
       for (int i = 0; i < NV; i++) sHx[i * (FX + 1) * TY + e] = H[i];
       sAx[e] = amax;
-    }
-    // ---- y faces: FX x (TY+1) ---------------------------------------------
-    for (int e = tid; e < FX * (TY + 1); e += NTHR) {
+     } else {
+      const int e = e0 - NFX;
       const int fj = e % FX, fl = e / FX;
       const int j = j0 + fj, kf = k0 + fl;
       bool need = (j < nx) && (kf <= ny) &&
@@ -850,8 +897,11 @@ __device__ __forceinline__ void flux_body(const Params& P, const Grid& G, int mo
 #pragma unroll
       for (int i = 0; i < NV; i++) sHy[i * FX * (TY + 1) + e] = H[i];
       sAy[e] = amax;
+     }
     }
-    __syncthreads();
+  }
+  __syncthreads();
+  if (valid) {
     // ---- cells: divergence, explicit terms, CFL candidate -----------------
     {
       const int lj = tid % FX, lk = tid / FX;
@@ -861,18 +911,24 @@ __device__ __forceinline__ void flux_body(const Params& P, const Grid& G, int mo
       if (tid < FX * TY && j < nx && k < ny && sact[tid]) {
         const int ew = lk * (FX + 1) + lj;
         const int es = lk * FX + lj;
-        F qp[NP], ex[NV];
-        const int idx = (lk + HALO) * FRX + lj + HALO;
+        F ex[NV];
+        if constexpr (ONE) {
 #pragma unroll
-        for (int i = 0; i < NP; i++) qp[i] = sq[i * FRX * RY + idx];
-        const signed char sc = ssc[idx];
-        F ap = F(0.0), ax = F(0.0), ay = F(0.0);
-        if (sc == 2) {
-          ap = F(G.arcp[c]);
-          ax = F(G.arcx[c]);
-          ay = F(G.arcy[c]);
+          for (int i = 0; i < NV; i++) ex[i] = sEx[i * FX * TY + tid];
+        } else {
+          F qp[NP];
+          const int idx = (lk + HALO) * FRX + lj + HALO;
+#pragma unroll
+          for (int i = 0; i < NP; i++) qp[i] = sq[i * FRX * RY + idx];
+          const signed char sc = ssc[idx];
+          F ap = F(0.0), ax = F(0.0), ay = F(0.0);
+          if (sc == 2) {
+            ap = F(G.arcp[c]);
+            ax = F(G.arcx[c]);
+            ay = F(G.arcy[c]);
+          }
+          expl_terms<NS, F>(P, F(G.bx[c]), F(G.by[c]), qp, ex, t, ap, ax, ay);
         }
-        expl_terms<NS, F>(P, F(G.bx[c]), F(G.by[c]), qp, ex, t, ap, ax, ay);
 #pragma unroll
         for (int i = 0; i < NV; i++) {
           F d = F(0.0);
@@ -904,6 +960,44 @@ __device__ __forceinline__ void flux_body(const Params& P, const Grid& G, int mo
       base = __shfl(base, 0, 64);
       if (work) G.work[base + __popcll(bal & ((1ULL << lane) - 1ULL))] = c;
     }
+  
+  }
+}
+
+// k_flux's work: the hot half tiles of the latest active-set pass.  THR =
+// NTHR: each block takes half tiles one after the other.  THR = 2 NTHR (one
+// block per CU, the grid no larger than the CUs): with at most one half tile
+// per block, all THR threads take it and each phase is one pass; with more
+// half tiles, the two NTHR-thread halves of a block each take one, in their
+// own LDS set.  The results do not depend on the split.
+template <int NS, typename F = double, int THR = NTHR>
+__device__ __forceinline__ void flux_body(const Params& P, const Grid& G, int mode, double t_arg,
+                                          int par) {
+  extern __shared__ double smem[];
+  __shared__ double red[THR / 64];
+  const int tid = threadIdx.x;
+  double my_dt = DINF;
+  const int n_list = (int)G.st->n_tiles[par];
+  const int* list = G.tile_list + (size_t)par * 2 * G.ntx * G.nty;
+  const double t = (mode == 1) ? G.st->t : t_arg;
+  F* base = reinterpret_cast<F*>(smem);
+  if constexpr (THR == NTHR) {
+    for (int it = blockIdx.x; it < n_list; it += gridDim.x)
+      flux_half<NS, F, NTHR>(P, G, base, tid, list[it], true, t, my_dt);
+  } else {
+    static_assert(THR == 2 * NTHR, "k_flux blocks are NTHR or 2 NTHR threads");
+    if (n_list <= (int)gridDim.x) {
+      for (int it = blockIdx.x; it < n_list; it += gridDim.x)
+        flux_half<NS, F, THR>(P, G, base, tid, list[it], true, t, my_dt);
+    } else {
+      const int sub = tid / NTHR;
+      F* mine = base + sub * (flux_lds_bytes<NS, F, NTHR>() / sizeof(F));
+      for (int pp = blockIdx.x; 2 * pp < n_list; pp += gridDim.x) {
+        const int it = 2 * pp + sub;
+        flux_half<NS, F, NTHR>(P, G, mine, tid - sub * NTHR, (it < n_list) ? list[it] : 0,
+                               it < n_list, t, my_dt);
+      }
+    }
   }
   // ---- CFL min: wavefront shuffle, LDS, one atomic per block ---------------
   // (every candidate is positive, so the IEEE bit patterns order as the values)
@@ -917,15 +1011,15 @@ __device__ __forceinline__ void flux_body(const Params& P, const Grid& G, int mo
   __syncthreads();
   if (tid == 0) {
     double b = red[0];
-    for (int w = 1; w < NTHR / 64; w++) b = (red[w] < b) ? red[w] : b;
+    for (int w = 1; w < THR / 64; w++) b = (red[w] < b) ? red[w] : b;
     if (b < DINF) atomicMin(&G.st->dtmin_bits, (unsigned long long)__double_as_longlong(b));
   }
 }
 
 template <int NS, typename F>
-__global__ __launch_bounds__(NTHR) void k_flux(Params P, Grid G, int mode, double t_arg,
-                                               int par) {
-  flux_body<NS, F>(P, G, mode, t_arg, par);
+__global__ __launch_bounds__(FLUX_THR) void k_flux(Params P, Grid G, int mode, double t_arg,
+                                                   int par) {
+  flux_body<NS, F, FLUX_THR>(P, G, mode, t_arg, par);
 }
 
 // The CFL time step from the blocks' minimum (timestep's result,
@@ -999,23 +1093,25 @@ __device__ void eval_jacobian(const Params& P, double xi, double dt, const doubl
     implicit_c<NS>(P, qc, n1, n2, xi);
     const double J1 = 1.0 * n1.im * P.one_by_hcs;
     const double J2 = 1.0 * n2.im * P.one_by_hcs;
-    if (col == 1) {
-      L11 = (1.0 * qorg[1]) - dt * 1.0 * J1;
-      L21 = 0.0 - dt * 1.0 * J2;
+    // column col: (L1col, L2col) = (diag_1 - dt J1, diag_2 - dt J2) with the
+    // diagonal qorg entry in row col and 0 in the other row
+    const double a = ((col == 1) ? (1.0 * qorg[1]) : 0.0) - dt * 1.0 * J1;
+    const double b = ((col == 1) ? 0.0 : (1.0 * qorg[2])) - dt * 1.0 * J2;
+    if constexpr (LANES == 2) {
+      // selects rather than stores into a lane-indexed column: the latter
+      // puts the four entries in scratch memory
+      const double oa = __shfl_xor(a, 1), ob = __shfl_xor(b, 1);
+      const bool first = lane == 0;
+      L11 = first ? a : oa;
+      L21 = first ? b : ob;
+      L12 = first ? oa : a;
+      L22 = first ? ob : b;
+    } else if (col == 1) {
+      L11 = a;
+      L21 = b;
     } else {
-      L12 = 0.0 - dt * 1.0 * J1;
-      L22 = (1.0 * qorg[2]) - dt * 1.0 * J2;
-    }
-  }
-  if constexpr (LANES == 2) {
-    const double a = (lane == 0) ? L11 : L12, b = (lane == 0) ? L21 : L22;
-    const double oa = __shfl_xor(a, 1), ob = __shfl_xor(b, 1);
-    if (lane == 0) {
-      L12 = oa;
-      L22 = ob;
-    } else {
-      L11 = oa;
-      L21 = ob;
+      L12 = a;
+      L22 = b;
     }
   }
 }
@@ -1856,16 +1952,30 @@ static int bind_dts(imex_handle* h, const double* dts, int nsteps) {
   return 0;
 }
 
-template <int NS>
+template <int NS, typename F, int THR>
+static size_t flux_lds_t() {
+  if (THR == NTHR) return flux_lds_bytes<NS, F, NTHR>();
+  // one one-pass set, or two NTHR-thread sets side by side
+  return std::max(flux_lds_bytes<NS, F, 2 * NTHR>(), 2 * flux_lds_bytes<NS, F, NTHR>());
+}
+template <int NS, int THR>
 static size_t flux_lds(bool fp32) {
-  return fp32 ? flux_lds_bytes<NS, float>() : flux_lds_bytes<NS, double>();
+  return fp32 ? flux_lds_t<NS, float, THR>() : flux_lds_t<NS, double, THR>();
 }
 
-static size_t lds_for(int ns, bool fp32 = false) {
+// dynamic LDS of k_flux (FLUX_THR threads) or, with batch set, of kb_flux
+static size_t lds_for(int ns, bool fp32 = false, bool batch = false) {
+  if (batch) {
+    switch (ns) {
+      case 1: return flux_lds<1, NTHR>(fp32);
+      case 2: return flux_lds<2, NTHR>(fp32);
+      default: return flux_lds<3, NTHR>(fp32);
+    }
+  }
   switch (ns) {
-    case 1: return flux_lds<1>(fp32);
-    case 2: return flux_lds<2>(fp32);
-    default: return flux_lds<3>(fp32);
+    case 1: return flux_lds<1, FLUX_THR>(fp32);
+    case 2: return flux_lds<2, FLUX_THR>(fp32);
+    default: return flux_lds<3, FLUX_THR>(fp32);
   }
 }
 
@@ -2075,7 +2185,7 @@ extern "C" int imex_create(const imex_config* cfg, imex_handle** out) {
   if (P.n_thk * P.n_pdyn > 0)
     if ((rc = dalloc(h, &G.vuln, N))) return rc;
   if ((rc = dalloc(h, &G.mask, N))) return rc;
-  if ((rc = dalloc(h, &G.tile_list, 2 * (size_t)h->ntiles))) return rc;
+  if ((rc = dalloc(h, &G.tile_list, 4 * (size_t)h->ntiles))) return rc;  // [2][2 ntiles] half tiles
   if ((rc = dalloc(h, &G.work, N))) return rc;
   if ((rc = dalloc(h, &G.cand, 3 * (size_t)h->ntiles))) return rc;
   if ((rc = dalloc(h, &G.cand_stamp, (size_t)h->ntiles))) return rc;
@@ -2261,11 +2371,13 @@ static int mask(imex_handle* h, int solve_all, int driven = 0) {
   return 0;
 }
 static int flux(imex_handle* h, int mode, double t, int step_idx) {
-  const int nb = grid_for(h, 4, 2 * h->ntiles);  // work items are half tiles
+  // one block of FLUX_THR threads per CU (its registers and LDS set fill the
+  // CU): the blocks are all resident, so a half tile per block is one round
+  const int nb = grid_for(h, 1, 2 * h->ntiles);  // work items are half tiles
   switch (h->P.ns) {
-    case 1: hipLaunchKernelGGL((h->fp32 ? k_flux<1, float> : k_flux<1, double>), dim3(nb), dim3(NTHR), h->lds, h->stream, h->P, h->G, mode, t, h->par); break;
-    case 2: hipLaunchKernelGGL((h->fp32 ? k_flux<2, float> : k_flux<2, double>), dim3(nb), dim3(NTHR), h->lds, h->stream, h->P, h->G, mode, t, h->par); break;
-    default: hipLaunchKernelGGL((h->fp32 ? k_flux<3, float> : k_flux<3, double>), dim3(nb), dim3(NTHR), h->lds, h->stream, h->P, h->G, mode, t, h->par); break;
+    case 1: hipLaunchKernelGGL((h->fp32 ? k_flux<1, float> : k_flux<1, double>), dim3(nb), dim3(FLUX_THR), h->lds, h->stream, h->P, h->G, mode, t, h->par); break;
+    case 2: hipLaunchKernelGGL((h->fp32 ? k_flux<2, float> : k_flux<2, double>), dim3(nb), dim3(FLUX_THR), h->lds, h->stream, h->P, h->G, mode, t, h->par); break;
+    default: hipLaunchKernelGGL((h->fp32 ? k_flux<3, float> : k_flux<3, double>), dim3(nb), dim3(FLUX_THR), h->lds, h->stream, h->P, h->G, mode, t, h->par); break;
   }
   HCHK(h, hipGetLastError());
   (void)step_idx;
@@ -2680,7 +2792,7 @@ extern "C" int imex_batch_create(int n_solid, imex_batch** out) {
   int ncu = 0;
   if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, b->device) == hipSuccess && ncu > 0)
     b->n_cu = ncu;
-  b->lds = lds_for(n_solid);
+  b->lds = lds_for(n_solid, false, true);
   switch (n_solid) {
     case 1: BCHK(b, hipFuncSetAttribute((const void*)kb_flux<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b->lds)); break;
     case 2: BCHK(b, hipFuncSetAttribute((const void*)kb_flux<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b->lds)); break;

@@ -6,9 +6,12 @@
 //             boundary frame, dilated by the diamond of radius n_RK = 2; one
 //             byte per cell.  Visits the candidate tiles the previous pass
 //             listed (hot tiles and their neighbours) -- every other tile
-//             holds only its frame band -- and lists the "hot" active tiles
-//             (a wet or source-ring cell within two cells) for k_flux.
-//   k_flux    persistent blocks over the halves of the hot tiles; per
+//             holds only its frame band -- and lists the "hot" active half
+//             tiles (a wet or source-ring cell within two cells) for k_flux.
+//   k_flux    one 512-thread block per CU over the hot half tiles (one half
+//             tile per block in one pass per phase, or two per block on its
+//             two 256-thread halves when there are more half tiles than
+//             blocks; batch members: 256-thread blocks); per
 //             FX x TY half tile with a two-cell halo in LDS: primitive states -> MUSCL/minmod face
 //             states -> local speeds -> Kurganov-Tadmor face fluxes -> flux
 //             divergence minus explicit terms (the stage-1 right-hand side),
@@ -52,7 +55,7 @@ constexpr int FRX = FX + 2 * HALO;
 struct DevState {
   double t, dt, dt_old, dt_old_old, t_end, dt_cfl;
   unsigned long long dtmin_bits;
-  unsigned int n_tiles[2];  // active tiles listed by k_mask, per list slot
+  unsigned int n_tiles[2];  // hot half tiles listed by k_mask, per list slot
   unsigned int err;
   unsigned int n_work;      // cells k_flux listed for k_update
   unsigned int n_cand[3];   // candidate tiles of k_mask passes, three slots
@@ -83,7 +86,7 @@ struct Grid {
   double* erosion;   // [ns][N] eroded thickness (erosion runs only)
   unsigned long long* vuln;  // [N] bit (i_thk*n_pdyn + i_pdyn)
   unsigned char* mask;       // [N]
-  int* tile_list;            // [2][ntiles] active tiles, two slots
+  int* tile_list;            // [2][2 ntiles] hot half tiles (2 tile + half), two slots
   int* work;                 // [N] active cells with a non-zero state or right-hand side
   int* cand;                 // [3][ntiles] tiles the next k_mask pass visits
   unsigned int* cand_stamp;  // [ntiles] pass that last listed the tile

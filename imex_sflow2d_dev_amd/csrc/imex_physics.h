@@ -867,7 +867,22 @@ __device__ inline bool mass_exchange(const Params& P, const double* qp, const do
       if (two) sv[i0 + 1] = (lane == 0) ? o : s;
     }
   }
-  const double hindered = any ? pow_cr(1.0 - fmin_f(1.0, at / 0.6), 4.65, tab) : 0.0;
+  // the two remaining powers of the cell, the hindered-settling factor
+  // (1 - a/0.6)**4.65 and the entrainment's Ri**2.4, are independent: with
+  // a lane pair, lane 0 takes the first and lane 1 the second
+  const bool ent = P.entrainment && (h > 0.0) && (m.Ri > 0.0);
+  double hindered, pent;
+  if constexpr (LANES == 2) {
+    const bool mine = (lane == 0) ? any : ent;
+    const double px = (lane == 0) ? 1.0 - fmin_f(1.0, at / 0.6) : m.Ri;
+    const double pr = mine ? pow_cr(px, (lane == 0) ? 4.65 : 2.4, tab) : 0.0;
+    const double po = __shfl_xor(pr, 1);
+    hindered = (lane == 0) ? pr : po;
+    pent = (lane == 0) ? po : pr;
+  } else {
+    hindered = any ? pow_cr(1.0 - fmin_f(1.0, at / 0.6), 4.65, tab) : 0.0;
+    pent = ent ? pow_cr(m.Ri, 2.4, tab) : 0.0;
+  }
 #pragma unroll
   for (int i = 0; i < NS; i++) {
     if ((alphas[i] > 0.0) && P.settling) {
@@ -889,8 +904,8 @@ __device__ inline bool mass_exchange(const Params& P, const double* qp, const do
   // before using it (src/constitutive_2d.f90:5044)
   const double T_er = P.erodible_deposit ? T : P.T_erodible;
   double air = 0.0;
-  if (P.entrainment && (h > 0.0) && (m.Ri > 0.0)) {
-    double ec = 0.075 / sqrt(1.0 + 718.0 * pow_cr(m.Ri, 2.4, tab));
+  if (ent) {
+    double ec = 0.075 / sqrt(1.0 + 718.0 * pent);
     air = ec * sqrt(mod_vel2);
   }
   double rdt = 0.0, rert = 0.0, A = 0.0, D = 0.0;
