@@ -453,12 +453,68 @@ def main():
     print(json.dumps(out), flush=True)
 
 
+def ensemble_host_baseline(r, t_end: float, seconds: float = 20.0):
+    """The reference's own ensemble pattern on this host
+    (EXAMPLES/EXAMPLE_ENSEMBLE_ETNA/launch_jobs.py: independent runs, one
+    single-threaded process each): a sample of members spread over the
+    ensemble's grid sizes, one process per physical core at once, each timed
+    for up to `seconds` on the reference build (oracle/ref_ensemble.py).  A
+    member that does not finish is extrapolated by its steps: time = elapsed
+    x (the member's step count in the batch) / (steps done) -- its early,
+    smaller flow makes this an underestimate of the CPU time.  runs/hr = the
+    ensemble's member count over (mean member time x members / cores)."""
+    from oracle import refpy
+    if not refpy.available("perf"):
+        return None
+    n, topo = host_cores()
+    members = sorted(set(np.linspace(0, r["members"] - 1, n).astype(int).tolist()))
+    env = dict(os.environ)
+    env.update(OMP_NUM_THREADS="1", OMP_PROC_BIND="true")
+    env.setdefault("OMP_STACKSIZE", "256M")
+    procs = []
+    for m in members:
+        cmd = [sys.executable, "-m", "oracle.ref_ensemble", "--member", str(m),
+               "--t-end", str(t_end), "--seconds", str(seconds)]
+        procs.append((m, subprocess.Popen(cmd, cwd=HERE, env=env, stdin=subprocess.DEVNULL,
+                                          stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                                          text=True)))
+    res = []
+    for m, p in procs:
+        out, err = p.communicate(timeout=seconds * 20 + 600)
+        if p.returncode != 0:
+            print(f"ensemble baseline: member {m} failed ({p.returncode}): {err[-1000:]}",
+                  file=sys.stderr, flush=True)
+            return None
+        d = json.loads([ln for ln in out.splitlines() if ln.startswith("{")][-1])
+        need = r["_member_steps"][m]
+        d["member_time_s"] = (d["elapsed_s"] if d["finished"]
+                              else d["elapsed_s"] * need / max(d["steps"], 1))
+        res.append(d)
+    mean_t = float(np.mean([d["member_time_s"] for d in res]))
+    wall = mean_t * r["members"] / len(members)
+    return {
+        "value": r["members"] / wall * 3600.0, "unit": "runs/hr", "cores": len(members),
+        "kind": "reference",
+        "sample": (f"members {members} of the {r['members']}, each a single-threaded process of "
+                   f"the reference build (OMP_NUM_THREADS=1, {len(members)} at once on "
+                   f"{len(members)} physical cores), timed for up to {seconds:.0f} s each; "
+                   f"{sum(d['finished'] for d in res)} finished, the others extrapolated by "
+                   f"their batch step counts; mean member time {mean_t:.1f} s; host {topo}"),
+    }
+
+
 def bench_c4(args):
     """The batched ensemble (256 members of 128^2..1024^2 cells, each to its
-    own t_end) against the same runs one after the other on the same GPU."""
+    own t_end) against the same runs one after the other on the same GPU, in
+    FP64 (value) and in the FP32 form (beside it); the reference's own
+    ensemble pattern on the host cores as cpu_baseline."""
     from imex_sflow2d_dev_amd import ensemble
-    r = ensemble.bench_ensemble(n_members=args.members, t_end=args.t_end,
-                                log=lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True))
+    say = lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True)  # noqa: E731
+    r = ensemble.bench_ensemble(n_members=args.members, t_end=args.t_end, log=say)
+    r32 = ensemble.bench_ensemble(n_members=args.members, t_end=args.t_end, log=say, precision=1)
+    base = None if args.no_cpu else ensemble_host_baseline(r, args.t_end)
+    r.pop("_member_steps")
+    r32.pop("_member_steps")
     out = {
         "metric": "ensemble runs/hr",
         "value": r["runs_per_hour_batched"],
@@ -473,7 +529,9 @@ def bench_c4(args):
         "dtype": "f64",
         "data": "synthetic (seeded PCG64 fractal DEMs and member parameters, generated in-process)",
         "config": dict({"workload": (f"c4: {r['members']} members, sides log-uniform in 128..1024 "
-                                     f"cells (dx 10 m), each to t_end = {r['t_end_s']} s, FP64")}, **r),
+                                     f"cells (dx 10 m), each to t_end = {r['t_end_s']} s, FP64; "
+                                     f"the FP32 form beside it")}, **r, fp32_form=r32),
+        "cpu_baseline": base,
     }
     print(json.dumps(out), flush=True)
 

@@ -2693,25 +2693,38 @@ struct Member {
   Grid G;
 };
 
-__global__ __launch_bounds__(NTHR) void kb_mask(const Member* M, int par, unsigned int seq) {
+// The pass parity and number of the batch's next step, in device memory so
+// that one captured step replays: the step's launches read them and its
+// commit launch advances them.
+struct BatchCtl {
+  int par;
+  unsigned int seq;
+};
+__global__ void kb_set_ctl(BatchCtl* c, int par, unsigned int seq) {
+  c->par = par;
+  c->seq = seq;
+}
+
+template <typename S>
+__global__ __launch_bounds__(NTHR) void kb_mask(const Member* M, const BatchCtl* ctl) {
   const Member& m = M[blockIdx.y];
   const DevState* st = m.G.st;
   if (st->done) return;
-  mask_body<double>(m.P, m.G, 0, par, seq, st->full_next ? 1 : 0);
+  mask_body<S>(m.P, m.G, 0, ctl->par, ctl->seq, st->full_next ? 1 : 0);
 }
-template <int NS>
-__global__ __launch_bounds__(NTHR) void kb_flux(const Member* M, int par) {
+template <int NS, typename F, int THR>
+__global__ __launch_bounds__(THR) void kb_flux(const Member* M, const BatchCtl* ctl) {
   const Member& m = M[blockIdx.y];
   if (m.G.st->done) return;
-  flux_body<NS, double>(m.P, m.G, 1, 0.0, par);
+  flux_body<NS, F, THR>(m.P, m.G, 1, 0.0, ctl->par);
 }
-template <int NS>
+template <int NS, typename S>
 __global__ __launch_bounds__(NTHR) void kb_update(const Member* M) {
   const Member& m = M[blockIdx.y];
   if (m.G.st->done) return;
-  update_body<NS, 7, double>(m.P, m.G, 0.0, 1, 0);
+  update_body<NS, 7, S>(m.P, m.G, 0.0, 1, 0);
 }
-__global__ void kb_commit(const Member* M, int n) {
+__global__ void kb_commit(const Member* M, int n, BatchCtl* ctl) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
     DevState* st = M[i].G.st;
@@ -2719,6 +2732,10 @@ __global__ void kb_commit(const Member* M, int n) {
       commit_step(st);
       st->full_next = 0u;
     }
+  }
+  if (i == 0) {
+    ctl->par ^= 1;
+    ctl->seq += 1u;
   }
 }
 __global__ void kb_status(const Member* M, int n, unsigned int* done, double* t) {
@@ -2734,7 +2751,15 @@ struct imex_batch {
   int ns = 0;
   int device = 0;
   int n_cu = 256;
+  bool fp32 = false;                 // members of the FP32 form
   int gx = 0;                        // blocks per member and launch; 0: from the member count
+  int flux_thr = NTHR;               // kb_flux block: NTHR (two passes per phase) or 2 NTHR
+  bool graphs = true;                // replay captured steps (else launch kernel by kernel)
+  BatchCtl* d_ctl = nullptr;
+  // captured steps: one step and GSTEPS steps, for the launch shape in key
+  hipGraphExec_t g1 = nullptr, gk = nullptr;
+  hipGraph_t graph1 = nullptr, graphk = nullptr;
+  long long key[4] = {0, 0, 0, 0};
   hipStream_t stream = nullptr;
   std::vector<imex_handle*> mem;
   Member* d_mem = nullptr;
@@ -2780,11 +2805,44 @@ static int reset_lists(imex_handle* h, double t_end, unsigned int full_next) {
   return 0;
 }
 
-extern "C" int imex_batch_create(int n_solid, imex_batch** out) {
-  if (!out || n_solid < 1 || n_solid > 3) return -2;
+// the kb_flux kernel of a batch's solid classes, precision and block size
+static const void* kb_flux_fn(int ns, bool fp32, int thr) {
+  if (thr == NTHR) {
+    switch (ns) {
+      case 1: return fp32 ? (const void*)kb_flux<1, float, NTHR> : (const void*)kb_flux<1, double, NTHR>;
+      case 2: return fp32 ? (const void*)kb_flux<2, float, NTHR> : (const void*)kb_flux<2, double, NTHR>;
+      default: return fp32 ? (const void*)kb_flux<3, float, NTHR> : (const void*)kb_flux<3, double, NTHR>;
+    }
+  }
+  switch (ns) {
+    case 1: return fp32 ? (const void*)kb_flux<1, float, FLUX_THR> : (const void*)kb_flux<1, double, FLUX_THR>;
+    case 2: return fp32 ? (const void*)kb_flux<2, float, FLUX_THR> : (const void*)kb_flux<2, double, FLUX_THR>;
+    default: return fp32 ? (const void*)kb_flux<3, float, FLUX_THR> : (const void*)kb_flux<3, double, FLUX_THR>;
+  }
+}
+static const void* kb_update_fn(int ns, bool fp32) {
+  switch (ns) {
+    case 1: return fp32 ? (const void*)kb_update<1, float> : (const void*)kb_update<1, double>;
+    case 2: return fp32 ? (const void*)kb_update<2, float> : (const void*)kb_update<2, double>;
+    default: return fp32 ? (const void*)kb_update<3, float> : (const void*)kb_update<3, double>;
+  }
+}
+
+static void batch_drop_graphs(imex_batch* b) {
+  if (b->g1) (void)hipGraphExecDestroy(b->g1);
+  if (b->gk) (void)hipGraphExecDestroy(b->gk);
+  if (b->graph1) (void)hipGraphDestroy(b->graph1);
+  if (b->graphk) (void)hipGraphDestroy(b->graphk);
+  b->g1 = b->gk = nullptr;
+  b->graph1 = b->graphk = nullptr;
+}
+
+extern "C" int imex_batch_create_form(int n_solid, int precision, imex_batch** out) {
+  if (!out || n_solid < 1 || n_solid > 3 || precision < 0 || precision > 1) return -2;
   imex_batch* b = new imex_batch();
   *out = b;
   b->ns = n_solid;
+  b->fp32 = precision == 1;
   const char* dev = getenv("IMEX_DEVICE");
   b->device = dev ? atoi(dev) : 0;
   BCHK(b, hipSetDevice(b->device));
@@ -2792,13 +2850,22 @@ extern "C" int imex_batch_create(int n_solid, imex_batch** out) {
   int ncu = 0;
   if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, b->device) == hipSuccess && ncu > 0)
     b->n_cu = ncu;
-  b->lds = lds_for(n_solid, false, true);
-  switch (n_solid) {
-    case 1: BCHK(b, hipFuncSetAttribute((const void*)kb_flux<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b->lds)); break;
-    case 2: BCHK(b, hipFuncSetAttribute((const void*)kb_flux<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b->lds)); break;
-    default: BCHK(b, hipFuncSetAttribute((const void*)kb_flux<3>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b->lds)); break;
-  }
+  for (int thr : {NTHR, FLUX_THR})
+    BCHK(b, hipFuncSetAttribute(kb_flux_fn(n_solid, b->fp32, thr), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds_for(n_solid, b->fp32, thr == NTHR)));
+  b->lds = lds_for(n_solid, b->fp32, true);
+  BCHK(b, hipMalloc(&b->d_ctl, sizeof(BatchCtl)));
+  // the first step's pass parity and number
+  b->par = 0;
+  b->seq = 0;
+  hipLaunchKernelGGL(kb_set_ctl, dim3(1), dim3(1), 0, b->stream, b->d_ctl, 1, 1u);
+  BCHK(b, hipGetLastError());
+  BCHK(b, hipStreamSynchronize(b->stream));
   return 0;
+}
+
+extern "C" int imex_batch_create(int n_solid, imex_batch** out) {
+  return imex_batch_create_form(n_solid, 0, out);
 }
 
 extern "C" int imex_batch_set_blocks(imex_batch* b, int blocks_per_member) {
@@ -2807,9 +2874,18 @@ extern "C" int imex_batch_set_blocks(imex_batch* b, int blocks_per_member) {
   return 0;
 }
 
+extern "C" int imex_batch_set_launch(imex_batch* b, int flux_threads, int graphs) {
+  if (!b || (flux_threads != NTHR && flux_threads != FLUX_THR) || graphs < 0 || graphs > 1) return -2;
+  b->flux_thr = flux_threads;
+  b->lds = lds_for(b->ns, b->fp32, flux_threads == NTHR);
+  b->graphs = graphs == 1;
+  return 0;
+}
+
 extern "C" int imex_batch_add(imex_batch* b, imex_handle* h, double t_end) {
   // members step together with no per-member slope pass: no topography change
-  if (!b || !h || h->P.ns != b->ns || h->device != b->device || h->fp32 || h->P.topo_change)
+  if (!b || !h || h->P.ns != b->ns || h->device != b->device || h->fp32 != b->fp32 ||
+      h->P.topo_change)
     return -2;
   for (imex_handle* x : b->mem)
     if (x == h) return -2;
@@ -2871,6 +2947,46 @@ static int batch_table(imex_batch* b) {
   return 0;
 }
 
+// One batch step: the active sets, the fluxes and the updates of every
+// member, then the commit (which also advances the step's pass parity and
+// number in BatchCtl).  Kernel arguments stay the same from step to step.
+static int batch_launch_step(imex_batch* b, int n, int gx) {
+  const dim3 g(gx, n), blk(NTHR);
+  if (b->fp32)
+    hipLaunchKernelGGL(kb_mask<float>, g, blk, 0, b->stream, b->d_mem, b->d_ctl);
+  else
+    hipLaunchKernelGGL(kb_mask<double>, g, blk, 0, b->stream, b->d_mem, b->d_ctl);
+  const Member* m = b->d_mem;
+  const BatchCtl* c = b->d_ctl;
+  void* fargs[] = {(void*)&m, (void*)&c};
+  BCHK(b, hipLaunchKernel(kb_flux_fn(b->ns, b->fp32, b->flux_thr), g, dim3(b->flux_thr), fargs, b->lds,
+                          b->stream));
+  void* uargs[] = {(void*)&m};
+  BCHK(b, hipLaunchKernel(kb_update_fn(b->ns, b->fp32), g, blk, uargs, 0, b->stream));
+  hipLaunchKernelGGL(kb_commit, dim3((n + 63) / 64), dim3(64), 0, b->stream, b->d_mem, n, b->d_ctl);
+  BCHK(b, hipGetLastError());
+  return 0;
+}
+
+constexpr int GSTEPS = 8;  // steps in the longer captured graph
+
+// captures k steps of the current launch shape into an executable graph
+static int batch_capture(imex_batch* b, int n, int gx, int k, hipGraph_t* graph, hipGraphExec_t* exec) {
+  BCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  int rc = 0;
+  for (int i = 0; i < k && !rc; i++) rc = batch_launch_step(b, n, gx);
+  hipGraph_t gr = nullptr;
+  const hipError_t e = hipStreamEndCapture(b->stream, &gr);
+  if (rc) {
+    if (gr) (void)hipGraphDestroy(gr);
+    return rc;
+  }
+  if (e != hipSuccess) return bset_err(b, "hipStreamEndCapture", e);
+  *graph = gr;
+  BCHK(b, hipGraphInstantiate(exec, gr, nullptr, nullptr, 0));
+  return 0;
+}
+
 // nsteps batch steps; members whose t reaches t_end stop on the way.
 extern "C" int imex_batch_run(imex_batch* b, int nsteps) {
   if (!b || nsteps < 0) return -2;
@@ -2881,28 +2997,27 @@ extern "C" int imex_batch_run(imex_batch* b, int nsteps) {
   if (n > 65535) return -2;
   // automatic: about 16 blocks per CU in all, at least 8 and at most 1024 per member
   const int gx = b->gx ? b->gx : std::min(1024, std::max(8, 16 * b->n_cu / n));
-  const dim3 g(gx, n), blk(NTHR);
-  for (int k = 0; k < nsteps; k++) {
-    b->par ^= 1;
-    b->seq++;
-    hipLaunchKernelGGL(kb_mask, g, blk, 0, b->stream, b->d_mem, b->par, b->seq);
-    switch (b->ns) {
-      case 1:
-        hipLaunchKernelGGL(kb_flux<1>, g, blk, b->lds, b->stream, b->d_mem, b->par);
-        hipLaunchKernelGGL(kb_update<1>, g, blk, 0, b->stream, b->d_mem);
-        break;
-      case 2:
-        hipLaunchKernelGGL(kb_flux<2>, g, blk, b->lds, b->stream, b->d_mem, b->par);
-        hipLaunchKernelGGL(kb_update<2>, g, blk, 0, b->stream, b->d_mem);
-        break;
-      default:
-        hipLaunchKernelGGL(kb_flux<3>, g, blk, b->lds, b->stream, b->d_mem, b->par);
-        hipLaunchKernelGGL(kb_update<3>, g, blk, 0, b->stream, b->d_mem);
-        break;
+  if (!b->graphs) {
+    for (int k = 0; k < nsteps; k++)
+      if (int rc = batch_launch_step(b, n, gx)) return rc;
+  } else {
+    const long long key[4] = {(long long)(uintptr_t)b->d_mem, n, gx, b->flux_thr};
+    if (!std::equal(key, key + 4, b->key)) {
+      BCHK(b, hipStreamSynchronize(b->stream));
+      batch_drop_graphs(b);
+      std::copy(key, key + 4, b->key);
     }
-    hipLaunchKernelGGL(kb_commit, dim3((n + 63) / 64), dim3(64), 0, b->stream, b->d_mem, n);
-    BCHK(b, hipGetLastError());
+    if (!b->g1)
+      if (int rc = batch_capture(b, n, gx, 1, &b->graph1, &b->g1)) return rc;
+    if (nsteps >= GSTEPS && !b->gk)
+      if (int rc = batch_capture(b, n, gx, GSTEPS, &b->graphk, &b->gk)) return rc;
+    int k = 0;
+    for (; k + GSTEPS <= nsteps; k += GSTEPS) BCHK(b, hipGraphLaunch(b->gk, b->stream));
+    for (; k < nsteps; k++) BCHK(b, hipGraphLaunch(b->g1, b->stream));
   }
+  // the host's copy of the pass parity and number (the device's advance per step)
+  b->par ^= nsteps & 1;
+  b->seq += (unsigned int)nsteps;
   return 0;
 }
 
@@ -2948,7 +3063,9 @@ extern "C" void imex_batch_destroy(imex_batch* b) {
   if (!b) return;
   (void)hipSetDevice(b->device);
   if (b->stream) (void)hipStreamSynchronize(b->stream);
+  batch_drop_graphs(b);
   if (b->d_mem) (void)hipFree(b->d_mem);
+  if (b->d_ctl) (void)hipFree(b->d_ctl);
   if (b->d_done) (void)hipFree(b->d_done);
   if (b->d_t) (void)hipFree(b->d_t);
   if (b->stream) (void)hipStreamDestroy(b->stream);

@@ -19,17 +19,34 @@ class Batch:
     fields and initial state uploaded); while in the batch their own step
     calls must not be used."""
 
-    def __init__(self, n_solid: int, blocks_per_member: int = 0):
+    def __init__(self, n_solid: int, blocks_per_member: int = 0, precision: int = 0,
+                 flux_threads: int = 256, graphs: bool = True):
+        """precision 0: FP64 members, 1: members of the FP32 form (every
+        member of a batch has the batch's precision).  flux_threads: the flux
+        kernel's block, 256 (a hot half tile in two passes per phase) or 512
+        (one pass per phase, or two half tiles per block when they outnumber
+        the blocks).  graphs: replay captured HIP graphs of the step rather
+        than launching its kernels one by one (the results are the same)."""
         self.L = native.lib()
         self.n_solid = n_solid
+        self.precision = precision
         b = ctypes.c_void_p()
-        rc = self.L.imex_batch_create(ctypes.c_int(n_solid), ctypes.byref(b))
+        rc = self.L.imex_batch_create_form(ctypes.c_int(n_solid), ctypes.c_int(precision),
+                                           ctypes.byref(b))
         self.b = b
-        self._check(rc, "imex_batch_create")
+        self._check(rc, "imex_batch_create_form")
         self._check(self.L.imex_batch_set_blocks(self.b, ctypes.c_int(blocks_per_member)),
                     "imex_batch_set_blocks")
+        self.set_launch(flux_threads, graphs)
         self.members: List[Solver] = []
         self.t_end: Dict[int, float] = {}
+
+    def set_launch(self, flux_threads: int = 256, graphs: bool = True):
+        self._check(self.L.imex_batch_set_launch(self.b, ctypes.c_int(flux_threads),
+                                                 ctypes.c_int(1 if graphs else 0)),
+                    "imex_batch_set_launch")
+        self.flux_threads = flux_threads
+        self.graphs = graphs
 
     def _err(self) -> str:
         buf = ctypes.create_string_buffer(512)
@@ -103,28 +120,46 @@ def run_alone(s: Solver, t_end: float, max_steps: int = 10 ** 7) -> int:
     return n
 
 
-def autotune_blocks(batch: Batch, candidates=(0, 4, 8, 16, 32, 64), steps: int = 8):
-    """Picks the blocks per member (the waves per CU of a batch launch; 0 =
-    the library's choice from the member count) on the batch itself: each
-    candidate times `steps` batch steps of the members,
-    which advance meanwhile (useful work, nothing is undone), and the fastest
-    is kept.  Returns (best, {candidate: seconds per step})."""
+def autotune(batch: Batch, blocks=(0, 4, 8, 16, 32, 64), flux_threads=(256, 512),
+             steps: int = 8):
+    """Picks the launch shape of a batch on the batch itself: the flux
+    kernel's work shape (a hot half tile on 256 threads in two passes per
+    phase, or on 512 threads in one pass) and the blocks per member (with
+    the member count, the waves per CU of each launch; 0 = the library's
+    choice).  Each candidate times `steps` batch steps of the members, which
+    advance meanwhile (useful work, nothing is undone), and the fastest is
+    kept.  Returns ((blocks, flux_threads), {"blocks/threads": seconds per
+    step})."""
     import time
     table = {}
-    for gx in candidates:
-        batch._check(batch.L.imex_batch_set_blocks(batch.b, ctypes.c_int(gx)), "imex_batch_set_blocks")
-        batch.status()
-        t0 = time.perf_counter()
-        batch.run(steps)
-        batch.status()
-        table[gx] = (time.perf_counter() - t0) / steps
+    for ft in flux_threads:
+        batch.set_launch(ft, batch.graphs)
+        for gx in blocks:
+            batch._check(batch.L.imex_batch_set_blocks(batch.b, ctypes.c_int(gx)),
+                         "imex_batch_set_blocks")
+            batch.status()
+            batch.run(1)           # the capture of a new launch shape, outside the clock
+            batch.status()
+            t0 = time.perf_counter()
+            batch.run(steps)
+            batch.status()
+            table[(gx, ft)] = (time.perf_counter() - t0) / steps
     best = min(table, key=table.get)
-    batch._check(batch.L.imex_batch_set_blocks(batch.b, ctypes.c_int(best)), "imex_batch_set_blocks")
-    return best, table
+    batch.set_launch(best[1], batch.graphs)
+    batch._check(batch.L.imex_batch_set_blocks(batch.b, ctypes.c_int(best[0])),
+                 "imex_batch_set_blocks")
+    return best, {f"{k[0]}/{k[1]}": v for k, v in table.items()}
+
+
+def autotune_blocks(batch: Batch, candidates=(0, 4, 8, 16, 32, 64), steps: int = 8):
+    """autotune over the blocks per member only (the flux shape kept)."""
+    best, table = autotune(batch, blocks=candidates, flux_threads=(batch.flux_threads,),
+                           steps=steps)
+    return best[0], table
 
 
 def bench_ensemble(n_members: int = 256, t_end: float = 30.0, chunk: int = 25,
-                   min_side: int = 128, max_side: int = 1024, log=None):
+                   min_side: int = 128, max_side: int = 1024, log=None, precision: int = 0):
     """The batched ensemble against the same runs one after the other through
     the single-run device driver.  Set-up (DEMs, static fields, uploads) is
     outside both clocks.  Returns a dict of the measurements."""
@@ -135,6 +170,7 @@ def bench_ensemble(n_members: int = 256, t_end: float = 30.0, chunk: int = 25,
     specs = []
     for m in range(n_members):
         cfg, B = workloads.config4_member(m, min_side=min_side, max_side=max_side, t_end=t_end)
+        cfg.precision = precision
         st, q0 = init.setup(cfg, B)
         specs.append((cfg, st, q0))
     say(f"host set-up of {n_members} members: {time.perf_counter() - t_set:.1f} s")
@@ -142,12 +178,12 @@ def bench_ensemble(n_members: int = 256, t_end: float = 30.0, chunk: int = 25,
 
     # batched
     sol = [Solver(*sp) for sp in specs]
-    b = Batch(n_solid=1)
+    b = Batch(n_solid=1, precision=precision)
     for s, (cfg, _, _) in zip(sol, specs):
         b.add(s, cfg.t_end)
     b.status()
     t0 = time.perf_counter()
-    best, table = autotune_blocks(b)
+    best, table = autotune(b)
     steps = b.run_to_end(chunk=chunk)
     done, nsteps, tfin = b.status()
     wall_batch = time.perf_counter() - t0
@@ -182,7 +218,9 @@ def bench_ensemble(n_members: int = 256, t_end: float = 30.0, chunk: int = 25,
         "runs_per_hour_batched": n_members / wall_batch * 3600.0,
         "runs_per_hour_sequential": n_members / wall_seq * 3600.0,
         "speedup": wall_seq / wall_batch,
-        "blocks_per_member": best,
-        "autotune_s_per_step": {str(k): v for k, v in table.items()},
+        "precision": "fp32" if precision else "fp64",
+        "blocks_per_member": best[0], "flux_threads": best[1], "hip_graphs": True,
+        "autotune_s_per_step": table,
         "batch_equals_alone_checked_members": sorted(q_batch), "batch_equals_alone": bool(same),
+        "_member_steps": nsteps.tolist(),
     }

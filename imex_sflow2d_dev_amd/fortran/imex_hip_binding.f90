@@ -326,11 +326,23 @@ MODULE imex_hip_binding
        INTEGER(c_int), VALUE :: n_solid
        TYPE(c_ptr), INTENT(OUT) :: b
      END FUNCTION imex_batch_create
+     INTEGER(c_int) FUNCTION imex_batch_create_form(n_solid, precision, b)                   &
+          BIND(C, name='imex_batch_create_form')
+       IMPORT :: c_int, c_ptr
+       INTEGER(c_int), VALUE :: n_solid, precision
+       TYPE(c_ptr), INTENT(OUT) :: b
+     END FUNCTION imex_batch_create_form
      INTEGER(c_int) FUNCTION imex_batch_set_blocks(b, blocks) BIND(C, name='imex_batch_set_blocks')
        IMPORT :: c_int, c_ptr
        TYPE(c_ptr), VALUE :: b
        INTEGER(c_int), VALUE :: blocks
      END FUNCTION imex_batch_set_blocks
+     INTEGER(c_int) FUNCTION imex_batch_set_launch(b, flux_threads, graphs)                   &
+          BIND(C, name='imex_batch_set_launch')
+       IMPORT :: c_int, c_ptr
+       TYPE(c_ptr), VALUE :: b
+       INTEGER(c_int), VALUE :: flux_threads, graphs
+     END FUNCTION imex_batch_set_launch
      INTEGER(c_int) FUNCTION imex_batch_add(b, h, t_end) BIND(C, name='imex_batch_add')
        IMPORT :: c_int, c_ptr, c_double
        TYPE(c_ptr), VALUE :: b, h

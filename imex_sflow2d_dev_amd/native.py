@@ -34,7 +34,8 @@ EXPORTS = [
     "imex_timestep", "imex_rk_step", "imex_erosion_deposition", "imex_post_step",
     "imex_run_steps", "imex_run_steps_timed", "imex_calib_copy", "imex_get_time",
     "imex_get_time_history", "imex_get_counters", "imex_set_time", "imex_download", "imex_last_error", "imex_destroy",
-    "imex_batch_create", "imex_batch_set_blocks", "imex_batch_add", "imex_batch_remove",
+    "imex_batch_create", "imex_batch_create_form", "imex_batch_set_blocks",
+    "imex_batch_set_launch", "imex_batch_add", "imex_batch_remove",
     "imex_batch_run", "imex_batch_status", "imex_batch_last_error", "imex_batch_destroy",
     "imex_test_qc_to_qp", "imex_test_qp_to_qc", "imex_test_fluxes", "imex_test_speeds",
     "imex_test_implicit_cplx", "imex_test_implicit_real", "imex_test_semi_impl", "imex_test_expl",

@@ -332,9 +332,20 @@ int imex_format_value(int fmt, double x, char *out);
  * same handle.  All members have the same n_solid and device. */
 typedef struct imex_batch imex_batch;
 int imex_batch_create(int n_solid, imex_batch **b);
+/* as imex_batch_create, for members of the given imex_config.precision (0:
+ * FP64, 1: the FP32 form); every member of a batch has its precision */
+int imex_batch_create_form(int n_solid, int precision, imex_batch **b);
 /* blocks per member and launch; 0 (the default): about 16 blocks per CU
  * shared by the members, 8 to 1024 per member */
 int imex_batch_set_blocks(imex_batch *b, int blocks_per_member);
+/* launch shape: flux_threads 256 (the default: each hot half tile of a
+ * member on 256 threads, two passes per phase) or 512 (one pass per phase,
+ * or two half tiles per block when a member has more than blocks); graphs 1
+ * (the default): each step is a captured HIP graph (one of 8 steps and one
+ * of 1 step, recaptured when the members or the launch shape change)
+ * replayed by imex_batch_run, 0: the step's kernels are launched one by
+ * one.  Results do not depend on either setting. */
+int imex_batch_set_launch(imex_batch *b, int flux_threads, int graphs);
 /* the handle joins at the next step and runs until its t reaches t_end; the
  * handle's own calls must not be used while it is a member */
 int imex_batch_add(imex_batch *b, imex_handle *h, double t_end);
