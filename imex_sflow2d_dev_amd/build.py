@@ -24,11 +24,28 @@ FDRIVER = os.path.join(LIBDIR, "imex_fdriver")
 ARCH = os.environ.get("IMEX_OFFLOAD_ARCH", "gfx950")
 
 
-def compile_cmd(out: str = LIB):
+# the phase-clock build: the same library with per-cell shader-clock readings
+# of the update kernel (imex_physics.h, PhaseSlot), read by phase_clocks.py
+LIB_PHASES = os.path.join(LIBDIR, "libimexhip_phases.so")
+
+
+def compile_cmd(out: str = LIB, defines=()):
     return [HIPCC, f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
             "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-            "-I", os.path.join(ROOT, "include"), "-I", os.path.join(HERE, "csrc"),
-            "-o", out] + SRCS
+            "-I", os.path.join(ROOT, "include"), "-I", os.path.join(HERE, "csrc")] + \
+        [f"-D{d}" for d in defines] + ["-o", out] + SRCS
+
+
+def build_phase_clocks() -> str:
+    """_lib/libimexhip_phases.so, the library with the phase clocks."""
+    os.makedirs(LIBDIR, exist_ok=True)
+    tmp = LIB_PHASES + ".tmp"
+    r = subprocess.run(compile_cmd(tmp, ("IMEX_PHASE_CLOCKS",)), capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("hipcc failed for libimexhip_phases.so")
+    os.replace(tmp, LIB_PHASES)
+    return LIB_PHASES
 
 
 def build(verbose: bool = False) -> str:
@@ -65,4 +82,7 @@ def build_fortran() -> str:
 
 
 if __name__ == "__main__":
-    print(build(verbose=True))
+    if "--phase-clocks" in sys.argv[1:]:
+        print(build_phase_clocks())
+    else:
+        print(build(verbose=True))

@@ -1144,7 +1144,7 @@ template <int NS, int LANES>
 __device__ bool lnsrch(const Params& P, double xi, double dt, const double* xold, const double* qorg,
                        const double* q0, double fold, const double* g, double* p, double* x,
                        double& f_out, double* rt, double stpmax, const double* Rni,
-                       unsigned int& err, int lane) {
+                       unsigned int& err, int lane, PhaseClk* pc = nullptr) {
   constexpr int NV = 4 + NS;
   const double TOLX = kEps;
   double sum = 0.0;
@@ -1187,6 +1187,7 @@ __device__ bool lnsrch(const Params& P, double xi, double dt, const double* xold
   // step) would loop for ever; the bound turns that into an error.
   int trials = 0;
   while (true) {
+    pc_add(pc, PC_TRIALS, 1ULL);
     const double a = (lane == 0) ? base : 0.5 * base;
 #pragma unroll
     for (int i = 0; i < NV; i++) xt[i] = xold[i] + a * p[i];
@@ -1243,7 +1244,8 @@ __device__ bool lnsrch(const Params& P, double xi, double dt, const double* xold
 
 template <int NS, int LANES>
 __device__ int solve_rk_step(const Params& P, double xi, double dt, double* qj, const double* q0,
-                             const double* Rni, unsigned int& err, int lane) {
+                             const double* Rni, unsigned int& err, int lane,
+                             PhaseClk* pc = nullptr) {
   constexpr int NV = 4 + NS;
   const double TOLF = 1.0e-10, STPMX = 100.0;
   double qorg[NV], qrel[NV], rt[NV], xold[NV], p[NV], grad[NV];
@@ -1263,7 +1265,9 @@ __device__ int solve_rk_step(const Params& P, double xi, double dt, double* qj, 
 #pragma unroll
     for (int i = 0; i < NV; i++) mx = fmax_f(mx, fabs(rt[i]));
     if (mx < TOLF) break;
+    const unsigned long long cj = pc_now();
     eval_jacobian<NS, LANES>(P, xi, dt, qrel, qorg, L11, L21, L12, L22, lane);
+    pc_add(pc, PC_JAC, pc_now() - cj);
     double d1[NV - 2];
     int ne = 0;
 #pragma unroll
@@ -1312,8 +1316,10 @@ __device__ int solve_rk_step(const Params& P, double xi, double dt, double* qj, 
         for (int r = 0; r < NV; r++) s += rt[r] * Lrc(r, i, qorg, L11, L21, L12, L22);
         grad[i] = s;
       }
+      const unsigned long long cl = pc_now();
       fresh = lnsrch<NS, LANES>(P, xi, dt, xold, qorg, q0, scal_f_old, grad, p, qrel, scal_f, rt,
-                                stpmax, Rni, err, lane);
+                                stpmax, Rni, err, lane, pc);
+      pc_add(pc, PC_LNSRCH, pc_now() - cl);
     } else {
 #pragma unroll
       for (int i = 0; i < NV; i++) qrel[i] = xold[i] + p[i];
@@ -1342,7 +1348,8 @@ __device__ int solve_rk_step(const Params& P, double xi, double dt, double* qj, 
 // stage 2 + assembly for one cell; q holds q0 on entry, the new state on exit
 template <int NS, int LANES>
 __device__ void stage2(const Params& P, double xi, double dt, double* q, const double* X,
-                       unsigned int& err, int lane) {
+                       unsigned int& err, int lane, PhaseClk* pc = nullptr) {
+  pc_mark(pc, PC_STAGE2);
   constexpr int NV = 4 + NS, NP = NV + 2;
   double q0[NV], qfv[NV], qp[NP], q_si[NV], qg[NV], si[NV], Rj[NV], pd;
 #pragma unroll
@@ -1385,7 +1392,10 @@ __device__ void stage2(const Params& P, double xi, double dt, double* q, const d
       const double B = 0.0 + (0.0 + 0.0) * 0.0;
       Rj[i] = (A - B) - 1.0 * si[i];
     }
-    solve_rk_step<NS, LANES>(P, xi, dt, qg, q0, Rj, err, lane);
+    pc_mark(pc, PC_NEWTON0);
+    const int its = solve_rk_step<NS, LANES>(P, xi, dt, qg, q0, Rj, err, lane, pc);
+    pc_mark(pc, PC_NEWTON1);
+    pc_add(pc, PC_ITS, (unsigned long long)its);
     if (q_si[1] * q_si[1] + q_si[2] * q_si[2] == 0.0) {
       qg[1] = 0.0;
       qg[2] = 0.0;
@@ -1457,7 +1467,7 @@ __device__ void stage2(const Params& P, double xi, double dt, double* q, const d
 template <int NS, int LANES>
 __device__ void erosion_deposition(const Params& P, double dt, double* q, double* depo,
                                    double* erod, double* ero, double& topo, unsigned int& err,
-                                   const ddm::dd* tab, int lane) {
+                                   const ddm::dd* tab, int lane, PhaseClk* pc = nullptr) {
   constexpr int NV = 4 + NS, NP = NV + 2;
   double qp[NP], pd, dep[NS], eq[NV], ers[NS];
   if (q[0] > 0.0) {
@@ -1467,7 +1477,8 @@ __device__ void erosion_deposition(const Params& P, double dt, double* q, double
     for (int i = 0; i < NP; i++) qp[i] = 0.0;
     qp[3] = P.T_amb;
   }
-  if (mass_exchange<NS, LANES>(P, qp, erod, dt, dep, eq, ers, topo, tab, lane)) err |= IMEX_ERR_NEG_DEP;
+  if (mass_exchange<NS, LANES>(P, qp, erod, dt, dep, eq, ers, topo, tab, lane, pc))
+    err |= IMEX_ERR_NEG_DEP;
 #pragma unroll
   for (int i = 0; i < NV; i++) q[i] = q[i] + dt * eq[i];
 #pragma unroll
@@ -1563,6 +1574,31 @@ __device__ void epilogue(const Params& P, const double* q, double& hmax, double&
 // with the epilogue ends the step: dt history, t += dt, CFL minimum reset.
 // S: the storage type of the state and right-hand side (float in the FP32
 // form); the update itself runs in double.
+// The phase-clock build's record: per work-list entry of the update
+// launches that run with the recording switched on (the last one wins), the
+// clock readings relative to PC_START and the counts, for the first
+// kPhaseCells entries.  The two lanes of a pair run together, so their
+// cycles are the pair's; the settling count is the larger of the two lanes'
+// fixed points.
+#ifdef IMEX_PHASE_CLOCKS
+constexpr int kPhaseCells = 65536;
+__device__ unsigned long long g_phase[kPhaseCells * PC_NSLOT];
+__device__ int g_phase_on;
+__device__ __forceinline__ void record_phase_clocks(const PhaseClk& pc, int i, int lane) {
+  if (!g_phase_on || i >= kPhaseCells) return;
+  unsigned long long* r = g_phase + (size_t)i * PC_NSLOT;
+  const unsigned long long s_its = pc.t[PC_SETTLE_ITS];
+  const unsigned long long s_other = __shfl_xor(s_its, 1);
+  if (lane != 0) return;
+  for (int k = 1; k <= PC_END; k++) r[k] = pc.t[k] ? pc.t[k] - pc.t[PC_START] : 0ULL;
+  r[PC_START] = 0ULL;
+  for (int k = PC_ITS; k < PC_NSLOT; k++) r[k] = pc.t[k];
+  r[PC_SETTLE_ITS] = (s_its > s_other) ? s_its : s_other;  // the pair's longer fixed point
+}
+#else
+__device__ __forceinline__ void record_phase_clocks(const PhaseClk&, int, int) {}
+#endif
+
 template <int NS, int PARTS, typename S = double>
 __device__ __forceinline__ void update_body(const Params& P, const Grid& G, double dt_arg, int mode,
                                             int step_idx) {
@@ -1614,6 +1650,9 @@ __device__ __forceinline__ void update_body(const Params& P, const Grid& G, doub
   const bool sed = SED && (P.settling || P.entrainment || P.erosion_coeff > 0.0);
   const bool er = sed && G.erodible != nullptr;
   for (int i = (int)(blockIdx.x * NTHR + threadIdx.x) / LANES; i < n; i += stride) {
+    PhaseClk pcl{};
+    PhaseClk* pc = kPhaseClocks ? &pcl : nullptr;
+    pc_mark(pc, PC_START);
     const int c = ALL ? i : G.work[i];
     double q[NV], X[NV];
 #pragma unroll
@@ -1623,7 +1662,10 @@ __device__ __forceinline__ void update_body(const Params& P, const Grid& G, doub
       for (int v = 0; v < NV; v++) X[v] = (double)state_R<S>(G)[(size_t)v * N + c];
     }
     const bool wet0 = q[0] > 0.0;
-    if constexpr ((PARTS & 1) != 0) stage2<NS, LANES>(P, cell_xi(P, G.noise, c), dt, q, X, err, lane);
+    if constexpr ((PARTS & 1) != 0) {
+      stage2<NS, LANES>(P, cell_xi(P, G.noise, c), dt, q, X, err, lane, pc);
+      pc_mark(pc, PC_STAGE2END);
+    }
     // The loads the rest of the cell's update needs are issued here, before
     // the sedimentation (whose settling fixed point covers their latency),
     // not where their values are first used: a load issued at its use would
@@ -1655,7 +1697,9 @@ __device__ __forceinline__ void update_body(const Params& P, const Grid& G, doub
     if constexpr (SED) {
       if (sed) {
         double topo;
-        erosion_deposition<NS, LANES>(P, dt, q, depo, erod, ero, topo, err, s_pow, lane);
+        pc_mark(pc, PC_SED0);
+        erosion_deposition<NS, LANES>(P, dt, q, depo, erod, ero, topo, err, s_pow, lane, pc);
+        pc_mark(pc, PC_SED1);
         if (P.topo_change) bc = bc + dt * topo;
       }
       // wet at the step start, exactly dry after the step's last change of q
@@ -1694,6 +1738,10 @@ __device__ __forceinline__ void update_body(const Params& P, const Grid& G, doub
         G.mvel[c] = mv;
         if (G.vuln) G.vuln[c] = vb;
       }
+    }
+    if constexpr (kPhaseClocks) {
+      pc_mark(pc, PC_END);
+      record_phase_clocks(pcl, i, lane);
     }
   }
   if (err && lead) atomicOr(&G.st->err, err);
@@ -3500,3 +3548,17 @@ extern "C" int imex_test_arith(imex_handle* h, int n, const double* a, const dou
   if (fin(h)) return -1;
   return down(h, out, o, (size_t)4 * n);
 }
+
+#ifdef IMEX_PHASE_CLOCKS
+// phase-clock build only (not part of include/imex_hip.h): switch the record
+// on for the following update launches, and read it
+extern "C" int imex_phase_clocks_enable(int on) {
+  return hipMemcpyToSymbol(HIP_SYMBOL(g_phase_on), &on, sizeof(int)) == hipSuccess ? 0 : -1;
+}
+extern "C" int imex_phase_clocks_read(unsigned long long* out, int n_cells) {
+  if (!out || n_cells < 0) return -2;
+  const size_t n = (size_t)std::min(n_cells, kPhaseCells) * PC_NSLOT;
+  return hipMemcpyFromSymbol(out, HIP_SYMBOL(g_phase), n * sizeof(unsigned long long)) == hipSuccess
+             ? 0 : -1;
+}
+#endif

@@ -50,6 +50,52 @@ struct Params {
   double hcs, one_by_hcs;  // complex step n_vars*eps and its reciprocal
 };
 
+// Per-cell phase clocks of the update kernel, compiled in only by the
+// phase-clock build (-DIMEX_PHASE_CLOCKS; imex_sflow2d_dev_amd/build.py
+// --phase-clocks, read by imex_sflow2d_dev_amd/phase_clocks.py): shader-clock
+// readings at the phase boundaries of one cell's update and its work counts.
+// In every other build the calls below compile to nothing.
+#ifdef IMEX_PHASE_CLOCKS
+constexpr bool kPhaseClocks = true;
+#else
+constexpr bool kPhaseClocks = false;
+#endif
+enum PhaseSlot {
+  PC_START = 0,      // the cell's update begins (work-list entry read)
+  PC_STAGE2 = 1,     // stage 2 begins (state and right-hand side loaded)
+  PC_NEWTON0 = 2,    // the Newton solve begins (q_fv, semi-implicit terms done)
+  PC_NEWTON1 = 3,    // the Newton solve ends
+  PC_STAGE2END = 4,  // assembly checks done
+  PC_SED0 = 5,       // sedimentation begins
+  PC_SED1 = 6,       // sedimentation ends
+  PC_END = 7,        // the cell's stores are issued
+  PC_ITS = 8,        // Newton iterations
+  PC_JAC = 9,        // cycles in the complex-step Jacobian
+  PC_LNSRCH = 10,    // cycles in the line search
+  PC_TRIALS = 11,    // line-search rounds (two trials each on a lane pair)
+  PC_SETTLE = 12,    // cycles in the settling-velocity fixed points
+  PC_SETTLE_ITS = 13,  // settling fixed-point iterations (this lane's classes)
+  PC_POW = 14,       // cycles in the hindered-settling and entrainment powers
+  PC_NSLOT = 16
+};
+struct PhaseClk {
+  unsigned long long t[PC_NSLOT];
+};
+__device__ __forceinline__ unsigned long long pc_now() {
+  if constexpr (kPhaseClocks) return clock64();
+  return 0ULL;
+}
+__device__ __forceinline__ void pc_mark(PhaseClk* pc, int slot) {
+  if constexpr (kPhaseClocks) {
+    if (pc) pc->t[slot] = clock64();
+  }
+}
+__device__ __forceinline__ void pc_add(PhaseClk* pc, int slot, unsigned long long v) {
+  if constexpr (kPhaseClocks) {
+    if (pc) pc->t[slot] += v;
+  }
+}
+
 // Fortran MIN(a, b) / MAX(a, b) as the reference build evaluates them:
 // (a < b) ? a : b and (a > b) ? a : b, so a NaN or a signed-zero tie
 // returns the second argument.
@@ -765,7 +811,8 @@ __device__ inline bool implicit_r(const Params& P, const double* q, double& nh1,
 
 // settling_velocity (Schiller-Naumann): src/constitutive_2d.f90:5326-5404
 __device__ inline double settling_velocity(const Params& P, double diam, double rhos,
-                                           double rhoc, double ikv, const ddm::dd* tab) {
+                                           double rhoc, double ikv, const ddm::dd* tab,
+                                           PhaseClk* pc = nullptr) {
   double inv_sqrt_C_D = 1.0;
   if (rhos <= rhoc) return 0.0;
   double const_part = sqrt(1.3333333333333333 * (rhos / rhoc - 1.0) * diam * P.grav);
@@ -774,6 +821,7 @@ __device__ inline double settling_velocity(const Params& P, double diam, double 
   if (Rey <= 1000.0) {
 #pragma unroll 1
     for (int i = 0; i < 20; i++) {
+      pc_add(pc, PC_SETTLE_ITS, 1ULL);
       double old = sv;
       inv_sqrt_C_D = sqrt(Rey / (24.0 * (1.0 + 0.15 * pow_cr(Rey, 0.687, tab))));
       sv = const_part * inv_sqrt_C_D;
@@ -795,7 +843,8 @@ __device__ inline double settling_velocity(const Params& P, double diam, double 
 template <int NS, int LANES = 1>
 __device__ inline bool mass_exchange(const Params& P, const double* qp, const double* erod,
                                      double dt, double* dep, double* eq, double* ers,
-                                     double& topo, const ddm::dd* tab, int lane = 0) {
+                                     double& topo, const ddm::dd* tab, int lane = 0,
+                                     PhaseClk* pc = nullptr) {
   constexpr int NV = 4 + NS;
   topo = 0.0;
 #pragma unroll
@@ -861,7 +910,9 @@ __device__ inline bool mass_exchange(const Params& P, const double* qp, const do
       const double d = (lane == 0 || !two) ? P.diam_s[i0] : P.diam_s[two ? i0 + 1 : i0];
       const double r = (lane == 0 || !two) ? P.rho_s[i0] : P.rho_s[two ? i0 + 1 : i0];
       double s = 0.0;
-      if (mine && (a > 0.0) && P.settling) s = settling_velocity(P, d, r, m.rho_c, ikv, tab);
+      const unsigned long long c0 = pc_now();
+      if (mine && (a > 0.0) && P.settling) s = settling_velocity(P, d, r, m.rho_c, ikv, tab, pc);
+      pc_add(pc, PC_SETTLE, pc_now() - c0);
       const double o = __shfl_xor(s, 1);
       sv[i0] = (lane == 0) ? s : o;
       if (two) sv[i0 + 1] = (lane == 0) ? o : s;
@@ -872,6 +923,7 @@ __device__ inline bool mass_exchange(const Params& P, const double* qp, const do
   // a lane pair, lane 0 takes the first and lane 1 the second
   const bool ent = P.entrainment && (h > 0.0) && (m.Ri > 0.0);
   double hindered, pent;
+  const unsigned long long c1 = pc_now();
   if constexpr (LANES == 2) {
     const bool mine = (lane == 0) ? any : ent;
     const double px = (lane == 0) ? 1.0 - fmin_f(1.0, at / 0.6) : m.Ri;
@@ -883,6 +935,7 @@ __device__ inline bool mass_exchange(const Params& P, const double* qp, const do
     hindered = any ? pow_cr(1.0 - fmin_f(1.0, at / 0.6), 4.65, tab) : 0.0;
     pent = ent ? pow_cr(m.Ri, 2.4, tab) : 0.0;
   }
+  pc_add(pc, PC_POW, pc_now() - c1);
 #pragma unroll
   for (int i = 0; i < NS; i++) {
     if ((alphas[i] > 0.0) && P.settling) {

@@ -94,6 +94,7 @@ struct oracle_s {
      * states, used only to count reads of states left from an earlier call */
     int call_id;
     int *stL, *stR, *stB, *stT, *stQP;
+    unsigned char *stale_cell;  /* cells next to a stale read that differs (cumulative) */
     /* 0: x**y by the C library's pow, as the reference build calls it;
      * 1: x**y correctly rounded (quad-precision powq rounded once) */
     int pow_mode;
@@ -870,9 +871,13 @@ static void qp_to_qp2(const double *qp, double *qp2)
 /* stale-state instrumentation: a neighbour's primitive state that was not
  * written for this reconstruction call and is not the dry state a fresh
  * evaluation of a dry cell gives */
-static inline void note_stale_qp(oracle_t *o, int c, const double *qpx)
+static inline void note_stale_qp(oracle_t *o, int c, const double *qpx, int reader)
 {
-    if (o->stQP[c] != o->call_id && qpx[(size_t)(o->nv + 2) * c] != 0.0) o->stale++;
+    if (o->stQP[c] != o->call_id && qpx[(size_t)(o->nv + 2) * c] != 0.0) {
+        o->stale++;
+        o->stale_cell[c] = 1;
+        o->stale_cell[reader] = 1;
+    }
 }
 
 /* reconstruction: src/solver_2d.f90:3095-3924 */
@@ -895,10 +900,10 @@ static void reconstruction(oracle_t *o, const double *q_expl, const double *qp_e
         }
         xs[1] = o->x_comp[j];
         ys[1] = o->y_comp[k];
-        if (j > 0) note_stale_qp(o, c - 1, qp_expl);
-        if (j < nx - 1) note_stale_qp(o, c + 1, qp_expl);
-        if (k > 0) note_stale_qp(o, c - nx, qp_expl);
-        if (k < ny - 1) note_stale_qp(o, c + nx, qp_expl);
+        if (j > 0) note_stale_qp(o, c - 1, qp_expl, c);
+        if (j < nx - 1) note_stale_qp(o, c + 1, qp_expl, c);
+        if (k > 0) note_stale_qp(o, c - nx, qp_expl, c);
+        if (k < ny - 1) note_stale_qp(o, c + nx, qp_expl, c);
         for (int i = 0; i < nv; i++) {
             st[1] = qpc[i];
             /* x direction */
@@ -1134,9 +1139,13 @@ static void reconstruction(oracle_t *o, const double *q_expl, const double *qp_e
 }
 
 /* a face side not written by this reconstruction call that is not dry */
-static inline void note_stale_face(oracle_t *o, int stamp, const double *qpf)
+static inline void note_stale_face(oracle_t *o, int stamp, const double *qpf, int ca, int cb)
 {
-    if (stamp != o->call_id && qpf[0] != 0.0) o->stale++;
+    if (stamp != o->call_id && qpf[0] != 0.0) {
+        o->stale++;
+        if (ca >= 0) o->stale_cell[ca] = 1;
+        if (cb >= 0) o->stale_cell[cb] = 1;
+    }
 }
 
 /* eval_speeds: src/solver_2d.f90:3937-4014 */
@@ -1146,8 +1155,9 @@ static void eval_speeds(oracle_t *o)
     for (int l = 0; l < o->nfx; l++) {
         const int j = o->jsx[l], k = o->ksx[l], f = FX(o, j, k);
         double lmin, lmax, rmin, rmax;
-        note_stale_face(o, o->stL[f], o->qpL + (size_t)np * f);
-        note_stale_face(o, o->stR[f], o->qpR + (size_t)np * f);
+        const int xa = (j > 0) ? CELL(o, j - 1, k) : -1, xb = (j < o->nx) ? CELL(o, j, k) : -1;
+        note_stale_face(o, o->stL[f], o->qpL + (size_t)np * f, xa, xb);
+        note_stale_face(o, o->stR[f], o->qpR + (size_t)np * f, xa, xb);
         eval_local_speeds(o, o->qpL + (size_t)np * f, 1.0, 1, &lmin, &lmax);
         eval_local_speeds(o, o->qpR + (size_t)np * f, 1.0, 1, &rmin, &rmax);
         o->axn[f] = fmin_f(fmin_f(lmin, rmin), 0.0);
@@ -1156,8 +1166,9 @@ static void eval_speeds(oracle_t *o)
     for (int l = 0; l < o->nfy; l++) {
         const int j = o->jsy[l], k = o->ksy[l], f = FY(o, j, k);
         double bmin, bmax, tmin, tmax;
-        note_stale_face(o, o->stB[f], o->qpB + (size_t)np * f);
-        note_stale_face(o, o->stT[f], o->qpT + (size_t)np * f);
+        const int ya = (k > 0) ? CELL(o, j, k - 1) : -1, yb = (k < o->ny) ? CELL(o, j, k) : -1;
+        note_stale_face(o, o->stB[f], o->qpB + (size_t)np * f, ya, yb);
+        note_stale_face(o, o->stT[f], o->qpT + (size_t)np * f, ya, yb);
         eval_local_speeds(o, o->qpB + (size_t)np * f, 1.0, 2, &bmin, &bmax);
         eval_local_speeds(o, o->qpT + (size_t)np * f, 1.0, 2, &tmin, &tmax);
         o->ayn[f] = fmin_f(fmin_f(bmin, tmin), 0.0);
@@ -1917,6 +1928,7 @@ oracle_t *oracle_create(const int *ip, const double *rp)
     o->stL = zalloc(sizeof(int) * NFX); o->stR = zalloc(sizeof(int) * NFX);
     o->stB = zalloc(sizeof(int) * NFY); o->stT = zalloc(sizeof(int) * NFY);
     o->stQP = zalloc(sizeof(int) * N);
+    o->stale_cell = zalloc(N);
     /* boundary ring of the mask (src/solver_2d.f90:351-354) */
     for (int j = 0; j < nx; j++) { o->mask[CELL(o, j, 0)] = 1; o->mask[CELL(o, j, ny - 1)] = 1; }
     for (int k = 0; k < ny; k++) { o->mask[CELL(o, 0, k)] = 1; o->mask[CELL(o, nx - 1, k)] = 1; }
@@ -1936,7 +1948,7 @@ void oracle_destroy(oracle_t *o)
                     o->qT, o->qpL, o->qpR, o->qpB, o->qpT, o->axn, o->axp, o->ayn, o->ayp,
                     o->axmax, o->aymax, o->Hx, o->Hy, o->mask, o->mtmp, o->mx, o->my, o->jc,
                     o->kc, o->jsx, o->ksx, o->jsy, o->ksy, o->hmax, o->pdynmax, o->modvelmax,
-                    o->deposit, o->vuln, o->stL, o->stR, o->stB, o->stT, o->stQP,
+                    o->deposit, o->vuln, o->stL, o->stR, o->stB, o->stT, o->stQP, o->stale_cell,
                     o->erodible, o->erosion, o->bcent};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
     free(o);
@@ -2073,6 +2085,14 @@ void oracle_work_stats(long *out)
                         g_wk.hist[2], g_wk.hist[3], g_wk.hist[4], g_wk.hist[5], g_wk.hist[6],
                         g_wk.hist[7]};
     memcpy(out, v, sizeof v);
+}
+
+/* 1 for every cell next to a stale read that differs from a fresh
+ * evaluation since the handle was created (the cells whose results the
+ * reference's stale-state behaviour can change) */
+void oracle_get_stale_cells(const oracle_t *o, unsigned char *out)
+{
+    memcpy(out, o->stale_cell, (size_t)o->nx * o->ny);
 }
 
 void oracle_get_counters(const oracle_t *o, long *dried, long *stale)

@@ -40,7 +40,7 @@ def _lib():
         lib.oracle_create.argtypes = [_I, _D]
         for name in ("oracle_destroy", "oracle_set_static", "oracle_init_state",
                      "oracle_run", "oracle_get_q", "oracle_set_q", "oracle_get_fields",
-                     "oracle_get_vuln", "oracle_get_mask", "oracle_get_counters",
+                     "oracle_get_vuln", "oracle_get_mask", "oracle_get_counters", "oracle_get_stale_cells",
                      "oracle_set_time", "oracle_watch_cell",
                      "oracle_qc_to_qp", "oracle_qp_to_qc", "oracle_mixt_var",
                      "oracle_fluxes", "oracle_speeds", "oracle_implicit",
@@ -270,6 +270,14 @@ class Oracle:
         seen = self.lib.oracle_watched_faces(self.h, _dp(pre), _dp(post))
         nv = self.nv
         return pre.reshape(2, 8)[:, :nv], post.reshape(2, 8)[:, :nv], seen
+
+    def stale_cells(self):
+        """[k, j] bool: cells next to a stale read that differed from a fresh
+        evaluation since creation."""
+        nx, ny = self.cfg.nx, self.cfg.ny
+        out = np.zeros(nx * ny, np.uint8)
+        self.lib.oracle_get_stale_cells(self.h, out.ctypes.data_as(ctypes.c_void_p))
+        return out.reshape(ny, nx).astype(bool)
 
     def counters(self):
         d = ctypes.c_long()

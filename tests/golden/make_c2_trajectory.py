@@ -11,6 +11,8 @@ side; each takes about half an hour on one to four host cores):
     python tests/golden/make_c2_trajectory.py oracle [--workdir DIR]
     python tests/golden/make_c2_trajectory.py ref    [--workdir DIR]
     python tests/golden/make_c2_trajectory.py merge  [--workdir DIR]
+    python tests/golden/make_c2_trajectory.py oracle-merge [--workdir DIR]  (a new oracle
+        run into the committed fixture, keeping its reference fields)
     python tests/golden/make_c2_trajectory.py cause-cr   [--workdir DIR]   (after merge)
     python tests/golden/make_c2_trajectory.py cause-libm [--workdir DIR]
     python tests/golden/make_c2_trajectory.py cause-merge [--workdir DIR]
@@ -106,6 +108,13 @@ def run_oracle(wd, steps):
             q = o.q()
             dep = np.ascontiguousarray(np.transpose(o.fields()["deposit"], (1, 2, 0)))
             ck[s + 1] = _ckpt(q, dep)
+            # the reference's stale-state reads that differ from a fresh
+            # evaluation so far, the cells next to them (the device evaluates
+            # fresh), and the hash of the deposit outside those cells
+            stale = o.stale_cells()
+            ck[s + 1]["stale_reads"] = int(o.counters()[1])
+            ck[s + 1]["stale_cells"] = np.flatnonzero(stale.reshape(-1)).tolist()
+            ck[s + 1]["dep_fresh_sha"] = _digest(np.where(stale[:, :, None], 0.0, dep))
             if (s + 1) in FULL_STATES:
                 np.save(os.path.join(wd, f"oracle_q_{s + 1}.npy"), q)
             print(f"oracle step {s + 1}: {time.time() - t0:.0f} s, err bits so far "
@@ -217,6 +226,13 @@ def merge(wd):
         ck_q_sha=np.array([cko[str(s)]["q_sha"] for s in CHECKPOINTS]),
         ck_dep_sha=np.array([cko[str(s)]["dep_sha"] for s in CHECKPOINTS]),
         ck_n_wet=np.array([cko[str(s)]["n_wet"] for s in CHECKPOINTS], np.int64),
+        ck_stale_reads=np.array([cko[str(s)].get("stale_reads", -1) for s in CHECKPOINTS],
+                                np.int64),
+        ck_dep_fresh_sha=np.array([cko[str(s)].get("dep_fresh_sha", "") for s in CHECKPOINTS]),
+        ck_stale_cells=np.concatenate([np.asarray(cko[str(s)].get("stale_cells", []), np.int64)
+                                       for s in CHECKPOINTS] + [np.zeros(0, np.int64)]),
+        ck_stale_cells_n=np.array([len(cko[str(s)].get("stale_cells", [])) for s in CHECKPOINTS],
+                                  np.int64),
         ck_sum_q1=np.array([cko[str(s)]["sum_q1"] for s in CHECKPOINTS]),
         ref_dts=dts_r, ref_ncells=r["ncells"],
         ref_nan=np.array(ev, np.int32).reshape(-1, 3),
@@ -231,12 +247,37 @@ def merge(wd):
                           dt_max_rel=dt_rel), indent=1))
 
 
-def run_cause(wd, mode):
-    """Both oracle modes up to each NaN event of the correctly rounded run:
-    the watched faces' fluxes at the event steps and the state before."""
+def merge_oracle(wd):
+    """A new oracle run into the committed fixture, keeping its reference
+    fields: the run must repeat the committed trajectory (dt, error bits,
+    active cells, q at every checkpoint); its checkpoints add the stale-read
+    records."""
+    o = np.load(os.path.join(wd, "oracle.npz"), allow_pickle=False)
+    g = dict(np.load(OUT, allow_pickle=False))
+    cko = json.loads(str(o["ck"]))
+    assert np.array_equal(o["dts"], g["dts"]) and np.array_equal(o["ncells"], g["ncells"])
+    assert np.array_equal(o["err"].astype(np.uint8), g["err"])
+    assert [cko[str(s)]["q_sha"] for s in CHECKPOINTS] == [str(x) for x in g["ck_q_sha"]]
+    assert [cko[str(s)]["dep_sha"] for s in CHECKPOINTS] == [str(x) for x in g["ck_dep_sha"]]
+    g["ck_stale_reads"] = np.array([cko[str(s)]["stale_reads"] for s in CHECKPOINTS], np.int64)
+    g["ck_dep_fresh_sha"] = np.array([cko[str(s)]["dep_fresh_sha"] for s in CHECKPOINTS])
+    g["ck_stale_cells"] = np.concatenate([np.asarray(cko[str(s)]["stale_cells"], np.int64)
+                                          for s in CHECKPOINTS] + [np.zeros(0, np.int64)])
+    g["ck_stale_cells_n"] = np.array([len(cko[str(s)]["stale_cells"]) for s in CHECKPOINTS],
+                                     np.int64)
+    np.savez_compressed(OUT, **g)
+    print(json.dumps({str(s): [cko[str(s)]["stale_reads"], len(cko[str(s)]["stale_cells"])]
+                      for s in CHECKPOINTS}))
+
+
+def run_cause(wd, mode, max_step=None):
+    """Both oracle modes up to each NaN event of the correctly rounded run
+    (the events up to max_step): the watched faces' fluxes at the event steps
+    and the state before."""
     from oracle import oraclepy
     g = np.load(OUT, allow_pickle=False)
-    events = [(int(a), int(b), int(c)) for a, b, c in zip(g["nan_step"], g["nan_j"], g["nan_k"])]
+    events = [(int(a), int(b), int(c)) for a, b, c in zip(g["nan_step"], g["nan_j"], g["nan_k"])
+              if max_step is None or int(a) <= max_step]
     cfg, B, st, q0 = _setup()
     o = oraclepy.Oracle(cfg, st, q0)
     o.set_pow_mode(1 if mode == "cr" else 0)
@@ -290,8 +331,10 @@ def merge_cause(wd):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("mode", choices=["oracle", "ref", "ref-child", "merge", "cause-cr",
-                                     "cause-libm", "cause-merge"])
+    ap.add_argument("mode", choices=["oracle", "ref", "ref-child", "merge", "oracle-merge",
+                                     "cause-cr", "cause-libm", "cause-merge"])
+    ap.add_argument("--max-step", type=int, default=None,
+                    help="cause-*: only the NaN events up to this step")
     ap.add_argument("--workdir", default="/tmp/imex_c2_trajectory")
     ap.add_argument("--steps", type=int, default=STEPS)
     a = ap.parse_args()
@@ -304,10 +347,12 @@ def main():
         run_ref_child(a.workdir, a.steps)
     elif a.mode == "merge":
         merge(a.workdir)
+    elif a.mode == "oracle-merge":
+        merge_oracle(a.workdir)
     elif a.mode == "cause-cr":
-        run_cause(a.workdir, "cr")
+        run_cause(a.workdir, "cr", a.max_step)
     elif a.mode == "cause-libm":
-        run_cause(a.workdir, "libm")
+        run_cause(a.workdir, "libm", a.max_step)
     else:
         merge_cause(a.workdir)
 
