@@ -144,11 +144,12 @@ def cpu_baseline(workload: str, state: dict, seconds: float, threads: int = 0,
         env.setdefault("OMP_STACKSIZE", "1G")
         cmd = [sys.executable, "-m", "oracle.ref_timing", "--workload", workload,
                "--state", path, "--seconds", str(seconds), "--max-steps", str(max_steps)]
+        # the child's progress lines go to this process's stderr
         r = subprocess.run(cmd, cwd=HERE, env=env, stdin=subprocess.DEVNULL,
-                           capture_output=True, text=True, timeout=3000)
+                           stdout=subprocess.PIPE, stderr=None, text=True, timeout=3000)
         if r.returncode != 0:
-            print(f"cpu_baseline: reference run failed ({r.returncode}): {r.stderr[-2000:]}",
-                  file=sys.stderr, flush=True)
+            print(f"cpu_baseline: reference run failed ({r.returncode})", file=sys.stderr,
+                  flush=True)
             return None
         line = [ln for ln in r.stdout.splitlines() if ln.startswith("{")][-1]
         res = json.loads(line)
@@ -171,7 +172,9 @@ def cpu_baseline(workload: str, state: dict, seconds: float, threads: int = 0,
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5000)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed driver steps (default: the workload's whole run less the warmup; "
+                         "c3: 3000)")
     ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--workload", default="c2",
                     help="c2 (default), c1, c2-small, c3, c4 (the batched ensemble), c5 "
@@ -208,6 +211,9 @@ def main():
               "(the multi-GPU decomposition is a design, not part of this build)",
               file=sys.stderr)
         sys.exit(2)
+    args.steps_given = args.steps is not None
+    if args.steps is None:
+        args.steps = max(1, RUN_STEPS.get(args.workload, 5000) - args.warmup)
     if args.steps < 1 or args.warmup < 0:
         sys.exit("--steps must be >= 1 and --warmup >= 0")
 
@@ -542,7 +548,7 @@ def bench_c3(args):
     form's cell-stage updates/s, beside the FP64 rate and the relative
     differences of runout area, maximum thickness, deposit and mass."""
     from imex_sflow2d_dev_amd import init, precision, workloads
-    steps = args.steps if args.steps != 5000 else 3000
+    steps = args.steps if args.steps_given else 3000
     say = lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True)  # noqa: E731
     cfg, B = workloads.config3()
     static, q0 = init.setup(cfg, B)

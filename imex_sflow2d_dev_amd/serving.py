@@ -138,17 +138,21 @@ def serve(reqs: List[Request], budget_bytes: float = 200e9, chunk: int = 20, log
     return wall
 
 
-def fifo_service_times(reqs: List[Request]):
+def fifo_service_times(reqs: List[Request], log=None):
     """Each request alone through the single-run device driver, to the step
-    count the batch took for it (the same run); returns service seconds."""
+    count the batch took for it (the same run); returns service seconds.
+    Reports its progress every 50 requests through log."""
+    say = log or (lambda *a: None)
     out = []
-    for r in reqs:
+    for i, r in enumerate(reqs):
         s = Solver(r.cfg, r.static, r.q0)
         s.run_steps(0)
         t1 = time.perf_counter()
         s.run_steps(int(r.steps), t_end=r.t_end, record_dt=False)
         out.append(time.perf_counter() - t1)
         s.close()
+        if (i + 1) % 50 == 0 or i + 1 == len(reqs):
+            say(f"FIFO service times: {i + 1} of {len(reqs)} requests, {sum(out):.1f} s")
     return np.array(out)
 
 
@@ -171,7 +175,7 @@ def bench_serving(rate: float, duration: float, t_end=(60.0, 600.0), side=(64, 2
     arr = np.array([r.arrival for r in reqs])
     ttr = np.array([r.finish - r.arrival for r in reqs])
     work = float(sum(r.cells_stage_updates for r in reqs))
-    svc = fifo_service_times(reqs)
+    svc = fifo_service_times(reqs, log=say)
     cf = fifo_completion(arr, svc)
     ttr_f = cf - arr
     wall_f = float(cf.max()) if len(reqs) else 0.0

@@ -44,12 +44,17 @@ def _work(args, out):
     steps = 0
     cells = 0
     elapsed = 0.0
+    last = 0.0
     while steps < args.max_steps and elapsed < args.seconds:
         t0 = time.perf_counter()
         _, nc = run.run(1)
         elapsed += time.perf_counter() - t0
         steps += 1
         cells += int(nc[0])
+        if elapsed - last >= 15.0:   # progress of long samples, on stderr
+            last = elapsed
+            sys.stderr.write(f"[ref] {steps} steps, {elapsed:.0f} s\n")
+            sys.stderr.flush()
     run.close()
     out.update(steps=steps, elapsed_s=elapsed, active_cells=cells, setup_s=t_setup,
                threads=int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
