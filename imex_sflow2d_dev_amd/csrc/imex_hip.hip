@@ -1048,10 +1048,11 @@ __device__ __forceinline__ double driver_dt(const Params& P, const DevState* st)
 // ===========================================================================
 template <int NS>
 __device__ void eval_f(const Params& P, double xi, double dt, const double* qj, const double* q0,
-                       const double* Rni, double* f, double& scal) {
+                       const double* Rni, double* f, double& scal, PhaseClk* pc = nullptr) {
   constexpr int NV = 4 + NS;
   double n1, n2;
   if (!implicit_r<NS>(P, qj, n1, n2, xi)) {
+    pc_add(pc, PC_CPLX, 1ULL);
     cx qc[NV];
 #pragma unroll
     for (int i = 0; i < NV; i++) qc[i] = C(qj[i], 0.0);
@@ -1193,7 +1194,7 @@ __device__ bool lnsrch(const Params& P, double xi, double dt, const double* xold
     for (int i = 0; i < NV; i++) xt[i] = xold[i] + a * p[i];
 #pragma unroll
     for (int i = 0; i < NV; i++) qj[i] = xt[i] * qorg[i];
-    eval_f<NS>(P, xi, dt, qj, q0, Rni, rtt, ft);
+    eval_f<NS>(P, xi, dt, qj, q0, Rni, rtt, ft, pc);
     double next = base;
     int decided = -1;
     bool accept = false;
@@ -1260,7 +1261,7 @@ __device__ int solve_rk_step(const Params& P, double xi, double dt, double* qj, 
   int it;
   bool fresh = false;  // rt and scal_f already hold the residual at qj
   for (it = 1; it <= 100; it++) {
-    if (!fresh) eval_f<NS>(P, xi, dt, qj, q0, Rni, rt, scal_f);
+    if (!fresh) eval_f<NS>(P, xi, dt, qj, q0, Rni, rt, scal_f, pc);
     double mx = 0.0;
 #pragma unroll
     for (int i = 0; i < NV; i++) mx = fmax_f(mx, fabs(rt[i]));
@@ -1325,7 +1326,7 @@ __device__ int solve_rk_step(const Params& P, double xi, double dt, double* qj, 
       for (int i = 0; i < NV; i++) qrel[i] = xold[i] + p[i];
 #pragma unroll
       for (int i = 0; i < NV; i++) qj[i] = qrel[i] * qorg[i];
-      eval_f<NS>(P, xi, dt, qj, q0, Rni, rt, scal_f);
+      eval_f<NS>(P, xi, dt, qj, q0, Rni, rt, scal_f, pc);
       fresh = true;
     }
 #pragma unroll

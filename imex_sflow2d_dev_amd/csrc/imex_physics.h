@@ -76,6 +76,7 @@ enum PhaseSlot {
   PC_SETTLE = 12,    // cycles in the settling-velocity fixed points
   PC_SETTLE_ITS = 13,  // settling fixed-point iterations (this lane's classes)
   PC_POW = 14,       // cycles in the hindered-settling and entrainment powers
+  PC_CPLX = 15,      // residual evaluations that took the complex path
   PC_NSLOT = 16
 };
 struct PhaseClk {

@@ -1,5 +1,6 @@
 """Per-cell phase clocks of the update kernel on the 2048 x 2048
-radial-source run (two solid classes, settling and entrainment, FP64).
+radial-source run (two solid classes, settling and entrainment, FP64; or
+another named workload, --workload).
 
 Runs the phase-clock build of the library (_lib/libimexhip_phases.so, made by
 `python -m imex_sflow2d_dev_amd.build --phase-clocks`: the same kernels with
@@ -27,12 +28,13 @@ from . import build, native
 
 SLOTS = ["start", "stage2", "newton0", "newton1", "stage2_end", "sed0", "sed1", "end",
          "newton_its", "jacobian_cycles", "line_search_cycles", "line_search_rounds",
-         "settling_cycles", "settling_its", "powers_cycles", "unused"]
+         "settling_cycles", "settling_its", "powers_cycles", "complex_residuals"]
 
 
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--steps", default="600,1500,2500,3500,4200")
+    ap.add_argument("--workload", default="c2", help="c2 (default) or c1")
     ap.add_argument("--json", default="", help="also write the summaries to this file")
     args = ap.parse_args()
     steps = sorted(int(x) for x in args.steps.split(","))
@@ -43,7 +45,7 @@ def main():
     from . import init, workloads
     from .solver import Solver
     L = ctypes.CDLL(build.LIB_PHASES)
-    cfg, B = workloads.config2()
+    cfg, B = workloads.by_name(args.workload)
     st, q0 = init.setup(cfg, B)
     s = Solver(cfg, st, q0, raise_on_error=False)
     done, out = 0, {}
@@ -71,13 +73,13 @@ def main():
             "epilogue + stores": b[:, 7] - b[:, 6],
             "whole cell": b[:, 7],
         }
-        cnt = {k: b[:, i] for i, k in enumerate(SLOTS) if 8 <= i <= 14}
+        cnt = {k: b[:, i] for i, k in enumerate(SLOTS) if 8 <= i <= 15}
         row = {"step": target, "work_cells": int(w1 - w0)}
         row["cycles"] = {k: {"p50": float(np.median(v)), "p99": float(np.percentile(v, 99)),
                              "max": int(v.max())} for k, v in seg.items()}
         row["counts"] = {k: {"p50": float(np.median(v)), "max": int(v.max()),
                              "hist": np.bincount(np.minimum(v, 40)).tolist()
-                             if "its" in k or "rounds" in k else None}
+                             if "its" in k or "rounds" in k or "complex" in k else None}
                          for k, v in cnt.items()}
         j = int(np.argmax(b[:, 7]))
         row["slowest_cell"] = {k: int(v[j]) for k, v in list(seg.items()) + list(cnt.items())}

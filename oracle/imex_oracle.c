@@ -84,6 +84,7 @@ struct oracle_s {
     unsigned char *vuln;
     double t, dt, dt_old, dt_old_old;
     long dried, stale;
+    int fresh;              /* 1: states left by an earlier step read as a fresh evaluation */
     int err;
     long nan_cell;          /* first cell (j + nx k) whose assembled state held a NaN, -1 none */
     /* the two x faces of a watched cell: KT fluxes before the solid-flux cap
@@ -881,6 +882,32 @@ static inline void note_stale_qp(oracle_t *o, int c, const double *qpx, int read
 }
 
 /* reconstruction: src/solver_2d.f90:3095-3924 */
+/* the primitive state a fresh evaluation gives a dry cell outside the
+ * source disc: zeros, T = T_ambient */
+static inline void dry_qp(const oracle_t *o, double *qp)
+{
+    for (int i = 0; i < o->nv + 2; i++) qp[i] = 0.0;
+    qp[3] = o->T_amb;
+}
+
+/* Fresh mode: the primitive states of the cells next to the active set that
+ * this call does not evaluate (dry cells outside the set, whose stored state
+ * may be left from a step when they were wet) are set to what a fresh
+ * evaluation gives them, before the active cells' reconstruction reads them. */
+static void fresh_neighbour_qp(oracle_t *o, double *qp_expl)
+{
+    const int nx = o->nx, ny = o->ny, np = o->nv + 2;
+    for (int l = 0; l < o->ncells; l++) {
+        const int j = o->jc[l], k = o->kc[l];
+        const int c = CELL(o, j, k);
+        const int nb[4] = {j > 0 ? c - 1 : -1, j < nx - 1 ? c + 1 : -1, k > 0 ? c - nx : -1,
+                           k < ny - 1 ? c + nx : -1};
+        for (int m = 0; m < 4; m++)
+            if (nb[m] >= 0 && o->stQP[nb[m]] != o->call_id && o->scell[nb[m]] != 1)
+                dry_qp(o, qp_expl + (size_t)np * nb[m]);
+    }
+}
+
 static void reconstruction(oracle_t *o, const double *q_expl, const double *qp_expl)
 {
     const int nx = o->nx, ny = o->ny, nv = o->nv, np = nv + 2;
@@ -889,6 +916,7 @@ static void reconstruction(oracle_t *o, const double *q_expl, const double *qp_e
     (void)q_expl;
     o->call_id++;
     for (int l = 0; l < o->ncells; l++) o->stQP[CELL(o, o->jc[l], o->kc[l])] = o->call_id;
+    if (o->fresh) fresh_neighbour_qp(o, (double *)qp_expl);
     for (int l = 0; l < o->ncells; l++) {
         const int j = o->jc[l], k = o->kc[l];
         const int c = CELL(o, j, k);
@@ -1149,9 +1177,30 @@ static inline void note_stale_face(oracle_t *o, int stamp, const double *qpf, in
 }
 
 /* eval_speeds: src/solver_2d.f90:3937-4014 */
+/* Fresh mode: a face side this call's reconstruction did not write (the side
+ * of a dry cell outside the active set) takes what a fresh evaluation gives
+ * it: the dry cell's reconstruction from dry neighbours, the dry primitive
+ * state and a zero conservative state. */
+static void fresh_face_sides(oracle_t *o)
+{
+    const int nv = o->nv, np = nv + 2;
+    struct { int *st; double *qp, *q; } side[4] = {{o->stL, o->qpL, o->qL}, {o->stR, o->qpR, o->qR},
+                                                   {o->stB, o->qpB, o->qB}, {o->stT, o->qpT, o->qT}};
+    for (int d = 0; d < 4; d++) {
+        const int nf = d < 2 ? o->nfx : o->nfy;
+        for (int l = 0; l < nf; l++) {
+            const int f = d < 2 ? FX(o, o->jsx[l], o->ksx[l]) : FY(o, o->jsy[l], o->ksy[l]);
+            if (side[d].st[f] == o->call_id) continue;
+            dry_qp(o, side[d].qp + (size_t)np * f);
+            for (int i = 0; i < nv; i++) side[d].q[(size_t)nv * f + i] = 0.0;
+        }
+    }
+}
+
 static void eval_speeds(oracle_t *o)
 {
     const int np = o->nv + 2;
+    if (o->fresh) fresh_face_sides(o);
     for (int l = 0; l < o->nfx; l++) {
         const int j = o->jsx[l], k = o->ksx[l], f = FX(o, j, k);
         double lmin, lmax, rmin, rmax;
@@ -2043,6 +2092,11 @@ void oracle_get_mask(const oracle_t *o, int *m)
 }
 
 void oracle_set_pow_mode(oracle_t *o, int mode) { o->pow_mode = mode; }
+
+/* 1: read the states that the reference keeps from an earlier step (for dry
+ * cells that left the active set) as a fresh evaluation gives them, as the
+ * device does; 0 (default): as the reference reads them */
+void oracle_set_fresh(oracle_t *o, int fresh) { o->fresh = fresh; }
 
 void oracle_set_time(oracle_t *o, double t, double dt_old, double dt_old_old)
 {

@@ -59,6 +59,7 @@ double oracle_get_t(const oracle_t *o);
  * the state a fresh evaluation gives (the reference's stale-state quirk) */
 void oracle_get_counters(const oracle_t *o, long *dried, long *stale_reads);
 void oracle_get_stale_cells(const oracle_t *o, unsigned char *out);
+void oracle_set_fresh(oracle_t *o, int fresh);
 /* first cell (j + nx*k) whose assembled state held a NaN in the last
  * oracle_run call (the reference's print-and-pause at
  * src/solver_2d.f90:1382-1397), -1 when none */

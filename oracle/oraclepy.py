@@ -40,7 +40,7 @@ def _lib():
         lib.oracle_create.argtypes = [_I, _D]
         for name in ("oracle_destroy", "oracle_set_static", "oracle_init_state",
                      "oracle_run", "oracle_get_q", "oracle_set_q", "oracle_get_fields",
-                     "oracle_get_vuln", "oracle_get_mask", "oracle_get_counters", "oracle_get_stale_cells",
+                     "oracle_get_vuln", "oracle_get_mask", "oracle_get_counters", "oracle_get_stale_cells", "oracle_set_fresh",
                      "oracle_set_time", "oracle_watch_cell",
                      "oracle_qc_to_qp", "oracle_qp_to_qc", "oracle_mixt_var",
                      "oracle_fluxes", "oracle_speeds", "oracle_implicit",
@@ -198,6 +198,12 @@ class Oracle:
         m = np.zeros((self.cfg.ny, self.cfg.nx), np.int32)
         self.lib.oracle_get_mask(self.h, _ip(m))
         return m
+
+    def set_fresh(self, fresh: bool):
+        """True: states the reference keeps from an earlier step (dry cells
+        that left the active set) are read as a fresh evaluation gives them,
+        as on the device."""
+        self.lib.oracle_set_fresh(self.h, 1 if fresh else 0)
 
     def set_pow_mode(self, mode: int):
         """0: the C library's pow (the reference's); 1: correctly rounded."""

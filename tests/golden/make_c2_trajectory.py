@@ -13,6 +13,10 @@ side; each takes about half an hour on one to four host cores):
     python tests/golden/make_c2_trajectory.py merge  [--workdir DIR]
     python tests/golden/make_c2_trajectory.py oracle-merge [--workdir DIR]  (a new oracle
         run into the committed fixture, keeping its reference fields)
+    python tests/golden/make_c2_trajectory.py oracle-fresh [--workdir DIR]
+    python tests/golden/make_c2_trajectory.py fresh-merge [--workdir DIR]  (the run in the
+        oracle's fresh mode, as the device evaluates the states the reference keeps
+        from an earlier step, as the fixture's fresh_* fields)
     python tests/golden/make_c2_trajectory.py cause-cr   [--workdir DIR]   (after merge)
     python tests/golden/make_c2_trajectory.py cause-libm [--workdir DIR]
     python tests/golden/make_c2_trajectory.py cause-merge [--workdir DIR]
@@ -81,11 +85,12 @@ def _ckpt(q, dep):
                 sum_q1=math.fsum(q[:, :, 0][wet].tolist()))
 
 
-def run_oracle(wd, steps):
+def run_oracle(wd, steps, fresh=False):
     from oracle import oraclepy
     cfg, B, st, q0 = _setup()
     o = oraclepy.Oracle(cfg, st, q0)
     o.set_pow_mode(1)
+    o.set_fresh(fresh)
     dts = np.zeros(steps)
     nc = np.zeros(steps, np.int32)
     err = np.zeros(steps, np.int32)
@@ -120,7 +125,8 @@ def run_oracle(wd, steps):
             print(f"oracle step {s + 1}: {time.time() - t0:.0f} s, err bits so far "
                   f"{int(np.bitwise_or.reduce(err[:s + 1]))}, nan events {len(nan_step)}",
                   flush=True)
-    np.savez(os.path.join(wd, "oracle.npz"), dts=dts, ncells=nc, err=err,
+    np.savez(os.path.join(wd, "oracle_fresh.npz" if fresh else "oracle.npz"), dts=dts,
+             ncells=nc, err=err,
              nan_step=np.array(nan_step, np.int32), nan_j=np.array(nan_j, np.int32),
              nan_k=np.array(nan_k, np.int32), work=work,
              ck=json.dumps({str(k): v for k, v in ck.items()}))
@@ -270,6 +276,37 @@ def merge_oracle(wd):
                       for s in CHECKPOINTS}))
 
 
+def merge_fresh(wd):
+    """The oracle run in fresh mode (states the reference keeps from an
+    earlier step read as a fresh evaluation gives them, as the device does)
+    into the fixture as the fresh_* fields: it must repeat the committed
+    trajectory wherever the committed run made no stale read that differed
+    from a fresh evaluation."""
+    o = np.load(os.path.join(wd, "oracle_fresh.npz"), allow_pickle=False)
+    g = dict(np.load(OUT, allow_pickle=False))
+    cko = json.loads(str(o["ck"]))
+    clean = [i for i, s in enumerate(CHECKPOINTS) if int(g["ck_stale_reads"][i]) == 0]
+    last_clean = CHECKPOINTS[clean[-1]]
+    assert np.array_equal(o["dts"][:last_clean], g["dts"][:last_clean])
+    for i in clean:
+        assert cko[str(CHECKPOINTS[i])]["q_sha"] == str(g["ck_q_sha"][i])
+        assert cko[str(CHECKPOINTS[i])]["dep_sha"] == str(g["ck_dep_sha"][i])
+    g["fresh_dts"] = o["dts"]
+    g["fresh_ncells"] = o["ncells"]
+    g["fresh_err"] = o["err"].astype(np.uint8)
+    g["fresh_nan_step"] = o["nan_step"]
+    g["fresh_ck_q_sha"] = np.array([cko[str(s)]["q_sha"] for s in CHECKPOINTS])
+    g["fresh_ck_dep_sha"] = np.array([cko[str(s)]["dep_sha"] for s in CHECKPOINTS])
+    g["fresh_ck_n_wet"] = np.array([cko[str(s)]["n_wet"] for s in CHECKPOINTS], np.int64)
+    g["fresh_ck_stale_reads"] = np.array([cko[str(s)]["stale_reads"] for s in CHECKPOINTS],
+                                         np.int64)
+    np.savez_compressed(OUT, **g)
+    d = np.nonzero(o["dts"] != g["dts"])[0]
+    print(json.dumps(dict(first_dt_difference=int(d[0]) + 1 if len(d) else None,
+                          fresh_nan_steps=o["nan_step"].tolist(),
+                          fresh_n_wet=g["fresh_ck_n_wet"].tolist())))
+
+
 def run_cause(wd, mode, max_step=None):
     """Both oracle modes up to each NaN event of the correctly rounded run
     (the events up to max_step): the watched faces' fluxes at the event steps
@@ -331,8 +368,9 @@ def merge_cause(wd):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("mode", choices=["oracle", "ref", "ref-child", "merge", "oracle-merge",
-                                     "cause-cr", "cause-libm", "cause-merge"])
+    ap.add_argument("mode", choices=["oracle", "oracle-fresh", "ref", "ref-child", "merge",
+                                     "oracle-merge", "fresh-merge", "cause-cr", "cause-libm",
+                                     "cause-merge"])
     ap.add_argument("--max-step", type=int, default=None,
                     help="cause-*: only the NaN events up to this step")
     ap.add_argument("--workdir", default="/tmp/imex_c2_trajectory")
@@ -341,6 +379,10 @@ def main():
     os.makedirs(a.workdir, exist_ok=True)
     if a.mode == "oracle":
         run_oracle(a.workdir, a.steps)
+    elif a.mode == "oracle-fresh":
+        run_oracle(a.workdir, a.steps, fresh=True)
+    elif a.mode == "fresh-merge":
+        merge_fresh(a.workdir)
     elif a.mode == "ref":
         run_ref(a.workdir, a.steps)
     elif a.mode == "ref-child":
