@@ -199,6 +199,9 @@ def main():
     ap.add_argument("--rates", default="2,8,32", help="c5: arrival rates per second")
     ap.add_argument("--window", type=float, default=20.0, help="c5: arrival window (wall s)")
     ap.add_argument("--serve-t-end", default="60,600", help="c5: t_end range of a request (s)")
+    ap.add_argument("--serve-classes", type=int, default=1,
+                    help="c5: solid classes of every request (1: the ensemble members' physics; "
+                         "2 or 3 with settling)")
     ap.add_argument("--out-t-end", type=float, default=200.0,
                     help="c2-outputs: physical end time (s)")
     ap.add_argument("--out-dt", type=float, default=100.0,
@@ -592,7 +595,7 @@ def bench_c5(args):
     t_end = tuple(float(x) for x in args.serve_t_end.split(","))
     res = []
     for rate in [float(x) for x in args.rates.split(",")]:
-        r = serving.bench_serving(rate, args.window, t_end=t_end,
+        r = serving.bench_serving(rate, args.window, t_end=t_end, n_solid=args.serve_classes,
                                   log=lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True))
         print(f"[bench] rate {rate}/s: {json.dumps(r)}", file=sys.stderr, flush=True)
         res.append(r)
@@ -612,7 +615,7 @@ def bench_c5(args):
         "data": "synthetic (seeded PCG64 request stream, DEMs and parameters, generated in-process)",
         "config": {"workload": (f"c5: Poisson arrivals at {args.rates} per s over {args.window} s, sides "
                                 f"log-uniform in 64..2048 cells (dx 10 m), t_end U[{args.serve_t_end}] s, "
-                                f"200 GB admission budget, FP64"),
+                                f"200 GB admission budget, {args.serve_classes} solid class(es), FP64"),
                    "rates": res},
     }
     print(json.dumps(out), flush=True)

@@ -50,7 +50,7 @@ class Request:
 
 
 def make_requests(rate: float, duration: float, seed: int = 5, t_end=(60.0, 600.0),
-                  side=(64, 2048)) -> List[Request]:
+                  side=(64, 2048), n_solid: int = 1) -> List[Request]:
     """Poisson arrivals over `duration` seconds; inputs built on the host
     (the request's payload) before any clock starts."""
     rng = np.random.Generator(np.random.PCG64(5000 + seed))
@@ -61,7 +61,7 @@ def make_requests(rate: float, duration: float, seed: int = 5, t_end=(60.0, 600.
             break
         m = 100000 * seed + len(out)
         cfg, B = workloads.config4_member(m, min_side=side[0], max_side=side[1],
-                                          t_end=float(rng.uniform(*t_end)))
+                                          t_end=float(rng.uniform(*t_end)), n_solid=n_solid)
         st, q0 = init.setup(cfg, B)
         out.append(Request(len(out), t, cfg, st, q0))
     return out
@@ -81,7 +81,8 @@ def serve(reqs: List[Request], budget_bytes: float = 200e9, chunk: int = 20, log
     largest bytes per cell seen so far; once created, the run is accounted at
     its handle's real device footprint."""
     say = log or (lambda *a: None)
-    b = Batch(n_solid=1)
+    # every request of a stream has the stream's n_solid (a batch takes one)
+    b = Batch(n_solid=reqs[0].cfg.n_solid if reqs else 1)
     waiting, running = [], []
     pending = sorted(reqs, key=lambda r: r.arrival)
     used = 0.0
@@ -166,10 +167,10 @@ def fifo_completion(arrivals, service):
 
 
 def bench_serving(rate: float, duration: float, t_end=(60.0, 600.0), side=(64, 2048),
-                  budget_bytes: float = 200e9, seed: int = 5, log=None):
+                  budget_bytes: float = 200e9, seed: int = 5, log=None, n_solid: int = 1):
     say = log or (lambda *a: None)
     t = time.perf_counter()
-    reqs = make_requests(rate, duration, seed=seed, t_end=t_end, side=side)
+    reqs = make_requests(rate, duration, seed=seed, t_end=t_end, side=side, n_solid=n_solid)
     say(f"{len(reqs)} requests generated in {time.perf_counter() - t:.1f} s")
     wall = serve(reqs, budget_bytes=budget_bytes, log=say)
     arr = np.array([r.arrival for r in reqs])

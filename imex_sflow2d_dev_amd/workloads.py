@@ -117,7 +117,7 @@ def config2(nx: int = 2048, ny: int = 2048, dx: float = 5.0, seed_member: int = 
 
 
 def config4_member(member: int, min_side: int = 128, max_side: int = 1024,
-                   t_end: float = 30.0):
+                   t_end: float = 30.0, n_solid: int = 1):
     """One member of the batched ensemble: a grid side log-uniform in
     [min_side, max_side] (dx = 10 m), a seeded fractal DEM (exponent -2.5,
     RMS 10 m) on a 5-degree slope, and per member an initial volume
@@ -146,6 +146,17 @@ def config4_member(member: int, min_side: int = 128, max_side: int = 1024,
     cfg.xi = rng.uniform(200.0, 2000.0)
     cfg.T_collapse = rng.uniform(400.0, 900.0)
     cfg.alphas_collapse = [rng.uniform(0.01, 0.1)]
+    if n_solid > 1:
+        # the particle fraction split over n_solid classes of the radial-source
+        # workload's sizes (1e-3, 1e-4 m; 1e-2 m first for three), with settling
+        diam = {2: [1.0e-3, 1.0e-4], 3: [1.0e-2, 1.0e-3, 1.0e-4]}[n_solid]
+        cfg.n_solid = n_solid
+        cfg.rho_s = [2500.0] * n_solid
+        cfg.diam_s = diam
+        cfg.sp_heat_s = [1200.0] * n_solid
+        cfg.alphas_collapse = [cfg.alphas_collapse[0] / n_solid] * n_solid
+        cfg.alphas_source = [0.0] * n_solid
+        cfg.settling_flag = True
     cfg.t_end = t_end
     cfg.validate()
     return cfg, np.ascontiguousarray(B)

@@ -115,6 +115,23 @@ def test_served_results_equal_runs_alone():
         s.close()
 
 
+def test_served_two_class_requests_equal_runs_alone():
+    """Requests with two solid classes and settling (a batch of n_solid = 2):
+    every served result equals the same request run alone."""
+    from imex_sflow2d_dev_amd import serving
+    reqs = serving.make_requests(rate=40.0, duration=0.25, seed=13, t_end=(2.0, 4.0),
+                                 side=(40, 80), n_solid=2)
+    assert len(reqs) >= 3 and all(r.cfg.n_solid == 2 and r.cfg.settling_flag for r in reqs)
+    budget = 2.5 * max(r.footprint for r in reqs)
+    serving.serve(reqs, budget_bytes=budget, chunk=6, keep=True)
+    for r in reqs:
+        s = Solver(r.cfg, r.static, r.q0)
+        run_alone(s, r.t_end)
+        assert s.t == r.t, r.idx
+        _bits(s.q, r.q, f"request {r.idx}")
+        s.close()
+
+
 def test_serving_admits_an_over_budget_request_alone():
     """Admission accounts each run at its handle's real device footprint
     (imex_get_footprint); a request larger than the whole budget runs alone
