@@ -16,7 +16,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIBDIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(LIBDIR, "libimexhip.so")
-SRCS = [os.path.join(HERE, "csrc", f) for f in ("imex_hip.hip", "imex_output.hip", "imex_writers.cpp")]
+SRCS = [os.path.join(HERE, "csrc", f)
+        for f in ("imex_hip.hip", "imex_output.hip", "imex_pore.hip", "imex_writers.cpp")]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FC = os.environ.get("IMEX_FC", "/opt/rocm/bin/amdflang")
 FDIR = os.path.join(HERE, "fortran")

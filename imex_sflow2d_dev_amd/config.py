@@ -102,6 +102,20 @@ class RunConfig:
     sym_noise: float = 0.0
     noise_pow_val: float = -1.0
     length_spatial_corr: float = -1.0
+    # pore pressure (PORE_PRESSURE_PARAMETERS, src/inpout_2d.f90:389-391,
+    # defaults :536-542; maximum_solid_packing and sphericity_s from
+    # SOLID_TRANSPORT_PARAMETERS, defaults :819, :854): the conserved
+    # variable q1 * p_exc after the solid classes, its implicit decay with
+    # f_inhibit (OFF, STEP, STATIC, DYNAMIC) and the gas loss
+    pore_pressure_flag: bool = False
+    gas_loss_flag: bool = False
+    f_inhibit_mode: str = "OFF"
+    N_inh: int = 2
+    alpha_trans: float = 0.0
+    hydraulic_permeability: float = 0.0
+    dynamic_permeability_flag: bool = False
+    maximum_solid_packing: float = 0.64
+    sphericity_s: List[float] = dataclasses.field(default_factory=lambda: [1.0, 1.0, 1.0])
     # vulnerability table (VULNERABILITY_TABLE_PARAMETERS)
     thickness_levels: List[float] = dataclasses.field(default_factory=list)
     dyn_pres_levels: List[float] = dataclasses.field(default_factory=list)
@@ -112,7 +126,11 @@ class RunConfig:
     # ---- derived quantities ------------------------------------------------
     @property
     def n_vars(self) -> int:
-        return 4 + self.n_solid
+        return 4 + self.n_solid + (1 if self.pore_pressure_flag else 0)
+
+    @property
+    def f_inhibit_code(self) -> int:
+        return ("OFF", "STEP", "STATIC", "DYNAMIC").index(self.f_inhibit_mode)
 
     @property
     def n_eqns(self) -> int:
@@ -181,6 +199,26 @@ class RunConfig:
             raise ValueError("mu(I) rheology needs mu_s, mu_2, I_0 and muI_inf >= 0")
         if self.erosion_coeff < 0.0:
             raise ValueError("erosion_coeff must be >= 0")
+        if self.f_inhibit_mode not in ("OFF", "STEP", "STATIC", "DYNAMIC"):
+            # src/inpout_2d.f90:681-687
+            raise ValueError("f_inhibit_mode must be OFF, STEP, STATIC or DYNAMIC")
+        if not (0.0 < self.maximum_solid_packing < 1.0):
+            raise ValueError("maximum_solid_packing must be in (0, 1)")  # :1989-1995
+        if any(not (0.0 < x <= 1.0) for x in self.sphericity_s[:self.n_solid]):
+            raise ValueError("sphericity_s must be in (0, 1]")  # :1797-1803
+        if self.pore_pressure_flag:
+            if self.f_inhibit_mode in ("STATIC", "DYNAMIC"):
+                if self.N_inh <= 0 or self.alpha_trans <= 0.0:
+                    # src/inpout_2d.f90:698-710
+                    raise ValueError("STATIC/DYNAMIC f_inhibit needs N_inh > 0 and "
+                                     "alpha_trans > 0")
+                if self.maximum_solid_packing < self.alpha_trans:
+                    raise ValueError("maximum_solid_packing must be >= alpha_trans")
+            if self.radial_source_flag:
+                raise ValueError("pore pressure with the radial source is not on this path")
+            if self.erosion_coeff > 0.0 or self.settling_flag or self.entrainment_flag:
+                raise ValueError("pore pressure with settling, entrainment or erosion is "
+                                 "not on this path")
         if self.erosion_coeff > 0.0:
             if not (0.0 <= self.erodible_porosity < 1.0):
                 raise ValueError("erodible_porosity must be in [0, 1)")

@@ -44,11 +44,30 @@ struct imex_handle {
   bool copy_inflight = false;
   double* stage[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   size_t stage_cap[7] = {0, 0, 0, 0, 0, 0, 0};
+  // pore-pressure runs (imex_pore.hip): primitive states, face states,
+  // face fluxes and wave-speed maxima of the whole grid
+  double *pp_qp = nullptr, *pp_qpL = nullptr, *pp_qpR = nullptr, *pp_qpB = nullptr,
+         *pp_qpT = nullptr, *pp_Hx = nullptr, *pp_Hy = nullptr, *pp_amx = nullptr,
+         *pp_amy = nullptr;
   std::string err;
 };
 
 // releases what imex_output.hip created (streams, events); called by imex_destroy
 void imex_output_release(imex_handle* h);
+
+// pore-pressure runs (imex_pore.hip): the buffers of the general path, and
+// the launches of one device-driven step after its active-set pass
+int imex_pore_alloc(imex_handle* h);
+int imex_pore_step(imex_handle* h, int step_idx);
+
+// the device-side driver's dt: the CFL step clipped to t_end and limited to
+// 1.1 * (dt_old + dt_old_old) / 2, src/IMEX_SfloW2D.f90:417-439
+__device__ __forceinline__ double driver_dt(const imex::Params& P, const imex::DevState* st) {
+  double dt = imex::fmin_f(P.max_dt, __longlong_as_double((long long)st->dtmin_bits));
+  const double t = st->t;
+  if (t + dt > st->t_end) dt = st->t_end - t;
+  return imex::fmin_f(dt, 1.1 * 0.5 * (st->dt_old + st->dt_old_old));
+}
 
 static inline int set_err(imex_handle* h, const char* what, hipError_t e) {
   if (h) {

@@ -13,8 +13,7 @@
 #include <string>
 #include <vector>
 
-#include "imex_hip.h"
-#include "imex_kernels.h"
+#include "imex_handle.h"
 
 using namespace imex;
 
@@ -1031,15 +1030,8 @@ __global__ void k_dt(Params P, Grid G) {
   st->dtmin_bits = 0x7ff0000000000000ULL;
 }
 
-// The device-side driver's dt: the CFL step clipped to t_end and limited to
-// 1.1 * (dt_old + dt_old_old) / 2, src/IMEX_SfloW2D.f90:417-439.  Reads only
-// fields no kernel of the step writes before the sedimentation launch.
-__device__ __forceinline__ double driver_dt(const Params& P, const DevState* st) {
-  double dt = fmin_f(P.max_dt, __longlong_as_double((long long)st->dtmin_bits));
-  const double t = st->t;
-  if (t + dt > st->t_end) dt = st->t_end - t;
-  return fmin_f(dt, 1.1 * 0.5 * (st->dt_old + st->dt_old_old));
-}
+// (driver_dt, the device-side driver's dt, is in imex_handle.h; it reads
+// only fields no kernel of the step writes before the sedimentation launch)
 
 // ===========================================================================
 // stage 2 of imex_RK_solver for one cell: src/solver_2d.f90:1103-1338 with
@@ -2092,6 +2084,19 @@ extern "C" int imex_create(const imex_config* cfg, imex_handle** out) {
   if (rheo != 1 && rheo != 11 && rheo != 12) return -3;
   if (rheo != 1 && !(cfg->mu_s >= 0.0 && cfg->mu_2 >= 0.0 && cfg->I_0 >= 0.0 && cfg->muI_inf >= 0.0))
     return -3;
+  // pore pressure (src/inpout_2d.f90:681-710, :1989-2037): the general path
+  // of imex_pore.hip, device-driven steps, without the features it does not
+  // carry
+  if (cfg->pore_pressure_flag) {
+    if (cfg->f_inhibit_mode < 0 || cfg->f_inhibit_mode > 3 || cfg->radial_source_flag ||
+        cfg->settling_flag || cfg->entrainment_flag || cfg->erosion_coeff != 0.0 ||
+        cfg->topo_change_flag || cfg->precision != 0)
+      return -3;
+    const double mp = cfg->maximum_solid_packing > 0.0 ? cfg->maximum_solid_packing : 0.64;
+    if (cfg->f_inhibit_mode >= 2 &&
+        (cfg->N_inh <= 0 || cfg->N_inh > 15 || !(cfg->alpha_trans > 0.0) || mp < cfg->alpha_trans))
+      return -3;
+  }
   imex_handle* h = new imex_handle();
   h->cfg = *cfg;
   h->fp32 = cfg->precision == 1;
@@ -2125,7 +2130,26 @@ extern "C" int imex_create(const imex_config* cfg, imex_handle** out) {
   P.nx = cfg->nx;
   P.ny = cfg->ny;
   P.ns = cfg->n_solid;
-  P.nv = 4 + cfg->n_solid;
+  P.nv = 4 + cfg->n_solid + (cfg->pore_pressure_flag ? 1 : 0);
+  P.pore = cfg->pore_pressure_flag != 0;
+  if (P.pore) {
+    P.gas_loss = cfg->gas_loss_flag != 0;
+    P.finh = cfg->f_inhibit_mode;
+    P.N_inh = cfg->N_inh;
+    P.dyn_perm = cfg->dynamic_permeability_flag != 0;
+    P.hyd_perm = cfg->hydraulic_permeability;
+    P.alpha_trans = cfg->alpha_trans;
+    for (int i = 0; i < 3; i++) P.sph_s[i] = cfg->sphericity_s[i] > 0.0 ? cfg->sphericity_s[i] : 1.0;
+    // precompute_pascal_coefficient (src/constitutive_2d.f90:419-466)
+    for (int n = 0; n <= P.N_inh && n < 16; n++) {
+      const int Nn = P.N_inh;
+      double p1 = 1.0, p2 = 1.0;
+      for (int k = 0; k < n; k++) p1 = p1 * (-(double)Nn - 1.0 - (double)k) / (double)(k + 1);
+      for (int k = 0; k < Nn - n; k++) p2 = p2 * (2.0 * (double)Nn + 1.0 - (double)k) / (double)(k + 1);
+      P.pascal[n] = p1 * p2;
+    }
+  }
+  P.max_pack = cfg->maximum_solid_packing > 0.0 ? cfg->maximum_solid_packing : 0.64;
   P.energy = cfg->energy_flag;
   P.settling = cfg->settling_flag;
   P.entrainment = cfg->entrainment_flag;
@@ -2197,7 +2221,7 @@ extern "C" int imex_create(const imex_config* cfg, imex_handle** out) {
     P.thk[i] = cfg->thickness_levels[i];
     P.pdyn[i] = cfg->dyn_pres_levels[i];
   }
-  P.hcs = (4 + cfg->n_solid) * 2.220446049250313e-16;
+  P.hcs = P.nv * 2.220446049250313e-16;  // n_vars * EPSILON (src/solver_2d.f90:274-275)
   P.one_by_hcs = 1.0 / P.hcs;
 
   Grid& G = h->G;
@@ -2279,6 +2303,7 @@ extern "C" int imex_create(const imex_config* cfg, imex_handle** out) {
     HCHK(h, hipStreamSynchronize(h->stream));
   }
   if ((rc = frame_band(h, nullptr))) return rc;
+  if (P.pore && (rc = imex_pore_alloc(h))) return rc;
   h->lds = lds_for(ns, h->fp32);
   {
     int max_lds = 0;
@@ -2484,14 +2509,25 @@ extern "C" int imex_upload_state(imex_handle* h, const double* q_aos, double t, 
   return 0;
 }
 
+// pore-pressure runs step through imex_run_steps only
+static int device_driven_only(imex_handle* h) {
+  if (h->P.pore) {
+    h->err = "pore_pressure_flag: runs with pore pressure take device-driven steps (imex_run_steps)";
+    return -2;
+  }
+  return 0;
+}
+
 extern "C" int imex_check_solve(imex_handle* h, int solve_all) {
   if (!h) return -2;
+  if (int rc = device_driven_only(h)) return rc;
   HCHK(h, hipSetDevice(h->device));
   return mask(h, solve_all);
 }
 
 extern "C" int imex_timestep(imex_handle* h, double t, double* dt_out) {
   if (!h) return -2;
+  if (int rc = device_driven_only(h)) return rc;
   HCHK(h, hipSetDevice(h->device));
   int rc;
   if ((rc = flux(h, 0, t, 0))) return rc;
@@ -2510,12 +2546,14 @@ static int take_err(imex_handle* h, unsigned* out) {
 
 extern "C" int imex_rk_step(imex_handle* h, double dt) {
   if (!h) return -2;
+  if (int rc = device_driven_only(h)) return rc;
   HCHK(h, hipSetDevice(h->device));
   return update(h, 1, dt, 0);
 }
 
 extern "C" int imex_erosion_deposition(imex_handle* h, double dt) {
   if (!h) return -2;
+  if (int rc = device_driven_only(h)) return rc;
   if (int rc = need_topography(h)) return rc;
   HCHK(h, hipSetDevice(h->device));
   return update(h, 2, dt, 0);
@@ -2653,6 +2691,13 @@ extern "C" int imex_run_steps(imex_handle* h, int nsteps, double t_end, double* 
   }
   int rc;
   for (int n = 0; n < nsteps; n++) {
+    if (h->P.pore) {
+      // the general path: every tile's active set, then imex_pore.hip's launches
+      h->mask_full_next = true;
+      if ((rc = mask(h, 0, 1))) return rc;
+      if ((rc = imex_pore_step(h, n))) return rc;
+      continue;
+    }
     if ((rc = mask(h, 0, 1))) return rc;
     if ((rc = flux(h, 1, 0.0, n))) return rc;
     if ((rc = update(h, 1 | 2 | 4, 0.0, 1, n))) return rc;
@@ -2670,6 +2715,7 @@ extern "C" int imex_run_steps_timed(imex_handle* h, int nsteps, double t_end, do
                                     int sample_every, double* kernel_ms, int* n_sampled,
                                     int64_t* cells) {
   if (!h || nsteps < 0 || sample_every < 1) return -2;
+  if (int rc = device_driven_only(h)) return rc;  // its kernel split is the step kernels'
   if (int rc = need_topography(h)) return rc;
   HCHK(h, hipSetDevice(h->device));
   if (int rc = bind_dts(h, dts, nsteps)) return rc;
@@ -2934,7 +2980,7 @@ extern "C" int imex_batch_set_launch(imex_batch* b, int flux_threads, int graphs
 extern "C" int imex_batch_add(imex_batch* b, imex_handle* h, double t_end) {
   // members step together with no per-member slope pass: no topography change
   if (!b || !h || h->P.ns != b->ns || h->device != b->device || h->fp32 != b->fp32 ||
-      h->P.topo_change)
+      h->P.topo_change || h->P.pore)
     return -2;
   for (imex_handle* x : b->mem)
     if (x == h) return -2;

@@ -548,6 +548,27 @@ __host__ __device__ __forceinline__ bool pow_fast(double x, double y, double& ou
   return true;
 }
 
+// Correctly rounded log, log1p and exp from the double-double routines above
+// (their pairs are within 2**-100 of the exact values, so the rounding of
+// the pair is the correct one unless the exact value lies within about
+// 2**-47 ulp of a rounding boundary).  Used by the pore-pressure terms,
+// where the reference reaches the C library's cpow.
+__host__ __device__ inline double log_cr(double x) { return log_dd(x).hi; }
+__host__ __device__ inline double exp_cr(double x) { return exp_dd(dd{x, 0.0}); }
+// log(1 + u) for u > -1: 1 + u = hi + lo exactly, log(hi) + log1p(lo/hi)
+// with lo/hi as a pair and log1p(e) = e - e**2/2 (|e| < 2**-53)
+__host__ __device__ inline double log1p_cr(double u) {
+  const dd w = two_sum(1.0, u);
+  const dd L = log_dd(w.hi);
+  const dd e = div(dd{w.lo, 0.0}, dd{w.hi, 0.0});
+  const dd t = add(e, dd{-0.5 * (e.hi * e.hi), 0.0});
+  return add(L, t).hi;
+}
+// x*x + y*y - 1 rounded once (the pair sum is exact to 2**-104)
+__host__ __device__ inline double x2y2m1_cr(double x, double y) {
+  return add(add(two_prod(x, x), dd{-1.0, 0.0}), two_prod(y, y)).hi;
+}
+
 }  // namespace ddm
 
 // x**y correctly rounded; the special cases follow C's pow.  tab: the tables

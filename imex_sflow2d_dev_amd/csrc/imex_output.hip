@@ -516,10 +516,21 @@ extern "C" int imex_snapshot_cols(imex_handle* h, unsigned kind) {
   return -2;
 }
 
+// the output kernels are instantiated for the n_solid + 4 variables of runs
+// without pore pressure
+static int no_pore(imex_handle* h, const char* what) {
+  if (h->P.pore) {
+    h->err = std::string(what) + ": not built for runs with pore pressure";
+    return -2;
+  }
+  return 0;
+}
+
 extern "C" int imex_snapshot_begin(imex_handle* h, unsigned what, double* q, double* q2d,
                                    double* dep2d, double* p2d, double* fields, double* ero2d,
                                    double* erd2d) {
   if (!h || (what & ~127u)) return -2;
+  if (int rc = no_pore(h, "imex_snapshot_begin")) return rc;
   double* outs[NKIND] = {q, q2d, dep2d, p2d, fields, ero2d, erd2d};
   for (int b = 0; b < NKIND; b++)
     if ((what & (1u << b)) && !outs[b]) return -2;
@@ -626,6 +637,7 @@ extern "C" int imex_get_topography(imex_handle* h, double* B_cent) {
 extern "C" int imex_runout_reduce(imex_handle* h, double x0_runout, double y0_runout,
                                   double alphas_threshold, unsigned flags, imex_runout_red* out) {
   if (!h || !out) return -2;
+  if (int rc = no_pore(h, "imex_runout_reduce")) return rc;
   if ((flags & IMEX_RUNOUT_B) && !h->G.bcent) {
     h->err = "imex_runout_reduce: IMEX_RUNOUT_B needs imex_set_topography first";
     return -2;
@@ -681,6 +693,7 @@ extern "C" int imex_runout_reduce(imex_handle* h, double x0_runout, double y0_ru
 
 extern "C" int imex_probes(imex_handle* h, int n, const double* xy, double* out) {
   if (!h || n < 0 || (n > 0 && (!xy || !out))) return -2;
+  if (int rc = no_pore(h, "imex_probes")) return rc;
   if (n == 0) return 0;
   HCHK(h, hipSetDevice(h->device));
   const Params& P = h->P;

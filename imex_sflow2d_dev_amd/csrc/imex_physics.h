@@ -48,6 +48,10 @@ struct Params {
   double vel_source, T_source, h_source, time_param[4], pi_g;
   double thk[8], pdyn[8];
   double hcs, one_by_hcs;  // complex step n_vars*eps and its reciprocal
+  // pore pressure (PORE_PRESSURE_PARAMETERS): the conserved variable
+  // q1 * p_exc after the solid classes (kernels instantiated with NX = 1)
+  int pore, gas_loss, finh, N_inh, dyn_perm;  // finh: 0 OFF, 1 STEP, 2 STATIC, 3 DYNAMIC
+  double hyd_perm, alpha_trans, max_pack, sph_s[3], pascal[16];
 };
 
 // Per-cell phase clocks of the update kernel, compiled in only by the
@@ -216,6 +220,61 @@ __device__ inline cx csqrt_c(cx z) {
   return C(r, copysign(s, y));
 }
 
+// Fortran COMPLEX**REAL in the reference build: the C library's cpow(z, c +
+// 0i) = cexp(c * clog(z)), evaluated here with the same steps and correctly
+// rounded functions: clog's real part by the C library's case split (|x| = 1;
+// 1 < |x| < 2 with |y| < 1; 0.5 <= |x| < 1 with |y| < eps/2, or with
+// x**2 + y**2 >= 0.5; otherwise log(hypot)), its imaginary part atan2(y, x),
+// the product with c + 0i as (cL - 0 th, c th + 0 L), and cexp(a + ib) as
+// exp(a) (cos b, sin b) with (cos b, sin b) = (1, b) for |b| <= DBL_MIN.
+// atan2, sin and cos are correctly rounded for the arguments the solver
+// gives them (|y| <= 2**-30 x, x > 0; |b| < 2**-27: y/x, b and 1); others take
+// the device library's functions.
+__device__ inline cx cpow_c(cx z, double c) {
+  double ax = fabs(z.re), ay = fabs(z.im);
+  if (ax < ay) {
+    const double t = ax;
+    ax = ay;
+    ay = t;
+  }
+  double L;
+  if (ax == 1.0) {
+    L = ddm::log1p_cr(ay * ay) / 2.0;
+  } else if (ax > 1.0 && ax < 2.0 && ay < 1.0) {
+    double d2m1 = (ax - 1.0) * (ax + 1.0);
+    if (ay >= kEps) d2m1 += ay * ay;
+    L = ddm::log1p_cr(d2m1) / 2.0;
+  } else if (ax < 1.0 && ax >= 0.5 && ay < kEps / 2.0) {
+    const double d2m1 = (ax - 1.0) * (ax + 1.0);
+    L = ddm::log1p_cr(d2m1) / 2.0;
+  } else if (ax < 1.0 && ax >= 0.5 && ax * ax + ay * ay >= 0.5) {
+    L = ddm::log1p_cr(ddm::x2y2m1_cr(ax, ay)) / 2.0;
+  } else {
+    L = ddm::log_cr(hypot_dd(ax, ay)) - 0.0 * 0x1.62e42fefa39efp-1;
+  }
+  double th;
+  if (z.re > 0.0 && fabs(z.im) <= 0x1.0p-30 * z.re)
+    th = z.im / z.re;
+  else
+    th = atan2(z.im, z.re);
+  const double a = c * L - 0.0 * th, b = c * th + 0.0 * L;
+  double sb, cb;
+  if (fabs(b) > 2.2250738585072014e-308) {
+    if (fabs(b) < 0x1.0p-27) {
+      sb = b;
+      cb = 1.0;
+    } else {
+      sb = sin(b);
+      cb = cos(b);
+    }
+  } else {
+    sb = b;
+    cb = 1.0;
+  }
+  const double E = ddm::exp_cr(a);
+  return C(E * cb, E * sb);
+}
+
 // ===========================================================================
 // constitutive_2d
 // ===========================================================================
@@ -228,13 +287,16 @@ __device__ inline cx csqrt_c(cx z) {
 
 // r_phys_var (configured branch) + qc_to_qp:
 // src/constitutive_2d.f90:550-976 and :1637-1690.  Also returns p_dyn.
-template <int NS, typename T = double>
+// NX: extra conserved variables after the solid classes (1: the pore
+// pressure q1 * p_exc at index 4 + NS; 0 everywhere else).
+template <int NS, typename T = double, int NX = 0>
 __device__ inline void qc_to_qp(const Params& P, const T* qc, T* qp, T& p_dyn) {
-  constexpr int NV = 4 + NS;
+  constexpr int NV = 4 + NS + NX;
   T xs[NS];
-  T inv_qj1;
+  T inv_qj1, pexc = T(0.0);
   if (qc[0] > T(kEps)) {
     inv_qj1 = T(1.0) / qc[0];
+    if constexpr (NX == 1) pexc = qc[4 + NS] * inv_qj1;  // :659
 #pragma unroll
     for (int i = 0; i < NS; i++) xs[i] = qc[4 + i] * inv_qj1;
     T s = T(0.0);
@@ -302,14 +364,15 @@ __device__ inline void qc_to_qp(const Params& P, const T* qc, T* qp, T& p_dyn) {
   qp[3] = Tm;
 #pragma unroll
   for (int i = 0; i < NS; i++) qp[4 + i] = xs[i] * rm * T(P.inv_rho_s[i]) * h;
+  if constexpr (NX == 1) qp[4 + NS] = pexc;  // :1683
   qp[NV] = u;
   qp[NV + 1] = v;
 }
 
 // qp_to_qc (configured branch): src/constitutive_2d.f90:1718-2149
-template <int NS, typename T = double>
+template <int NS, typename T = double, int NX = 0>
 __device__ inline void qp_to_qc(const Params& P, const T* qp, T* qc) {
-  constexpr int NV = 4 + NS;
+  constexpr int NV = 4 + NS + NX;
   T h = qp[0];
   if (!(h > T(kEps))) {
 #pragma unroll
@@ -359,6 +422,7 @@ __device__ inline void qp_to_qc(const Params& P, const T* qp, T* qc) {
   }
 #pragma unroll
   for (int i = 0; i < NS; i++) qc[4 + i] = xs[i] * qc[0];
+  if constexpr (NX == 1) qc[4 + NS] = qp[4 + NS] * qc[0];  // :1981, :2143
 }
 
 template <typename T = double>
@@ -368,9 +432,9 @@ struct MixtT {
 using Mixt = MixtT<double>;
 
 // mixt_var (+ eval_sp_heat when sp_heat): src/constitutive_2d.f90:1342-1607
-template <int NS, typename T = double>
+template <int NS, typename T = double, int NX = 0>
 __device__ inline MixtT<T> mixt_var(const Params& P, const T* qp, bool sp_heat) {
-  constexpr int NV = 4 + NS;
+  constexpr int NV = 4 + NS + NX;
   MixtT<T> m;
   T h = qp[0];
   if (qp[0] <= T(kEps)) {
@@ -423,10 +487,10 @@ __device__ inline MixtT<T> mixt_var(const Params& P, const T* qp, bool sp_heat) 
 }
 
 // eval_local_speeds_x / _y with grav_coeff = 1: src/constitutive_2d.f90:2203-2307
-template <int NS, typename T = double>
+template <int NS, typename T = double, int NX = 0>
 __device__ inline void local_speeds(const Params& P, const T* qp, int dir, T& vmin, T& vmax) {
-  constexpr int NV = 4 + NS;
-  MixtT<T> m = mixt_var<NS, T>(P, qp, false);
+  constexpr int NV = 4 + NS + NX;
+  MixtT<T> m = mixt_var<NS, T, NX>(P, qp, false);
   T h = qp[0];
   T w = (dir == 1) ? qp[NV] : qp[NV + 1];
   T rg = m.red_grav;
@@ -441,13 +505,13 @@ __device__ inline void local_speeds(const Params& P, const T* qp, int dir, T& vm
 }
 
 // eval_fluxes with grav_coeff = 1: src/constitutive_2d.f90:2325-2518
-template <int NS, typename T = double>
+template <int NS, typename T = double, int NX = 0>
 __device__ inline void phys_flux(const Params& P, const T* qc, const T* qp, int dir, T* f) {
-  constexpr int NV = 4 + NS;
+  constexpr int NV = 4 + NS + NX;
   if (qp[0] > T(kEps)) {
     T h = qp[0];
     T w = (dir == 1) ? qp[NV] : qp[NV + 1];
-    MixtT<T> m = mixt_var<NS, T>(P, qp, false);
+    MixtT<T> m = mixt_var<NS, T, NX>(P, qp, false);
     T Pr = T(0.5) * m.rho_m * T(1.0) * m.red_grav * (h * h);
     f[0] = w * qc[0] * T(1.0);
     if (dir == 1) {
@@ -467,6 +531,8 @@ __device__ inline void phys_flux(const Params& P, const T* qc, const T* qp, int 
 #pragma unroll
       for (int i = 0; i < NS; i++) f[4 + i] = f[4 + i] / S * f[0];
     }
+    // flux of q1 * p_exc: :2435-2439, :2497-2501
+    if constexpr (NX == 1) f[4 + NS] = w * qc[4 + NS] * T(1.0);
   } else {
 #pragma unroll
     for (int i = 0; i < NV; i++) f[i] = T(0.0);
@@ -477,16 +543,16 @@ __device__ inline void phys_flux(const Params& P, const T* qc, const T* qp, int 
 // no curvature, d_grav_coeff = 0: src/constitutive_2d.f90:3024-3431.  The
 // source's time factor and inflow state are computed in double and rounded
 // to T.
-template <int NS, typename T = double>
+template <int NS, typename T = double, int NX = 0>
 __device__ inline void expl_terms(const Params& P, T Bx, T By, const T* qp, T* ex, double time,
                                   T arc_p, T arc_x, T arc_y) {
-  constexpr int NV = 4 + NS;
+  constexpr int NV = 4 + NS + NX;
 #pragma unroll
   for (int i = 0; i < NV; i++) ex[i] = T(0.0);
   if ((qp[0] <= T(kEps)) && (arc_p == T(0.0))) return;
   if (qp[0] > T(kEps)) {
     T h = qp[0], u = qp[NV], v = qp[NV + 1];
-    MixtT<T> m = mixt_var<NS, T>(P, qp, true);
+    MixtT<T> m = mixt_var<NS, T, NX>(P, qp, true);
     T tg = m.red_grav + T(0.0);
     ex[1] = -(T(1.0) * m.rho_m * tg * h * Bx) + T(0.5) * m.rho_m * m.red_grav * (h * h) * T(0.0);
     ex[2] = -(T(1.0) * m.rho_m * tg * h * By) + T(0.5) * m.rho_m * m.red_grav * (h * h) * T(0.0);
@@ -522,7 +588,7 @@ __device__ inline void expl_terms(const Params& P, T Bx, T By, const T* qp, T* e
       qs[3] = P.T_source;
 #pragma unroll
       for (int i = 0; i < NS; i++) qs[4 + i] = P.alphas_source[i] * qs[0];
-      Mixt m = mixt_var<NS>(P, qs, true);
+      Mixt m = mixt_var<NS, double, NX>(P, qs, true);
       const T tc = T(t_coeff);
       const T h_dot = T(P.h_source) * T(P.vel_source) * arc_p / (T(P.dx) * T(P.dy));
       const T rmx = T(m.rho_m);
@@ -578,19 +644,20 @@ __device__ inline double mu_of_I(const Params& P, double diam, double rhop, doub
 // eval_nh_semi_impl_terms, grav_coeff = 1: src/constitutive_2d.f90:4027-4547;
 // Voellmy Coulomb part (model 1, :4164-4213) or the mu(I) friction of models
 // 11 and 12 (:4384-4537)
-template <int NS>
+template <int NS, int NX = 0>
 __device__ inline void semi_impl(const Params& P, const double* qp, double* si) {
-  constexpr int NV = 4 + NS;
+  constexpr int NV = 4 + NS + NX;
 #pragma unroll
   for (int i = 0; i < NV; i++) si[i] = 0.0;
   double h = qp[0], u = qp[NV], v = qp[NV + 1];
-  Mixt m = mixt_var<NS>(P, qp, false);
+  Mixt m = mixt_var<NS, double, NX>(P, qp, false);
   double w = 0.0;
   double mod_vel = sqrt(u * u + v * v + w * w);
   if (P.rheology == 1) {
     if (mod_vel > 0.0) {
       double tt = P.mu * m.rho_m * (m.red_grav + 0.0) * h * sqrt(1.0);
       tt = fmax_f(0.0, tt);
+      if constexpr (NX == 1) tt = fmax_f(0.0, tt - P.mu * sqrt(1.0) * qp[4 + NS]);  // :4191-4198
       tt = fmax_f(0.0, tt);
       si[1] = si[1] - tt * u / mod_vel;
       si[2] = si[2] - tt * v / mod_vel;
@@ -608,6 +675,7 @@ __device__ inline void semi_impl(const Params& P, const double* qp, double* si) 
     }
     if (h < kEps) h = kEps;
     double vse = m.rho_m * (m.red_grav + 0.0) * h;
+    if constexpr (NX == 1) vse = m.rho_m * (m.red_grav + 0.0) * h - qp[4 + NS];  // :4423-4427
     vse = fmax_f(vse, 0.0);
     double ens = fmax_f(0.0, vse * sqrt(1.0));
     if (ens < kEps) ens = kEps;
@@ -628,18 +696,123 @@ __device__ __forceinline__ double cell_xi(const Params& P, const double* Z, int 
   return x;
 }
 
+// Carman-Kozeny grain size d * psi of the dynamic permeability: the class's
+// diameter and sphericity for one class; the Sauter diameter and the
+// surface-area-weighted sphericity for several (src/constitutive_2d.f90:
+// 3853-3868, :4978-4990)
+template <int NS>
+__device__ inline double perm_grain(const Params& P, const double* al) {
+  if (NS > 1) {
+    double s = 0.0, t = 0.0;
+#pragma unroll
+    for (int i = 0; i < NS; i++) s += al[i];
+    double diam;
+    if (s < kEps) {
+#pragma unroll
+      for (int i = 0; i < NS; i++) t += P.diam_s[i];
+      diam = t / (double)NS;
+    } else {
+#pragma unroll
+      for (int i = 0; i < NS; i++) t += al[i] / P.diam_s[i];
+      diam = s / t;
+    }
+    double a = 0.0, b = 0.0;
+#pragma unroll
+    for (int i = 0; i < NS; i++) a += al[i] * (P.diam_s[i] * P.diam_s[i]) * P.sph_s[i];
+#pragma unroll
+    for (int i = 0; i < NS; i++) b += al[i] * (P.diam_s[i] * P.diam_s[i]);
+    return diam * (a / b);
+  }
+  return P.diam_s[0] * P.sph_s[0];
+}
+
+// 10_wp**(-0.28_wp) as the reference build folds it at compile time
+constexpr double kTenPowM028 = 0x1.0cb3903cf06e8p-1;
+
+// 1 - sum_n pascal(n) tt**(N + n + 1), tt = (x - a_t)/(max_pack - a_t)
+// clamped to [0, 1]: f_inhibit's generalised smoothstep (STATIC, DYNAMIC;
+// src/constitutive_2d.f90:3904-3927, :4911-4934)
+__device__ inline double finh_sum(const Params& P, double x, double at) {
+  const double width = P.max_pack - at;
+  double tt = (x - at) / width;
+  if (tt < 0.0) tt = 0.0;
+  if (tt > 1.0) tt = 1.0;
+  double s = 0.0;
+  for (int n = 0; n <= P.N_inh; n++) {
+    const double term = P.pascal[n] * pow_cr(tt, (double)(P.N_inh + n) + 1.0);
+    s = s + term;
+  }
+  return 1.0 - s;
+}
+
+// the pore-pressure row of eval_implicit_terms in complex arithmetic:
+// src/constitutive_2d.f90:3844-3993
+template <int NS>
+__device__ inline cx pore_source_c(const Params& P, cx h, cx T, cx rho_m, const cx* alphas,
+                                   cx exc) {
+  const double h_threshold = 1.0e-10;
+  const double gamma_gas = P.cp_a / (P.cp_a - P.R_a);
+  const cx gas_compr = C(1.0 / (gamma_gas * P.pres), 0.0);
+  const cx rho_gas = cdiv(C(P.pres, 0.0), cmul(C(P.R_a, 0.0), T));
+  cx sa = C(0.0, 0.0);
+#pragma unroll
+  for (int i = 0; i < NS; i++) sa = cadd(sa, alphas[i]);
+  const cx porosity = csub(C(1.0, 0.0), sa);
+  double hyd = P.hyd_perm;
+  if (P.dyn_perm) {
+    double al[NS];
+#pragma unroll
+    for (int i = 0; i < NS; i++) al[i] = alphas[i].re;
+    const double ds = perm_grain<NS>(P, al);
+    const cx num = cmul(cpow_c(porosity, 3.0), C(ds * ds, 0.0));
+    const cx den = cmul(C(150.0, 0.0), cpow_c(csub(C(1.0, 0.0), porosity), 2.0));
+    hyd = cdiv(num, den).re;
+  }
+  const cx D = cdiv(C(hyd, 0.0),
+                    cmul(cmul(cmul(porosity, C(P.kin_visc_c, 0.0)), rho_gas), gas_compr));
+  if (!((exc.re > 0.0) && (h.re > 0.0))) return C(0.0, 0.0);
+  cx f;
+  if (P.finh == 0) {
+    f = C(1.0, 0.0);
+  } else if (P.finh == 1) {
+    f = (sa.re < P.max_pack) ? C(1.0, 0.0) : C(0.0, 0.0);
+  } else {
+    double at = P.alpha_trans;
+    if (P.finh == 3) {
+      at = cmul(C(kTenPowM028, 0.0), cpow_c(h, 0.12)).re;
+      if (at >= P.max_pack) at = 0.99 * P.max_pack;
+    }
+    const double width = P.max_pack - at;
+    double tt = (sa.re - at) / width;
+    if (tt < 0.0) tt = 0.0;
+    if (tt > 1.0) tt = 1.0;
+    cx sc = C(0.0, 0.0);
+    for (int n = 0; n <= P.N_inh; n++) {
+      const double term = P.pascal[n] * pow_cr(tt, (double)(P.N_inh + n) + 1.0);
+      sc = cadd(sc, C(term, 0.0));
+    }
+    f = csub(C(1.0, 0.0), sc);
+  }
+  const double c2 = (P.pi_g / 2.0) * (P.pi_g / 2.0);
+  const cx hm = (h_threshold > h.re) ? C(h_threshold, 0.0) : h;
+  const cx r = cmul(cmul(cdiv(cmul(cmul(rho_m, C(c2, 0.0)), D), hm), exc), f);
+  return C(-r.re, -r.im);
+}
+
 // c_phys_var + eval_implicit_terms (Voellmy turbulent part) in complex
 // arithmetic: src/constitutive_2d.f90:1154-1322, :3513-3683, :3994-4009.
 // Only the momentum rows are non-zero; they are returned in nh1, nh2.
-template <int NS>
-__device__ inline void implicit_c(const Params& P, const cx* qj, cx& nh1, cx& nh2, double xi) {
+template <int NS, int NX = 0>
+__device__ inline void implicit_c(const Params& P, const cx* qj, cx& nh1, cx& nh2, double xi,
+                                  cx* nhp = nullptr) {
   nh1 = C(0.0, 0.0);
   nh2 = C(0.0, 0.0);
   // mu(I) model 11 has no implicit friction term (:3660 takes models 1, 12)
-  if (P.rheology == 11) return;
-  cx u, v, rho_m;
+  if (NX == 0 && P.rheology == 11) return;
+  cx u, v, rho_m, T, hh, exc, alphas[NS];
   if (qj[0].re > kEps) {
     cx inv_cqj1 = cdiv(C(1.0, 0.0), qj[0]);
+    exc = (NX == 1) ? cmul(qj[4 + NS], inv_cqj1) : C(0.0, 0.0);  // :1195
     cx xs[NS];
 #pragma unroll
     for (int i = 0; i < NS; i++) xs[i] = cmul(qj[4 + i], inv_cqj1);
@@ -652,7 +825,6 @@ __device__ inline void implicit_c(const Params& P, const cx* qj, cx& nh1, cx& nh
 #pragma unroll
     for (int i = 0; i < NS; i++) dot = cadd(dot, cmul(cconj(xs[i]), C(P.cp_s[i], 0.0)));
     cx sp_heat_mix = cadd(dot, cmul(xc, sp_heat_c));
-    cx T;
     if (qj[0].re > P.eps_sing) {
       if (P.energy) {
         cx ke = cmul(cmul(C(0.5, 0.0), cadd(csq(qj[1]), csq(qj[2]))), inv_cqj1);
@@ -671,6 +843,11 @@ __device__ inline void implicit_c(const Params& P, const cx* qj, cx& nh1, cx& nh
     for (int i = 0; i < NS; i++) d2 = cadd(d2, cmul(cconj(xs[i]), C(P.inv_rho_s[i], 0.0)));
     cx inv_rhom = cadd(d2, cmul(xc, inv_rho_c));
     rho_m = cdiv(C(1.0, 0.0), inv_rhom);
+    if constexpr (NX == 1) {  // :1286-1290
+#pragma unroll
+      for (int i = 0; i < NS; i++) alphas[i] = cmul(cmul(rho_m, xs[i]), C(P.inv_rho_s[i], 0.0));
+      hh = cmul(qj[0], inv_rhom);
+    }
     if (qj[0].re > P.eps_sing) {
       u = cmul(qj[1], inv_cqj1);
       v = cmul(qj[2], inv_cqj1);
@@ -685,16 +862,24 @@ __device__ inline void implicit_c(const Params& P, const cx* qj, cx& nh1, cx& nh
     u = C(0.0, 0.0);
     v = C(0.0, 0.0);
     rho_m = C(P.rho_a_amb, 0.0);
+    T = C(P.T_amb, 0.0);
+    hh = C(0.0, 0.0);
+    exc = C(0.0, 0.0);
+#pragma unroll
+    for (int i = 0; i < NS; i++) alphas[i] = C(0.0, 0.0);
   }
-  cx red_grav = cmul(cdiv(csub(rho_m, C(P.rho_a_amb, 0.0)), rho_m), C(P.grav, 0.0));
-  cx w = C(0.0, 0.0);
-  cx mod_vel2 = cadd(cadd(csq(u), csq(v)), csq(w));
-  cx mod_vel = csqrt_c(mod_vel2);
-  if (mod_vel.re != 0.0) {
-    cx ts = cmul(cdiv(cmul(rho_m, red_grav), C(xi, 0.0)), mod_vel2);
-    nh1 = csub(nh1, cmul(ts, cdiv(u, mod_vel)));
-    nh2 = csub(nh2, cmul(ts, cdiv(v, mod_vel)));
+  if (P.rheology != 11) {
+    cx red_grav = cmul(cdiv(csub(rho_m, C(P.rho_a_amb, 0.0)), rho_m), C(P.grav, 0.0));
+    cx w = C(0.0, 0.0);
+    cx mod_vel2 = cadd(cadd(csq(u), csq(v)), csq(w));
+    cx mod_vel = csqrt_c(mod_vel2);
+    if (mod_vel.re != 0.0) {
+      cx ts = cmul(cdiv(cmul(rho_m, red_grav), C(xi, 0.0)), mod_vel2);
+      nh1 = csub(nh1, cmul(ts, cdiv(u, mod_vel)));
+      nh2 = csub(nh2, cmul(ts, cdiv(v, mod_vel)));
+    }
   }
+  if constexpr (NX == 1) *nhp = pore_source_c<NS>(P, hh, T, rho_m, alphas, exc);
 }
 
 // The residual evaluation of the Newton solve calls the implicit terms at a
@@ -841,12 +1026,12 @@ __device__ inline double settling_velocity(const Params& P, double diam, double 
 // must call this together; lane `lane` computes the settling velocities of
 // the classes i with i % 2 == lane and the pair swaps them, so the slowest
 // part (the drag fixed point) runs on both lanes at once.
-template <int NS, int LANES = 1>
+template <int NS, int LANES = 1, int NX = 0>
 __device__ inline bool mass_exchange(const Params& P, const double* qp, const double* erod,
                                      double dt, double* dep, double* eq, double* ers,
                                      double& topo, const ddm::dd* tab, int lane = 0,
                                      PhaseClk* pc = nullptr) {
-  constexpr int NV = 4 + NS;
+  constexpr int NV = 4 + NS + NX;
   topo = 0.0;
 #pragma unroll
   for (int i = 0; i < NS; i++) {
@@ -882,7 +1067,7 @@ __device__ inline bool mass_exchange(const Params& P, const double* qp, const do
   double cpe = te * P.erodible_porosity;
   if (at <= 0.0) return false;
   double T = qp[3];
-  Mixt m = mixt_var<NS>(P, qp, true);
+  Mixt m = mixt_var<NS, double, NX>(P, qp, true);
   double ikv = 1.0 / P.kin_visc_c;
   bool err = false;
   // settling velocities (pure functions of the class and rho_c) and the
@@ -950,9 +1135,39 @@ __device__ inline bool mass_exchange(const Params& P, const double* qp, const do
 #pragma unroll
   for (int i = 0; i < NS; i++) sdep += dep[i];
   double cpl = 0.0 + P.coeff_porosity * sdep;
-  // the continuous-phase loss is limited by the maximum solid packing (0.64)
-  // and by the continuous phase present (src/constitutive_2d.f90:5034-5041)
-  cpl = fmin_f(cpl, h * fmax_f(0.0, 0.64 - at) / dt);
+  double max_pack = 0.64;
+  if constexpr (NX == 1) {
+    max_pack = P.max_pack;
+    // gas loss driven by the excess pore pressure, inhibited by f_inhibit:
+    // src/constitutive_2d.f90:4887-5032
+    if (P.gas_loss && at < P.max_pack) {
+      const double pexc = qp[4 + NS];
+      double f_inh = 1.0;
+      if (P.finh == 1) {
+        f_inh = (at < P.max_pack) ? 1.0 : 0.0;
+      } else if (P.finh == 2) {
+        f_inh = finh_sum(P, at, P.alpha_trans);
+      } else if (P.finh == 3) {
+        if (h < kEps) h = kEps;
+        double atd = kTenPowM028 * pow_cr(h, 0.12, tab);
+        if (atd >= P.max_pack) atd = 0.99 * P.max_pack;
+        f_inh = finh_sum(P, at, atd);
+      }
+      double hyd = P.hyd_perm;
+      if (P.dyn_perm) {
+        const double ds = perm_grain<NS>(P, alphas), pr = 1.0 - at;
+        hyd = ((pr * pr) * pr * (ds * ds)) / (150.0 * (at * at));
+      }
+      // MAX(1.e-5, r_h) with a default-real literal
+      const double hmin = (double)1.0e-5f;
+      const double vel = hyd / (P.kin_visc_c * m.rho_c) / fmax_f(hmin, h) * 0.5 * P.pi_g * pexc;
+      cpl = cpl + vel * f_inh;
+    }
+  }
+  // the continuous-phase loss is limited by the maximum solid packing (0.64
+  // without pore pressure) and by the continuous phase present
+  // (src/constitutive_2d.f90:5034-5041)
+  cpl = fmin_f(cpl, h * fmax_f(0.0, max_pack - at) / dt);
   cpl = fmin_f(cpl, (h * (1.0 - at) - P.coeff_porosity * (h * at - dt * sdep)) / dt);
   // with erodible_deposit_flag the reference sets T_erodible to the cell's T
   // before using it (src/constitutive_2d.f90:5044)
@@ -983,6 +1198,7 @@ __device__ inline bool mass_exchange(const Params& P, const double* qp, const do
             P.T_amb * P.cp_a * P.rho_a_amb * air;
 #pragma unroll
   for (int i = 0; i < NS; i++) eq[4 + i] = P.rho_s[i] * (ers[i] - dep[i]);
+  if constexpr (NX == 1) eq[4 + NS] = eq[0] * qp[4 + NS];  // :5140-5145
   // rate of change of the topography (src/constitutive_2d.f90:5149), used
   // only with topography change
   if (P.topo_change) {

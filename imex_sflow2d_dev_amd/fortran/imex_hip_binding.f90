@@ -59,6 +59,12 @@ MODULE imex_hip_binding
      INTEGER(c_int32_t) :: rheology_model
      REAL(c_double) :: mu_s, mu_2, muI_inf, I_0
      INTEGER(c_int32_t) :: stochastic_flag
+     ! pore pressure (PORE_PRESSURE_PARAMETERS): f_inhibit_mode 0 OFF,
+     ! 1 STEP, 2 STATIC, 3 DYNAMIC; device-driven steps only
+     INTEGER(c_int32_t) :: pore_pressure_flag, gas_loss_flag, f_inhibit_mode, N_inh
+     INTEGER(c_int32_t) :: dynamic_permeability_flag
+     REAL(c_double) :: hydraulic_permeability, alpha_trans, maximum_solid_packing
+     REAL(c_double) :: sphericity_s(3)
   END TYPE imex_config
 
   ! typedef struct imex_static: borrowed pointers to nx*ny arrays (x

@@ -52,6 +52,9 @@ def qp_to_qc(cfg: RunConfig, qp: np.ndarray) -> np.ndarray:
         qc[..., 3] = h * rho_m * cp_mix * T
     for i in range(ns):
         qc[..., 4 + i] = xs[i] * qc[..., 0]
+    if cfg.pore_pressure_flag:
+        # q1 * p_exc (src/constitutive_2d.f90:1981, :2143)
+        qc[..., 4 + ns] = qp[..., 4 + ns] * qc[..., 0]
     qc[~wet] = 0.0
     return qc
 
@@ -87,6 +90,9 @@ def collapsing_volume(cfg: RunConfig) -> np.ndarray:
     qp[..., 0] = np.where(sel, frac * cfg.h_collapse, 0.0)
     for i in range(ns):
         qp[..., 4 + i] = np.where(sel, frac * cfg.h_collapse * cfg.alphas_collapse[i], 0.0)
+    if cfg.pore_pressure_flag:
+        # the cylinder's excess pore pressure is 1 (src/init_2d.f90:146-147)
+        qp[..., 4 + ns] = np.where(sel, 1.0, 0.0)
     return qp_to_qc(cfg, qp)
 
 

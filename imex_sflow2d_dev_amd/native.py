@@ -80,6 +80,11 @@ class ImexConfig(ctypes.Structure):
         ("rheology_model", ctypes.c_int32),
         ("mu_s", ctypes.c_double), ("mu_2", ctypes.c_double), ("muI_inf", ctypes.c_double),
         ("I_0", ctypes.c_double), ("stochastic_flag", ctypes.c_int32),
+        ("pore_pressure_flag", ctypes.c_int32), ("gas_loss_flag", ctypes.c_int32),
+        ("f_inhibit_mode", ctypes.c_int32), ("N_inh", ctypes.c_int32),
+        ("dynamic_permeability_flag", ctypes.c_int32),
+        ("hydraulic_permeability", ctypes.c_double), ("alpha_trans", ctypes.c_double),
+        ("maximum_solid_packing", ctypes.c_double), ("sphericity_s", ctypes.c_double * 3),
     ]
 
 
@@ -181,6 +186,16 @@ def make_config(cfg: RunConfig) -> ImexConfig:
     c.rheology_model = cfg.rheology_model
     c.mu_s, c.mu_2, c.muI_inf, c.I_0 = cfg.mu_s, cfg.mu_2, cfg.muI_inf, cfg.I_0
     c.stochastic_flag = int(cfg.stochastic_flag)
+    c.pore_pressure_flag = int(cfg.pore_pressure_flag)
+    c.gas_loss_flag = int(cfg.gas_loss_flag)
+    c.f_inhibit_mode = cfg.f_inhibit_code
+    c.N_inh = int(cfg.N_inh)
+    c.dynamic_permeability_flag = int(cfg.dynamic_permeability_flag)
+    c.hydraulic_permeability = cfg.hydraulic_permeability
+    c.alpha_trans = cfg.alpha_trans
+    c.maximum_solid_packing = cfg.maximum_solid_packing
+    for i in range(3):
+        c.sphericity_s[i] = cfg.sphericity_s[i] if i < len(cfg.sphericity_s) else 1.0
     return c
 
 

@@ -76,6 +76,24 @@ typedef struct imex_config {
    * turbulent term of models 1 and 12 (src/constitutive_2d.f90:3663-3673);
    * Z is 0 until imex_set_noise gives it */
   int32_t stochastic_flag;
+  /* pore pressure (PORE_PRESSURE_PARAMETERS, src/inpout_2d.f90:389-391,
+   * :536-542, :681-710; maximum_solid_packing and sphericity_s from
+   * SOLID_TRANSPORT_PARAMETERS, :819, :854): the conserved variable
+   * q1 * p_exc after the solid classes (n_vars = 5 + n_solid), its implicit
+   * decay (src/constitutive_2d.f90:3844-3993) with f_inhibit_mode 0 OFF,
+   * 1 STEP, 2 STATIC, 3 DYNAMIC (smoothstep of order N_inh), the
+   * Carman-Kozeny permeability when dynamic_permeability_flag, the gas loss
+   * (:4887-5032) when gas_loss_flag, and the effective normal stress of the
+   * friction laws.  Device-driven steps only (imex_run_steps); not with the
+   * radial source, settling, entrainment, erosion, topography change, the
+   * FP32 form, batches or the output kernels.  maximum_solid_packing 0
+   * means the reference's default 0.64; sphericity_s 0 means 1. */
+  int32_t pore_pressure_flag;
+  int32_t gas_loss_flag;
+  int32_t f_inhibit_mode;
+  int32_t N_inh;
+  int32_t dynamic_permeability_flag;
+  double hydraulic_permeability, alpha_trans, maximum_solid_packing, sphericity_s[3];
 } imex_config;
 
 /* Per-cell static fields computed once by the host (src/geometry_2d.f90:555-895

@@ -99,6 +99,12 @@ struct oracle_s {
     /* 0: x**y by the C library's pow, as the reference build calls it;
      * 1: x**y correctly rounded (quad-precision powq rounded once) */
     int pow_mode;
+    /* pore pressure (PORE_PRESSURE_PARAMETERS, src/inpout_2d.f90:389-391,
+     * :536-542, :681-710; maximum_solid_packing and sphericity_s from
+     * SOLID_TRANSPORT_PARAMETERS): the conserved variable q1 * p_exc at
+     * index ipp = 4 + ns (0-based) */
+    int pore, gas_loss, finh, N_inh, dyn_perm, ipp;  /* finh: 0 OFF 1 STEP 2 STATIC 3 DYNAMIC */
+    double hyd_perm, alpha_trans, max_pack, sph_s[3], pascal[64];
 };
 
 /* Work counters of the stage-2 Newton solves and the settling iterations
@@ -139,6 +145,22 @@ double oracle_pow_cr(double x, double y)
 static inline double rpow(const oracle_t *o, double x, double y)
 {
     return o->pow_mode ? oracle_pow_cr(x, y) : pow(x, y);
+}
+
+/* The other C-library functions the pore-pressure terms reach (through
+ * the C library's cpow, which Fortran COMPLEX**REAL calls), each in the
+ * mode of x**y: the C library's own (= the reference) or correctly rounded
+ * (binary128 evaluation rounded once, the rounding the HIP path computes). */
+static inline double r_log(const oracle_t *o, double x) { return o->pow_mode ? (double)logq(x) : log(x); }
+static inline double r_log1p(const oracle_t *o, double x) { return o->pow_mode ? (double)log1pq(x) : log1p(x); }
+static inline double r_exp(const oracle_t *o, double x) { return o->pow_mode ? (double)expq(x) : exp(x); }
+static inline double r_atan2(const oracle_t *o, double y, double x)
+{
+    return o->pow_mode ? (double)atan2q(y, x) : atan2(y, x);
+}
+static inline double r_hypot(const oracle_t *o, double x, double y)
+{
+    return o->pow_mode ? (double)hypotq(x, y) : hypot(x, y);
 }
 
 /* ---------------------------------------------------------------------- */
@@ -204,6 +226,54 @@ static inline cx csqrt_c(cx a)
     return C(creal(z), cimag(z));
 }
 
+/* Fortran COMPLEX**REAL: the reference build calls the C library's
+ * cpow(z, c + 0i) (checked on amdflang's output for z**3.0_wp and z**r).
+ * The C library computes it as cexp(c * clog(z)); in the correctly rounded
+ * mode the same steps run with correctly rounded log1p / log / hypot /
+ * atan2 / exp / sin / cos: clog's real part by its case split (|x| = 1,
+ * 1 < |x| < 2, 0.5 <= |x| < 1 with a tiny or a larger |y|, otherwise
+ * log(hypot)), the product with c + 0i as (cL - 0 th, c th + 0 L), cexp as
+ * exp(a) (cos b, sin b), with (cos b, sin b) = (1, b) for |b| <= DBL_MIN.
+ * Finite arguments with |z| well inside the double range only (the
+ * solver's porosities and thicknesses). */
+static cx cpow_r(const oracle_t *o, cx z, double c)
+{
+    if (!o->pow_mode) {
+        double complex r = cpow(CMPLX(z.re, z.im), CMPLX(c, 0.0));
+        return C(creal(r), cimag(r));
+    }
+    double ax = fabs(z.re), ay = fabs(z.im);
+    if (ax < ay) { double t = ax; ax = ay; ay = t; }
+    double L;
+    if (ax == 1.0) {
+        L = r_log1p(o, ay * ay) / 2.0;
+    } else if (ax > 1.0 && ax < 2.0 && ay < 1.0) {
+        double d2m1 = (ax - 1.0) * (ax + 1.0);
+        if (ay >= EPS) d2m1 += ay * ay;
+        L = r_log1p(o, d2m1) / 2.0;
+    } else if (ax < 1.0 && ax >= 0.5 && ay < EPS / 2.0) {
+        double d2m1 = (ax - 1.0) * (ax + 1.0);
+        L = r_log1p(o, d2m1) / 2.0;
+    } else if (ax < 1.0 && ax >= 0.5 && ax * ax + ay * ay >= 0.5) {
+        __float128 X = ax, Y = ay;
+        L = r_log1p(o, (double)(X * X + Y * Y - 1)) / 2.0;
+    } else {
+        L = r_log(o, r_hypot(o, ax, ay)) - 0.0 * 0x1.62e42fefa39efp-1;
+    }
+    double th = r_atan2(o, z.im, z.re);
+    double a = c * L - 0.0 * th, b = c * th + 0.0 * L;
+    double sb, cb;
+    if (fabs(b) > 2.2250738585072014e-308) {
+        sb = (double)sinq(b);
+        cb = (double)cosq(b);
+    } else {
+        sb = b;
+        cb = 1.0;
+    }
+    double E = r_exp(o, a);
+    return C(E * cb, E * sb);
+}
+
 /* ====================================================================== */
 /* constitutive_2d                                                         */
 /* ====================================================================== */
@@ -213,7 +283,7 @@ static inline cx csqrt_c(cx a)
  *  :775-797 carrier density, :799-830 mixture, :944-972 velocity, p_dyn) */
 static void r_phys_var(const oracle_t *o, const double *qc, double *h, double *u, double *v,
                        double *alphas, double *rho_m, double *T, double *red_grav,
-                       double *p_dyn)
+                       double *p_dyn, double *pexc)
 {
     const int ns = o->ns;
     double xs[3], inv_qj1;
@@ -227,8 +297,10 @@ static void r_phys_var(const oracle_t *o, const double *qc, double *h, double *u
             for (int i = 0; i < ns; i++) sx += xs[i];
             for (int i = 0; i < ns; i++) xs[i] = xs[i] / sx;
         }
+        /* excess pore pressure: src/constitutive_2d.f90:659 */
+        *pexc = o->pore ? qc[o->ipp] * inv_qj1 : 0.0;
     } else {
-        *h = 0.0; *u = 0.0; *v = 0.0;
+        *h = 0.0; *u = 0.0; *v = 0.0; *pexc = 0.0;
         for (int i = 0; i < ns; i++) alphas[i] = 0.0;
         *rho_m = o->rho_a_amb;
         *T = o->T_amb;
@@ -283,14 +355,15 @@ static void r_phys_var(const oracle_t *o, const double *qc, double *h, double *u
 /* qc_to_qp: src/constitutive_2d.f90:1637-1690 */
 static void qc_to_qp(const oracle_t *o, const double *qc, double *qp, double *p_dyn)
 {
-    double h, u, v, alphas[3], rho_m, T, rg;
-    r_phys_var(o, qc, &h, &u, &v, alphas, &rho_m, &T, &rg, p_dyn);
+    double h, u, v, alphas[3], rho_m, T, rg, pexc;
+    r_phys_var(o, qc, &h, &u, &v, alphas, &rho_m, &T, &rg, p_dyn, &pexc);
     const int nv = o->nv;
     qp[0] = h;
     qp[1] = h * u;
     qp[2] = h * v;
     qp[3] = T;
     for (int i = 0; i < o->ns; i++) qp[4 + i] = alphas[i] * h;
+    if (o->pore) qp[o->ipp] = pexc;  /* :1683 */
     qp[nv] = u;
     qp[nv + 1] = v;
 }
@@ -341,6 +414,7 @@ static void qp_to_qc(const oracle_t *o, const double *qp, double *qc)
         qc[3] = h * rho_m * cp_mix * T;
     }
     for (int i = 0; i < ns; i++) qc[4 + i] = xs[i] * qc[0];
+    if (o->pore) qc[o->ipp] = qp[o->ipp] * qc[0];  /* :1981, :2143 */
 }
 
 /* eval_sp_heat, gas without liquid: src/constitutive_2d.f90:1513-1607 (:1568-1604) */
@@ -450,6 +524,8 @@ static void eval_fluxes(const oracle_t *o, const double *qc, const double *qp, d
         if ((f[0] > 0.0) && (S / f[0] > 1.0)) {
             for (int i = 0; i < ns; i++) f[4 + i] = f[4 + i] / S * f[0];
         }
+        /* flux of q1 * p_exc: src/constitutive_2d.f90:2435-2439, :2497-2501 */
+        if (o->pore) f[o->ipp] = w * qc[o->ipp] * 1.0;
     } else {
         for (int i = 0; i < nv; i++) f[i] = 0.0;
     }
@@ -546,6 +622,96 @@ static double average_density_solids(const oracle_t *o, const double *al)
     return t / s;
 }
 
+/* Carman-Kozeny permeability with the mean sphericity (dynamic
+ * permeability): src/constitutive_2d.f90:3860-3876 (complex), :4975-4992
+ * (real).  For one class the class's diameter and sphericity; for several
+ * the Sauter diameter and the surface-area-weighted sphericity. */
+static void perm_grain(const oracle_t *o, const double *al, double *diam, double *sph)
+{
+    if (o->ns > 1) {
+        *diam = sauter_diameter(o, al);
+        double a = 0.0, b = 0.0;
+        for (int i = 0; i < o->ns; i++) a += al[i] * (o->diam_s[i] * o->diam_s[i]) * o->sph_s[i];
+        for (int i = 0; i < o->ns; i++) b += al[i] * (o->diam_s[i] * o->diam_s[i]);
+        *sph = a / b;
+    } else {
+        *diam = o->diam_s[0];
+        *sph = o->sph_s[0];
+    }
+}
+
+/* the smoothstep sum of f_inhibit (STATIC and DYNAMIC modes):
+ * 1 - sum_n pascal(n) tt**(N + n + 1), tt = (x - a_t)/(max_pack - a_t)
+ * clamped to [0, 1] (src/constitutive_2d.f90:3904-3927, :4911-4934) */
+static double finh_sum(const oracle_t *o, double x, double at)
+{
+    double width = o->max_pack - at;
+    double tt = (x - at) / width;
+    if (tt < 0.0) tt = 0.0;
+    if (tt > 1.0) tt = 1.0;
+    double s = 0.0;
+    for (int n = 0; n <= o->N_inh; n++) {
+        double term = o->pascal[n] * rpow(o, tt, (double)(o->N_inh + n) + 1.0);
+        s = s + term;
+    }
+    return 1.0 - s;
+}
+
+/* 10_wp**(-0.28_wp) as the reference build folds it at compile time */
+#define TEN_POW_M028 0x1.0cb3903cf06e8p-1
+
+/* the pore-pressure row of eval_implicit_terms (complex):
+ * src/constitutive_2d.f90:3844-3993 */
+static cx pore_source_c(const oracle_t *o, cx h, cx T, cx rho_m, const cx *alphas, cx exc)
+{
+    const double h_threshold = 1.0e-10;
+    double gamma_gas = o->cp_a / (o->cp_a - o->R_a);
+    cx gas_compr = C(1.0 / (gamma_gas * o->pres), 0.0);
+    cx rho_gas = cdiv(C(o->pres, 0.0), cmul(C(o->R_a, 0.0), T));
+    cx sa = C(0.0, 0.0);
+    for (int i = 0; i < o->ns; i++) sa = cadd(sa, alphas[i]);
+    cx porosity = csub(C(1.0, 0.0), sa);
+    double hyd = o->hyd_perm;
+    if (o->dyn_perm) {
+        double al[3], diam, sph;
+        for (int i = 0; i < o->ns; i++) al[i] = alphas[i].re;
+        perm_grain(o, al, &diam, &sph);
+        double ds = diam * sph;
+        cx num = cmul(cpow_r(o, porosity, 3.0), C(ds * ds, 0.0));
+        cx den = cmul(C(150.0, 0.0), cpow_r(o, csub(C(1.0, 0.0), porosity), 2.0));
+        hyd = cdiv(num, den).re;
+    }
+    cx D = cdiv(C(hyd, 0.0),
+                cmul(cmul(cmul(porosity, C(o->kin_visc_c, 0.0)), rho_gas), gas_compr));
+    if (!((exc.re > 0.0) && (h.re > 0.0))) return C(0.0, 0.0);
+    cx f;
+    if (o->finh == 0) {
+        f = C(1.0, 0.0);
+    } else if (o->finh == 1) {
+        f = (sa.re < o->max_pack) ? C(1.0, 0.0) : C(0.0, 0.0);
+    } else {
+        double at = o->alpha_trans;
+        if (o->finh == 3) {
+            at = cmul(C(TEN_POW_M028, 0.0), cpow_r(o, h, 0.12)).re;
+            if (at >= o->max_pack) at = 0.99 * o->max_pack;
+        }
+        double width = o->max_pack - at;
+        double tt = (sa.re - at) / width;
+        if (tt < 0.0) tt = 0.0;
+        if (tt > 1.0) tt = 1.0;
+        cx sc = C(0.0, 0.0);
+        for (int n = 0; n <= o->N_inh; n++) {
+            double term = o->pascal[n] * rpow(o, tt, (double)(o->N_inh + n) + 1.0);
+            sc = cadd(sc, C(term, 0.0));
+        }
+        f = csub(C(1.0, 0.0), sc);
+    }
+    double c2 = (o->pi_g / 2.0) * (o->pi_g / 2.0);
+    cx hm = (h_threshold > h.re) ? C(h_threshold, 0.0) : h;
+    cx r = cmul(cmul(cdiv(cmul(cmul(rho_m, C(c2, 0.0)), D), hm), exc), f);
+    return C(-r.re, -r.im);
+}
+
 /* eval_nh_semi_impl_terms: src/constitutive_2d.f90:4027-4547; Voellmy
  * Coulomb part (model 1, :4164-4213) and the mu(I) friction of models 11
  * and 12 (:4384-4537, the two branches compute the same terms) */
@@ -563,6 +729,8 @@ static void eval_nh_semi_impl_terms(const oracle_t *o, double G, const double *q
             double centr = 0.0;
             double tt = o->mu * rm * (rg + centr) * h * sqrt(G);
             tt = fmax_f(0.0, tt);
+            /* :4191-4198 */
+            if (o->pore) tt = fmax_f(0.0, tt - o->mu * sqrt(G) * qp[o->ipp]);
             tt = fmax_f(0.0, tt);
             si[1] = si[1] - tt * u / mod_vel;
             si[2] = si[2] - tt * v / mod_vel;
@@ -581,6 +749,7 @@ static void eval_nh_semi_impl_terms(const oracle_t *o, double G, const double *q
         if (h < EPS) h = EPS;
         double centr = 0.0;
         double vse = rm * (rg + centr) * h;
+        if (o->pore) vse = rm * (rg + centr) * h - qp[o->ipp];  /* :4423-4427, :4502-4506 */
         vse = fmax_f(vse, 0.0);
         double ens = fmax_f(0.0, vse * sqrt(G));
         if (ens < EPS) ens = EPS;
@@ -600,11 +769,12 @@ static void eval_implicit_terms_c(const oracle_t *o, const cx *qj, cx *nh)
 {
     const int ns = o->ns, nv = o->nv;
     for (int i = 0; i < nv; i++) nh[i] = C(0.0, 0.0);
-    cx u, v, rho_m, inv_rhom;
+    cx u, v, rho_m, inv_rhom, T, hh, alphas[3], exc;
     if (qj[0].re > EPS) {
         cx inv_cqj1 = cdiv(C(1.0, 0.0), qj[0]);
         cx xs[3];
         for (int i = 0; i < ns; i++) xs[i] = cmul(qj[4 + i], inv_cqj1);
+        exc = o->pore ? cmul(qj[o->ipp], inv_cqj1) : C(0.0, 0.0);  /* :1195 */
         cx xs_tot = C(0.0, 0.0);
         for (int i = 0; i < ns; i++) xs_tot = cadd(xs_tot, xs[i]);
         cx xc = csub(C(1.0, 0.0), xs_tot);
@@ -612,7 +782,6 @@ static void eval_implicit_terms_c(const oracle_t *o, const cx *qj, cx *nh)
         cx dot = C(0.0, 0.0);
         for (int i = 0; i < ns; i++) dot = cadd(dot, cmul(cconj(xs[i]), C(o->cp_s[i], 0.0)));
         cx sp_heat_mix = cadd(dot, cmul(xc, sp_heat_c));
-        cx T;
         if (qj[0].re > o->eps_sing) {
             if (o->energy) {
                 cx ke = cmul(cmul(C(0.5, 0.0), cadd(csq(qj[1]), csq(qj[2]))), inv_cqj1);
@@ -630,6 +799,9 @@ static void eval_implicit_terms_c(const oracle_t *o, const cx *qj, cx *nh)
         for (int i = 0; i < ns; i++) d2 = cadd(d2, cmul(cconj(xs[i]), C(o->inv_rho_s[i], 0.0)));
         inv_rhom = cadd(d2, cmul(xc, inv_rho_c));
         rho_m = cdiv(C(1.0, 0.0), inv_rhom);
+        /* :1286-1290 */
+        for (int i = 0; i < ns; i++) alphas[i] = cmul(cmul(rho_m, xs[i]), C(o->inv_rho_s[i], 0.0));
+        hh = cmul(qj[0], inv_rhom);
         if (qj[0].re > o->eps_sing) {
             u = cmul(qj[1], inv_cqj1);
             v = cmul(qj[2], inv_cqj1);
@@ -646,6 +818,10 @@ static void eval_implicit_terms_c(const oracle_t *o, const cx *qj, cx *nh)
         v = C(0.0, 0.0);
         rho_m = C(o->rho_a_amb, 0.0);
         inv_rhom = cdiv(C(1.0, 0.0), rho_m);
+        T = C(o->T_amb, 0.0);
+        hh = C(0.0, 0.0);
+        for (int i = 0; i < ns; i++) alphas[i] = C(0.0, 0.0);
+        exc = C(0.0, 0.0);
     }
     (void)inv_rhom;
     cx red_grav = cmul(cdiv(csub(rho_m, C(o->rho_a_amb, 0.0)), rho_m), C(o->grav, 0.0));
@@ -657,6 +833,7 @@ static void eval_implicit_terms_c(const oracle_t *o, const cx *qj, cx *nh)
         nh[1] = csub(nh[1], cmul(ts, cdiv(u, mod_vel)));
         nh[2] = csub(nh[2], cmul(ts, cdiv(v, mod_vel)));
     }
+    if (o->pore) nh[o->ipp] = pore_source_c(o, hh, T, rho_m, alphas, exc);
 }
 
 static void eval_implicit_terms_r(const oracle_t *o, const double *q, double *nh)
@@ -753,9 +930,37 @@ static int eval_mass_exchange_terms(const oracle_t *o, const double *qp, const d
     for (int i = 0; i < ns; i++) sdep += dep[i];
     double cpl = 0.0;
     cpl = cpl + o->coeff_porosity * sdep;
-    /* limits by the maximum solid packing (0.64, src/inpout_2d.f90:854) and
+    /* gas loss driven by the excess pore pressure, inhibited by f_inhibit:
+     * src/constitutive_2d.f90:4887-5032 */
+    if (o->pore && o->gas_loss && at < o->max_pack) {
+        const double pexc = qp[o->ipp];
+        double f_inh = 1.0, hl = h;
+        if (o->finh == 1) {
+            f_inh = (at < o->max_pack) ? 1.0 : 0.0;
+        } else if (o->finh == 2) {
+            f_inh = finh_sum(o, at, o->alpha_trans);
+        } else if (o->finh == 3) {
+            if (hl < EPS) hl = EPS;
+            double atd = TEN_POW_M028 * rpow(o, hl, 0.12);
+            if (atd >= o->max_pack) atd = 0.99 * o->max_pack;
+            f_inh = finh_sum(o, at, atd);
+        }
+        double hyd = o->hyd_perm;
+        if (o->dyn_perm) {
+            double diam, sph;
+            perm_grain(o, alphas, &diam, &sph);
+            double ds = diam * sph, pr = 1.0 - at;
+            hyd = ((pr * pr) * pr * (ds * ds)) / (150.0 * (at * at));
+        }
+        /* MAX(1.e-5, r_h): a default-real literal */
+        const double hmin = (double)1.0e-5f;
+        double vel = hyd / (o->kin_visc_c * rc) / fmax_f(hmin, hl) * 0.5 * o->pi_g * pexc;
+        cpl = cpl + vel * f_inh;
+        h = hl;
+    }
+    /* limits by the maximum solid packing (src/inpout_2d.f90:854) and
      * by the continuous phase present: src/constitutive_2d.f90:5034-5041 */
-    cpl = fmin_f(cpl, h * fmax_f(0.0, 0.64 - at) / dt);
+    cpl = fmin_f(cpl, h * fmax_f(0.0, o->max_pack - at) / dt);
     cpl = fmin_f(cpl, (h * (1.0 - at) - o->coeff_porosity * (h * at - dt * sdep)) / dt);
     const double T_erodible = o->erodible_deposit ? T : o->T_erodible;
     double air = 0.0;
@@ -780,6 +985,7 @@ static int eval_mass_exchange_terms(const oracle_t *o, const double *qp, const d
         eq[3] = -(T * (A + rc * cpc * cpl)) + T_erodible * (D + o->rho_c_sub * cpc * cpe) +
                 o->T_amb * o->cp_a * o->rho_a_amb * air;
     for (int i = 0; i < ns; i++) eq[4 + i] = o->rho_s[i] * (ers[i] - dep[i]);
+    if (o->pore) eq[o->ipp] = eq[0] * qp[o->ipp];  /* :5140-5145 */
     /* topo_term: src/constitutive_2d.f90:5149 */
     double dep_tot = 0.0, ers_tot = 0.0;
     for (int i = 0; i < ns; i++) dep_tot += dep[i];
@@ -1370,6 +1576,73 @@ static void eval_f(const oracle_t *o, double dt, const double *qj, const double 
     *scal_f = 0.5 * s;
 }
 
+/* implicit_flag (src/constitutive_2d.f90:478-528): the momentum equations
+ * and, with pore pressure, the pore-pressure equation */
+static inline int implicit_var(const oracle_t *o, int i)
+{
+    return i == 1 || i == 2 || (o->pore && i == o->ipp);
+}
+
+/* DGESV for the n_nh x n_nh implicit block (n_nh = 3 with pore pressure):
+ * src/solver_2d.f90:1905-1917.  The reference links LAPACK from the OpenBLAS
+ * build that SciPy bundles; its result is reproduced here by a left-looking
+ * (Crout) LU with partial pivoting: column j first takes the earlier row
+ * interchanges, then a(i,j) -= dot(a(i,0:m), a(0:m,j)) with the dot product
+ * accumulated by fused multiply-adds from 0 (m = i for the rows above the
+ * diagonal, m = j below), the pivot is the first largest |a(i,j)|, rows j
+ * and p are swapped in columns 0..j, and the column below the pivot is
+ * scaled by the reciprocal of the pivot; the solve permutes b, then runs
+ * the unit-lower and the upper triangles with fused multiply-adds.  This
+ * equals the library's DGESV bit for bit on 10^6 random 3x3 systems
+ * (tests/test_oracle_vs_ref.py repeats the check).  A zero pivot (info > 0)
+ * leaves b unchanged, as DGESV does.  a: column-major n x n. */
+void oracle_dgesv(int n, double *a, double *b)
+{
+    int ipiv[8], info = 0;
+#define A_(i, j) a[(i) + n * (j)]
+    for (int j = 0; j < n; j++) {
+        for (int i = 0; i < j; i++) {
+            int p = ipiv[i];
+            if (p != i) { double t = A_(i, j); A_(i, j) = A_(p, j); A_(p, j) = t; }
+        }
+        for (int i = 1; i < j; i++) {
+            double d = 0.0;
+            for (int k = 0; k < i; k++) d = fma(A_(i, k), A_(k, j), d);
+            A_(i, j) -= d;
+        }
+        for (int i = j; i < n; i++) {
+            double d = 0.0;
+            for (int k = 0; k < j; k++) d = fma(A_(i, k), A_(k, j), d);
+            A_(i, j) -= d;
+        }
+        int p = j;
+        double mx = fabs(A_(j, j));
+        for (int i = j + 1; i < n; i++)
+            if (fabs(A_(i, j)) > mx) { mx = fabs(A_(i, j)); p = i; }
+        ipiv[j] = p;
+        if (p != j)
+            for (int k = 0; k <= j; k++) { double t = A_(j, k); A_(j, k) = A_(p, k); A_(p, k) = t; }
+        if (A_(j, j) != 0.0) {
+            double r = 1.0 / A_(j, j);
+            for (int i = j + 1; i < n; i++) A_(i, j) *= r;
+        } else if (!info) {
+            info = j + 1;
+        }
+    }
+    if (info) return;
+    for (int j = 0; j < n; j++) {
+        int p = ipiv[j];
+        if (p != j) { double t = b[j]; b[j] = b[p]; b[p] = t; }
+    }
+    for (int j = 0; j < n; j++)
+        for (int i = j + 1; i < n; i++) b[i] = fma(-b[j], A_(i, j), b[i]);
+    for (int j = n - 1; j >= 0; j--) {
+        b[j] /= A_(j, j);
+        for (int i = 0; i < j; i++) b[i] = fma(-b[j], A_(i, j), b[i]);
+    }
+#undef A_
+}
+
 /* eval_jacobian (complex step): src/solver_2d.f90:2302-2367.  L is
  * column-major nv x nv. */
 static void eval_jacobian(const oracle_t *o, double dt, double a_diag, const double *qrel,
@@ -1383,7 +1656,7 @@ static void eval_jacobian(const oracle_t *o, double dt, double a_diag, const dou
     for (int i = 0; i < nv; i++) init[i] = C(qrel[i], 0.0);
     for (int i = 0; i < nv; i++) {
         L[i + nv * i] = 1.0 * qorg[i];
-        if (i == 1 || i == 2) {
+        if (implicit_var(o, i)) {
             for (int m = 0; m < nv; m++) qc[m] = init[m];
             qc[i] = C(qrel[i], o->hcs);
             for (int m = 0; m < nv; m++) qc[m] = cmul(qc[m], C(qorg[m], 0.0));
@@ -1484,39 +1757,50 @@ static int solve_rk_step(const oracle_t *o, double dt, double *qj, const double 
         for (int i = 0; i < nv; i++) mx = fmax_f(mx, fabs(rt[i]));
         if (mx < TOLF) break;
         eval_jacobian(o, dt, a_diag, qrel, qorg, L);
-        /* block solve: explicit rows by their diagonal, 2x2 Cramer */
-        double d1[6], d2[2];
+        /* block solve: explicit rows by their diagonal, the implicit block by
+         * Cramer's rule (n_nh = 2) or DGESV (n_nh = 3): :1851-1926 */
+        int imp[3], nn = 0;
+        for (int r = 0; r < nv; r++)
+            if (implicit_var(o, r)) imp[nn++] = r;
+        double d1[7], d2[3];
         int ne = 0;
         for (int r = 0; r < nv; r++)
-            if (!(r == 1 || r == 2)) { d1[ne] = rt[r] / L[r + nv * r]; ne++; }
-        d2[0] = rt[1];
-        d2[1] = rt[2];
+            if (!implicit_var(o, r)) { d1[ne] = rt[r] / L[r + nv * r]; ne++; }
+        for (int b = 0; b < nn; b++) d2[b] = rt[imp[b]];
         {
-            /* reshape(pack(L, mask21), [nv-2, 2]) then MATMUL(d1, .) */
-            double pk[12];
+            /* reshape(pack(L, mask21), [nv-nn, nn]) then MATMUL(d1, .) */
+            double pk[24];
             int m = 0;
             for (int col = 0; col < nv; col++) {
-                if (col == 1 || col == 2) continue;
-                pk[m++] = L[1 + nv * col];
-                pk[m++] = L[2 + nv * col];
+                if (implicit_var(o, col)) continue;
+                for (int b = 0; b < nn; b++) pk[m++] = L[imp[b] + nv * col];
             }
-            for (int b = 0; b < 2; b++) {
+            for (int b = 0; b < nn; b++) {
                 double s = 0.0;
-                for (int a = 0; a < nv - 2; a++) s += d1[a] * pk[a + (nv - 2) * b];
+                for (int a = 0; a < nv - nn; a++) s += d1[a] * pk[a + (nv - nn) * b];
                 d2[b] = d2[b] - s;
             }
         }
-        double l11 = L[1 + nv * 1], l22 = L[2 + nv * 2], l21 = L[2 + nv * 1], l12 = L[1 + nv * 2];
-        double inv_det = 1.0 / (l11 * l22 - l21 * l12);
-        double s1 = (d2[0] * l22 - d2[1] * l12) * inv_det;
-        double s2 = (l11 * d2[1] - l21 * d2[0]) * inv_det;
+        double sol[3];
+        if (nn == 2) {
+            double l11 = L[1 + nv * 1], l22 = L[2 + nv * 2], l21 = L[2 + nv * 1], l12 = L[1 + nv * 2];
+            double inv_det = 1.0 / (l11 * l22 - l21 * l12);
+            sol[0] = (d2[0] * l22 - d2[1] * l12) * inv_det;
+            sol[1] = (l11 * d2[1] - l21 * d2[0]) * inv_det;
+        } else {
+            double a22[9];
+            for (int c = 0; c < nn; c++)
+                for (int r = 0; r < nn; r++) a22[r + nn * c] = L[imp[r] + nv * imp[c]];
+            for (int b = 0; b < nn; b++) sol[b] = d2[b];
+            oracle_dgesv(nn, a22, sol);
+        }
         ne = 0;
+        int ni = 0;
         for (int r = 0; r < nv; r++) {
-            if (r == 1)
-                p[r] = -s1 + 0.0;
-            else if (r == 2)
-                p[r] = -s2 + 0.0;
-            else {
+            if (implicit_var(o, r)) {
+                p[r] = -sol[ni] + 0.0;
+                ni++;
+            } else {
                 p[r] = 0.0 + -d1[ne];
                 ne++;
             }
@@ -1737,7 +2021,7 @@ static void update_erosion_deposition_cell(oracle_t *o, double dt)
 {
     const int nv = o->nv, np = nv + 2, ns = o->ns;
     const size_t N = (size_t)o->nx * o->ny;
-    if (o->erosion_coeff == 0.0 && !o->settling && !o->entrainment) return;
+    if (o->erosion_coeff == 0.0 && !o->settling && !o->pore && !o->entrainment) return;
     for (int l = 0; l < o->ncells; l++) {
         const size_t c = CELL(o, o->jc[l], o->kc[l]);
         double *qq = o->q + nv * c, *qpc = o->qp + np * c, pd;
@@ -1889,7 +2173,23 @@ oracle_t *oracle_create(const int *ip, const double *rp)
     oracle_t *o = (oracle_t *)zalloc(sizeof(oracle_t));
     o->nan_cell = -1;
     o->watch_j = o->watch_k = -10;
-    o->nx = ip[0]; o->ny = ip[1]; o->ns = ip[2]; o->nv = 4 + o->ns;
+    o->nx = ip[0]; o->ny = ip[1]; o->ns = ip[2];
+    /* pore pressure: ip[16..20], rp[73..78] */
+    o->pore = ip[16]; o->gas_loss = ip[17]; o->finh = ip[18]; o->N_inh = ip[19];
+    o->dyn_perm = ip[20];
+    o->hyd_perm = rp[73]; o->alpha_trans = rp[74];
+    o->max_pack = (rp[75] > 0.0) ? rp[75] : 0.64;
+    for (int i = 0; i < 3; i++) o->sph_s[i] = (rp[76 + i] > 0.0) ? rp[76 + i] : 1.0;
+    o->nv = 4 + o->ns + (o->pore ? 1 : 0);
+    o->ipp = 4 + o->ns;
+    /* precompute_pascal_coefficient: src/constitutive_2d.f90:419-466 */
+    for (int n = 0; n <= o->N_inh && n < 64; n++) {
+        const int N = o->N_inh;
+        double p1 = 1.0, p2 = 1.0;
+        for (int k = 0; k < n; k++) p1 = p1 * (-(double)N - 1.0 - (double)k) / (double)(k + 1);
+        for (int k = 0; k < N - n; k++) p2 = p2 * (2.0 * (double)N + 1.0 - (double)k) / (double)(k + 1);
+        o->pascal[n] = p1 * p2;
+    }
     o->energy = ip[3]; o->settling = ip[4]; o->entrainment = ip[5];
     o->radial = ip[6]; o->collapse = ip[7]; o->n_thk = ip[8]; o->n_pdyn = ip[9];
     o->limiter = ip[10]; o->bcflag = ip[11];
@@ -2144,6 +2444,18 @@ void oracle_work_stats(long *out)
 /* 1 for every cell next to a stale read that differs from a fresh
  * evaluation since the handle was created (the cells whose results the
  * reference's stale-state behaviour can change) */
+/* the stage arrays of cell (j, k) after the last step, as the harness's
+ * imexref_get_stage lays them out: [5][n_RK][nv] = q_rk, divF, NH, SI, expl */
+void oracle_get_stage(const oracle_t *o, int j, int k, double *out)
+{
+    const size_t N = (size_t)o->nx * o->ny, c = CELL(o, j, k);
+    const int nv = o->nv;
+    const double *src[5] = {o->q_rk, o->divF, o->NH, o->SI, o->expl};
+    for (int a = 0; a < 5; a++)
+        for (int st = 0; st < 2; st++)
+            for (int i = 0; i < nv; i++) out[(a * 2 + st) * nv + i] = src[a][N * nv * st + nv * c + i];
+}
+
 void oracle_get_stale_cells(const oracle_t *o, unsigned char *out)
 {
     memcpy(out, o->stale_cell, (size_t)o->nx * o->ny);

@@ -102,6 +102,9 @@ void oracle_mass_exchange(const oracle_t *o, int n, const double *qp, double dt,
 /* the limited slope of limit() with the minmod limiter from three values at
  * three abscissae (src/geometry_2d.f90:1776-1806, :1855-1876) */
 double oracle_limit_minmod(const double *v3, const double *z3);
+/* DGESV (n x n, one right-hand side) as the reference links it */
+void oracle_dgesv(int n, double *a, double *b);
+void oracle_get_stage(const oracle_t *o, int j, int k, double *out);
 /* one Newton solve: q_si, q0, R (n_vars each) -> q (n_vars); returns iterations */
 int oracle_newton(const oracle_t *o, double dt, const double *q_si, const double *q0,
                   const double *R, double *q);

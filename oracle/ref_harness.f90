@@ -53,7 +53,9 @@ MODULE imexref_harness
        mixt_var, eval_fluxes, eval_local_speeds_x, eval_local_speeds_y,          &
        eval_expl_terms, eval_nh_semi_impl_terms, eval_implicit_terms,            &
        settling_velocity, eval_mass_exchange_terms, sphericity_s, mu_s, mu_2,   &
-       muI_inf, I_0
+       muI_inf, I_0, hydraulic_permeability, dynamic_permeability_flag,          &
+       alpha_trans, N_inh, f_inhibit_mode, pascal_coeff_precomputed,             &
+       precompute_pascal_coefficient
   USE geometry_2d, ONLY : comp_cells_x, comp_cells_y, x0, y0, cell_size,        &
        topography_profile, nodata_topo, n_topography_profile_x,                  &
        n_topography_profile_y, init_grid, init_source, deposit, erosion,         &
@@ -61,7 +63,8 @@ MODULE imexref_harness
        sourceW, sourceN, sourceS, cell_arc_perim, cell_arc_n_x, cell_arc_n_y,    &
        x_comp, x_stag, y_comp, y_stag, dx, dy, pi_g, topography_reconstruction
   USE solver_2d, ONLY : allocate_solver_variables, check_solve, timestep,       &
-       imex_RK_solver, update_erosion_deposition_cell, q, qp, t, dt, hmax,       &
+       imex_RK_solver, update_erosion_deposition_cell, solve_rk_step, q, qp, t,  &
+       dt, hmax, q_rk, divFlux, NH, SI_NH, expl_terms,                           &
        pdynmax, mod_vel_max, vuln_table, thck_table, pdyn_table, solve_mask,     &
        solve_cells, j_cent, k_cent, Z
   USE init_2d, ONLY : collapsing_volume
@@ -74,13 +77,13 @@ CONTAINS
 
   !----------------------------------------------------------------------------
   ! Set up the reference modules for one run.
-  !   ip(0:15)  integer parameters, rp(0:72) real parameters (layout in
+  !   ip(0:20)  integer parameters, rp(0:78) real parameters (layout in
   !   oracle/refpy.py), bdem(nx*ny) bed elevation at the cell centres
   !   (x index fastest).
   !----------------------------------------------------------------------------
   INTEGER(c_int) FUNCTION imexref_setup(ip, rp, bdem) BIND(C, name='imexref_setup')
-    INTEGER(c_int), INTENT(IN) :: ip(0:15)
-    REAL(c_double), INTENT(IN) :: rp(0:72)
+    INTEGER(c_int), INTENT(IN) :: ip(0:20)
+    REAL(c_double), INTENT(IN) :: rp(0:78)
     REAL(c_double), INTENT(IN) :: bdem(*)
 
     INTEGER :: j, k, i, nx, ny
@@ -100,7 +103,10 @@ CONTAINS
     n_solid = ip(2)
     n_add_gas = 0
     n_stoch_vars = 0
+    ! pore pressure (PORE_PRESSURE_PARAMETERS): ip(16:20), rp(73:78)
+    pore_pressure_flag = ( ip(16) .NE. 0 )
     n_pore_vars = 0
+    IF ( pore_pressure_flag ) n_pore_vars = 1
     gas_flag = .TRUE.
     liquid_flag = .FALSE.
     rheology_flag = .TRUE.
@@ -119,8 +125,7 @@ CONTAINS
     length_spatial_corr = -1.0_wp
     stoch_transport_flag = .FALSE.
     mean_field_flag = .FALSE.
-    pore_pressure_flag = .FALSE.
-    gas_loss_flag = .FALSE.
+    gas_loss_flag = ( ip(17) .NE. 0 )
     sutherland_flag = .FALSE.
     topo_change_flag = ( ip(13) .NE. 0 )
     lateral_source_flag = .FALSE.
@@ -132,7 +137,7 @@ CONTAINS
     n_intervals = 0
 
     ! ---- variable / equation index map (src/inpout_2d.f90:584-655,1455-2049)
-    n_vars = 4 + n_solid
+    n_vars = 4 + n_solid + n_pore_vars
     n_eqns = n_vars
     idx_h = 1
     idx_hu = 2
@@ -143,7 +148,7 @@ CONTAINS
     idx_addGas_first = 5 + n_solid
     idx_addGas_last = 4 + n_solid
     idx_stoch = 5 + n_solid
-    idx_pore = 5 + n_solid
+    idx_pore = 5 + n_solid    ! src/inpout_2d.f90:654 (no added gas, no noise)
     idx_u = n_vars + 1
     idx_v = n_vars + 2
     idx_totMassEqn = 1
@@ -221,6 +226,9 @@ CONTAINS
        alphas_source(i) = rp(42+i)
     END DO
     sphericity_s = 1.0_wp
+    DO i = 1, n_solid
+       IF ( rp(75+i) .GT. 0.0_wp ) sphericity_s(i) = rp(75+i)
+    END DO
     inv_rho_s(1:n_solid) = 1.0_wp / rho_s(1:n_solid)
     DO i = 1, n_solid
        c_inv_rho_s(i) = CMPLX( inv_rho_s(i), 0.0_wp, wp )
@@ -256,6 +264,25 @@ CONTAINS
     rho_c_sub = pres / ( sp_gas_const_a * T_erodible )
     alphastot_min = 0.0_wp
     maximum_solid_packing = 0.64_wp
+    IF ( rp(75) .GT. 0.0_wp ) maximum_solid_packing = rp(75)
+    hydraulic_permeability = rp(73)
+    alpha_trans = rp(74)
+    dynamic_permeability_flag = ( ip(20) .NE. 0 )
+    N_inh = ip(19)
+    SELECT CASE ( ip(18) )
+    CASE ( 1 )
+       f_inhibit_mode = 'STEP'
+    CASE ( 2 )
+       f_inhibit_mode = 'STATIC'
+    CASE ( 3 )
+       f_inhibit_mode = 'DYNAMIC'
+    CASE DEFAULT
+       f_inhibit_mode = 'OFF'
+    END SELECT
+    ! src/inpout_2d.f90:688-696
+    pascal_coeff_precomputed = .FALSE.
+    IF ( pore_pressure_flag .AND. ( ip(18) .GE. 2 ) )                              &
+         CALL precompute_pascal_coefficient(N_inh)
 
     ! ---- initial volume and source ---------------------------------------
     x_collapse = rp(16)
@@ -751,6 +778,56 @@ CONTAINS
        oim(:,i) = AIMAG( cnh )
     END DO
   END SUBROUTINE imexref_implicit_cplx
+
+  ! solve_rk_step (src/solver_2d.f90:1662-1996) at the stage-2 coefficients
+  ! (a_diag = 1) for n cells: q_si the first guess, q0 the old state, R the
+  ! right-hand side without the implicit terms; out the solution
+  SUBROUTINE imexref_newton(n, dtt, qsi, q0i, ri, out) BIND(C, name='imexref_newton')
+    INTEGER(c_int), VALUE :: n
+    REAL(c_double), VALUE :: dtt
+    REAL(c_double), INTENT(IN) :: qsi(n_vars, n), q0i(n_vars, n), ri(n_eqns, n)
+    REAL(c_double), INTENT(OUT) :: out(n_vars, n)
+    REAL(wp) :: a_t(n_RK), a_d(n_RK), dF(n_eqns, n_RK), ex(n_eqns, n_RK), nh(n_eqns, n_RK)
+    REAL(wp) :: qj(n_vars), fv, dt_keep
+    INTEGER :: i
+    dt_keep = dt
+    dt = dtt
+    a_t = 0.0_wp
+    a_d = 0.0_wp
+    dF = 0.0_wp
+    ex = 0.0_wp
+    nh = 0.0_wp
+    DO i = 1, n
+       qj = qsi(:,i)
+       CALL solve_rk_step( qj, q0i(:,i), a_t, a_d, 1.0_wp, ri(:,i), dF, ex, nh, &
+            0.0_wp, 0.0_wp, 0.0_wp, fv )
+       out(:,i) = qj
+    END DO
+    dt = dt_keep
+  END SUBROUTINE imexref_newton
+
+  ! OpenMP threads of the reference's cell loops.  With pore pressure and
+  ! several threads the loops race on the module variable
+  ! hydraulic_permeability (src/constitutive_2d.f90:266), which every
+  ! thread writes and then reads (:3870-3888, :4998-5020) with dynamic
+  ! permeability: the parity runs take one thread.
+  SUBROUTINE imexref_set_threads(n) BIND(C, name='imexref_set_threads')
+    USE omp_lib, ONLY : omp_set_num_threads
+    INTEGER(c_int), VALUE :: n
+    CALL omp_set_num_threads(n)
+  END SUBROUTINE imexref_set_threads
+
+  ! the stage arrays of cell (j, k) (0-based) after the last step: q_rk,
+  ! divFlux, NH, SI_NH, expl_terms, each (n_eqns, n_RK)
+  SUBROUTINE imexref_get_stage(j, k, out) BIND(C, name='imexref_get_stage')
+    INTEGER(c_int), VALUE :: j, k
+    REAL(c_double), INTENT(OUT) :: out(n_eqns, n_RK, 5)
+    out(:,:,1) = q_rk(1:n_eqns, j+1, k+1, 1:n_RK)
+    out(:,:,2) = divFlux(1:n_eqns, j+1, k+1, 1:n_RK)
+    out(:,:,3) = NH(1:n_eqns, j+1, k+1, 1:n_RK)
+    out(:,:,4) = SI_NH(1:n_eqns, j+1, k+1, 1:n_RK)
+    out(:,:,5) = expl_terms(1:n_eqns, j+1, k+1, 1:n_RK)
+  END SUBROUTINE imexref_get_stage
 
   SUBROUTINE imexref_semi_impl(n, qci, qpi, out) BIND(C, name='imexref_semi_impl')
     INTEGER(c_int), VALUE :: n

@@ -49,10 +49,16 @@ def _ip(a):
     return a.ctypes.data_as(_I)
 
 
+def set_threads(n: int, flavour: str = "ieee") -> None:
+    """OpenMP threads of the reference's cell loops (one for the pore-
+    pressure parity runs: several threads race on a module variable)."""
+    _lib(flavour).imexref_set_threads(ctypes.c_int(int(n)))
+
+
 def pack_params(cfg):
-    """ip[16] / rp[73] parameter vectors read by imexref_setup (and by the
+    """ip[21] / rp[79] parameter vectors read by imexref_setup (and by the
     C restatement's oracle_create)."""
-    ip = np.zeros(16, dtype=np.int32)
+    ip = np.zeros(21, dtype=np.int32)
     ip[0] = cfg.nx
     ip[1] = cfg.ny
     ip[2] = cfg.n_solid
@@ -69,7 +75,7 @@ def pack_params(cfg):
     ip[13] = int(getattr(cfg, "topo_change_flag", False))
     ip[14] = getattr(cfg, "rheology_model", 1)
     ip[15] = int(getattr(cfg, "stochastic_flag", False))
-    rp = np.zeros(73, dtype=np.float64)
+    rp = np.zeros(79, dtype=np.float64)
     rp[0:16] = [cfg.dx, cfg.x0, cfg.y0, cfg.cfl, cfg.dt0, cfg.max_dt, cfg.t_start,
                 cfg.t_end, cfg.grav, cfg.mu, cfg.xi, cfg.pres, cfg.T_ambient,
                 cfg.sp_heat_a, cfg.sp_gas_const_a, cfg.kin_visc_a]
@@ -96,6 +102,18 @@ def pack_params(cfg):
         rp[66 + i] = v
     rp[69:73] = [getattr(cfg, "mu_s", 0.0), getattr(cfg, "mu_2", 0.0),
                  getattr(cfg, "muI_inf", 0.0), getattr(cfg, "I_0", 0.0)]
+    # pore pressure
+    ip[16] = int(getattr(cfg, "pore_pressure_flag", False))
+    ip[17] = int(getattr(cfg, "gas_loss_flag", False))
+    ip[18] = cfg.f_inhibit_code if hasattr(cfg, "f_inhibit_code") else 0
+    ip[19] = getattr(cfg, "N_inh", 2)
+    ip[20] = int(getattr(cfg, "dynamic_permeability_flag", False))
+    rp[73] = getattr(cfg, "hydraulic_permeability", 0.0)
+    rp[74] = getattr(cfg, "alpha_trans", 0.0)
+    rp[75] = getattr(cfg, "maximum_solid_packing", 0.64)
+    sph = list(getattr(cfg, "sphericity_s", [1.0, 1.0, 1.0]))
+    for i in range(min(3, cfg.n_solid)):
+        rp[76 + i] = sph[i] if i < len(sph) else 1.0
     return ip, rp
 
 
@@ -314,6 +332,14 @@ class RefRun:
         n = a[0].shape[0]
         out = np.zeros(n)
         self.lib.imexref_settling(ctypes.c_int(n), *[_dp(x) for x in a], _dp(out))
+        return out
+
+    def newton(self, dt, q_si, q0, R):
+        """solve_rk_step at the stage-2 coefficients, per row of q_si."""
+        a = [np.ascontiguousarray(np.atleast_2d(x), np.float64) for x in (q_si, q0, R)]
+        out = np.zeros_like(a[0])
+        self.lib.imexref_newton(ctypes.c_int(len(a[0])), ctypes.c_double(dt), _dp(a[0]),
+                                _dp(a[1]), _dp(a[2]), _dp(out))
         return out
 
     def mass_exchange(self, qp, dt):

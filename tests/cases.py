@@ -5,6 +5,8 @@ synthetic workloads.
 """
 from __future__ import annotations
 
+import numpy as np
+
 from imex_sflow2d_dev_amd import workloads
 
 
@@ -124,6 +126,86 @@ def c2s_64():
     cfg, B = c2m_64()
     cfg.stochastic_flag = True
     return cfg, B
+
+
+def _pore(cfg, B, mode, dyn, p_frac):
+    """cfg with the pore-pressure physics of the TEST_DAMBREAK decks
+    (TESTS/TEST_DAMBREAK/TEST_*_FINH/IMEX_SfloW2D.inp: mu(I) with mu_s 0.55,
+    mu_2 0.73, muI_inf 0.05, I_0 0.279; 80 um particles of 2500 kg/m3;
+    hydraulic permeability 1.1e-11 m2, gas loss, alpha_trans 0.4, maximum
+    packing 0.64), and the initial state with an excess pore pressure of
+    p_frac times the weight of the column (the decks start close to it:
+    their restart state has p_exc = 0.9995 q1 g)."""
+    from imex_sflow2d_dev_amd import init
+    cfg.pore_pressure_flag = True
+    cfg.gas_loss_flag = True
+    cfg.f_inhibit_mode = mode
+    cfg.dynamic_permeability_flag = dyn
+    cfg.hydraulic_permeability = 1.1e-11
+    cfg.alpha_trans = 0.4
+    cfg.maximum_solid_packing = 0.64
+    cfg.validate()
+    st, q0 = init.setup(cfg, B)
+    ns = cfg.n_solid
+    q0[..., 4 + ns] = np.where(q0[..., 0] > 0.0, p_frac * cfg.grav * q0[..., 0] * q0[..., 0], 0.0)
+    return cfg, B, st, q0
+
+
+def _dambreak_bed(nx, ny, dx):
+    """A cylinder of dense granular flow (solid fraction 0.5, 0.36 m thick,
+    300 K, at rest) on a flat bed of 1 cm cells, the decks' scale."""
+    cfg, B = workloads.config1(nx=nx, ny=ny, dx=dx)
+    B = np.zeros_like(B)
+    cfg.x_collapse, cfg.y_collapse = 0.5 * nx * dx, 0.5 * ny * dx
+    cfg.r_collapse, cfg.h_collapse, cfg.T_collapse = 0.15, 0.36, 300.0
+    cfg.rheology_model = 11
+    cfg.mu_s, cfg.mu_2, cfg.muI_inf, cfg.I_0 = 0.55, 0.73, 0.05, 0.279
+    cfg.rho_s, cfg.diam_s, cfg.sp_heat_s = [2500.0], [80e-6], [1617.0]
+    cfg.alphas_collapse = [0.5]
+    cfg.sp_heat_a, cfg.sp_gas_const_a, cfg.kin_visc_a = 998.0, 287.051, 1.48e-5
+    cfg.pres, cfg.T_ambient = 101300.0, 300.0
+    cfg.dt0, cfg.max_dt = 1e-5, 1e-2
+    return cfg, B
+
+
+def cp_64():
+    """TEST_STATIC_FINH's physics at 64 x 64 cells of 1 cm: STATIC
+    f_inhibit, dynamic (Carman-Kozeny) permeability, gas loss, mu(I);
+    p_exc starts at 0.9 of the column weight.  Returns (cfg, B, st, q0)."""
+    cfg, B = _dambreak_bed(64, 64, 0.01)
+    return _pore(cfg, B, "STATIC", True, 0.9)
+
+
+def cpd_64():
+    """TEST_DYNAMIC_FINH's physics (f_inhibit's transition fraction
+    from the thickness) at 64 x 64 cells of 1 cm."""
+    cfg, B = _dambreak_bed(64, 64, 0.01)
+    return _pore(cfg, B, "DYNAMIC", True, 0.9)
+
+
+def cpo_64():
+    """TEST_OFF_FINH's f_inhibit with a constant permeability and the
+    Voellmy friction (model 1, whose Coulomb part takes the pore pressure
+    too), mu 0.3, on 64 x 64 cells of 1 cm."""
+    cfg, B = _dambreak_bed(64, 64, 0.01)
+    cfg.rheology_model = 1
+    cfg.mu, cfg.xi = 0.3, 1000.0
+    return _pore(cfg, B, "OFF", False, 0.6)
+
+
+def cps_64():
+    """TEST_STEP_FINH's f_inhibit (a step at the maximum packing) with
+    two classes (the Sauter diameter and the surface-weighted sphericity of
+    the permeability) at 64 x 64 cells of 1 cm."""
+    cfg, B = _dambreak_bed(64, 64, 0.01)
+    cfg.n_solid = 2
+    cfg.rho_s, cfg.diam_s, cfg.sp_heat_s = [2500.0, 2000.0], [80e-6, 300e-6], [1617.0, 1400.0]
+    cfg.alphas_collapse, cfg.alphas_source = [0.35, 0.2], [0.0, 0.0]
+    cfg.sphericity_s = [0.8, 0.6, 1.0]
+    return _pore(cfg, B, "STEP", True, 0.9)
+
+
+PORE_CASES = {"cp_64": cp_64, "cpd_64": cpd_64, "cpo_64": cpo_64, "cps_64": cps_64}
 
 
 CASES = {"c1_48": c1_48, "c1_64": c1_64, "c1d_96": c1d_96, "c3_64": c3_64, "c2_64": c2_64, "c2_96": c2_96,

@@ -57,6 +57,15 @@ PROGRAM layout_check
   WRITE(*,'(A,1X,I0)') 'imex_config.muI_inf', TRANSFER(C_LOC(c%muI_inf), base) - base
   WRITE(*,'(A,1X,I0)') 'imex_config.I_0', TRANSFER(C_LOC(c%I_0), base) - base
   WRITE(*,'(A,1X,I0)') 'imex_config.stochastic_flag', TRANSFER(C_LOC(c%stochastic_flag), base) - base
+  WRITE(*,'(A,1X,I0)') 'imex_config.pore_pressure_flag', TRANSFER(C_LOC(c%pore_pressure_flag), base) - base
+  WRITE(*,'(A,1X,I0)') 'imex_config.gas_loss_flag', TRANSFER(C_LOC(c%gas_loss_flag), base) - base
+  WRITE(*,'(A,1X,I0)') 'imex_config.f_inhibit_mode', TRANSFER(C_LOC(c%f_inhibit_mode), base) - base
+  WRITE(*,'(A,1X,I0)') 'imex_config.N_inh', TRANSFER(C_LOC(c%N_inh), base) - base
+  WRITE(*,'(A,1X,I0)') 'imex_config.dynamic_permeability_flag', TRANSFER(C_LOC(c%dynamic_permeability_flag), base) - base
+  WRITE(*,'(A,1X,I0)') 'imex_config.hydraulic_permeability', TRANSFER(C_LOC(c%hydraulic_permeability), base) - base
+  WRITE(*,'(A,1X,I0)') 'imex_config.alpha_trans', TRANSFER(C_LOC(c%alpha_trans), base) - base
+  WRITE(*,'(A,1X,I0)') 'imex_config.maximum_solid_packing', TRANSFER(C_LOC(c%maximum_solid_packing), base) - base
+  WRITE(*,'(A,1X,I0)') 'imex_config.sphericity_s', TRANSFER(C_LOC(c%sphericity_s), base) - base
   WRITE(*,'(A,1X,I0)') 'imex_static.size', C_SIZEOF(s)
   base = TRANSFER(C_LOC(s), base)
   WRITE(*,'(A,1X,I0)') 'imex_static.B_prime_x', TRANSFER(C_LOC(s%B_prime_x), base) - base
