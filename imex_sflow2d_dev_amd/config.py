@@ -12,7 +12,7 @@ from __future__ import annotations
 
 import dataclasses
 import math
-from typing import List
+from typing import List, Optional
 
 import numpy as np
 
@@ -116,6 +116,14 @@ class RunConfig:
     dynamic_permeability_flag: bool = False
     maximum_solid_packing: float = 0.64
     sphericity_s: List[float] = dataclasses.field(default_factory=lambda: [1.0, 1.0, 1.0])
+    # per-variable boundary conditions (WEST/EAST/SOUTH/NORTH_BOUNDARY_
+    # CONDITIONS, src/solver_2d.f90:3190-3484): bc_flags[side][var] with
+    # sides W, E, S, N and the conservative variables' order; flag 0
+    # Dirichlet value, 1 slope value, 2 one-sided slope.  None: flag 1,
+    # value 0 everywhere (the path's default; other boundaries are on the
+    # pore-pressure path only)
+    bc_flags: Optional[List[List[int]]] = None
+    bc_values: Optional[List[List[float]]] = None
     # vulnerability table (VULNERABILITY_TABLE_PARAMETERS)
     thickness_levels: List[float] = dataclasses.field(default_factory=list)
     dyn_pres_levels: List[float] = dataclasses.field(default_factory=list)
@@ -191,6 +199,14 @@ class RunConfig:
             raise ValueError("only the minmod limiter (1) is on this path")
         if self.bc_flag != 1:
             raise ValueError("only Neumann (flag 1) boundaries are on this path")
+        if self.bc_flags is not None:
+            if not self.pore_pressure_flag:
+                raise ValueError("per-variable boundary conditions are on the pore-pressure path only")
+            if (len(self.bc_flags) != 4 or len(self.bc_values) != 4
+                    or any(len(f) < self.n_vars for f in self.bc_flags)
+                    or any(len(v) < self.n_vars for v in self.bc_values)
+                    or any(x not in (0, 1, 2) for f in self.bc_flags for x in f[:self.n_vars])):
+                raise ValueError("bc_flags / bc_values: 4 sides x n_vars, flags 0, 1 or 2")
         if self.rheology_model not in (1, 11, 12):
             raise ValueError("rheology_model must be 1 (Voellmy-Salm), 11 or 12 (mu(I))")
         if self.rheology_model in (11, 12) and min(self.mu_s, self.mu_2, self.I_0,

@@ -2097,6 +2097,12 @@ extern "C" int imex_create(const imex_config* cfg, imex_handle** out) {
         (cfg->N_inh <= 0 || cfg->N_inh > 15 || !(cfg->alpha_trans > 0.0) || mp < cfg->alpha_trans))
       return -3;
   }
+  // boundaries other than zero gradient: the pore-pressure path only
+  if (cfg->bc_custom) {
+    if (!cfg->pore_pressure_flag) return -3;
+    for (int i = 0; i < 32; i++)
+      if (cfg->bc_flag[i] < 0 || cfg->bc_flag[i] > 2) return -3;
+  }
   imex_handle* h = new imex_handle();
   h->cfg = *cfg;
   h->fp32 = cfg->precision == 1;
@@ -2150,6 +2156,11 @@ extern "C" int imex_create(const imex_config* cfg, imex_handle** out) {
     }
   }
   P.max_pack = cfg->maximum_solid_packing > 0.0 ? cfg->maximum_solid_packing : 0.64;
+  for (int sd = 0; sd < 4; sd++)
+    for (int i = 0; i < 8; i++) {
+      P.bcf[sd][i] = cfg->bc_custom ? cfg->bc_flag[8 * sd + i] : 1;
+      P.bcv[sd][i] = cfg->bc_custom ? cfg->bc_value[8 * sd + i] : 0.0;
+    }
   P.energy = cfg->energy_flag;
   P.settling = cfg->settling_flag;
   P.entrainment = cfg->entrainment_flag;

@@ -52,6 +52,10 @@ struct Params {
   // q1 * p_exc after the solid classes (kernels instantiated with NX = 1)
   int pore, gas_loss, finh, N_inh, dyn_perm;  // finh: 0 OFF, 1 STEP, 2 STATIC, 3 DYNAMIC
   double hyd_perm, alpha_trans, max_pack, sph_s[3], pascal[16];
+  // per-variable boundary conditions of the pore-pressure path: side 0 W,
+  // 1 E, 2 S, 3 N; flag 0 Dirichlet value, 1 slope value, 2 one-sided slope
+  int bcf[4][8];
+  double bcv[4][8];
 };
 
 // Per-cell phase clocks of the update kernel, compiled in only by the

@@ -125,23 +125,58 @@ __global__ __launch_bounds__(PTHR) void k_pp_recon(Params P, Grid G, PoreBufs B)
       ys0 = G.yc[k - 1];
       ys2 = G.yc[k + 1];
     }
+    // boundary slopes (:3190-3290, :3362-3420): flag 0 limits against the
+    // Dirichlet value at the boundary face, 1 takes the value as the slope,
+    // 2 the one-sided difference; the boundary face takes a Dirichlet value
+    // (:3316-3356, :3452-3482)
 #pragma unroll
     for (int i = 0; i < NV; i++) {
       const double v1 = at(c, i);
-      double pr = 0.0;
-      if (j > 0 && j < nx - 1) pr = minmod3(at(c - 1, i), v1, at(c + 1, i), xs0, xs1, xs2);
+      double pr;
+      if (j == 0) {
+        const int fl = P.bcf[0][i];
+        pr = (fl == 0) ? minmod3(P.bcv[0][i], v1, at(c + 1, i), xs0, xs1, xs2)
+                       : ((fl == 1) ? P.bcv[0][i] : (at(c + 1, i) - v1) * P.one_by_dx);
+      } else if (j == nx - 1) {
+        const int fl = P.bcf[1][i];
+        pr = (fl == 0) ? minmod3(at(c - 1, i), v1, P.bcv[1][i], xs0, xs1, xs2)
+                       : ((fl == 1) ? P.bcv[1][i] : (v1 - at(c - 1, i)) * P.one_by_dx);
+      } else {
+        pr = minmod3(at(c - 1, i), v1, at(c + 1, i), xs0, xs1, xs2);
+      }
       double dq = 1.0 * P.dx2 * pr;
       W[i] = v1 - dq;
       E[i] = v1 + dq;
-      if (j == 0 && i == 1) W[i] = fmin_f(W[i], 0.0);
-      if (j == nx - 1 && i == 1) E[i] = fmax_f(E[i], 0.0);
-      pr = 0.0;
-      if (k > 0 && k < ny - 1) pr = minmod3(at(c - nx, i), v1, at(c + nx, i), ys0, ys1, ys2);
+      if (j == 0) {
+        if (P.bcf[0][i] == 0) W[i] = P.bcv[0][i];
+        else if (i == 1) W[i] = fmin_f(W[i], 0.0);
+      }
+      if (j == nx - 1) {
+        if (P.bcf[1][i] == 0) E[i] = P.bcv[1][i];
+        else if (i == 1) E[i] = fmax_f(E[i], 0.0);
+      }
+      if (k == 0) {
+        const int fl = P.bcf[2][i];
+        pr = (fl == 0) ? minmod3(P.bcv[2][i], v1, at(c + nx, i), ys0, ys1, ys2)
+                       : ((fl == 1) ? P.bcv[2][i] : (at(c + nx, i) - v1) * P.one_by_dy);
+      } else if (k == ny - 1) {
+        const int fl = P.bcf[3][i];
+        pr = (fl == 0) ? minmod3(at(c - nx, i), v1, P.bcv[3][i], ys0, ys1, ys2)
+                       : ((fl == 1) ? P.bcv[3][i] : (v1 - at(c - nx, i)) * P.one_by_dy);
+      } else {
+        pr = minmod3(at(c - nx, i), v1, at(c + nx, i), ys0, ys1, ys2);
+      }
       dq = 1.0 * P.dy2 * pr;
       S[i] = v1 - dq;
       Nn[i] = v1 + dq;
-      if (k == 0 && i == 2) S[i] = fmin_f(S[i], 0.0);
-      if (k == ny - 1 && i == 2) Nn[i] = fmax_f(Nn[i], 0.0);
+      if (k == 0) {
+        if (P.bcf[2][i] == 0) S[i] = P.bcv[2][i];
+        else if (i == 2) S[i] = fmin_f(S[i], 0.0);
+      }
+      if (k == ny - 1) {
+        if (P.bcf[3][i] == 0) Nn[i] = P.bcv[3][i];
+        else if (i == 2) Nn[i] = fmax_f(Nn[i], 0.0);
+      }
     }
     // u and v with the limiters of hu and hv; at a boundary the outer
     // stencil point is the face state's own velocity

@@ -65,6 +65,11 @@ MODULE imex_hip_binding
      INTEGER(c_int32_t) :: dynamic_permeability_flag
      REAL(c_double) :: hydraulic_permeability, alpha_trans, maximum_solid_packing
      REAL(c_double) :: sphericity_s(3)
+     ! per-variable boundary conditions (pore-pressure runs): with bc_custom,
+     ! bc_flag(8 side + var) and bc_value for the sides W, E, S, N
+     INTEGER(c_int32_t) :: bc_custom
+     INTEGER(c_int32_t) :: bc_flag(32)
+     REAL(c_double) :: bc_value(32)
   END TYPE imex_config
 
   ! typedef struct imex_static: borrowed pointers to nx*ny arrays (x

@@ -85,6 +85,8 @@ class ImexConfig(ctypes.Structure):
         ("dynamic_permeability_flag", ctypes.c_int32),
         ("hydraulic_permeability", ctypes.c_double), ("alpha_trans", ctypes.c_double),
         ("maximum_solid_packing", ctypes.c_double), ("sphericity_s", ctypes.c_double * 3),
+        ("bc_custom", ctypes.c_int32), ("bc_flag", ctypes.c_int32 * 32),
+        ("bc_value", ctypes.c_double * 32),
     ]
 
 
@@ -196,6 +198,12 @@ def make_config(cfg: RunConfig) -> ImexConfig:
     c.maximum_solid_packing = cfg.maximum_solid_packing
     for i in range(3):
         c.sphericity_s[i] = cfg.sphericity_s[i] if i < len(cfg.sphericity_s) else 1.0
+    if cfg.bc_flags is not None:
+        c.bc_custom = 1
+        for sd in range(4):
+            for i in range(cfg.n_vars):
+                c.bc_flag[8 * sd + i] = int(cfg.bc_flags[sd][i])
+                c.bc_value[8 * sd + i] = float(cfg.bc_values[sd][i])
     return c
 
 

@@ -94,6 +94,15 @@ typedef struct imex_config {
   int32_t N_inh;
   int32_t dynamic_permeability_flag;
   double hydraulic_permeability, alpha_trans, maximum_solid_packing, sphericity_s[3];
+  /* per-variable boundary conditions (WEST/EAST/SOUTH/NORTH_BOUNDARY_CONDITIONS,
+   * src/solver_2d.f90:3190-3484), pore-pressure runs only: with bc_custom,
+   * bc_flag[8 side + var] and bc_value[8 side + var] for the sides W, E, S,
+   * N and the conservative variables in order; flag 0 Dirichlet value, 1
+   * slope value, 2 one-sided slope.  Without it every flag is 1 with value
+   * 0 (the zero-gradient boundaries of the rest of the path). */
+  int32_t bc_custom;
+  int32_t bc_flag[32];
+  double bc_value[32];
 } imex_config;
 
 /* Per-cell static fields computed once by the host (src/geometry_2d.f90:555-895

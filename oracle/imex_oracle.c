@@ -105,6 +105,12 @@ struct oracle_s {
      * index ipp = 4 + ns (0-based) */
     int pore, gas_loss, finh, N_inh, dyn_perm, ipp;  /* finh: 0 OFF 1 STEP 2 STATIC 3 DYNAMIC */
     double hyd_perm, alpha_trans, max_pack, sph_s[3], pascal[64];
+    /* per-variable boundary conditions (bcW/bcE/bcS/bcN of the
+     * *_BOUNDARY_CONDITIONS namelists, src/solver_2d.f90:3190-3484):
+     * side 0 W, 1 E, 2 S, 3 N; flag 0 Dirichlet value, 1 slope value,
+     * 2 one-sided slope.  Without bc_custom every flag is 1 with value 0. */
+    int bc_custom, bcf[4][8];
+    double bcv[4][8];
 };
 
 /* Work counters of the stage-2 Newton solves and the settling iterations
@@ -1148,8 +1154,15 @@ static void reconstruction(oracle_t *o, const double *q_expl, const double *qp_e
                     st[0] = sbd[i];
                     st[2] = qp_expl[(size_t)np * (c + 1) + i];
                     pr_x[i] = limit_minmod(st, xs);
+                } else if (o->bcf[0][i] == 0) {
+                    st[0] = o->bcv[0][i];
+                    st[2] = qp_expl[(size_t)np * (c + 1) + i];
+                    pr_x[i] = limit_minmod(st, xs);
+                } else if (o->bcf[0][i] == 1) {
+                    pr_x[i] = o->bcv[0][i];
                 } else {
-                    pr_x[i] = 0.0; /* bcW%flag = 1, value 0 */
+                    pr_x[i] = (qp_expl[(size_t)np * (c + 1) + i] - qp_expl[(size_t)np * c + i]) *
+                              o->one_by_dx;
                 }
             } else if (j == nx - 1) {
                 xs[2] = o->x_stag[nx];
@@ -1158,8 +1171,15 @@ static void reconstruction(oracle_t *o, const double *q_expl, const double *qp_e
                     st[2] = sbd[i];
                     st[0] = qp_expl[(size_t)np * (c - 1) + i];
                     pr_x[i] = limit_minmod(st, xs);
+                } else if (o->bcf[1][i] == 0) {
+                    st[2] = o->bcv[1][i];
+                    st[0] = qp_expl[(size_t)np * (c - 1) + i];
+                    pr_x[i] = limit_minmod(st, xs);
+                } else if (o->bcf[1][i] == 1) {
+                    pr_x[i] = o->bcv[1][i];
                 } else {
-                    pr_x[i] = 0.0;
+                    pr_x[i] = (qp_expl[(size_t)np * c + i] - qp_expl[(size_t)np * (c - 1) + i]) *
+                              o->one_by_dx;
                 }
             } else {
                 xs[0] = o->x_comp[j - 1];
@@ -1182,21 +1202,41 @@ static void reconstruction(oracle_t *o, const double *q_expl, const double *qp_e
             qrecE[i] = st[1] + dq;
             if (j == 0) {
                 if (src2) qrecW[i] = sbd[i];
+                else if (o->bcf[0][i] == 0) qrecW[i] = o->bcv[0][i];
                 else if (i == 1) qrecW[i] = fmin_f(qrecW[i], 0.0);
             }
             if (j == nx - 1) {
                 if (src2) qrecE[i] = sbd[i];
+                else if (o->bcf[1][i] == 0) qrecE[i] = o->bcv[1][i];
                 else if (i == 1) qrecE[i] = fmax_f(qrecE[i], 0.0);
             }
             /* y direction */
             if (k == 0) {
                 ys[0] = o->y_stag[0];
                 ys[2] = o->y_comp[k + 1];
-                pr_y[i] = 0.0;
+                if (o->bcf[2][i] == 0) {
+                    st[0] = o->bcv[2][i];
+                    st[2] = qp_expl[(size_t)np * (c + nx) + i];
+                    pr_y[i] = limit_minmod(st, ys);
+                } else if (o->bcf[2][i] == 1) {
+                    pr_y[i] = o->bcv[2][i];
+                } else {
+                    pr_y[i] = (qp_expl[(size_t)np * (c + nx) + i] - qp_expl[(size_t)np * c + i]) *
+                              o->one_by_dy;
+                }
             } else if (k == ny - 1) {
                 ys[0] = o->y_comp[k - 1];
                 ys[2] = o->y_stag[ny];
-                pr_y[i] = 0.0;
+                if (o->bcf[3][i] == 0) {
+                    st[2] = o->bcv[3][i];
+                    st[0] = qp_expl[(size_t)np * (c - nx) + i];
+                    pr_y[i] = limit_minmod(st, ys);
+                } else if (o->bcf[3][i] == 1) {
+                    pr_y[i] = o->bcv[3][i];
+                } else {
+                    pr_y[i] = (qp_expl[(size_t)np * c + i] - qp_expl[(size_t)np * (c - nx) + i]) *
+                              o->one_by_dy;
+                }
             } else {
                 ys[0] = o->y_comp[k - 1];
                 ys[2] = o->y_comp[k + 1];
@@ -1216,8 +1256,14 @@ static void reconstruction(oracle_t *o, const double *q_expl, const double *qp_e
             dq = 1.0 * o->dy2 * pr_y[i];
             qrecS[i] = st[1] - dq;
             qrecN[i] = st[1] + dq;
-            if (k == 0 && i == 2) qrecS[i] = fmin_f(qrecS[i], 0.0);
-            if (k == ny - 1 && i == 2) qrecN[i] = fmax_f(qrecN[i], 0.0);
+            if (k == 0) {
+                if (o->bcf[2][i] == 0) qrecS[i] = o->bcv[2][i];
+                else if (i == 2) qrecS[i] = fmin_f(qrecS[i], 0.0);
+            }
+            if (k == ny - 1) {
+                if (o->bcf[3][i] == 0) qrecN[i] = o->bcv[3][i];
+                else if (i == 2) qrecN[i] = fmax_f(qrecN[i], 0.0);
+            }
         }
         for (int i = nv; i < nv + 2; i++) {
             /* x direction, u and v with the limiters of hu, hv */
@@ -2181,6 +2227,12 @@ oracle_t *oracle_create(const int *ip, const double *rp)
     o->max_pack = (rp[75] > 0.0) ? rp[75] : 0.64;
     for (int i = 0; i < 3; i++) o->sph_s[i] = (rp[76 + i] > 0.0) ? rp[76 + i] : 1.0;
     o->nv = 4 + o->ns + (o->pore ? 1 : 0);
+    o->bc_custom = ip[21];
+    for (int sd = 0; sd < 4; sd++)
+        for (int i = 0; i < 8; i++) {
+            o->bcf[sd][i] = o->bc_custom ? ip[22 + 8 * sd + i] : 1;
+            o->bcv[sd][i] = o->bc_custom ? rp[79 + 8 * sd + i] : 0.0;
+        }
     o->ipp = 4 + o->ns;
     /* precompute_pascal_coefficient: src/constitutive_2d.f90:419-466 */
     for (int n = 0; n <= o->N_inh && n < 64; n++) {

@@ -77,13 +77,13 @@ CONTAINS
 
   !----------------------------------------------------------------------------
   ! Set up the reference modules for one run.
-  !   ip(0:20)  integer parameters, rp(0:78) real parameters (layout in
+  !   ip(0:53)  integer parameters, rp(0:110) real parameters (layout in
   !   oracle/refpy.py), bdem(nx*ny) bed elevation at the cell centres
   !   (x index fastest).
   !----------------------------------------------------------------------------
   INTEGER(c_int) FUNCTION imexref_setup(ip, rp, bdem) BIND(C, name='imexref_setup')
-    INTEGER(c_int), INTENT(IN) :: ip(0:20)
-    REAL(c_double), INTENT(IN) :: rp(0:78)
+    INTEGER(c_int), INTENT(IN) :: ip(0:53)
+    REAL(c_double), INTENT(IN) :: rp(0:110)
     REAL(c_double), INTENT(IN) :: bdem(*)
 
     INTEGER :: j, k, i, nx, ny
@@ -193,6 +193,21 @@ CONTAINS
     bcE(1:n_vars)%value = 0.0_wp
     bcS(1:n_vars)%value = 0.0_wp
     bcN(1:n_vars)%value = 0.0_wp
+    ! per-variable boundary conditions (the *_BOUNDARY_CONDITIONS namelists):
+    ! ip(21) switches them on, flags ip(22 + 8 side + i - 1) and values
+    ! rp(79 + 8 side + i - 1), side 0 W, 1 E, 2 S, 3 N
+    IF ( ip(21) .NE. 0 ) THEN
+       DO i = 1, n_vars
+          bcW(i)%flag = ip(22 + i - 1)
+          bcE(i)%flag = ip(30 + i - 1)
+          bcS(i)%flag = ip(38 + i - 1)
+          bcN(i)%flag = ip(46 + i - 1)
+          bcW(i)%value = rp(79 + i - 1)
+          bcE(i)%value = rp(87 + i - 1)
+          bcS(i)%value = rp(95 + i - 1)
+          bcN(i)%value = rp(103 + i - 1)
+       END DO
+    END IF
 
     ! ---- physics constants (src/inpout_2d.f90:485,1313,1356,1379) -------
     grav = rp(8)

@@ -56,9 +56,9 @@ def set_threads(n: int, flavour: str = "ieee") -> None:
 
 
 def pack_params(cfg):
-    """ip[21] / rp[79] parameter vectors read by imexref_setup (and by the
+    """ip[54] / rp[111] parameter vectors read by imexref_setup (and by the
     C restatement's oracle_create)."""
-    ip = np.zeros(21, dtype=np.int32)
+    ip = np.zeros(54, dtype=np.int32)
     ip[0] = cfg.nx
     ip[1] = cfg.ny
     ip[2] = cfg.n_solid
@@ -75,7 +75,7 @@ def pack_params(cfg):
     ip[13] = int(getattr(cfg, "topo_change_flag", False))
     ip[14] = getattr(cfg, "rheology_model", 1)
     ip[15] = int(getattr(cfg, "stochastic_flag", False))
-    rp = np.zeros(79, dtype=np.float64)
+    rp = np.zeros(111, dtype=np.float64)
     rp[0:16] = [cfg.dx, cfg.x0, cfg.y0, cfg.cfl, cfg.dt0, cfg.max_dt, cfg.t_start,
                 cfg.t_end, cfg.grav, cfg.mu, cfg.xi, cfg.pres, cfg.T_ambient,
                 cfg.sp_heat_a, cfg.sp_gas_const_a, cfg.kin_visc_a]
@@ -114,6 +114,14 @@ def pack_params(cfg):
     sph = list(getattr(cfg, "sphericity_s", [1.0, 1.0, 1.0]))
     for i in range(min(3, cfg.n_solid)):
         rp[76 + i] = sph[i] if i < len(sph) else 1.0
+    # per-variable boundary conditions, sides W, E, S, N
+    bcf, bcv = getattr(cfg, "bc_flags", None), getattr(cfg, "bc_values", None)
+    if bcf is not None:
+        ip[21] = 1
+        for sd in range(4):
+            for i in range(cfg.n_vars):
+                ip[22 + 8 * sd + i] = bcf[sd][i]
+                rp[79 + 8 * sd + i] = bcv[sd][i]
     return ip, rp
 
 

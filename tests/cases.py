@@ -208,6 +208,43 @@ def cps_64():
 PORE_CASES = {"cp_64": cp_64, "cpd_64": cpd_64, "cpo_64": cpo_64, "cps_64": cps_64}
 
 
+def dambreak_deck(mode="STATIC"):
+    """The reference's TEST_DAMBREAK decks (TESTS/TEST_DAMBREAK/TEST_<mode>_FINH/
+    IMEX_SfloW2D.inp, f_inhibit_mode <mode>): 300 x 15 cells of 1 cm, flat
+    bed, mu(I) friction, one 80 um class, pore pressure with gas loss and
+    dynamic permeability, per-variable boundaries (h and h alpha_s one-sided
+    slope, flag 2; the others zero slope, flag 1), the deck's restart state
+    example_2D_0000.q_2d (committed as tests/golden/dambreak_q0.npz), to
+    T_END = 0.3 s.  Returns (cfg, B, st, q0)."""
+    from conftest import golden_path
+    from imex_sflow2d_dev_amd import init
+    from imex_sflow2d_dev_amd.config import RunConfig
+    cfg = RunConfig(nx=300, ny=15, dx=0.01, x0=0.0, y0=0.0)
+    cfg.t_start, cfg.t_end = 0.0, 0.3
+    cfg.dt0, cfg.max_dt, cfg.cfl = 1.0e-5, 1.0e-2, 0.23999999463558197
+    cfg.n_solid, cfg.energy_flag = 1, True
+    cfg.grav = 9.8100000000000005
+    cfg.rheology_model = 11
+    cfg.mu_s, cfg.mu_2, cfg.muI_inf, cfg.I_0 = 0.55, 0.73, 0.05, 0.279
+    cfg.rho_s, cfg.diam_s, cfg.sp_heat_s = [2500.0], [80.0e-6], [1617.0]
+    cfg.alphas_collapse, cfg.alphas_source = [0.0], [0.0]
+    cfg.collapsing_volume_flag = False
+    cfg.sp_heat_a, cfg.sp_gas_const_a, cfg.kin_visc_a = 998.0, 287.051, 1.48e-5
+    cfg.pres, cfg.T_ambient = 101300.0, 300.0
+    cfg.pore_pressure_flag, cfg.gas_loss_flag = True, True
+    cfg.hydraulic_permeability, cfg.alpha_trans = 1.10e-11, 0.4
+    cfg.f_inhibit_mode, cfg.dynamic_permeability_flag = mode, True
+    cfg.maximum_solid_packing = 0.640
+    side = [2, 1, 1, 1, 2, 1]
+    cfg.bc_flags = [list(side) for _ in range(4)]
+    cfg.bc_values = [[0.0] * 6 for _ in range(4)]
+    cfg.validate()
+    B = np.zeros((cfg.ny, cfg.nx))
+    st, _ = init.setup(cfg, B)
+    q0 = np.load(golden_path("dambreak_q0.npz"), allow_pickle=False)["q0"]
+    return cfg, B, st, q0
+
+
 CASES = {"c1_48": c1_48, "c1_64": c1_64, "c1d_96": c1d_96, "c3_64": c3_64, "c2_64": c2_64, "c2_96": c2_96,
          "c1e_48": c1e_48, "c2e_64": c2e_64, "c1m_64": c1m_64, "c2m_64": c2m_64,
          "c1s_64": c1s_64, "c2s_64": c2s_64}

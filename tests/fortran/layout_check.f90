@@ -66,6 +66,9 @@ PROGRAM layout_check
   WRITE(*,'(A,1X,I0)') 'imex_config.alpha_trans', TRANSFER(C_LOC(c%alpha_trans), base) - base
   WRITE(*,'(A,1X,I0)') 'imex_config.maximum_solid_packing', TRANSFER(C_LOC(c%maximum_solid_packing), base) - base
   WRITE(*,'(A,1X,I0)') 'imex_config.sphericity_s', TRANSFER(C_LOC(c%sphericity_s), base) - base
+  WRITE(*,'(A,1X,I0)') 'imex_config.bc_custom', TRANSFER(C_LOC(c%bc_custom), base) - base
+  WRITE(*,'(A,1X,I0)') 'imex_config.bc_flag', TRANSFER(C_LOC(c%bc_flag), base) - base
+  WRITE(*,'(A,1X,I0)') 'imex_config.bc_value', TRANSFER(C_LOC(c%bc_value), base) - base
   WRITE(*,'(A,1X,I0)') 'imex_static.size', C_SIZEOF(s)
   base = TRANSFER(C_LOC(s), base)
   WRITE(*,'(A,1X,I0)') 'imex_static.B_prime_x', TRANSFER(C_LOC(s%B_prime_x), base) - base

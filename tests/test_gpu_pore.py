@@ -84,3 +84,38 @@ def test_pore_host_driven_entry_points_refuse():
     assert L.imex_check_solve(s.h, 0) == -2
     assert L.imex_rk_step(s.h, native.ctypes.c_double(1e-4)) == -2
     s.close()
+
+
+DAMBREAK_MODES = {"static": "STATIC", "dynamic": "DYNAMIC", "off": "OFF", "step": "STEP"}
+# the correctly rounded run's distance from the reference's at T_END (see
+# tests/test_oracle_pore.py): 1e-10; 5e-8 for the DYNAMIC deck, whose two
+# roundings part at step 116 and end 2.1e-8 apart
+DAMBREAK_TOL = {"static": 1e-10, "dynamic": 5e-8, "off": 1e-10, "step": 1e-10}
+
+
+@pytest.mark.parametrize("key", sorted(DAMBREAK_MODES))
+def test_dambreak_decks_on_the_gpu(key):
+    """The reference's TEST_DAMBREAK decks (300 x 15 cells, their restart
+    state, per-variable boundaries: h and h alpha_s one-sided slopes) run on
+    the GPU to T_END = 0.3 s: bit for bit with the correctly rounded oracle,
+    within 1e-10 of the reference's own run (5e-8 for the DYNAMIC deck)."""
+    cfg, B, st, q0 = cases.dambreak_deck(DAMBREAK_MODES[key])
+    g = dict(np.load(golden_path(f"dambreak_{key}.npz"), allow_pickle=False))
+    steps = int(g["steps"])
+    print(f"dambreak deck {key}: {steps} steps to t_end {cfg.t_end}", flush=True)
+    s = Solver(cfg, st, q0, raise_on_error=False)
+    s.err = 0
+    dts = s.run_steps(steps + 3, t_end=cfg.t_end)
+    np.testing.assert_array_equal(dts[:steps], g["cr_dts"])
+    assert np.all(dts[steps:] == 0.0)  # the call stops at t_end
+    np.testing.assert_array_equal(s.q, g["cr_q"])
+    f = s.fields()
+    for k in ("hmax", "pdynmax", "mod_vel_max"):
+        np.testing.assert_array_equal(f[k], g["cr_field_" + k], err_msg=k)
+    assert s.err == 0
+    q, qr = s.q, g["q"]
+    w = q[..., 0] > 0.0
+    assert np.array_equal(w, qr[..., 0] > 0.0)
+    scale = np.maximum(np.abs(qr[w]).max(axis=0), 1e-300)
+    assert (np.abs(q[w] - qr[w]) / scale).max() <= DAMBREAK_TOL[key]
+    s.close()
