@@ -51,3 +51,23 @@ def test_fresh_mode_with_stale_reads_changes_only_the_layers_of_their_cells():
     assert np.array_equal(pa[:, ~stale], pb[:, ~stale])
     np.testing.assert_allclose(pa, pb, rtol=0, atol=1e-25)
     np.testing.assert_allclose(ea, eb, rtol=0, atol=1e-25)
+
+
+def test_c2_trajectory_fixture_fresh_and_stale_readings_agree():
+    """The two oracle runs of the committed 5000-step C2 fixture (the
+    reference's stale-state reading and the fresh reading the device uses)
+    are one run until the stale reads begin: the same dt, active cells and
+    error bits up to step 4710, the same q at every checkpoint, the same
+    deposit at every checkpoint before the first stale read, and the same
+    NaN steps."""
+    from conftest import golden_path
+    g = np.load(golden_path("c2_trajectory.npz"), allow_pickle=False)
+    n = 4710
+    np.testing.assert_array_equal(g["fresh_dts"][:n], g["dts"][:n])
+    np.testing.assert_array_equal(g["fresh_err"], g["err"])
+    np.testing.assert_array_equal(g["fresh_ncells"][:n], g["ncells"][:n])
+    np.testing.assert_array_equal(g["fresh_ck_q_sha"], g["ck_q_sha"])
+    clean = g["ck_stale_reads"] == 0
+    assert clean[:18].all() and not clean[18:].any()
+    np.testing.assert_array_equal(g["fresh_ck_dep_sha"][clean], g["ck_dep_sha"][clean])
+    np.testing.assert_array_equal(g["fresh_nan_step"], g["nan_step"])
