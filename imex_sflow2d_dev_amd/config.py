@@ -102,6 +102,12 @@ class RunConfig:
     sym_noise: float = 0.0
     noise_pow_val: float = -1.0
     length_spatial_corr: float = -1.0
+    # the transported noise (stoch_transport_flag, src/inpout_2d.f90:1466-1475):
+    # q1 * Z as a conserved variable after the solid classes, advected with
+    # the flow; its initial value is q1 times the noise (getSteadyStateZ,
+    # src/stochastic_module.f90:110-126), the friction keeps reading the
+    # fixed noise
+    stoch_transport_flag: bool = False
     # pore pressure (PORE_PRESSURE_PARAMETERS, src/inpout_2d.f90:389-391,
     # defaults :536-542; maximum_solid_packing and sphericity_s from
     # SOLID_TRANSPORT_PARAMETERS, defaults :819, :854): the conserved
@@ -134,7 +140,8 @@ class RunConfig:
     # ---- derived quantities ------------------------------------------------
     @property
     def n_vars(self) -> int:
-        return 4 + self.n_solid + (1 if self.pore_pressure_flag else 0)
+        return (4 + self.n_solid + (1 if self.stoch_transport_flag else 0)
+                + (1 if self.pore_pressure_flag else 0))
 
     @property
     def f_inhibit_code(self) -> int:
@@ -200,8 +207,9 @@ class RunConfig:
         if self.bc_flag != 1:
             raise ValueError("only Neumann (flag 1) boundaries are on this path")
         if self.bc_flags is not None:
-            if not self.pore_pressure_flag:
-                raise ValueError("per-variable boundary conditions are on the pore-pressure path only")
+            if not (self.pore_pressure_flag or self.stoch_transport_flag):
+                raise ValueError("per-variable boundary conditions are on the pore-pressure and "
+                                 "transported-noise path only")
             if (len(self.bc_flags) != 4 or len(self.bc_values) != 4
                     or any(len(f) < self.n_vars for f in self.bc_flags)
                     or any(len(v) < self.n_vars for v in self.bc_values)
@@ -222,6 +230,15 @@ class RunConfig:
             raise ValueError("maximum_solid_packing must be in (0, 1)")  # :1989-1995
         if any(not (0.0 < x <= 1.0) for x in self.sphericity_s[:self.n_solid]):
             raise ValueError("sphericity_s must be in (0, 1]")  # :1797-1803
+        if self.stoch_transport_flag:
+            if not self.stochastic_flag:
+                raise ValueError("stoch_transport_flag needs stochastic_flag")
+            if self.radial_source_flag or self.erosion_coeff > 0.0 or self.settling_flag \
+                    or self.entrainment_flag:
+                raise ValueError("the transported noise with the radial source, settling, "
+                                 "entrainment or erosion is not on this path")
+            if self.n_vars > 8:
+                raise ValueError("the transported noise: at most 8 conserved variables")
         if self.pore_pressure_flag:
             if self.f_inhibit_mode in ("STATIC", "DYNAMIC"):
                 if self.N_inh <= 0 or self.alpha_trans <= 0.0:

@@ -2097,9 +2097,18 @@ extern "C" int imex_create(const imex_config* cfg, imex_handle** out) {
         (cfg->N_inh <= 0 || cfg->N_inh > 15 || !(cfg->alpha_trans > 0.0) || mp < cfg->alpha_trans))
       return -3;
   }
-  // boundaries other than zero gradient: the pore-pressure path only
+  // the transported noise (src/inpout_2d.f90:1466-1475): with the
+  // stochastic friction, on the general path, which takes what the
+  // pore-pressure runs take; the boundary tables hold 8 variables
+  if (cfg->stoch_transport_flag) {
+    if (!cfg->stochastic_flag || cfg->radial_source_flag || cfg->settling_flag ||
+        cfg->entrainment_flag || cfg->erosion_coeff != 0.0 || cfg->topo_change_flag ||
+        cfg->precision != 0 || 4 + cfg->n_solid + 1 + (cfg->pore_pressure_flag ? 1 : 0) > 8)
+      return -3;
+  }
+  // boundaries other than zero gradient: the general path only
   if (cfg->bc_custom) {
-    if (!cfg->pore_pressure_flag) return -3;
+    if (!cfg->pore_pressure_flag && !cfg->stoch_transport_flag) return -3;
     for (int i = 0; i < 32; i++)
       if (cfg->bc_flag[i] < 0 || cfg->bc_flag[i] > 2) return -3;
   }
@@ -2136,8 +2145,10 @@ extern "C" int imex_create(const imex_config* cfg, imex_handle** out) {
   P.nx = cfg->nx;
   P.ny = cfg->ny;
   P.ns = cfg->n_solid;
-  P.nv = 4 + cfg->n_solid + (cfg->pore_pressure_flag ? 1 : 0);
+  P.nv = 4 + cfg->n_solid + (cfg->stoch_transport_flag ? 1 : 0) + (cfg->pore_pressure_flag ? 1 : 0);
   P.pore = cfg->pore_pressure_flag != 0;
+  P.stoch_tr = cfg->stoch_transport_flag != 0;
+  P.general = P.pore || P.stoch_tr;
   if (P.pore) {
     P.gas_loss = cfg->gas_loss_flag != 0;
     P.finh = cfg->f_inhibit_mode;
@@ -2314,7 +2325,7 @@ extern "C" int imex_create(const imex_config* cfg, imex_handle** out) {
     HCHK(h, hipStreamSynchronize(h->stream));
   }
   if ((rc = frame_band(h, nullptr))) return rc;
-  if (P.pore && (rc = imex_pore_alloc(h))) return rc;
+  if (P.general && (rc = imex_pore_alloc(h))) return rc;
   h->lds = lds_for(ns, h->fp32);
   {
     int max_lds = 0;
@@ -2520,10 +2531,12 @@ extern "C" int imex_upload_state(imex_handle* h, const double* q_aos, double t, 
   return 0;
 }
 
-// pore-pressure runs step through imex_run_steps only
+// runs on the general path (pore pressure, the transported noise) step
+// through imex_run_steps only
 static int device_driven_only(imex_handle* h) {
-  if (h->P.pore) {
-    h->err = "pore_pressure_flag: runs with pore pressure take device-driven steps (imex_run_steps)";
+  if (h->P.general) {
+    h->err = "pore_pressure_flag / stoch_transport_flag: these runs take device-driven steps "
+             "(imex_run_steps)";
     return -2;
   }
   return 0;
@@ -2702,7 +2715,7 @@ extern "C" int imex_run_steps(imex_handle* h, int nsteps, double t_end, double* 
   }
   int rc;
   for (int n = 0; n < nsteps; n++) {
-    if (h->P.pore) {
+    if (h->P.general) {
       // the general path: every tile's active set, then imex_pore.hip's launches
       h->mask_full_next = true;
       if ((rc = mask(h, 0, 1))) return rc;
@@ -2991,7 +3004,7 @@ extern "C" int imex_batch_set_launch(imex_batch* b, int flux_threads, int graphs
 extern "C" int imex_batch_add(imex_batch* b, imex_handle* h, double t_end) {
   // members step together with no per-member slope pass: no topography change
   if (!b || !h || h->P.ns != b->ns || h->device != b->device || h->fp32 != b->fp32 ||
-      h->P.topo_change || h->P.pore)
+      h->P.topo_change || h->P.general)
     return -2;
   for (imex_handle* x : b->mem)
     if (x == h) return -2;

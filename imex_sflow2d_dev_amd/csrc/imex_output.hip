@@ -517,10 +517,10 @@ extern "C" int imex_snapshot_cols(imex_handle* h, unsigned kind) {
 }
 
 // the output kernels are instantiated for the n_solid + 4 variables of runs
-// without pore pressure
+// without pore pressure or the transported noise
 static int no_pore(imex_handle* h, const char* what) {
-  if (h->P.pore) {
-    h->err = std::string(what) + ": not built for runs with pore pressure";
+  if (h->P.general) {
+    h->err = std::string(what) + ": not built for runs with pore pressure or the transported noise";
     return -2;
   }
   return 0;

@@ -49,8 +49,11 @@ struct Params {
   double thk[8], pdyn[8];
   double hcs, one_by_hcs;  // complex step n_vars*eps and its reciprocal
   // pore pressure (PORE_PRESSURE_PARAMETERS): the conserved variable
-  // q1 * p_exc after the solid classes (kernels instantiated with NX = 1)
+  // q1 * p_exc after the solid classes (and after the transported noise)
   int pore, gas_loss, finh, N_inh, dyn_perm;  // finh: 0 OFF, 1 STEP, 2 STATIC, 3 DYNAMIC
+  // the transported noise (stoch_transport_flag): q1 * Z after the solid
+  // classes; general: pore or stoch_tr, the runs of imex_pore.hip's path
+  int stoch_tr, general;
   double hyd_perm, alpha_trans, max_pack, sph_s[3], pascal[16];
   // per-variable boundary conditions of the pore-pressure path: side 0 W,
   // 1 E, 2 S, 3 N; flag 0 Dirichlet value, 1 slope value, 2 one-sided slope
@@ -289,18 +292,29 @@ __device__ inline cx cpow_c(cx z, double c) {
 // T(...) below is an identity and the operations are the reference's), float
 // in the FP32 form's flux kernel.
 
+// NX: the extra conserved variables after the solid classes, as bits
+// (0 everywhere but the general path of csrc/imex_pore.hip): 2 the
+// transported noise q1 * Z at index 4 + NS (idx_stoch), 1 the pore pressure
+// q1 * p_exc after it (idx_pore; src/inpout_2d.f90:642-655)
+constexpr int nx_count(int NX) { return (NX & 1) + ((NX >> 1) & 1); }
+constexpr bool has_pore(int NX) { return (NX & 1) != 0; }
+constexpr bool has_stoch(int NX) { return (NX & 2) != 0; }
+template <int NS, int NX>
+constexpr int pore_at() { return 4 + NS + ((NX >> 1) & 1); }
+template <int NS>
+constexpr int stoch_at() { return 4 + NS; }
+
 // r_phys_var (configured branch) + qc_to_qp:
 // src/constitutive_2d.f90:550-976 and :1637-1690.  Also returns p_dyn.
-// NX: extra conserved variables after the solid classes (1: the pore
-// pressure q1 * p_exc at index 4 + NS; 0 everywhere else).
 template <int NS, typename T = double, int NX = 0>
 __device__ inline void qc_to_qp(const Params& P, const T* qc, T* qp, T& p_dyn) {
-  constexpr int NV = 4 + NS + NX;
+  constexpr int NV = 4 + NS + nx_count(NX), PI = pore_at<NS, NX>(), SI = stoch_at<NS>();
   T xs[NS];
-  T inv_qj1, pexc = T(0.0);
+  T inv_qj1, pexc = T(0.0), zs = T(0.0);
   if (qc[0] > T(kEps)) {
     inv_qj1 = T(1.0) / qc[0];
-    if constexpr (NX == 1) pexc = qc[4 + NS] * inv_qj1;  // :659
+    if constexpr (has_stoch(NX)) zs = qc[SI] * inv_qj1;  // :657
+    if constexpr (has_pore(NX)) pexc = qc[PI] * inv_qj1;  // :659
 #pragma unroll
     for (int i = 0; i < NS; i++) xs[i] = qc[4 + i] * inv_qj1;
     T s = T(0.0);
@@ -368,7 +382,8 @@ __device__ inline void qc_to_qp(const Params& P, const T* qc, T* qp, T& p_dyn) {
   qp[3] = Tm;
 #pragma unroll
   for (int i = 0; i < NS; i++) qp[4 + i] = xs[i] * rm * T(P.inv_rho_s[i]) * h;
-  if constexpr (NX == 1) qp[4 + NS] = pexc;  // :1683
+  if constexpr (has_stoch(NX)) qp[SI] = zs;  // :1681
+  if constexpr (has_pore(NX)) qp[PI] = pexc;  // :1683
   qp[NV] = u;
   qp[NV + 1] = v;
 }
@@ -376,7 +391,7 @@ __device__ inline void qc_to_qp(const Params& P, const T* qc, T* qp, T& p_dyn) {
 // qp_to_qc (configured branch): src/constitutive_2d.f90:1718-2149
 template <int NS, typename T = double, int NX = 0>
 __device__ inline void qp_to_qc(const Params& P, const T* qp, T* qc) {
-  constexpr int NV = 4 + NS + NX;
+  constexpr int NV = 4 + NS + nx_count(NX), PI = pore_at<NS, NX>(), SI = stoch_at<NS>();
   T h = qp[0];
   if (!(h > T(kEps))) {
 #pragma unroll
@@ -426,7 +441,8 @@ __device__ inline void qp_to_qc(const Params& P, const T* qp, T* qc) {
   }
 #pragma unroll
   for (int i = 0; i < NS; i++) qc[4 + i] = xs[i] * qc[0];
-  if constexpr (NX == 1) qc[4 + NS] = qp[4 + NS] * qc[0];  // :1981, :2143
+  if constexpr (has_stoch(NX)) qc[SI] = qp[SI] * qc[0];  // :1979, :2141
+  if constexpr (has_pore(NX)) qc[PI] = qp[PI] * qc[0];  // :1981, :2143
 }
 
 template <typename T = double>
@@ -438,7 +454,7 @@ using Mixt = MixtT<double>;
 // mixt_var (+ eval_sp_heat when sp_heat): src/constitutive_2d.f90:1342-1607
 template <int NS, typename T = double, int NX = 0>
 __device__ inline MixtT<T> mixt_var(const Params& P, const T* qp, bool sp_heat) {
-  constexpr int NV = 4 + NS + NX;
+  constexpr int NV = 4 + NS + nx_count(NX);
   MixtT<T> m;
   T h = qp[0];
   if (qp[0] <= T(kEps)) {
@@ -493,7 +509,7 @@ __device__ inline MixtT<T> mixt_var(const Params& P, const T* qp, bool sp_heat) 
 // eval_local_speeds_x / _y with grav_coeff = 1: src/constitutive_2d.f90:2203-2307
 template <int NS, typename T = double, int NX = 0>
 __device__ inline void local_speeds(const Params& P, const T* qp, int dir, T& vmin, T& vmax) {
-  constexpr int NV = 4 + NS + NX;
+  constexpr int NV = 4 + NS + nx_count(NX);
   MixtT<T> m = mixt_var<NS, T, NX>(P, qp, false);
   T h = qp[0];
   T w = (dir == 1) ? qp[NV] : qp[NV + 1];
@@ -511,7 +527,7 @@ __device__ inline void local_speeds(const Params& P, const T* qp, int dir, T& vm
 // eval_fluxes with grav_coeff = 1: src/constitutive_2d.f90:2325-2518
 template <int NS, typename T = double, int NX = 0>
 __device__ inline void phys_flux(const Params& P, const T* qc, const T* qp, int dir, T* f) {
-  constexpr int NV = 4 + NS + NX;
+  constexpr int NV = 4 + NS + nx_count(NX);
   if (qp[0] > T(kEps)) {
     T h = qp[0];
     T w = (dir == 1) ? qp[NV] : qp[NV + 1];
@@ -535,8 +551,9 @@ __device__ inline void phys_flux(const Params& P, const T* qc, const T* qp, int 
 #pragma unroll
       for (int i = 0; i < NS; i++) f[4 + i] = f[4 + i] / S * f[0];
     }
-    // flux of q1 * p_exc: :2435-2439, :2497-2501
-    if constexpr (NX == 1) f[4 + NS] = w * qc[4 + NS] * T(1.0);
+    // fluxes of q1 * Z and q1 * p_exc: :2428-2439, :2490-2501
+    if constexpr (has_stoch(NX)) f[stoch_at<NS>()] = w * qc[stoch_at<NS>()] * T(1.0);
+    if constexpr (has_pore(NX)) f[pore_at<NS, NX>()] = w * qc[pore_at<NS, NX>()] * T(1.0);
   } else {
 #pragma unroll
     for (int i = 0; i < NV; i++) f[i] = T(0.0);
@@ -550,7 +567,7 @@ __device__ inline void phys_flux(const Params& P, const T* qc, const T* qp, int 
 template <int NS, typename T = double, int NX = 0>
 __device__ inline void expl_terms(const Params& P, T Bx, T By, const T* qp, T* ex, double time,
                                   T arc_p, T arc_x, T arc_y) {
-  constexpr int NV = 4 + NS + NX;
+  constexpr int NV = 4 + NS + nx_count(NX);
 #pragma unroll
   for (int i = 0; i < NV; i++) ex[i] = T(0.0);
   if ((qp[0] <= T(kEps)) && (arc_p == T(0.0))) return;
@@ -650,7 +667,7 @@ __device__ inline double mu_of_I(const Params& P, double diam, double rhop, doub
 // 11 and 12 (:4384-4537)
 template <int NS, int NX = 0>
 __device__ inline void semi_impl(const Params& P, const double* qp, double* si) {
-  constexpr int NV = 4 + NS + NX;
+  constexpr int NV = 4 + NS + nx_count(NX);
 #pragma unroll
   for (int i = 0; i < NV; i++) si[i] = 0.0;
   double h = qp[0], u = qp[NV], v = qp[NV + 1];
@@ -661,7 +678,8 @@ __device__ inline void semi_impl(const Params& P, const double* qp, double* si) 
     if (mod_vel > 0.0) {
       double tt = P.mu * m.rho_m * (m.red_grav + 0.0) * h * sqrt(1.0);
       tt = fmax_f(0.0, tt);
-      if constexpr (NX == 1) tt = fmax_f(0.0, tt - P.mu * sqrt(1.0) * qp[4 + NS]);  // :4191-4198
+      if constexpr (has_pore(NX))  // :4191-4198
+        tt = fmax_f(0.0, tt - P.mu * sqrt(1.0) * qp[pore_at<NS, NX>()]);
       tt = fmax_f(0.0, tt);
       si[1] = si[1] - tt * u / mod_vel;
       si[2] = si[2] - tt * v / mod_vel;
@@ -679,7 +697,8 @@ __device__ inline void semi_impl(const Params& P, const double* qp, double* si) 
     }
     if (h < kEps) h = kEps;
     double vse = m.rho_m * (m.red_grav + 0.0) * h;
-    if constexpr (NX == 1) vse = m.rho_m * (m.red_grav + 0.0) * h - qp[4 + NS];  // :4423-4427
+    if constexpr (has_pore(NX))  // :4423-4427
+      vse = m.rho_m * (m.red_grav + 0.0) * h - qp[pore_at<NS, NX>()];
     vse = fmax_f(vse, 0.0);
     double ens = fmax_f(0.0, vse * sqrt(1.0));
     if (ens < kEps) ens = kEps;
@@ -812,11 +831,11 @@ __device__ inline void implicit_c(const Params& P, const cx* qj, cx& nh1, cx& nh
   nh1 = C(0.0, 0.0);
   nh2 = C(0.0, 0.0);
   // mu(I) model 11 has no implicit friction term (:3660 takes models 1, 12)
-  if (NX == 0 && P.rheology == 11) return;
+  if (!has_pore(NX) && P.rheology == 11) return;
   cx u, v, rho_m, T, hh, exc, alphas[NS];
   if (qj[0].re > kEps) {
     cx inv_cqj1 = cdiv(C(1.0, 0.0), qj[0]);
-    exc = (NX == 1) ? cmul(qj[4 + NS], inv_cqj1) : C(0.0, 0.0);  // :1195
+    exc = has_pore(NX) ? cmul(qj[pore_at<NS, NX>()], inv_cqj1) : C(0.0, 0.0);  // :1195
     cx xs[NS];
 #pragma unroll
     for (int i = 0; i < NS; i++) xs[i] = cmul(qj[4 + i], inv_cqj1);
@@ -847,7 +866,7 @@ __device__ inline void implicit_c(const Params& P, const cx* qj, cx& nh1, cx& nh
     for (int i = 0; i < NS; i++) d2 = cadd(d2, cmul(cconj(xs[i]), C(P.inv_rho_s[i], 0.0)));
     cx inv_rhom = cadd(d2, cmul(xc, inv_rho_c));
     rho_m = cdiv(C(1.0, 0.0), inv_rhom);
-    if constexpr (NX == 1) {  // :1286-1290
+    if constexpr (has_pore(NX)) {  // :1286-1290
 #pragma unroll
       for (int i = 0; i < NS; i++) alphas[i] = cmul(cmul(rho_m, xs[i]), C(P.inv_rho_s[i], 0.0));
       hh = cmul(qj[0], inv_rhom);
@@ -883,7 +902,7 @@ __device__ inline void implicit_c(const Params& P, const cx* qj, cx& nh1, cx& nh
       nh2 = csub(nh2, cmul(ts, cdiv(v, mod_vel)));
     }
   }
-  if constexpr (NX == 1) *nhp = pore_source_c<NS>(P, hh, T, rho_m, alphas, exc);
+  if constexpr (has_pore(NX)) *nhp = pore_source_c<NS>(P, hh, T, rho_m, alphas, exc);
 }
 
 // The residual evaluation of the Newton solve calls the implicit terms at a
@@ -1035,7 +1054,7 @@ __device__ inline bool mass_exchange(const Params& P, const double* qp, const do
                                      double dt, double* dep, double* eq, double* ers,
                                      double& topo, const ddm::dd* tab, int lane = 0,
                                      PhaseClk* pc = nullptr) {
-  constexpr int NV = 4 + NS + NX;
+  constexpr int NV = 4 + NS + nx_count(NX);
   topo = 0.0;
 #pragma unroll
   for (int i = 0; i < NS; i++) {
@@ -1140,12 +1159,12 @@ __device__ inline bool mass_exchange(const Params& P, const double* qp, const do
   for (int i = 0; i < NS; i++) sdep += dep[i];
   double cpl = 0.0 + P.coeff_porosity * sdep;
   double max_pack = 0.64;
-  if constexpr (NX == 1) {
+  if constexpr (has_pore(NX)) {
     max_pack = P.max_pack;
     // gas loss driven by the excess pore pressure, inhibited by f_inhibit:
     // src/constitutive_2d.f90:4887-5032
     if (P.gas_loss && at < P.max_pack) {
-      const double pexc = qp[4 + NS];
+      const double pexc = qp[pore_at<NS, NX>()];
       double f_inh = 1.0;
       if (P.finh == 1) {
         f_inh = (at < P.max_pack) ? 1.0 : 0.0;
@@ -1202,7 +1221,8 @@ __device__ inline bool mass_exchange(const Params& P, const double* qp, const do
             P.T_amb * P.cp_a * P.rho_a_amb * air;
 #pragma unroll
   for (int i = 0; i < NS; i++) eq[4 + i] = P.rho_s[i] * (ers[i] - dep[i]);
-  if constexpr (NX == 1) eq[4 + NS] = eq[0] * qp[4 + NS];  // :5140-5145
+  if constexpr (has_stoch(NX)) eq[stoch_at<NS>()] = eq[0] * qp[stoch_at<NS>()];  // :5132-5137
+  if constexpr (has_pore(NX)) eq[pore_at<NS, NX>()] = eq[0] * qp[pore_at<NS, NX>()];  // :5140-5145
   // rate of change of the topography (src/constitutive_2d.f90:5149), used
   // only with topography change
   if (P.topo_change) {

@@ -1,4 +1,11 @@
-// The time step of runs with pore pressure (pore_pressure_flag) on gfx950.
+// The time step of runs with pore pressure (pore_pressure_flag) or the
+// transported noise (stoch_transport_flag) on gfx950: the general path.
+//
+// The transported noise adds the conserved variable q1 * Z after the solid
+// classes (idx_stoch, before the pore pressure): a passive tracer advected
+// with the flow (src/constitutive_2d.f90:2428-2431, :2490-2493), an explicit
+// row of the Newton system, and the source eqns_term(1) * Z of the mass
+// exchange (:5132-5137); the friction keeps reading the fixed noise plane.
 //
 // Pore pressure adds a conserved variable, q1 * p_exc, after the solid
 // classes (n_vars = 5 + n_solid), a third implicit equation (its decay by
@@ -50,9 +57,9 @@ __device__ __forceinline__ bool step_done(const Grid& G) { return G.st->done != 
 
 // primitive state of one cell: qc_to_qp when wet, else zeros and T_ambient
 // (src/solver_2d.f90:944-957 for timestep, :1300-1308 for the stages)
-template <int NS>
+template <int NS, int NX>
 __global__ __launch_bounds__(PTHR) void k_pp_prim(Params P, Grid G, PoreBufs B) {
-  constexpr int NV = 5 + NS, NP = NV + 2;
+  constexpr int NV = 4 + NS + nx_count(NX), NP = NV + 2;
   if (step_done(G)) return;
   const int N = G.N;
   for (int c = blockIdx.x * PTHR + threadIdx.x; c < N; c += gridDim.x * PTHR) {
@@ -60,7 +67,7 @@ __global__ __launch_bounds__(PTHR) void k_pp_prim(Params P, Grid G, PoreBufs B) 
 #pragma unroll
     for (int i = 0; i < NV; i++) q[i] = G.q[(size_t)i * N + c];
     if (q[0] > 0.0) {
-      qc_to_qp<NS, double, 1>(P, q, qp, pd);
+      qc_to_qp<NS, double, NX>(P, q, qp, pd);
     } else {
 #pragma unroll
       for (int i = 0; i < NP; i++) qp[i] = 0.0;
@@ -91,9 +98,9 @@ __device__ __forceinline__ double minmod3(double v0, double v1, double v2, doubl
 // boundary flags 1 with value 0, no lateral source): its W, E, S and N face
 // states into the face arrays, the boundary faces' outer sides copied from
 // the inner ones (:3926-3935)
-template <int NS>
+template <int NS, int NX>
 __global__ __launch_bounds__(PTHR) void k_pp_recon(Params P, Grid G, PoreBufs B) {
-  constexpr int NV = 5 + NS, NP = NV + 2;
+  constexpr int NV = 4 + NS + nx_count(NX), NP = NV + 2;
   if (step_done(G)) return;
   const int nx = G.nx, ny = G.ny, N = G.N;
   const size_t NFX = (size_t)(nx + 1) * ny, NFY = (size_t)nx * (ny + 1);
@@ -248,9 +255,9 @@ __global__ __launch_bounds__(PTHR) void k_pp_recon(Params P, Grid G, PoreBufs B)
 
 // eval_speeds + eval_flux_KT of one face (src/solver_2d.f90:2834-3049,
 // :3937-4014): x faces first (f < NFX), then y faces
-template <int NS>
+template <int NS, int NX>
 __global__ __launch_bounds__(PTHR) void k_pp_faces(Params P, Grid G, PoreBufs B) {
-  constexpr int NV = 5 + NS, NP = NV + 2;
+  constexpr int NV = 4 + NS + nx_count(NX), NP = NV + 2;
   if (step_done(G)) return;
   const int nx = G.nx, ny = G.ny;
   const int NFX = (nx + 1) * ny, NFY = nx * (ny + 1);
@@ -265,14 +272,14 @@ __global__ __launch_bounds__(PTHR) void k_pp_faces(Params P, Grid G, PoreBufs B)
       pa[i] = sa[(size_t)i * nf + f];
       pb[i] = sb[(size_t)i * nf + f];
     }
-    qp_to_qc<NS, double, 1>(P, pa, qa);
-    qp_to_qc<NS, double, 1>(P, pb, qb);
+    qp_to_qc<NS, double, NX>(P, pa, qa);
+    qp_to_qc<NS, double, NX>(P, pb, qb);
     double amin, amax, bmin, bmax;
-    local_speeds<NS, double, 1>(P, pa, dir, amin, amax);
-    local_speeds<NS, double, 1>(P, pb, dir, bmin, bmax);
+    local_speeds<NS, double, NX>(P, pa, dir, amin, amax);
+    local_speeds<NS, double, NX>(P, pb, dir, bmin, bmax);
     const double an = fmin_f(fmin_f(amin, bmin), 0.0), ap = fmax_f(fmax_f(amax, bmax), 0.0);
-    phys_flux<NS, double, 1>(P, qa, pa, dir, fa);
-    phys_flux<NS, double, 1>(P, qb, pb, dir, fb);
+    phys_flux<NS, double, NX>(P, qa, pa, dir, fa);
+    phys_flux<NS, double, NX>(P, qb, pb, dir, fb);
     double H[NV];
 #pragma unroll
     for (int i = 0; i < NV; i++) {
@@ -321,9 +328,9 @@ __global__ __launch_bounds__(PTHR) void k_pp_faces(Params P, Grid G, PoreBufs B)
 // active cells: the stage-1 right-hand side divF - expl (eval_hyperbolic_terms
 // :2616-2693, eval_expl_terms) into G.R and the CFL candidate of the cell
 // (timestep, :873-1006) as one atomicMin on its bit pattern
-template <int NS>
+template <int NS, int NX>
 __global__ __launch_bounds__(PTHR) void k_pp_cells(Params P, Grid G, PoreBufs B) {
-  constexpr int NV = 5 + NS, NP = NV + 2;
+  constexpr int NV = 4 + NS + nx_count(NX), NP = NV + 2;
   if (step_done(G)) return;
   const int nx = G.nx, ny = G.ny, N = G.N;
   const size_t NFX = (size_t)(nx + 1) * ny, NFY = (size_t)nx * (ny + 1);
@@ -335,7 +342,7 @@ __global__ __launch_bounds__(PTHR) void k_pp_cells(Params P, Grid G, PoreBufs B)
     double qp[NP], ex[NV];
 #pragma unroll
     for (int i = 0; i < NP; i++) qp[i] = B.qp[(size_t)i * N + c];
-    expl_terms<NS, double, 1>(P, G.bx[c], G.by[c], qp, ex, G.st->t, 0.0, 0.0, 0.0);
+    expl_terms<NS, double, NX>(P, G.bx[c], G.by[c], qp, ex, G.st->t, 0.0, 0.0, 0.0);
 #pragma unroll
     for (int i = 0; i < NV; i++) {
       double d = 0.0;
@@ -356,36 +363,38 @@ __global__ __launch_bounds__(PTHR) void k_pp_cells(Params P, Grid G, PoreBufs B)
 }
 
 // ---------------------------------------------------------------------------
-// stage 2 with the 3 x 3 Newton system (src/solver_2d.f90:1103-1338,
-// :1662-2367), as the reference runs it for n_nh = 3
+// stage 2 with the Newton system (src/solver_2d.f90:1103-1338,
+// :1662-2367): hu, hv and the pore pressure are implicit (n_nh = 3, solved
+// by DGESV), or hu and hv alone (n_nh = 2, Cramer's rule); the solid classes
+// and the transported noise are explicit rows
 // ---------------------------------------------------------------------------
-template <int NS>
+template <int NS, int NX>
 __device__ __forceinline__ bool implicit_var(int i) {
-  return i == 1 || i == 2 || i == 4 + NS;
+  return i == 1 || i == 2 || (has_pore(NX) && i == pore_at<NS, NX>());
 }
 
 // eval_implicit_terms at a real state: the complex code on (q, 0)
-template <int NS>
+template <int NS, int NX>
 __device__ void implicit_real(const Params& P, double xi, const double* q, double* nh) {
-  constexpr int NV = 5 + NS;
+  constexpr int NV = 4 + NS + nx_count(NX);
   cx qc[NV];
 #pragma unroll
   for (int i = 0; i < NV; i++) qc[i] = C(q[i], 0.0);
   cx n1, n2, np;
-  implicit_c<NS, 1>(P, qc, n1, n2, xi, &np);
+  implicit_c<NS, NX>(P, qc, n1, n2, xi, &np);
 #pragma unroll
   for (int i = 0; i < NV; i++) nh[i] = 0.0;
   nh[1] = n1.re;
   nh[2] = n2.re;
-  nh[4 + NS] = np.re;
+  if constexpr (has_pore(NX)) nh[pore_at<NS, NX>()] = np.re;
 }
 
-template <int NS>
+template <int NS, int NX>
 __device__ void eval_f_pp(const Params& P, double xi, double dt, const double* qj, const double* q0,
                           const double* Rni, double* f, double& scal) {
-  constexpr int NV = 5 + NS;
+  constexpr int NV = 4 + NS + nx_count(NX);
   double nh[NV];
-  implicit_real<NS>(P, xi, qj, nh);
+  implicit_real<NS, NX>(P, xi, qj, nh);
   double s = 0.0;
 #pragma unroll
   for (int i = 0; i < NV; i++) {
@@ -399,24 +408,27 @@ __device__ void eval_f_pp(const Params& P, double xi, double dt, const double* q
 }
 
 // eval_jacobian (complex step, :2302-2367): L column-major NV x NV
-template <int NS>
+template <int NS, int NX>
 __device__ void eval_jacobian_pp(const Params& P, double xi, double dt, const double* qrel,
                                  const double* qorg, double* L) {
-  constexpr int NV = 5 + NS;
+  constexpr int NV = 4 + NS + nx_count(NX);
 #pragma unroll
   for (int i = 0; i < NV * NV; i++) L[i] = 0.0;
 #pragma unroll
   for (int i = 0; i < NV; i++) {
     L[i + NV * i] = 1.0 * qorg[i];
-    if (implicit_var<NS>(i)) {
+    if (implicit_var<NS, NX>(i)) {
       cx qc[NV];
 #pragma unroll
       for (int m = 0; m < NV; m++) qc[m] = cmul(C(qrel[m], (m == i) ? P.hcs : 0.0), C(qorg[m], 0.0));
       cx n1, n2, np;
-      implicit_c<NS, 1>(P, qc, n1, n2, xi, &np);
+      implicit_c<NS, NX>(P, qc, n1, n2, xi, &np);
 #pragma unroll
       for (int r = 0; r < NV; r++) {
-        const double im = (r == 1) ? n1.im : ((r == 2) ? n2.im : ((r == 4 + NS) ? np.im : 0.0));
+        const double im =
+            (r == 1) ? n1.im
+                     : ((r == 2) ? n2.im
+                                 : ((has_pore(NX) && r == pore_at<NS, NX>()) ? np.im : 0.0));
         const double J = 1.0 * im * P.one_by_hcs;
         L[r + NV * i] = L[r + NV * i] - dt * 1.0 * J;
       }
@@ -509,12 +521,12 @@ __device__ void dgesv3(double* a, double* b) {
 }
 
 // lnsrch (:2021-2226), one thread per cell
-template <int NS>
+template <int NS, int NX>
 __device__ void lnsrch_pp(const Params& P, double xi, double dt, const double* xold,
                           const double* qorg, const double* q0, double fold, const double* g,
                           double* p, double* x, double& f_out, double* rt, double stpmax,
                           const double* Rni, unsigned int& err) {
-  constexpr int NV = 5 + NS;
+  constexpr int NV = 4 + NS + nx_count(NX);
   const double TOLX = kEps;
   double sum = 0.0;
 #pragma unroll
@@ -546,7 +558,7 @@ __device__ void lnsrch_pp(const Params& P, double xi, double dt, const double* x
     for (int i = 0; i < NV; i++) x[i] = xold[i] + alam * p[i];
 #pragma unroll
     for (int i = 0; i < NV; i++) qj[i] = x[i] * qorg[i];
-    eval_f_pp<NS>(P, xi, dt, qj, q0, Rni, rt, f);
+    eval_f_pp<NS, NX>(P, xi, dt, qj, q0, Rni, rt, f);
     if (f <= 0.9 * fold) {
       break;
     } else if (alam < alamin) {
@@ -590,10 +602,10 @@ __device__ void lnsrch_pp(const Params& P, double xi, double dt, const double* x
   f_out = f;
 }
 
-template <int NS>
+template <int NS, int NX>
 __device__ void solve_rk_step_pp(const Params& P, double xi, double dt, double* qj, const double* q0,
                                  const double* Rni, unsigned int& err) {
-  constexpr int NV = 5 + NS, NE = NV - 3;
+  constexpr int NV = 4 + NS + nx_count(NX), NH = has_pore(NX) ? 3 : 2, NE = NV - NH;
   const double TOLF = 1.0e-10, STPMX = 100.0;
   double qorg[NV], qrel[NV], rt[NV], xold[NV], p[NV], grad[NV], L[NV * NV];
   double scal_f = 0.0, scal_f_old;
@@ -605,19 +617,20 @@ __device__ void solve_rk_step_pp(const Params& P, double xi, double dt, double* 
   for (int i = 0; i < NV; i++) qrel[i] = qj[i] / qorg[i];
   bool fresh = false;
   for (int it = 1; it <= 100; it++) {
-    if (!fresh) eval_f_pp<NS>(P, xi, dt, qj, q0, Rni, rt, scal_f);
+    if (!fresh) eval_f_pp<NS, NX>(P, xi, dt, qj, q0, Rni, rt, scal_f);
     double mx = 0.0;
 #pragma unroll
     for (int i = 0; i < NV; i++) mx = fmax_f(mx, fabs(rt[i]));
     if (mx < TOLF) break;
-    eval_jacobian_pp<NS>(P, xi, dt, qrel, qorg, L);
-    // explicit rows by their diagonal, the implicit block by DGESV (:1851-1926)
-    double d1[NE], d2[3];
-    int imp[3];
+    eval_jacobian_pp<NS, NX>(P, xi, dt, qrel, qorg, L);
+    // explicit rows by their diagonal, the implicit block by DGESV or
+    // Cramer's rule (:1851-1926)
+    double d1[NE], d2[NH];
+    int imp[NH];
     int ne = 0, nn = 0;
 #pragma unroll
     for (int r = 0; r < NV; r++) {
-      if (implicit_var<NS>(r)) {
+      if (implicit_var<NS, NX>(r)) {
         imp[nn] = r;
         d2[nn] = rt[r];
         nn++;
@@ -627,35 +640,44 @@ __device__ void solve_rk_step_pp(const Params& P, double xi, double dt, double* 
       }
     }
     {
-      // MATMUL(d1, reshape(pack(L, mask21), [NE, 3])): the implicit rows of
+      // MATMUL(d1, reshape(pack(L, mask21), [NE, NH])): the implicit rows of
       // the explicit columns, in pack order
-      double pk[NE * 3];
+      double pk[NE * NH];
       int m = 0;
 #pragma unroll
       for (int col = 0; col < NV; col++) {
-        if (implicit_var<NS>(col)) continue;
+        if (implicit_var<NS, NX>(col)) continue;
 #pragma unroll
-        for (int b = 0; b < 3; b++) pk[m++] = L[imp[b] + NV * col];
+        for (int b = 0; b < NH; b++) pk[m++] = L[imp[b] + NV * col];
       }
 #pragma unroll
-      for (int b = 0; b < 3; b++) {
+      for (int b = 0; b < NH; b++) {
         double s = 0.0;
 #pragma unroll
         for (int a = 0; a < NE; a++) s += d1[a] * pk[a + NE * b];
         d2[b] = d2[b] - s;
       }
     }
-    double a22[9];
+    double a22[NH * NH];
 #pragma unroll
-    for (int cc = 0; cc < 3; cc++)
+    for (int cc = 0; cc < NH; cc++)
 #pragma unroll
-      for (int r = 0; r < 3; r++) a22[r + 3 * cc] = L[imp[r] + NV * imp[cc]];
-    dgesv3(a22, d2);
+      for (int r = 0; r < NH; r++) a22[r + NH * cc] = L[imp[r] + NV * imp[cc]];
+    if constexpr (NH == 3) {
+      dgesv3(a22, d2);
+    } else {
+      // :1891-1901
+      const double inv_det = 1.0 / (a22[0] * a22[3] - a22[1] * a22[2]);
+      const double s1 = (d2[0] * a22[3] - d2[1] * a22[2]) * inv_det;
+      const double s2 = (a22[0] * d2[1] - a22[1] * d2[0]) * inv_det;
+      d2[0] = s1;
+      d2[1] = s2;
+    }
     ne = 0;
     nn = 0;
 #pragma unroll
     for (int r = 0; r < NV; r++) {
-      if (implicit_var<NS>(r)) {
+      if (implicit_var<NS, NX>(r)) {
         p[r] = -d2[nn] + 0.0;
         nn++;
       } else {
@@ -678,7 +700,7 @@ __device__ void solve_rk_step_pp(const Params& P, double xi, double dt, double* 
         for (int r = 0; r < NV; r++) s += rt[r] * L[r + NV * i];
         grad[i] = s;
       }
-      lnsrch_pp<NS>(P, xi, dt, xold, qorg, q0, scal_f_old, grad, p, qrel, scal_f, rt, stpmax, Rni,
+      lnsrch_pp<NS, NX>(P, xi, dt, xold, qorg, q0, scal_f_old, grad, p, qrel, scal_f, rt, stpmax, Rni,
                     err);
       fresh = true;
     } else {
@@ -686,7 +708,7 @@ __device__ void solve_rk_step_pp(const Params& P, double xi, double dt, double* 
       for (int i = 0; i < NV; i++) qrel[i] = xold[i] + p[i];
 #pragma unroll
       for (int i = 0; i < NV; i++) qj[i] = qrel[i] * qorg[i];
-      eval_f_pp<NS>(P, xi, dt, qj, q0, Rni, rt, scal_f);
+      eval_f_pp<NS, NX>(P, xi, dt, qj, q0, Rni, rt, scal_f);
       fresh = true;
     }
 #pragma unroll
@@ -706,10 +728,10 @@ __device__ void solve_rk_step_pp(const Params& P, double xi, double dt, double* 
 }
 
 // stage 2 and assemble_sol for one cell; q holds q0 on entry
-template <int NS>
+template <int NS, int NX>
 __device__ void stage2_pp(const Params& P, double xi, double dt, double* q, const double* X,
                           unsigned int& err) {
-  constexpr int NV = 5 + NS, NP = NV + 2;
+  constexpr int NV = 4 + NS + nx_count(NX), NP = NV + 2;
   double q0[NV], qfv[NV], qp[NP], q_si[NV], qg[NV], si[NV], Rj[NV], pd;
 #pragma unroll
   for (int i = 0; i < NV; i++) q0[i] = q[i];
@@ -723,9 +745,9 @@ __device__ void stage2_pp(const Params& P, double xi, double dt, double* q, cons
     Bv = Bv + (0.0 + 0.0) * 0.0;
     qfv[i] = q0[i] - dt * (A - Bv);
   }
-  qc_to_qp<NS, double, 1>(P, qfv, qp, pd);
+  qc_to_qp<NS, double, NX>(P, qfv, qp, pd);
   if (qfv[0] > 0.0) {
-    semi_impl<NS, 1>(P, qp, si);
+    semi_impl<NS, NX>(P, qp, si);
 #pragma unroll
     for (int i = 0; i < NV; i++) q_si[i] = qfv[i] + dt * 1.0 * si[i];
     if ((qfv[1] * qfv[1] + qfv[2] * qfv[2]) == 0.0) {
@@ -751,7 +773,7 @@ __device__ void stage2_pp(const Params& P, double xi, double dt, double* q, cons
       const double Bv = 0.0 + (0.0 + 0.0) * 0.0;
       Rj[i] = (A - Bv) - 1.0 * si[i];
     }
-    solve_rk_step_pp<NS>(P, xi, dt, qg, q0, Rj, err);
+    solve_rk_step_pp<NS, NX>(P, xi, dt, qg, q0, Rj, err);
     if (q_si[1] * q_si[1] + q_si[2] * q_si[2] == 0.0) {
       qg[1] = 0.0;
       qg[2] = 0.0;
@@ -798,7 +820,7 @@ __device__ void stage2_pp(const Params& P, double xi, double dt, double* q, cons
         if (q[4 + i] > -1.0e-7) q[4 + i] = 0.0;
     }
   }
-  qc_to_qp<NS, double, 1>(P, q, qp, pd);
+  qc_to_qp<NS, double, NX>(P, q, qp, pd);
   if (qp[3] < 273.0) err |= IMEX_ERR_COLD;
   double S = 0.0;
 #pragma unroll
@@ -819,21 +841,21 @@ __device__ void stage2_pp(const Params& P, double xi, double dt, double* q, cons
 
 // update_erosion_deposition_cell (:2382-2597) with the mass exchange of the
 // pore-pressure runs (gas loss; no settling, entrainment or erosion here)
-template <int NS>
+template <int NS, int NX>
 __device__ void mass_exchange_pp(const Params& P, double dt, double* q, double* depo,
                                  unsigned int& err) {
-  constexpr int NV = 5 + NS, NP = NV + 2;
+  constexpr int NV = 4 + NS + nx_count(NX), NP = NV + 2;
   double qp[NP], pd, dep[NS], eq[NV], ers[NS], erod[NS], topo;
 #pragma unroll
   for (int i = 0; i < NS; i++) erod[i] = 0.0;
   if (q[0] > 0.0) {
-    qc_to_qp<NS, double, 1>(P, q, qp, pd);
+    qc_to_qp<NS, double, NX>(P, q, qp, pd);
   } else {
 #pragma unroll
     for (int i = 0; i < NP; i++) qp[i] = 0.0;
     qp[3] = P.T_amb;
   }
-  if (mass_exchange<NS, 1, 1>(P, qp, erod, dt, dep, eq, ers, topo, nullptr))
+  if (mass_exchange<NS, 1, NX>(P, qp, erod, dt, dep, eq, ers, topo, nullptr))
     err |= IMEX_ERR_NEG_DEP;
 #pragma unroll
   for (int i = 0; i < NV; i++) q[i] = q[i] + dt * eq[i];
@@ -868,8 +890,8 @@ __device__ void mass_exchange_pp(const Params& P, double dt, double* q, double* 
   }
   double rg;
   if (q[0] > 0.0) {
-    qc_to_qp<NS, double, 1>(P, q, qp, pd);
-    rg = mixt_var<NS, double, 1>(P, qp, false).red_grav;
+    qc_to_qp<NS, double, NX>(P, q, qp, pd);
+    rg = mixt_var<NS, double, NX>(P, qp, false).red_grav;
   } else {
     rg = 0.0;
   }
@@ -881,13 +903,13 @@ __device__ void mass_exchange_pp(const Params& P, double dt, double* q, double* 
 
 // driver epilogue (src/IMEX_SfloW2D.f90:449-505): running maxima and the
 // vulnerability bits of one cell
-template <int NS>
+template <int NS, int NX>
 __device__ void epilogue_pp(const Params& P, const double* q, double& hmax, double& pdynmax,
                             double& mvel, unsigned long long& vuln) {
-  constexpr int NV = 5 + NS, NP = NV + 2;
+  constexpr int NV = 4 + NS + nx_count(NX), NP = NV + 2;
   if (!(q[0] > 0.0)) return;
   double qp[NP], pd;
-  qc_to_qp<NS, double, 1>(P, q, qp, pd);
+  qc_to_qp<NS, double, NX>(P, q, qp, pd);
   hmax = fmax_f(hmax, qp[0]);
   const double ru = qp[NV], rv = qp[NV + 1];
   const double mv = sqrt(ru * ru + rv * rv);
@@ -908,9 +930,9 @@ __device__ void epilogue_pp(const Params& P, const double* q, double& hmax, doub
   vuln = vuln | bits;
 }
 
-template <int NS>
+template <int NS, int NX>
 __global__ __launch_bounds__(PTHR) void k_pp_update(Params P, Grid G, int step_idx) {
-  constexpr int NV = 5 + NS;
+  constexpr int NV = 4 + NS + nx_count(NX);
   DevState* st = G.st;
   if (st->done) {
     if (blockIdx.x == 0 && threadIdx.x == 0 && G.dts) G.dts[step_idx] = 0.0;
@@ -938,12 +960,14 @@ __global__ __launch_bounds__(PTHR) void k_pp_update(Params P, Grid G, int step_i
 #pragma unroll
     for (int v = 0; v < NS; v++) depo[v] = G.dep[(size_t)v * N + c];
     const bool wet0 = q[0] > 0.0;
-    stage2_pp<NS>(P, cell_xi(P, G.noise, c), dt, q, X, err);
-    mass_exchange_pp<NS>(P, dt, q, depo, err);
+    stage2_pp<NS, NX>(P, cell_xi(P, G.noise, c), dt, q, X, err);
+    // update_erosion_deposition_cell returns at once without erosion,
+    // settling, entrainment and pore pressure (src/solver_2d.f90:2439-2441)
+    if constexpr (has_pore(NX)) mass_exchange_pp<NS, NX>(P, dt, q, depo, err);
     if (wet0 && q[0] == 0.0) dried++;
     double hm = G.hmax[c], pm = G.pdyn[c], mv = G.mvel[c];
     unsigned long long vb = G.vuln ? G.vuln[c] : 0ULL;
-    epilogue_pp<NS>(P, q, hm, pm, mv, vb);
+    epilogue_pp<NS, NX>(P, q, hm, pm, mv, vb);
 #pragma unroll
     for (int v = 0; v < NV; v++) G.q[(size_t)v * N + c] = q[v];
 #pragma unroll
@@ -958,7 +982,7 @@ __global__ __launch_bounds__(PTHR) void k_pp_update(Params P, Grid G, int step_i
   if (ncell) atomicAdd(&st->work, (unsigned long long)ncell);
 }
 
-template <int NS>
+template <int NS, int NX>
 int launch_step(imex_handle* h, const PoreBufs& B, int step_idx) {
   const Grid& G = h->G;
   const Params& P = h->P;
@@ -966,11 +990,11 @@ int launch_step(imex_handle* h, const PoreBufs& B, int step_idx) {
   const int nfaces = (nx + 1) * ny + nx * (ny + 1);
   const int gc = std::max(1, std::min((G.N + PTHR - 1) / PTHR, 4 * h->n_cu));
   const int gf = std::max(1, std::min((nfaces + PTHR - 1) / PTHR, 4 * h->n_cu));
-  hipLaunchKernelGGL(k_pp_prim<NS>, dim3(gc), dim3(PTHR), 0, h->stream, P, G, B);
-  hipLaunchKernelGGL(k_pp_recon<NS>, dim3(gc), dim3(PTHR), 0, h->stream, P, G, B);
-  hipLaunchKernelGGL(k_pp_faces<NS>, dim3(gf), dim3(PTHR), 0, h->stream, P, G, B);
-  hipLaunchKernelGGL(k_pp_cells<NS>, dim3(gc), dim3(PTHR), 0, h->stream, P, G, B);
-  hipLaunchKernelGGL(k_pp_update<NS>, dim3(gc), dim3(PTHR), 0, h->stream, P, G, step_idx);
+  hipLaunchKernelGGL((k_pp_prim<NS, NX>), dim3(gc), dim3(PTHR), 0, h->stream, P, G, B);
+  hipLaunchKernelGGL((k_pp_recon<NS, NX>), dim3(gc), dim3(PTHR), 0, h->stream, P, G, B);
+  hipLaunchKernelGGL((k_pp_faces<NS, NX>), dim3(gf), dim3(PTHR), 0, h->stream, P, G, B);
+  hipLaunchKernelGGL((k_pp_cells<NS, NX>), dim3(gc), dim3(PTHR), 0, h->stream, P, G, B);
+  hipLaunchKernelGGL((k_pp_update<NS, NX>), dim3(gc), dim3(PTHR), 0, h->stream, P, G, step_idx);
   HCHK(h, hipGetLastError());
   return 0;
 }
@@ -994,13 +1018,23 @@ int imex_pore_alloc(imex_handle* h) {
   return 0;
 }
 
-// one device-driven step of a pore-pressure run after the active-set pass
+// one device-driven step of a run on the general path (pore pressure, the
+// transported noise or both) after the active-set pass
 int imex_pore_step(imex_handle* h, int step_idx) {
   const PoreBufs B{h->pp_qp, h->pp_qpL, h->pp_qpR, h->pp_qpB, h->pp_qpT,
                    h->pp_Hx, h->pp_Hy, h->pp_amx, h->pp_amy};
-  switch (h->P.ns) {
-    case 1: return launch_step<1>(h, B, step_idx);
-    case 2: return launch_step<2>(h, B, step_idx);
-    default: return launch_step<3>(h, B, step_idx);
+  const int x = (h->P.pore ? 1 : 0) | (h->P.stoch_tr ? 2 : 0);
+  switch (h->P.ns * 4 + x) {
+    case 5: return launch_step<1, 1>(h, B, step_idx);
+    case 6: return launch_step<1, 2>(h, B, step_idx);
+    case 7: return launch_step<1, 3>(h, B, step_idx);
+    case 9: return launch_step<2, 1>(h, B, step_idx);
+    case 10: return launch_step<2, 2>(h, B, step_idx);
+    case 11: return launch_step<2, 3>(h, B, step_idx);
+    case 13: return launch_step<3, 1>(h, B, step_idx);
+    case 14: return launch_step<3, 2>(h, B, step_idx);
+    default:
+      h->err = "general path: no kernels for this class count and extra variables";
+      return -2;
   }
 }

@@ -70,6 +70,8 @@ MODULE imex_hip_binding
      INTEGER(c_int32_t) :: bc_custom
      INTEGER(c_int32_t) :: bc_flag(32)
      REAL(c_double) :: bc_value(32)
+     ! the transported noise q1 * Z (stoch_transport_flag; needs stochastic_flag)
+     INTEGER(c_int32_t) :: stoch_transport_flag
   END TYPE imex_config
 
   ! typedef struct imex_static: borrowed pointers to nx*ny arrays (x

@@ -52,9 +52,13 @@ def qp_to_qc(cfg: RunConfig, qp: np.ndarray) -> np.ndarray:
         qc[..., 3] = h * rho_m * cp_mix * T
     for i in range(ns):
         qc[..., 4 + i] = xs[i] * qc[..., 0]
+    if cfg.stoch_transport_flag:
+        # q1 * Z (src/constitutive_2d.f90:1979, :2141)
+        qc[..., 4 + ns] = qp[..., 4 + ns] * qc[..., 0]
     if cfg.pore_pressure_flag:
         # q1 * p_exc (src/constitutive_2d.f90:1981, :2143)
-        qc[..., 4 + ns] = qp[..., 4 + ns] * qc[..., 0]
+        ip = 4 + ns + (1 if cfg.stoch_transport_flag else 0)
+        qc[..., ip] = qp[..., ip] * qc[..., 0]
     qc[~wet] = 0.0
     return qc
 
@@ -91,8 +95,10 @@ def collapsing_volume(cfg: RunConfig) -> np.ndarray:
     for i in range(ns):
         qp[..., 4 + i] = np.where(sel, frac * cfg.h_collapse * cfg.alphas_collapse[i], 0.0)
     if cfg.pore_pressure_flag:
-        # the cylinder's excess pore pressure is 1 (src/init_2d.f90:146-147)
-        qp[..., 4 + ns] = np.where(sel, 1.0, 0.0)
+        # the cylinder's excess pore pressure is 1 and the transported
+        # noise 0 until getSteadyStateZ sets it (src/init_2d.f90:143-147)
+        ip = 4 + ns + (1 if cfg.stoch_transport_flag else 0)
+        qp[..., ip] = np.where(sel, 1.0, 0.0)
     return qp_to_qc(cfg, qp)
 
 

@@ -87,6 +87,7 @@ class ImexConfig(ctypes.Structure):
         ("maximum_solid_packing", ctypes.c_double), ("sphericity_s", ctypes.c_double * 3),
         ("bc_custom", ctypes.c_int32), ("bc_flag", ctypes.c_int32 * 32),
         ("bc_value", ctypes.c_double * 32),
+        ("stoch_transport_flag", ctypes.c_int32),
     ]
 
 
@@ -189,6 +190,7 @@ def make_config(cfg: RunConfig) -> ImexConfig:
     c.mu_s, c.mu_2, c.muI_inf, c.I_0 = cfg.mu_s, cfg.mu_2, cfg.muI_inf, cfg.I_0
     c.stochastic_flag = int(cfg.stochastic_flag)
     c.pore_pressure_flag = int(cfg.pore_pressure_flag)
+    c.stoch_transport_flag = int(cfg.stoch_transport_flag)
     c.gas_loss_flag = int(cfg.gas_loss_flag)
     c.f_inhibit_mode = cfg.f_inhibit_code
     c.N_inh = int(cfg.N_inh)

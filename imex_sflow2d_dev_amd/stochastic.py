@@ -69,3 +69,17 @@ def steady_state_noise(cfg, seed: int = 0) -> np.ndarray:
         elif cfg.sym_noise < 0.0:
             Z = -(np.abs(Z) ** cfg.noise_pow_val)
     return Z
+
+
+def transport_noise(cfg, q: np.ndarray, Z: np.ndarray) -> np.ndarray:
+    """q [k, j, var] with the transported noise set to q1 * Z, as
+    getSteadyStateZ does after its burn-in when stoch_transport_flag is on
+    (src/stochastic_module.f90:110-126: qp = Z and q = q1 * Z in the solved
+    cells; elsewhere q1 is 0)."""
+    if not cfg.stoch_transport_flag:
+        raise ValueError("transport_noise: the run does not transport the noise")
+    if q.shape[-1] != cfg.n_vars or Z.shape != q.shape[:2]:
+        raise ValueError("transport_noise: q must be [ny, nx, n_vars] and Z [ny, nx]")
+    out = np.array(q, dtype=np.float64, copy=True)
+    out[..., 4 + cfg.n_solid] = out[..., 0] * Z
+    return out

@@ -95,7 +95,8 @@ typedef struct imex_config {
   int32_t dynamic_permeability_flag;
   double hydraulic_permeability, alpha_trans, maximum_solid_packing, sphericity_s[3];
   /* per-variable boundary conditions (WEST/EAST/SOUTH/NORTH_BOUNDARY_CONDITIONS,
-   * src/solver_2d.f90:3190-3484), pore-pressure runs only: with bc_custom,
+   * src/solver_2d.f90:3190-3484), pore-pressure and transported-noise runs
+   * only: with bc_custom,
    * bc_flag[8 side + var] and bc_value[8 side + var] for the sides W, E, S,
    * N and the conservative variables in order; flag 0 Dirichlet value, 1
    * slope value, 2 one-sided slope.  Without it every flag is 1 with value
@@ -103,6 +104,17 @@ typedef struct imex_config {
   int32_t bc_custom;
   int32_t bc_flag[32];
   double bc_value[32];
+  /* the transported noise (STOCHASTIC_PARAMETERS stoch_transport_flag,
+   * src/inpout_2d.f90:1466-1475; needs stochastic_flag): the conserved
+   * variable q1 * Z after the solid classes and before the pore pressure
+   * (n_vars = 4 + n_solid + 1 [+ 1]), advected with the flow
+   * (src/constitutive_2d.f90:2428-2431, :2490-2493) and exchanged with the
+   * mass (:5132-5137); the friction keeps reading the noise plane of
+   * imex_set_noise.  The host gives its initial value q1 * Z
+   * (getSteadyStateZ, src/stochastic_module.f90:110-126).  Device-driven
+   * steps only, with the restrictions of pore-pressure runs and at most 8
+   * variables. */
+  int32_t stoch_transport_flag;
 } imex_config;
 
 /* Per-cell static fields computed once by the host (src/geometry_2d.f90:555-895

@@ -102,8 +102,11 @@ struct oracle_s {
     /* pore pressure (PORE_PRESSURE_PARAMETERS, src/inpout_2d.f90:389-391,
      * :536-542, :681-710; maximum_solid_packing and sphericity_s from
      * SOLID_TRANSPORT_PARAMETERS): the conserved variable q1 * p_exc at
-     * index ipp = 4 + ns (0-based) */
+     * index ipp = 4 + ns + stoch_tr (0-based) */
     int pore, gas_loss, finh, N_inh, dyn_perm, ipp;  /* finh: 0 OFF 1 STEP 2 STATIC 3 DYNAMIC */
+    /* the transported noise (stoch_transport_flag, src/inpout_2d.f90:642-644,
+     * :1466-1475): the conserved variable q1 * Z at index ist = 4 + ns */
+    int stoch_tr, ist;
     double hyd_perm, alpha_trans, max_pack, sph_s[3], pascal[64];
     /* per-variable boundary conditions (bcW/bcE/bcS/bcN of the
      * *_BOUNDARY_CONDITIONS namelists, src/solver_2d.f90:3190-3484):
@@ -289,7 +292,7 @@ static cx cpow_r(const oracle_t *o, cx z, double c)
  *  :775-797 carrier density, :799-830 mixture, :944-972 velocity, p_dyn) */
 static void r_phys_var(const oracle_t *o, const double *qc, double *h, double *u, double *v,
                        double *alphas, double *rho_m, double *T, double *red_grav,
-                       double *p_dyn, double *pexc)
+                       double *p_dyn, double *pexc, double *zs)
 {
     const int ns = o->ns;
     double xs[3], inv_qj1;
@@ -303,10 +306,12 @@ static void r_phys_var(const oracle_t *o, const double *qc, double *h, double *u
             for (int i = 0; i < ns; i++) sx += xs[i];
             for (int i = 0; i < ns; i++) xs[i] = xs[i] / sx;
         }
-        /* excess pore pressure: src/constitutive_2d.f90:659 */
+        /* transported noise and excess pore pressure:
+         * src/constitutive_2d.f90:657, :659 */
+        *zs = o->stoch_tr ? qc[o->ist] * inv_qj1 : 0.0;
         *pexc = o->pore ? qc[o->ipp] * inv_qj1 : 0.0;
     } else {
-        *h = 0.0; *u = 0.0; *v = 0.0; *pexc = 0.0;
+        *h = 0.0; *u = 0.0; *v = 0.0; *pexc = 0.0; *zs = 0.0;
         for (int i = 0; i < ns; i++) alphas[i] = 0.0;
         *rho_m = o->rho_a_amb;
         *T = o->T_amb;
@@ -361,14 +366,15 @@ static void r_phys_var(const oracle_t *o, const double *qc, double *h, double *u
 /* qc_to_qp: src/constitutive_2d.f90:1637-1690 */
 static void qc_to_qp(const oracle_t *o, const double *qc, double *qp, double *p_dyn)
 {
-    double h, u, v, alphas[3], rho_m, T, rg, pexc;
-    r_phys_var(o, qc, &h, &u, &v, alphas, &rho_m, &T, &rg, p_dyn, &pexc);
+    double h, u, v, alphas[3], rho_m, T, rg, pexc, zs;
+    r_phys_var(o, qc, &h, &u, &v, alphas, &rho_m, &T, &rg, p_dyn, &pexc, &zs);
     const int nv = o->nv;
     qp[0] = h;
     qp[1] = h * u;
     qp[2] = h * v;
     qp[3] = T;
     for (int i = 0; i < o->ns; i++) qp[4 + i] = alphas[i] * h;
+    if (o->stoch_tr) qp[o->ist] = zs;  /* :1681 */
     if (o->pore) qp[o->ipp] = pexc;  /* :1683 */
     qp[nv] = u;
     qp[nv + 1] = v;
@@ -420,6 +426,7 @@ static void qp_to_qc(const oracle_t *o, const double *qp, double *qc)
         qc[3] = h * rho_m * cp_mix * T;
     }
     for (int i = 0; i < ns; i++) qc[4 + i] = xs[i] * qc[0];
+    if (o->stoch_tr) qc[o->ist] = qp[o->ist] * qc[0];  /* :1979, :2141 */
     if (o->pore) qc[o->ipp] = qp[o->ipp] * qc[0];  /* :1981, :2143 */
 }
 
@@ -530,7 +537,9 @@ static void eval_fluxes(const oracle_t *o, const double *qc, const double *qp, d
         if ((f[0] > 0.0) && (S / f[0] > 1.0)) {
             for (int i = 0; i < ns; i++) f[4 + i] = f[4 + i] / S * f[0];
         }
-        /* flux of q1 * p_exc: src/constitutive_2d.f90:2435-2439, :2497-2501 */
+        /* fluxes of q1 * Z and q1 * p_exc: src/constitutive_2d.f90:2428-2439,
+         * :2490-2501 */
+        if (o->stoch_tr) f[o->ist] = w * qc[o->ist] * 1.0;
         if (o->pore) f[o->ipp] = w * qc[o->ipp] * 1.0;
     } else {
         for (int i = 0; i < nv; i++) f[i] = 0.0;
@@ -991,6 +1000,7 @@ static int eval_mass_exchange_terms(const oracle_t *o, const double *qp, const d
         eq[3] = -(T * (A + rc * cpc * cpl)) + T_erodible * (D + o->rho_c_sub * cpc * cpe) +
                 o->T_amb * o->cp_a * o->rho_a_amb * air;
     for (int i = 0; i < ns; i++) eq[4 + i] = o->rho_s[i] * (ers[i] - dep[i]);
+    if (o->stoch_tr) eq[o->ist] = eq[0] * qp[o->ist];  /* :5132-5137 */
     if (o->pore) eq[o->ipp] = eq[0] * qp[o->ipp];  /* :5140-5145 */
     /* topo_term: src/constitutive_2d.f90:5149 */
     double dep_tot = 0.0, ers_tot = 0.0;
@@ -2226,14 +2236,16 @@ oracle_t *oracle_create(const int *ip, const double *rp)
     o->hyd_perm = rp[73]; o->alpha_trans = rp[74];
     o->max_pack = (rp[75] > 0.0) ? rp[75] : 0.64;
     for (int i = 0; i < 3; i++) o->sph_s[i] = (rp[76 + i] > 0.0) ? rp[76 + i] : 1.0;
-    o->nv = 4 + o->ns + (o->pore ? 1 : 0);
+    o->stoch_tr = ip[54];
+    o->nv = 4 + o->ns + (o->stoch_tr ? 1 : 0) + (o->pore ? 1 : 0);
     o->bc_custom = ip[21];
     for (int sd = 0; sd < 4; sd++)
         for (int i = 0; i < 8; i++) {
             o->bcf[sd][i] = o->bc_custom ? ip[22 + 8 * sd + i] : 1;
             o->bcv[sd][i] = o->bc_custom ? rp[79 + 8 * sd + i] : 0.0;
         }
-    o->ipp = 4 + o->ns;
+    o->ist = 4 + o->ns;
+    o->ipp = 4 + o->ns + (o->stoch_tr ? 1 : 0);
     /* precompute_pascal_coefficient: src/constitutive_2d.f90:419-466 */
     for (int n = 0; n <= o->N_inh && n < 64; n++) {
         const int N = o->N_inh;

@@ -77,12 +77,12 @@ CONTAINS
 
   !----------------------------------------------------------------------------
   ! Set up the reference modules for one run.
-  !   ip(0:53)  integer parameters, rp(0:110) real parameters (layout in
+  !   ip(0:54)  integer parameters, rp(0:110) real parameters (layout in
   !   oracle/refpy.py), bdem(nx*ny) bed elevation at the cell centres
   !   (x index fastest).
   !----------------------------------------------------------------------------
   INTEGER(c_int) FUNCTION imexref_setup(ip, rp, bdem) BIND(C, name='imexref_setup')
-    INTEGER(c_int), INTENT(IN) :: ip(0:53)
+    INTEGER(c_int), INTENT(IN) :: ip(0:54)
     REAL(c_double), INTENT(IN) :: rp(0:110)
     REAL(c_double), INTENT(IN) :: bdem(*)
 
@@ -102,7 +102,9 @@ CONTAINS
     ! ---- model switches for the configured path -------------------------
     n_solid = ip(2)
     n_add_gas = 0
+    ! the transported noise (src/inpout_2d.f90:1466-1475): ip(54)
     n_stoch_vars = 0
+    IF ( ip(54) .NE. 0 ) n_stoch_vars = 1
     ! pore pressure (PORE_PRESSURE_PARAMETERS): ip(16:20), rp(73:78)
     pore_pressure_flag = ( ip(16) .NE. 0 )
     n_pore_vars = 0
@@ -119,11 +121,11 @@ CONTAINS
     curvature_term_flag = .FALSE.
     vertical_profiles_flag = .FALSE.
     bottom_conc_flag = .FALSE.
-    ! stochastic friction without transport of the noise: Z is set by
-    ! imexref_set_noise (0 after allocate_solver_variables)
+    ! stochastic friction: Z is set by imexref_set_noise (0 after
+    ! allocate_solver_variables); with transport the host gives q1 * Z too
     stochastic_flag = ( ip(15) .NE. 0 )
     length_spatial_corr = -1.0_wp
-    stoch_transport_flag = .FALSE.
+    stoch_transport_flag = ( ip(54) .NE. 0 )
     mean_field_flag = .FALSE.
     gas_loss_flag = ( ip(17) .NE. 0 )
     sutherland_flag = .FALSE.
@@ -137,7 +139,7 @@ CONTAINS
     n_intervals = 0
 
     ! ---- variable / equation index map (src/inpout_2d.f90:584-655,1455-2049)
-    n_vars = 4 + n_solid + n_pore_vars
+    n_vars = 4 + n_solid + n_stoch_vars + n_pore_vars
     n_eqns = n_vars
     idx_h = 1
     idx_hu = 2
@@ -148,7 +150,7 @@ CONTAINS
     idx_addGas_first = 5 + n_solid
     idx_addGas_last = 4 + n_solid
     idx_stoch = 5 + n_solid
-    idx_pore = 5 + n_solid    ! src/inpout_2d.f90:654 (no added gas, no noise)
+    idx_pore = 5 + n_solid + n_stoch_vars    ! src/inpout_2d.f90:654 (no added gas)
     idx_u = n_vars + 1
     idx_v = n_vars + 2
     idx_totMassEqn = 1

@@ -56,9 +56,9 @@ def set_threads(n: int, flavour: str = "ieee") -> None:
 
 
 def pack_params(cfg):
-    """ip[54] / rp[111] parameter vectors read by imexref_setup (and by the
+    """ip[55] / rp[111] parameter vectors read by imexref_setup (and by the
     C restatement's oracle_create)."""
-    ip = np.zeros(54, dtype=np.int32)
+    ip = np.zeros(55, dtype=np.int32)
     ip[0] = cfg.nx
     ip[1] = cfg.ny
     ip[2] = cfg.n_solid
@@ -122,6 +122,8 @@ def pack_params(cfg):
             for i in range(cfg.n_vars):
                 ip[22 + 8 * sd + i] = bcf[sd][i]
                 rp[79 + 8 * sd + i] = bcv[sd][i]
+    # the transported noise
+    ip[54] = int(getattr(cfg, "stoch_transport_flag", False))
     return ip, rp
 
 

@@ -208,6 +208,60 @@ def cps_64():
 PORE_CASES = {"cp_64": cp_64, "cpd_64": cpd_64, "cpo_64": cpo_64, "cps_64": cps_64}
 
 
+def with_transported_noise(cfg, q0, Z):
+    """cfg and q0 with the noise transported by the flow
+    (stoch_transport_flag): the conserved variable q1 * Z inserted after the
+    solid classes, as getSteadyStateZ sets it
+    (src/stochastic_module.f90:110-126)."""
+    from imex_sflow2d_dev_amd import stochastic
+    cfg.stochastic_flag = True
+    cfg.stoch_transport_flag = True
+    cfg.validate()
+    q = np.insert(q0, 4 + cfg.n_solid, 0.0, axis=-1)
+    return cfg, stochastic.transport_noise(cfg, q, Z)
+
+
+def c1t_64():
+    """c1s_64 (collapse, Voellmy friction with the stochastic xi) with the
+    noise transported: (cfg, B, static, q0, Z)."""
+    from imex_sflow2d_dev_amd import init
+    cfg, B = c1_64()
+    cfg.stochastic_flag = True
+    Z = stochastic_noise(cfg)
+    st, q0 = init.setup(cfg, B)
+    cfg, q0 = with_transported_noise(cfg, q0, Z)
+    return cfg, B, st, q0, Z
+
+
+def cpot_64():
+    """cpo_64 (pore pressure, f_inhibit OFF, Voellmy friction, gas loss)
+    with the stochastic xi and the noise transported: the gas loss makes the
+    noise's mass-exchange source eqns_term(1) * Z non-zero."""
+    cfg, B, st, q0 = cpo_64()
+    Z = stochastic_noise(cfg)
+    cfg, q0 = with_transported_noise(cfg, q0, Z)
+    return cfg, B, st, q0, Z
+
+
+def c2t_48():
+    """Two classes, the transported noise and no pore pressure (the general
+    path's 2 x 2 Newton system) on 48 x 48 cells of 10 m."""
+    from imex_sflow2d_dev_amd import init
+    cfg, B = workloads.config1(nx=48, ny=48, dx=10.0)
+    cfg.x_collapse, cfg.y_collapse, cfg.r_collapse = 150.0, 240.0, 90.0
+    cfg.n_solid = 2
+    cfg.rho_s, cfg.diam_s, cfg.sp_heat_s = [2500.0, 1500.0], [1e-3, 1e-4], [1200.0, 1200.0]
+    cfg.alphas_collapse, cfg.alphas_source = [0.06, 0.04], [0.0, 0.0]
+    cfg.stochastic_flag = True
+    Z = stochastic_noise(cfg)
+    st, q0 = init.setup(cfg, B)
+    cfg, q0 = with_transported_noise(cfg, q0, Z)
+    return cfg, B, st, q0, Z
+
+
+TRANSPORT_CASES = {"c1t_64": c1t_64, "cpot_64": cpot_64, "c2t_48": c2t_48}
+
+
 def dambreak_deck(mode="STATIC"):
     """The reference's TEST_DAMBREAK decks (TESTS/TEST_DAMBREAK/TEST_<mode>_FINH/
     IMEX_SfloW2D.inp, f_inhibit_mode <mode>): 300 x 15 cells of 1 cm, flat

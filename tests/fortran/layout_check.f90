@@ -69,6 +69,7 @@ PROGRAM layout_check
   WRITE(*,'(A,1X,I0)') 'imex_config.bc_custom', TRANSFER(C_LOC(c%bc_custom), base) - base
   WRITE(*,'(A,1X,I0)') 'imex_config.bc_flag', TRANSFER(C_LOC(c%bc_flag), base) - base
   WRITE(*,'(A,1X,I0)') 'imex_config.bc_value', TRANSFER(C_LOC(c%bc_value), base) - base
+  WRITE(*,'(A,1X,I0)') 'imex_config.stoch_transport_flag', TRANSFER(C_LOC(c%stoch_transport_flag), base) - base
   WRITE(*,'(A,1X,I0)') 'imex_static.size', C_SIZEOF(s)
   base = TRANSFER(C_LOC(s), base)
   WRITE(*,'(A,1X,I0)') 'imex_static.B_prime_x', TRANSFER(C_LOC(s%B_prime_x), base) - base

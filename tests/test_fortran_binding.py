@@ -48,7 +48,7 @@ def test_bind_c_types_match_the_c_structs(tmp_path):
                                text=True).stdout)
     c = _layout(subprocess.run([str(tmp_path / "lc")], check=True, capture_output=True,
                                text=True).stdout)
-    assert len(c) == 4 + 45 + 9 + 3 + 7 + 9 + 17
+    assert len(c) == 4 + 45 + 9 + 3 + 7 + 9 + 17 + 1
     assert f == c
     # the ctypes mirrors of the Python binding have the same layout
     for name, cls in (("imex_config", native.ImexConfig), ("imex_static", native.ImexStatic),
