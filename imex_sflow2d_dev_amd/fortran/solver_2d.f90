@@ -244,11 +244,13 @@ CONTAINS
     USE parameters_2d, ONLY : energy_flag, radial_source_flag, n_thickness_levels,       &
          n_dyn_pres_levels, thickness_levels, dyn_pres_levels, cfl, max_dt, vel_source,   &
          T_source, h_source, time_param, alphas_source, erodible_deposit_flag,            &
-         topo_change_flag, rheology_model, stochastic_flag
+         topo_change_flag, rheology_model, stochastic_flag, pore_pressure_flag,           &
+         stoch_transport_flag, gas_loss_flag, bcW, bcE, bcS, bcN
     USE constitutive_2d, ONLY : settling_flag, entrainment_flag, grav, mu, xi, pres,       &
          T_ambient, sp_heat_a, sp_gas_const_a, kin_visc_a, rho_s, diam_s, sp_heat_s,      &
          erosion_coeff, erodible_porosity, T_erodible, erodible_fract, mu_s, mu_2,        &
-         muI_inf, I_0
+         muI_inf, I_0, f_inhibit_mode, N_inh, alpha_trans, hydraulic_permeability,        &
+         dynamic_permeability_flag, maximum_solid_packing, sphericity_s
     USE geometry_2d, ONLY : dx, x0, y0
 
     TYPE(imex_config), INTENT(OUT) :: c
@@ -313,6 +315,58 @@ CONTAINS
     c%muI_inf = muI_inf
     c%I_0 = I_0
     c%stochastic_flag = MERGE(1, 0, stochastic_flag)
+    ! pore pressure and the transported noise (the general path)
+    c%pore_pressure_flag = MERGE(1, 0, pore_pressure_flag)
+    c%stoch_transport_flag = MERGE(1, 0, stoch_transport_flag)
+    c%gas_loss_flag = 0
+    c%f_inhibit_mode = 0
+    c%N_inh = 0
+    c%dynamic_permeability_flag = 0
+    c%hydraulic_permeability = 0.0_c_double
+    c%alpha_trans = 0.0_c_double
+    c%maximum_solid_packing = 0.0_c_double
+    c%sphericity_s = 1.0_c_double
+    IF ( pore_pressure_flag ) THEN
+       c%gas_loss_flag = MERGE(1, 0, gas_loss_flag)
+       SELECT CASE ( TRIM(f_inhibit_mode) )
+       CASE ('STEP')
+          c%f_inhibit_mode = 1
+       CASE ('STATIC')
+          c%f_inhibit_mode = 2
+       CASE ('DYNAMIC')
+          c%f_inhibit_mode = 3
+       CASE DEFAULT
+          c%f_inhibit_mode = 0
+       END SELECT
+       c%N_inh = N_inh
+       c%dynamic_permeability_flag = MERGE(1, 0, dynamic_permeability_flag)
+       c%hydraulic_permeability = hydraulic_permeability
+       c%alpha_trans = alpha_trans
+       c%maximum_solid_packing = maximum_solid_packing
+       IF ( ALLOCATED(sphericity_s) ) THEN
+          DO i = 1, MIN(n_solid, 3)
+             c%sphericity_s(i) = sphericity_s(i)
+          END DO
+       END IF
+    END IF
+    ! the per-variable boundary conditions the namelists set
+    ! (src/inpout_2d.f90:2495-3123); the general path takes them
+    c%bc_custom = 0
+    c%bc_flag = 1
+    c%bc_value = 0.0_c_double
+    IF ( ( pore_pressure_flag .OR. stoch_transport_flag ) .AND. ALLOCATED(bcW) ) THEN
+       c%bc_custom = 1
+       DO i = 1, MIN(n_vars, 8)
+          c%bc_flag(i) = bcW(i)%flag
+          c%bc_flag(8 + i) = bcE(i)%flag
+          c%bc_flag(16 + i) = bcS(i)%flag
+          c%bc_flag(24 + i) = bcN(i)%flag
+          c%bc_value(i) = bcW(i)%value
+          c%bc_value(8 + i) = bcE(i)%value
+          c%bc_value(16 + i) = bcS(i)%value
+          c%bc_value(24 + i) = bcN(i)%value
+       END DO
+    END IF
 
   END SUBROUTINE fill_config
 
