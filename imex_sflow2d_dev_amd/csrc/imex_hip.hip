@@ -2587,7 +2587,10 @@ extern "C" int imex_post_step(imex_handle* h, double t_new, imex_step_stats* out
   if (!h) return -2;
   HCHK(h, hipSetDevice(h->device));
   int rc;
-  if ((rc = update(h, 4, 0.0, 0))) return rc;
+  // runs on the general path (pore pressure, the transported noise) had
+  // their epilogue inside each device-driven step; the diagnostics below
+  // read only h, q1 and the solid masses, the same in their layout
+  if (!h->P.general && (rc = update(h, 4, 0.0, 0))) return rc;
   DevState s;
   if (read_state(h, &s)) return -1;
   s.t = t_new;
@@ -2652,7 +2655,10 @@ extern "C" int imex_post_step_noalloc(imex_handle* h, double t_new, imex_step_st
     return -2;
   }
   int rc;
-  if ((rc = update(h, 4, 0.0, 0))) return rc;
+  // runs on the general path (pore pressure, the transported noise) had
+  // their epilogue inside each device-driven step; the diagnostics below
+  // read only h, q1 and the solid masses, the same in their layout
+  if (!h->P.general && (rc = update(h, 4, 0.0, 0))) return rc;
   DevState s;
   if (read_state(h, &s)) return -1;
   s.t = t_new;

@@ -54,7 +54,9 @@ PROGRAM imex_fdriver
   READ(u) t, dt0, t_end
   n = cfg%nx * cfg%ny
   ns = cfg%n_solid
-  nv = 4 + ns
+  ! the transported noise and the pore pressure after the solid classes
+  nv = 4 + ns + MERGE(1, 0, cfg%stoch_transport_flag /= 0) + &
+       MERGE(1, 0, cfg%pore_pressure_flag /= 0)
   ALLOCATE(bx(n), by(n), ap(n), ax(n), ay(n), sc(n), sd(n))
   ALLOCATE(q(nv * n), dts(MAX(nsteps, 1)), hm(n), pd(n), mv(n), dep(n * ns))
   READ(u) bx, by, sc, sd, ap, ax, ay

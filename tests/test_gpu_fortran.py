@@ -97,3 +97,35 @@ def test_fortran_driver_matches_oracle(tmp_path, name, steps, mode):
     ref = a * math.fsum(q[:, :, 0].ravel().tolist())
     assert abs(res["stats"][0] - ref) <= 4 * np.spacing(ref)
     assert res["stats"][2] == a * float(np.count_nonzero(q[:, :, 0] > 1e-7))
+
+
+def test_fortran_driver_runs_a_dambreak_deck():
+    """The reference's TEST_STATIC_FINH deck (pore pressure, mu(I), the
+    per-variable boundaries; 300 x 15 cells from its restart state to
+    T_END = 0.3 s) through imex_fdriver's imex_run_steps loop: bit for bit
+    with the correctly rounded oracle's run in tests/golden/dambreak_static.npz,
+    and the stdout diagnostics of the final state."""
+    from conftest import golden_path
+    assert os.path.exists(build.FDRIVER), "imex_fdriver is not built (build())"
+    cfg, B, st, q0 = cases.dambreak_deck("STATIC")
+    g = dict(np.load(golden_path("dambreak_static.npz"), allow_pickle=False))
+    steps = int(g["steps"])
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        fin, fout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
+        _write_input(fin, cfg, st, q0, steps, 1)
+        print(f"imex_fdriver dambreak STATIC {steps} steps mode 1", flush=True)
+        r = subprocess.run([build.FDRIVER, fin, fout], capture_output=True, text=True,
+                           timeout=300, stdin=subprocess.DEVNULL)
+        assert r.returncode == 0, r.stdout + r.stderr
+        res = _read_output(fout, cfg, steps)
+    np.testing.assert_array_equal(res["dts"], g["cr_dts"])
+    np.testing.assert_array_equal(res["q"], g["cr_q"])
+    for k in ("hmax", "pdynmax", "mod_vel_max"):
+        np.testing.assert_array_equal(res[k], g["cr_field_" + k], err_msg=k)
+    assert res["err"] == 0
+    q = g["cr_q"]
+    a = cfg.dx * cfg.dx
+    ref = a * math.fsum(q[:, :, 0].ravel().tolist())
+    assert abs(res["stats"][0] - ref) <= 4 * np.spacing(ref)
+    assert res["stats"][2] == a * float(np.count_nonzero(q[:, :, 0] > 1e-7))
