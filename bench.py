@@ -178,7 +178,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--workload", default="c2",
                     help="c2 (default), c1, c2-small, c3, c4 (the batched ensemble), c5 "
-                         "(serving) or c2-outputs (the run with periodic outputs)")
+                         "(serving), c2-outputs (the run with periodic outputs) or dambreak "
+                         "(the reference's TEST_STATIC_FINH deck: pore pressure)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the reference CPU sample")
     ap.add_argument("--skip-whole-run", action="store_true",
@@ -246,6 +247,9 @@ def main():
         return
     if args.workload == "c2-outputs":
         bench_outputs(args)
+        return
+    if args.workload == "dambreak":
+        bench_dambreak(args)
         return
 
     t_host = time.perf_counter()
@@ -583,6 +587,71 @@ def bench_c3(args):
                                           "relative_difference_fp32_vs_fp64":
                                               rs["relative_difference"]}},
     }
+    print(json.dumps(out), flush=True)
+
+
+def bench_dambreak(args):
+    """The reference's TEST_STATIC_FINH deck (TESTS/TEST_DAMBREAK: pore
+    pressure with f_inhibit STATIC, mu(I) friction, gas loss, the
+    per-variable boundaries; 300 x 15 cells of 1 cm from its restart state to
+    T_END = 0.3 s) on the general path (csrc/imex_pore.hip), device-driven;
+    value = cell-stage updates/s of the whole run (2 x active cells summed
+    over the steps / wall time), timed from the uploaded state after an
+    untimed run that finds the step count.  Beside it, the reference's own
+    run of the deck (oracle/_ref, -Ofast build, one OpenMP thread: with
+    several its cell loops race on the module variable
+    hydraulic_permeability)."""
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests"))
+    import cases  # the deck's configuration and restart state
+    import numpy as np
+    from imex_sflow2d_dev_amd.solver import Solver
+    say = lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True)  # noqa: E731
+    cfg, B, st, q0 = cases.dambreak_deck("STATIC")
+    s = Solver(cfg, st, q0, raise_on_error=False)
+    steps = int(np.count_nonzero(s.run_steps(20000, t_end=cfg.t_end)))
+    s.close()
+    say(f"dambreak STATIC: {steps} steps to t_end {cfg.t_end}")
+    s = Solver(cfg, st, q0, raise_on_error=False)
+    a0, _ = s.counters()
+    t0 = time.perf_counter()
+    dts = s.run_steps(steps, t_end=cfg.t_end)
+    wall = time.perf_counter() - t0
+    a1, _ = s.counters()
+    err = s.err
+    s.close()
+    value = 2.0 * (a1 - a0) / wall
+    out = {
+        "metric": "cell-stage updates/s",
+        "value": value,
+        "unit": "cell-stage updates/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": 0,
+        "ms_per_step": wall * 1e3 / steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "the reference deck's restart state (tests/golden/dambreak_q0.npz)",
+        "config": {"workload": (f"dambreak: TEST_STATIC_FINH, {cfg.nx}x{cfg.ny} cells dx={cfg.dx} m, "
+                                f"pore pressure, f_inhibit STATIC, to t_end {cfg.t_end} s"),
+                   "active_cells": int(a1 - a0), "t_end_reached": float(np.sum(dts)),
+                   "physics_error_bits": int(err), "wall_s": wall},
+    }
+    if not args.no_cpu:
+        from oracle import refpy
+        if refpy.available("perf"):
+            refpy.set_threads(1, "perf")
+            run = refpy.RefRun(cfg, B, flavour="perf")
+            run.set_q(q0)
+            t0 = time.perf_counter()
+            _, ncells = run.run(steps)
+            cw = time.perf_counter() - t0
+            run.close()
+            out["cpu_baseline"] = {"value": 2.0 * float(np.sum(ncells)) / cw,
+                                   "unit": "cell-stage updates/s",
+                                   "cores": 1, "kind": "reference",
+                                   "sample": f"the whole deck, {steps} driver steps, {cw:.2f} s"}
     print(json.dumps(out), flush=True)
 
 
